@@ -1,0 +1,195 @@
+// abi.cpp -- extern "C" boundary of libtild.so (declared in include/tild.h).
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/tild.h"
+#include "engine.hpp"
+#include "system.hpp"
+
+struct tild_system { tild::System sys; };
+struct tild_engine {
+    std::unique_ptr<tild::Batch> batch;
+    int id = -1;
+    std::string err;
+};
+
+namespace {
+thread_local std::string g_last_error;
+tild_status fail(tild_status s, const std::string& m) { g_last_error = m; return s; }
+tild_status from_rc(int rc, std::string& err, tild_engine* e) {
+    if (rc >= 0) return TILD_OK;
+    if (e) e->err = err;
+    g_last_error = err;
+    return (tild_status)(-rc);
+}
+}  // namespace
+
+extern "C" {
+
+const char* tild_last_error(void) { return g_last_error.c_str(); }
+
+tild_status tild_system_load(const char* input_text, const char* base_dir, tild_system** out) {
+    if (!out || !input_text) return fail(TILD_ERR_ARG, "null argument");
+    *out = nullptr;
+    auto s = std::make_unique<tild_system>();
+    std::string err;
+    int rc = tild::load_system(input_text, base_dir ? base_dir : "", s->sys, err);
+    if (rc) return fail((tild_status)rc, err);
+    *out = s.release();
+    return TILD_OK;
+}
+
+void tild_system_free(tild_system* sys) { delete sys; }
+
+tild_status tild_system_info(const tild_system* s, const char* key, double* v) {
+    if (!s || !key || !v) return fail(TILD_ERR_ARG, "null argument");
+    const tild::System& S = s->sys;
+    std::string k(key);
+    if (k == "ns") *v = S.ns;
+    else if (k == "dim") *v = S.Dim;
+    else if (k == "ntypes") *v = S.ntypes;
+    else if (k == "M") *v = S.M;
+    else if (k == "Nx0") *v = S.Nx[0];
+    else if (k == "Nx1") *v = S.Nx[1];
+    else if (k == "Nx2") *v = S.Dim > 2 ? S.Nx[2] : 1;
+    else if (k == "pmeorder") *v = S.pmeorder;
+    else if (k == "n_styles") *v = (double)S.styles().size();
+    else if (k == "do_charges") *v = S.charges.enabled ? 1 : 0;
+    else if (k == "n_bonds_total") *v = S.n_total_bonds;
+    else if (k == "nbond_types") *v = S.nbond_types;
+    else if (k == "max_steps") *v = S.max_steps;
+    else if (k == "log_freq") *v = S.log_freq;
+    else if (k == "n_groups") *v = (double)S.Groups.size();
+    else if (k == "n_integrators") *v = (double)S.Integrators.size();
+    else if (k == "seed") *v = S.rand_seed;
+    else if (k == "L0") *v = S.L[0];
+    else if (k == "L1") *v = S.L[1];
+    else if (k == "L2") *v = S.L[2];
+    else if (k == "dx0") *v = S.dx[0];
+    else if (k == "dx1") *v = S.dx[1];
+    else if (k == "dx2") *v = S.dx[2];
+    else if (k == "gvol") *v = S.gvol;
+    else if (k == "V") *v = S.V;
+    else if (k == "delt") *v = S.delt;
+    else if (k == "noise_mag") *v = S.noise_mag;
+    else if (k == "bjerrum") *v = S.charges.bjerrum_length;
+    else if (k == "smear") *v = S.charges.smearing_length;
+    else return fail(TILD_ERR_ARG, "unknown key " + k);
+    return TILD_OK;
+}
+
+tild_status tild_system_get_positions(const tild_system* s, float* x, size_t n) {
+    if (!s || !x || n != s->sys.x.size()) return fail(TILD_ERR_ARG, "size mismatch");
+    std::memcpy(x, s->sys.x.data(), n * sizeof(float));
+    return TILD_OK;
+}
+tild_status tild_system_get_types(const tild_system* s, int* tp, size_t n) {
+    if (!s || !tp || n != (size_t)s->sys.ns) return fail(TILD_ERR_ARG, "size mismatch");
+    std::memcpy(tp, s->sys.tp.data(), n * sizeof(int));
+    return TILD_OK;
+}
+tild_status tild_system_get_molecules(const tild_system* s, int* mol, size_t n) {
+    if (!s || !mol || n != (size_t)s->sys.ns) return fail(TILD_ERR_ARG, "size mismatch");
+    std::memcpy(mol, s->sys.molecID.data(), n * sizeof(int));
+    return TILD_OK;
+}
+tild_status tild_system_get_charges(const tild_system* s, float* q, size_t n) {
+    if (!s || !q || n != (size_t)s->sys.ns) return fail(TILD_ERR_ARG, "size mismatch");
+    std::memcpy(q, s->sys.q.data(), n * sizeof(float));
+    return TILD_OK;
+}
+tild_status tild_system_get_bonds(const tild_system* s, int* nb, int* bt, int* ty, size_t ns) {
+    if (!s || !nb || !bt || !ty || ns != (size_t)s->sys.ns) return fail(TILD_ERR_ARG, "size mismatch");
+    std::memcpy(nb, s->sys.n_bonds.data(), ns * sizeof(int));
+    std::memcpy(bt, s->sys.bonded_to.data(), ns * 3 * sizeof(int));
+    std::memcpy(ty, s->sys.bond_type.data(), ns * 3 * sizeof(int));
+    return TILD_OK;
+}
+tild_status tild_system_group_sites(const tild_system* s, int g, int* sites, size_t cap, int* nsites) {
+    if (!s || g < 0 || g >= (int)s->sys.Groups.size()) return fail(TILD_ERR_ARG, "no such group");
+    const auto& idx = s->sys.Groups[g].index;
+    if (nsites) *nsites = (int)idx.size();
+    if (sites) {
+        if (cap < idx.size()) return fail(TILD_ERR_ARG, "buffer too small");
+        std::memcpy(sites, idx.data(), idx.size() * sizeof(int));
+    }
+    return TILD_OK;
+}
+
+tild_status tild_engine_create(const tild_system* s, const tild_opts* opts, tild_engine** out) {
+    if (!s || !out) return fail(TILD_ERR_ARG, "null argument");
+    *out = nullptr;
+    tild_opts o{};
+    if (opts) o = *opts;
+    auto e = std::make_unique<tild_engine>();
+    e->batch = std::make_unique<tild::Batch>(o.device);
+    std::string err;
+    int rc = e->batch->init(err);
+    if (rc < 0) return from_rc(rc, err, nullptr);
+    uint32_t seed = o.use_seed ? o.seed : s->sys.rand_seed;
+    int id = e->batch->add(s->sys, seed, err);
+    if (id < 0) return from_rc(id, err, nullptr);
+    e->id = id;
+    *out = e.release();
+    return TILD_OK;
+}
+
+void tild_engine_destroy(tild_engine* e) { delete e; }
+
+const char* tild_engine_last_error(const tild_engine* e) { return e ? e->err.c_str() : g_last_error.c_str(); }
+
+tild_status tild_engine_forces(tild_engine* e) {
+    if (!e) return fail(TILD_ERR_ARG, "null engine");
+    std::string err;
+    int rc = e->batch->forces(e->id, err);
+    if (rc >= 0) rc = e->batch->sync(err);
+    return from_rc(rc, err, e);
+}
+
+tild_status tild_engine_step(tild_engine* e, int n) {
+    if (!e || n < 0) return fail(TILD_ERR_ARG, "bad argument");
+    std::string err;
+    int rc = e->batch->step({e->id}, n, err);
+    if (rc >= 0) rc = e->batch->sync(err);
+    return from_rc(rc, err, e);
+}
+
+tild_status tild_engine_properties(tild_engine* e, tild_props* out) {
+    if (!e || !out) return fail(TILD_ERR_ARG, "null argument");
+    std::string err;
+    int rc = e->batch->properties(e->id, out, err);
+    tild_status st = from_rc(rc, err, e);
+    if (st == TILD_OK && out->nan_flag) { e->err = "NaN energy"; return TILD_ERR_NAN; }
+    return st;
+}
+
+#define ENGINE_COPY(call)                                   \
+    if (!e) return fail(TILD_ERR_ARG, "null engine");       \
+    std::string err;                                        \
+    int rc = e->batch->call;                                \
+    return from_rc(rc, err, e);
+
+tild_status tild_engine_get_positions(tild_engine* e, float* x, size_t n) { ENGINE_COPY(get_positions(e->id, x, n, err)) }
+tild_status tild_engine_set_positions(tild_engine* e, const float* x, size_t n) { ENGINE_COPY(set_positions(e->id, x, n, err)) }
+tild_status tild_engine_get_forces(tild_engine* e, float* f, size_t n) { ENGINE_COPY(get_forces(e->id, f, n, err)) }
+tild_status tild_engine_get_density(tild_engine* e, int t, float* rho, size_t n) { ENGINE_COPY(get_density(e->id, t, rho, n, err)) }
+tild_status tild_engine_get_grid_field(tild_engine* e, int t, int d, float* g, size_t n) { ENGINE_COPY(get_grid_field(e->id, t, d, g, n, err)) }
+tild_status tild_engine_synchronize(tild_engine* e) { ENGINE_COPY(sync(err)) }
+tild_status tild_engine_get_bins(tild_engine* e, int* base, size_t n) { ENGINE_COPY(get_bins(e->id, base, n, err)) }
+
+tild_status tild_engine_profile(tild_engine* e, int nsteps, double* phase_ms, int nphase) {
+    if (!e) return fail(TILD_ERR_ARG, "null engine");
+    std::string err;
+    int rc = e->batch->profile({e->id}, nsteps, phase_ms, nphase, err);
+    return from_rc(rc, err, e);
+}
+
+tild_status tild_engine_get_step(tild_engine* e, int* step) {
+    if (!e || !step) return fail(TILD_ERR_ARG, "null argument");
+    *step = e->batch->rep(e->id)->step;
+    return TILD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,163 @@
+// device.hpp -- device-side descriptors and helpers shared by the TILD kernels.
+//
+// Every kernel is launched over the work of one or more replicas: block b of a
+// launch looks up (replica, local block) in a per-phase map, and reads that
+// replica's geometry, plans and buffers from a RepDev descriptor in device memory.
+// A single-replica step is the same launches with a one-entry map.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace tild {
+
+constexpr int kMaxTypes = 16;      // particle types per replica
+constexpr int kMaxStages = 12;     // FFT stages per axis
+constexpr int kFftThreads = 256;   // threads of every FFT block
+constexpr int kFftElems = 16;      // complex values per thread per FFT stage (registers)
+constexpr int kMaxQuant = 128;     // property quantities per replica
+
+// Mixed-radix Stockham plan of one axis.  Stage s has radix rad[s] and
+// p_s = prod(rad[0..s-1]); its twiddles w_{p R}^{r k} (forward sign, k < p, 1 <= r < R)
+// start at tw + tw_off[s], index k*(R-1) + (r-1).
+struct AxisPlan {
+    int N;
+    int nst;
+    int rad[kMaxStages];
+    int tw_off[kMaxStages];
+    const float2* tw;
+};
+
+// Per-replica descriptor (device memory).
+struct RepDev {
+    // geometry (initialize.cu:32-48)
+    int dim, Nx, Ny, Nz, Nxh, M, Mh, ns, ntypes, nf, P;
+    float L[3], Lh[3], dx[3], gvol, kfac[3];   // kfac = 2 pi / L
+    // fixed-point densities: acc = round(w * 2^F), rho = acc * inv
+    float fix_scale, fixq_scale;
+    double fix_inv, fixq_inv;
+    // FFT plans (x, y, z) and chunk sizes of each pass
+    AxisPlan ax[3];
+    int nl_x;      // line pairs per block, forward x pass
+    int nl_mid;    // batch columns per block, middle (y) pass
+    int nl_last;   // batch columns per block, fused last-axis pass
+    int nr_c2r;    // rows per block, inverse x pass
+    // particle state
+    float* X[2];            // ns*dim, double-buffered by step parity
+    float* XO;              // ns*dim, GJF previous position
+    const int* tp;          // ns
+    const float* q;         // ns or null
+    const int* bond_j;      // ns*3, -1 = empty slot
+    const int* bond_t;      // ns*3
+    const float* bond_k;    // per bond type
+    const float* bond_r0;
+    const signed char* ikind;  // ns: 0 none, 1 EM, 2 GJF
+    float delt, noise_mag;
+    float gjf_b[kMaxTypes], gjf_a[kMaxTypes], gjf_c1[kMaxTypes], gjf_c2[kMaxTypes],
+          gjf_noise[kMaxTypes];
+    float* fout;            // ns*dim (optional)
+    // grids
+    unsigned long long* acc;   // nf*M fixed-point accumulators
+    float* rho;                // nf*M densities (x fastest)
+    float2* H;                 // nf*Mh half spectra
+    float2* Gk;                // nf*dim*Mh half spectra of the gathered fields
+    float4* G4;                // nf*M (g_x, g_y, g_z, rho) records for the gather
+    const float* Ctab;         // (T(T+1)/2)*Mh coupling tables (include 1/M), + charge table
+    int ctab_charge;           // row of the charge kernel in Ctab (or -1)
+    // property tables
+    int nstyles, npairs, nq;
+    int st_t1[64], st_t2[64];
+    int pr_t1[32], pr_t2[32];
+    const float* Etab;         // nstyles*Mh : u_s(k) * gvol / M
+    const float2* Vtab;        // npairs*nP*Mh : vir_k * gvol / (V D M)
+    const float* Eltab;        // Mh : charge potential / M
+    double* partials;          // nq * nchunk partial sums
+    int nchunk;
+    double* bpart;             // 7 * ceil(ns/256) bond-property partials
+    // counters
+    int step;                  // completed steps; X[step & 1] is current
+    uint32_t seed;
+    int nP;
+};
+
+struct WorkItem { int rep, blk; };
+
+// ---------------------------------------------------------------- Philox / normals
+// Philox4x32-10 (Salmon et al., SC'11).
+__device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint32_t lo0 = 0xD2511F53u * c[0];
+        uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
+        uint32_t lo1 = 0xCD9E8D57u * c[2];
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        uint32_t n0 = hi1 ^ c[1] ^ k0;
+        uint32_t n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+    }
+}
+
+// Four standard normals for (seed, site, step): Box-Muller on (k + 1/2) / 2^23
+// uniforms, which are exact in FP32.
+__device__ inline void normals4(uint32_t seed, uint32_t site, uint32_t step, float z[4]) {
+    uint32_t c[4] = {site, step, 0u, 0u};
+    philox4x32_10(c, seed, 0u);
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        float u1 = ((float)(c[2 * p] >> 9) + 0.5f) * (1.0f / 8388608.0f);
+        float u2 = ((float)(c[2 * p + 1] >> 9) + 0.5f) * (1.0f / 8388608.0f);
+        float r = sqrtf(-2.0f * logf(u1));
+        float s, co;
+        sincospif(2.0f * u2, &s, &co);
+        z[2 * p] = r * co;
+        z[2 * p + 1] = r * s;
+    }
+}
+
+// ---------------------------------------------------------------- stencil
+// PME B-spline weights of orders 0-4 (device_grid_utils.cu:3-49), including the
+// order-4 term sx3 = sx2*sx2 of the reference.
+__device__ inline void spline_weights(float gdx, float H, float* W, int P) {
+    float sx = gdx / H, sx2, sx3, sx4;
+    if (P == 0) { W[0] = 1.f; }
+    else if (P == 1) { W[0] = 0.5f - sx; W[1] = 0.5f + sx; }
+    else if (P == 2) {
+        sx2 = sx * sx;
+        W[0] = 0.125f - 0.5f * sx + 0.5f * sx2; W[1] = 0.75f - sx2; W[2] = 0.125f + 0.5f * sx + 0.5f * sx2;
+    } else if (P == 3) {
+        sx2 = sx * sx; sx3 = sx2 * sx;
+        W[0] = (1.f - 6.f * sx + 12.f * sx2 - 8.f * sx3) / 48.f;
+        W[1] = (23.f - 30.f * sx - 12.f * sx2 + 24.f * sx3) / 48.f;
+        W[2] = (23.f + 30.f * sx - 12.f * sx2 - 24.f * sx3) / 48.f;
+        W[3] = (1.f + 6.f * sx + 12.f * sx2 + 8.f * sx3) / 48.f;
+    } else {
+        sx2 = sx * sx; sx3 = sx2 * sx2; sx4 = sx2 * sx2;
+        W[0] = (1.f - 8.f * sx + 24.f * sx2 - 32.f * sx3 + 16.f * sx4) / 384.f;
+        W[1] = (19.f - 44.f * sx + 24.f * sx2 + 16.f * sx3 - 16.f * sx4) / 96.f;
+        W[2] = (115.f - 120.f * sx2 + 48.f * sx4) / 192.f;
+        W[3] = (19.f + 44.f * sx + 24.f * sx2 - 16.f * sx3 - 16.f * sx4) / 96.f;
+        W[4] = (1.f + 8.f * sx + 24.f * sx2 + 32.f * sx3 + 16.f * sx4) / 384.f;
+    }
+}
+
+// Bin and weights of one coordinate (device_grid_utils.cu:136-146): the base node
+// uses IEEE FP32 division exactly as the reference, so bins match bit for bit.
+// n0 is the first stencil node, already wrapped into [0, N).
+__device__ inline int stencil_1d(float x, float dx, int N, int P, float* W) {
+    int g;
+    float gdx;
+    if ((P & 1) == 0) {
+        g = (int)((x + 0.5f * dx) / dx);
+        gdx = x - (float)g * dx;
+    } else {
+        g = (int)(x / dx);
+        gdx = x - ((float)g + 0.5f) * dx;
+    }
+    spline_weights(gdx, dx, W, P);
+    int n0 = (g - P / 2) % N;
+    return n0 < 0 ? n0 + N : n0;
+}
+
+__device__ inline int wrap_next(int n, int N) { return (n + 1 == N) ? 0 : n + 1; }
+
+}  // namespace tild

@@ -1,0 +1,753 @@
+// engine.cpp -- host side of the TILD step engine: replica setup, k-space tables,
+// work maps and the per-step launch sequence.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace tild {
+
+#define HIPCHK(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            err = std::string(#call) + " failed: " + hipGetErrorString(e_);           \
+            return -TILD_ERR_DEVICE;                                                  \
+        }                                                                             \
+    } while (0)
+
+// ---------------------------------------------------------------- FFT plans
+int make_axis_plan(int N, AxisPlan& pl, std::vector<float2>& tw, std::string& err) {
+    std::memset(&pl, 0, sizeof(pl));
+    pl.N = N;
+    if (N < 1 || N > kFftBudget) { err = "FFT length " + std::to_string(N) + " outside [1, 4096]"; return -TILD_ERR_UNSUPPORTED; }
+    int n = N, e2 = 0;
+    while (n % 2 == 0) { n /= 2; ++e2; }
+    std::vector<int> rad;
+    for (int k = 0; k < e2 / 3; ++k) rad.push_back(8);
+    if (e2 % 3 == 1) rad.push_back(2);
+    if (e2 % 3 == 2) rad.push_back(4);
+    for (int p : {3, 5, 7})
+        while (n % p == 0) { rad.push_back(p); n /= p; }
+    if (n != 1) {
+        err = "FFT length " + std::to_string(N) + " has a prime factor above 7";
+        return -TILD_ERR_UNSUPPORTED;
+    }
+    if ((int)rad.size() > kMaxStages) { err = "too many FFT stages"; return -TILD_ERR_UNSUPPORTED; }
+    pl.nst = (int)rad.size();
+    int p = 1;
+    tw.clear();
+    for (int s = 0; s < pl.nst; ++s) {
+        int R = rad[s];
+        pl.rad[s] = R;
+        pl.tw_off[s] = (int)tw.size();
+        for (int k = 0; k < p; ++k)
+            for (int r = 1; r < R; ++r) {
+                double ang = -2.0 * kPi * (double)r * (double)k / (double)(p * R);
+                tw.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
+            }
+        p *= R;
+    }
+    if (tw.empty()) tw.push_back(make_float2(1.f, 0.f));
+    return 0;
+}
+
+// ---------------------------------------------------------------- helpers
+static int fixed_bits(double bound) {
+    // integer bits needed for sums up to `bound` (with sign) -> fractional bits
+    int ib = (int)std::ceil(std::log2(bound + 1.0)) + 1;
+    int F = 62 - ib;
+    return std::max(16, std::min(44, F));
+}
+
+template <class T>
+static int dalloc(Replica& R, T** p, size_t count, std::string& err, bool zero = false) {
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    HIPCHK(hipMalloc((void**)p, bytes));
+    R.allocs.push_back((void*)*p);
+    R.bytes += bytes;
+    if (zero) HIPCHK(hipMemset(*p, 0, bytes));
+    return 0;
+}
+
+Batch::~Batch() {
+    if (device_ >= 0) (void)hipSetDevice(device_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (auto& kv : set_cache_) {
+        free_maps(kv.second->maps);
+        if (kv.second->d_ids) (void)hipFree(kv.second->d_ids);
+    }
+    for (auto& r : reps_)
+        if (r) {
+            free_maps(r->maps);
+            for (void* p : r->allocs) (void)hipFree(p);
+        }
+    if (d_reps_) (void)hipFree(d_reps_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Batch::free_maps(PhaseMap* maps) {
+    for (int p = 0; p < PH_COUNT; ++p)
+        if (maps[p].dev) { (void)hipFree(maps[p].dev); maps[p].dev = nullptr; }
+}
+
+int Batch::init(std::string& err) {
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device_ < 0 || device_ >= ndev) { err = "no HIP device " + std::to_string(device_); return -TILD_ERR_DEVICE; }
+    HIPCHK(hipSetDevice(device_));
+    HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    return 0;
+}
+
+size_t Batch::estimate_bytes(const System& S) {
+    size_t D = S.Dim, ns = S.ns, M = S.M;
+    size_t Nxh = S.Nx[0] / 2 + 1, Mh = M / S.Nx[0] * Nxh;
+    size_t nf = S.ntypes + (S.charges.enabled ? 1 : 0);
+    size_t T = S.ntypes, nst = S.styles().size();
+    size_t part = ns * (4 * D * 4 + 4 + 4 + 24 + 1);
+    size_t grids = nf * M * (8 + 4 + 16) + nf * Mh * 8 * (1 + D);
+    size_t tabs = Mh * 4 * (T * (T + 1) / 2 + 1 + nst + 1) + Mh * 8 * (D == 3 ? 6 : 3) * nst;
+    return part + grids + tabs + (1 << 20);
+}
+
+// ---------------------------------------------------------------- replica setup
+int Batch::add(const System& S, uint32_t seed, std::string& err) {
+    HIPCHK(hipSetDevice(device_));
+    auto Rp = std::make_unique<Replica>();
+    Replica& R = *Rp;
+    RepDev& h = R.h;
+    std::memset(&h, 0, sizeof(h));
+    const int D = S.Dim;
+    if (S.ntypes > kMaxTypes) { err = "more than 16 particle types"; return -TILD_ERR_UNSUPPORTED; }
+    if (S.ns <= 0) { err = "no particles"; return -TILD_ERR_CONFIG; }
+    h.dim = D; h.Nx = S.Nx[0]; h.Ny = S.Nx[1]; h.Nz = D == 3 ? S.Nx[2] : 1;
+    h.Nxh = h.Nx / 2 + 1;
+    h.M = S.M; h.Mh = h.Nxh * h.Ny * h.Nz;
+    h.ns = S.ns; h.ntypes = S.ntypes; h.nf = S.ntypes + (S.charges.enabled ? 1 : 0); h.P = S.pmeorder;
+    h.nP = S.n_P_comps;
+    for (int d = 0; d < 3; ++d) { h.L[d] = S.L[d]; h.Lh[d] = S.Lh[d]; h.dx[d] = S.dx[d]; }
+    h.gvol = S.gvol;
+    for (int d = 0; d < D; ++d) h.kfac[d] = (float)(2.0 * kPi / (double)S.L[d]);
+    R.ns = S.ns; R.dim = D; R.ntypes = S.ntypes; R.nf = h.nf; R.M = h.M; R.Mh = h.Mh; R.nP = h.nP;
+    for (int d = 0; d < 3; ++d) R.L[d] = S.L[d];
+    R.V = 1.0;
+    for (int d = 0; d < D; ++d) R.V *= (double)S.L[d];
+
+    // fixed point: |w| <= 2 for every supported order; sums over all sites
+    double qmax = 0.0;
+    for (float qq : S.q) qmax = std::max(qmax, (double)std::fabs(qq));
+    int F = fixed_bits(2.0 * S.ns), Fq = fixed_bits(2.0 * S.ns * std::max(qmax, 1e-30));
+    h.fix_scale = std::ldexp(1.0f, F);
+    h.fixq_scale = std::ldexp(1.0f, Fq);
+    h.fix_inv = std::ldexp(1.0, -F) / (double)S.gvol;
+    h.fixq_inv = std::ldexp(1.0, -Fq) / (double)S.gvol;
+
+    // FFT plans
+    std::vector<float2> twall;
+    int toff[3] = {0, 0, 0};
+    for (int a = 0; a < D; ++a) {
+        std::vector<float2> tw;
+        int rc = make_axis_plan(S.Nx[a], h.ax[a], tw, err);
+        if (rc) return rc;
+        toff[a] = (int)twall.size();
+        twall.insert(twall.end(), tw.begin(), tw.end());
+    }
+    float2* d_tw;
+    if (int rc = dalloc(R, &d_tw, twall.size(), err)) return rc;
+    HIPCHK(hipMemcpy(d_tw, twall.data(), twall.size() * sizeof(float2), hipMemcpyHostToDevice));
+    for (int a = 0; a < D; ++a) h.ax[a].tw = d_tw + toff[a];
+    if (D == 2) h.ax[2] = h.ax[1];
+
+    // chunking of each pass
+    const int rows = h.M / h.Nx;
+    const int npairs_x = (h.nf * rows + 1) / 2;
+    h.nl_x = std::max(1, std::min(kFftBudget / h.Nx, npairs_x));
+    const int la = D - 1, Nla = la == 2 ? h.Nz : h.Ny, Q = la == 2 ? h.Ny * h.Nxh : h.Nxh;
+    if (Nla * h.nf > kFftBudget || 2 * lds_bytes(Nla, h.nf) > kMaxLds) { err = "grid too large for the fused k-space pass"; return -TILD_ERR_UNSUPPORTED; }
+    int nl = std::max(1, kFftBudget / (Nla * h.nf));
+    while (nl > 1 && 2 * lds_bytes(Nla, h.nf * nl) > kMaxLds) --nl;
+    h.nl_last = std::min(nl, Q);
+    const int nc = (h.nf * D + 1) / 2;
+    if (nc * h.Nx > kFftBudget) { err = "grid too large for the inverse x pass"; return -TILD_ERR_UNSUPPORTED; }
+    h.nr_c2r = std::max(1, std::min(kFftBudget / (nc * h.Nx), rows));
+    h.nl_mid = mid_geom(h, 1, h.nf).nl;
+
+    // particle state
+    const size_t nsD = (size_t)S.ns * D;
+    if (int rc = dalloc(R, &h.X[0], nsD, err)) return rc;
+    if (int rc = dalloc(R, &h.X[1], nsD, err)) return rc;
+    if (int rc = dalloc(R, &h.XO, nsD, err)) return rc;
+    HIPCHK(hipMemcpy(h.X[0], S.x.data(), nsD * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h.X[1], S.x.data(), nsD * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(h.XO, 0, nsD * 4));   // xo = x (device_comm_utils.cu:15-16): zero displacement
+    int* d_tp;
+    if (int rc = dalloc(R, &d_tp, S.ns, err)) return rc;
+    HIPCHK(hipMemcpy(d_tp, S.tp.data(), S.ns * 4, hipMemcpyHostToDevice));
+    h.tp = d_tp;
+    if (S.charges.enabled) {
+        float* d_q;
+        if (int rc = dalloc(R, &d_q, S.ns, err)) return rc;
+        HIPCHK(hipMemcpy(d_q, S.q.data(), S.ns * 4, hipMemcpyHostToDevice));
+        h.q = d_q;
+    }
+    {
+        std::vector<int> bj((size_t)S.ns * 3, -1), bt((size_t)S.ns * 3, 0);
+        for (int i = 0; i < S.ns; ++i)
+            for (int b = 0; b < S.n_bonds[i]; ++b) {
+                bj[(size_t)i * 3 + b] = S.bonded_to[(size_t)i * kMaxBonds + b];
+                bt[(size_t)i * 3 + b] = S.bond_type[(size_t)i * kMaxBonds + b];
+            }
+        int *d_bj, *d_bt;
+        float *d_bk, *d_br;
+        if (int rc = dalloc(R, &d_bj, bj.size(), err)) return rc;
+        if (int rc = dalloc(R, &d_bt, bt.size(), err)) return rc;
+        if (int rc = dalloc(R, &d_bk, S.bond_k.size(), err)) return rc;
+        if (int rc = dalloc(R, &d_br, S.bond_req.size(), err)) return rc;
+        HIPCHK(hipMemcpy(d_bj, bj.data(), bj.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_bt, bt.data(), bt.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_bk, S.bond_k.data(), S.bond_k.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_br, S.bond_req.data(), S.bond_req.size() * 4, hipMemcpyHostToDevice));
+        h.bond_j = d_bj; h.bond_t = d_bt; h.bond_k = d_bk; h.bond_r0 = d_br;
+    }
+    // integrators (integrator.cu:24-39): per-site kind; a site may belong to one integrator
+    {
+        std::vector<signed char> kind(S.ns, 0);
+        for (const auto& I : S.Integrators) {
+            const Group& g = S.Groups[I.group_index];
+            for (int i : g.index) {
+                if (kind[i]) { err = "site " + std::to_string(i + 1) + " is in more than one integrator group"; return -TILD_ERR_UNSUPPORTED; }
+                kind[i] = I.name == "EM" ? 1 : 2;
+            }
+        }
+        signed char* d_k;
+        if (int rc = dalloc(R, &d_k, S.ns, err)) return rc;
+        HIPCHK(hipMemcpy(d_k, kind.data(), S.ns, hipMemcpyHostToDevice));
+        h.ikind = d_k;
+    }
+    h.delt = S.delt;
+    h.noise_mag = S.noise_mag;
+    for (int t = 0; t < S.ntypes; ++t) {   // device_GJF_integrator.cu:25-28, 36, 46-48 in FP32
+        float m = S.mass[t], Df = S.Diff[t], dt = S.delt;
+        h.gjf_b[t] = m / (m + dt / (2.0f * Df));
+        h.gjf_a[t] = (1.0f - dt / (2.0f * Df * m)) / (1.0f + dt / (2.0f * Df * m));
+        h.gjf_c1[t] = h.gjf_b[t] * (dt * dt) / m;
+        h.gjf_c2[t] = h.gjf_b[t] * dt / (2.0f * m);
+        h.gjf_noise[t] = S.noise_mag / Df;
+    }
+    if (int rc = dalloc(R, &R.d_fout, nsD, err, true)) return rc;
+    h.fout = R.d_fout;
+    h.seed = seed;
+    h.step = 0;
+
+    // grids
+    const size_t M = h.M, Mh = h.Mh, nf = h.nf;
+    if (int rc = dalloc(R, &h.acc, nf * M, err, true)) return rc;
+    if (int rc = dalloc(R, &h.rho, nf * M, err, true)) return rc;
+    if (int rc = dalloc(R, &h.H, nf * Mh, err, true)) return rc;
+    if (int rc = dalloc(R, &h.Gk, nf * D * Mh, err, true)) return rc;
+    if (int rc = dalloc(R, &h.G4, nf * M, err, true)) return rc;
+    R.nblk_part = (S.ns + 255) / 256;
+    if (int rc = dalloc(R, &R.d_bpart, 7 * (size_t)R.nblk_part, err, true)) return rc;
+    h.bpart = R.d_bpart;
+
+    int id = -1;
+    for (int k = 0; k < (int)reps_.size(); ++k) if (!reps_[k]) { id = k; break; }
+    if (id < 0) { id = (int)reps_.size(); reps_.emplace_back(); }
+    reps_[id] = std::move(Rp);
+    Replica& RR = *reps_[id];
+    if (int rc = upload_desc(id, err)) { remove(id); return rc; }
+    if (int rc = build_maps({id}, RR.maps, err)) { remove(id); return rc; }
+    if (int rc = build_tables(RR, S, err)) { remove(id); return rc; }
+    if (int rc = upload_desc(id, err)) { remove(id); return rc; }
+    HIPCHK(hipStreamSynchronize(stream_));
+    return id;
+}
+
+void Batch::remove(int id) {
+    if (id < 0 || id >= (int)reps_.size() || !reps_[id]) return;
+    (void)hipStreamSynchronize(stream_);
+    free_maps(reps_[id]->maps);
+    for (void* p : reps_[id]->allocs) (void)hipFree(p);
+    reps_[id].reset();
+    for (auto it = set_cache_.begin(); it != set_cache_.end();) {
+        if (std::find(it->first.begin(), it->first.end(), id) != it->first.end()) {
+            free_maps(it->second->maps);
+            if (it->second->d_ids) (void)hipFree(it->second->d_ids);
+            it = set_cache_.erase(it);
+        } else ++it;
+    }
+}
+
+int Batch::upload_desc(int id, std::string& err) {
+    if (id >= d_cap_) {
+        int cap = std::max(16, d_cap_);
+        while (cap <= id) cap *= 2;
+        RepDev* nd;
+        HIPCHK(hipMalloc((void**)&nd, sizeof(RepDev) * cap));
+        if (d_reps_) {
+            HIPCHK(hipMemcpyAsync(nd, d_reps_, sizeof(RepDev) * d_cap_, hipMemcpyDeviceToDevice, stream_));
+            HIPCHK(hipStreamSynchronize(stream_));
+            HIPCHK(hipFree(d_reps_));
+        }
+        d_reps_ = nd;
+        d_cap_ = cap;
+    }
+    Replica& R = *reps_[id];
+    R.h.step = R.step;
+    HIPCHK(hipMemcpyAsync(d_reps_ + id, &R.h, sizeof(RepDev), hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return 0;
+}
+
+// ---------------------------------------------------------------- work maps
+int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& err) {
+    for (int p = 0; p < PH_COUNT; ++p) { maps[p].host.clear(); maps[p].lds = 0; maps[p].n = 0; }
+    for (int id : ids) {
+        const Replica& R = *reps_[id];
+        const RepDev& h = R.h;
+        const int D = h.dim, rows = h.M / h.Nx;
+        auto add = [&](Phase ph, int nblk, size_t lds) {
+            for (int b = 0; b < nblk; ++b) maps[ph].host.push_back(WorkItem{id, b});
+            maps[ph].lds = std::max(maps[ph].lds, lds);
+        };
+        add(PH_PART, R.nblk_part, 0);
+        const int npairs = (h.nf * rows + 1) / 2;
+        add(PH_X, (npairs + h.nl_x - 1) / h.nl_x, lds_bytes(h.Nx, h.nl_x));
+        {
+            MidGeom g = mid_geom(h, 1, h.nf);
+            add(PH_MIDY_H, g.nouter * g.nchunk, lds_bytes(g.N, g.nl));
+        }
+        if (D == 3) {
+            MidGeom g = mid_geom(h, 1, h.nf * D);
+            add(PH_MIDY_G, g.nouter * g.nchunk, lds_bytes(g.N, g.nl));
+            MidGeom gz = mid_geom(h, 2, h.nf);
+            add(PH_MIDZ_H, gz.nouter * gz.nchunk, lds_bytes(gz.N, gz.nl));
+        }
+        const int la = D - 1, Nla = la == 2 ? h.Nz : h.Ny, Q = la == 2 ? h.Ny * h.Nxh : h.Nxh;
+        add(PH_LAST, (Q + h.nl_last - 1) / h.nl_last, 2 * lds_bytes(Nla, h.nf * h.nl_last));
+        const int nc = (h.nf * D + 1) / 2;
+        add(PH_C2R, (rows + h.nr_c2r - 1) / h.nr_c2r, lds_bytes(h.Nx, h.nr_c2r * nc));
+        add(PH_KPROPS, h.nchunk > 0 ? h.nchunk : 1, 0);
+    }
+    for (int p = 0; p < PH_COUNT; ++p) {
+        if (maps[p].dev) { (void)hipFree(maps[p].dev); maps[p].dev = nullptr; }
+        maps[p].n = (int)maps[p].host.size();
+        if (maps[p].lds > kMaxLds) { err = "pass needs more LDS than a block may use"; return -TILD_ERR_UNSUPPORTED; }
+        if (maps[p].n) {
+            HIPCHK(hipMalloc((void**)&maps[p].dev, maps[p].n * sizeof(WorkItem)));
+            HIPCHK(hipMemcpy(maps[p].dev, maps[p].host.data(), maps[p].n * sizeof(WorkItem), hipMemcpyHostToDevice));
+        }
+    }
+    return 0;
+}
+
+Batch::SetMaps* Batch::maps_for(const std::vector<int>& ids, std::string& err) {
+    auto it = set_cache_.find(ids);
+    if (it != set_cache_.end()) return it->second.get();
+    auto m = std::make_unique<SetMaps>();
+    if (build_maps(ids, m->maps, err)) return nullptr;
+    m->nids = (int)ids.size();
+    if (hipMalloc((void**)&m->d_ids, std::max<size_t>(1, ids.size()) * sizeof(int)) != hipSuccess ||
+        hipMemcpy(m->d_ids, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+        err = "work map allocation failed";
+        return nullptr;
+    }
+    SetMaps* raw = m.get();
+    set_cache_[ids] = std::move(m);
+    return raw;
+}
+
+// ---------------------------------------------------------------- tables
+// k^2 of half-spectrum index idx = (z*Ny + y)*Nxh + kx with the reference's
+// wavenumber convention (device_array_utils.cu:60-82), in FP64.
+static inline double k2_of(const RepDev& h, size_t idx, double* kv) {
+    int kx = (int)(idx % h.Nxh);
+    size_t rest = idx / h.Nxh;
+    int ky = (int)(rest % h.Ny);
+    int kz = (int)(rest / h.Ny);
+    int n[3] = {kx, ky, kz}, N[3] = {h.Nx, h.Ny, h.Nz};
+    double k2 = 0.0;
+    for (int d = 0; d < h.dim; ++d) {
+        int s = (2 * n[d] < N[d]) ? n[d] : n[d] - N[d];
+        kv[d] = 2.0 * kPi * (double)s / (double)h.L[d];
+        k2 += kv[d] * kv[d];
+    }
+    return k2;
+}
+
+int Batch::build_tables(Replica& R, const System& S, std::string& err) {
+    RepDev& h = R.h;
+    const int T = S.ntypes, D = h.dim, nP = h.nP;
+    const size_t Mh = h.Mh;
+    const double M = (double)h.M, gvol = (double)S.gvol;
+    auto styles = S.styles();
+    R.nstyles = (int)styles.size();
+    if (R.nstyles > 64) { err = "more than 64 pair styles"; return -TILD_ERR_UNSUPPORTED; }
+    // ordered type pairs of the styles (virial tables are summed per pair)
+    std::vector<std::pair<int, int>> pairs;
+    std::vector<int> pair_of(styles.size());
+    for (size_t s = 0; s < styles.size(); ++s) {
+        std::pair<int, int> pq(styles[s]->type1, styles[s]->type2);
+        auto it = std::find(pairs.begin(), pairs.end(), pq);
+        if (it == pairs.end()) { pairs.push_back(pq); pair_of[s] = (int)pairs.size() - 1; }
+        else pair_of[s] = (int)(it - pairs.begin());
+    }
+    if (pairs.size() > 32) { err = "more than 32 type pairs"; return -TILD_ERR_UNSUPPORTED; }
+    R.npairs = (int)pairs.size();
+    h.nstyles = R.nstyles; h.npairs = R.npairs;
+    for (size_t s = 0; s < styles.size(); ++s) { h.st_t1[s] = styles[s]->type1; h.st_t2[s] = styles[s]->type2; }
+    for (size_t p = 0; p < pairs.size(); ++p) { h.pr_t1[p] = pairs[p].first; h.pr_t2[p] = pairs[p].second; }
+    R.nq = R.nstyles + R.npairs * nP + (S.charges.enabled ? 1 : 0);
+    h.nq = R.nq;
+    h.nchunk = (int)std::max<size_t>(1, std::min<size_t>(512, (Mh + 4095) / 4096));
+
+    // coupling tables C_IJ (I <= J) / M, then the charge kernel
+    const int ncp = T * (T + 1) / 2;
+    const int nrows = ncp + (S.charges.enabled ? 1 : 0);
+    h.ctab_charge = S.charges.enabled ? ncp : -1;
+    std::vector<float> ctab((size_t)nrows * Mh), etab((size_t)std::max(1, R.nstyles) * Mh), eltab;
+    if (S.charges.enabled) eltab.resize(Mh);
+    auto pidx = [&](int I, int J) { if (I > J) std::swap(I, J); return I * T - I * (I - 1) / 2 + (J - I); };
+    std::vector<double> c(ncp);
+    for (size_t idx = 0; idx < Mh; ++idx) {
+        double kv[3];
+        double k2 = k2_of(h, idx, kv);
+        std::fill(c.begin(), c.end(), 0.0);
+        for (size_t s = 0; s < styles.size(); ++s) {
+            const PairStyle* ps = styles[s];
+            double u = ps->kspace(k2, ps->initial_prefactor);
+            // each style acts on type1 from type2 and on type2 from type1 (pair_style.cu:81-196)
+            c[pidx(ps->type1, ps->type2)] += (ps->type1 == ps->type2) ? 2.0 * u : u;
+            etab[s * Mh + idx] = (float)(u * gvol / M);
+        }
+        for (int p = 0; p < ncp; ++p) ctab[(size_t)p * Mh + idx] = (float)(c[p] / M);
+        if (S.charges.enabled) {
+            double g1 = S.charges.potential(k2);
+            double a = S.charges.smearing_length;
+            ctab[(size_t)ncp * Mh + idx] = (float)(g1 * std::exp(-k2 / (2.0 * a * a)) / M);
+            eltab[idx] = (float)(g1 / M);
+        }
+    }
+    float *d_ctab, *d_etab, *d_eltab = nullptr;
+    if (int rc = dalloc(R, &d_ctab, ctab.size(), err)) return rc;
+    if (int rc = dalloc(R, &d_etab, etab.size(), err)) return rc;
+    HIPCHK(hipMemcpy(d_ctab, ctab.data(), ctab.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_etab, etab.data(), etab.size() * 4, hipMemcpyHostToDevice));
+    if (S.charges.enabled) {
+        if (int rc = dalloc(R, &d_eltab, Mh, err)) return rc;
+        HIPCHK(hipMemcpy(d_eltab, eltab.data(), Mh * 4, hipMemcpyHostToDevice));
+    }
+    h.Ctab = d_ctab; h.Etab = d_etab; h.Eltab = d_eltab;
+    double* d_part;
+    if (int rc = dalloc(R, &d_part, (size_t)std::max(1, R.nq) * h.nchunk, err, true)) return rc;
+    h.partials = d_part;
+    if (int rc = dalloc(R, &R.d_results, (size_t)R.nq + 8, err, true)) return rc;
+
+    // virial tables (pair_style.cu:262-276), on the device:
+    //   f_d(r) = IFFT(-i k_d u(k)), vir_c(r) = dr_a f_b, V_c = FFT(vir_c) gvol/(V D M^2)
+    float2* d_vtab;
+    if (int rc = dalloc(R, &d_vtab, std::max<size_t>(1, (size_t)R.npairs * nP * Mh), err, true)) return rc;
+    h.Vtab = d_vtab;
+    if (R.nstyles > 0) {
+        int id = -1;
+        for (int k = 0; k < (int)reps_.size(); ++k) if (reps_[k].get() == &R) id = k;
+        if (int rc = upload_desc(id, err)) return rc;
+        float2 *tmpK, *tmpV;
+        float *fr, *vr;
+        HIPCHK(hipMalloc((void**)&tmpK, (size_t)D * Mh * sizeof(float2)));
+        HIPCHK(hipMalloc((void**)&tmpV, (size_t)nP * Mh * sizeof(float2)));
+        HIPCHK(hipMalloc((void**)&fr, (size_t)D * h.M * sizeof(float)));
+        HIPCHK(hipMalloc((void**)&vr, (size_t)nP * h.M * sizeof(float)));
+        std::vector<float2> hk((size_t)D * Mh), hv((size_t)nP * Mh);
+        std::vector<double> vacc((size_t)R.npairs * nP * Mh * 2, 0.0);
+        const double scale = gvol / (R.V * (double)D * M * M);
+        const int rows = h.M / h.Nx;
+        // generic pass maps for this replica
+        auto iota = [&](int nblk, WorkItem** dm) -> int {
+            std::vector<WorkItem> v(nblk);
+            for (int b = 0; b < nblk; ++b) v[b] = WorkItem{id, b};
+            HIPCHK(hipMalloc((void**)dm, nblk * sizeof(WorkItem)));
+            HIPCHK(hipMemcpy(*dm, v.data(), nblk * sizeof(WorkItem), hipMemcpyHostToDevice));
+            return 0;
+        };
+        int rc = 0;
+        std::vector<WorkItem*> tmp_maps;
+        auto run_mid = [&](int axis, int nfb, int inverse, float2* buf) -> int {
+            MidGeom g = mid_geom(h, axis, nfb);
+            WorkItem* dm;
+            if (int e = iota(g.nouter * g.nchunk, &dm)) return e;
+            tmp_maps.push_back(dm);
+            LaunchCtx c{d_reps_, dm, g.nouter * g.nchunk, lds_bytes(g.N, g.nl), stream_};
+            HIPCHK(launch_mid(c, axis, 0, nfb, inverse, buf));
+            return 0;
+        };
+        for (size_t s = 0; s < styles.size() && !rc; ++s) {
+            const PairStyle* ps = styles[s];
+            for (size_t idx = 0; idx < Mh; ++idx) {
+                double kv[3];
+                double k2 = k2_of(h, idx, kv);
+                double u = ps->kspace(k2, ps->initial_prefactor);
+                int n[3] = {(int)(idx % h.Nxh), (int)((idx / h.Nxh) % h.Ny), (int)(idx / h.Nxh / h.Ny)};
+                int N[3] = {h.Nx, h.Ny, h.Nz};
+                for (int d = 0; d < D; ++d) {
+                    double kd = (2 * n[d] == N[d]) ? 0.0 : kv[d];
+                    // -i k_d u (k_d = 0 on the d-Nyquist plane)
+                    hk[(size_t)d * Mh + idx] = make_float2(0.0f, (float)(-kd * u));
+                }
+            }
+            HIPCHK(hipMemcpyAsync(tmpK, hk.data(), hk.size() * sizeof(float2), hipMemcpyHostToDevice, stream_));
+            if (D == 3) { if ((rc = run_mid(2, D, 1, tmpK))) break; }
+            if ((rc = run_mid(1, D, 1, tmpK))) break;
+            {
+                const int nc = (D + 1) / 2;
+                int nr = std::max(1, std::min(kFftBudget / (nc * h.Nx), rows));
+                // x_c2r uses h.nr_c2r; the generic call needs its own row chunk
+                RepDev hh = h;
+                hh.nr_c2r = nr;
+                HIPCHK(hipMemcpyAsync(d_reps_ + id, &hh, sizeof(RepDev), hipMemcpyHostToDevice, stream_));
+                WorkItem* dm;
+                int nblk = (rows + nr - 1) / nr;
+                if ((rc = iota(nblk, &dm))) break;
+                tmp_maps.push_back(dm);
+                LaunchCtx c{d_reps_, dm, nblk, lds_bytes(h.Nx, nr * nc), stream_};
+                HIPCHK(launch_x_c2r(c, false, D, tmpK, fr));
+                HIPCHK(launch_virial_source(stream_, d_reps_, id, h.M, fr, vr));
+                // forward R2C of the nP virial fields
+                const int npairs = (nP * rows + 1) / 2;
+                int nlx = std::max(1, std::min(kFftBudget / h.Nx, npairs));
+                hh.nl_x = nlx;
+                HIPCHK(hipMemcpyAsync(d_reps_ + id, &hh, sizeof(RepDev), hipMemcpyHostToDevice, stream_));
+                int nbx = (npairs + nlx - 1) / nlx;
+                if ((rc = iota(nbx, &dm))) break;
+                tmp_maps.push_back(dm);
+                LaunchCtx cx{d_reps_, dm, nbx, lds_bytes(h.Nx, nlx), stream_};
+                HIPCHK(launch_x_r2c(cx, false, nP * rows, vr, tmpV));
+                HIPCHK(hipMemcpyAsync(d_reps_ + id, &h, sizeof(RepDev), hipMemcpyHostToDevice, stream_));
+            }
+            if ((rc = run_mid(1, nP, 0, tmpV))) break;
+            if (D == 3) { if ((rc = run_mid(2, nP, 0, tmpV))) break; }
+            HIPCHK(hipMemcpyAsync(hv.data(), tmpV, hv.size() * sizeof(float2), hipMemcpyDeviceToHost, stream_));
+            HIPCHK(hipStreamSynchronize(stream_));
+            const int p = pair_of[s];
+            for (int cc = 0; cc < nP; ++cc)
+                for (size_t idx = 0; idx < Mh; ++idx) {
+                    size_t o = (((size_t)p * nP + cc) * Mh + idx) * 2;
+                    vacc[o] += hv[(size_t)cc * Mh + idx].x * scale;
+                    vacc[o + 1] += hv[(size_t)cc * Mh + idx].y * scale;
+                }
+        }
+        (void)hipStreamSynchronize(stream_);
+        for (WorkItem* m : tmp_maps) (void)hipFree(m);
+        (void)hipFree(tmpK); (void)hipFree(tmpV); (void)hipFree(fr); (void)hipFree(vr);
+        if (rc) return rc;
+        std::vector<float2> vt((size_t)R.npairs * nP * Mh);
+        for (size_t k = 0; k < vt.size(); ++k) vt[k] = make_float2((float)vacc[2 * k], (float)vacc[2 * k + 1]);
+        HIPCHK(hipMemcpy(d_vtab, vt.data(), vt.size() * sizeof(float2), hipMemcpyHostToDevice));
+    }
+    // kprops map for this replica (nchunk known now)
+    return build_maps({(int)(std::find_if(reps_.begin(), reps_.end(), [&](const std::unique_ptr<Replica>& p) { return p.get() == &R; }) - reps_.begin())}, R.maps, err);
+}
+
+// ---------------------------------------------------------------- step pipeline
+int Batch::enqueue_step(SetMaps& m, bool integrate, bool write_forces, std::string& err) {
+    PhaseMap* mp = m.maps;
+    auto ctx = [&](Phase p) { return LaunchCtx{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_}; };
+    bool any3d = mp[PH_MIDY_G].n > 0;
+    HIPCHK(launch_spread(ctx(PH_PART)));
+    HIPCHK(launch_x_r2c(ctx(PH_X), true, 0, nullptr, nullptr));
+    if (any3d) HIPCHK(launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr));
+    HIPCHK(launch_last_fused(ctx(PH_LAST), 0));
+    if (any3d) HIPCHK(launch_mid(ctx(PH_MIDY_G), 1, 1, 0, 1, nullptr));
+    HIPCHK(launch_x_c2r(ctx(PH_C2R), true, 0, nullptr, nullptr));
+    HIPCHK(launch_particles(ctx(PH_PART), integrate, write_forces ? 1 : 0));
+    if (integrate) HIPCHK(launch_advance(stream_, d_reps_, m.d_ids, m.nids));
+    return 0;
+}
+
+int Batch::forces(int id, std::string& err) {
+    Replica* R = rep(id);
+    if (!R) { err = "no such replica"; return -TILD_ERR_ARG; }
+    HIPCHK(hipSetDevice(device_));
+    SetMaps* m = maps_for({id}, err);
+    if (!m) return -TILD_ERR_DEVICE;
+    return enqueue_step(*m, false, true, err);
+}
+
+int Batch::step(const std::vector<int>& ids_in, int n, std::string& err) {
+    std::vector<int> ids = ids_in;
+    if (ids.empty())
+        for (int k = 0; k < (int)reps_.size(); ++k) if (reps_[k] && reps_[k]->active) ids.push_back(k);
+    if (ids.empty() || n <= 0) return 0;
+    HIPCHK(hipSetDevice(device_));
+    SetMaps* m = maps_for(ids, err);
+    if (!m) return -TILD_ERR_DEVICE;
+    for (int s = 0; s < n; ++s) {
+        if (int rc = enqueue_step(*m, true, false, err)) return rc;
+        for (int id : ids) reps_[id]->step += 1;
+    }
+    for (int id : ids) reps_[id]->h.step = reps_[id]->step;
+    return 0;
+}
+
+int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std::string& err) {
+    if (n <= 0 || !ms || nph < kProfPhases) { err = "profile: bad arguments"; return -TILD_ERR_ARG; }
+    HIPCHK(hipSetDevice(device_));
+    SetMaps* m = maps_for(ids, err);
+    if (!m) return -TILD_ERR_DEVICE;
+    hipEvent_t ev[kProfPhases + 1];
+    for (int k = 0; k <= kProfPhases; ++k) HIPCHK(hipEventCreate(&ev[k]));
+    std::vector<double> acc(kProfPhases, 0.0);
+    PhaseMap* mp = m->maps;
+    auto ctx = [&](Phase p) { return LaunchCtx{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_}; };
+    bool any3d = mp[PH_MIDY_G].n > 0;
+    int rc = 0;
+    for (int s = 0; s < n && !rc; ++s) {
+        hipError_t e0 = hipSuccess;
+        auto rec = [&](int k) { if (e0 == hipSuccess) e0 = hipEventRecord(ev[k], stream_); };
+        rec(0);
+        if (e0 == hipSuccess) e0 = launch_spread(ctx(PH_PART));
+        rec(1);
+        if (e0 == hipSuccess) e0 = launch_x_r2c(ctx(PH_X), true, 0, nullptr, nullptr);
+        rec(2);
+        if (e0 == hipSuccess && any3d) e0 = launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr);
+        rec(3);
+        if (e0 == hipSuccess) e0 = launch_last_fused(ctx(PH_LAST), 0);
+        rec(4);
+        if (e0 == hipSuccess && any3d) e0 = launch_mid(ctx(PH_MIDY_G), 1, 1, 0, 1, nullptr);
+        rec(5);
+        if (e0 == hipSuccess) e0 = launch_x_c2r(ctx(PH_C2R), true, 0, nullptr, nullptr);
+        rec(6);
+        if (e0 == hipSuccess) e0 = launch_particles(ctx(PH_PART), true, 0);
+        rec(7);
+        if (e0 == hipSuccess) e0 = launch_advance(stream_, d_reps_, m->d_ids, m->nids);
+        rec(8);
+        if (e0 == hipSuccess) e0 = hipEventSynchronize(ev[kProfPhases]);
+        if (e0 != hipSuccess) { err = std::string("profile: ") + hipGetErrorString(e0); rc = -TILD_ERR_DEVICE; break; }
+        for (int k = 0; k < kProfPhases; ++k) {
+            float t = 0.f;
+            (void)hipEventElapsedTime(&t, ev[k], ev[k + 1]);
+            acc[k] += t;
+        }
+        for (int id : ids) reps_[id]->step += 1;
+    }
+    for (int id : ids) reps_[id]->h.step = reps_[id]->step;
+    for (int k = 0; k <= kProfPhases; ++k) (void)hipEventDestroy(ev[k]);
+    for (int k = 0; k < kProfPhases; ++k) ms[k] = acc[k] / n;
+    return rc;
+}
+
+int Batch::forward_spectra(int id, std::string& err) {
+    Replica& R = *reps_[id];
+    PhaseMap* mp = R.maps;
+    auto ctx = [&](Phase p) { return LaunchCtx{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_}; };
+    HIPCHK(launch_x_r2c(ctx(PH_X), false, 0, nullptr, nullptr));
+    HIPCHK(launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr));
+    if (R.dim == 3) HIPCHK(launch_mid(ctx(PH_MIDZ_H), 2, 0, 0, 0, nullptr));
+    return 0;
+}
+
+int Batch::properties(int id, tild_props* out, std::string& err) {
+    Replica* Rp = rep(id);
+    if (!Rp || !out) { err = "bad replica or output"; return -TILD_ERR_ARG; }
+    Replica& R = *Rp;
+    HIPCHK(hipSetDevice(device_));
+    if (int rc = forward_spectra(id, err)) return rc;
+    PhaseMap* mp = R.maps;
+    auto ctx = [&](Phase p) { return LaunchCtx{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_}; };
+    HIPCHK(launch_kprops(ctx(PH_KPROPS), R.nq));
+    HIPCHK(launch_reduce_rows(stream_, R.h.partials, R.nq, R.h.nchunk, R.d_results));
+    HIPCHK(launch_bond_props(ctx(PH_PART)));
+    HIPCHK(launch_reduce_rows(stream_, R.d_bpart, 7, R.nblk_part, R.d_results + R.nq));
+    std::vector<double> res((size_t)R.nq + 7);
+    HIPCHK(hipMemcpyAsync(res.data(), R.d_results, res.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    std::memset(out, 0, sizeof(*out));
+    const int nP = R.nP;
+    out->n_styles = R.nstyles;
+    double Ust = 0.0;
+    out->nan_flag = 0;
+    for (int s = 0; s < R.nstyles && s < TILD_MAX_STYLES; ++s) {
+        out->U_style[s] = res[s];
+        Ust += res[s];
+        if (std::isnan(res[s])) out->nan_flag = 1;
+    }
+    double P[6] = {0, 0, 0, 0, 0, 0};
+    for (int p = 0; p < R.npairs; ++p)
+        for (int c = 0; c < nP; ++c) P[c] += res[R.nstyles + p * nP + c];
+    out->Uelec = (R.nf > R.ntypes) ? res[R.nstyles + R.npairs * nP] : 0.0;
+    out->Ubond = res[R.nq];
+    const double inv = 1.0 / R.V / (double)R.dim;
+    for (int c = 0; c < nP; ++c) out->Ptens[c] = P[c] + res[R.nq + 1 + c] * inv;
+    out->Upe = Ust + out->Uelec + out->Ubond;
+    out->step = R.step;
+    return 0;
+}
+
+// ---------------------------------------------------------------- host copies
+int Batch::get_positions(int id, float* x, size_t n, std::string& err) {
+    Replica* R = rep(id);
+    if (!R || !x || n != (size_t)R->ns * R->dim) { err = "get_positions: size mismatch"; return -TILD_ERR_ARG; }
+    HIPCHK(hipMemcpyAsync(x, R->h.X[R->step & 1], n * 4, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return 0;
+}
+
+int Batch::set_positions(int id, const float* x, size_t n, std::string& err) {
+    Replica* R = rep(id);
+    if (!R || !x || n != (size_t)R->ns * R->dim) { err = "set_positions: size mismatch"; return -TILD_ERR_ARG; }
+    HIPCHK(hipMemcpyAsync(R->h.X[R->step & 1], x, n * 4, hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipMemsetAsync(R->h.XO, 0, n * 4, stream_));   // xo = x
+    HIPCHK(hipStreamSynchronize(stream_));
+    return 0;
+}
+
+int Batch::get_forces(int id, float* f, size_t n, std::string& err) {
+    Replica* R = rep(id);
+    if (!R || !f || n != (size_t)R->ns * R->dim) { err = "get_forces: size mismatch"; return -TILD_ERR_ARG; }
+    HIPCHK(hipMemcpyAsync(f, R->d_fout, n * 4, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return 0;
+}
+
+int Batch::get_density(int id, int type, float* rho, size_t n, std::string& err) {
+    Replica* R = rep(id);
+    if (!R || !rho || n != (size_t)R->M || type < 0 || type >= R->nf) { err = "get_density: bad type or size"; return -TILD_ERR_ARG; }
+    HIPCHK(hipMemcpyAsync(rho, R->h.rho + (size_t)type * R->M, n * 4, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return 0;
+}
+
+int Batch::get_grid_field(int id, int type, int dir, float* g, size_t n, std::string& err) {
+    Replica* R = rep(id);
+    if (!R || !g || n != (size_t)R->M || type < 0 || type >= R->nf || dir < 0 || dir >= R->dim) {
+        err = "get_grid_field: bad type, direction or size";
+        return -TILD_ERR_ARG;
+    }
+    std::vector<float4> tmp(R->M);
+    HIPCHK(hipMemcpyAsync(tmp.data(), R->h.G4 + (size_t)type * R->M, (size_t)R->M * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    for (int i = 0; i < R->M; ++i) g[i] = dir == 0 ? tmp[i].x : (dir == 1 ? tmp[i].y : tmp[i].z);
+    return 0;
+}
+
+int Batch::get_bins(int id, int* out, size_t n, std::string& err) {
+    Replica* R = rep(id);
+    if (!R || !out || n != (size_t)R->ns * R->dim) { err = "get_bins: size mismatch"; return -TILD_ERR_ARG; }
+    int* d;
+    HIPCHK(hipMalloc((void**)&d, n * sizeof(int)));
+    hipError_t e1 = launch_bins(stream_, d_reps_, id, R->ns, d);
+    hipError_t e2 = hipMemcpyAsync(out, d, n * sizeof(int), hipMemcpyDeviceToHost, stream_);
+    hipError_t e3 = hipStreamSynchronize(stream_);
+    (void)hipFree(d);
+    HIPCHK(e1); HIPCHK(e2); HIPCHK(e3);
+    return 0;
+}
+
+int Batch::sync(std::string& err) {
+    HIPCHK(hipStreamSynchronize(stream_));
+    return 0;
+}
+
+}  // namespace tild

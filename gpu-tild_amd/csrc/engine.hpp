@@ -1,0 +1,106 @@
+// engine.hpp -- device-resident TILD replicas and their step pipeline.
+//
+// A Batch owns a HIP stream, an array of replica descriptors in device memory and
+// the per-phase work maps.  A single engine (C ABI tild_engine) is a Batch with one
+// replica; batched sweeps advance every active replica with the same ~8 launches
+// per step.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/tild.h"
+#include "device.hpp"
+#include "launch.hpp"
+#include "system.hpp"
+
+namespace tild {
+
+constexpr int kProfPhases = 8;   // spread, x_r2c, y_fwd, last_fused, y_inv, x_c2r, particles, advance
+
+enum Phase { PH_PART = 0, PH_X, PH_MIDY_H, PH_MIDY_G, PH_MIDZ_H, PH_LAST, PH_C2R, PH_KPROPS, PH_COUNT };
+
+struct PhaseMap {
+    std::vector<WorkItem> host;
+    WorkItem* dev = nullptr;
+    size_t lds = 0;
+    int n = 0;
+};
+
+struct Replica {
+    RepDev h{};                       // host copy of the device descriptor
+    std::vector<void*> allocs;        // owned device allocations
+    int ns = 0, dim = 3, ntypes = 0, nf = 0, M = 0, Mh = 0, nP = 6;
+    int nstyles = 0, npairs = 0, nq = 0;
+    float L[3] = {1, 1, 1};
+    double V = 1.0;
+    int nblk_part = 0;
+    size_t bytes = 0;                 // device memory held
+    bool active = true;
+    int step = 0;                     // host mirror of h.step
+    PhaseMap maps[PH_COUNT];          // single-replica work maps
+    double* d_results = nullptr;      // nq + 7 results of the last property evaluation
+    double* d_bpart = nullptr;        // 7 * nblk_part bond partials
+    float* d_fout = nullptr;
+};
+
+class Batch {
+public:
+    explicit Batch(int device) : device_(device) {}
+    ~Batch();
+    int init(std::string& err);
+
+    // Build a replica from a parsed deck; returns its id (>= 0) or a negative tild_status.
+    int add(const System& S, uint32_t seed, std::string& err);
+    void remove(int id);
+    int size() const { return (int)reps_.size(); }
+    Replica* rep(int id) { return (id >= 0 && id < (int)reps_.size() && reps_[id]) ? reps_[id].get() : nullptr; }
+    static size_t estimate_bytes(const System& S);
+
+    // forces() of one replica at its current positions (no integration).
+    int forces(int id, std::string& err);
+    // n steps of every replica in `ids` (empty = all active), one launch per phase.
+    int step(const std::vector<int>& ids, int n, std::string& err);
+    // n steps with an event pair around every launch; ms[k] = mean ms of phase k.
+    int profile(const std::vector<int>& ids, int n, double* ms, int nph, std::string& err);
+    // calc_properties() of one replica.
+    int properties(int id, tild_props* out, std::string& err);
+
+    int get_positions(int id, float* x, size_t n, std::string& err);
+    int set_positions(int id, const float* x, size_t n, std::string& err);
+    int get_forces(int id, float* f, size_t n, std::string& err);
+    int get_density(int id, int type, float* rho, size_t n, std::string& err);
+    int get_grid_field(int id, int type, int dir, float* g, size_t n, std::string& err);
+    int get_bins(int id, int* base, size_t n, std::string& err);
+    int sync(std::string& err);
+    hipStream_t stream() const { return stream_; }
+
+    // Work maps of an arbitrary replica set (cached by set).
+    struct SetMaps { PhaseMap maps[PH_COUNT]; int* d_ids = nullptr; int nids = 0; };
+    SetMaps* maps_for(const std::vector<int>& ids, std::string& err);
+
+    // Step pipeline of a replica set with prepared maps (graph-capturable).
+    int enqueue_step(SetMaps& m, bool integrate, bool write_forces, std::string& err);
+
+private:
+    int device_;
+    hipStream_t stream_ = nullptr;
+    std::vector<std::unique_ptr<Replica>> reps_;
+    RepDev* d_reps_ = nullptr;
+    int d_cap_ = 0;
+    std::map<std::vector<int>, std::unique_ptr<SetMaps>> set_cache_;
+
+    int upload_desc(int id, std::string& err);
+    int build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& err);
+    int build_tables(Replica& R, const System& S, std::string& err);
+    int forward_spectra(int id, std::string& err);   // rho -> H (properties)
+    void free_maps(PhaseMap* maps);
+};
+
+// Plan of one FFT axis: radices and twiddle table (forward sign).
+int make_axis_plan(int N, AxisPlan& pl, std::vector<float2>& tw, std::string& err);
+
+}  // namespace tild

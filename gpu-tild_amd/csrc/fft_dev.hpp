@@ -1,0 +1,170 @@
+// fft_dev.hpp -- in-LDS mixed-radix Stockham FFT executed by one thread block.
+//
+// Data: nb independent lines of length N, element n of line b at buf[n*ldb + b].
+// Only the forward transform (e^{-2 pi i nk/N}) is implemented here; callers form
+// the inverse as conj(FFT(conj(x))) by conjugating on load and store.
+//
+// Each stage (radix R, p = product of earlier radices) is done in place: every
+// thread first loads the R inputs of each of its butterflies into registers, the
+// block synchronises, then each butterfly is twiddled, transformed and written to
+// its Stockham output slots:  u_r = x[i + r N/R] w_{pR}^{r k},  k = i mod p,
+//                             y[(i - k) R + k + q p] = DFT_R(u)_q.
+#pragma once
+#include "device.hpp"
+
+namespace tild {
+
+__device__ inline float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ inline float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }   // a * (-i)
+__device__ inline float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+
+// Forward DFT of small sizes, in place on v[0..R-1].
+template <int R> struct Dft;
+
+template <> struct Dft<2> {
+    __device__ static void run(float2* v) {
+        float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b); v[1] = csub(a, b);
+    }
+};
+template <> struct Dft<3> {
+    __device__ static void run(float2* v) {
+        const float s3 = 0.86602540378443864676f;   // sin(2 pi / 3)
+        float2 t = cadd(v[1], v[2]);
+        float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
+        float2 d = csub(v[1], v[2]);
+        float2 e = make_float2(s3 * d.y, -s3 * d.x);  // -i sin(2pi/3) d
+        v[0] = cadd(v[0], t);
+        v[1] = cadd(m, e);
+        v[2] = csub(m, e);
+    }
+};
+template <> struct Dft<4> {
+    __device__ static void run(float2* v) {
+        float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        float2 t2 = cadd(v[1], v[3]), t3 = mul_mi(csub(v[1], v[3]));
+        v[0] = cadd(t0, t2); v[2] = csub(t0, t2);
+        v[1] = cadd(t1, t3); v[3] = csub(t1, t3);
+    }
+};
+template <> struct Dft<8> {
+    __device__ static void run(float2* v) {
+        const float h = 0.70710678118654752440f;
+        float2 a[4] = {v[0], v[2], v[4], v[6]}, b[4] = {v[1], v[3], v[5], v[7]};
+        Dft<4>::run(a); Dft<4>::run(b);
+        // twiddles w8^k, k = 0..3: 1, (h,-h), -i, (-h,-h)
+        float2 b1 = make_float2(h * (b[1].x + b[1].y), h * (b[1].y - b[1].x));
+        float2 b2 = mul_mi(b[2]);
+        float2 b3 = make_float2(h * (b[3].y - b[3].x), -h * (b[3].x + b[3].y));
+        v[0] = cadd(a[0], b[0]); v[4] = csub(a[0], b[0]);
+        v[1] = cadd(a[1], b1);   v[5] = csub(a[1], b1);
+        v[2] = cadd(a[2], b2);   v[6] = csub(a[2], b2);
+        v[3] = cadd(a[3], b3);   v[7] = csub(a[3], b3);
+    }
+};
+// Odd prime radices: real/imaginary symmetric pairs (Rader-free direct form).
+template <int R> struct DftOdd {
+    __device__ static void run(float2* v, const float* c, const float* s) {
+        // c[j] = cos(2 pi j / R), s[j] = sin(2 pi j / R), j = 1..(R-1)/2
+        float2 out[R];
+        float2 sum = v[0];
+#pragma unroll
+        for (int j = 1; j < R; ++j) sum = cadd(sum, v[j]);
+        out[0] = sum;
+#pragma unroll
+        for (int q = 1; q <= (R - 1) / 2; ++q) {
+            float2 re = v[0], im = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int j = 1; j <= (R - 1) / 2; ++j) {
+                int e = (j * q) % R;
+                int ee = e <= (R - 1) / 2 ? e : R - e;
+                float cc = c[ee];
+                float ss = e <= (R - 1) / 2 ? s[ee] : -s[ee];
+                float2 ap = cadd(v[j], v[R - j]), am = csub(v[j], v[R - j]);
+                re.x += cc * ap.x; re.y += cc * ap.y;
+                im.x += ss * am.x; im.y += ss * am.y;
+            }
+            // X_q = re - i*im ; X_{R-q} = re + i*im
+            out[q] = make_float2(re.x + im.y, re.y - im.x);
+            out[R - q] = make_float2(re.x - im.y, re.y + im.x);
+        }
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[j] = out[j];
+    }
+};
+template <> struct Dft<5> {
+    __device__ static void run(float2* v) {
+        const float c[3] = {1.f, 0.30901699437494742410f, -0.80901699437494742410f};
+        const float s[3] = {0.f, 0.95105651629515357212f, 0.58778525229247312917f};
+        DftOdd<5>::run(v, c, s);
+    }
+};
+template <> struct Dft<7> {
+    __device__ static void run(float2* v) {
+        const float c[4] = {1.f, 0.62348980185873353053f, -0.22252093395631440429f, -0.90096886790241912624f};
+        const float s[4] = {0.f, 0.78183148246802980871f, 0.97492791218182360702f, 0.43388373911755812048f};
+        DftOdd<7>::run(v, c, s);
+    }
+};
+
+template <int R>
+__device__ inline void fft_stage(float2* buf, int ldb, int nb, int N, int p, const float2* tw) {
+    constexpr int MAXB = (kFftElems + R - 1) / R;
+    const int m = N / R, nbf = m * nb;
+    float2 v[MAXB][R];
+    int tb[MAXB], ti[MAXB];
+#pragma unroll
+    for (int j = 0; j < MAXB; ++j) {
+        int task = threadIdx.x + j * kFftThreads;
+        tb[j] = -1;
+        if (task < nbf) {
+            int b = task % nb, i = task / nb;
+            tb[j] = b; ti[j] = i;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[j][r] = buf[(i + r * m) * ldb + b];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < MAXB; ++j) {
+        if (tb[j] >= 0) {
+            int b = tb[j], i = ti[j], k = i % p;
+            if (p > 1) {
+                const float2* w = tw + k * (R - 1);
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[j][r] = cmul(v[j][r], w[r - 1]);
+            }
+            Dft<R>::run(v[j]);
+            int o = (i - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) buf[(o + q * p) * ldb + b] = v[j][q];
+        }
+    }
+    __syncthreads();
+}
+
+// Forward FFT of nb lines in LDS.  Requires N*nb <= kFftElems*kFftThreads and a
+// block of kFftThreads threads; callers synchronise before (data loaded) -- this
+// function synchronises after each stage.
+__device__ inline void lds_fft(float2* buf, int ldb, int nb, const AxisPlan& pl) {
+    int p = 1;
+    for (int s = 0; s < pl.nst; ++s) {
+        const int R = pl.rad[s];
+        const float2* tw = pl.tw + pl.tw_off[s];
+        switch (R) {
+            case 2: fft_stage<2>(buf, ldb, nb, pl.N, p, tw); break;
+            case 3: fft_stage<3>(buf, ldb, nb, pl.N, p, tw); break;
+            case 4: fft_stage<4>(buf, ldb, nb, pl.N, p, tw); break;
+            case 5: fft_stage<5>(buf, ldb, nb, pl.N, p, tw); break;
+            case 7: fft_stage<7>(buf, ldb, nb, pl.N, p, tw); break;
+            default: fft_stage<8>(buf, ldb, nb, pl.N, p, tw); break;
+        }
+        p *= R;
+    }
+}
+
+}  // namespace tild

@@ -1,0 +1,645 @@
+// kernels.hip -- per-step kernels of the TILD engine (CDNA4 / gfx950).
+//
+// Step pipeline for each replica (3D; 2D drops the middle passes):
+//   k_spread        particles -> fixed-point density accumulators (deterministic)
+//   k_x_r2c         accumulators -> rho (real) -> x-axis R2C half spectra H
+//   k_mid (fwd)     y-axis FFT of H
+//   k_last_fused    z-axis FFT, k-space coupling sum_J C_IJ(k) rho_J(k), -i k_d,
+//                   inverse z-axis FFT -> Gk
+//   k_mid (inv)     inverse y-axis FFT of Gk
+//   k_x_c2r         inverse x-axis C2R -> float4 (g_x, g_y, g_z, rho) records
+//   k_particles     gather + bonds + EM/GJF update (double-buffered positions)
+// Log steps add k_kprops (energies/virial from the spectra), k_bond_props and
+// k_reduce (fixed-order FP64 sums).
+#include "device.hpp"
+#include "fft_dev.hpp"
+#include "launch.hpp"
+
+namespace tild {
+
+// ------------------------------------------------------------------ spread
+// One thread per particle deposits its (P+1)^D stencil into 64-bit fixed-point
+// accumulators with integer atomics: integer addition is associative, so the
+// result is independent of thread order (unlike float atomicAdd,
+// device_grid_utils.cu:173-331).
+__global__ void __launch_bounds__(256) k_spread(const RepDev* __restrict__ reps,
+                                                const WorkItem* __restrict__ wmap) {
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    const int i = wi.blk * 256 + threadIdx.x;
+    if (i >= r.ns) return;
+    const int D = r.dim, P = r.P;
+    const float* x = r.X[r.step & 1];
+    float W[3][5];
+    int n0[3];
+    n0[2] = 0; W[2][0] = 1.f;
+    for (int d = 0; d < D; ++d) {
+        int N = d == 0 ? r.Nx : (d == 1 ? r.Ny : r.Nz);
+        n0[d] = stencil_1d(x[i * D + d], r.dx[d], N, P, W[d]);
+    }
+    const int t = r.tp[i];
+    unsigned long long* acc = r.acc + (size_t)t * r.M;
+    const bool charged = r.q != nullptr;
+    const float qi = charged ? r.q[i] : 0.f;
+    unsigned long long* accq = r.acc + (size_t)r.ntypes * r.M;
+    const int nz = D == 3 ? P + 1 : 1;
+    int iz0 = n0[2];
+    for (int iz = 0, nzz = iz0; iz < nz; ++iz, nzz = wrap_next(nzz, r.Nz)) {
+        for (int iy = 0, nyy = n0[1]; iy <= P; ++iy, nyy = wrap_next(nyy, r.Ny)) {
+            const int rowb = (nzz * r.Ny + nyy) * r.Nx;
+            for (int ix = 0, nxx = n0[0]; ix <= P; ++ix, nxx = wrap_next(nxx, r.Nx)) {
+                // Wx*Wy*Wz in the reference's product order (device_grid_utils.cu:329)
+                float w = D == 3 ? W[0][ix] * W[1][iy] * W[2][iz] : W[0][ix] * W[1][iy];
+                long long v = __float2ll_rn(w * r.fix_scale);
+                atomicAdd(acc + rowb + nxx, (unsigned long long)v);
+                if (charged) {
+                    long long vq = __float2ll_rn((w * qi) * r.fixq_scale);
+                    atomicAdd(accq + rowb + nxx, (unsigned long long)vq);
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ x R2C
+// Two real rows a, b are transformed as one complex row z = a + ib; then
+// A_k = (Z_k + conj Z_{-k})/2, B_k = (Z_k - conj Z_{-k})/(2i), k in [0, N/2].
+// FROM_ACC: rows come from the fixed-point accumulators (converted, written to rho
+// and cleared); otherwise from rho.
+template <bool FROM_ACC>
+__global__ void __launch_bounds__(kFftThreads) k_x_r2c(const RepDev* __restrict__ reps,
+                                                       const WorkItem* __restrict__ wmap,
+                                                       int nrows_override, const float* __restrict__ in_override,
+                                                       float2* __restrict__ out_override) {
+    extern __shared__ float2 lds[];
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    const int N = r.Nx, Nxh = r.Nxh;
+    const int rows_per_field = r.M / r.Nx;
+    const int nrows = nrows_override > 0 ? nrows_override : r.nf * rows_per_field;
+    const int npairs = (nrows + 1) / 2;
+    const int nl = r.nl_x;
+    const int p0 = wi.blk * nl;
+    const int nb = min(nl, npairs - p0);
+    const int ldb = nb | 1;
+    const float* rin = in_override ? in_override : r.rho;
+    float2* Hout = out_override ? out_override : r.H;
+    for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
+        const int b = task / N, n = task % N;
+        const int ra = 2 * (p0 + b), rb = ra + 1;
+        float a, bb = 0.f;
+        if (FROM_ACC) {
+            const int fa = ra / rows_per_field;
+            size_t ia = (size_t)ra * N + n;
+            a = (float)((double)(long long)r.acc[ia] * (fa < r.ntypes ? r.fix_inv : r.fixq_inv));
+            r.acc[ia] = 0ull;
+            r.rho[ia] = a;
+            if (rb < nrows) {
+                const int fb = rb / rows_per_field;
+                size_t ib = (size_t)rb * N + n;
+                bb = (float)((double)(long long)r.acc[ib] * (fb < r.ntypes ? r.fix_inv : r.fixq_inv));
+                r.acc[ib] = 0ull;
+                r.rho[ib] = bb;
+            }
+        } else {
+            a = rin[(size_t)ra * N + n];
+            if (rb < nrows) bb = rin[(size_t)rb * N + n];
+        }
+        lds[n * ldb + b] = make_float2(a, bb);
+    }
+    __syncthreads();
+    lds_fft(lds, ldb, nb, r.ax[0]);
+    for (int task = threadIdx.x; task < nb * Nxh; task += kFftThreads) {
+        const int b = task / Nxh, k = task % Nxh;
+        const float2 Zk = lds[k * ldb + b];
+        const float2 Zm = lds[((N - k) % N) * ldb + b];
+        const int ra = 2 * (p0 + b), rb = ra + 1;
+        Hout[(size_t)ra * Nxh + k] = make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y));
+        if (rb < nrows) Hout[(size_t)rb * Nxh + k] = make_float2(0.5f * (Zk.y + Zm.y), 0.5f * (Zm.x - Zk.x));
+    }
+}
+
+// ------------------------------------------------------------------ strided pass
+// FFT along axis `axis` (1 = y, 2 = z) of nfb half-spectrum fields stored in buf
+// (x fastest, Nxh columns).  Lines are batched over the contiguous columns so that
+// every global access is a run of consecutive complex values.
+__global__ void __launch_bounds__(kFftThreads) k_mid(const RepDev* __restrict__ reps,
+                                                     const WorkItem* __restrict__ wmap,
+                                                     int axis, int use_gk, int nfb_override,
+                                                     int inverse, float2* __restrict__ buf_override) {
+    extern __shared__ float2 lds[];
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    MidGeom g = mid_geom(r, axis, nfb_override > 0 ? nfb_override : (use_gk ? r.nf * r.dim : r.nf));
+    float2* buf = buf_override ? buf_override : (use_gk ? r.Gk : r.H);
+    const int o = wi.blk / g.nchunk, c = wi.blk % g.nchunk;
+    const int b0 = c * g.nl;
+    const int nb = min(g.nl, g.batch - b0);
+    const int ldb = nb | 1;
+    float2* base = buf + (size_t)o * g.ostride + b0;
+    for (int task = threadIdx.x; task < nb * g.N; task += kFftThreads) {
+        const int b = task % nb, n = task / nb;
+        float2 v = base[(size_t)n * g.lstride + b];
+        lds[n * ldb + b] = inverse ? conjf2(v) : v;
+    }
+    __syncthreads();
+    lds_fft(lds, ldb, nb, r.ax[axis]);
+    for (int task = threadIdx.x; task < nb * g.N; task += kFftThreads) {
+        const int b = task % nb, n = task / nb;
+        float2 v = lds[n * ldb + b];
+        base[(size_t)n * g.lstride + b] = inverse ? conjf2(v) : v;
+    }
+}
+
+// ------------------------------------------------------------------ fused last axis
+// Last-axis forward FFT of every field, the k-space coupling
+//   w_I(k) = sum_J C_IJ(k) rho_J(k)       (C includes 1/M and counts each style in
+//                                          both orientations, pair_style.cu:81-196)
+//   w_c(k) = G_c(k) rho_c(k)              (smeared charges, double smearing,
+//                                          device_array_utils.cu:199-241)
+// then for each direction d the line -i k_d w(k) (k_d = 0 on the d-Nyquist plane,
+// where the reference's term is anti-Hermitian and drops out of Re IFFT) is inverse
+// transformed along the last axis into Gk[f*dim + d].
+__global__ void __launch_bounds__(kFftThreads) k_last_fused(const RepDev* __restrict__ reps,
+                                                            const WorkItem* __restrict__ wmap,
+                                                            int store_spectrum) {
+    extern __shared__ float2 lds[];
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    const int la = r.dim - 1;                       // last axis
+    const int N = la == 2 ? r.Nz : r.Ny;
+    const int Q = la == 2 ? r.Ny * r.Nxh : r.Nxh;   // plane size = line stride
+    const int nf = r.nf, T = r.ntypes;
+    const int q0 = wi.blk * r.nl_last;
+    const int nb = min(r.nl_last, Q - q0);
+    const int nlines = nf * nb;
+    const int ldb = nlines | 1;
+    float2* A = lds;
+    float2* B = lds + (size_t)N * ldb;
+    for (int task = threadIdx.x; task < nlines * N; task += kFftThreads) {
+        const int b = task % nb, rest = task / nb, f = rest % nf, n = rest / nf;
+        A[n * ldb + f * nb + b] = r.H[(size_t)f * r.Mh + (size_t)n * Q + q0 + b];
+    }
+    __syncthreads();
+    lds_fft(A, ldb, nlines, r.ax[la]);
+    if (store_spectrum) {
+        for (int task = threadIdx.x; task < nlines * N; task += kFftThreads) {
+            const int b = task % nb, rest = task / nb, f = rest % nf, n = rest / nf;
+            r.H[(size_t)f * r.Mh + (size_t)n * Q + q0 + b] = A[n * ldb + f * nb + b];
+        }
+    }
+    // coupling, in place per element
+    for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
+        const int b = task % nb, n = task / nb;
+        const size_t idx = (size_t)n * Q + q0 + b;
+        float2 rh[kMaxTypes + 1];
+        for (int f = 0; f < nf; ++f) rh[f] = A[n * ldb + f * nb + b];
+        int pidx = 0;
+        float2 w[kMaxTypes + 1];
+        for (int I = 0; I < T; ++I) w[I] = make_float2(0.f, 0.f);
+        for (int I = 0; I < T; ++I)
+            for (int J = I; J < T; ++J, ++pidx) {
+                const float c = r.Ctab[(size_t)pidx * r.Mh + idx];
+                w[I].x += c * rh[J].x; w[I].y += c * rh[J].y;
+                if (J != I) { w[J].x += c * rh[I].x; w[J].y += c * rh[I].y; }
+            }
+        if (nf > T) {
+            const float c = r.Ctab[(size_t)r.ctab_charge * r.Mh + idx];
+            w[T] = make_float2(c * rh[T].x, c * rh[T].y);
+        }
+        for (int f = 0; f < nf; ++f) A[n * ldb + f * nb + b] = w[f];
+    }
+    __syncthreads();
+    for (int d = 0; d < r.dim; ++d) {
+        for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
+            const int b = task % nb, n = task / nb;
+            const int q = q0 + b;
+            int sidx, Nd;
+            if (d == 0) { sidx = q % r.Nxh; Nd = r.Nx; }
+            else if (d == la) { sidx = n; Nd = N; }
+            else { sidx = q / r.Nxh; Nd = r.Ny; }
+            float kd;
+            if (2 * sidx == Nd) kd = 0.f;
+            else kd = r.kfac[d] * (float)(2 * sidx < Nd ? sidx : sidx - Nd);
+            for (int f = 0; f < nf; ++f) {
+                float2 w = A[n * ldb + f * nb + b];
+                // conj(-i k w) = (k w.y, k w.x): the inverse is taken as conj(FFT(conj))
+                B[n * ldb + f * nb + b] = make_float2(kd * w.y, kd * w.x);
+            }
+        }
+        __syncthreads();
+        lds_fft(B, ldb, nlines, r.ax[la]);
+        for (int task = threadIdx.x; task < nlines * N; task += kFftThreads) {
+            const int b = task % nb, rest = task / nb, f = rest % nf, n = rest / nf;
+            r.Gk[(size_t)(f * r.dim + d) * r.Mh + (size_t)n * Q + q0 + b] = conjf2(B[n * ldb + f * nb + b]);
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ x C2R
+// Inverse x-axis transform of half-spectrum rows.  Two real output lines share one
+// complex FFT: Z_k = A_k + i B_k with the Hermitian extension Z_{N-k} = conj A_k +
+// i conj B_k; the imaginary parts of the self-conjugate columns (k = 0, N/2) are
+// dropped, which is the Hermitian projection the reference's Re(C2C inverse) takes.
+// RECORDS: output float4 (g_0, g_1, g_2, rho_f) per field and node (fields are lines
+// f*dim + d of Gk); otherwise plain real rows to out_override.
+template <bool RECORDS>
+__global__ void __launch_bounds__(kFftThreads) k_x_c2r(const RepDev* __restrict__ reps,
+                                                       const WorkItem* __restrict__ wmap,
+                                                       int nlines_override, const float2* __restrict__ in_override,
+                                                       float* __restrict__ out_override) {
+    extern __shared__ float2 lds[];
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    const int N = r.Nx, Nxh = r.Nxh;
+    const int rows_per_field = r.M / r.Nx;
+    const int nlines = RECORDS ? r.nf * r.dim : nlines_override;     // real lines per row
+    const int nc = (nlines + 1) / 2;                                  // complex lines per row
+    const int nr = r.nr_c2r;
+    const int row0 = wi.blk * nr;
+    const int rows = min(nr, rows_per_field - row0);
+    const int nb = rows * nc;
+    const int ldb = nb | 1;
+    const float2* Gin = RECORDS ? r.Gk : in_override;
+    const bool even = (N & 1) == 0;
+    for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
+        const int k = task % N, b = task / N;
+        const int row = row0 + b / nc, j = b % nc;
+        const int la = 2 * j, lb = la + 1;
+        const bool upper = k >= Nxh;
+        const int kk = upper ? N - k : k;
+        float2 A = Gin[((size_t)la * rows_per_field + row) * Nxh + kk];
+        float2 Bv = lb < nlines ? Gin[((size_t)lb * rows_per_field + row) * Nxh + kk] : make_float2(0.f, 0.f);
+        if (kk == 0 || (even && 2 * kk == N)) { A.y = 0.f; Bv.y = 0.f; }
+        if (upper) { A.y = -A.y; Bv.y = -Bv.y; }
+        // Z = A + i B ; store conj(Z) for the conj(FFT(conj)) inverse
+        lds[k * ldb + b] = make_float2(A.x - Bv.y, -(A.y + Bv.x));
+    }
+    __syncthreads();
+    lds_fft(lds, ldb, nb, r.ax[0]);
+    if (RECORDS) {
+        const int D = r.dim;
+        for (int task = threadIdx.x; task < rows * r.nf * N; task += kFftThreads) {
+            const int x = task % N, rest = task / N, f = rest % r.nf, rr = rest / r.nf;
+            const int row = row0 + rr;
+            float g[3] = {0.f, 0.f, 0.f};
+            for (int d = 0; d < D; ++d) {
+                const int line = f * D + d;
+                const float2 z = lds[x * ldb + rr * nc + line / 2];
+                g[d] = (line & 1) ? -z.y : z.x;      // conj of the forward result
+            }
+            const size_t node = (size_t)row * N + x;
+            const float rho = f < r.ntypes ? r.rho[(size_t)f * r.M + node] : 1.0f;
+            r.G4[(size_t)f * r.M + node] = make_float4(g[0], g[1], g[2], rho);
+        }
+    } else {
+        for (int task = threadIdx.x; task < rows * nlines * N; task += kFftThreads) {
+            const int x = task % N, rest = task / N, line = rest % nlines, rr = rest / nlines;
+            const float2 z = lds[x * ldb + rr * nc + line / 2];
+            out_override[((size_t)line * rows_per_field + row0 + rr) * N + x] = (line & 1) ? -z.y : z.x;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ particles
+// Gather (device_grid_utils.cu:346-452) + harmonic bonds (device_bonds.cu:4-60) +
+// EM (device_EM_integrator.cu:5-36) / GJF (device_GJF_integrator.cu:4-62).
+// The gather reads g = all_f/rho directly; nodes with rho <= 0 contribute nothing
+// (device_grid_utils.cu:640).  Positions are read from X[step&1] and written to
+// X[(step+1)&1], so bonds always see x_n.
+template <bool INTEGRATE>
+__global__ void __launch_bounds__(256) k_particles(const RepDev* __restrict__ reps,
+                                                   const WorkItem* __restrict__ wmap,
+                                                   int write_forces) {
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    const int i = wi.blk * 256 + threadIdx.x;
+    if (i >= r.ns) return;
+    const int D = r.dim, P = r.P;
+    const float* __restrict__ x = r.X[r.step & 1];
+    float xi[3] = {0.f, 0.f, 0.f};
+    for (int d = 0; d < D; ++d) xi[d] = x[i * D + d];
+    float W[3][5];
+    int n0[3];
+    n0[2] = 0; W[2][0] = 1.f;
+    for (int d = 0; d < D; ++d) {
+        int N = d == 0 ? r.Nx : (d == 1 ? r.Ny : r.Nz);
+        n0[d] = stencil_1d(xi[d], r.dx[d], N, P, W[d]);
+    }
+    const int t = r.tp[i];
+    const float4* __restrict__ G = r.G4 + (size_t)t * r.M;
+    const bool charged = r.q != nullptr;
+    const float qi = charged ? r.q[i] : 0.f;
+    const float4* __restrict__ Gc = r.G4 + (size_t)r.ntypes * r.M;
+    float f[3] = {0.f, 0.f, 0.f}, fc[3] = {0.f, 0.f, 0.f};
+    const int nz = D == 3 ? P + 1 : 1;
+    for (int iz = 0, nzz = n0[2]; iz < nz; ++iz, nzz = wrap_next(nzz, r.Nz)) {
+        for (int iy = 0, nyy = n0[1]; iy <= P; ++iy, nyy = wrap_next(nyy, r.Ny)) {
+            const int rowb = (nzz * r.Ny + nyy) * r.Nx;
+            for (int ix = 0, nxx = n0[0]; ix <= P; ++ix, nxx = wrap_next(nxx, r.Nx)) {
+                const float w = D == 3 ? W[0][ix] * W[1][iy] * W[2][iz] : W[0][ix] * W[1][iy];
+                const float4 g = G[rowb + nxx];
+                if (g.w > 0.f) { f[0] += w * g.x; f[1] += w * g.y; f[2] += w * g.z; }
+                if (charged) {
+                    const float4 e = Gc[rowb + nxx];
+                    const float wq = w * qi;
+                    fc[0] += wq * e.x; fc[1] += wq * e.y; fc[2] += wq * e.z;
+                }
+            }
+        }
+    }
+    for (int d = 0; d < D; ++d) f[d] += fc[d];
+    // bonds: every bond evaluated from both ends, no atomics
+    float lf[3] = {0.f, 0.f, 0.f};
+    for (int b = 0; b < 3; ++b) {
+        const int j = r.bond_j[i * 3 + b];
+        if (j < 0) break;
+        const int bt = r.bond_t[i * 3 + b];
+        float dr[3] = {0.f, 0.f, 0.f}, r2 = 0.f;
+        for (int d = 0; d < D; ++d) {
+            dr[d] = xi[d] - x[j * D + d];
+            if (dr[d] > r.Lh[d]) dr[d] -= r.L[d];
+            else if (dr[d] < -r.Lh[d]) dr[d] += r.L[d];
+            r2 += dr[d] * dr[d];
+        }
+        float mf = 0.f;
+        if (r2 > 1.0e-4f) {
+            const float rr = sqrtf(r2);
+            mf = 2.0f * r.bond_k[bt] * (rr - r.bond_r0[bt]) / rr;
+        }
+        for (int d = 0; d < D; ++d) lf[d] -= mf * dr[d];
+    }
+    for (int d = 0; d < D; ++d) f[d] += lf[d];
+    if (write_forces && r.fout)
+        for (int d = 0; d < D; ++d) r.fout[i * D + d] = f[d];
+    if (!INTEGRATE) return;
+    float* __restrict__ xn = r.X[(r.step + 1) & 1];
+    const int kind = r.ikind[i];
+    if (kind == 0) {
+        for (int d = 0; d < D; ++d) xn[i * D + d] = xi[d];
+        return;
+    }
+    const uint32_t stepc = (uint32_t)(r.step + 1);
+    float z[4];
+    normals4(r.seed, (uint32_t)i, stepc, z);
+    if (kind == 1) {
+        for (int d = 0; d < D; ++d) {
+            float v = xi[d] + r.delt * f[d] + r.noise_mag * z[d];
+            if (v > r.L[d]) v -= r.L[d];
+            else if (v < 0.0f) v += r.L[d];
+            xn[i * D + d] = v;
+        }
+    } else {
+        float zo[4] = {0.f, 0.f, 0.f, 0.f};
+        if (stepc > 1u) normals4(r.seed, (uint32_t)i, stepc - 1u, zo);
+        // GJF (device_GJF_integrator.cu:25-58) in displacement form: with
+        // b = m/(m + dt/2D) and a = (1 - dt/2Dm)/(1 + dt/2Dm), 2b - a = 1 exactly, so
+        //   x' = 2b x - a xo + c1 f + c2 (beta + beta_old)
+        //      = x + [a (x - xo) + c1 f + c2 (beta + beta_old)].
+        // XO holds the unwrapped displacement x - xo, which removes the reference's
+        // minimum-image unwrap of xo and keeps FP32 rounding off the large terms.
+        const float a = r.gjf_a[t], c1 = r.gjf_c1[t], c2 = r.gjf_c2[t], nm = r.gjf_noise[t];
+        for (int d = 0; d < D; ++d) {
+            const float nn = nm * z[d], no = nm * zo[d];
+            const float dn = a * r.XO[i * D + d] + c1 * f[d] + c2 * (nn + no);
+            float v = xi[d] + dn;
+            r.XO[i * D + d] = dn;
+            if (v > r.L[d]) v -= r.L[d];
+            else if (v < 0.0f) v += r.L[d];
+            xn[i * D + d] = v;
+        }
+    }
+}
+
+// First stencil node of every coordinate (the bins of device_grid_utils.cu:136-146).
+__global__ void __launch_bounds__(256) k_bins(const RepDev* __restrict__ reps, int rep, int* __restrict__ out) {
+    const RepDev& r = reps[rep];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= r.ns) return;
+    const float* x = r.X[r.step & 1];
+    float W[5];
+    for (int d = 0; d < r.dim; ++d) {
+        int N = d == 0 ? r.Nx : (d == 1 ? r.Ny : r.Nz);
+        out[i * r.dim + d] = stencil_1d(x[i * r.dim + d], r.dx[d], N, r.P, W);
+    }
+}
+
+// ------------------------------------------------------------------ properties
+// Non-bonded energies and virial by Parseval over the stored half spectra H:
+//   U_s  = gvol/M sum_k w_k u_s(k) Re(conj rho_t1 rho_t2)     (pair_style.cu:204-233)
+//   P_pc = gvol/(V D M) sum_k w_k Re(conj rho_t1 V_pc(k) rho_t2)   (pair_style.cu:235-260)
+//   Uel  = 1/M sum_k w_k phi(k) |rho_c|^2                     (calc_properties.cu:106-116)
+// with w_k = 1 on the self-conjugate x columns and 2 elsewhere.  grid.y = quantity.
+__global__ void __launch_bounds__(256) k_kprops(const RepDev* __restrict__ reps,
+                                                const WorkItem* __restrict__ wmap) {
+    __shared__ double red[256];
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    const int qn = blockIdx.y;
+    if (qn >= r.nq) return;
+    const int nP = r.nP;
+    double s = 0.0;
+    const int per = (r.Mh + r.nchunk - 1) / r.nchunk;
+    const int k0 = wi.blk * per, k1 = min(r.Mh, k0 + per);
+    const bool even = (r.Nx & 1) == 0;
+    for (int idx = k0 + threadIdx.x; idx < k1; idx += 256) {
+        const int kx = idx % r.Nxh;
+        const double w = (kx == 0 || (even && 2 * kx == r.Nx)) ? 1.0 : 2.0;
+        if (qn < r.nstyles) {
+            const float2 a = r.H[(size_t)r.st_t1[qn] * r.Mh + idx], b = r.H[(size_t)r.st_t2[qn] * r.Mh + idx];
+            const double re = (double)a.x * b.x + (double)a.y * b.y;
+            s += w * (double)r.Etab[(size_t)qn * r.Mh + idx] * re;
+        } else if (qn < r.nstyles + r.npairs * nP) {
+            const int pq = qn - r.nstyles, p = pq / nP, c = pq % nP;
+            const float2 a = r.H[(size_t)r.pr_t1[p] * r.Mh + idx], b = r.H[(size_t)r.pr_t2[p] * r.Mh + idx];
+            const float2 v = r.Vtab[((size_t)p * nP + c) * r.Mh + idx];
+            // Re(conj(a) v b)
+            const double ab_re = (double)a.x * b.x + (double)a.y * b.y;
+            const double ab_im = (double)a.x * b.y - (double)a.y * b.x;
+            s += w * ((double)v.x * ab_re - (double)v.y * ab_im);
+        } else {
+            const float2 c = r.H[(size_t)r.ntypes * r.Mh + idx];
+            s += w * (double)r.Eltab[idx] * ((double)c.x * c.x + (double)c.y * c.y);
+        }
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) r.partials[(size_t)qn * r.nchunk + wi.blk] = red[0];
+}
+
+// Bond energy and virial at the current positions (bonds.cu:3-52): each bond once
+// from its lower-index end, no r^2 guard.  Quantities: Ubond, vir[0..nP-1].
+__global__ void __launch_bounds__(256) k_bond_props(const RepDev* __restrict__ reps,
+                                                    const WorkItem* __restrict__ wmap) {
+    __shared__ double red[7][256];
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    const int i = wi.blk * 256 + threadIdx.x;
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (i < r.ns) {
+        const int D = r.dim;
+        const float* x = r.X[r.step & 1];
+        for (int b = 0; b < 3; ++b) {
+            const int j = r.bond_j[i * 3 + b];
+            if (j < 0) break;
+            if (j < i) continue;
+            const int bt = r.bond_t[i * 3 + b];
+            double dr[3] = {0, 0, 0}, r2 = 0.0;
+            for (int d = 0; d < D; ++d) {
+                float df = x[i * D + d] - x[j * D + d];
+                if (df > r.Lh[d]) df -= r.L[d];
+                else if (df < -r.Lh[d]) df += r.L[d];
+                dr[d] = df;
+                r2 += dr[d] * dr[d];
+            }
+            const double rr = sqrt(r2), del = rr - (double)r.bond_r0[bt];
+            acc[0] += del * del * (double)r.bond_k[bt];
+            const double mf = 2.0 * (double)r.bond_k[bt] * del / rr;
+            acc[1] += -mf * dr[0] * dr[0];
+            acc[2] += -mf * dr[1] * dr[1];
+            if (D == 2) acc[3] += -mf * dr[0] * dr[1];
+            else {
+                acc[3] += -mf * dr[2] * dr[2];
+                acc[4] += -mf * dr[0] * dr[1];
+                acc[5] += -mf * dr[0] * dr[2];
+                acc[6] += -mf * dr[1] * dr[2];
+            }
+        }
+    }
+    for (int q = 0; q < 7; ++q) red[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            for (int q = 0; q < 7; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + o];
+        __syncthreads();
+    }
+    const int nblk = (r.ns + 255) / 256;
+    if (threadIdx.x < 7) r.bpart[(size_t)threadIdx.x * nblk + wi.blk] = red[threadIdx.x][0];
+}
+
+// Fixed-order sums of per-block partials: out[q] = sum_b in[q*n + b].
+__global__ void __launch_bounds__(256) k_reduce_rows(const double* __restrict__ in, int n,
+                                                     double* __restrict__ out) {
+    __shared__ double red[256];
+    const int q = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < n; b += 256) s += in[(size_t)q * n + b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = red[0];
+}
+
+__global__ void k_advance(RepDev* reps, const int* __restrict__ which, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) reps[which[k]].step += 1;
+}
+
+// Virial source fields of one pair style (d_initVirial, device_array_utils.cu:456-494):
+// vir_c(r) = dr_a(r) f_b(r) with dr the minimum image of the node position
+// (wrap test in FP32 as the reference), components xx yy [zz] xy [xz yz].
+__global__ void k_virial_source(const RepDev* __restrict__ reps, int rep,
+                                const float* __restrict__ fr, float* __restrict__ vr) {
+    const RepDev& r = reps[rep];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= r.M) return;
+    const int D = r.dim;
+    int nn[3];
+    nn[0] = i % r.Nx;
+    nn[1] = (i / r.Nx) % r.Ny;
+    nn[2] = D == 3 ? i / (r.Nx * r.Ny) : 0;
+    float dr[3];
+    for (int j = 0; j < D; ++j) {
+        float v = (float)nn[j] * r.dx[j];
+        if (v > r.Lh[j]) v -= r.L[j];
+        else if (v < -r.Lh[j]) v += r.L[j];
+        dr[j] = v;
+    }
+    float f[3];
+    for (int d = 0; d < D; ++d) f[d] = fr[(size_t)d * r.M + i];
+    const size_t M = r.M;
+    vr[0 * M + i] = dr[0] * f[0];
+    vr[1 * M + i] = dr[1] * f[1];
+    if (D == 2) vr[2 * M + i] = dr[1] * f[0];
+    else {
+        vr[2 * M + i] = dr[2] * f[2];
+        vr[3 * M + i] = dr[1] * f[0];
+        vr[4 * M + i] = dr[2] * f[0];
+        vr[5 * M + i] = dr[2] * f[1];
+    }
+}
+
+// ------------------------------------------------------------------ host launchers
+#define CHECK_LAUNCH() do { hipError_t e__ = hipGetLastError(); if (e__ != hipSuccess) return e__; } while (0)
+
+hipError_t launch_spread(const LaunchCtx& c) {
+    if (!c.n) return hipSuccess;
+    hipLaunchKernelGGL(k_spread, dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap);
+    CHECK_LAUNCH(); return hipSuccess;
+}
+hipError_t launch_x_r2c(const LaunchCtx& c, bool from_acc, int nrows_override, const float* in, float2* out) {
+    if (!c.n) return hipSuccess;
+    if (from_acc) hipLaunchKernelGGL(k_x_r2c<true>, dim3(c.n), dim3(kFftThreads), c.lds, c.stream, c.reps, c.wmap, nrows_override, in, out);
+    else hipLaunchKernelGGL(k_x_r2c<false>, dim3(c.n), dim3(kFftThreads), c.lds, c.stream, c.reps, c.wmap, nrows_override, in, out);
+    CHECK_LAUNCH(); return hipSuccess;
+}
+hipError_t launch_mid(const LaunchCtx& c, int axis, int use_gk, int nfb, int inverse, float2* buf) {
+    if (!c.n) return hipSuccess;
+    hipLaunchKernelGGL(k_mid, dim3(c.n), dim3(kFftThreads), c.lds, c.stream, c.reps, c.wmap, axis, use_gk, nfb, inverse, buf);
+    CHECK_LAUNCH(); return hipSuccess;
+}
+hipError_t launch_last_fused(const LaunchCtx& c, int store_spectrum) {
+    if (!c.n) return hipSuccess;
+    hipLaunchKernelGGL(k_last_fused, dim3(c.n), dim3(kFftThreads), c.lds, c.stream, c.reps, c.wmap, store_spectrum);
+    CHECK_LAUNCH(); return hipSuccess;
+}
+hipError_t launch_x_c2r(const LaunchCtx& c, bool records, int nlines, const float2* in, float* out) {
+    if (!c.n) return hipSuccess;
+    if (records) hipLaunchKernelGGL(k_x_c2r<true>, dim3(c.n), dim3(kFftThreads), c.lds, c.stream, c.reps, c.wmap, nlines, in, out);
+    else hipLaunchKernelGGL(k_x_c2r<false>, dim3(c.n), dim3(kFftThreads), c.lds, c.stream, c.reps, c.wmap, nlines, in, out);
+    CHECK_LAUNCH(); return hipSuccess;
+}
+hipError_t launch_particles(const LaunchCtx& c, bool integrate, int write_forces) {
+    if (!c.n) return hipSuccess;
+    if (integrate) hipLaunchKernelGGL(k_particles<true>, dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap, write_forces);
+    else hipLaunchKernelGGL(k_particles<false>, dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap, write_forces);
+    CHECK_LAUNCH(); return hipSuccess;
+}
+hipError_t launch_kprops(const LaunchCtx& c, int nq_max) {
+    if (!c.n || nq_max <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_kprops, dim3(c.n, nq_max), dim3(256), 0, c.stream, c.reps, c.wmap);
+    CHECK_LAUNCH(); return hipSuccess;
+}
+hipError_t launch_bond_props(const LaunchCtx& c) {
+    if (!c.n) return hipSuccess;
+    hipLaunchKernelGGL(k_bond_props, dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap);
+    CHECK_LAUNCH(); return hipSuccess;
+}
+hipError_t launch_reduce_rows(hipStream_t s, const double* in, int nrows, int n, double* out) {
+    if (nrows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_rows, dim3(nrows), dim3(256), 0, s, in, n, out);
+    CHECK_LAUNCH(); return hipSuccess;
+}
+hipError_t launch_advance(hipStream_t s, RepDev* reps, const int* which, int n) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_advance, dim3((n + 255) / 256), dim3(256), 0, s, reps, which, n);
+    CHECK_LAUNCH(); return hipSuccess;
+}
+hipError_t launch_bins(hipStream_t s, const RepDev* reps, int rep, int ns, int* out) {
+    hipLaunchKernelGGL(k_bins, dim3((ns + 255) / 256), dim3(256), 0, s, reps, rep, out);
+    CHECK_LAUNCH(); return hipSuccess;
+}
+hipError_t launch_virial_source(hipStream_t s, const RepDev* reps, int rep, int M, const float* fr, float* vr) {
+    hipLaunchKernelGGL(k_virial_source, dim3((M + 255) / 256), dim3(256), 0, s, reps, rep, fr, vr);
+    CHECK_LAUNCH(); return hipSuccess;
+}
+
+}  // namespace tild

@@ -1,0 +1,56 @@
+// launch.hpp -- block geometry of every pass (shared by kernels and host launchers)
+// and the host-side launcher declarations.
+#pragma once
+#include "device.hpp"
+
+namespace tild {
+
+// Largest batch of complex values one FFT block holds across its stages.
+constexpr int kFftBudget = kFftElems * kFftThreads;        // 4096
+constexpr size_t kMaxLds = 65536;
+
+// Strided pass along y (axis 1) or z (axis 2) of nfb half-spectrum fields.
+struct MidGeom {
+    int N, lstride, batch, nouter, ostride, nl, nchunk;
+};
+__host__ __device__ inline MidGeom mid_geom(const RepDev& r, int axis, int nfb) {
+    MidGeom g;
+    const int Nz = r.dim == 3 ? r.Nz : 1;
+    if (axis == 1) {
+        g.N = r.Ny; g.lstride = r.Nxh; g.batch = r.Nxh;
+        g.nouter = nfb * Nz; g.ostride = r.Ny * r.Nxh;
+    } else {
+        g.N = r.Nz; g.lstride = r.Ny * r.Nxh; g.batch = r.Ny * r.Nxh;
+        g.nouter = nfb; g.ostride = r.Nz * r.Ny * r.Nxh;
+    }
+    int nl = kFftBudget / g.N;
+    if (nl < 1) nl = 1;
+    if (nl > g.batch) nl = g.batch;
+    g.nl = nl;
+    g.nchunk = (g.batch + nl - 1) / nl;
+    return g;
+}
+inline size_t lds_bytes(int N, int nlines) { return (size_t)N * (size_t)(nlines | 1) * sizeof(float2); }
+
+struct LaunchCtx {
+    const RepDev* reps;
+    const WorkItem* wmap;
+    int n;              // blocks
+    size_t lds;         // dynamic LDS bytes
+    hipStream_t stream;
+};
+
+hipError_t launch_spread(const LaunchCtx& c);
+hipError_t launch_x_r2c(const LaunchCtx& c, bool from_acc, int nrows_override, const float* in, float2* out);
+hipError_t launch_mid(const LaunchCtx& c, int axis, int use_gk, int nfb, int inverse, float2* buf);
+hipError_t launch_last_fused(const LaunchCtx& c, int store_spectrum);
+hipError_t launch_x_c2r(const LaunchCtx& c, bool records, int nlines, const float2* in, float* out);
+hipError_t launch_particles(const LaunchCtx& c, bool integrate, int write_forces);
+hipError_t launch_kprops(const LaunchCtx& c, int nq_max);
+hipError_t launch_bond_props(const LaunchCtx& c);
+hipError_t launch_reduce_rows(hipStream_t s, const double* in, int nrows, int n, double* out);
+hipError_t launch_advance(hipStream_t s, RepDev* reps, const int* which, int n);
+hipError_t launch_bins(hipStream_t s, const RepDev* reps, int rep, int ns, int* out);
+hipError_t launch_virial_source(hipStream_t s, const RepDev* reps, int rep, int M, const float* fr, float* vr);
+
+}  // namespace tild

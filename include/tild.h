@@ -1,0 +1,154 @@
+/*
+ * tild.h -- C ABI of the MI355X TILD step engine (libtild.so).
+ *
+ * The reference (rar-ensemble/gpu-tild) is a monolithic CUDA executable whose
+ * per-step path is driven through process globals and the PairStyle / Group /
+ * Integrator / NList classes.  This header replaces that path with an explicit,
+ * re-entrant boundary: plain pointers and sizes, status codes instead of exit(),
+ * no torch or HIP types in any signature.  Each entry point names the reference
+ * interface it stands in for.
+ *
+ * Threading: one host thread per engine; all device work of an engine runs on its
+ * own HIP stream.  Engines are independent (no process globals).
+ */
+#ifndef TILD_H
+#define TILD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The reference calls die() -> exit(1) on every error
+   (die.cu:3-15; e.g. read_input.cu:517-519, forces.cu:61-66). */
+typedef enum {
+    TILD_OK = 0,
+    TILD_ERR_PARSE = 1,       /* input deck or data file malformed */
+    TILD_ERR_CONFIG = 2,      /* deck is well formed but not runnable (missing grid, bad group...) */
+    TILD_ERR_OOM = 3,         /* device or host allocation failed */
+    TILD_ERR_DEVICE = 4,      /* HIP runtime error */
+    TILD_ERR_NAN = 5,         /* NaN energy at a property evaluation (main.cu:250-291) */
+    TILD_ERR_ARG = 6,         /* bad argument (size mismatch, null pointer, index range) */
+    TILD_ERR_BUDGET = 7,      /* pool admission: job can never fit the HBM budget */
+    TILD_ERR_UNSUPPORTED = 8  /* feature of the reference outside this engine's scope */
+} tild_status;
+
+#define TILD_MAX_STYLES 64
+
+typedef struct tild_system tild_system;   /* parsed deck, host only */
+typedef struct tild_engine tild_engine;   /* one device-resident replica */
+
+/* ---------------------------------------------------------------- host only --- */
+
+/* Parse an input deck with the reference grammar and read its data file.
+   Replaces read_input() (read_input.cu:12-533) with read_config / read_charge_config
+   (config_utils.cu:5-364); relative file names resolve against base_dir.
+   No GPU is touched.  On failure *out is NULL and tild_last_error() says why. */
+tild_status tild_system_load(const char* input_text, const char* base_dir, tild_system** out);
+void        tild_system_free(tild_system* sys);
+
+/* Message of the last failed call on this thread (never NULL). */
+const char* tild_last_error(void);
+
+/* Scalar facts about a parsed deck.  Keys: "ns", "dim", "ntypes", "M", "Nx0", "Nx1",
+   "Nx2", "pmeorder", "n_styles", "do_charges", "n_bonds_total", "nbond_types",
+   "max_steps", "log_freq", "n_groups", "n_integrators", "seed", "L0", "L1", "L2",
+   "dx0", "dx1", "dx2", "gvol", "V", "delt", "noise_mag".  Mirrors the globals of
+   globals.h:47-80 after initialize() (initialize.cu:32-48). */
+tild_status tild_system_info(const tild_system* sys, const char* key, double* value);
+
+/* Per-site arrays of a parsed deck; n is the element count of the buffer(s). */
+tild_status tild_system_get_positions(const tild_system* sys, float* x, size_t n);       /* ns*Dim */
+tild_status tild_system_get_types(const tild_system* sys, int* tp, size_t n);           /* ns, 0-based */
+tild_status tild_system_get_molecules(const tild_system* sys, int* mol, size_t n);      /* ns, 0-based */
+tild_status tild_system_get_charges(const tild_system* sys, float* q, size_t n);        /* ns */
+/* Bond tables exactly as config_utils.cu:96-125 builds them: n_bonds[ns],
+   bonded_to[ns*3], bond_type[ns*3] (MAX_BONDS = 3, globals.h:26). */
+tild_status tild_system_get_bonds(const tild_system* sys, int* n_bonds, int* bonded_to,
+                                  int* bond_type, size_t ns);
+/* Site list of group g (make_group_all / make_group_type, group.cu:30-60). */
+tild_status tild_system_group_sites(const tild_system* sys, int group, int* sites,
+                                    size_t cap, int* nsites);
+
+/* ---------------------------------------------------------------- engine ------ */
+
+typedef struct {
+    int device;          /* HIP device ordinal */
+    int use_seed;        /* nonzero: override the deck's rand_seed (read_input.cu:450-454) */
+    uint32_t seed;
+    int spread_mode;     /* 0 auto, 1 global fixed-point atomics, 2 LDS-tiled sorted bins */
+    int reserved[8];
+} tild_opts;
+
+/* Properties of calc_properties() (calc_properties.cu:20-116): FP64 reductions. */
+typedef struct {
+    double Upe;                  /* sum of style energies + Uelec + Ubond */
+    double Ubond;                /* host_bonds (bonds.cu:3-52) at current positions */
+    double Uelec;                /* calc_electrostatic_energy (calc_properties.cu:106-116) */
+    double Ptens[6];             /* xx yy zz xy xz yz (3D) | xx yy xy (2D) */
+    int n_styles;                /* energies below: gaussians, erfs, gaussian_erfs in deck order */
+    double U_style[TILD_MAX_STYLES];
+    int step;
+    int nan_flag;                /* a style energy is NaN (main.cu:250-288) */
+} tild_props;
+
+/* Allocate device state for one replica, build k-space tables and upload the
+   particles.  Replaces initialize() (initialize.cu:20-85) + allocate_device_memory
+   (:93-222) + allocate_device_potentials (:224-244) + send_box_params_to_device
+   (device_comm_utils.cu:90-140). */
+tild_status tild_engine_create(const tild_system* sys, const tild_opts* opts, tild_engine** out);
+void        tild_engine_destroy(tild_engine* e);
+const char* tild_engine_last_error(const tild_engine* e);
+
+/* forces() at the current positions (forces.cu:42-196): spread, k-space pair
+   forces and charges, gather, bonds.  Leaves the particle forces readable. */
+tild_status tild_engine_forces(tild_engine* e);
+
+/* n iterations of the main-loop body (main.cu:106-117): step += 1, forces(x_n),
+   Integrate_2 for every integrator (integrator.cu:24-39) -> x_{n+1}. */
+tild_status tild_engine_step(tild_engine* e, int n);
+
+/* calc_properties() after the last forces()/step: non-bonded energies and virial
+   from the densities of that force evaluation, bonds at the current positions
+   (the reference's mixed-time convention, calc_properties.cu:49-59). */
+tild_status tild_engine_properties(tild_engine* e, tild_props* out);
+
+/* Host <-> device copies (cuda_collect_x / send_3n_to_device,
+   device_comm_utils.cu:5-31; cuda_collect_f :82-87; cuda_collect_rho :33-44).
+   set_positions also resets the GJF previous position to x (device_comm_utils.cu:15-16). */
+tild_status tild_engine_get_positions(tild_engine* e, float* x, size_t n);       /* ns*Dim */
+tild_status tild_engine_set_positions(tild_engine* e, const float* x, size_t n); /* ns*Dim */
+tild_status tild_engine_get_forces(tild_engine* e, float* f, size_t n);          /* ns*Dim */
+/* Density of type t on the grid (x fastest); t == ntypes returns the charge density. */
+tild_status tild_engine_get_density(tild_engine* e, int type, float* rho, size_t n);
+/* Gathered field of type t (t == ntypes: electric field), direction d: the grid
+   quantity the gather interpolates, i.e. all_f{x,y,z}[t]/rho[t] of
+   device_grid_utils.cu:424-452, or E_d of pair_style_charges.cu:71-79. */
+tild_status tild_engine_get_grid_field(tild_engine* e, int type, int dir, float* g, size_t n);
+/* First stencil node of every coordinate, ns*Dim ints: the grid bin the reference
+   computes from FP32 x / dx (device_grid_utils.cu:136-146, :153-226), wrapped. */
+tild_status tild_engine_get_bins(tild_engine* e, int* base, size_t n);
+/* nsteps of tild_engine_step with a HIP event pair around every kernel launch on the
+   engine's stream; phase_ms[k] receives the mean duration of phase k (nphase >= 8):
+   0 spread, 1 x-axis R2C, 2 y-axis FFT, 3 fused last-axis k-space pass, 4 inverse
+   y-axis FFT, 5 x-axis C2R, 6 gather+bonds+integrate, 7 step counter. */
+tild_status tild_engine_profile(tild_engine* e, int nsteps, double* phase_ms, int nphase);
+tild_status tild_engine_get_step(tild_engine* e, int* step);
+tild_status tild_engine_synchronize(tild_engine* e);
+
+/* ---------------------------------------------------------------- building blocks */
+
+/* Batched single-precision real<->complex 2D/3D transforms on device memory with the
+   engine's Stockham kernels (the reference uses cuFFT C2C plans, initialize.cu:217-220).
+   Layout: x fastest; complex output has Nx[0]/2+1 columns.  Forward is e^{-ikr},
+   inverse unnormalised.  `stream` is a hipStream_t (NULL = default stream). */
+tild_status tild_fft_r2c(int dim, const int* Nx, int batch, const float* d_in,
+                         float* d_out, void* stream);
+tild_status tild_fft_c2r(int dim, const int* Nx, int batch, const float* d_in,
+                         float* d_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

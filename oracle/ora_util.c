@@ -37,9 +37,10 @@ void ora_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
-/* Uniform in (0,1) from the top 24 bits: exactly representable in FP32 and FP64,
-   so the product (FP32) and the oracle (FP64) start Box-Muller from the same value. */
-static double u24(uint32_t v) { return ((double)(v >> 8) + 0.5) * (1.0 / 16777216.0); }
+/* Uniform in (0,1) from the top 23 bits, (k + 1/2) / 2^23: exactly representable in
+   FP32 and FP64, so the product (FP32) and the oracle (FP64) start Box-Muller from the
+   same value. */
+static double u23(uint32_t v) { return ((double)(v >> 9) + 0.5) * (1.0 / 8388608.0); }
 
 void ora_normals(uint32_t seed, uint32_t particle, uint32_t step, uint32_t stream, double z[4]) {
     uint32_t ctr[4] = {particle, step, stream, 0u};
@@ -47,7 +48,7 @@ void ora_normals(uint32_t seed, uint32_t particle, uint32_t step, uint32_t strea
     uint32_t o[4];
     ora_philox4x32_10(ctr, key, o);
     for (int p = 0; p < 2; ++p) {
-        double u1 = u24(o[2 * p]), u2 = u24(o[2 * p + 1]);
+        double u1 = u23(o[2 * p]), u2 = u23(o[2 * p + 1]);
         double r = sqrt(-2.0 * log(u1));
         /* angle 2*pi*u2 (Box & Muller 1958) */
         z[2 * p]     = r * cos(2.0 * ORA_PI * u2);

@@ -1,0 +1,137 @@
+"""GPU parity of the HIP engine against the FP64 oracle (called through the C ABI)."""
+import numpy as np
+import pytest
+
+import decks
+import gpu_util as U
+import oracle_ctypes as O
+import tild
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(text, d, seed=7):
+    s = tild.System(text, d)
+    return s, tild.Engine(s, device=0, seed=seed), O.System(text, d, seed=seed)
+
+
+CASES = {
+    "3d_gauss": lambda t: decks.melt3d(t),
+    "3d_mixed": lambda t: decks.melt3d(t, styles=decks.MIXED),
+    "3d_odd_grid": lambda t: decks.melt3d(t, grid=(15, 14, 21), box=(6.0, 5.6, 7.0)),
+    "2d_gauss": lambda t: decks.melt2d(t),
+    "charged": lambda t: decks.charged(t),
+    "p0": lambda t: decks.melt3d(t, pmeorder=0),
+    "p2": lambda t: decks.melt3d(t, pmeorder=2),
+    "p3": lambda t: decks.melt3d(t, pmeorder=3),
+    "p4": lambda t: decks.melt3d(t, pmeorder=4),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_density_and_forces(case, tmp_path):
+    text, d = CASES[case](tmp_path)
+    s, e, o = _pair(text, d)
+    print(f"[{case}] ns={s.ns} grid={s.grid_shape}", flush=True)
+    f = e.forces()
+    fo = o.forces()
+    for t in range(e.nfields):
+        rho = e.density(t).astype(np.float64)
+        ro = o.density(t) if t < s.ntypes else o.charge_density()
+        err = np.abs(rho - ro).max() / max(np.abs(ro).max(), 1e-30)
+        print(f"  rho[{t}] max rel err {err:.2e}")
+        assert err < 2e-6
+    ef = U.rel_norm(f, fo)
+    print(f"  |df|/|f| = {ef:.2e}  max|df|/rms(f) = {np.abs(f - fo).max() / np.sqrt((fo ** 2).mean()):.2e}")
+    assert ef < 1e-5
+    e.close()
+
+
+@pytest.mark.parametrize("case", ["3d_gauss", "3d_mixed", "2d_gauss", "charged"])
+def test_properties(case, tmp_path):
+    text, d = CASES[case](tmp_path)
+    s, e, o = _pair(text, d)
+    e.forces()
+    p = e.properties()
+    o.forces()
+    po = o.properties()
+    print(f"[{case}] engine {p}\n  oracle {po}")
+    for k in ("Upe", "Ubond", "Uelec"):
+        assert abs(p[k] - po[k]) <= 1e-5 * max(abs(po[k]), 1e-12), k
+    np.testing.assert_allclose(p["U_styles"], po["U_styles"], rtol=1e-5, atol=1e-9)
+    scale = np.abs(po["Ptens"]).max()
+    assert np.abs(p["Ptens"] - po["Ptens"]).max() <= 1e-5 * scale
+    e.close()
+
+
+def test_coacervate_step0_matches_reference_log(tmp_path):
+    text, d = U.coacervate_dir(tmp_path)
+    s = tild.System(text, d)
+    e = tild.Engine(s, device=0, seed=1)
+    e.forces()
+    p = e.properties()
+    g = U.coacervate_golden()
+    print("engine", p, "\nreference data.dat", g)
+    assert abs(p["Upe"] / g["Upe"] - 1) < 3e-5
+    assert abs(p["Ubond"] / g["Ubond"] - 1) < 3e-5
+    assert abs(p["U_styles"][0] / g["Ugauss0"] - 1) < 3e-5
+    np.testing.assert_allclose(p["Ptens"], g["Ptens"], rtol=3e-5, atol=3e-7)
+    e.close()
+
+
+@pytest.mark.parametrize("integ", ["EM", "GJF"])
+def test_trajectory_20_steps(integ, tmp_path):
+    text, d = decks.melt3d(tmp_path, integrator=integ, delt=0.005 if integ == "EM" else 0.01)
+    s, e, o = _pair(text, d, seed=21)
+    e.step(20)
+    o.run(20)
+    x = e.positions().astype(np.float64)
+    xo = o.positions()
+    L = [s.info("L0"), s.info("L1"), s.info("L2")][:s.dim]
+    dx = U.min_image(x - xo, L)
+    err = np.abs(dx).max()
+    print(f"[{integ}] 20-step max |dx| = {err:.2e}")
+    assert err < 1e-4
+    e.close()
+
+
+def test_run_twice_bitwise_identical(tmp_path):
+    text, d = decks.melt3d(tmp_path, integrator="GJF", delt=0.01)
+    runs = []
+    for _ in range(2):
+        s = tild.System(text, d)
+        e = tild.Engine(s, device=0, seed=5)
+        e.step(10)
+        runs.append((e.positions(), e.density(0)))
+        e.close()
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1])
+
+
+@pytest.mark.parametrize("P", [1, 2, 3])
+def test_bins_bit_exact(P, tmp_path):
+    text, d = decks.melt3d(tmp_path, grid=(15, 14, 21), box=(6.0, 5.6, 7.0), pmeorder=P)
+    s, e, o = _pair(text, d)
+    x = s.positions()
+    L = np.array([s.info("L0"), s.info("L1"), s.info("L2")], dtype=np.float32)
+    dx = np.array([s.info("dx0"), s.info("dx1"), s.info("dx2")], dtype=np.float32)
+    rng = np.random.default_rng(P)
+    x[0] = 0.0
+    x[1] = L
+    k = rng.integers(0, 14, size=(60, 3)).astype(np.float32)
+    x[2:62] = k * dx                                  # on grid nodes
+    x[62:122] = (k + np.float32(0.5)) * dx            # on node midpoints
+    x[122:182] = np.nextafter(k * dx, np.float32(-1))  # one ulp below nodes
+    x[182:190] = -1e-6
+    e.set_positions(x)
+    o.set_positions(x.astype(np.float64))
+    eb = e.bins()
+    o.forces()
+    inds, _ = o.stencil()
+    first = inds[:, 0]
+    Nx, Ny = s.Nx[0], s.Nx[1]
+    ob = np.stack([first % Nx, (first // Nx) % Ny, first // (Nx * Ny)], axis=1)
+    bad = np.nonzero((eb != ob).any(axis=1))[0]
+    print(f"P={P}: {len(bad)} mismatching bins of {s.ns}")
+    assert len(bad) == 0, (bad[:5], eb[bad[:5]], ob[bad[:5]], x[bad[:5]])
+    e.close()
