@@ -125,6 +125,7 @@ tild_status tild_engine_create(const tild_system* s, const tild_opts* opts, tild
     if (opts) o = *opts;
     auto e = std::make_unique<tild_engine>();
     e->batch = std::make_unique<tild::Batch>(o.device);
+    e->batch->set_spread(o.spread_mode, o.reserved[0] > 0 ? o.reserved[0] : 1024, o.reserved[1], o.reserved[2]);
     std::string err;
     int rc = e->batch->init(err);
     if (rc < 0) return from_rc(rc, err, nullptr);

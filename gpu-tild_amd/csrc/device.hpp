@@ -13,7 +13,7 @@ namespace tild {
 constexpr int kMaxTypes = 16;      // particle types per replica
 constexpr int kMaxStages = 12;     // FFT stages per axis
 constexpr int kFftThreads = 256;   // threads of every FFT block
-constexpr int kFftElems = 16;      // complex values per thread per FFT stage (registers)
+constexpr int kFftElems = 8;       // complex values per thread per FFT call (LDS budget)
 constexpr int kMaxQuant = 128;     // property quantities per replica
 
 // Mixed-radix Stockham plan of one axis.  Stage s has radix rad[s] and
@@ -73,6 +73,14 @@ struct RepDev {
     double* partials;          // nq * nchunk partial sums
     int nchunk;
     double* bpart;             // 7 * ceil(ns/256) bond-property partials
+    // sorted-bin deposition: full-x slab tiles of By x Bz rows (By | Ny, Bz | Nz)
+    int spread_mode;           // 1 global atomics, 2 LDS-tiled sorted bins
+    int By, Bz, nty, ntz, ntiles, nbuckets;
+    int* tcnt;                 // nbuckets counters (zero between steps)
+    int* toff;                 // nbuckets + 1 exclusive offsets
+    int2* tkey;                // ns (bucket, slot)
+    float4* trec;              // ns sorted (x, y, z, type) records
+    float* trecq;              // ns sorted charges (charged decks)
     // counters
     int step;                  // completed steps; X[step & 1] is current
     uint32_t seed;

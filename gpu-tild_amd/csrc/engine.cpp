@@ -53,6 +53,36 @@ int make_axis_plan(int N, AxisPlan& pl, std::vector<float2>& tw, std::string& er
     return 0;
 }
 
+// ---------------------------------------------------------------- tiles
+size_t tile_lds_bytes(int Nx, int nf, int By, int Bz) {
+    size_t rows = (size_t)nf * By * Bz;
+    return rows * Nx * sizeof(unsigned long long) + lds_bytes(Nx, (int)((rows + 1) / 2));
+}
+
+bool choose_tiles(int dim, const int* Nx, int P, int nf, int target_tiles, int want_by, int want_bz,
+                  int& By, int& Bz) {
+    const int Ny = Nx[1], Nz = dim == 3 ? Nx[2] : 1;
+    double best = 1e300;
+    bool found = false;
+    for (int by = P + 1; by <= Ny / 2; ++by) {
+        if (Ny % by) continue;
+        if (want_by > 0 && by != want_by) continue;
+        for (int bz = (dim == 3 ? P + 1 : 1); bz <= (dim == 3 ? Nz / 2 : 1); ++bz) {
+            if (Nz % bz) continue;
+            if (want_bz > 0 && dim == 3 && bz != want_bz) continue;
+            const int rows = nf * by * bz;
+            if ((long)Nx[0] * ((rows + 1) / 2) > kFftBudget) continue;
+            if (tile_lds_bytes(Nx[0], nf, by, bz) > kMaxLds) continue;
+            const int ntiles = (Ny / by) * (Nz / bz);
+            // prefer about target_tiles blocks, then square-ish tiles
+            double score = std::fabs(std::log((double)ntiles / target_tiles)) + 0.05 * std::fabs(std::log((double)by / bz));
+            if (dim == 2) score = std::fabs(std::log((double)ntiles / target_tiles));
+            if (score < best) { best = score; By = by; Bz = bz; found = true; }
+        }
+    }
+    return found;
+}
+
 // ---------------------------------------------------------------- helpers
 static int fixed_bits(double bound) {
     // integer bits needed for sums up to `bound` (with sign) -> fractional bits
@@ -165,9 +195,9 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
     const int npairs_x = (h.nf * rows + 1) / 2;
     h.nl_x = std::max(1, std::min(kFftBudget / h.Nx, npairs_x));
     const int la = D - 1, Nla = la == 2 ? h.Nz : h.Ny, Q = la == 2 ? h.Ny * h.Nxh : h.Nxh;
-    if (Nla * h.nf > kFftBudget || 2 * lds_bytes(Nla, h.nf) > kMaxLds) { err = "grid too large for the fused k-space pass"; return -TILD_ERR_UNSUPPORTED; }
+    if (Nla * h.nf > kFftBudget || 3 * fft_buf_bytes(Nla, h.nf) > kMaxLds) { err = "grid too large for the fused k-space pass"; return -TILD_ERR_UNSUPPORTED; }
     int nl = std::max(1, kFftBudget / (Nla * h.nf));
-    while (nl > 1 && 2 * lds_bytes(Nla, h.nf * nl) > kMaxLds) --nl;
+    while (nl > 1 && 3 * fft_buf_bytes(Nla, h.nf * nl) > kMaxLds) --nl;
     h.nl_last = std::min(nl, Q);
     const int nc = (h.nf * D + 1) / 2;
     if (nc * h.Nx > kFftBudget) { err = "grid too large for the inverse x pass"; return -TILD_ERR_UNSUPPORTED; }
@@ -252,6 +282,27 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
     if (int rc = dalloc(R, &R.d_bpart, 7 * (size_t)R.nblk_part, err, true)) return rc;
     h.bpart = R.d_bpart;
 
+    // spread mode: LDS-tiled sorted bins when a tile shape fits, else global atomics
+    h.spread_mode = 1;
+    if (spread_mode_ != 1) {
+        int By = 0, Bz = 0;
+        if (choose_tiles(D, S.Nx, S.pmeorder, h.nf, tile_target_, tile_by_, tile_bz_, By, Bz)) {
+            h.spread_mode = 2;
+            h.By = By; h.Bz = Bz;
+            h.nty = h.Ny / By; h.ntz = h.Nz / Bz;
+            h.ntiles = h.nty * h.ntz;
+            h.nbuckets = h.ntiles * 4;
+            if (int rc = dalloc(R, &h.tcnt, h.nbuckets, err, true)) return rc;
+            if (int rc = dalloc(R, &h.toff, (size_t)h.nbuckets + 1, err, true)) return rc;
+            if (int rc = dalloc(R, &h.tkey, S.ns, err)) return rc;
+            if (int rc = dalloc(R, &h.trec, S.ns, err)) return rc;
+            if (S.charges.enabled) { if (int rc = dalloc(R, &h.trecq, S.ns, err)) return rc; }
+        } else if (spread_mode_ == 2) {
+            err = "no tile shape fits this grid for the tiled spread";
+            return -TILD_ERR_UNSUPPORTED;
+        }
+    }
+
     int id = -1;
     for (int k = 0; k < (int)reps_.size(); ++k) if (!reps_[k]) { id = k; break; }
     if (id < 0) { id = (int)reps_.size(); reps_.emplace_back(); }
@@ -314,7 +365,16 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
         };
         add(PH_PART, R.nblk_part, 0);
         const int npairs = (h.nf * rows + 1) / 2;
-        add(PH_X, (npairs + h.nl_x - 1) / h.nl_x, lds_bytes(h.Nx, h.nl_x));
+        const int nbx = (npairs + h.nl_x - 1) / h.nl_x;
+        add(PH_XP, nbx, lds_bytes(h.Nx, h.nl_x));
+        if (h.spread_mode == 2) {
+            add(PH_TPART, R.nblk_part, 0);
+            add(PH_SCAN, 1, 0);
+            add(PH_TILE, h.ntiles, tile_lds_bytes(h.Nx, h.nf, h.By, h.Bz));
+        } else {
+            add(PH_SPREADA, R.nblk_part, 0);
+            add(PH_XA, nbx, lds_bytes(h.Nx, h.nl_x));
+        }
         {
             MidGeom g = mid_geom(h, 1, h.nf);
             add(PH_MIDY_H, g.nouter * g.nchunk, lds_bytes(g.N, g.nl));
@@ -326,7 +386,7 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
             add(PH_MIDZ_H, gz.nouter * gz.nchunk, lds_bytes(gz.N, gz.nl));
         }
         const int la = D - 1, Nla = la == 2 ? h.Nz : h.Ny, Q = la == 2 ? h.Ny * h.Nxh : h.Nxh;
-        add(PH_LAST, (Q + h.nl_last - 1) / h.nl_last, 2 * lds_bytes(Nla, h.nf * h.nl_last));
+        add(PH_LAST, (Q + h.nl_last - 1) / h.nl_last, 3 * fft_buf_bytes(Nla, h.nf * h.nl_last));
         const int nc = (h.nf * D + 1) / 2;
         add(PH_C2R, (rows + h.nr_c2r - 1) / h.nr_c2r, lds_bytes(h.Nx, h.nr_c2r * nc));
         add(PH_KPROPS, h.nchunk > 0 ? h.nchunk : 1, 0);
@@ -555,8 +615,12 @@ int Batch::enqueue_step(SetMaps& m, bool integrate, bool write_forces, std::stri
     PhaseMap* mp = m.maps;
     auto ctx = [&](Phase p) { return LaunchCtx{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_}; };
     bool any3d = mp[PH_MIDY_G].n > 0;
-    HIPCHK(launch_spread(ctx(PH_PART)));
-    HIPCHK(launch_x_r2c(ctx(PH_X), true, 0, nullptr, nullptr));
+    HIPCHK(launch_spread(ctx(PH_SPREADA)));
+    HIPCHK(launch_x_r2c(ctx(PH_XA), true, 0, nullptr, nullptr));
+    HIPCHK(launch_tile_count(ctx(PH_TPART)));
+    HIPCHK(launch_tile_scan(ctx(PH_SCAN)));
+    HIPCHK(launch_tile_scatter(ctx(PH_TPART)));
+    HIPCHK(launch_tile_deposit(ctx(PH_TILE)));
     if (any3d) HIPCHK(launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr));
     HIPCHK(launch_last_fused(ctx(PH_LAST), 0));
     if (any3d) HIPCHK(launch_mid(ctx(PH_MIDY_G), 1, 1, 0, 1, nullptr));
@@ -607,9 +671,13 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
         hipError_t e0 = hipSuccess;
         auto rec = [&](int k) { if (e0 == hipSuccess) e0 = hipEventRecord(ev[k], stream_); };
         rec(0);
-        if (e0 == hipSuccess) e0 = launch_spread(ctx(PH_PART));
+        if (e0 == hipSuccess) e0 = launch_spread(ctx(PH_SPREADA));
+        if (e0 == hipSuccess) e0 = launch_tile_count(ctx(PH_TPART));
+        if (e0 == hipSuccess) e0 = launch_tile_scan(ctx(PH_SCAN));
+        if (e0 == hipSuccess) e0 = launch_tile_scatter(ctx(PH_TPART));
         rec(1);
-        if (e0 == hipSuccess) e0 = launch_x_r2c(ctx(PH_X), true, 0, nullptr, nullptr);
+        if (e0 == hipSuccess) e0 = launch_x_r2c(ctx(PH_XA), true, 0, nullptr, nullptr);
+        if (e0 == hipSuccess) e0 = launch_tile_deposit(ctx(PH_TILE));
         rec(2);
         if (e0 == hipSuccess && any3d) e0 = launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr);
         rec(3);
@@ -642,7 +710,7 @@ int Batch::forward_spectra(int id, std::string& err) {
     Replica& R = *reps_[id];
     PhaseMap* mp = R.maps;
     auto ctx = [&](Phase p) { return LaunchCtx{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_}; };
-    HIPCHK(launch_x_r2c(ctx(PH_X), false, 0, nullptr, nullptr));
+    HIPCHK(launch_x_r2c(ctx(PH_XP), false, 0, nullptr, nullptr));
     HIPCHK(launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr));
     if (R.dim == 3) HIPCHK(launch_mid(ctx(PH_MIDZ_H), 2, 0, 0, 0, nullptr));
     return 0;

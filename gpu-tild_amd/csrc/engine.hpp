@@ -21,7 +21,10 @@ namespace tild {
 
 constexpr int kProfPhases = 8;   // spread, x_r2c, y_fwd, last_fused, y_inv, x_c2r, particles, advance
 
-enum Phase { PH_PART = 0, PH_X, PH_MIDY_H, PH_MIDY_G, PH_MIDZ_H, PH_LAST, PH_C2R, PH_KPROPS, PH_COUNT };
+// Work-map phases.  *A: replicas on the global-atomic spread; T*: replicas on the
+// LDS-tiled spread; XP: x-axis R2C of stored densities (properties).
+enum Phase { PH_PART = 0, PH_SPREADA, PH_XA, PH_XP, PH_TPART, PH_SCAN, PH_TILE, PH_MIDY_H, PH_MIDY_G,
+             PH_MIDZ_H, PH_LAST, PH_C2R, PH_KPROPS, PH_COUNT };
 
 struct PhaseMap {
     std::vector<WorkItem> host;
@@ -50,6 +53,11 @@ struct Replica {
 class Batch {
 public:
     explicit Batch(int device) : device_(device) {}
+    // spread mode for replicas added from now on: 0 auto, 1 global atomics, 2 tiled;
+    // target block count and forced tile shape (0 = free) of the tiled mode.
+    void set_spread(int mode, int target_tiles = 1024, int by = 0, int bz = 0) {
+        spread_mode_ = mode; tile_target_ = target_tiles; tile_by_ = by; tile_bz_ = bz;
+    }
     ~Batch();
     int init(std::string& err);
 
@@ -87,6 +95,7 @@ public:
 
 private:
     int device_;
+    int spread_mode_ = 0, tile_target_ = 1024, tile_by_ = 0, tile_bz_ = 0;
     hipStream_t stream_ = nullptr;
     std::vector<std::unique_ptr<Replica>> reps_;
     RepDev* d_reps_ = nullptr;
@@ -102,5 +111,12 @@ private:
 
 // Plan of one FFT axis: radices and twiddle table (forward sign).
 int make_axis_plan(int N, AxisPlan& pl, std::vector<float2>& tw, std::string& err);
+
+// Tile shape of the sorted-bin deposition (By | Ny, Bz | Nz, halo below one tile,
+// LDS and FFT-budget limits).  Returns false when no shape fits.  `target_tiles`
+// steers the choice; want_by/want_bz > 0 force a shape (autotuner sweeps).
+bool choose_tiles(int dim, const int* Nx, int P, int nf, int target_tiles, int want_by, int want_bz,
+                  int& By, int& Bz);
+size_t tile_lds_bytes(int Nx, int nf, int By, int Bz);
 
 }  // namespace tild

@@ -4,11 +4,10 @@
 // Only the forward transform (e^{-2 pi i nk/N}) is implemented here; callers form
 // the inverse as conj(FFT(conj(x))) by conjugating on load and store.
 //
-// Each stage (radix R, p = product of earlier radices) is done in place: every
-// thread first loads the R inputs of each of its butterflies into registers, the
-// block synchronises, then each butterfly is twiddled, transformed and written to
-// its Stockham output slots:  u_r = x[i + r N/R] w_{pR}^{r k},  k = i mod p,
-//                             y[(i - k) R + k + q p] = DFT_R(u)_q.
+// Each stage (radix R, p = product of earlier radices) reads one LDS buffer and
+// writes the other (Stockham autosort, no bit reversal); a thread keeps only one
+// butterfly in registers at a time:
+//     u_r = x[i + r N/R] w_{pR}^{r k},  k = i mod p,   y[(i - k) R + k + q p] = DFT_R(u)_q.
 #pragma once
 #include "device.hpp"
 
@@ -112,59 +111,50 @@ template <> struct Dft<7> {
 };
 
 template <int R>
-__device__ inline void fft_stage(float2* buf, int ldb, int nb, int N, int p, const float2* tw) {
-    constexpr int MAXB = (kFftElems + R - 1) / R;
+__device__ inline void fft_stage(const float2* __restrict__ src, float2* __restrict__ dst, int ldb, int nb,
+                                 int N, int p, const float2* __restrict__ tw) {
     const int m = N / R, nbf = m * nb;
-    float2 v[MAXB][R];
-    int tb[MAXB], ti[MAXB];
+    for (int task = threadIdx.x; task < nbf; task += kFftThreads) {
+        const int b = task % nb, i = task / nb, k = i % p;
+        float2 v[R];
 #pragma unroll
-    for (int j = 0; j < MAXB; ++j) {
-        int task = threadIdx.x + j * kFftThreads;
-        tb[j] = -1;
-        if (task < nbf) {
-            int b = task % nb, i = task / nb;
-            tb[j] = b; ti[j] = i;
+        for (int r = 0; r < R; ++r) v[r] = src[(i + r * m) * ldb + b];
+        if (p > 1) {
+            const float2* w = tw + k * (R - 1);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[j][r] = buf[(i + r * m) * ldb + b];
+            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], w[r - 1]);
         }
+        Dft<R>::run(v);
+        const int o = (i - k) * R + k;
+#pragma unroll
+        for (int q = 0; q < R; ++q) dst[(o + q * p) * ldb + b] = v[q];
     }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < MAXB; ++j) {
-        if (tb[j] >= 0) {
-            int b = tb[j], i = ti[j], k = i % p;
-            if (p > 1) {
-                const float2* w = tw + k * (R - 1);
-#pragma unroll
-                for (int r = 1; r < R; ++r) v[j][r] = cmul(v[j][r], w[r - 1]);
-            }
-            Dft<R>::run(v[j]);
-            int o = (i - k) * R + k;
-#pragma unroll
-            for (int q = 0; q < R; ++q) buf[(o + q * p) * ldb + b] = v[j][q];
-        }
-    }
-    __syncthreads();
 }
 
-// Forward FFT of nb lines in LDS.  Requires N*nb <= kFftElems*kFftThreads and a
-// block of kFftThreads threads; callers synchronise before (data loaded) -- this
-// function synchronises after each stage.
-__device__ inline void lds_fft(float2* buf, int ldb, int nb, const AxisPlan& pl) {
+// Forward FFT of nb lines held in A (element n of line b at A[n*ldb + b]), using B
+// as the ping-pong partner.  Requires N*nb <= kFftBudget and a synchronised block
+// (A fully written).  Returns the buffer holding the result (A or B); every stage
+// ends with a barrier, so the result is visible to the whole block on return.
+__device__ inline float2* lds_fft(float2* A, float2* B, int ldb, int nb, const AxisPlan& pl) {
+    float2* src = A;
+    float2* dst = B;
     int p = 1;
     for (int s = 0; s < pl.nst; ++s) {
         const int R = pl.rad[s];
         const float2* tw = pl.tw + pl.tw_off[s];
         switch (R) {
-            case 2: fft_stage<2>(buf, ldb, nb, pl.N, p, tw); break;
-            case 3: fft_stage<3>(buf, ldb, nb, pl.N, p, tw); break;
-            case 4: fft_stage<4>(buf, ldb, nb, pl.N, p, tw); break;
-            case 5: fft_stage<5>(buf, ldb, nb, pl.N, p, tw); break;
-            case 7: fft_stage<7>(buf, ldb, nb, pl.N, p, tw); break;
-            default: fft_stage<8>(buf, ldb, nb, pl.N, p, tw); break;
+            case 2: fft_stage<2>(src, dst, ldb, nb, pl.N, p, tw); break;
+            case 3: fft_stage<3>(src, dst, ldb, nb, pl.N, p, tw); break;
+            case 4: fft_stage<4>(src, dst, ldb, nb, pl.N, p, tw); break;
+            case 5: fft_stage<5>(src, dst, ldb, nb, pl.N, p, tw); break;
+            case 7: fft_stage<7>(src, dst, ldb, nb, pl.N, p, tw); break;
+            default: fft_stage<8>(src, dst, ldb, nb, pl.N, p, tw); break;
         }
+        __syncthreads();
+        float2* t = src; src = dst; dst = t;
         p *= R;
     }
+    return src;
 }
 
 }  // namespace tild

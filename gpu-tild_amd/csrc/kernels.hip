@@ -108,11 +108,11 @@ __global__ void __launch_bounds__(kFftThreads) k_x_r2c(const RepDev* __restrict_
         lds[n * ldb + b] = make_float2(a, bb);
     }
     __syncthreads();
-    lds_fft(lds, ldb, nb, r.ax[0]);
+    const float2* res = lds_fft(lds, lds + (size_t)N * ldb, ldb, nb, r.ax[0]);
     for (int task = threadIdx.x; task < nb * Nxh; task += kFftThreads) {
         const int b = task / Nxh, k = task % Nxh;
-        const float2 Zk = lds[k * ldb + b];
-        const float2 Zm = lds[((N - k) % N) * ldb + b];
+        const float2 Zk = res[k * ldb + b];
+        const float2 Zm = res[((N - k) % N) * ldb + b];
         const int ra = 2 * (p0 + b), rb = ra + 1;
         Hout[(size_t)ra * Nxh + k] = make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y));
         if (rb < nrows) Hout[(size_t)rb * Nxh + k] = make_float2(0.5f * (Zk.y + Zm.y), 0.5f * (Zm.x - Zk.x));
@@ -143,10 +143,10 @@ __global__ void __launch_bounds__(kFftThreads) k_mid(const RepDev* __restrict__ 
         lds[n * ldb + b] = inverse ? conjf2(v) : v;
     }
     __syncthreads();
-    lds_fft(lds, ldb, nb, r.ax[axis]);
+    const float2* res = lds_fft(lds, lds + (size_t)g.N * ldb, ldb, nb, r.ax[axis]);
     for (int task = threadIdx.x; task < nb * g.N; task += kFftThreads) {
         const int b = task % nb, n = task / nb;
-        float2 v = lds[n * ldb + b];
+        float2 v = res[n * ldb + b];
         base[(size_t)n * g.lstride + b] = inverse ? conjf2(v) : v;
     }
 }
@@ -174,42 +174,41 @@ __global__ void __launch_bounds__(kFftThreads) k_last_fused(const RepDev* __rest
     const int nb = min(r.nl_last, Q - q0);
     const int nlines = nf * nb;
     const int ldb = nlines | 1;
-    float2* A = lds;
-    float2* B = lds + (size_t)N * ldb;
+    float2* A0 = lds;
+    float2* B0 = lds + (size_t)N * ldb;
+    float2* C0 = lds + 2 * (size_t)N * ldb;
     for (int task = threadIdx.x; task < nlines * N; task += kFftThreads) {
         const int b = task % nb, rest = task / nb, f = rest % nf, n = rest / nf;
-        A[n * ldb + f * nb + b] = r.H[(size_t)f * r.Mh + (size_t)n * Q + q0 + b];
+        A0[n * ldb + f * nb + b] = r.H[(size_t)f * r.Mh + (size_t)n * Q + q0 + b];
     }
     __syncthreads();
-    lds_fft(A, ldb, nlines, r.ax[la]);
-    if (store_spectrum) {
-        for (int task = threadIdx.x; task < nlines * N; task += kFftThreads) {
-            const int b = task % nb, rest = task / nb, f = rest % nf, n = rest / nf;
-            r.H[(size_t)f * r.Mh + (size_t)n * Q + q0 + b] = A[n * ldb + f * nb + b];
-        }
-    }
-    // coupling, in place per element
+    float2* A = lds_fft(A0, B0, ldb, nlines, r.ax[la]);   // spectra rho_f(k)
+    float2* Wb = (A == A0) ? B0 : A0;                      // coupled fields w_f(k)
+    // coupling w_I = sum_J C_IJ rho_J (C symmetric, rows I <= J), charge w_c = G_c rho_c
     for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
         const int b = task % nb, n = task / nb;
         const size_t idx = (size_t)n * Q + q0 + b;
-        float2 rh[kMaxTypes + 1];
-        for (int f = 0; f < nf; ++f) rh[f] = A[n * ldb + f * nb + b];
-        int pidx = 0;
-        float2 w[kMaxTypes + 1];
-        for (int I = 0; I < T; ++I) w[I] = make_float2(0.f, 0.f);
-        for (int I = 0; I < T; ++I)
-            for (int J = I; J < T; ++J, ++pidx) {
+        const int row = n * ldb + b;
+        for (int I = 0; I < T; ++I) {
+            float2 acc = make_float2(0.f, 0.f);
+            for (int J = 0; J < T; ++J) {
+                const int lo = I < J ? I : J, hi = I < J ? J : I;
+                const int pidx = lo * T - lo * (lo - 1) / 2 + (hi - lo);
                 const float c = r.Ctab[(size_t)pidx * r.Mh + idx];
-                w[I].x += c * rh[J].x; w[I].y += c * rh[J].y;
-                if (J != I) { w[J].x += c * rh[I].x; w[J].y += c * rh[I].y; }
+                const float2 rh = A[row + J * nb];
+                acc.x += c * rh.x; acc.y += c * rh.y;
             }
+            Wb[row + I * nb] = acc;
+        }
         if (nf > T) {
             const float c = r.Ctab[(size_t)r.ctab_charge * r.Mh + idx];
-            w[T] = make_float2(c * rh[T].x, c * rh[T].y);
+            const float2 rh = A[row + T * nb];
+            Wb[row + T * nb] = make_float2(c * rh.x, c * rh.y);
         }
-        for (int f = 0; f < nf; ++f) A[n * ldb + f * nb + b] = w[f];
     }
     __syncthreads();
+    float2* B = A;                                         // work line of direction d
+    float2* C = C0;                                        // its ping-pong partner
     for (int d = 0; d < r.dim; ++d) {
         for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
             const int b = task % nb, n = task / nb;
@@ -222,16 +221,16 @@ __global__ void __launch_bounds__(kFftThreads) k_last_fused(const RepDev* __rest
             if (2 * sidx == Nd) kd = 0.f;
             else kd = r.kfac[d] * (float)(2 * sidx < Nd ? sidx : sidx - Nd);
             for (int f = 0; f < nf; ++f) {
-                float2 w = A[n * ldb + f * nb + b];
+                float2 w = Wb[n * ldb + f * nb + b];
                 // conj(-i k w) = (k w.y, k w.x): the inverse is taken as conj(FFT(conj))
                 B[n * ldb + f * nb + b] = make_float2(kd * w.y, kd * w.x);
             }
         }
         __syncthreads();
-        lds_fft(B, ldb, nlines, r.ax[la]);
+        const float2* res = lds_fft(B, C, ldb, nlines, r.ax[la]);
         for (int task = threadIdx.x; task < nlines * N; task += kFftThreads) {
             const int b = task % nb, rest = task / nb, f = rest % nf, n = rest / nf;
-            r.Gk[(size_t)(f * r.dim + d) * r.Mh + (size_t)n * Q + q0 + b] = conjf2(B[n * ldb + f * nb + b]);
+            r.Gk[(size_t)(f * r.dim + d) * r.Mh + (size_t)n * Q + q0 + b] = conjf2(res[n * ldb + f * nb + b]);
         }
         __syncthreads();
     }
@@ -277,7 +276,7 @@ __global__ void __launch_bounds__(kFftThreads) k_x_c2r(const RepDev* __restrict_
         lds[k * ldb + b] = make_float2(A.x - Bv.y, -(A.y + Bv.x));
     }
     __syncthreads();
-    lds_fft(lds, ldb, nb, r.ax[0]);
+    const float2* res = lds_fft(lds, lds + (size_t)N * ldb, ldb, nb, r.ax[0]);
     if (RECORDS) {
         const int D = r.dim;
         for (int task = threadIdx.x; task < rows * r.nf * N; task += kFftThreads) {
@@ -286,7 +285,7 @@ __global__ void __launch_bounds__(kFftThreads) k_x_c2r(const RepDev* __restrict_
             float g[3] = {0.f, 0.f, 0.f};
             for (int d = 0; d < D; ++d) {
                 const int line = f * D + d;
-                const float2 z = lds[x * ldb + rr * nc + line / 2];
+                const float2 z = res[x * ldb + rr * nc + line / 2];
                 g[d] = (line & 1) ? -z.y : z.x;      // conj of the forward result
             }
             const size_t node = (size_t)row * N + x;
@@ -296,7 +295,7 @@ __global__ void __launch_bounds__(kFftThreads) k_x_c2r(const RepDev* __restrict_
     } else {
         for (int task = threadIdx.x; task < rows * nlines * N; task += kFftThreads) {
             const int x = task % N, rest = task / N, line = rest % nlines, rr = rest / nlines;
-            const float2 z = lds[x * ldb + rr * nc + line / 2];
+            const float2 z = res[x * ldb + rr * nc + line / 2];
             out_override[((size_t)line * rows_per_field + row0 + rr) * N + x] = (line & 1) ? -z.y : z.x;
         }
     }

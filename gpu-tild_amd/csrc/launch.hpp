@@ -5,8 +5,9 @@
 
 namespace tild {
 
-// Largest batch of complex values one FFT block holds across its stages.
-constexpr int kFftBudget = kFftElems * kFftThreads;        // 4096
+// Largest batch of complex values one FFT block transforms at once (two LDS
+// ping-pong buffers of this size).
+constexpr int kFftBudget = kFftElems * kFftThreads;        // 2048
 constexpr size_t kMaxLds = 65536;
 
 // Strided pass along y (axis 1) or z (axis 2) of nfb half-spectrum fields.
@@ -26,11 +27,13 @@ __host__ __device__ inline MidGeom mid_geom(const RepDev& r, int axis, int nfb) 
     int nl = kFftBudget / g.N;
     if (nl < 1) nl = 1;
     if (nl > g.batch) nl = g.batch;
-    g.nl = nl;
     g.nchunk = (g.batch + nl - 1) / nl;
+    g.nl = (g.batch + g.nchunk - 1) / g.nchunk;     // balanced chunks
     return g;
 }
-inline size_t lds_bytes(int N, int nlines) { return (size_t)N * (size_t)(nlines | 1) * sizeof(float2); }
+// one FFT buffer of N x nlines complex values, and the ping-pong pair a pass needs
+inline size_t fft_buf_bytes(int N, int nlines) { return (size_t)N * (size_t)(nlines | 1) * sizeof(float2); }
+inline size_t lds_bytes(int N, int nlines) { return 2 * fft_buf_bytes(N, nlines); }
 
 struct LaunchCtx {
     const RepDev* reps;
@@ -50,6 +53,10 @@ hipError_t launch_kprops(const LaunchCtx& c, int nq_max);
 hipError_t launch_bond_props(const LaunchCtx& c);
 hipError_t launch_reduce_rows(hipStream_t s, const double* in, int nrows, int n, double* out);
 hipError_t launch_advance(hipStream_t s, RepDev* reps, const int* which, int n);
+hipError_t launch_tile_count(const LaunchCtx& c);
+hipError_t launch_tile_scan(const LaunchCtx& c);
+hipError_t launch_tile_scatter(const LaunchCtx& c);
+hipError_t launch_tile_deposit(const LaunchCtx& c);
 hipError_t launch_bins(hipStream_t s, const RepDev* reps, int rep, int ns, int* out);
 hipError_t launch_virial_source(hipStream_t s, const RepDev* reps, int rep, int M, const float* fr, float* vr);
 

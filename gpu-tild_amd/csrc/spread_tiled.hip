@@ -1,0 +1,211 @@
+// spread_tiled.hip -- sorted-bin, LDS-tiled particle-to-grid deposition fused with
+// the x-axis R2C transform (replaces the global atomics of
+// device_grid_utils.cu:232-343 with per-tile LDS accumulation).
+//
+// The grid is cut into full-x slabs of By x Bz rows.  Every step:
+//   k_tile_count    bucket = (tile of the particle's first stencil row, halo class)
+//   k_tile_scan     exclusive offsets of the buckets (one block per replica)
+//   k_tile_scatter  particles copied into bucket order as (x, y, z, type) records
+//   k_tile_deposit  one block per tile: deposits its own particles and the halo
+//                   particles of its three lower neighbours into 64-bit
+//                   fixed-point LDS accumulators, converts its rows to densities
+//                   and transforms them along x (R2C) straight from LDS.
+// Integer accumulation makes the densities independent of bucket order, tile
+// shape and thread scheduling: they are bit-identical to the global-atomic path.
+//
+// Halo classes: a particle whose stencil rows also reach the next tile in y (yb)
+// and/or z (zb) is tagged 0 none, 1 y only, 2 y and z, 3 z only.  This order makes
+// every neighbour's needed subset contiguous: y-neighbour classes [1,3),
+// z-neighbour [2,4), yz-neighbour [2,3).
+#include "device.hpp"
+#include "fft_dev.hpp"
+#include "launch.hpp"
+
+namespace tild {
+
+__device__ inline int tile_bucket(const RepDev& r, const float* x, int i) {
+    const int D = r.dim, P = r.P;
+    float W[5];
+    const int sy = stencil_1d(x[i * D + 1], r.dx[1], r.Ny, P, W);
+    const int sz = D == 3 ? stencil_1d(x[i * D + 2], r.dx[2], r.Nz, P, W) : 0;
+    const int ty = sy / r.By, tz = sz / r.Bz;
+    const bool yb = (sy % r.By) + P >= r.By;
+    const bool zb = D == 3 && (sz % r.Bz) + P >= r.Bz;
+    const int c = (yb && zb) ? 2 : (yb ? 1 : (zb ? 3 : 0));
+    return (tz * r.nty + ty) * 4 + c;
+}
+
+__global__ void __launch_bounds__(256) k_tile_count(const RepDev* __restrict__ reps,
+                                                    const WorkItem* __restrict__ wmap) {
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    const int i = wi.blk * 256 + threadIdx.x;
+    if (i >= r.ns) return;
+    const int b = tile_bucket(r, r.X[r.step & 1], i);
+    const int slot = atomicAdd(r.tcnt + b, 1);
+    r.tkey[i] = make_int2(b, slot);
+}
+
+// Exclusive scan of the bucket counts; clears the counts for the next step.
+__global__ void __launch_bounds__(1024) k_tile_scan(const RepDev* __restrict__ reps,
+                                                    const WorkItem* __restrict__ wmap) {
+    __shared__ int part[1024];
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    const int nb = r.nbuckets;
+    const int per = (nb + 1023) / 1024;
+    const int b0 = threadIdx.x * per, b1 = min(nb, b0 + per);
+    int s = 0;
+    for (int b = b0; b < b1; ++b) s += r.tcnt[b];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {      // Hillis-Steele inclusive scan
+        int v = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = threadIdx.x > 0 ? part[threadIdx.x - 1] : 0;
+    for (int b = b0; b < b1; ++b) {
+        const int c = r.tcnt[b];
+        r.toff[b] = run;
+        run += c;
+        r.tcnt[b] = 0;
+    }
+    if (threadIdx.x == 1023) r.toff[nb] = part[1023];
+}
+
+__global__ void __launch_bounds__(256) k_tile_scatter(const RepDev* __restrict__ reps,
+                                                      const WorkItem* __restrict__ wmap) {
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    const int i = wi.blk * 256 + threadIdx.x;
+    if (i >= r.ns) return;
+    const int D = r.dim;
+    const float* x = r.X[r.step & 1];
+    const int2 k = r.tkey[i];
+    const int pos = r.toff[k.x] + k.y;
+    r.trec[pos] = make_float4(x[i * D], x[i * D + 1], D == 3 ? x[i * D + 2] : 0.f, __int_as_float(r.tp[i]));
+    if (r.trecq) r.trecq[pos] = r.q[i];
+}
+
+// Deposit one tile's rows and transform them along x.
+__global__ void __launch_bounds__(kFftThreads) k_tile_deposit(const RepDev* __restrict__ reps,
+                                                              const WorkItem* __restrict__ wmap) {
+    extern __shared__ unsigned long long tile_lds[];
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    const int D = r.dim, P = r.P, Nx = r.Nx, Nxh = r.Nxh;
+    const int By = r.By, Bz = r.Bz, nf = r.nf;
+    const int t = wi.blk, ty = t % r.nty, tz = t / r.nty;
+    const int y0 = ty * By, z0 = tz * Bz;
+    const int rows = nf * By * Bz;
+    unsigned long long* patch = tile_lds;                         // [f][lz][ly][x]
+    float2* fbuf = (float2*)(tile_lds + (size_t)rows * Nx);
+    for (int k = threadIdx.x; k < rows * Nx; k += kFftThreads) patch[k] = 0ull;
+    __syncthreads();
+    // source ranges: own tile (all classes) and the lower neighbours' halo classes
+    int ra[4], rb[4], nr = 0;
+    {
+        const int tym = (ty + r.nty - 1) % r.nty;
+        const int tzm = (tz + r.ntz - 1) % r.ntz;
+        const int own = tz * r.nty + ty;
+        ra[nr] = r.toff[own * 4]; rb[nr] = r.toff[own * 4 + 4]; ++nr;
+        const int ny_ = tz * r.nty + tym;
+        ra[nr] = r.toff[ny_ * 4 + 1]; rb[nr] = r.toff[ny_ * 4 + 3]; ++nr;
+        if (D == 3) {
+            const int nz_ = tzm * r.nty + ty;
+            ra[nr] = r.toff[nz_ * 4 + 2]; rb[nr] = r.toff[nz_ * 4 + 4]; ++nr;
+            const int nyz = tzm * r.nty + tym;
+            ra[nr] = r.toff[nyz * 4 + 2]; rb[nr] = r.toff[nyz * 4 + 3]; ++nr;
+        }
+    }
+    const int T = r.ntypes;
+    for (int s = 0; s < nr; ++s) {
+        for (int p = ra[s] + threadIdx.x; p < rb[s]; p += kFftThreads) {
+            const float4 rec = r.trec[p];
+            const int tp = __float_as_int(rec.w);
+            float W[3][5];
+            int n0[3];
+            n0[0] = stencil_1d(rec.x, r.dx[0], Nx, P, W[0]);
+            n0[1] = stencil_1d(rec.y, r.dx[1], r.Ny, P, W[1]);
+            if (D == 3) n0[2] = stencil_1d(rec.z, r.dx[2], r.Nz, P, W[2]);
+            else { n0[2] = 0; W[2][0] = 1.f; }
+            const float qi = r.trecq ? r.trecq[p] : 0.f;
+            const int nz = D == 3 ? P + 1 : 1;
+            for (int iz = 0, zz = n0[2]; iz < nz; ++iz, zz = wrap_next(zz, r.Nz)) {
+                const int lz = zz - z0;
+                if (lz < 0 || lz >= Bz) continue;
+                for (int iy = 0, yy = n0[1]; iy <= P; ++iy, yy = wrap_next(yy, r.Ny)) {
+                    const int ly = yy - y0;
+                    if (ly < 0 || ly >= By) continue;
+                    unsigned long long* rowp = patch + ((size_t)(tp * Bz + lz) * By + ly) * Nx;
+                    unsigned long long* rowq = patch + ((size_t)(T * Bz + lz) * By + ly) * Nx;
+                    for (int ix = 0, xx = n0[0]; ix <= P; ++ix, xx = wrap_next(xx, Nx)) {
+                        const float w = D == 3 ? W[0][ix] * W[1][iy] * W[2][iz] : W[0][ix] * W[1][iy];
+                        atomicAdd(rowp + xx, (unsigned long long)__float2ll_rn(w * r.fix_scale));
+                        if (r.trecq)
+                            atomicAdd(rowq + xx, (unsigned long long)__float2ll_rn((w * qi) * r.fixq_scale));
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // densities of the tile rows -> rho and the x-axis R2C (pairs of rows)
+    const int npairs = (rows + 1) / 2;
+    const int ldb = npairs | 1;
+    for (int task = threadIdx.x; task < npairs * Nx; task += kFftThreads) {
+        const int j = task / Nx, n = task % Nx;
+        float v[2] = {0.f, 0.f};
+        for (int h = 0; h < 2; ++h) {
+            const int rr = 2 * j + h;
+            if (rr >= rows) break;
+            const int f = rr / (By * Bz), lz = (rr / By) % Bz, ly = rr % By;
+            const double inv = f < T ? r.fix_inv : r.fixq_inv;
+            v[h] = (float)((double)(long long)patch[(size_t)rr * Nx + n] * inv);
+            r.rho[(size_t)f * r.M + ((size_t)(z0 + lz) * r.Ny + (y0 + ly)) * Nx + n] = v[h];
+        }
+        fbuf[n * ldb + j] = make_float2(v[0], v[1]);
+    }
+    __syncthreads();
+    const float2* res = lds_fft(fbuf, fbuf + (size_t)Nx * ldb, ldb, npairs, r.ax[0]);
+    for (int task = threadIdx.x; task < npairs * Nxh; task += kFftThreads) {
+        const int j = task / Nxh, k = task % Nxh;
+        const float2 Zk = res[k * ldb + j];
+        const float2 Zm = res[((Nx - k) % Nx) * ldb + j];
+        for (int h = 0; h < 2; ++h) {
+            const int rr = 2 * j + h;
+            if (rr >= rows) break;
+            const int f = rr / (By * Bz), lz = (rr / By) % Bz, ly = rr % By;
+            const size_t row = ((size_t)f * r.Nz + (z0 + lz)) * r.Ny + (y0 + ly);
+            r.H[row * Nxh + k] = h == 0 ? make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y))
+                                        : make_float2(0.5f * (Zk.y + Zm.y), 0.5f * (Zm.x - Zk.x));
+        }
+    }
+}
+
+#define CHECK_LAUNCH() do { hipError_t e__ = hipGetLastError(); if (e__ != hipSuccess) return e__; } while (0)
+
+hipError_t launch_tile_count(const LaunchCtx& c) {
+    if (!c.n) return hipSuccess;
+    hipLaunchKernelGGL(k_tile_count, dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap);
+    CHECK_LAUNCH(); return hipSuccess;
+}
+hipError_t launch_tile_scan(const LaunchCtx& c) {
+    if (!c.n) return hipSuccess;
+    hipLaunchKernelGGL(k_tile_scan, dim3(c.n), dim3(1024), 0, c.stream, c.reps, c.wmap);
+    CHECK_LAUNCH(); return hipSuccess;
+}
+hipError_t launch_tile_scatter(const LaunchCtx& c) {
+    if (!c.n) return hipSuccess;
+    hipLaunchKernelGGL(k_tile_scatter, dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap);
+    CHECK_LAUNCH(); return hipSuccess;
+}
+hipError_t launch_tile_deposit(const LaunchCtx& c) {
+    if (!c.n) return hipSuccess;
+    hipLaunchKernelGGL(k_tile_deposit, dim3(c.n), dim3(kFftThreads), c.lds, c.stream, c.reps, c.wmap);
+    CHECK_LAUNCH(); return hipSuccess;
+}
+
+}  // namespace tild

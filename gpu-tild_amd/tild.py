@@ -174,9 +174,13 @@ class System:
 class Engine:
     """One device-resident replica (tild_engine)."""
 
-    def __init__(self, system, device=0, seed=None):
+    def __init__(self, system, device=0, seed=None, spread_mode=0, tile_target=0, tile_by=0, tile_bz=0):
         o = Opts()
         o.device = device
+        o.spread_mode = spread_mode
+        o.reserved[0] = tile_target
+        o.reserved[1] = tile_by
+        o.reserved[2] = tile_bz
         if seed is not None:
             o.use_seed = 1
             o.seed = int(seed) & 0xFFFFFFFF
