@@ -141,8 +141,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this engine runs on one GPU only (replicas shard across GPUs by design, "
-              "see DESIGN.md); --gpus must be 1", file=sys.stderr)
+        print("bench.py: this engine runs on one GPU only (independent replicas shard across GPUs "
+              "without communication); --gpus must be 1", file=sys.stderr)
         sys.exit(2)
     run(args)
 

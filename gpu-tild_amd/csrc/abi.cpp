@@ -14,6 +14,10 @@ struct tild_engine {
     int id = -1;
     std::string err;
 };
+struct tild_batch {
+    std::unique_ptr<tild::Batch> batch;
+    std::string err;
+};
 
 namespace {
 thread_local std::string g_last_error;
@@ -56,11 +60,17 @@ tild_status tild_system_info(const tild_system* s, const char* key, double* v) {
     else if (k == "Nx2") *v = S.Dim > 2 ? S.Nx[2] : 1;
     else if (k == "pmeorder") *v = S.pmeorder;
     else if (k == "n_styles") *v = (double)S.styles().size();
+    else if (k == "n_gaussians") *v = (double)S.Gausses.size();
+    else if (k == "n_erfs") *v = (double)S.Erfs.size();
+    else if (k == "n_gaussian_erfs") *v = (double)S.GaussErfs.size();
     else if (k == "do_charges") *v = S.charges.enabled ? 1 : 0;
     else if (k == "n_bonds_total") *v = S.n_total_bonds;
     else if (k == "nbond_types") *v = S.nbond_types;
     else if (k == "max_steps") *v = S.max_steps;
     else if (k == "log_freq") *v = S.log_freq;
+    else if (k == "traj_freq") *v = S.traj_freq;
+    else if (k == "grid_freq") *v = S.grid_freq;
+    else if (k == "binary_freq") *v = S.bin_freq;
     else if (k == "n_groups") *v = (double)S.Groups.size();
     else if (k == "n_integrators") *v = (double)S.Integrators.size();
     else if (k == "seed") *v = S.rand_seed;
@@ -77,6 +87,16 @@ tild_status tild_system_info(const tild_system* s, const char* key, double* v) {
     else if (k == "bjerrum") *v = S.charges.bjerrum_length;
     else if (k == "smear") *v = S.charges.smearing_length;
     else return fail(TILD_ERR_ARG, "unknown key " + k);
+    return TILD_OK;
+}
+
+tild_status tild_system_info_text(const tild_system* s, const char* key, char* buf, size_t cap) {
+    if (!s || !key || !buf || cap == 0) return fail(TILD_ERR_ARG, "null argument");
+    std::string k(key), v;
+    if (k == "dump_name") v = s->sys.dump_name;
+    else return fail(TILD_ERR_ARG, "unknown key " + k);
+    if (v.size() + 1 > cap) return fail(TILD_ERR_ARG, "buffer too small");
+    std::memcpy(buf, v.c_str(), v.size() + 1);
     return TILD_OK;
 }
 
@@ -187,9 +207,137 @@ tild_status tild_engine_profile(tild_engine* e, int nsteps, double* phase_ms, in
     return from_rc(rc, err, e);
 }
 
+// ------------------------------------------------------------------ batch
+tild_status tild_batch_create(const tild_opts* opts, tild_batch** out) {
+    if (!out) return fail(TILD_ERR_ARG, "null argument");
+    *out = nullptr;
+    tild_opts o{};
+    if (opts) o = *opts;
+    auto b = std::make_unique<tild_batch>();
+    b->batch = std::make_unique<tild::Batch>(o.device);
+    b->batch->set_spread(o.spread_mode, o.reserved[0] > 0 ? o.reserved[0] : 1024, o.reserved[1], o.reserved[2]);
+    std::string err;
+    int rc = b->batch->init(err);
+    if (rc < 0) return from_rc(rc, err, nullptr);
+    *out = b.release();
+    return TILD_OK;
+}
+
+void tild_batch_destroy(tild_batch* b) { delete b; }
+
+const char* tild_batch_last_error(const tild_batch* b) { return b ? b->err.c_str() : g_last_error.c_str(); }
+
+#define BATCH_CALL(call)                                    \
+    if (!b) return fail(TILD_ERR_ARG, "null batch");        \
+    std::string err;                                        \
+    int rc = b->batch->call;                                \
+    if (rc < 0) { b->err = err; g_last_error = err; return (tild_status)(-rc); } \
+    return TILD_OK;
+
+tild_status tild_batch_add(tild_batch* b, const tild_system* s, uint32_t seed, int* replica) {
+    if (!b || !s || !replica) return fail(TILD_ERR_ARG, "null argument");
+    std::string err;
+    int id = b->batch->add(s->sys, seed, err);
+    if (id < 0) { b->err = err; g_last_error = err; return (tild_status)(-id); }
+    *replica = id;
+    return TILD_OK;
+}
+
+tild_status tild_batch_remove(tild_batch* b, int replica) {
+    if (!b || !b->batch->rep(replica)) return fail(TILD_ERR_ARG, "no such replica");
+    b->batch->remove(replica);
+    return TILD_OK;
+}
+
+tild_status tild_batch_step(tild_batch* b, const int* replicas, int nrep, int nsteps, int use_graph) {
+    if (!b || nsteps < 0 || nrep < 0 || (nrep > 0 && !replicas)) return fail(TILD_ERR_ARG, "bad argument");
+    std::vector<int> ids(replicas, replicas + nrep);
+    for (int id : ids) if (!b->batch->rep(id)) return fail(TILD_ERR_ARG, "no such replica");
+    std::string err;
+    int rc = b->batch->step(ids, nsteps, err, use_graph != 0);
+    if (rc >= 0) rc = b->batch->sync(err);
+    if (rc < 0) { b->err = err; g_last_error = err; return (tild_status)(-rc); }
+    return TILD_OK;
+}
+
+tild_status tild_batch_properties(tild_batch* b, int replica, tild_props* out) {
+    if (!out) return fail(TILD_ERR_ARG, "null argument");
+    BATCH_CALL(properties(replica, out, err))
+}
+tild_status tild_batch_get_positions(tild_batch* b, int r, float* x, size_t n) { BATCH_CALL(get_positions(r, x, n, err)) }
+tild_status tild_batch_set_positions(tild_batch* b, int r, const float* x, size_t n) { BATCH_CALL(set_positions(r, x, n, err)) }
+tild_status tild_batch_get_density(tild_batch* b, int r, int t, float* rho, size_t n) { BATCH_CALL(get_density(r, t, rho, n, err)) }
+tild_status tild_batch_replica_step(tild_batch* b, int r, int* step) {
+    if (!b || !step || !b->batch->rep(r)) return fail(TILD_ERR_ARG, "no such replica");
+    *step = b->batch->rep(r)->step;
+    return TILD_OK;
+}
+tild_status tild_batch_profile(tild_batch* b, const int* replicas, int nrep, int nsteps, double* ms, int nphase) {
+    if (!b || nrep < 0 || (nrep > 0 && !replicas)) return fail(TILD_ERR_ARG, "bad argument");
+    std::vector<int> ids(replicas, replicas + nrep);
+    BATCH_CALL(profile(ids, nsteps, ms, nphase, err))
+}
+
 tild_status tild_engine_get_step(tild_engine* e, int* step) {
     if (!e || !step) return fail(TILD_ERR_ARG, "null argument");
     *step = e->batch->rep(e->id)->step;
+    return TILD_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ job server
+#include "server.hpp"
+
+struct tild_server {
+    std::unique_ptr<tild::Server> srv;
+};
+
+extern "C" {
+
+tild_status tild_server_create(const tild_server_opts* opts, tild_server** out) {
+    if (!out) return fail(TILD_ERR_ARG, "null argument");
+    *out = nullptr;
+    tild_server_opts o{};
+    if (opts) o = *opts;
+    if (o.chunk_steps < 0 || o.max_replicas < 0) return fail(TILD_ERR_ARG, "bad server options");
+    auto s = std::make_unique<tild_server>();
+    s->srv = std::make_unique<tild::Server>(o);
+    std::string err;
+    int rc = s->srv->start(err);
+    if (rc < 0) return from_rc(rc, err, nullptr);
+    *out = s.release();
+    return TILD_OK;
+}
+
+void tild_server_destroy(tild_server* s) { delete s; }
+
+#define SERVER_CALL(call)                                   \
+    if (!s) return fail(TILD_ERR_ARG, "null server");       \
+    std::string err;                                        \
+    int rc = s->srv->call;                                  \
+    if (rc < 0) return from_rc(rc, err, nullptr);           \
+    return TILD_OK;
+
+tild_status tild_server_submit(tild_server* s, const char* text, const char* base, uint32_t seed, int nsteps,
+                               int* job) {
+    if (!text || !job) return fail(TILD_ERR_ARG, "null argument");
+    SERVER_CALL(submit(text, base ? base : "", seed, nsteps, job, err))
+}
+tild_status tild_server_poll(tild_server* s, int job, int* state) {
+    if (!state) return fail(TILD_ERR_ARG, "null argument");
+    if (!s) return fail(TILD_ERR_ARG, "null server");
+    if (s->srv->poll(job, state) < 0) return fail(TILD_ERR_ARG, "no such job");
+    return TILD_OK;
+}
+tild_status tild_server_wait(tild_server* s, int job, double timeout_s) { SERVER_CALL(wait(job, timeout_s, err)) }
+tild_status tild_server_drain(tild_server* s, double timeout_s) { SERVER_CALL(drain(timeout_s, err)) }
+tild_status tild_server_result(tild_server* s, int job, tild_props* props, float* x, size_t n, double* lat) {
+    SERVER_CALL(result(job, props, x, n, lat, err))
+}
+tild_status tild_server_stats_get(tild_server* s, tild_server_stats* out) {
+    if (!s || !out) return fail(TILD_ERR_ARG, "null argument");
+    s->srv->stats(out);
     return TILD_OK;
 }
 

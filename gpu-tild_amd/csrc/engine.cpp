@@ -107,6 +107,7 @@ Batch::~Batch() {
     for (auto& kv : set_cache_) {
         free_maps(kv.second->maps);
         if (kv.second->d_ids) (void)hipFree(kv.second->d_ids);
+        if (kv.second->graph) (void)hipGraphExecDestroy(kv.second->graph);
     }
     for (auto& r : reps_)
         if (r) {
@@ -136,7 +137,8 @@ size_t Batch::estimate_bytes(const System& S) {
     size_t Nxh = S.Nx[0] / 2 + 1, Mh = M / S.Nx[0] * Nxh;
     size_t nf = S.ntypes + (S.charges.enabled ? 1 : 0);
     size_t T = S.ntypes, nst = S.styles().size();
-    size_t part = ns * (4 * D * 4 + 4 + 4 + 24 + 1);
+    // X[2], XO, fout; type, charge, bond tables, integrator kind; sorted-bin keys and records
+    size_t part = ns * (4 * D * 4 + 4 + 4 + 24 + 1) + ns * (8 + 16 + 4);
     size_t grids = nf * M * (8 + 4 + 16) + nf * Mh * 8 * (1 + D);
     size_t tabs = Mh * 4 * (T * (T + 1) / 2 + 1 + nst + 1) + Mh * 8 * (D == 3 ? 6 : 3) * nst;
     return part + grids + tabs + (1 << 20);
@@ -326,6 +328,7 @@ void Batch::remove(int id) {
         if (std::find(it->first.begin(), it->first.end(), id) != it->first.end()) {
             free_maps(it->second->maps);
             if (it->second->d_ids) (void)hipFree(it->second->d_ids);
+            if (it->second->graph) (void)hipGraphExecDestroy(it->second->graph);
             it = set_cache_.erase(it);
         } else ++it;
     }
@@ -333,7 +336,10 @@ void Batch::remove(int id) {
 
 int Batch::upload_desc(int id, std::string& err) {
     if (id >= d_cap_) {
-        int cap = std::max(16, d_cap_);
+        // captured graphs hold the descriptor array's address: drop them
+        for (auto& kv : set_cache_)
+            if (kv.second->graph) { (void)hipGraphExecDestroy(kv.second->graph); kv.second->graph = nullptr; }
+        int cap = std::max(256, d_cap_);
         while (cap <= id) cap *= 2;
         RepDev* nd;
         HIPCHK(hipMalloc((void**)&nd, sizeof(RepDev) * cap));
@@ -377,7 +383,8 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
         }
         {
             MidGeom g = mid_geom(h, 1, h.nf);
-            add(PH_MIDY_H, g.nouter * g.nchunk, lds_bytes(g.N, g.nl));
+            add(PH_PY, g.nouter * g.nchunk, lds_bytes(g.N, g.nl));
+            if (D == 3) add(PH_MIDY_H, g.nouter * g.nchunk, lds_bytes(g.N, g.nl));
         }
         if (D == 3) {
             MidGeom g = mid_geom(h, 1, h.nf * D);
@@ -614,16 +621,15 @@ int Batch::build_tables(Replica& R, const System& S, std::string& err) {
 int Batch::enqueue_step(SetMaps& m, bool integrate, bool write_forces, std::string& err) {
     PhaseMap* mp = m.maps;
     auto ctx = [&](Phase p) { return LaunchCtx{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_}; };
-    bool any3d = mp[PH_MIDY_G].n > 0;
     HIPCHK(launch_spread(ctx(PH_SPREADA)));
     HIPCHK(launch_x_r2c(ctx(PH_XA), true, 0, nullptr, nullptr));
     HIPCHK(launch_tile_count(ctx(PH_TPART)));
     HIPCHK(launch_tile_scan(ctx(PH_SCAN)));
     HIPCHK(launch_tile_scatter(ctx(PH_TPART)));
     HIPCHK(launch_tile_deposit(ctx(PH_TILE)));
-    if (any3d) HIPCHK(launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr));
+    HIPCHK(launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr));
     HIPCHK(launch_last_fused(ctx(PH_LAST), 0));
-    if (any3d) HIPCHK(launch_mid(ctx(PH_MIDY_G), 1, 1, 0, 1, nullptr));
+    HIPCHK(launch_mid(ctx(PH_MIDY_G), 1, 1, 0, 1, nullptr));
     HIPCHK(launch_x_c2r(ctx(PH_C2R), true, 0, nullptr, nullptr));
     HIPCHK(launch_particles(ctx(PH_PART), integrate, write_forces ? 1 : 0));
     if (integrate) HIPCHK(launch_advance(stream_, d_reps_, m.d_ids, m.nids));
@@ -639,7 +645,7 @@ int Batch::forces(int id, std::string& err) {
     return enqueue_step(*m, false, true, err);
 }
 
-int Batch::step(const std::vector<int>& ids_in, int n, std::string& err) {
+int Batch::step(const std::vector<int>& ids_in, int n, std::string& err, bool use_graph) {
     std::vector<int> ids = ids_in;
     if (ids.empty())
         for (int k = 0; k < (int)reps_.size(); ++k) if (reps_[k] && reps_[k]->active) ids.push_back(k);
@@ -647,8 +653,23 @@ int Batch::step(const std::vector<int>& ids_in, int n, std::string& err) {
     HIPCHK(hipSetDevice(device_));
     SetMaps* m = maps_for(ids, err);
     if (!m) return -TILD_ERR_DEVICE;
+    if (use_graph && !m->graph) {
+        // Capture one integrating step.  Every kernel argument is constant across steps:
+        // step counters and position parity live in the device descriptors and are
+        // advanced by the last node, so the same graph serves every step.
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_step(*m, true, false, err);
+        hipError_t ec = hipStreamEndCapture(stream_, &g);
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        HIPCHK(ec);
+        hipError_t ei = hipGraphInstantiate(&m->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(ei);
+    }
     for (int s = 0; s < n; ++s) {
-        if (int rc = enqueue_step(*m, true, false, err)) return rc;
+        if (use_graph) HIPCHK(hipGraphLaunch(m->graph, stream_));
+        else if (int rc = enqueue_step(*m, true, false, err)) return rc;
         for (int id : ids) reps_[id]->step += 1;
     }
     for (int id : ids) reps_[id]->h.step = reps_[id]->step;
@@ -665,7 +686,6 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
     std::vector<double> acc(kProfPhases, 0.0);
     PhaseMap* mp = m->maps;
     auto ctx = [&](Phase p) { return LaunchCtx{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_}; };
-    bool any3d = mp[PH_MIDY_G].n > 0;
     int rc = 0;
     for (int s = 0; s < n && !rc; ++s) {
         hipError_t e0 = hipSuccess;
@@ -679,11 +699,11 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
         if (e0 == hipSuccess) e0 = launch_x_r2c(ctx(PH_XA), true, 0, nullptr, nullptr);
         if (e0 == hipSuccess) e0 = launch_tile_deposit(ctx(PH_TILE));
         rec(2);
-        if (e0 == hipSuccess && any3d) e0 = launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr);
+        if (e0 == hipSuccess) e0 = launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr);
         rec(3);
         if (e0 == hipSuccess) e0 = launch_last_fused(ctx(PH_LAST), 0);
         rec(4);
-        if (e0 == hipSuccess && any3d) e0 = launch_mid(ctx(PH_MIDY_G), 1, 1, 0, 1, nullptr);
+        if (e0 == hipSuccess) e0 = launch_mid(ctx(PH_MIDY_G), 1, 1, 0, 1, nullptr);
         rec(5);
         if (e0 == hipSuccess) e0 = launch_x_c2r(ctx(PH_C2R), true, 0, nullptr, nullptr);
         rec(6);
@@ -711,7 +731,7 @@ int Batch::forward_spectra(int id, std::string& err) {
     PhaseMap* mp = R.maps;
     auto ctx = [&](Phase p) { return LaunchCtx{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_}; };
     HIPCHK(launch_x_r2c(ctx(PH_XP), false, 0, nullptr, nullptr));
-    HIPCHK(launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr));
+    HIPCHK(launch_mid(ctx(PH_PY), 1, 0, 0, 0, nullptr));
     if (R.dim == 3) HIPCHK(launch_mid(ctx(PH_MIDZ_H), 2, 0, 0, 0, nullptr));
     return 0;
 }

@@ -23,8 +23,10 @@ constexpr int kProfPhases = 8;   // spread, x_r2c, y_fwd, last_fused, y_inv, x_c
 
 // Work-map phases.  *A: replicas on the global-atomic spread; T*: replicas on the
 // LDS-tiled spread; XP: x-axis R2C of stored densities (properties).
+// MIDY_H/MIDY_G: y-axis passes of the step (3D replicas); PY: forward y-axis pass
+// of stored spectra for properties (every replica: it is the last axis in 2D).
 enum Phase { PH_PART = 0, PH_SPREADA, PH_XA, PH_XP, PH_TPART, PH_SCAN, PH_TILE, PH_MIDY_H, PH_MIDY_G,
-             PH_MIDZ_H, PH_LAST, PH_C2R, PH_KPROPS, PH_COUNT };
+             PH_MIDZ_H, PH_PY, PH_LAST, PH_C2R, PH_KPROPS, PH_COUNT };
 
 struct PhaseMap {
     std::vector<WorkItem> host;
@@ -70,8 +72,9 @@ public:
 
     // forces() of one replica at its current positions (no integration).
     int forces(int id, std::string& err);
-    // n steps of every replica in `ids` (empty = all active), one launch per phase.
-    int step(const std::vector<int>& ids, int n, std::string& err);
+    // n steps of every replica in `ids` (empty = all active), one launch per phase;
+    // with use_graph the step is captured once per replica set and replayed.
+    int step(const std::vector<int>& ids, int n, std::string& err, bool use_graph = false);
     // n steps with an event pair around every launch; ms[k] = mean ms of phase k.
     int profile(const std::vector<int>& ids, int n, double* ms, int nph, std::string& err);
     // calc_properties() of one replica.
@@ -87,7 +90,12 @@ public:
     hipStream_t stream() const { return stream_; }
 
     // Work maps of an arbitrary replica set (cached by set).
-    struct SetMaps { PhaseMap maps[PH_COUNT]; int* d_ids = nullptr; int nids = 0; };
+    struct SetMaps {
+        PhaseMap maps[PH_COUNT];
+        int* d_ids = nullptr;
+        int nids = 0;
+        hipGraphExec_t graph = nullptr;   // one captured integrating step of this set
+    };
     SetMaps* maps_for(const std::vector<int>& ids, std::string& err);
 
     // Step pipeline of a replica set with prepared maps (graph-capturable).

@@ -1,0 +1,269 @@
+// server.cpp -- continuous batching of TILD jobs on one device (server.hpp).
+#include "server.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+namespace tild {
+
+double Server::now() const {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+Server::~Server() {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
+    }
+    cv_.notify_all();
+    if (worker_.joinable()) worker_.join();
+    batch_.reset();
+}
+
+int Server::start(std::string& err) {
+    if (opts_.chunk_steps <= 0) opts_.chunk_steps = 100;
+    batch_ = std::make_unique<Batch>(opts_.device);
+    batch_->set_spread(0);
+    if (int rc = batch_->init(err)) return rc;
+    if (opts_.hbm_budget == 0) {
+        size_t fr = 0, tot = 0;
+        hipError_t e = hipMemGetInfo(&fr, &tot);
+        if (e != hipSuccess) { err = std::string("hipMemGetInfo: ") + hipGetErrorString(e); return -TILD_ERR_DEVICE; }
+        opts_.hbm_budget = (size_t)(0.9 * (double)fr);
+    }
+    t0_ = now();
+    worker_ = std::thread([this] { loop(); });
+    return 0;
+}
+
+int Server::submit(const std::string& text, const std::string& base, uint32_t seed, int nsteps, int* job,
+                   std::string& err) {
+    if (nsteps < 1) { err = "a job needs at least one step"; return -TILD_ERR_ARG; }
+    auto J = std::make_unique<Job>();
+    J->sys = std::make_unique<System>();
+    if (int rc = load_system(text, base, *J->sys, err)) return -rc;
+    J->bytes = Batch::estimate_bytes(*J->sys);
+    J->seed = seed;
+    J->nsteps = nsteps;
+    J->ns = J->sys->ns;
+    J->dim = J->sys->Dim;
+    std::lock_guard<std::mutex> lk(mu_);
+    if (!fatal_.empty()) { err = fatal_; return -TILD_ERR_DEVICE; }
+    if (J->bytes > opts_.hbm_budget) {
+        err = "job needs about " + std::to_string(J->bytes) + " bytes of device memory, budget is " +
+              std::to_string(opts_.hbm_budget);
+        return -TILD_ERR_BUDGET;
+    }
+    J->id = next_id_++;
+    J->t_submit = now();
+    *job = J->id;
+    queue_.push_back(J->id);
+    jobs_[J->id] = std::move(J);
+    cv_.notify_all();
+    return 0;
+}
+
+// Worker: admit, advance one chunk, retire; repeat.  Only this thread touches the Batch.
+void Server::loop() {
+    (void)hipSetDevice(opts_.device);
+    std::string err;
+    for (;;) {
+        // --- admission (arrival order; the queue head waits until it fits)
+        std::vector<Job*> admit;
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            cv_.wait(lk, [&] { return stop_ || !queue_.empty() || !running_.empty(); });
+            if (stop_) break;
+            size_t used = bytes_in_use_;
+            int nrun = (int)running_.size();
+            while (!queue_.empty()) {
+                Job* J = jobs_[queue_.front()].get();
+                if (used + J->bytes > opts_.hbm_budget) break;
+                if (opts_.max_replicas > 0 && nrun >= opts_.max_replicas) break;
+                used += J->bytes;
+                ++nrun;
+                admit.push_back(J);
+                queue_.pop_front();
+            }
+            bytes_in_use_ = used;   // reserved under the estimate until the replica exists
+        }
+        for (Job* J : admit) {
+            const double ts = now();
+            const int id = batch_->add(*J->sys, J->seed, err);
+            std::lock_guard<std::mutex> lk(mu_);
+            bytes_in_use_ -= J->bytes;
+            if (id < 0) {
+                J->state = TILD_JOB_FAILED;
+                J->err = err;
+                J->t_done = now();
+                done_cv_.notify_all();
+                continue;
+            }
+            J->replica = id;
+            J->bytes = batch_->rep(id)->bytes;
+            bytes_in_use_ += J->bytes;
+            peak_bytes_ = std::max(peak_bytes_, bytes_in_use_);
+            J->t_start = ts;
+            J->state = TILD_JOB_RUNNING;
+            J->sys.reset();   // the replica holds everything the step needs
+            running_.push_back(J->id);
+        }
+        // --- one chunk of every running job, cut at the nearest finish
+        std::vector<int> reps, ids;
+        int chunk = opts_.chunk_steps;
+        double beads = 0;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            for (int jid : running_) {
+                Job* J = jobs_[jid].get();
+                chunk = std::min(chunk, J->nsteps - J->done_steps);
+            }
+            for (int jid : running_) {
+                Job* J = jobs_[jid].get();
+                reps.push_back(J->replica);
+                ids.push_back(jid);
+                beads += J->ns;
+            }
+        }
+        if (reps.empty()) continue;
+        std::sort(reps.begin(), reps.end());
+        const double tc = now();
+        int rc = 0;
+        if (chunk > 0) {
+            rc = batch_->step(reps, chunk, err, opts_.use_graph != 0);
+            if (!rc) rc = batch_->sync(err);
+        }
+        const double dt = now() - tc;
+        // --- retire the finished jobs
+        std::vector<Job*> fin;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            busy_s_ += dt;
+            bead_steps_ += beads * std::max(chunk, 0);
+            for (int jid : ids) {
+                Job* J = jobs_[jid].get();
+                if (rc) { J->state = TILD_JOB_FAILED; J->err = err; fin.push_back(J); continue; }
+                J->done_steps += chunk;
+                if (J->done_steps >= J->nsteps) fin.push_back(J);
+            }
+            if (rc) fatal_ = err;
+        }
+        for (Job* J : fin) {
+            std::string e2;
+            if (J->state != TILD_JOB_FAILED) {
+                int r2 = batch_->properties(J->replica, &J->props, e2);
+                if (!r2) {
+                    J->x.resize((size_t)J->ns * J->dim);
+                    r2 = batch_->get_positions(J->replica, J->x.data(), J->x.size(), e2);
+                }
+                if (r2) { J->state = TILD_JOB_FAILED; J->err = e2; }
+            }
+            const size_t held = batch_->rep(J->replica) ? batch_->rep(J->replica)->bytes : 0;
+            batch_->remove(J->replica);
+            std::lock_guard<std::mutex> lk(mu_);
+            bytes_in_use_ -= held;
+            if (J->state != TILD_JOB_FAILED) J->state = TILD_JOB_DONE;
+            J->t_done = now();
+            running_.erase(std::find(running_.begin(), running_.end(), J->id));
+        }
+        if (!fin.empty()) done_cv_.notify_all();
+        if (rc) {
+            // a device error ends the server: fail whatever is still queued
+            std::lock_guard<std::mutex> lk(mu_);
+            for (int jid : queue_) {
+                Job* J = jobs_[jid].get();
+                J->state = TILD_JOB_FAILED;
+                J->err = fatal_;
+                J->t_done = now();
+            }
+            queue_.clear();
+            done_cv_.notify_all();
+            break;
+        }
+    }
+}
+
+int Server::poll(int job, int* state) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = jobs_.find(job);
+    if (it == jobs_.end()) return -TILD_ERR_ARG;
+    *state = it->second->state;
+    return 0;
+}
+
+int Server::wait(int job, double timeout_s, std::string& err) {
+    std::unique_lock<std::mutex> lk(mu_);
+    auto it = jobs_.find(job);
+    if (it == jobs_.end()) { err = "no such job"; return -TILD_ERR_ARG; }
+    Job* J = it->second.get();
+    auto done = [&] { return J->state == TILD_JOB_DONE || J->state == TILD_JOB_FAILED; };
+    if (timeout_s < 0) done_cv_.wait(lk, done);
+    else if (!done_cv_.wait_for(lk, std::chrono::duration<double>(timeout_s), done)) {
+        err = "timed out waiting for job " + std::to_string(job);
+        return -TILD_ERR_ARG;
+    }
+    if (J->state == TILD_JOB_FAILED) { err = J->err; return -TILD_ERR_DEVICE; }
+    return 0;
+}
+
+int Server::drain(double timeout_s, std::string& err) {
+    std::unique_lock<std::mutex> lk(mu_);
+    auto idle = [&] { return queue_.empty() && running_.empty(); };
+    if (timeout_s < 0) done_cv_.wait(lk, idle);
+    else if (!done_cv_.wait_for(lk, std::chrono::duration<double>(timeout_s), idle)) {
+        err = "timed out draining the server";
+        return -TILD_ERR_ARG;
+    }
+    return 0;
+}
+
+int Server::result(int job, tild_props* props, float* x, size_t n, double* latency, std::string& err) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = jobs_.find(job);
+    if (it == jobs_.end()) { err = "no such job"; return -TILD_ERR_ARG; }
+    Job* J = it->second.get();
+    if (J->state == TILD_JOB_FAILED) { err = J->err; return -TILD_ERR_DEVICE; }
+    if (J->state != TILD_JOB_DONE) { err = "job not finished"; return -TILD_ERR_ARG; }
+    if (props) *props = J->props;
+    if (x) {
+        if (n < J->x.size()) { err = "position buffer too small"; return -TILD_ERR_ARG; }
+        std::copy(J->x.begin(), J->x.end(), x);
+    }
+    if (latency) *latency = J->t_done - J->t_submit;
+    return 0;
+}
+
+static double quantile(std::vector<double> v, double q) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    // nearest-rank quantile
+    size_t k = (size_t)std::ceil(q * (double)v.size());
+    k = std::min(std::max<size_t>(k, 1), v.size());
+    return v[k - 1];
+}
+
+void Server::stats(tild_server_stats* s) {
+    std::lock_guard<std::mutex> lk(mu_);
+    *s = tild_server_stats{};
+    std::vector<double> lat;
+    for (auto& kv : jobs_) {
+        const Job* J = kv.second.get();
+        s->submitted += 1;
+        if (J->state == TILD_JOB_QUEUED) s->queued += 1;
+        else if (J->state == TILD_JOB_RUNNING) s->running += 1;
+        else if (J->state == TILD_JOB_DONE) { s->done += 1; lat.push_back(J->t_done - J->t_submit); }
+        else s->failed += 1;
+    }
+    s->bytes_in_use = bytes_in_use_;
+    s->peak_bytes = peak_bytes_;
+    s->budget = opts_.hbm_budget;
+    s->busy_s = busy_s_;
+    s->elapsed_s = now() - t0_;
+    s->bead_steps = bead_steps_;
+    s->p50_s = quantile(lat, 0.50);
+    s->p99_s = quantile(lat, 0.99);
+}
+
+}  // namespace tild

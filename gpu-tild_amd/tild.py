@@ -17,7 +17,7 @@ STATUS = {0: "OK", 1: "PARSE", 2: "CONFIG", 3: "OOM", 4: "DEVICE", 5: "NAN", 6: 
 
 # Every entry point include/tild.h declares (checked by the CPU tests).
 EXPORTS = [
-    "tild_system_load", "tild_system_free", "tild_last_error", "tild_system_info",
+    "tild_system_load", "tild_system_free", "tild_last_error", "tild_system_info", "tild_system_info_text",
     "tild_system_get_positions", "tild_system_get_types", "tild_system_get_molecules",
     "tild_system_get_charges", "tild_system_get_bonds", "tild_system_group_sites",
     "tild_engine_create", "tild_engine_destroy", "tild_engine_last_error",
@@ -25,7 +25,14 @@ EXPORTS = [
     "tild_engine_get_positions", "tild_engine_set_positions", "tild_engine_get_forces",
     "tild_engine_get_density", "tild_engine_get_grid_field", "tild_engine_get_step",
     "tild_engine_synchronize", "tild_engine_get_bins", "tild_engine_profile", "tild_fft_r2c", "tild_fft_c2r",
+    "tild_batch_create", "tild_batch_destroy", "tild_batch_last_error", "tild_batch_add", "tild_batch_remove",
+    "tild_batch_step", "tild_batch_properties", "tild_batch_get_positions", "tild_batch_set_positions",
+    "tild_batch_get_density", "tild_batch_replica_step", "tild_batch_profile",
+    "tild_server_create", "tild_server_destroy", "tild_server_submit", "tild_server_poll", "tild_server_wait",
+    "tild_server_drain", "tild_server_result", "tild_server_stats_get",
 ]
+
+JOB_STATES = {0: "queued", 1: "running", 2: "done", 3: "failed"}
 
 
 class TildError(RuntimeError):
@@ -54,6 +61,22 @@ class Props(ctypes.Structure):
 _lib = None
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("chunk_steps", ctypes.c_int), ("max_replicas", ctypes.c_int),
+                ("use_graph", ctypes.c_int), ("hbm_budget", ctypes.c_size_t), ("reserved", ctypes.c_int * 8)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int), ("queued", ctypes.c_int), ("running", ctypes.c_int),
+                ("done", ctypes.c_int), ("failed", ctypes.c_int), ("bytes_in_use", ctypes.c_size_t),
+                ("peak_bytes", ctypes.c_size_t), ("budget", ctypes.c_size_t), ("busy_s", ctypes.c_double),
+                ("elapsed_s", ctypes.c_double), ("bead_steps", ctypes.c_double), ("p50_s", ctypes.c_double),
+                ("p99_s", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -66,6 +89,7 @@ def lib():
         L.tild_system_load.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]
         L.tild_system_free.argtypes = [vp]
         L.tild_system_info.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]
+        L.tild_system_info_text.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, sz]
         L.tild_system_get_positions.argtypes = [vp, fp, sz]
         L.tild_system_get_types.argtypes = [vp, ip, sz]
         L.tild_system_get_molecules.argtypes = [vp, ip, sz]
@@ -88,6 +112,28 @@ def lib():
         L.tild_engine_synchronize.argtypes = [vp]
         L.tild_engine_get_bins.argtypes = [vp, ip, sz]
         L.tild_engine_profile.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L.tild_batch_create.argtypes = [ctypes.POINTER(Opts), ctypes.POINTER(vp)]
+        L.tild_batch_destroy.argtypes = [vp]
+        L.tild_batch_last_error.restype = ctypes.c_char_p
+        L.tild_batch_last_error.argtypes = [vp]
+        L.tild_batch_add.argtypes = [vp, vp, ctypes.c_uint32, ip]
+        L.tild_batch_remove.argtypes = [vp, ctypes.c_int]
+        L.tild_batch_step.argtypes = [vp, ip, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.tild_batch_properties.argtypes = [vp, ctypes.c_int, ctypes.POINTER(Props)]
+        L.tild_batch_get_positions.argtypes = [vp, ctypes.c_int, fp, sz]
+        L.tild_batch_set_positions.argtypes = [vp, ctypes.c_int, fp, sz]
+        L.tild_batch_get_density.argtypes = [vp, ctypes.c_int, ctypes.c_int, fp, sz]
+        L.tild_batch_replica_step.argtypes = [vp, ctypes.c_int, ip]
+        L.tild_batch_profile.argtypes = [vp, ip, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L.tild_server_create.argtypes = [ctypes.POINTER(ServerOpts), ctypes.POINTER(vp)]
+        L.tild_server_destroy.argtypes = [vp]
+        L.tild_server_submit.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint32, ctypes.c_int, ip]
+        L.tild_server_poll.argtypes = [vp, ctypes.c_int, ip]
+        L.tild_server_wait.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.tild_server_drain.argtypes = [vp, ctypes.c_double]
+        L.tild_server_result.argtypes = [vp, ctypes.c_int, ctypes.POINTER(Props), fp, sz,
+                                         ctypes.POINTER(ctypes.c_double)]
+        L.tild_server_stats_get.argtypes = [vp, ctypes.POINTER(ServerStats)]
         L.tild_fft_r2c.argtypes = [ctypes.c_int, ip, ctypes.c_int, vp, vp, vp]
         L.tild_fft_c2r.argtypes = [ctypes.c_int, ip, ctypes.c_int, vp, vp, vp]
         _lib = L
@@ -260,6 +306,144 @@ class Engine:
 
     def synchronize(self):
         self._chk(lib().tild_engine_synchronize(self._h), "synchronize")
+
+
+class Batch:
+    """Many device-resident replicas advanced together (tild_batch)."""
+
+    def __init__(self, device=0, spread_mode=0, tile_target=0):
+        o = Opts()
+        o.device = device
+        o.spread_mode = spread_mode
+        o.reserved[0] = tile_target
+        h = ctypes.c_void_p()
+        _check(lib().tild_batch_create(ctypes.byref(o), ctypes.byref(h)), "tild_batch_create")
+        self._h = h
+        self.meta = {}
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().tild_batch_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, st, where):
+        if st != 0:
+            raise TildError(st, f"{where}: {lib().tild_batch_last_error(self._h).decode()}")
+
+    def add(self, system, seed):
+        r = ctypes.c_int()
+        self._chk(lib().tild_batch_add(self._h, system._h, int(seed) & 0xFFFFFFFF, ctypes.byref(r)), "add")
+        self.meta[r.value] = (system.ns, system.dim, system.M, system.grid_shape)
+        return r.value
+
+    def remove(self, r):
+        self._chk(lib().tild_batch_remove(self._h, r), "remove")
+        self.meta.pop(r, None)
+
+    def step(self, n, replicas=None, graph=True):
+        ids = list(replicas) if replicas is not None else []
+        arr = (ctypes.c_int * max(1, len(ids)))(*ids)
+        self._chk(lib().tild_batch_step(self._h, arr if ids else None, len(ids), int(n), 1 if graph else 0), "step")
+
+    def properties(self, r):
+        p = Props()
+        self._chk(lib().tild_batch_properties(self._h, r, ctypes.byref(p)), "properties")
+        return p.as_dict()
+
+    def positions(self, r):
+        ns, dim = self.meta[r][:2]
+        a = np.zeros((ns, dim), dtype=np.float32)
+        self._chk(lib().tild_batch_get_positions(self._h, r, _fp(a), a.size), "get_positions")
+        return a
+
+    def set_positions(self, r, x):
+        ns, dim = self.meta[r][:2]
+        a = np.ascontiguousarray(x, dtype=np.float32).reshape(ns, dim)
+        self._chk(lib().tild_batch_set_positions(self._h, r, _fp(a), a.size), "set_positions")
+
+    def density(self, r, t):
+        M, shape = self.meta[r][2], self.meta[r][3]
+        a = np.zeros(M, dtype=np.float32)
+        self._chk(lib().tild_batch_get_density(self._h, r, t, _fp(a), a.size), "get_density")
+        return a.reshape(shape)
+
+    def replica_step(self, r):
+        s = ctypes.c_int()
+        self._chk(lib().tild_batch_replica_step(self._h, r, ctypes.byref(s)), "replica_step")
+        return s.value
+
+    def profile(self, nsteps, replicas=None):
+        ids = list(replicas) if replicas is not None else list(self.meta)
+        arr = (ctypes.c_int * max(1, len(ids)))(*ids)
+        a = np.zeros(8, dtype=np.float64)
+        self._chk(lib().tild_batch_profile(self._h, arr, len(ids), int(nsteps),
+                                           a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 8), "profile")
+        return dict(zip(Engine.PHASES, a.tolist()))
+
+
+class Server:
+    """Continuous-batching job server on one device (tild_server).
+
+    submit() parses the deck on the calling thread and queues the job; a worker
+    thread inside the library admits, steps and retires jobs.  The ctypes calls
+    release the GIL, so wait()/drain() block only the calling thread."""
+
+    def __init__(self, device=0, chunk_steps=100, max_replicas=0, use_graph=True, hbm_budget=0):
+        o = ServerOpts()
+        o.device, o.chunk_steps, o.max_replicas = device, chunk_steps, max_replicas
+        o.use_graph, o.hbm_budget = 1 if use_graph else 0, int(hbm_budget)
+        h = ctypes.c_void_p()
+        _check(lib().tild_server_create(ctypes.byref(o), ctypes.byref(h)), "tild_server_create")
+        self._h = h
+        self.meta = {}
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().tild_server_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, input_text, base_dir, seed, nsteps, ns=None, dim=None):
+        j = ctypes.c_int()
+        _check(lib().tild_server_submit(self._h, input_text.encode(), str(base_dir).encode(),
+                                        int(seed) & 0xFFFFFFFF, int(nsteps), ctypes.byref(j)), "submit")
+        self.meta[j.value] = (ns, dim)
+        return j.value
+
+    def poll(self, job):
+        s = ctypes.c_int()
+        _check(lib().tild_server_poll(self._h, job, ctypes.byref(s)), "poll")
+        return JOB_STATES[s.value]
+
+    def wait(self, job, timeout=-1.0):
+        _check(lib().tild_server_wait(self._h, job, float(timeout)), "wait")
+
+    def drain(self, timeout=-1.0):
+        _check(lib().tild_server_drain(self._h, float(timeout)), "drain")
+
+    def result(self, job, positions=True):
+        """(properties dict, positions (ns, dim) or None, seconds from submission to result)."""
+        p = Props()
+        lat = ctypes.c_double()
+        ns, dim = self.meta.get(job, (None, None))
+        x = None
+        if positions and ns:
+            x = np.zeros((ns, dim), dtype=np.float32)
+        _check(lib().tild_server_result(self._h, job, ctypes.byref(p), _fp(x) if x is not None else None,
+                                        x.size if x is not None else 0, ctypes.byref(lat)), "result")
+        return p.as_dict(), x, lat.value
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().tild_server_stats_get(self._h, ctypes.byref(st)), "stats")
+        return st.as_dict()
 
 
 def fft_r2c(dim, Nx, batch, d_in, d_out, stream=None):

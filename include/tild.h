@@ -52,11 +52,15 @@ void        tild_system_free(tild_system* sys);
 const char* tild_last_error(void);
 
 /* Scalar facts about a parsed deck.  Keys: "ns", "dim", "ntypes", "M", "Nx0", "Nx1",
-   "Nx2", "pmeorder", "n_styles", "do_charges", "n_bonds_total", "nbond_types",
-   "max_steps", "log_freq", "n_groups", "n_integrators", "seed", "L0", "L1", "L2",
-   "dx0", "dx1", "dx2", "gvol", "V", "delt", "noise_mag".  Mirrors the globals of
-   globals.h:47-80 after initialize() (initialize.cu:32-48). */
+   "Nx2", "pmeorder", "n_styles", "n_gaussians", "n_erfs", "n_gaussian_erfs",
+   "do_charges", "n_bonds_total", "nbond_types", "max_steps", "log_freq", "traj_freq",
+   "grid_freq", "binary_freq", "n_groups", "n_integrators", "seed", "L0", "L1", "L2",
+   "dx0", "dx1", "dx2", "gvol", "V", "delt", "noise_mag", "bjerrum", "smear".  Mirrors
+   the globals of globals.h:47-80 after initialize() (initialize.cu:32-48). */
 tild_status tild_system_info(const tild_system* sys, const char* key, double* value);
+/* Text-valued deck settings.  Keys: "dump_name" (traj_name, read_input.cu:504-512;
+   default traj.lammpstrj, :561).  Copies a NUL-terminated string of at most cap bytes. */
+tild_status tild_system_info_text(const tild_system* sys, const char* key, char* buf, size_t cap);
 
 /* Per-site arrays of a parsed deck; n is the element count of the buffer(s). */
 tild_status tild_system_get_positions(const tild_system* sys, float* x, size_t n);       /* ns*Dim */
@@ -136,6 +140,78 @@ tild_status tild_engine_get_bins(tild_engine* e, int* base, size_t n);
 tild_status tild_engine_profile(tild_engine* e, int nsteps, double* phase_ms, int nphase);
 tild_status tild_engine_get_step(tild_engine* e, int* step);
 tild_status tild_engine_synchronize(tild_engine* e);
+
+/* ---------------------------------------------------------------- replica batches */
+
+/* Many independent replicas (uneven boxes, grids, chain counts, pair styles) resident
+   on one device.  tild_batch_step advances a set of them together: every per-step
+   kernel is launched once for the whole set (blocks map to (replica, tile) work
+   items), and with use_graph the step is captured once per set in a hipGraph and
+   replayed.  The reference has no counterpart (one system per process,
+   main.cu:36-227); each replica follows the same per-step semantics as tild_engine. */
+typedef struct tild_batch tild_batch;
+tild_status tild_batch_create(const tild_opts* opts, tild_batch** out);
+void        tild_batch_destroy(tild_batch* b);
+const char* tild_batch_last_error(const tild_batch* b);
+tild_status tild_batch_add(tild_batch* b, const tild_system* sys, uint32_t seed, int* replica);
+tild_status tild_batch_remove(tild_batch* b, int replica);
+/* replicas == NULL/nrep == 0: every replica of the batch */
+tild_status tild_batch_step(tild_batch* b, const int* replicas, int nrep, int nsteps, int use_graph);
+tild_status tild_batch_properties(tild_batch* b, int replica, tild_props* out);
+tild_status tild_batch_get_positions(tild_batch* b, int replica, float* x, size_t n);
+tild_status tild_batch_set_positions(tild_batch* b, int replica, const float* x, size_t n);
+tild_status tild_batch_get_density(tild_batch* b, int replica, int type, float* rho, size_t n);
+tild_status tild_batch_replica_step(tild_batch* b, int replica, int* step);
+/* as tild_engine_profile, over a replica set */
+tild_status tild_batch_profile(tild_batch* b, const int* replicas, int nrep, int nsteps,
+                               double* phase_ms, int nphase);
+
+/* ---------------------------------------------------------------- job server */
+
+/* Continuous batching of independent simulation jobs on one device.  A job is a deck
+   (+ base directory for its data file), a Philox seed and a step count; the server
+   parses it on the submitting thread, admits queued jobs in arrival order while their
+   device memory fits the HBM budget, advances all running jobs together in chunks of
+   chunk_steps (each chunk one tild_batch_step of the running set, cut short at the
+   nearest job end), and retires finished jobs with their final properties and
+   positions, freeing their memory for the queue.  A job's trajectory does not depend
+   on what runs beside it.  The reference has no counterpart (one system per process,
+   main.cu:36-227). */
+typedef struct tild_server tild_server;
+typedef struct {
+    int device;
+    int chunk_steps;      /* steps between admission points; 0 = 100 */
+    int max_replicas;     /* running-job cap; 0 = budget only */
+    int use_graph;        /* replay one captured step per running set */
+    size_t hbm_budget;    /* bytes of device memory for jobs; 0 = 90% of free memory */
+    int reserved[8];
+} tild_server_opts;
+
+enum { TILD_JOB_QUEUED = 0, TILD_JOB_RUNNING = 1, TILD_JOB_DONE = 2, TILD_JOB_FAILED = 3 };
+
+typedef struct {
+    int submitted, queued, running, done, failed;
+    size_t bytes_in_use, peak_bytes, budget;
+    double busy_s;        /* wall time spent stepping */
+    double elapsed_s;     /* since tild_server_create */
+    double bead_steps;    /* sum over chunks of (beads running) x (steps) */
+    double p50_s, p99_s;  /* time from submission to result of finished jobs */
+} tild_server_stats;
+
+tild_status tild_server_create(const tild_server_opts* opts, tild_server** out);
+/* Stops the worker after its current chunk; unfinished jobs are dropped. */
+void        tild_server_destroy(tild_server* s);
+/* TILD_ERR_BUDGET: the job alone exceeds the budget.  Parse errors as tild_system_load. */
+tild_status tild_server_submit(tild_server* s, const char* input_text, const char* base_dir,
+                               uint32_t seed, int nsteps, int* job);
+tild_status tild_server_poll(tild_server* s, int job, int* state);
+/* timeout_s < 0: no limit.  A failed job returns its error. */
+tild_status tild_server_wait(tild_server* s, int job, double timeout_s);
+tild_status tild_server_drain(tild_server* s, double timeout_s);
+/* Final properties, positions (ns*Dim, may be NULL) and time-to-result of a done job. */
+tild_status tild_server_result(tild_server* s, int job, tild_props* props, float* x, size_t n,
+                               double* latency_s);
+tild_status tild_server_stats_get(tild_server* s, tild_server_stats* out);
 
 /* ---------------------------------------------------------------- building blocks */
 

@@ -125,3 +125,12 @@ def test_library_exports_every_declared_entry_point():
 def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run(["python", str(REPO / "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode != 0 and "one GPU" in r.stderr
+
+
+def test_tild_run_reports_bad_input(tmp_path):
+    exe = pathlib.Path(tild.__file__).resolve().parent / "bin" / "tild-run"
+    (tmp_path / "input").write_text("Dim 3\nNx 10\nfrobnicate 3\n")
+    r = subprocess.run([str(exe), "-in", "input"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "Invalid keyword: frobnicate" in r.stderr
+    r = subprocess.run([str(exe)], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr

@@ -187,6 +187,64 @@ def test_two_particle_gaussian_force_matches_analytic(tmp_path):
     assert errs[0] / errs[1] > 3.0, errs     # halving dx cuts the error ~4x
 
 
+def _radial_gauss_conv(f, r, sigma, smax):
+    """(G_sigma * f)(r) for an isotropic f supported on [0, smax] (3D radial convolution)."""
+    from scipy import integrate
+
+    def g(s):
+        return s * f(s) * (math.exp(-(r - s) ** 2 / (2 * sigma ** 2)) - math.exp(-(r + s) ** 2 / (2 * sigma ** 2)))
+    v, _ = integrate.quad(g, 0.0, smax, limit=200)
+    return v / (r * sigma * math.sqrt(2 * math.pi))
+
+
+@pytest.mark.parametrize("kind", ["gaussian_erf", "erf"])
+def test_two_particle_erf_energies_match_real_space(kind, tmp_path):
+    # gaussian_erf: u(k) = A exp(-k^2 s^2/2) h(k), erf: A exp(-k^2 s^2/2) h(k)^2 with h the
+    # transform of a sphere of radius Rp (pair_style_gaussian_erf.cu:82-115,
+    # pair_style_erf.cu:122-157).  In real space that is A times a Gaussian smoothing
+    # of the sphere indicator, resp. of the two-sphere overlap volume
+    # pi (4R + d)(2R - d)^2 / 12, evaluated here by radial quadrature.
+    A, R, sig = 1.5, 1.0, 0.7
+    if kind == "erf":
+        f, smax = (lambda s: math.pi / 12 * (4 * R + s) * (2 * R - s) ** 2 if s < 2 * R else 0.0), 2 * R
+    else:
+        f, smax = (lambda s: 1.0 if s < R else 0.0), R
+    errs = []
+    for n in (40, 80):
+        text, d = _sys_without_bonds(tmp_path / str(n), [(kind, 1, 2, A, R, sig)], grid=(n, n, n))
+        o = O.System(text, d, seed=1)
+        o.forces()
+        x = o.positions()
+        ua = A * _radial_gauss_conv(f, float(np.linalg.norm(x[0] - x[1])), sig, smax)
+        errs.append(abs(o.properties()["U_styles"][0] / ua - 1.0))
+    assert errs[1] < 0.03 and errs[0] / errs[1] > 2.0, errs
+
+
+def test_two_charge_force_matches_smeared_coulomb(tmp_path):
+    # The force path smears twice (device_array_utils.cu:213, 237-238): the field is that
+    # of 4 pi lB / k^2 exp(-k^2/a^2), i.e. lB erf(a r / 2) / r in real space.
+    lB, a = 1.0, 2.0
+    errs = []
+    for n in (40, 80):
+        dd = tmp_path / str(n)
+        dd.mkdir()
+        pos = np.array([[10.05, 10.13, 9.93], [11.27, 10.41, 9.66]])
+        tild_gen.write_data(dd / "input.data", pos, np.array([1, 2]), np.arange(1, 3), [], (20.0, 20.0, 20.0),
+                            2, 3, charges=np.array([1.0, -1.0]))
+        text = tild_gen.deck("input.data", 3, (n, n, n), [("gaussian", 1, 2, 0.0, 1.0)], charges=(lB, a))
+        o = O.System(text.replace("bond 1 harmonic 1.5 0.0\n", ""), dd, seed=1)
+        f = o.forces()
+        x = o.positions()
+        rv = x[0] - x[1]
+        r = float(np.linalg.norm(rv))
+        b = a / 2
+        dV = -lB * (2 * b / math.sqrt(math.pi) * math.exp(-b * b * r * r) / r - math.erf(b * r) / r ** 2)
+        fa = -dV * rv / r
+        errs.append(np.linalg.norm(f[0] - fa) / np.linalg.norm(fa))
+        assert np.linalg.norm(f[0] + f[1]) < 1e-9 * np.linalg.norm(fa)
+    assert errs[1] < 0.05 and errs[0] / errs[1] > 2.5, errs
+
+
 def test_force_is_minus_energy_gradient(tmp_path):
     # ik-differentiated forces are not the exact gradient of the gridded energy; the
     # mismatch vanishes as dx -> 0 (checked at two resolutions)
