@@ -63,6 +63,7 @@ tild_status tild_system_info(const tild_system* s, const char* key, double* v) {
     else if (k == "n_gaussians") *v = (double)S.Gausses.size();
     else if (k == "n_erfs") *v = (double)S.Erfs.size();
     else if (k == "n_gaussian_erfs") *v = (double)S.GaussErfs.size();
+    else if (k == "n_fieldphases") *v = (double)S.Fields.size();
     else if (k == "do_charges") *v = S.charges.enabled ? 1 : 0;
     else if (k == "n_bonds_total") *v = S.n_total_bonds;
     else if (k == "nbond_types") *v = S.nbond_types;
@@ -145,7 +146,9 @@ tild_status tild_engine_create(const tild_system* s, const tild_opts* opts, tild
     if (opts) o = *opts;
     auto e = std::make_unique<tild_engine>();
     e->batch = std::make_unique<tild::Batch>(o.device);
-    e->batch->set_spread(o.spread_mode, o.reserved[0] > 0 ? o.reserved[0] : 1024, o.reserved[1], o.reserved[2]);
+    e->batch->set_spread(o.spread_mode, o.tile_target > 0 ? o.tile_target : 512, o.tile_by, o.tile_bz);
+    e->batch->set_reorder(o.reorder_every == 0 ? -1 : (o.reorder_every < 0 ? 0 : o.reorder_every));
+    e->batch->set_fft_tuning({o.fft_mid_elems, o.fft_last_lines, o.fft_c2r_rows});
     std::string err;
     int rc = e->batch->init(err);
     if (rc < 0) return from_rc(rc, err, nullptr);
@@ -172,7 +175,8 @@ tild_status tild_engine_forces(tild_engine* e) {
 tild_status tild_engine_step(tild_engine* e, int n) {
     if (!e || n < 0) return fail(TILD_ERR_ARG, "bad argument");
     std::string err;
-    int rc = e->batch->step({e->id}, n, err);
+    // several steps replay one captured step graph (same kernels, same results)
+    int rc = e->batch->step({e->id}, n, err, n >= 2);
     if (rc >= 0) rc = e->batch->sync(err);
     return from_rc(rc, err, e);
 }
@@ -215,7 +219,9 @@ tild_status tild_batch_create(const tild_opts* opts, tild_batch** out) {
     if (opts) o = *opts;
     auto b = std::make_unique<tild_batch>();
     b->batch = std::make_unique<tild::Batch>(o.device);
-    b->batch->set_spread(o.spread_mode, o.reserved[0] > 0 ? o.reserved[0] : 1024, o.reserved[1], o.reserved[2]);
+    b->batch->set_spread(o.spread_mode, o.tile_target > 0 ? o.tile_target : 512, o.tile_by, o.tile_bz);
+    b->batch->set_reorder(o.reorder_every == 0 ? -1 : (o.reorder_every < 0 ? 0 : o.reorder_every));
+    b->batch->set_fft_tuning({o.fft_mid_elems, o.fft_last_lines, o.fft_c2r_rows});
     std::string err;
     int rc = b->batch->init(err);
     if (rc < 0) return from_rc(rc, err, nullptr);
@@ -276,6 +282,14 @@ tild_status tild_batch_profile(tild_batch* b, const int* replicas, int nrep, int
     if (!b || nrep < 0 || (nrep > 0 && !replicas)) return fail(TILD_ERR_ARG, "bad argument");
     std::vector<int> ids(replicas, replicas + nrep);
     BATCH_CALL(profile(ids, nsteps, ms, nphase, err))
+}
+
+tild_status tild_engine_nlist(tild_engine* e, float rc, const int* sites, int nsites, int* cell_of, int* counts,
+                              int* neigh, size_t cap, int* cell_max, int* ncells) {
+    if (!e || nsites < 0 || (nsites > 0 && !sites)) return fail(TILD_ERR_ARG, "bad argument");
+    std::string err;
+    int rc_ = e->batch->nlist(e->id, rc, sites, nsites, cell_of, counts, neigh, cap, cell_max, ncells, err);
+    return from_rc(rc_, err, e);
 }
 
 tild_status tild_engine_get_step(tild_engine* e, int* step) {

@@ -41,6 +41,7 @@ struct RepDev {
     int nl_mid;    // batch columns per block, middle (y) pass
     int nl_last;   // batch columns per block, fused last-axis pass
     int nr_c2r;    // rows per block, inverse x pass
+    int mid_elems; // complex values per block of the middle-axis passes (0: kFftBudget)
     // particle state
     float* X[2];            // ns*dim, double-buffered by step parity
     float* XO;              // ns*dim, GJF previous position
@@ -55,6 +56,15 @@ struct RepDev {
     float gjf_b[kMaxTypes], gjf_a[kMaxTypes], gjf_c1[kMaxTypes], gjf_c2[kMaxTypes],
           gjf_noise[kMaxTypes];
     float* fout;            // ns*dim (optional)
+    int* pid;               // ns: original site id of each slot (noise key, host order)
+    // spare buffers of the periodic spatial reorder, swapped with the live arrays
+    float* XO_alt;
+    int* tp_alt;
+    float* q_alt;
+    int* bond_j_alt;
+    int* bond_t_alt;
+    signed char* ikind_alt;
+    int* pid_alt;
     // grids
     unsigned long long* acc;   // nf*M fixed-point accumulators
     float* rho;                // nf*M densities (x fastest)
@@ -62,6 +72,7 @@ struct RepDev {
     float2* Gk;                // nf*dim*Mh half spectra of the gathered fields
     float4* G4;                // nf*M (g_x, g_y, g_z, rho) records for the gather
     const float* Ctab;         // (T(T+1)/2)*Mh coupling tables (include 1/M), + charge table
+    const float* FPtab;        // T*dim*M net static field-phase forces per type (or null)
     int ctab_charge;           // row of the charge kernel in Ctab (or -1)
     // property tables
     int nstyles, npairs, nq;

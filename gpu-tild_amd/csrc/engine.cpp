@@ -21,7 +21,7 @@ namespace tild {
 int make_axis_plan(int N, AxisPlan& pl, std::vector<float2>& tw, std::string& err) {
     std::memset(&pl, 0, sizeof(pl));
     pl.N = N;
-    if (N < 1 || N > kFftBudget) { err = "FFT length " + std::to_string(N) + " outside [1, 4096]"; return -TILD_ERR_UNSUPPORTED; }
+    if (N < 1 || N > kFftBudget) { err = "FFT length " + std::to_string(N) + " outside [1, " + std::to_string(kFftBudget) + "]"; return -TILD_ERR_UNSUPPORTED; }
     int n = N, e2 = 0;
     while (n % 2 == 0) { n /= 2; ++e2; }
     std::vector<int> rad;
@@ -67,6 +67,9 @@ bool choose_tiles(int dim, const int* Nx, int P, int nf, int target_tiles, int w
     for (int by = P + 1; by <= Ny / 2; ++by) {
         if (Ny % by) continue;
         if (want_by > 0 && by != want_by) continue;
+        // even By keeps the x-transform row pairs (y, y+1) of the global-atomic path, so
+        // the tile shape does not touch a single bit of the result
+        if (want_by <= 0 && Ny % 2 == 0 && by % 2) continue;
         for (int bz = (dim == 3 ? P + 1 : 1); bz <= (dim == 3 ? Nz / 2 : 1); ++bz) {
             if (Nz % bz) continue;
             if (want_bz > 0 && dim == 3 && bz != want_bz) continue;
@@ -138,9 +141,11 @@ size_t Batch::estimate_bytes(const System& S) {
     size_t nf = S.ntypes + (S.charges.enabled ? 1 : 0);
     size_t T = S.ntypes, nst = S.styles().size();
     // X[2], XO, fout; type, charge, bond tables, integrator kind; sorted-bin keys and records
-    size_t part = ns * (4 * D * 4 + 4 + 4 + 24 + 1) + ns * (8 + 16 + 4);
+    // site ids; spare arrays, keys and sort scratch of the spatial reorder
+    size_t part = ns * (4 * D * 4 + 4 + 4 + 24 + 1) + ns * (8 + 16 + 4) + ns * 4 + ns * (4 * D + 100);
     size_t grids = nf * M * (8 + 4 + 16) + nf * Mh * 8 * (1 + D);
     size_t tabs = Mh * 4 * (T * (T + 1) / 2 + 1 + nst + 1) + Mh * 8 * (D == 3 ? 6 : 3) * nst;
+    if (!S.Fields.empty()) tabs += T * D * M * 4;
     return part + grids + tabs + (1 << 20);
 }
 
@@ -198,12 +203,18 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
     h.nl_x = std::max(1, std::min(kFftBudget / h.Nx, npairs_x));
     const int la = D - 1, Nla = la == 2 ? h.Nz : h.Ny, Q = la == 2 ? h.Ny * h.Nxh : h.Nxh;
     if (Nla * h.nf > kFftBudget || 3 * fft_buf_bytes(Nla, h.nf) > kMaxLds) { err = "grid too large for the fused k-space pass"; return -TILD_ERR_UNSUPPORTED; }
-    int nl = std::max(1, kFftBudget / (Nla * h.nf));
+    // about 1,280 complex values per field set per block measured best on the
+    // 80x80x160 melt (fewer, fuller blocks lose to the per-block latency chain)
+    int nl = std::max(1, std::min(kFftBudget, 1280) / (Nla * h.nf));
     while (nl > 1 && 3 * fft_buf_bytes(Nla, h.nf * nl) > kMaxLds) --nl;
     h.nl_last = std::min(nl, Q);
     const int nc = (h.nf * D + 1) / 2;
     if (nc * h.Nx > kFftBudget) { err = "grid too large for the inverse x pass"; return -TILD_ERR_UNSUPPORTED; }
     h.nr_c2r = std::max(1, std::min(kFftBudget / (nc * h.Nx), rows));
+    // optional block geometry (batching of independent lines: results do not change)
+    if (tune_.last_lines > 0) h.nl_last = std::min(h.nl_last, tune_.last_lines);
+    if (tune_.c2r_rows > 0) h.nr_c2r = std::min(h.nr_c2r, tune_.c2r_rows);
+    if (tune_.mid_elems > 0) h.mid_elems = std::min(kFftBudget, tune_.mid_elems);
     h.nl_mid = mid_geom(h, 1, h.nf).nl;
 
     // particle state
@@ -270,6 +281,12 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
     }
     if (int rc = dalloc(R, &R.d_fout, nsD, err, true)) return rc;
     h.fout = R.d_fout;
+    {
+        std::vector<int> ident(S.ns);
+        for (int i = 0; i < S.ns; ++i) ident[i] = i;
+        if (int rc = dalloc(R, &h.pid, S.ns, err)) return rc;
+        HIPCHK(hipMemcpy(h.pid, ident.data(), S.ns * 4, hipMemcpyHostToDevice));
+    }
     h.seed = seed;
     h.step = 0;
 
@@ -305,6 +322,48 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
         }
     }
 
+    // static field-phase forces, summed per type: +f on type1, -f on type2
+    if (!S.Fields.empty()) {
+        std::vector<float> fp((size_t)S.ntypes * D * M, 0.0f), fr;
+        for (const FieldPhase& F : S.Fields) {
+            fieldphase_force_table(S, F, fr);
+            for (int d = 0; d < D; ++d)
+                for (size_t i = 0; i < M; ++i) {
+                    fp[((size_t)F.type1 * D + d) * M + i] += fr[i * D + d];
+                    fp[((size_t)F.type2 * D + d) * M + i] -= fr[i * D + d];
+                }
+        }
+        float* d_fp;
+        if (int rc = dalloc(R, &d_fp, fp.size(), err)) return rc;
+        HIPCHK(hipMemcpy(d_fp, fp.data(), fp.size() * 4, hipMemcpyHostToDevice));
+        h.FPtab = d_fp;
+    }
+
+    // spatial reorder: worthwhile once the gather records no longer fit one XCD's L2
+    R.reorder_every = reorder_every_ >= 0 ? reorder_every_
+                                          : ((size_t)h.nf * M * sizeof(float4) > ((size_t)8 << 20) ? 20 : 0);
+    if (R.reorder_every > 0) {
+        const uint32_t maxkey = (uint32_t)(h.Ny * h.Nz - 1);
+        R.sort_bits = 1;
+        while (R.sort_bits < 32 && (maxkey >> R.sort_bits)) ++R.sort_bits;
+        if (int rc = dalloc(R, &h.XO_alt, nsD, err)) return rc;
+        if (int rc = dalloc(R, &h.tp_alt, S.ns, err)) return rc;
+        if (S.charges.enabled) { if (int rc = dalloc(R, &h.q_alt, S.ns, err)) return rc; }
+        if (int rc = dalloc(R, &h.bond_j_alt, (size_t)S.ns * 3, err)) return rc;
+        if (int rc = dalloc(R, &h.bond_t_alt, (size_t)S.ns * 3, err)) return rc;
+        if (int rc = dalloc(R, &h.ikind_alt, S.ns, err)) return rc;
+        if (int rc = dalloc(R, &h.pid_alt, S.ns, err)) return rc;
+        for (int k = 0; k < 2; ++k) {
+            if (int rc = dalloc(R, &R.rb.keys[k], S.ns, err)) return rc;
+            if (int rc = dalloc(R, &R.rb.vals[k], S.ns, err)) return rc;
+        }
+        R.rb.tmp_bytes = reorder_tmp_bytes(S.ns, R.sort_bits);
+        if (!R.rb.tmp_bytes) { err = "radix sort sizing failed"; return -TILD_ERR_DEVICE; }
+        unsigned char* tmp;
+        if (int rc = dalloc(R, &tmp, R.rb.tmp_bytes, err)) return rc;
+        R.rb.tmp = tmp;
+    }
+
     int id = -1;
     for (int k = 0; k < (int)reps_.size(); ++k) if (!reps_[k]) { id = k; break; }
     if (id < 0) { id = (int)reps_.size(); reps_.emplace_back(); }
@@ -314,8 +373,36 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
     if (int rc = build_maps({id}, RR.maps, err)) { remove(id); return rc; }
     if (int rc = build_tables(RR, S, err)) { remove(id); return rc; }
     if (int rc = upload_desc(id, err)) { remove(id); return rc; }
+    if (RR.reorder_every > 0) {
+        if (int rc = reorder(id, err)) { remove(id); return rc; }
+    }
     HIPCHK(hipStreamSynchronize(stream_));
     return id;
+}
+
+int Batch::reorder(int id, std::string& err) {
+    Replica& R = *reps_[id];
+    HIPCHK(launch_reorder(stream_, d_reps_, id, R.ns, R.sort_bits, R.rb));
+    // mirror the device-side swaps in the host copy of the descriptor
+    RepDev& h = R.h;
+    std::swap(h.X[0], h.X[1]);
+    std::swap(h.XO, h.XO_alt);
+    { const int* t = h.tp; h.tp = h.tp_alt; h.tp_alt = const_cast<int*>(t); }
+    { const signed char* t = h.ikind; h.ikind = h.ikind_alt; h.ikind_alt = const_cast<signed char*>(t); }
+    std::swap(h.pid, h.pid_alt);
+    if (h.q) { const float* t = h.q; h.q = h.q_alt; h.q_alt = const_cast<float*>(t); }
+    { const int* t = h.bond_j; h.bond_j = h.bond_j_alt; h.bond_j_alt = const_cast<int*>(t); }
+    { const int* t = h.bond_t; h.bond_t = h.bond_t_alt; h.bond_t_alt = const_cast<int*>(t); }
+    R.permuted = true;
+    R.since_sort = 0;
+    return 0;
+}
+
+int Batch::fetch_pid(const Replica& R, std::vector<int>& pid, std::string& err) {
+    pid.resize(R.ns);
+    HIPCHK(hipMemcpyAsync(pid.data(), R.h.pid, (size_t)R.ns * 4, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return 0;
 }
 
 void Batch::remove(int id) {
@@ -397,6 +484,20 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
         const int nc = (h.nf * D + 1) / 2;
         add(PH_C2R, (rows + h.nr_c2r - 1) / h.nr_c2r, lds_bytes(h.Nx, h.nr_c2r * nc));
         add(PH_KPROPS, h.nchunk > 0 ? h.nchunk : 1, 0);
+    }
+    // Workgroups go to the 8 XCDs round-robin (launch block w runs on XCD w % 8).  The
+    // particle-order passes hand each XCD a contiguous run of work items, i.e. one slab
+    // of a spatially ordered replica, so that its L2 holds that slab's grid records.
+    for (Phase ph : {PH_PART, PH_TPART, PH_SPREADA, PH_TILE}) {
+        std::vector<WorkItem>& v = maps[ph].host;
+        const size_t n = v.size(), q = n / kXcds, rm = n % kXcds;
+        if (n < 2 * (size_t)kXcds) continue;
+        std::vector<WorkItem> o(n);
+        for (size_t w = 0; w < n; ++w) {
+            const size_t x = w % kXcds, k = w / kXcds;
+            o[w] = v[x * q + std::min(x, rm) + k];
+        }
+        v.swap(o);
     }
     for (int p = 0; p < PH_COUNT; ++p) {
         if (maps[p].dev) { (void)hipFree(maps[p].dev); maps[p].dev = nullptr; }
@@ -667,12 +768,33 @@ int Batch::step(const std::vector<int>& ids_in, int n, std::string& err, bool us
         (void)hipGraphDestroy(g);
         HIPCHK(ei);
     }
-    for (int s = 0; s < n; ++s) {
-        if (use_graph) HIPCHK(hipGraphLaunch(m->graph, stream_));
-        else if (int rc = enqueue_step(*m, true, false, err)) return rc;
-        for (int id : ids) reps_[id]->step += 1;
+    // chunks end where a replica of the set is due for its spatial reorder; each
+    // replica reorders at multiples of its own interval, whatever set it runs in
+    int done = 0;
+    while (done < n) {
+        int chunk = n - done;
+        for (int id : ids) {
+            const Replica& R = *reps_[id];
+            if (R.reorder_every > 0) chunk = std::min(chunk, std::max(1, R.reorder_every - R.since_sort));
+        }
+        if (int rc = run_steps(*m, ids, chunk, use_graph, err)) return rc;
+        done += chunk;
+        for (int id : ids) {
+            Replica& R = *reps_[id];
+            if (R.reorder_every > 0 && (R.since_sort += chunk) >= R.reorder_every)
+                if (int rc = reorder(id, err)) return rc;
+        }
     }
     for (int id : ids) reps_[id]->h.step = reps_[id]->step;
+    return 0;
+}
+
+int Batch::run_steps(SetMaps& m, const std::vector<int>& ids, int n, bool use_graph, std::string& err) {
+    for (int s = 0; s < n; ++s) {
+        if (use_graph) HIPCHK(hipGraphLaunch(m.graph, stream_));
+        else if (int rc = enqueue_step(m, true, false, err)) return rc;
+        for (int id : ids) reps_[id]->step += 1;
+    }
     return 0;
 }
 
@@ -710,6 +832,11 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
         if (e0 == hipSuccess) e0 = launch_particles(ctx(PH_PART), true, 0);
         rec(7);
         if (e0 == hipSuccess) e0 = launch_advance(stream_, d_reps_, m->d_ids, m->nids);
+        for (int id : ids) {   // amortised spatial reorders are timed with the step counter
+            Replica& R = *reps_[id];
+            if (e0 == hipSuccess && R.reorder_every > 0 && ++R.since_sort >= R.reorder_every && reorder(id, err))
+                e0 = hipErrorUnknown;
+        }
         rec(8);
         if (e0 == hipSuccess) e0 = hipEventSynchronize(ev[kProfPhases]);
         if (e0 != hipSuccess) { err = std::string("profile: ") + hipGetErrorString(e0); rc = -TILD_ERR_DEVICE; break; }
@@ -774,18 +901,44 @@ int Batch::properties(int id, tild_props* out, std::string& err) {
 }
 
 // ---------------------------------------------------------------- host copies
+// out[site] = in[slot] for slot p holding site pid[p] (identity when pid is empty).
+template <class T>
+static void unpermute(const T* in, T* out, size_t n, const std::vector<int>& pid, int D) {
+    if (pid.empty()) { std::copy(in, in + n, out); return; }
+    for (size_t p = 0; p < pid.size(); ++p)
+        for (int d = 0; d < D; ++d) out[(size_t)pid[p] * D + d] = in[p * D + d];
+}
 int Batch::get_positions(int id, float* x, size_t n, std::string& err) {
     Replica* R = rep(id);
     if (!R || !x || n != (size_t)R->ns * R->dim) { err = "get_positions: size mismatch"; return -TILD_ERR_ARG; }
-    HIPCHK(hipMemcpyAsync(x, R->h.X[R->step & 1], n * 4, hipMemcpyDeviceToHost, stream_));
-    HIPCHK(hipStreamSynchronize(stream_));
+    if (!R->permuted) {
+        HIPCHK(hipMemcpyAsync(x, R->h.X[R->step & 1], n * 4, hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        return 0;
+    }
+    std::vector<float> tmp(n);
+    std::vector<int> pid;
+    HIPCHK(hipMemcpyAsync(tmp.data(), R->h.X[R->step & 1], n * 4, hipMemcpyDeviceToHost, stream_));
+    if (int rc = fetch_pid(*R, pid, err)) return rc;
+    unpermute(tmp.data(), x, n, pid, R->dim);
     return 0;
 }
 
 int Batch::set_positions(int id, const float* x, size_t n, std::string& err) {
     Replica* R = rep(id);
     if (!R || !x || n != (size_t)R->ns * R->dim) { err = "set_positions: size mismatch"; return -TILD_ERR_ARG; }
-    HIPCHK(hipMemcpyAsync(R->h.X[R->step & 1], x, n * 4, hipMemcpyHostToDevice, stream_));
+    std::vector<float> tmp;
+    const float* src = x;
+    if (R->permuted) {   // slot p holds site pid[p]
+        std::vector<int> pid;
+        if (int rc = fetch_pid(*R, pid, err)) return rc;
+        tmp.resize(n);
+        const int D = R->dim;
+        for (int p = 0; p < R->ns; ++p)
+            for (int d = 0; d < D; ++d) tmp[(size_t)p * D + d] = x[(size_t)pid[p] * D + d];
+        src = tmp.data();
+    }
+    HIPCHK(hipMemcpyAsync(R->h.X[R->step & 1], src, n * 4, hipMemcpyHostToDevice, stream_));
     HIPCHK(hipMemsetAsync(R->h.XO, 0, n * 4, stream_));   // xo = x
     HIPCHK(hipStreamSynchronize(stream_));
     return 0;
@@ -794,8 +947,16 @@ int Batch::set_positions(int id, const float* x, size_t n, std::string& err) {
 int Batch::get_forces(int id, float* f, size_t n, std::string& err) {
     Replica* R = rep(id);
     if (!R || !f || n != (size_t)R->ns * R->dim) { err = "get_forces: size mismatch"; return -TILD_ERR_ARG; }
-    HIPCHK(hipMemcpyAsync(f, R->d_fout, n * 4, hipMemcpyDeviceToHost, stream_));
-    HIPCHK(hipStreamSynchronize(stream_));
+    if (!R->permuted) {
+        HIPCHK(hipMemcpyAsync(f, R->d_fout, n * 4, hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        return 0;
+    }
+    std::vector<float> tmp(n);
+    std::vector<int> pid;
+    HIPCHK(hipMemcpyAsync(tmp.data(), R->d_fout, n * 4, hipMemcpyDeviceToHost, stream_));
+    if (int rc = fetch_pid(*R, pid, err)) return rc;
+    unpermute(tmp.data(), f, n, pid, R->dim);
     return 0;
 }
 
@@ -825,11 +986,91 @@ int Batch::get_bins(int id, int* out, size_t n, std::string& err) {
     if (!R || !out || n != (size_t)R->ns * R->dim) { err = "get_bins: size mismatch"; return -TILD_ERR_ARG; }
     int* d;
     HIPCHK(hipMalloc((void**)&d, n * sizeof(int)));
+    std::vector<int> tmp(n);
     hipError_t e1 = launch_bins(stream_, d_reps_, id, R->ns, d);
-    hipError_t e2 = hipMemcpyAsync(out, d, n * sizeof(int), hipMemcpyDeviceToHost, stream_);
+    hipError_t e2 = hipMemcpyAsync(tmp.data(), d, n * sizeof(int), hipMemcpyDeviceToHost, stream_);
     hipError_t e3 = hipStreamSynchronize(stream_);
     (void)hipFree(d);
     HIPCHK(e1); HIPCHK(e2); HIPCHK(e3);
+    std::vector<int> pid;
+    if (R->permuted) { if (int rc = fetch_pid(*R, pid, err)) return rc; }
+    unpermute(tmp.data(), out, n, pid, R->dim);
+    return 0;
+}
+
+int Batch::nlist(int id, float rc, const int* sites, int nsites, int* cell_of, int* counts, int* neigh,
+                 size_t cap, int* cell_max, int* ncells, std::string& err) {
+    Replica* R = rep(id);
+    if (!R || !(rc > 0.0f)) { err = "nlist: bad replica or cut-off"; return -TILD_ERR_ARG; }
+    const int D = R->dim, ns = R->ns;
+    // cell geometry in FP32 exactly as NList::Initialize (nlist.cu:36-51)
+    NlistArgs a{};
+    int total = 1;
+    float cell_vol = 1.0f, V = 1.0f;
+    for (int j = 0; j < D; ++j) V *= R->L[j];
+    for (int j = 0; j < 3; ++j) { a.nc[j] = 1; a.Lc[j] = 1.0f; }
+    for (int j = 0; j < D; ++j) {
+        a.nc[j] = (int)(R->L[j] / rc);
+        if (a.nc[j] < 1) { err = "nlist: cut-off longer than the box"; return -TILD_ERR_ARG; }
+        a.Lc[j] = R->L[j] / (float)a.nc[j];
+        total *= a.nc[j];
+        cell_vol *= a.Lc[j];
+    }
+    a.cmax = (int)((float)((float)ns * cell_vol / V) * 7.5);
+    a.rc = rc;
+    if (cell_max) *cell_max = a.cmax;
+    if (ncells) *ncells = total;
+    if (!neigh) return 0;   // size query
+    if (!cell_of || !counts || cap < (size_t)std::max(a.cmax, 1)) { err = "nlist: output buffers too small"; return -TILD_ERR_ARG; }
+    if (a.cmax < 1) { err = "nlist: cell capacity below one site"; return -TILD_ERR_CONFIG; }
+    a.cap = a.cmax;
+    std::vector<unsigned char> mem(ns, sites ? 0 : 1);
+    if (sites)
+        for (int k = 0; k < nsites; ++k) {
+            if (sites[k] < 0 || sites[k] >= ns) { err = "nlist: site id out of range"; return -TILD_ERR_ARG; }
+            mem[sites[k]] = 1;
+        }
+    HIPCHK(hipSetDevice(device_));
+    unsigned char* d_mem = nullptr;
+    int *d_cell_of = nullptr, *d_counts = nullptr, *d_neigh = nullptr, *d_cc = nullptr, *d_cs = nullptr, *d_ovf = nullptr;
+    auto cleanup = [&]() {
+        for (void* q : {(void*)d_mem, (void*)d_cell_of, (void*)d_counts, (void*)d_neigh, (void*)d_cc, (void*)d_cs, (void*)d_ovf})
+            if (q) (void)hipFree(q);
+    };
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t x) { if (e == hipSuccess && x != hipSuccess) e = x; };
+    chk(hipMalloc((void**)&d_mem, ns));
+    chk(hipMalloc((void**)&d_cell_of, (size_t)ns * 4));
+    chk(hipMalloc((void**)&d_counts, (size_t)ns * 4));
+    chk(hipMalloc((void**)&d_neigh, (size_t)ns * a.cap * 4));
+    chk(hipMalloc((void**)&d_cc, (size_t)total * 4));
+    chk(hipMalloc((void**)&d_cs, (size_t)total * a.cmax * 4));
+    chk(hipMalloc((void**)&d_ovf, 4));
+    if (e == hipSuccess) {
+        chk(hipMemcpyAsync(d_mem, mem.data(), ns, hipMemcpyHostToDevice, stream_));
+        chk(hipMemsetAsync(d_cc, 0, (size_t)total * 4, stream_));
+        chk(hipMemsetAsync(d_ovf, 0, 4, stream_));
+        a.member = d_mem; a.cell_of = d_cell_of; a.counts = d_counts; a.neigh = d_neigh;
+        a.cell_count = d_cc; a.cell_slots = d_cs; a.overflow = d_ovf;
+        chk(launch_nlist(stream_, d_reps_, id, ns, a));
+    }
+    int ovf = 0;
+    std::vector<int> nb;
+    if (e == hipSuccess) {
+        nb.resize((size_t)ns * a.cap);
+        chk(hipMemcpyAsync(&ovf, d_ovf, 4, hipMemcpyDeviceToHost, stream_));
+        chk(hipMemcpyAsync(cell_of, d_cell_of, (size_t)ns * 4, hipMemcpyDeviceToHost, stream_));
+        chk(hipMemcpyAsync(counts, d_counts, (size_t)ns * 4, hipMemcpyDeviceToHost, stream_));
+        chk(hipMemcpyAsync(nb.data(), d_neigh, nb.size() * 4, hipMemcpyDeviceToHost, stream_));
+        chk(hipStreamSynchronize(stream_));
+    }
+    cleanup();
+    if (e != hipSuccess) { err = std::string("nlist: ") + hipGetErrorString(e); return -TILD_ERR_DEVICE; }
+    if (ovf) { err = "nlist: more sites in a cell or list than CELL_MAX (nlist.cu:49-51)"; return -TILD_ERR_CONFIG; }
+    for (int i = 0; i < ns; ++i) {
+        const int c = std::min(counts[i], a.cap);
+        std::copy(nb.begin() + (size_t)i * a.cap, nb.begin() + (size_t)i * a.cap + c, neigh + (size_t)i * cap);
+    }
     return 0;
 }
 

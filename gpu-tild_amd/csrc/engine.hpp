@@ -19,6 +19,7 @@
 
 namespace tild {
 
+constexpr int kXcds = 8;         // accelerator complex dies of an MI355X
 constexpr int kProfPhases = 8;   // spread, x_r2c, y_fwd, last_fused, y_inv, x_c2r, particles, advance
 
 // Work-map phases.  *A: replicas on the global-atomic spread; T*: replicas on the
@@ -50,6 +51,10 @@ struct Replica {
     double* d_results = nullptr;      // nq + 7 results of the last property evaluation
     double* d_bpart = nullptr;        // 7 * nblk_part bond partials
     float* d_fout = nullptr;
+    // periodic spatial reorder of the particle arrays (0 = never)
+    int reorder_every = 0, since_sort = 0, sort_bits = 0;
+    bool permuted = false;            // slots no longer in site order (getters un-permute)
+    ReorderBufs rb;
 };
 
 class Batch {
@@ -57,9 +62,16 @@ public:
     explicit Batch(int device) : device_(device) {}
     // spread mode for replicas added from now on: 0 auto, 1 global atomics, 2 tiled;
     // target block count and forced tile shape (0 = free) of the tiled mode.
-    void set_spread(int mode, int target_tiles = 1024, int by = 0, int bz = 0) {
+    void set_spread(int mode, int target_tiles = 512, int by = 0, int bz = 0) {
         spread_mode_ = mode; tile_target_ = target_tiles; tile_by_ = by; tile_bz_ = bz;
     }
+    // spatial reorder interval of replicas added from now on: > 0 every n steps,
+    // 0 never, < 0 automatic (every 20 steps when the gather records outgrow L2)
+    void set_reorder(int every) { reorder_every_ = every; }
+    // FFT block geometry of replicas added from now on (0 = default): complex values
+    // per middle-axis block, lines per fused last-axis block, rows per inverse-x block
+    struct FftTuning { int mid_elems = 0, last_lines = 0, c2r_rows = 0; };
+    void set_fft_tuning(const FftTuning& t) { tune_ = t; }
     ~Batch();
     int init(std::string& err);
 
@@ -86,6 +98,9 @@ public:
     int get_density(int id, int type, float* rho, size_t n, std::string& err);
     int get_grid_field(int id, int type, int dir, float* g, size_t n, std::string& err);
     int get_bins(int id, int* base, size_t n, std::string& err);
+    // Neighbour lists of a site group at the current positions (sites == null: all).
+    int nlist(int id, float rc, const int* sites, int nsites, int* cell_of, int* counts, int* neigh,
+              size_t cap, int* cell_max, int* ncells, std::string& err);
     int sync(std::string& err);
     hipStream_t stream() const { return stream_; }
 
@@ -103,7 +118,8 @@ public:
 
 private:
     int device_;
-    int spread_mode_ = 0, tile_target_ = 1024, tile_by_ = 0, tile_bz_ = 0;
+    int spread_mode_ = 0, tile_target_ = 512, tile_by_ = 0, tile_bz_ = 0, reorder_every_ = -1;
+    FftTuning tune_;
     hipStream_t stream_ = nullptr;
     std::vector<std::unique_ptr<Replica>> reps_;
     RepDev* d_reps_ = nullptr;
@@ -114,6 +130,9 @@ private:
     int build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& err);
     int build_tables(Replica& R, const System& S, std::string& err);
     int forward_spectra(int id, std::string& err);   // rho -> H (properties)
+    int reorder(int id, std::string& err);           // spatial reorder of one replica
+    int run_steps(SetMaps& m, const std::vector<int>& ids, int n, bool use_graph, std::string& err);
+    int fetch_pid(const Replica& R, std::vector<int>& pid, std::string& err);
     void free_maps(PhaseMap* maps);
 };
 

@@ -290,6 +290,11 @@ __global__ void __launch_bounds__(kFftThreads) k_x_c2r(const RepDev* __restrict_
             }
             const size_t node = (size_t)row * N + x;
             const float rho = f < r.ntypes ? r.rho[(size_t)f * r.M + node] : 1.0f;
+            if (r.FPtab && f < r.ntypes)
+                // field phases (pair_style_fieldphases.cu:54-86): the grid force gains
+                // +-rho^2 f_d (d_prepareFieldForce then d_accumulateGridForce multiply
+                // by rho twice), i.e. rho f_d per unit density
+                for (int d = 0; d < D; ++d) g[d] += rho * r.FPtab[((size_t)f * D + d) * r.M + node];
             r.G4[(size_t)f * r.M + node] = make_float4(g[0], g[1], g[2], rho);
         }
     } else {
@@ -381,7 +386,8 @@ __global__ void __launch_bounds__(256) k_particles(const RepDev* __restrict__ re
     }
     const uint32_t stepc = (uint32_t)(r.step + 1);
     float z[4];
-    normals4(r.seed, (uint32_t)i, stepc, z);
+    const uint32_t site = (uint32_t)r.pid[i];   // original site id: slot order does not matter
+    normals4(r.seed, site, stepc, z);
     if (kind == 1) {
         for (int d = 0; d < D; ++d) {
             float v = xi[d] + r.delt * f[d] + r.noise_mag * z[d];
@@ -391,7 +397,7 @@ __global__ void __launch_bounds__(256) k_particles(const RepDev* __restrict__ re
         }
     } else {
         float zo[4] = {0.f, 0.f, 0.f, 0.f};
-        if (stepc > 1u) normals4(r.seed, (uint32_t)i, stepc - 1u, zo);
+        if (stepc > 1u) normals4(r.seed, site, stepc - 1u, zo);
         // GJF (device_GJF_integrator.cu:25-58) in displacement form: with
         // b = m/(m + dt/2D) and a = (1 - dt/2Dm)/(1 + dt/2Dm), 2b - a = 1 exactly, so
         //   x' = 2b x - a xo + c1 f + c2 (beta + beta_old)

@@ -24,7 +24,7 @@ __host__ __device__ inline MidGeom mid_geom(const RepDev& r, int axis, int nfb) 
         g.N = r.Nz; g.lstride = r.Ny * r.Nxh; g.batch = r.Ny * r.Nxh;
         g.nouter = nfb; g.ostride = r.Nz * r.Ny * r.Nxh;
     }
-    int nl = kFftBudget / g.N;
+    int nl = (r.mid_elems > 0 ? r.mid_elems : kFftBudget) / g.N;
     if (nl < 1) nl = 1;
     if (nl > g.batch) nl = g.batch;
     g.nchunk = (g.batch + nl - 1) / nl;
@@ -59,5 +59,33 @@ hipError_t launch_tile_scatter(const LaunchCtx& c);
 hipError_t launch_tile_deposit(const LaunchCtx& c);
 hipError_t launch_bins(hipStream_t s, const RepDev* reps, int rep, int ns, int* out);
 hipError_t launch_virial_source(hipStream_t s, const RepDev* reps, int rep, int M, const float* fr, float* vr);
+
+// Buffers of the periodic spatial reorder (one set per replica that reorders).
+struct ReorderBufs {
+    uint32_t* keys[2] = {nullptr, nullptr};   // ns row keys (unsorted, sorted)
+    int* vals[2] = {nullptr, nullptr};        // ns slot ids (identity, new order)
+    void* tmp = nullptr;                      // radix-sort scratch
+    size_t tmp_bytes = 0;
+};
+size_t reorder_tmp_bytes(int ns, int bits);
+
+// Neighbour-list build (device buffers, all indexed by site id except cell_slots).
+struct NlistArgs {
+    const unsigned char* member;   // ns group-membership flags
+    int* cell_of;                  // ns cell of each member (-1 outside the group)
+    int* counts;                   // ns list lengths
+    int* neigh;                    // ns * cap sorted neighbour site ids
+    int* cell_count;               // ncells
+    int* cell_slots;               // ncells * cmax slots
+    int* overflow;                 // 1: a cell exceeded cmax, 2: a list exceeded cap
+    float Lc[3];
+    int nc[3];
+    float rc;
+    int cmax, cap;
+};
+hipError_t launch_nlist(hipStream_t s, const RepDev* reps, int rep, int ns, const NlistArgs& a);
+// Sort the replica's beads by grid row (stable), permute every per-bead array and
+// swap the live and spare buffers in the device descriptor.
+hipError_t launch_reorder(hipStream_t s, RepDev* reps, int rep, int ns, int bits, ReorderBufs& b);
 
 }  // namespace tild

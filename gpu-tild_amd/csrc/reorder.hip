@@ -1,0 +1,127 @@
+// reorder.hip -- periodic spatial reordering of a replica's particle arrays.
+//
+// The reference keeps particles in file order for the whole run (config_utils.cu:
+// 5-364); its gather and spread then touch the grid in chain order.  Here, every K
+// steps, the slots of a replica are permuted so that beads whose stencils start in
+// the same grid row are neighbours in memory, rows ordered tile by tile (the tiles of
+// the sorted-bin deposition, z-major).  A wave of the particle kernel then reads a
+// few adjacent grid rows instead of 64 scattered ones, and a contiguous run of blocks
+// covers one slab of the grid.
+//
+// The permutation is a stable radix sort of the row keys (rocPRIM), so it depends
+// only on the positions and the previous order: runs stay bit-reproducible.  Every
+// per-step result is independent of slot order (integer deposition, per-bead gather
+// and bonds, noise keyed by the original site id `pid`), so trajectories are
+// bit-identical with and without reordering.  Bond partners are renumbered through
+// the inverse permutation.
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "device.hpp"
+#include "launch.hpp"
+
+namespace tild {
+
+// Row key of each bead: tile-major, then the row inside the tile (a bijection of the
+// Ny*Nz grid rows; with no tiles the row index itself).
+__global__ void __launch_bounds__(256) k_sort_keys(const RepDev* __restrict__ reps, int rep,
+                                                   uint32_t* __restrict__ keys, int* __restrict__ vals) {
+    const RepDev& r = reps[rep];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= r.ns) return;
+    const int D = r.dim;
+    const float* x = r.X[r.step & 1];
+    float W[5];
+    const int sy = stencil_1d(x[i * D + 1], r.dx[1], r.Ny, r.P, W);
+    const int sz = D == 3 ? stencil_1d(x[i * D + 2], r.dx[2], r.Nz, r.P, W) : 0;
+    uint32_t key;
+    if (r.spread_mode == 2) {
+        const int ty = sy / r.By, tz = sz / r.Bz;
+        key = (uint32_t)(((tz * r.nty + ty) * r.Bz + sz % r.Bz) * r.By + sy % r.By);
+    } else {
+        key = (uint32_t)(sz * r.Ny + sy);
+    }
+    keys[i] = key;
+    vals[i] = i;
+}
+
+// inv[old slot] = new slot
+__global__ void __launch_bounds__(256) k_reorder_inv(int ns, const int* __restrict__ order, int* __restrict__ inv) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p < ns) inv[order[p]] = p;
+}
+
+// Gather every per-bead array into the spare buffers in the new order.  Positions
+// go to the free half of the position double buffer.
+__global__ void __launch_bounds__(256) k_reorder_apply(const RepDev* __restrict__ reps, int rep,
+                                                       const int* __restrict__ order,
+                                                       const int* __restrict__ inv) {
+    const RepDev& r = reps[rep];
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= r.ns) return;
+    const int D = r.dim, i = order[p];
+    const float* __restrict__ x = r.X[r.step & 1];
+    float* __restrict__ xn = r.X[(r.step + 1) & 1];
+    for (int d = 0; d < D; ++d) {
+        xn[p * D + d] = x[i * D + d];
+        r.XO_alt[p * D + d] = r.XO[i * D + d];
+    }
+    r.tp_alt[p] = r.tp[i];
+    r.ikind_alt[p] = r.ikind[i];
+    r.pid_alt[p] = r.pid[i];
+    if (r.q) r.q_alt[p] = r.q[i];
+    for (int b = 0; b < 3; ++b) {
+        const int j = r.bond_j[i * 3 + b];
+        r.bond_j_alt[p * 3 + b] = j < 0 ? -1 : inv[j];
+        r.bond_t_alt[p * 3 + b] = r.bond_t[i * 3 + b];
+    }
+}
+
+template <class T, class U>
+__device__ inline void swap_ptr(T*& a, U*& b) {
+    T* t = a;
+    a = (T*)b;
+    b = (U*)t;
+}
+
+// Make the reordered arrays live (one thread; stream-ordered after the gather).
+__global__ void k_reorder_swap(RepDev* reps, int rep) {
+    RepDev& r = reps[rep];
+    swap_ptr(r.X[0], r.X[1]);
+    swap_ptr(r.XO, r.XO_alt);
+    swap_ptr(r.tp, r.tp_alt);
+    swap_ptr(r.ikind, r.ikind_alt);
+    swap_ptr(r.pid, r.pid_alt);
+    if (r.q) swap_ptr(r.q, r.q_alt);
+    swap_ptr(r.bond_j, r.bond_j_alt);
+    swap_ptr(r.bond_t, r.bond_t_alt);
+}
+
+#define CHECK_LAUNCH() do { hipError_t e__ = hipGetLastError(); if (e__ != hipSuccess) return e__; } while (0)
+
+size_t reorder_tmp_bytes(int ns, int bits) {
+    size_t bytes = 0;
+    if (rocprim::radix_sort_pairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                  (const int*)nullptr, (int*)nullptr, (unsigned)ns, 0u, (unsigned)bits) != hipSuccess)
+        return 0;
+    return bytes;
+}
+
+hipError_t launch_reorder(hipStream_t s, RepDev* reps, int rep, int ns, int bits, ReorderBufs& b) {
+    const int nb = (ns + 255) / 256;
+    hipLaunchKernelGGL(k_sort_keys, dim3(nb), dim3(256), 0, s, reps, rep, b.keys[0], b.vals[0]);
+    CHECK_LAUNCH();
+    size_t tb = b.tmp_bytes;
+    hipError_t e = rocprim::radix_sort_pairs(b.tmp, tb, b.keys[0], b.keys[1], b.vals[0], b.vals[1], (unsigned)ns,
+                                             0u, (unsigned)bits, s);
+    if (e != hipSuccess) return e;
+    int* inv = (int*)b.keys[0];   // the unsorted keys are no longer needed
+    hipLaunchKernelGGL(k_reorder_inv, dim3(nb), dim3(256), 0, s, ns, b.vals[1], inv);
+    CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_reorder_apply, dim3(nb), dim3(256), 0, s, reps, rep, b.vals[1], inv);
+    CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_reorder_swap, dim3(1), dim3(1), 0, s, reps, rep);
+    CHECK_LAUNCH();
+    return hipSuccess;
+}
+
+}  // namespace tild

@@ -35,15 +35,30 @@ __device__ inline int tile_bucket(const RepDev& r, const float* x, int i) {
     return (tz * r.nty + ty) * 4 + c;
 }
 
+// Slots are claimed per wave: lanes with the same bucket (neighbours in a spatially
+// ordered replica) share one atomic, issued by the lowest of them.
 __global__ void __launch_bounds__(256) k_tile_count(const RepDev* __restrict__ reps,
                                                     const WorkItem* __restrict__ wmap) {
     const WorkItem wi = wmap[blockIdx.x];
     const RepDev& r = reps[wi.rep];
     const int i = wi.blk * 256 + threadIdx.x;
-    if (i >= r.ns) return;
-    const int b = tile_bucket(r, r.X[r.step & 1], i);
-    const int slot = atomicAdd(r.tcnt + b, 1);
-    r.tkey[i] = make_int2(b, slot);
+    const bool valid = i < r.ns;
+    const int b = valid ? tile_bucket(r, r.X[r.step & 1], i) : -1;
+    const int lane = (int)__lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    unsigned long long todo = __ballot(valid);
+    int slot = 0;
+    while (todo) {   // wave-uniform: one round per distinct bucket in the wave
+        const int leader = __ffsll((long long)todo) - 1;
+        const int bl = __shfl(b, leader);
+        const unsigned long long same = __ballot(b == bl) & todo;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(r.tcnt + bl, __popcll(same));
+        base = __shfl(base, leader);
+        if (b == bl) slot = base + __popcll(same & below);
+        todo &= ~same;
+    }
+    if (valid) r.tkey[i] = make_int2(b, slot);
 }
 
 // Exclusive scan of the bucket counts; clears the counts for the next step.

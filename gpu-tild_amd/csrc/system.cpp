@@ -84,7 +84,53 @@ int System::derive(std::string& err) {
         grid_per_partic *= (pmeorder + 1);
     }
     n_P_comps = (Dim == 2) ? 3 : 6;
+    for (const FieldPhase& F : Fields) {
+        // pair_style_fieldphases.cu:89-185 defines phases 0, 2 and 3; phase 1 (BCC) has
+        // no branch there and leaves the field uninitialised
+        if (F.phase == 1) { err = "fieldphase phase 1 (BCC) is not defined by the reference"; return kUnsupported; }
+        if (F.phase != 0 && F.phase != 2 && F.phase != 3) { err = "unknown fieldphase phase " + std::to_string(F.phase); return kConfig; }
+        if (F.phase == 0 && (F.dir < 0 || F.dir >= Dim)) { err = "fieldphase direction out of range"; return kConfig; }
+        if (F.phase == 2 && (F.dir < 0 || F.dir > 2 || (Dim == 2 && F.dir != 2))) { err = "fieldphase cylinder direction out of range (2D needs 2)"; return kConfig; }
+        if (F.phase == 3 && Dim != 3) { err = "fieldphase gyroid needs Dim 3"; return kUnsupported; }
+    }
     return 0;
+}
+
+void fieldphase_force_table(const System& S, const FieldPhase& F, std::vector<float>& fr) {
+    const int D = S.Dim, M = S.M;
+    const float PI = 3.141592654f;     // globals.h:29
+    const float Ao = F.initial_prefactor;
+    fr.assign((size_t)M * D, 0.0f);
+    for (int ind = 0; ind < M; ++ind) {
+        const int n[3] = {ind % S.Nx[0], (ind / S.Nx[0]) % S.Nx[1], D == 3 ? ind / (S.Nx[0] * S.Nx[1]) : 0};
+        float r[3] = {0.f, 0.f, 0.f};
+        for (int j = 0; j < D; ++j) r[j] = float(n[j]) * S.dx[j];   // d_get_r, device_array_utils.cu:34-41
+        float* f = fr.data() + (size_t)ind * D;
+        if (F.phase == 0) {
+            const float a = 2.0f * PI * float(F.n_periods) / S.L[F.dir];
+            f[F.dir] = Ao * a * std::cos(a * r[F.dir]);
+        } else if (F.phase == 2) {
+            int d1 = 0, d2 = 1;
+            if (F.dir == d1) d1 = 2;
+            else if (F.dir == d2) d2 = 2;
+            const float a1 = 2.0f * PI * float(F.n_periods) / S.L[d1];
+            const float a2 = 2.0f * PI * float(F.n_periods) / S.L[d2];
+            if (D == 3) f[F.dir] = 0.0f;
+            f[d1] = Ao * a1 * std::sin(a1 * r[d1]) * std::cos(a2 * r[d2]);
+            f[d2] = Ao * a2 * std::sin(a2 * r[d2]) * std::cos(a1 * r[d1]);
+        } else {
+            float a[3], c[3], sn[3];
+            for (int j = 0; j < 3; ++j) {
+                a[j] = 2.0f * PI * float(F.n_periods) / S.L[j];
+                c[j] = std::cos(a[j] * r[j]);
+                sn[j] = std::sin(a[j] * r[j]);
+            }
+            const float e = sn[0] * c[1] + sn[1] * c[2] + sn[2] * c[0];
+            f[0] = -Ao * e * a[0] * (c[0] * c[1] - sn[2] * sn[0]);
+            f[1] = -Ao * e * a[1] * (c[1] * c[2] - sn[0] * sn[1]);
+            f[2] = -Ao * e * a[2] * (c[2] * c[0] - sn[1] * sn[2]);
+        }
+    }
 }
 
 // ---------------------------------------------------------------- helpers
@@ -367,12 +413,30 @@ int load_system(const std::string& text, const std::string& base_dir, System& S,
                 if (w == "n_gaussians") { S.Gausses.clear(); }
                 else if (w == "n_erfs") { S.Erfs.clear(); }
                 else if (w == "n_gaussian_erfs") { S.GaussErfs.clear(); }
+                else { S.Fields.clear(); }
                 for (int k = 0; k < iv; ++k) {
                     std::string sl = li < lines.size() ? lines[li] : std::string();
                     ++li;
                     auto t = words(sl);
                     if (t.empty() || t[0] != want) { err = "Read: " + (t.empty() ? std::string() : t[0]) + " but expected '" + want + "'"; return kParse; }
-                    if (want == "fieldphase") { err = "fieldphase pair style is not supported by this engine"; return kUnsupported; }
+                    if (want == "fieldphase") {   // read_input.cu:237-297
+                        FieldPhase F;
+                        int t1, t2;
+                        if (t.size() < 7 || !to_int(t[1], t1) || !to_int(t[2], t2) || !to_float(t[3], F.initial_prefactor) ||
+                            !to_int(t[4], F.phase) || !to_int(t[5], F.dir) || !to_int(t[6], F.n_periods))
+                            { err = "malformed fieldphase line: " + sl; return kParse; }
+                        if (t1 < 1 || t2 < 1 || t1 > S.ntypes || t2 > S.ntypes) { err = "Invalid particle type in FieldPhase potential input!"; return kParse; }
+                        F.type1 = t1 - 1; F.type2 = t2 - 1;
+                        F.final_prefactor = F.initial_prefactor;
+                        if (t.size() > 7 && t[7] == "ramp") {
+                            if (t.size() <= 8 || !to_float(t[8], F.final_prefactor)) { err = "ramp needs a final prefactor"; return kParse; }
+                            F.ramp = true;
+                            err = "ramped pair-style prefactors are not supported by this engine";
+                            return kUnsupported;
+                        }
+                        S.Fields.push_back(F);
+                        continue;
+                    }
                     size_t need = (want == "gaussian") ? 5 : 6;
                     int t1, t2; float A, p1, p2 = 0;
                     if (t.size() < need || !to_int(t[1], t1) || !to_int(t[2], t2) || !to_float(t[3], A) || !to_float(t[4], p1) ||

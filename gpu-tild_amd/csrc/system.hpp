@@ -70,6 +70,15 @@ struct Charges {
     double potential(double k2) const;
 };
 
+// Static field coupling (pair_style_fieldphases.h): type1 feels +w(r), type2 -w(r).
+// phase 0 lamellar along dir, 2 cylinders along dir, 3 gyroid (3D).
+struct FieldPhase {
+    int type1 = 0, type2 = 0;
+    float initial_prefactor = 0.0f, final_prefactor = 0.0f;
+    int phase = 0, dir = 0, n_periods = 1;
+    bool ramp = false;
+};
+
 struct Group {                        // group.h:11-27
     std::string name, command_line;
     std::vector<int> index;
@@ -112,6 +121,7 @@ struct System {
     std::vector<std::unique_ptr<Gaussian>> Gausses;
     std::vector<std::unique_ptr<Erf>> Erfs;
     std::vector<std::unique_ptr<GaussianErf>> GaussErfs;
+    std::vector<FieldPhase> Fields;
     std::vector<Group> Groups;
     std::vector<Integrator> Integrators;
     std::vector<std::string> notes;   // accepted-but-ignored keywords
@@ -125,6 +135,10 @@ struct System {
     std::vector<const PairStyle*> styles() const;
     int derive(std::string& err);
 };
+
+// Force table fr[M*Dim] of one field phase at the grid nodes, in FP32 as
+// init_device_fieldphase computes it (pair_style_fieldphases.cu:89-185).
+void fieldphase_force_table(const System& S, const FieldPhase& F, std::vector<float>& fr);
 
 // Parse `text` (reference grammar) and its data file(s).  0 on success.
 // Returns 1 for malformed input (TILD_ERR_PARSE), 2 for non-runnable decks

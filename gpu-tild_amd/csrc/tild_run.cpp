@@ -33,7 +33,7 @@ struct Run {
     float L[3] = {1, 1, 1}, dx[3] = {1, 1, 1};
     double V = 1;
     int max_steps = 0, log_freq = 0, traj_freq = 0, grid_freq = 0, bin_freq = 0;
-    int ng = 0, ne = 0, nge = 0;
+    int ng = 0, ne = 0, nge = 0, nfp = 0;
     std::vector<int> tp, mol;
     std::vector<float> q, x, rho;
     std::string dump_name;
@@ -143,6 +143,12 @@ int print_timestep(Run& R, int step, const tild_props& p) {
             if (std::isnan(p.U_style[s])) die_flag = 1;
         }
     }
+    // field phases: the reference prints their energy members, which nothing computes
+    // (calc_properties.cu:62-77 sums only the k-space styles); they are reported as 0
+    if (R.nfp) {
+        std::cout << "UFieldPhase: ";
+        for (int j = 0; j < R.nfp; ++j) { std::cout << 0 << " "; R.dout << 0 << " "; }
+    }
     R.dout << std::endl;
     std::cout << std::endl;
     return die_flag;
@@ -184,6 +190,7 @@ int main(int argc, char** argv) {
     R.traj_freq = (int)info(R, "traj_freq"); R.grid_freq = (int)info(R, "grid_freq");
     R.bin_freq = (int)info(R, "binary_freq");
     R.ng = (int)info(R, "n_gaussians"); R.ne = (int)info(R, "n_erfs"); R.nge = (int)info(R, "n_gaussian_erfs");
+    R.nfp = (int)info(R, "n_fieldphases");
     char name[4096];
     if (tild_system_info_text(R.sys, "dump_name", name, sizeof(name)) != TILD_OK) die("dump_name");
     R.dump_name = name;

@@ -24,7 +24,7 @@ EXPORTS = [
     "tild_engine_forces", "tild_engine_step", "tild_engine_properties",
     "tild_engine_get_positions", "tild_engine_set_positions", "tild_engine_get_forces",
     "tild_engine_get_density", "tild_engine_get_grid_field", "tild_engine_get_step",
-    "tild_engine_synchronize", "tild_engine_get_bins", "tild_engine_profile", "tild_fft_r2c", "tild_fft_c2r",
+    "tild_engine_synchronize", "tild_engine_get_bins", "tild_engine_profile", "tild_engine_nlist", "tild_fft_r2c", "tild_fft_c2r",
     "tild_batch_create", "tild_batch_destroy", "tild_batch_last_error", "tild_batch_add", "tild_batch_remove",
     "tild_batch_step", "tild_batch_properties", "tild_batch_get_positions", "tild_batch_set_positions",
     "tild_batch_get_density", "tild_batch_replica_step", "tild_batch_profile",
@@ -43,7 +43,9 @@ class TildError(RuntimeError):
 
 class Opts(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("use_seed", ctypes.c_int), ("seed", ctypes.c_uint32),
-                ("spread_mode", ctypes.c_int), ("reserved", ctypes.c_int * 8)]
+                ("spread_mode", ctypes.c_int), ("tile_target", ctypes.c_int), ("tile_by", ctypes.c_int),
+                ("tile_bz", ctypes.c_int), ("reorder_every", ctypes.c_int), ("fft_mid_elems", ctypes.c_int),
+                ("fft_last_lines", ctypes.c_int), ("fft_c2r_rows", ctypes.c_int), ("reserved", ctypes.c_int * 1)]
 
 
 class Props(ctypes.Structure):
@@ -112,6 +114,7 @@ def lib():
         L.tild_engine_synchronize.argtypes = [vp]
         L.tild_engine_get_bins.argtypes = [vp, ip, sz]
         L.tild_engine_profile.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L.tild_engine_nlist.argtypes = [vp, ctypes.c_float, ip, ctypes.c_int, ip, ip, ip, sz, ip, ip]
         L.tild_batch_create.argtypes = [ctypes.POINTER(Opts), ctypes.POINTER(vp)]
         L.tild_batch_destroy.argtypes = [vp]
         L.tild_batch_last_error.restype = ctypes.c_char_p
@@ -220,13 +223,14 @@ class System:
 class Engine:
     """One device-resident replica (tild_engine)."""
 
-    def __init__(self, system, device=0, seed=None, spread_mode=0, tile_target=0, tile_by=0, tile_bz=0):
+    def __init__(self, system, device=0, seed=None, spread_mode=0, tile_target=0, tile_by=0, tile_bz=0,
+                 reorder_every=0, fft_mid_elems=0, fft_last_lines=0, fft_c2r_rows=0):
         o = Opts()
         o.device = device
         o.spread_mode = spread_mode
-        o.reserved[0] = tile_target
-        o.reserved[1] = tile_by
-        o.reserved[2] = tile_bz
+        o.tile_target, o.tile_by, o.tile_bz = tile_target, tile_by, tile_bz
+        o.reorder_every = reorder_every
+        o.fft_mid_elems, o.fft_last_lines, o.fft_c2r_rows = fft_mid_elems, fft_last_lines, fft_c2r_rows
         if seed is not None:
             o.use_seed = 1
             o.seed = int(seed) & 0xFFFFFFFF
@@ -299,6 +303,22 @@ class Engine:
                   "profile")
         return dict(zip(self.PHASES, a.tolist()))
 
+    def nlist(self, rc, sites=None):
+        """(cell_of, counts, neigh (ns, cell_max), cell_max, ncells) by site id."""
+        cm, nc = ctypes.c_int(), ctypes.c_int()
+        sarr = None if sites is None else np.ascontiguousarray(sites, dtype=np.int32)
+        sp = _ip(sarr) if sarr is not None else None
+        nsites = 0 if sarr is None else len(sarr)
+        self._chk(lib().tild_engine_nlist(self._h, float(rc), sp, nsites, None, None, None, 0,
+                                          ctypes.byref(cm), ctypes.byref(nc)), "nlist size")
+        cap = max(cm.value, 1)
+        cell_of = np.zeros(self.ns, dtype=np.int32)
+        counts = np.zeros(self.ns, dtype=np.int32)
+        neigh = np.full((self.ns, cap), -1, dtype=np.int32)
+        self._chk(lib().tild_engine_nlist(self._h, float(rc), sp, nsites, _ip(cell_of), _ip(counts), _ip(neigh), cap,
+                                          ctypes.byref(cm), ctypes.byref(nc)), "nlist")
+        return cell_of, counts, neigh, cm.value, nc.value
+
     def step_count(self):
         s = ctypes.c_int()
         self._chk(lib().tild_engine_get_step(self._h, ctypes.byref(s)), "get_step")
@@ -311,11 +331,12 @@ class Engine:
 class Batch:
     """Many device-resident replicas advanced together (tild_batch)."""
 
-    def __init__(self, device=0, spread_mode=0, tile_target=0):
+    def __init__(self, device=0, spread_mode=0, tile_target=0, reorder_every=0):
         o = Opts()
         o.device = device
         o.spread_mode = spread_mode
-        o.reserved[0] = tile_target
+        o.tile_target = tile_target
+        o.reorder_every = reorder_every
         h = ctypes.c_void_p()
         _check(lib().tild_batch_create(ctypes.byref(o), ctypes.byref(h)), "tild_batch_create")
         self._h = h

@@ -54,7 +54,8 @@ def diblock(rng, n_chains, N, fA, box, dim):
 
 def deck(data_name, dim, grid, styles, integrator="EM", delt=0.005, seed=1, max_steps=1000,
          log_freq=1000, charges=None, pmeorder=1, bond_k=1.5, bond_r0=0.0, extra=()):
-    """Input deck text.  styles: list of (kind, t1, t2, A, params...)."""
+    """Input deck text.  styles: list of (kind, t1, t2, A, params...); a fieldphase
+    style is ("fieldphase", t1, t2, A, phase, dir, n_periods)."""
     L = [f"Dim {dim}", f"max_steps {max_steps}", f"log_freq {log_freq}", "binary_freq 0",
          f"pmeorder {pmeorder}", f"delt {delt}", f"rand_seed {seed}"]
     if charges is not None:
@@ -63,7 +64,7 @@ def deck(data_name, dim, grid, styles, integrator="EM", delt=0.005, seed=1, max_
           f"Nx {grid[0]}", f"Ny {grid[1]}"] + ([f"Nz {grid[2]}"] if dim == 3 else [])
     L.append(f"bond 1 harmonic {bond_k} {bond_r0}")
     L += list(extra)
-    kinds = {"gaussian": [], "erf": [], "gaussian_erf": []}
+    kinds = {"gaussian": [], "erf": [], "gaussian_erf": [], "fieldphase": []}
     for s in styles:
         kinds[s[0]].append(s)
     for k, lst in kinds.items():
@@ -160,3 +161,30 @@ def sweep_replica(dirpath, idx, seed=1):
     nch = max(1, int(round(3.0 * L ** 3 / N)))
     styles = [("gaussian", 1, 1, 1.0, 0.5), ("gaussian", 2, 2, 1.0, 0.5), ("gaussian", 1, 2, round(aab, 4), 0.5)]
     return melt(dirpath, nch, N, (L, L, L), (n, n, n), styles=styles, seed=seed * 1000 + idx, max_steps=2000)
+
+
+SERVE_GRIDS_3D = (32, 40, 48, 64, 80, 96)
+SERVE_GRIDS_2D = (64, 96, 128, 192, 256)
+
+
+def serve_job(dirpath, idx, seed=1):
+    """One job of the served stream: half 2D diblock films on 64^2..256^2 grids, half 3D
+    melts on 32^3..96^3 grids (dx = 0.5, rho0 = 3, N = 20, 12k-330k beads), with
+    1,000-5,000 steps.  Returns (deck text, steps)."""
+    rng = np.random.default_rng(seed * 7919 + idx)
+    steps = int(rng.integers(1000, 5001))
+    styles = [("gaussian", 1, 1, 1.0, 0.5), ("gaussian", 2, 2, 1.0, 0.5),
+              ("gaussian", 1, 2, round(float(rng.uniform(1.0, 4.0)), 4), 0.5)]
+    if idx % 2 == 0:
+        n = int(rng.choice(SERVE_GRIDS_2D))
+        L = 0.5 * n
+        nch = max(1, int(round(3.0 * L * L / 20)))
+        text = melt(dirpath, nch, 20, (L, L, 1.0), (n, n), dim=2, styles=styles, seed=seed * 1000 + idx,
+                    max_steps=steps)
+    else:
+        n = int(rng.choice(SERVE_GRIDS_3D))
+        L = 0.5 * n
+        nch = max(1, int(round(3.0 * L ** 3 / 20)))
+        text = melt(dirpath, nch, 20, (L, L, L), (n, n, n), styles=styles, seed=seed * 1000 + idx,
+                    max_steps=steps)
+    return text, steps

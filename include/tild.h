@@ -53,7 +53,7 @@ const char* tild_last_error(void);
 
 /* Scalar facts about a parsed deck.  Keys: "ns", "dim", "ntypes", "M", "Nx0", "Nx1",
    "Nx2", "pmeorder", "n_styles", "n_gaussians", "n_erfs", "n_gaussian_erfs",
-   "do_charges", "n_bonds_total", "nbond_types", "max_steps", "log_freq", "traj_freq",
+   "n_fieldphases", "do_charges", "n_bonds_total", "nbond_types", "max_steps", "log_freq", "traj_freq",
    "grid_freq", "binary_freq", "n_groups", "n_integrators", "seed", "L0", "L1", "L2",
    "dx0", "dx1", "dx2", "gvol", "V", "delt", "noise_mag", "bjerrum", "smear".  Mirrors
    the globals of globals.h:47-80 after initialize() (initialize.cu:32-48). */
@@ -82,7 +82,15 @@ typedef struct {
     int use_seed;        /* nonzero: override the deck's rand_seed (read_input.cu:450-454) */
     uint32_t seed;
     int spread_mode;     /* 0 auto, 1 global fixed-point atomics, 2 LDS-tiled sorted bins */
-    int reserved[8];
+    int tile_target;     /* tiled spread: aim for about this many tiles (0 = 512) */
+    int tile_by, tile_bz;/* tiled spread: force a By x Bz tile shape (0 = free) */
+    int reorder_every;   /* spatial reorder of the particle arrays: 0 automatic (every 20
+                            steps once the gather records outgrow L2), n > 0 every n
+                            steps, < 0 never.  Results do not depend on it. */
+    int fft_mid_elems;   /* FFT block geometry, 0 = default: complex values per block of */
+    int fft_last_lines;  /* the y pass, columns per block of the fused z pass, rows per  */
+    int fft_c2r_rows;    /* block of the inverse x pass.  Results do not depend on them. */
+    int reserved[1];
 } tild_opts;
 
 /* Properties of calc_properties() (calc_properties.cu:20-116): FP64 reductions. */
@@ -110,7 +118,8 @@ const char* tild_engine_last_error(const tild_engine* e);
 tild_status tild_engine_forces(tild_engine* e);
 
 /* n iterations of the main-loop body (main.cu:106-117): step += 1, forces(x_n),
-   Integrate_2 for every integrator (integrator.cu:24-39) -> x_{n+1}. */
+   Integrate_2 for every integrator (integrator.cu:24-39) -> x_{n+1}.  For n >= 2 the
+   step is captured once in a hipGraph and replayed. */
 tild_status tild_engine_step(tild_engine* e, int n);
 
 /* calc_properties() after the last forces()/step: non-bonded energies and virial
@@ -139,6 +148,18 @@ tild_status tild_engine_get_bins(tild_engine* e, int* base, size_t n);
    y-axis FFT, 5 x-axis C2R, 6 gather+bonds+integrate, 7 step counter. */
 tild_status tild_engine_profile(tild_engine* e, int nsteps, double* phase_ms, int nphase);
 tild_status tild_engine_get_step(tild_engine* e, int* step);
+
+/* Cell-list neighbour lists of a site group at the current positions
+   (NList::Initialize + AssignCells + BuildNList, nlist.cu:13-228; the reference builds
+   it on the device for its group name).  sites/nsites: the group's site ids (NULL = all
+   sites; tild_system_group_sites lists a deck group).  Outputs, by site id: cell_of[ns]
+   (x-fastest cell index, -1 outside the group), counts[ns] and neigh[ns*cap], each list
+   sorted ascending and including the site itself.  cap >= *cell_max (the reference's
+   CELL_MAX, nlist.cu:49-51); neigh == NULL only returns *cell_max and *ncells.  More sites
+   in a cell or a list than CELL_MAX is TILD_ERR_CONFIG (the reference overruns). */
+tild_status tild_engine_nlist(tild_engine* e, float rc, const int* sites, int nsites,
+                              int* cell_of, int* counts, int* neigh, size_t cap,
+                              int* cell_max, int* ncells);
 tild_status tild_engine_synchronize(tild_engine* e);
 
 /* ---------------------------------------------------------------- replica batches */

@@ -150,6 +150,74 @@ static void style_tables(ora_sys* s, ora_style* st) {
     free(vr);
 }
 
+
+/* init_device_fieldphase, pair_style_fieldphases.cu:89-185: the force table f_r(node)
+   of a static field w(r) at node positions r = float(n) dx (FP32, d_get_r,
+   device_array_utils.cu:34-41), arguments 2 PI n_periods / L with the reference's FP32
+   PI; trigonometry in FP64.  Phases: 0 lamellar along dir, 2 cylinders along dir, 3
+   gyroid (3D).  Phase 1 (BCC) has no branch in the reference (its table stays
+   uninitialised) and is rejected, as are the combinations that index past Dim. */
+static int field_tables(ora_sys* s, ora_style* st, char* err, int errlen) {
+    int D = s->dim, M = s->M;
+    const double PIf = (double)3.141592654f;
+    if (st->phase == 1) { snprintf(err, errlen, "fieldphase phase 1 (BCC) is not defined by the reference"); return -1; }
+    if (st->phase != 0 && st->phase != 2 && st->phase != 3) { snprintf(err, errlen, "unknown fieldphase phase %d", st->phase); return -1; }
+    if (st->phase == 0 && (st->dir < 0 || st->dir >= D)) { snprintf(err, errlen, "fieldphase direction out of range"); return -1; }
+    if (st->phase == 2 && (st->dir < 0 || st->dir > 2 || (D == 2 && st->dir != 2))) { snprintf(err, errlen, "fieldphase cylinder direction out of range"); return -1; }
+    if (st->phase == 3 && D != 3) { snprintf(err, errlen, "fieldphase gyroid needs Dim 3"); return -1; }
+    st->f_r = (double*)calloc((size_t)M * D, sizeof(double));
+    double A = st->A;
+    for (int i = 0; i < M; ++i) {
+        int nn[3];
+        unstack(s, i, nn);
+        double r[3] = {0, 0, 0};
+        for (int j = 0; j < D; ++j) r[j] = (double)((float)nn[j] * s->dxf[j]);
+        double* f = st->f_r + (size_t)i * D;
+        if (st->phase == 0) {
+            double a = 2.0 * PIf * (double)st->n_periods / (double)s->Lf[st->dir];
+            f[st->dir] = A * a * cos(a * r[st->dir]);
+        } else if (st->phase == 2) {
+            int d1 = 0, d2 = 1;
+            if (st->dir == d1) d1 = 2;
+            else if (st->dir == d2) d2 = 2;
+            double a1 = 2.0 * PIf * (double)st->n_periods / (double)s->Lf[d1];
+            double a2 = 2.0 * PIf * (double)st->n_periods / (double)s->Lf[d2];
+            if (D == 3) f[st->dir] = 0.0;
+            f[d1] = A * a1 * sin(a1 * r[d1]) * cos(a2 * r[d2]);
+            f[d2] = A * a2 * sin(a2 * r[d2]) * cos(a1 * r[d1]);
+        } else {
+            double a[3], c[3], sn[3];
+            for (int j = 0; j < 3; ++j) {
+                a[j] = 2.0 * PIf * (double)st->n_periods / (double)s->Lf[j];
+                c[j] = cos(a[j] * r[j]);
+                sn[j] = sin(a[j] * r[j]);
+            }
+            double e = sn[0] * c[1] + sn[1] * c[2] + sn[2] * c[0];
+            f[0] = -A * e * a[0] * (c[0] * c[1] - sn[2] * sn[0]);
+            f[1] = -A * e * a[1] * (c[1] * c[2] - sn[0] * sn[1]);
+            f[2] = -A * e * a[2] * (c[2] * c[0] - sn[1] * sn[2]);
+        }
+    }
+    return 0;
+}
+
+/* FieldPhase::CalcForces, pair_style_fieldphases.cu:54-86 with d_prepareFieldForce
+   (device_array_utils.cu:410-423) and d_accumulateGridForce (:362-372).  QUIRK: the
+   prepared product rho_t f_d is multiplied by rho_t once more on accumulation, so the
+   grid force is +rho_1^2 f_d on type1 and -rho_2^2 f_d on type2, and after the gather's
+   division by rho a bead feels rho(node) f_d(node). */
+static void calc_forces_field(ora_sys* s, const ora_style* st) {
+    int M = s->M, D = s->dim;
+    double* Fd[3] = {s->fx, s->fy, s->fz};
+    for (int d = 0; d < D; ++d)
+        for (int i = 0; i < M; ++i) {
+            double fr = st->f_r[(size_t)i * D + d];
+            double ra = s->rho[(size_t)st->t1 * M + i], rb = s->rho[(size_t)st->t2 * M + i];
+            Fd[d][(size_t)st->t1 * M + i] += ra * (ra * fr);
+            Fd[d][(size_t)st->t2 * M + i] += rb * (-rb * fr);
+        }
+}
+
 int ora_setup(ora_sys* s, char* err, int errlen) {
     int D = s->dim;
     s->M = 1; s->S = 1; s->gvolf = 1.0f;
@@ -176,8 +244,11 @@ int ora_setup(ora_sys* s, char* err, int errlen) {
     s->f = cz((size_t)s->ns * D);
     s->prev_noise = cz((size_t)s->ns * D);
     for (int k = 0; k < s->nst; ++k) {
-        if (s->st[k].kind == ORA_FIELDPHASE) ERR("fieldphase pair style is not implemented");
-        style_tables(s, &s->st[k]);
+        if (s->st[k].kind == ORA_FIELDPHASE) {
+            if (field_tables(s, &s->st[k], err, errlen)) return -1;
+        } else {
+            style_tables(s, &s->st[k]);
+        }
     }
     memcpy(s->xo, s->x, sizeof(double) * s->ns * D);   /* device_comm_utils.cu:15-16 */
     return 0;
@@ -343,9 +414,13 @@ void ora_forces(ora_sys* s) {
     memset(s->fz, 0, sizeof(double) * TM);
     spread(s);
     static const int order[3] = {ORA_GAUSS, ORA_ERF, ORA_GAUSS_ERF};
-    for (int o = 0; o < 3; ++o)
+    for (int o = 0; o < 3; ++o) {
         for (int k = 0; k < s->nst; ++k)
             if (s->st[k].kind == order[o]) calc_forces_style(s, &s->st[k]);
+        if (o == 1)   /* forces.cu:108-126: Gaussians, Erfs, FieldPhases, GaussianErfs */
+            for (int k = 0; k < s->nst; ++k)
+                if (s->st[k].kind == ORA_FIELDPHASE) calc_forces_field(s, &s->st[k]);
+    }
     if (s->do_charges) calc_charges(s);
     gather(s);
     if (s->n_total_bonds > 0) bonds(s);

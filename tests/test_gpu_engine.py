@@ -25,7 +25,15 @@ CASES = {
     "p2": lambda t: decks.melt3d(t, pmeorder=2),
     "p3": lambda t: decks.melt3d(t, pmeorder=3),
     "p4": lambda t: decks.melt3d(t, pmeorder=4),
+    "3d_fieldphase": lambda t: _with_fields(decks.melt3d(t), ["fieldphase 1 2 0.5 0 2 1", "fieldphase 1 2 0.3 2 0 1",
+                                                              "fieldphase 2 1 0.2 3 0 2"]),
+    "2d_fieldphase": lambda t: _with_fields(decks.melt2d(t), ["fieldphase 1 2 0.5 0 1 2", "fieldphase 1 2 0.4 2 2 1"]),
 }
+
+
+def _with_fields(deck, lines):
+    text, d = deck
+    return text + f"n_fieldphases {len(lines)}\n" + "\n".join(lines) + "\n", d
 
 
 @pytest.mark.parametrize("case", list(CASES))
@@ -174,3 +182,58 @@ def test_tiled_spread_bit_identical_to_atomic(case, tmp_path):
         print(f"[{case}] tiles {by}x{bz}: densities bit-identical; forces |df|/|f| = {U.rel_norm(ft, fa):.1e}")
         et.close()
     ea.close()
+
+
+REORDER_CASES = {
+    "3d_gjf": lambda t: decks.melt3d(t, integrator="GJF", delt=0.01, styles=decks.MIXED),
+    "2d_em": lambda t: decks.melt2d(t),
+    "charged": lambda t: decks.charged(t),
+    "p2_atomic": lambda t: decks.melt3d(t, pmeorder=2),
+}
+
+
+@pytest.mark.parametrize("case", list(REORDER_CASES))
+def test_spatial_reorder_is_invisible(case, tmp_path):
+    # Slots are permuted every 3 steps; everything a caller sees is in site order and
+    # bit-identical to a run that never reorders (integer deposition, per-bead gather
+    # and bonds, noise keyed by the original site id).
+    text, d = REORDER_CASES[case](tmp_path)
+    s = tild.System(text, d)
+    mode = 1 if case.endswith("atomic") else 0
+    print(f"[{case}] ns={s.ns} reorder every 3 steps vs never", flush=True)
+    runs = []
+    for every in (3, -1):
+        e = tild.Engine(s, device=0, seed=11, spread_mode=mode, reorder_every=every)
+        e.step(4)
+        e.step(9)
+        x = e.positions()
+        p = e.properties()
+        bins = e.bins()
+        e.forces()
+        f = e.get_forces()
+        runs.append((x, p, bins, f, [e.density(t) for t in range(e.nfields)]))
+        e.close()
+    (x0, p0, b0, f0, r0), (x1, p1, b1, f1, r1) = runs
+    assert np.array_equal(x0, x1)
+    assert np.array_equal(b0, b1)
+    assert np.array_equal(f0, f1)
+    for a, b in zip(r0, r1):
+        assert np.array_equal(a, b)
+    # FP64 property sums see the beads in another order: equal to rounding
+    assert abs(p0["Upe"] - p1["Upe"]) <= 1e-12 * abs(p1["Upe"])
+    assert np.allclose(p0["Ptens"], p1["Ptens"], rtol=1e-10, atol=1e-14)
+
+
+def test_reordered_run_matches_oracle(tmp_path):
+    text, d = decks.melt3d(tmp_path, integrator="GJF", delt=0.01)
+    s = tild.System(text, d)
+    e = tild.Engine(s, device=0, seed=3, reorder_every=2)
+    o = O.System(text, d, seed=3)
+    e.step(10)
+    o.run(10)
+    L = np.array([s.info("L0"), s.info("L1"), s.info("L2")])
+    dx = U.min_image(e.positions().astype(np.float64) - o.positions(), L)
+    err = np.abs(dx).max()
+    print(f"  10 GJF steps with reorders every 2: max |dx| = {err:.2e}")
+    assert err < 1e-4
+    e.close()

@@ -134,3 +134,24 @@ def test_tild_run_reports_bad_input(tmp_path):
     assert r.returncode == 1 and "Invalid keyword: frobnicate" in r.stderr
     r = subprocess.run([str(exe)], cwd=tmp_path, capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "usage" in r.stderr
+
+
+def test_fieldphase_decks(tmp_path):
+    text, d = decks.melt3d(tmp_path / "a")
+    s = tild.System(text + "n_fieldphases 2\nfieldphase 1 2 0.5 0 2 1\nfieldphase 1 2 0.3 3 0 2\n", d)
+    assert s.info("n_fieldphases") == 2 and s.info("n_styles") == 3
+    bad = [("fieldphase 1 2 0.5 1 0 1", 8),              # BCC: no field in the reference
+           ("fieldphase 1 2 0.5 0 3 1", 2),              # lamellae along a 4th axis
+           ("fieldphase 1 3 0.5 0 0 1", 1),              # type out of range
+           ("fieldphase 1 2 0.5 0 0 1 ramp 1.0", 8)]     # ramps are outside the engine
+    for line, status in bad:
+        with pytest.raises(tild.TildError) as ei:
+            tild.System(text + "n_fieldphases 1\n" + line + "\n", d)
+        assert ei.value.status == status, line
+    text2, d2 = decks.melt2d(tmp_path / "b")
+    with pytest.raises(tild.TildError) as ei:
+        tild.System(text2 + "n_fieldphases 1\nfieldphase 1 2 0.5 3 0 1\n", d2)
+    assert ei.value.status == 8
+    with pytest.raises(tild.TildError) as ei:   # 2D cylinders must run along the normal
+        tild.System(text2 + "n_fieldphases 1\nfieldphase 1 2 0.5 2 0 1\n", d2)
+    assert ei.value.status == 2

@@ -360,3 +360,56 @@ def test_nlist_matches_brute_force(tmp_path):
         ref = np.nonzero((dxv.astype(np.float32) ** 2).sum(axis=1) < np.float32(rc * rc))[0]
         got = np.sort(neigh[i, :counts[i]])
         assert np.array_equal(got, ref), i
+
+
+PI32 = float(np.float32(3.141592654))
+
+
+def _field_u(phase, dir_, A, L, r):
+    """The field w(r) of pair_style_fieldphases.cu:89-185 (written to fieldphase_map.dat)."""
+    a = [2 * PI32 / L[j] for j in range(3)]
+    if phase == 0:
+        return A * math.sin(a[dir_] * r[dir_])
+    if phase == 2:
+        d1, d2 = 0, 1
+        if dir_ == d1:
+            d1 = 2
+        elif dir_ == d2:
+            d2 = 2
+        return A * math.cos(a[d1] * r[d1]) * math.cos(a[d2] * r[d2])
+    e = (math.sin(a[0] * r[0]) * math.cos(a[1] * r[1]) + math.sin(a[1] * r[1]) * math.cos(a[2] * r[2])
+         + math.sin(a[2] * r[2]) * math.cos(a[0] * r[0]))
+    return A * (e * e - 1.44)
+
+
+@pytest.mark.parametrize("phase,dir_,c", [(0, 1, 1.0), (2, 2, -1.0), (2, 0, -1.0), (3, 0, -0.5)])
+def test_fieldphase_forces_follow_the_field(phase, dir_, c, tmp_path):
+    # The reference's force tables are c * grad w with c = +1 (lamellar), -1 (cylinders),
+    # -1/2 (gyroid).  Its grid force is +rho^2 f on type1 and -rho^2 f on type2, so a bead
+    # alone on a node (CIC weight 1, rho = 1/gvol) feels c grad w / gvol, a type-2 bead the
+    # opposite, and two beads stacked on one node twice as much (the rho^2 quirk).
+    A, L = 0.7, (8.0, 8.0, 8.0)
+    nodes = np.array([[1.5, 2.5, 3.0], [4.0, 1.0, 5.5], [6.5, 3.5, 2.0], [6.5, 3.5, 2.0]])
+    types = np.array([1, 2, 1, 1])
+    tild_gen.write_data(tmp_path / "input.data", nodes, types, np.arange(1, 5), [], L, 2, 3)
+    text = tild_gen.deck("input.data", 3, (16, 16, 16), [("fieldphase", 1, 2, A, phase, dir_, 1)])
+    o = O.System(text.replace("bond 1 harmonic 1.5 0.0\n", ""), tmp_path, seed=1)
+    f = o.forces()
+    gvol = 0.5 ** 3
+    h = 1e-5
+    for i, (mult, sgn) in enumerate([(1, 1), (1, -1), (2, 1), (2, 1)]):
+        g = np.zeros(3)
+        for ax in range(3):
+            rp, rm = nodes[i].copy(), nodes[i].copy()
+            rp[ax] += h
+            rm[ax] -= h
+            g[ax] = (_field_u(phase, dir_, A, L, rp) - _field_u(phase, dir_, A, L, rm)) / (2 * h)
+        want = sgn * mult * c * g / gvol
+        assert np.allclose(f[i], want, rtol=1e-6, atol=1e-6 * np.abs(want).max()), (i, f[i], want)
+
+
+def test_fieldphase_bcc_is_rejected(tmp_path):
+    text, d = decks.melt3d(tmp_path)
+    text += "n_fieldphases 1\nfieldphase 1 2 0.5 1 0 1\n"
+    with pytest.raises(ValueError, match="BCC"):
+        O.System(text, d, seed=1)
