@@ -100,6 +100,26 @@ struct RepDev {
 
 struct WorkItem { int rep, blk; };
 
+// Global-memory view of a pointer read from a descriptor.  Such pointers are generic
+// to the compiler, which then emits flat_* accesses that also wait on LDS traffic;
+// with the address space spelled out it emits global_* instructions.
+template <class T> using gmem = __attribute__((address_space(1))) T*;
+template <class T> __device__ __forceinline__ gmem<T> G(T* p) { return (gmem<T>)p; }
+// float2 / float4 are classes and cannot be copied out of an address-space-qualified
+// lvalue; these go through the compiler's native vector types instead.
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float2 gld(const float2* p) {
+    const f2v v = *(gmem<const f2v>)p;
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ float4 gld(const float4* p) {
+    const f4v v = *(gmem<const f4v>)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gst(float2* p, float2 a) { *(gmem<f2v>)p = f2v{a.x, a.y}; }
+__device__ __forceinline__ void gst(float4* p, float4 a) { *(gmem<f4v>)p = f4v{a.x, a.y, a.z, a.w}; }
+
 // ---------------------------------------------------------------- Philox / normals
 // Philox4x32-10 (Salmon et al., SC'11).
 __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {

@@ -210,7 +210,8 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
     h.nl_last = std::min(nl, Q);
     const int nc = (h.nf * D + 1) / 2;
     if (nc * h.Nx > kFftBudget) { err = "grid too large for the inverse x pass"; return -TILD_ERR_UNSUPPORTED; }
-    h.nr_c2r = std::max(1, std::min(kFftBudget / (nc * h.Nx), rows));
+    // about 1,000 complex values per block measured best for the inverse x pass
+    h.nr_c2r = std::max(1, std::min(1024 / (nc * h.Nx), rows));
     // optional block geometry (batching of independent lines: results do not change)
     if (tune_.last_lines > 0) h.nl_last = std::min(h.nl_last, tune_.last_lines);
     if (tune_.c2r_rows > 0) h.nr_c2r = std::min(h.nr_c2r, tune_.c2r_rows);

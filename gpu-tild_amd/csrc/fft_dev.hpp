@@ -13,25 +13,25 @@
 
 namespace tild {
 
-__device__ inline float2 cmul(float2 a, float2 b) {
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-__device__ inline float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }   // a * (-i)
-__device__ inline float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }   // a * (-i)
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
 
 // Forward DFT of small sizes, in place on v[0..R-1].
 template <int R> struct Dft;
 
 template <> struct Dft<2> {
-    __device__ static void run(float2* v) {
+    __device__ __forceinline__ static void run(float2* v) {
         float2 a = v[0], b = v[1];
         v[0] = cadd(a, b); v[1] = csub(a, b);
     }
 };
 template <> struct Dft<3> {
-    __device__ static void run(float2* v) {
+    __device__ __forceinline__ static void run(float2* v) {
         const float s3 = 0.86602540378443864676f;   // sin(2 pi / 3)
         float2 t = cadd(v[1], v[2]);
         float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
@@ -43,7 +43,7 @@ template <> struct Dft<3> {
     }
 };
 template <> struct Dft<4> {
-    __device__ static void run(float2* v) {
+    __device__ __forceinline__ static void run(float2* v) {
         float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
         float2 t2 = cadd(v[1], v[3]), t3 = mul_mi(csub(v[1], v[3]));
         v[0] = cadd(t0, t2); v[2] = csub(t0, t2);
@@ -51,7 +51,7 @@ template <> struct Dft<4> {
     }
 };
 template <> struct Dft<8> {
-    __device__ static void run(float2* v) {
+    __device__ __forceinline__ static void run(float2* v) {
         const float h = 0.70710678118654752440f;
         float2 a[4] = {v[0], v[2], v[4], v[6]}, b[4] = {v[1], v[3], v[5], v[7]};
         Dft<4>::run(a); Dft<4>::run(b);
@@ -67,7 +67,7 @@ template <> struct Dft<8> {
 };
 // Odd prime radices: real/imaginary symmetric pairs (Rader-free direct form).
 template <int R> struct DftOdd {
-    __device__ static void run(float2* v, const float* c, const float* s) {
+    __device__ __forceinline__ static void run(float2* v, const float* c, const float* s) {
         // c[j] = cos(2 pi j / R), s[j] = sin(2 pi j / R), j = 1..(R-1)/2
         float2 out[R];
         float2 sum = v[0];
@@ -96,14 +96,14 @@ template <int R> struct DftOdd {
     }
 };
 template <> struct Dft<5> {
-    __device__ static void run(float2* v) {
+    __device__ __forceinline__ static void run(float2* v) {
         const float c[3] = {1.f, 0.30901699437494742410f, -0.80901699437494742410f};
         const float s[3] = {0.f, 0.95105651629515357212f, 0.58778525229247312917f};
         DftOdd<5>::run(v, c, s);
     }
 };
 template <> struct Dft<7> {
-    __device__ static void run(float2* v) {
+    __device__ __forceinline__ static void run(float2* v) {
         const float c[4] = {1.f, 0.62348980185873353053f, -0.22252093395631440429f, -0.90096886790241912624f};
         const float s[4] = {0.f, 0.78183148246802980871f, 0.97492791218182360702f, 0.43388373911755812048f};
         DftOdd<7>::run(v, c, s);
@@ -111,7 +111,7 @@ template <> struct Dft<7> {
 };
 
 template <int R>
-__device__ inline void fft_stage(const float2* __restrict__ src, float2* __restrict__ dst, int ldb, int nb,
+__device__ __forceinline__ void fft_stage(const float2* __restrict__ src, float2* __restrict__ dst, int ldb, int nb,
                                  int N, int p, const float2* __restrict__ tw) {
     const int m = N / R, nbf = m * nb;
     for (int task = threadIdx.x; task < nbf; task += kFftThreads) {
@@ -122,7 +122,7 @@ __device__ inline void fft_stage(const float2* __restrict__ src, float2* __restr
         if (p > 1) {
             const float2* w = tw + k * (R - 1);
 #pragma unroll
-            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], w[r - 1]);
+            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], gld(w + r - 1));
         }
         Dft<R>::run(v);
         const int o = (i - k) * R + k;
@@ -135,7 +135,8 @@ __device__ inline void fft_stage(const float2* __restrict__ src, float2* __restr
 // as the ping-pong partner.  Requires N*nb <= kFftBudget and a synchronised block
 // (A fully written).  Returns the buffer holding the result (A or B); every stage
 // ends with a barrier, so the result is visible to the whole block on return.
-__device__ inline float2* lds_fft(float2* A, float2* B, int ldb, int nb, const AxisPlan& pl) {
+// Always inlined: the buffers are then known to be LDS (ds_* instructions, not flat).
+__device__ __forceinline__ float2* lds_fft(float2* A, float2* B, int ldb, int nb, const AxisPlan& pl) {
     float2* src = A;
     float2* dst = B;
     int p = 1;

@@ -82,8 +82,10 @@ __global__ void __launch_bounds__(kFftThreads) k_x_r2c(const RepDev* __restrict_
     const int p0 = wi.blk * nl;
     const int nb = min(nl, npairs - p0);
     const int ldb = nb | 1;
-    const float* rin = in_override ? in_override : r.rho;
+    gmem<const float> rin = G(in_override ? in_override : (const float*)r.rho);
     float2* Hout = out_override ? out_override : r.H;
+    gmem<unsigned long long> acc = G(r.acc);
+    gmem<float> rho_out = G(r.rho);
     for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
         const int b = task / N, n = task % N;
         const int ra = 2 * (p0 + b), rb = ra + 1;
@@ -91,15 +93,15 @@ __global__ void __launch_bounds__(kFftThreads) k_x_r2c(const RepDev* __restrict_
         if (FROM_ACC) {
             const int fa = ra / rows_per_field;
             size_t ia = (size_t)ra * N + n;
-            a = (float)((double)(long long)r.acc[ia] * (fa < r.ntypes ? r.fix_inv : r.fixq_inv));
-            r.acc[ia] = 0ull;
-            r.rho[ia] = a;
+            a = (float)((double)(long long)acc[ia] * (fa < r.ntypes ? r.fix_inv : r.fixq_inv));
+            acc[ia] = 0ull;
+            rho_out[ia] = a;
             if (rb < nrows) {
                 const int fb = rb / rows_per_field;
                 size_t ib = (size_t)rb * N + n;
-                bb = (float)((double)(long long)r.acc[ib] * (fb < r.ntypes ? r.fix_inv : r.fixq_inv));
-                r.acc[ib] = 0ull;
-                r.rho[ib] = bb;
+                bb = (float)((double)(long long)acc[ib] * (fb < r.ntypes ? r.fix_inv : r.fixq_inv));
+                acc[ib] = 0ull;
+                rho_out[ib] = bb;
             }
         } else {
             a = rin[(size_t)ra * N + n];
@@ -114,8 +116,8 @@ __global__ void __launch_bounds__(kFftThreads) k_x_r2c(const RepDev* __restrict_
         const float2 Zk = res[k * ldb + b];
         const float2 Zm = res[((N - k) % N) * ldb + b];
         const int ra = 2 * (p0 + b), rb = ra + 1;
-        Hout[(size_t)ra * Nxh + k] = make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y));
-        if (rb < nrows) Hout[(size_t)rb * Nxh + k] = make_float2(0.5f * (Zk.y + Zm.y), 0.5f * (Zm.x - Zk.x));
+        gst(Hout + (size_t)ra * Nxh + k, make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y)));
+        if (rb < nrows) gst(Hout + (size_t)rb * Nxh + k, make_float2(0.5f * (Zk.y + Zm.y), 0.5f * (Zm.x - Zk.x)));
     }
 }
 
@@ -139,7 +141,7 @@ __global__ void __launch_bounds__(kFftThreads) k_mid(const RepDev* __restrict__ 
     float2* base = buf + (size_t)o * g.ostride + b0;
     for (int task = threadIdx.x; task < nb * g.N; task += kFftThreads) {
         const int b = task % nb, n = task / nb;
-        float2 v = base[(size_t)n * g.lstride + b];
+        float2 v = gld(base + (size_t)n * g.lstride + b);
         lds[n * ldb + b] = inverse ? conjf2(v) : v;
     }
     __syncthreads();
@@ -147,7 +149,7 @@ __global__ void __launch_bounds__(kFftThreads) k_mid(const RepDev* __restrict__ 
     for (int task = threadIdx.x; task < nb * g.N; task += kFftThreads) {
         const int b = task % nb, n = task / nb;
         float2 v = res[n * ldb + b];
-        base[(size_t)n * g.lstride + b] = inverse ? conjf2(v) : v;
+        gst(base + (size_t)n * g.lstride + b, inverse ? conjf2(v) : v);
     }
 }
 
@@ -179,7 +181,7 @@ __global__ void __launch_bounds__(kFftThreads) k_last_fused(const RepDev* __rest
     float2* C0 = lds + 2 * (size_t)N * ldb;
     for (int task = threadIdx.x; task < nlines * N; task += kFftThreads) {
         const int b = task % nb, rest = task / nb, f = rest % nf, n = rest / nf;
-        A0[n * ldb + f * nb + b] = r.H[(size_t)f * r.Mh + (size_t)n * Q + q0 + b];
+        A0[n * ldb + f * nb + b] = gld(r.H + (size_t)f * r.Mh + (size_t)n * Q + q0 + b);
     }
     __syncthreads();
     float2* A = lds_fft(A0, B0, ldb, nlines, r.ax[la]);   // spectra rho_f(k)
@@ -194,14 +196,14 @@ __global__ void __launch_bounds__(kFftThreads) k_last_fused(const RepDev* __rest
             for (int J = 0; J < T; ++J) {
                 const int lo = I < J ? I : J, hi = I < J ? J : I;
                 const int pidx = lo * T - lo * (lo - 1) / 2 + (hi - lo);
-                const float c = r.Ctab[(size_t)pidx * r.Mh + idx];
+                const float c = G(r.Ctab)[(size_t)pidx * r.Mh + idx];
                 const float2 rh = A[row + J * nb];
                 acc.x += c * rh.x; acc.y += c * rh.y;
             }
             Wb[row + I * nb] = acc;
         }
         if (nf > T) {
-            const float c = r.Ctab[(size_t)r.ctab_charge * r.Mh + idx];
+            const float c = G(r.Ctab)[(size_t)r.ctab_charge * r.Mh + idx];
             const float2 rh = A[row + T * nb];
             Wb[row + T * nb] = make_float2(c * rh.x, c * rh.y);
         }
@@ -230,7 +232,7 @@ __global__ void __launch_bounds__(kFftThreads) k_last_fused(const RepDev* __rest
         const float2* res = lds_fft(B, C, ldb, nlines, r.ax[la]);
         for (int task = threadIdx.x; task < nlines * N; task += kFftThreads) {
             const int b = task % nb, rest = task / nb, f = rest % nf, n = rest / nf;
-            r.Gk[(size_t)(f * r.dim + d) * r.Mh + (size_t)n * Q + q0 + b] = conjf2(res[n * ldb + f * nb + b]);
+            gst(r.Gk + (size_t)(f * r.dim + d) * r.Mh + (size_t)n * Q + q0 + b, conjf2(res[n * ldb + f * nb + b]));
         }
         __syncthreads();
     }
@@ -260,7 +262,7 @@ __global__ void __launch_bounds__(kFftThreads) k_x_c2r(const RepDev* __restrict_
     const int rows = min(nr, rows_per_field - row0);
     const int nb = rows * nc;
     const int ldb = nb | 1;
-    const float2* Gin = RECORDS ? r.Gk : in_override;
+    const float2* Gin = RECORDS ? (const float2*)r.Gk : in_override;
     const bool even = (N & 1) == 0;
     for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
         const int k = task % N, b = task / N;
@@ -268,8 +270,8 @@ __global__ void __launch_bounds__(kFftThreads) k_x_c2r(const RepDev* __restrict_
         const int la = 2 * j, lb = la + 1;
         const bool upper = k >= Nxh;
         const int kk = upper ? N - k : k;
-        float2 A = Gin[((size_t)la * rows_per_field + row) * Nxh + kk];
-        float2 Bv = lb < nlines ? Gin[((size_t)lb * rows_per_field + row) * Nxh + kk] : make_float2(0.f, 0.f);
+        float2 A = gld(Gin + ((size_t)la * rows_per_field + row) * Nxh + kk);
+        float2 Bv = lb < nlines ? gld(Gin + ((size_t)lb * rows_per_field + row) * Nxh + kk) : make_float2(0.f, 0.f);
         if (kk == 0 || (even && 2 * kk == N)) { A.y = 0.f; Bv.y = 0.f; }
         if (upper) { A.y = -A.y; Bv.y = -Bv.y; }
         // Z = A + i B ; store conj(Z) for the conj(FFT(conj)) inverse
@@ -289,19 +291,19 @@ __global__ void __launch_bounds__(kFftThreads) k_x_c2r(const RepDev* __restrict_
                 g[d] = (line & 1) ? -z.y : z.x;      // conj of the forward result
             }
             const size_t node = (size_t)row * N + x;
-            const float rho = f < r.ntypes ? r.rho[(size_t)f * r.M + node] : 1.0f;
+            const float rho = f < r.ntypes ? G(r.rho)[(size_t)f * r.M + node] : 1.0f;
             if (r.FPtab && f < r.ntypes)
                 // field phases (pair_style_fieldphases.cu:54-86): the grid force gains
                 // +-rho^2 f_d (d_prepareFieldForce then d_accumulateGridForce multiply
                 // by rho twice), i.e. rho f_d per unit density
-                for (int d = 0; d < D; ++d) g[d] += rho * r.FPtab[((size_t)f * D + d) * r.M + node];
-            r.G4[(size_t)f * r.M + node] = make_float4(g[0], g[1], g[2], rho);
+                for (int d = 0; d < D; ++d) g[d] += rho * G(r.FPtab)[((size_t)f * D + d) * r.M + node];
+            gst(r.G4 + (size_t)f * r.M + node, make_float4(g[0], g[1], g[2], rho));
         }
     } else {
         for (int task = threadIdx.x; task < rows * nlines * N; task += kFftThreads) {
             const int x = task % N, rest = task / N, line = rest % nlines, rr = rest / nlines;
             const float2 z = res[x * ldb + rr * nc + line / 2];
-            out_override[((size_t)line * rows_per_field + row0 + rr) * N + x] = (line & 1) ? -z.y : z.x;
+            G(out_override)[((size_t)line * rows_per_field + row0 + rr) * N + x] = (line & 1) ? -z.y : z.x;
         }
     }
 }
@@ -312,6 +314,71 @@ __global__ void __launch_bounds__(kFftThreads) k_x_c2r(const RepDev* __restrict_
 // The gather reads g = all_f/rho directly; nodes with rho <= 0 contribute nothing
 // (device_grid_utils.cu:640).  Positions are read from X[step&1] and written to
 // X[(step+1)&1], so bonds always see x_n.
+// Stencil of NP = P + 1 points per axis, fully unrolled so that every grid record of
+// the bead is requested before the first one is used.  The rho > 0 mask is applied as
+// a zero weight: adding w*0 leaves every sum unchanged, and the loads stay
+// unconditional (a branch per node would serialise them).
+template <int NP>
+__device__ __forceinline__ void gather_nodes(const RepDev& r, int D, const float (&W)[3][5], const int (&n0)[3],
+                                             const float4* Gt, const float4* Gc, float qi, bool charged,
+                                             float (&f)[3]) {
+    int zz[NP], yy[NP], xx[NP];
+    zz[0] = n0[2]; yy[0] = n0[1]; xx[0] = n0[0];
+#pragma unroll
+    for (int k = 1; k < NP; ++k) {
+        zz[k] = wrap_next(zz[k - 1], r.Nz);
+        yy[k] = wrap_next(yy[k - 1], r.Ny);
+        xx[k] = wrap_next(xx[k - 1], r.Nx);
+    }
+    const int nzl = D == 3 ? NP : 1;
+    float fc[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int iz = 0; iz < NP; ++iz) {
+        if (iz >= nzl) break;
+#pragma unroll
+        for (int iy = 0; iy < NP; ++iy) {
+            const int rowb = (zz[iz] * r.Ny + yy[iy]) * r.Nx;
+#pragma unroll
+            for (int ix = 0; ix < NP; ++ix) {
+                const float w = D == 3 ? W[0][ix] * W[1][iy] * W[2][iz] : W[0][ix] * W[1][iy];
+                const float4 g = gld(Gt + rowb + xx[ix]);
+                const float m = g.w > 0.f ? w : 0.f;
+                f[0] += m * g.x; f[1] += m * g.y; f[2] += m * g.z;
+                if (charged) {
+                    const float4 e = gld(Gc + rowb + xx[ix]);
+                    const float wq = w * qi;
+                    fc[0] += wq * e.x; fc[1] += wq * e.y; fc[2] += wq * e.z;
+                }
+            }
+        }
+    }
+    for (int d = 0; d < D; ++d) f[d] += fc[d];
+}
+
+__device__ __forceinline__ void gather_nodes_any(const RepDev& r, int D, int P, const float (&W)[3][5], const int (&n0)[3],
+                                              const float4* Gt, const float4* Gc, float qi, bool charged,
+                                              float (&f)[3]) {
+    float fc[3] = {0.f, 0.f, 0.f};
+    const int nz = D == 3 ? P + 1 : 1;
+    for (int iz = 0, nzz = n0[2]; iz < nz; ++iz, nzz = wrap_next(nzz, r.Nz)) {
+        for (int iy = 0, nyy = n0[1]; iy <= P; ++iy, nyy = wrap_next(nyy, r.Ny)) {
+            const int rowb = (nzz * r.Ny + nyy) * r.Nx;
+            for (int ix = 0, nxx = n0[0]; ix <= P; ++ix, nxx = wrap_next(nxx, r.Nx)) {
+                const float w = D == 3 ? W[0][ix] * W[1][iy] * W[2][iz] : W[0][ix] * W[1][iy];
+                const float4 g = gld(Gt + rowb + nxx);
+                const float m = g.w > 0.f ? w : 0.f;
+                f[0] += m * g.x; f[1] += m * g.y; f[2] += m * g.z;
+                if (charged) {
+                    const float4 e = gld(Gc + rowb + nxx);
+                    const float wq = w * qi;
+                    fc[0] += wq * e.x; fc[1] += wq * e.y; fc[2] += wq * e.z;
+                }
+            }
+        }
+    }
+    for (int d = 0; d < D; ++d) f[d] += fc[d];
+}
+
 template <bool INTEGRATE>
 __global__ void __launch_bounds__(256) k_particles(const RepDev* __restrict__ reps,
                                                    const WorkItem* __restrict__ wmap,
@@ -321,7 +388,7 @@ __global__ void __launch_bounds__(256) k_particles(const RepDev* __restrict__ re
     const int i = wi.blk * 256 + threadIdx.x;
     if (i >= r.ns) return;
     const int D = r.dim, P = r.P;
-    const float* __restrict__ x = r.X[r.step & 1];
+    gmem<const float> x = G((const float*)r.X[r.step & 1]);
     float xi[3] = {0.f, 0.f, 0.f};
     for (int d = 0; d < D; ++d) xi[d] = x[i * D + d];
     float W[3][5];
@@ -331,35 +398,24 @@ __global__ void __launch_bounds__(256) k_particles(const RepDev* __restrict__ re
         int N = d == 0 ? r.Nx : (d == 1 ? r.Ny : r.Nz);
         n0[d] = stencil_1d(xi[d], r.dx[d], N, P, W[d]);
     }
-    const int t = r.tp[i];
-    const float4* __restrict__ G = r.G4 + (size_t)t * r.M;
+    const int t = G(r.tp)[i];
     const bool charged = r.q != nullptr;
-    const float qi = charged ? r.q[i] : 0.f;
-    const float4* __restrict__ Gc = r.G4 + (size_t)r.ntypes * r.M;
-    float f[3] = {0.f, 0.f, 0.f}, fc[3] = {0.f, 0.f, 0.f};
-    const int nz = D == 3 ? P + 1 : 1;
-    for (int iz = 0, nzz = n0[2]; iz < nz; ++iz, nzz = wrap_next(nzz, r.Nz)) {
-        for (int iy = 0, nyy = n0[1]; iy <= P; ++iy, nyy = wrap_next(nyy, r.Ny)) {
-            const int rowb = (nzz * r.Ny + nyy) * r.Nx;
-            for (int ix = 0, nxx = n0[0]; ix <= P; ++ix, nxx = wrap_next(nxx, r.Nx)) {
-                const float w = D == 3 ? W[0][ix] * W[1][iy] * W[2][iz] : W[0][ix] * W[1][iy];
-                const float4 g = G[rowb + nxx];
-                if (g.w > 0.f) { f[0] += w * g.x; f[1] += w * g.y; f[2] += w * g.z; }
-                if (charged) {
-                    const float4 e = Gc[rowb + nxx];
-                    const float wq = w * qi;
-                    fc[0] += wq * e.x; fc[1] += wq * e.y; fc[2] += wq * e.z;
-                }
-            }
-        }
-    }
-    for (int d = 0; d < D; ++d) f[d] += fc[d];
+    const float qi = charged ? G(r.q)[i] : 0.f;
+    const float4* Gt = r.G4 + (size_t)t * r.M;
+    const float4* Gc = r.G4 + (size_t)r.ntypes * r.M;
+    float f[3] = {0.f, 0.f, 0.f};
+    // P = 1 (the default order) and P = 0 fully unrolled; higher orders loop (fully
+    // unrolling 27-125 nodes would cost every order its registers)
+    if (P == 1) gather_nodes<2>(r, D, W, n0, Gt, Gc, qi, charged, f);
+    else if (P == 0) gather_nodes<1>(r, D, W, n0, Gt, Gc, qi, charged, f);
+    else gather_nodes_any(r, D, P, W, n0, Gt, Gc, qi, charged, f);
     // bonds: every bond evaluated from both ends, no atomics
     float lf[3] = {0.f, 0.f, 0.f};
+    gmem<const int> bj = G(r.bond_j), bt_ = G(r.bond_t);
     for (int b = 0; b < 3; ++b) {
-        const int j = r.bond_j[i * 3 + b];
+        const int j = bj[i * 3 + b];
         if (j < 0) break;
-        const int bt = r.bond_t[i * 3 + b];
+        const int bt = bt_[i * 3 + b];
         float dr[3] = {0.f, 0.f, 0.f}, r2 = 0.f;
         for (int d = 0; d < D; ++d) {
             dr[d] = xi[d] - x[j * D + d];
@@ -370,23 +426,23 @@ __global__ void __launch_bounds__(256) k_particles(const RepDev* __restrict__ re
         float mf = 0.f;
         if (r2 > 1.0e-4f) {
             const float rr = sqrtf(r2);
-            mf = 2.0f * r.bond_k[bt] * (rr - r.bond_r0[bt]) / rr;
+            mf = 2.0f * G(r.bond_k)[bt] * (rr - G(r.bond_r0)[bt]) / rr;
         }
         for (int d = 0; d < D; ++d) lf[d] -= mf * dr[d];
     }
     for (int d = 0; d < D; ++d) f[d] += lf[d];
     if (write_forces && r.fout)
-        for (int d = 0; d < D; ++d) r.fout[i * D + d] = f[d];
+        for (int d = 0; d < D; ++d) G(r.fout)[i * D + d] = f[d];
     if (!INTEGRATE) return;
-    float* __restrict__ xn = r.X[(r.step + 1) & 1];
-    const int kind = r.ikind[i];
+    gmem<float> xn = G(r.X[(r.step + 1) & 1]);
+    const int kind = G(r.ikind)[i];
     if (kind == 0) {
         for (int d = 0; d < D; ++d) xn[i * D + d] = xi[d];
         return;
     }
     const uint32_t stepc = (uint32_t)(r.step + 1);
     float z[4];
-    const uint32_t site = (uint32_t)r.pid[i];   // original site id: slot order does not matter
+    const uint32_t site = (uint32_t)G(r.pid)[i];   // original site id: slot order does not matter
     normals4(r.seed, site, stepc, z);
     if (kind == 1) {
         for (int d = 0; d < D; ++d) {
@@ -405,11 +461,12 @@ __global__ void __launch_bounds__(256) k_particles(const RepDev* __restrict__ re
         // XO holds the unwrapped displacement x - xo, which removes the reference's
         // minimum-image unwrap of xo and keeps FP32 rounding off the large terms.
         const float a = r.gjf_a[t], c1 = r.gjf_c1[t], c2 = r.gjf_c2[t], nm = r.gjf_noise[t];
+        gmem<float> xo = G(r.XO);
         for (int d = 0; d < D; ++d) {
             const float nn = nm * z[d], no = nm * zo[d];
-            const float dn = a * r.XO[i * D + d] + c1 * f[d] + c2 * (nn + no);
+            const float dn = a * xo[i * D + d] + c1 * f[d] + c2 * (nn + no);
             float v = xi[d] + dn;
-            r.XO[i * D + d] = dn;
+            xo[i * D + d] = dn;
             if (v > r.L[d]) v -= r.L[d];
             else if (v < 0.0f) v += r.L[d];
             xn[i * D + d] = v;

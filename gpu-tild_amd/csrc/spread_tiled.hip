@@ -23,8 +23,9 @@
 
 namespace tild {
 
-__device__ inline int tile_bucket(const RepDev& r, const float* x, int i) {
+__device__ inline int tile_bucket(const RepDev& r, const float* xp, int i) {
     const int D = r.dim, P = r.P;
+    gmem<const float> x = G(xp);
     float W[5];
     const int sy = stencil_1d(x[i * D + 1], r.dx[1], r.Ny, P, W);
     const int sz = D == 3 ? stencil_1d(x[i * D + 2], r.dx[2], r.Nz, P, W) : 0;
@@ -58,7 +59,7 @@ __global__ void __launch_bounds__(256) k_tile_count(const RepDev* __restrict__ r
         if (b == bl) slot = base + __popcll(same & below);
         todo &= ~same;
     }
-    if (valid) r.tkey[i] = make_int2(b, slot);
+    if (valid) G((long long*)r.tkey)[i] = (long long)(((unsigned long long)(unsigned)slot << 32) | (unsigned)b);
 }
 
 // Exclusive scan of the bucket counts; clears the counts for the next step.
@@ -97,11 +98,84 @@ __global__ void __launch_bounds__(256) k_tile_scatter(const RepDev* __restrict__
     const int i = wi.blk * 256 + threadIdx.x;
     if (i >= r.ns) return;
     const int D = r.dim;
-    const float* x = r.X[r.step & 1];
-    const int2 k = r.tkey[i];
-    const int pos = r.toff[k.x] + k.y;
-    r.trec[pos] = make_float4(x[i * D], x[i * D + 1], D == 3 ? x[i * D + 2] : 0.f, __int_as_float(r.tp[i]));
-    if (r.trecq) r.trecq[pos] = r.q[i];
+    gmem<const float> x = G((const float*)r.X[r.step & 1]);
+    const long long kk = G((const long long*)r.tkey)[i];
+    const int kb = (int)(unsigned)(kk & 0xffffffffll), ks = (int)(kk >> 32);
+    const int pos = G(r.toff)[kb] + ks;
+    gst(r.trec + pos, make_float4(x[i * D], x[i * D + 1], D == 3 ? x[i * D + 2] : 0.f, __int_as_float(G(r.tp)[i])));
+    if (r.trecq) G(r.trecq)[pos] = G(r.q)[i];
+}
+
+// Deposit one record into the tile's fixed-point LDS rows [f][lz][ly][x]; stencil
+// rows outside the tile belong to a neighbouring tile and are skipped.  NP = P + 1
+// points per axis, unrolled for the default order.
+template <int NP>
+__device__ __forceinline__ void deposit_record(const RepDev& r, const float4 rec, float qi,
+                                               unsigned long long* patch, int y0, int z0, int By, int Bz) {
+    const int D = r.dim, Nx = r.Nx, T = r.ntypes;
+    const int tp = __float_as_int(rec.w);
+    float W[3][5];
+    int n0[3];
+    n0[0] = stencil_1d(rec.x, r.dx[0], Nx, NP - 1, W[0]);
+    n0[1] = stencil_1d(rec.y, r.dx[1], r.Ny, NP - 1, W[1]);
+    if (D == 3) n0[2] = stencil_1d(rec.z, r.dx[2], r.Nz, NP - 1, W[2]);
+    else { n0[2] = 0; W[2][0] = 1.f; }
+    int xx[NP];
+    xx[0] = n0[0];
+#pragma unroll
+    for (int k = 1; k < NP; ++k) xx[k] = wrap_next(xx[k - 1], Nx);
+    const int nz = D == 3 ? NP : 1;
+    const bool charged = r.trecq != nullptr;
+    int zz = n0[2];
+#pragma unroll
+    for (int iz = 0; iz < NP; ++iz, zz = wrap_next(zz, r.Nz)) {
+        if (iz >= nz) break;
+        const int lz = zz - z0;
+        if (lz < 0 || lz >= Bz) continue;
+        int yy = n0[1];
+#pragma unroll
+        for (int iy = 0; iy < NP; ++iy, yy = wrap_next(yy, r.Ny)) {
+            const int ly = yy - y0;
+            if (ly < 0 || ly >= By) continue;
+            unsigned long long* rowp = patch + ((size_t)(tp * Bz + lz) * By + ly) * Nx;
+            unsigned long long* rowq = patch + ((size_t)(T * Bz + lz) * By + ly) * Nx;
+#pragma unroll
+            for (int ix = 0; ix < NP; ++ix) {
+                const float w = D == 3 ? W[0][ix] * W[1][iy] * W[2][iz] : W[0][ix] * W[1][iy];
+                atomicAdd(rowp + xx[ix], (unsigned long long)__float2ll_rn(w * r.fix_scale));
+                if (charged) atomicAdd(rowq + xx[ix], (unsigned long long)__float2ll_rn((w * qi) * r.fixq_scale));
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void deposit_record_any(const RepDev& r, const float4 rec, float qi,
+                                                   unsigned long long* patch, int y0, int z0, int By, int Bz) {
+    const int D = r.dim, P = r.P, Nx = r.Nx, T = r.ntypes;
+    const int tp = __float_as_int(rec.w);
+    float W[3][5];
+    int n0[3];
+    n0[0] = stencil_1d(rec.x, r.dx[0], Nx, P, W[0]);
+    n0[1] = stencil_1d(rec.y, r.dx[1], r.Ny, P, W[1]);
+    if (D == 3) n0[2] = stencil_1d(rec.z, r.dx[2], r.Nz, P, W[2]);
+    else { n0[2] = 0; W[2][0] = 1.f; }
+    const int nz = D == 3 ? P + 1 : 1;
+    for (int iz = 0, zz = n0[2]; iz < nz; ++iz, zz = wrap_next(zz, r.Nz)) {
+        const int lz = zz - z0;
+        if (lz < 0 || lz >= Bz) continue;
+        for (int iy = 0, yy = n0[1]; iy <= P; ++iy, yy = wrap_next(yy, r.Ny)) {
+            const int ly = yy - y0;
+            if (ly < 0 || ly >= By) continue;
+            unsigned long long* rowp = patch + ((size_t)(tp * Bz + lz) * By + ly) * Nx;
+            unsigned long long* rowq = patch + ((size_t)(T * Bz + lz) * By + ly) * Nx;
+            for (int ix = 0, xx = n0[0]; ix <= P; ++ix, xx = wrap_next(xx, Nx)) {
+                const float w = D == 3 ? W[0][ix] * W[1][iy] * W[2][iz] : W[0][ix] * W[1][iy];
+                atomicAdd(rowp + xx, (unsigned long long)__float2ll_rn(w * r.fix_scale));
+                if (r.trecq)
+                    atomicAdd(rowq + xx, (unsigned long long)__float2ll_rn((w * qi) * r.fixq_scale));
+            }
+        }
+    }
 }
 
 // Deposit one tile's rows and transform them along x.
@@ -125,45 +199,24 @@ __global__ void __launch_bounds__(kFftThreads) k_tile_deposit(const RepDev* __re
         const int tym = (ty + r.nty - 1) % r.nty;
         const int tzm = (tz + r.ntz - 1) % r.ntz;
         const int own = tz * r.nty + ty;
-        ra[nr] = r.toff[own * 4]; rb[nr] = r.toff[own * 4 + 4]; ++nr;
+        gmem<const int> toff = G((const int*)r.toff);
+        ra[nr] = toff[own * 4]; rb[nr] = toff[own * 4 + 4]; ++nr;
         const int ny_ = tz * r.nty + tym;
-        ra[nr] = r.toff[ny_ * 4 + 1]; rb[nr] = r.toff[ny_ * 4 + 3]; ++nr;
+        ra[nr] = toff[ny_ * 4 + 1]; rb[nr] = toff[ny_ * 4 + 3]; ++nr;
         if (D == 3) {
             const int nz_ = tzm * r.nty + ty;
-            ra[nr] = r.toff[nz_ * 4 + 2]; rb[nr] = r.toff[nz_ * 4 + 4]; ++nr;
+            ra[nr] = toff[nz_ * 4 + 2]; rb[nr] = toff[nz_ * 4 + 4]; ++nr;
             const int nyz = tzm * r.nty + tym;
-            ra[nr] = r.toff[nyz * 4 + 2]; rb[nr] = r.toff[nyz * 4 + 3]; ++nr;
+            ra[nr] = toff[nyz * 4 + 2]; rb[nr] = toff[nyz * 4 + 3]; ++nr;
         }
     }
     const int T = r.ntypes;
     for (int s = 0; s < nr; ++s) {
         for (int p = ra[s] + threadIdx.x; p < rb[s]; p += kFftThreads) {
-            const float4 rec = r.trec[p];
-            const int tp = __float_as_int(rec.w);
-            float W[3][5];
-            int n0[3];
-            n0[0] = stencil_1d(rec.x, r.dx[0], Nx, P, W[0]);
-            n0[1] = stencil_1d(rec.y, r.dx[1], r.Ny, P, W[1]);
-            if (D == 3) n0[2] = stencil_1d(rec.z, r.dx[2], r.Nz, P, W[2]);
-            else { n0[2] = 0; W[2][0] = 1.f; }
-            const float qi = r.trecq ? r.trecq[p] : 0.f;
-            const int nz = D == 3 ? P + 1 : 1;
-            for (int iz = 0, zz = n0[2]; iz < nz; ++iz, zz = wrap_next(zz, r.Nz)) {
-                const int lz = zz - z0;
-                if (lz < 0 || lz >= Bz) continue;
-                for (int iy = 0, yy = n0[1]; iy <= P; ++iy, yy = wrap_next(yy, r.Ny)) {
-                    const int ly = yy - y0;
-                    if (ly < 0 || ly >= By) continue;
-                    unsigned long long* rowp = patch + ((size_t)(tp * Bz + lz) * By + ly) * Nx;
-                    unsigned long long* rowq = patch + ((size_t)(T * Bz + lz) * By + ly) * Nx;
-                    for (int ix = 0, xx = n0[0]; ix <= P; ++ix, xx = wrap_next(xx, Nx)) {
-                        const float w = D == 3 ? W[0][ix] * W[1][iy] * W[2][iz] : W[0][ix] * W[1][iy];
-                        atomicAdd(rowp + xx, (unsigned long long)__float2ll_rn(w * r.fix_scale));
-                        if (r.trecq)
-                            atomicAdd(rowq + xx, (unsigned long long)__float2ll_rn((w * qi) * r.fixq_scale));
-                    }
-                }
-            }
+            const float4 rec = gld(r.trec + p);
+            const float qi = r.trecq ? G((const float*)r.trecq)[p] : 0.f;
+            if (P == 1) deposit_record<2>(r, rec, qi, patch, y0, z0, By, Bz);
+            else deposit_record_any(r, rec, qi, patch, y0, z0, By, Bz);
         }
     }
     __syncthreads();
@@ -179,7 +232,7 @@ __global__ void __launch_bounds__(kFftThreads) k_tile_deposit(const RepDev* __re
             const int f = rr / (By * Bz), lz = (rr / By) % Bz, ly = rr % By;
             const double inv = f < T ? r.fix_inv : r.fixq_inv;
             v[h] = (float)((double)(long long)patch[(size_t)rr * Nx + n] * inv);
-            r.rho[(size_t)f * r.M + ((size_t)(z0 + lz) * r.Ny + (y0 + ly)) * Nx + n] = v[h];
+            G(r.rho)[(size_t)f * r.M + ((size_t)(z0 + lz) * r.Ny + (y0 + ly)) * Nx + n] = v[h];
         }
         fbuf[n * ldb + j] = make_float2(v[0], v[1]);
     }
@@ -194,8 +247,8 @@ __global__ void __launch_bounds__(kFftThreads) k_tile_deposit(const RepDev* __re
             if (rr >= rows) break;
             const int f = rr / (By * Bz), lz = (rr / By) % Bz, ly = rr % By;
             const size_t row = ((size_t)f * r.Nz + (z0 + lz)) * r.Ny + (y0 + ly);
-            r.H[row * Nxh + k] = h == 0 ? make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y))
-                                        : make_float2(0.5f * (Zk.y + Zm.y), 0.5f * (Zm.x - Zk.x));
+            gst(r.H + row * Nxh + k, h == 0 ? make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y))
+                                            : make_float2(0.5f * (Zk.y + Zm.y), 0.5f * (Zm.x - Zk.x)));
         }
     }
 }
