@@ -42,6 +42,9 @@ struct RepDev {
     int nl_last;   // batch columns per block, fused last-axis pass
     int nr_c2r;    // rows per block, inverse x pass
     int mid_elems; // complex values per block of the middle-axis passes (0: kFftBudget)
+    int plane_rows;// rows per x chunk of the plane inverse pass (0: pass not used)
+    int plane_fwd; // 1: the tiled deposit writes rho only; x R2C + y FFT run per plane
+    int last_dirs; // directions per inverse batch of the fused last-axis pass (1..dim)
     // particle state
     float* X[2];            // ns*dim, double-buffered by step parity
     float* XO;              // ns*dim, GJF previous position
@@ -71,7 +74,9 @@ struct RepDev {
     float2* H;                 // nf*Mh half spectra
     float2* Gk;                // nf*dim*Mh half spectra of the gathered fields
     float4* G4;                // nf*M (g_x, g_y, g_z, rho) records for the gather
-    const float* Ctab;         // (T(T+1)/2)*Mh coupling tables (include 1/M), + charge table
+    const float* Ctab;         // coupling tables C_IJ/M (I <= J) + charge table, blocked as
+                               // [column block][last-axis index][row][nl_last columns]
+    int ctab_rows;             // rows of Ctab: T(T+1)/2 (+1 with charges)
     const float* FPtab;        // T*dim*M net static field-phase forces per type (or null)
     int ctab_charge;           // row of the charge kernel in Ctab (or -1)
     // property tables

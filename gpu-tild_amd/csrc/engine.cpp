@@ -132,6 +132,7 @@ int Batch::init(std::string& err) {
     if (device_ < 0 || device_ >= ndev) { err = "no HIP device " + std::to_string(device_); return -TILD_ERR_DEVICE; }
     HIPCHK(hipSetDevice(device_));
     HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    HIPCHK(configure_kernels());
     return 0;
 }
 
@@ -202,11 +203,12 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
     const int npairs_x = (h.nf * rows + 1) / 2;
     h.nl_x = std::max(1, std::min(kFftBudget / h.Nx, npairs_x));
     const int la = D - 1, Nla = la == 2 ? h.Nz : h.Ny, Q = la == 2 ? h.Ny * h.Nxh : h.Nxh;
-    if (Nla * h.nf > kFftBudget || 3 * fft_buf_bytes(Nla, h.nf) > kMaxLds) { err = "grid too large for the fused k-space pass"; return -TILD_ERR_UNSUPPORTED; }
+    h.last_dirs = tune_.last_dirs > 0 ? std::min(tune_.last_dirs, D) : 1;
+    if (Nla * h.nf > kFftBudget || last_lds_bytes(Nla, h.nf, h.last_dirs, 1) > kPlaneLds) { err = "grid too large for the fused k-space pass"; return -TILD_ERR_UNSUPPORTED; }
     // about 1,280 complex values per field set per block measured best on the
     // 80x80x160 melt (fewer, fuller blocks lose to the per-block latency chain)
     int nl = std::max(1, std::min(kFftBudget, 1280) / (Nla * h.nf));
-    while (nl > 1 && 3 * fft_buf_bytes(Nla, h.nf * nl) > kMaxLds) --nl;
+    while (nl > 1 && last_lds_bytes(Nla, h.nf, h.last_dirs, nl) > kPlaneLds) --nl;
     h.nl_last = std::min(nl, Q);
     const int nc = (h.nf * D + 1) / 2;
     if (nc * h.Nx > kFftBudget) { err = "grid too large for the inverse x pass"; return -TILD_ERR_UNSUPPORTED; }
@@ -216,6 +218,19 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
     if (tune_.last_lines > 0) h.nl_last = std::min(h.nl_last, tune_.last_lines);
     if (tune_.c2r_rows > 0) h.nr_c2r = std::min(h.nr_c2r, tune_.c2r_rows);
     if (tune_.mid_elems > 0) h.mid_elems = std::min(kFftBudget, tune_.mid_elems);
+    // fused y-inverse + x C2R per (field, z) plane when the D direction spectra of a
+    // plane and their ping-pong partner fit one block's LDS
+    // (opt-in: at one 157 KiB block per CU it measured slower than the two passes on
+    // the 80x80x160 melt)
+    h.plane_rows = 0;
+    if (D == 3 && tune_.plane > 0) {
+        const size_t ld1 = (size_t)((3 * h.Nxh) | 1);
+        if (2 * (size_t)h.Ny * ld1 * sizeof(float2) <= kPlaneLds) {
+            int cr = h.Ny;
+            while (cr > 0 && 2 * (size_t)h.Nx * (size_t)((2 * cr) | 1) > (size_t)h.Ny * ld1) --cr;
+            h.plane_rows = cr;
+        }
+    }
     h.nl_mid = mid_geom(h, 1, h.nf).nl;
 
     // particle state
@@ -365,6 +380,13 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
         R.rb.tmp = tmp;
     }
 
+    // forward plane pass: tiled 3D replicas with even Ny and even tile height (the
+    // row pairing (y, y+1) of the tiled x transform), planes within one block's LDS
+    h.plane_fwd = 0;
+    if (D == 3 && h.spread_mode == 2 && tune_.plane >= 0 && h.Ny % 2 == 0 && h.By % 2 == 0 &&
+        plane_r2c_lds(h) <= kPlaneLds)
+        h.plane_fwd = 1;
+
     int id = -1;
     for (int k = 0; k < (int)reps_.size(); ++k) if (!reps_[k]) { id = k; break; }
     if (id < 0) { id = (int)reps_.size(); reps_.emplace_back(); }
@@ -472,18 +494,23 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
         {
             MidGeom g = mid_geom(h, 1, h.nf);
             add(PH_PY, g.nouter * g.nchunk, lds_bytes(g.N, g.nl));
-            if (D == 3) add(PH_MIDY_H, g.nouter * g.nchunk, lds_bytes(g.N, g.nl));
+            if (D == 3 && h.plane_fwd) add(PH_PLANEF, h.nf * h.Nz, plane_r2c_lds(h));
+            else if (D == 3) add(PH_MIDY_H, g.nouter * g.nchunk, lds_bytes(g.N, g.nl));
         }
         if (D == 3) {
-            MidGeom g = mid_geom(h, 1, h.nf * D);
-            add(PH_MIDY_G, g.nouter * g.nchunk, lds_bytes(g.N, g.nl));
+            if (h.plane_rows > 0) {
+                add(PH_PLANE, h.nf * h.Nz, 2 * (size_t)h.Ny * (size_t)((3 * h.Nxh) | 1) * sizeof(float2));
+            } else {
+                MidGeom g = mid_geom(h, 1, h.nf * D);
+                add(PH_MIDY_G, g.nouter * g.nchunk, lds_bytes(g.N, g.nl));
+            }
             MidGeom gz = mid_geom(h, 2, h.nf);
             add(PH_MIDZ_H, gz.nouter * gz.nchunk, lds_bytes(gz.N, gz.nl));
         }
         const int la = D - 1, Nla = la == 2 ? h.Nz : h.Ny, Q = la == 2 ? h.Ny * h.Nxh : h.Nxh;
-        add(PH_LAST, (Q + h.nl_last - 1) / h.nl_last, 3 * fft_buf_bytes(Nla, h.nf * h.nl_last));
+        add(PH_LAST, (Q + h.nl_last - 1) / h.nl_last, last_lds_bytes(Nla, h.nf, h.last_dirs, h.nl_last));
         const int nc = (h.nf * D + 1) / 2;
-        add(PH_C2R, (rows + h.nr_c2r - 1) / h.nr_c2r, lds_bytes(h.Nx, h.nr_c2r * nc));
+        if (h.plane_rows <= 0) add(PH_C2R, (rows + h.nr_c2r - 1) / h.nr_c2r, lds_bytes(h.Nx, h.nr_c2r * nc));
         add(PH_KPROPS, h.nchunk > 0 ? h.nchunk : 1, 0);
     }
     // Workgroups go to the 8 XCDs round-robin (launch block w runs on XCD w % 8).  The
@@ -503,7 +530,7 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
     for (int p = 0; p < PH_COUNT; ++p) {
         if (maps[p].dev) { (void)hipFree(maps[p].dev); maps[p].dev = nullptr; }
         maps[p].n = (int)maps[p].host.size();
-        if (maps[p].lds > kMaxLds) { err = "pass needs more LDS than a block may use"; return -TILD_ERR_UNSUPPORTED; }
+        if (maps[p].lds > ((p == PH_PLANE || p == PH_PLANEF || p == PH_LAST) ? kPlaneLds : (size_t)kMaxLds)) { err = "pass needs more LDS than a block may use"; return -TILD_ERR_UNSUPPORTED; }
         if (maps[p].n) {
             HIPCHK(hipMalloc((void**)&maps[p].dev, maps[p].n * sizeof(WorkItem)));
             HIPCHK(hipMemcpy(maps[p].dev, maps[p].host.data(), maps[p].n * sizeof(WorkItem), hipMemcpyHostToDevice));
@@ -599,10 +626,26 @@ int Batch::build_tables(Replica& R, const System& S, std::string& err) {
             eltab[idx] = (float)(g1 / M);
         }
     }
+    // The fused last-axis pass reads the coupling rows of nl_last adjacent columns for
+    // every last-axis index; store them in exactly that order, block by block
+    // ([column block][last-axis index][row][column]), so each block streams one
+    // contiguous run instead of nl_last-wide pieces of every row.
+    const int la = h.dim - 1, Nla = la == 2 ? h.Nz : h.Ny;
+    const size_t Q = la == 2 ? (size_t)h.Ny * h.Nxh : (size_t)h.Nxh;
+    const size_t nl = (size_t)h.nl_last, nqb = (Q + nl - 1) / nl;
+    std::vector<float> cblk(nqb * Nla * nrows * nl, 0.0f);
+    for (size_t qb = 0; qb < nqb; ++qb)
+        for (int n = 0; n < Nla; ++n)
+            for (int p = 0; p < nrows; ++p)
+                for (size_t b = 0; b < nl; ++b) {
+                    const size_t q = qb * nl + b;
+                    if (q < Q) cblk[((qb * Nla + n) * nrows + p) * nl + b] = ctab[(size_t)p * Mh + (size_t)n * Q + q];
+                }
+    h.ctab_rows = nrows;
     float *d_ctab, *d_etab, *d_eltab = nullptr;
-    if (int rc = dalloc(R, &d_ctab, ctab.size(), err)) return rc;
+    if (int rc = dalloc(R, &d_ctab, cblk.size(), err)) return rc;
     if (int rc = dalloc(R, &d_etab, etab.size(), err)) return rc;
-    HIPCHK(hipMemcpy(d_ctab, ctab.data(), ctab.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_ctab, cblk.data(), cblk.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d_etab, etab.data(), etab.size() * 4, hipMemcpyHostToDevice));
     if (S.charges.enabled) {
         if (int rc = dalloc(R, &d_eltab, Mh, err)) return rc;
@@ -729,10 +772,12 @@ int Batch::enqueue_step(SetMaps& m, bool integrate, bool write_forces, std::stri
     HIPCHK(launch_tile_scan(ctx(PH_SCAN)));
     HIPCHK(launch_tile_scatter(ctx(PH_TPART)));
     HIPCHK(launch_tile_deposit(ctx(PH_TILE)));
+    HIPCHK(launch_plane_r2c(ctx(PH_PLANEF)));
     HIPCHK(launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr));
     HIPCHK(launch_last_fused(ctx(PH_LAST), 0));
     HIPCHK(launch_mid(ctx(PH_MIDY_G), 1, 1, 0, 1, nullptr));
     HIPCHK(launch_x_c2r(ctx(PH_C2R), true, 0, nullptr, nullptr));
+    HIPCHK(launch_plane_c2r(ctx(PH_PLANE)));
     HIPCHK(launch_particles(ctx(PH_PART), integrate, write_forces ? 1 : 0));
     if (integrate) HIPCHK(launch_advance(stream_, d_reps_, m.d_ids, m.nids));
     return 0;
@@ -822,6 +867,7 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
         if (e0 == hipSuccess) e0 = launch_x_r2c(ctx(PH_XA), true, 0, nullptr, nullptr);
         if (e0 == hipSuccess) e0 = launch_tile_deposit(ctx(PH_TILE));
         rec(2);
+        if (e0 == hipSuccess) e0 = launch_plane_r2c(ctx(PH_PLANEF));
         if (e0 == hipSuccess) e0 = launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr);
         rec(3);
         if (e0 == hipSuccess) e0 = launch_last_fused(ctx(PH_LAST), 0);
@@ -829,6 +875,7 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
         if (e0 == hipSuccess) e0 = launch_mid(ctx(PH_MIDY_G), 1, 1, 0, 1, nullptr);
         rec(5);
         if (e0 == hipSuccess) e0 = launch_x_c2r(ctx(PH_C2R), true, 0, nullptr, nullptr);
+        if (e0 == hipSuccess) e0 = launch_plane_c2r(ctx(PH_PLANE));
         rec(6);
         if (e0 == hipSuccess) e0 = launch_particles(ctx(PH_PART), true, 0);
         rec(7);

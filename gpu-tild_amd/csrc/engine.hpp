@@ -26,8 +26,11 @@ constexpr int kProfPhases = 8;   // spread, x_r2c, y_fwd, last_fused, y_inv, x_c
 // LDS-tiled spread; XP: x-axis R2C of stored densities (properties).
 // MIDY_H/MIDY_G: y-axis passes of the step (3D replicas); PY: forward y-axis pass
 // of stored spectra for properties (every replica: it is the last axis in 2D).
+// PLANE: y-inverse + x C2R of 3D replicas whose planes fit one block's LDS (they
+// skip MIDY_G and C2R); PLANEF: x R2C + y FFT of tiled 3D replicas with even Ny
+// (their deposit writes densities only and they skip MIDY_H).
 enum Phase { PH_PART = 0, PH_SPREADA, PH_XA, PH_XP, PH_TPART, PH_SCAN, PH_TILE, PH_MIDY_H, PH_MIDY_G,
-             PH_MIDZ_H, PH_PY, PH_LAST, PH_C2R, PH_KPROPS, PH_COUNT };
+             PH_MIDZ_H, PH_PY, PH_LAST, PH_C2R, PH_PLANE, PH_PLANEF, PH_KPROPS, PH_COUNT };
 
 struct PhaseMap {
     std::vector<WorkItem> host;
@@ -70,7 +73,7 @@ public:
     void set_reorder(int every) { reorder_every_ = every; }
     // FFT block geometry of replicas added from now on (0 = default): complex values
     // per middle-axis block, lines per fused last-axis block, rows per inverse-x block
-    struct FftTuning { int mid_elems = 0, last_lines = 0, c2r_rows = 0; };
+    struct FftTuning { int mid_elems = 0, last_lines = 0, c2r_rows = 0, plane = 0, last_dirs = 0; };
     void set_fft_tuning(const FftTuning& t) { tune_ = t; }
     ~Batch();
     int init(std::string& err);

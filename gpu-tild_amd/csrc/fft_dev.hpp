@@ -110,11 +110,11 @@ template <> struct Dft<7> {
     }
 };
 
-template <int R>
+template <int R, int NT>
 __device__ __forceinline__ void fft_stage(const float2* __restrict__ src, float2* __restrict__ dst, int ldb, int nb,
                                  int N, int p, const float2* __restrict__ tw) {
     const int m = N / R, nbf = m * nb;
-    for (int task = threadIdx.x; task < nbf; task += kFftThreads) {
+    for (int task = threadIdx.x; task < nbf; task += NT) {
         const int b = task % nb, i = task / nb, k = i % p;
         float2 v[R];
 #pragma unroll
@@ -136,6 +136,8 @@ __device__ __forceinline__ void fft_stage(const float2* __restrict__ src, float2
 // (A fully written).  Returns the buffer holding the result (A or B); every stage
 // ends with a barrier, so the result is visible to the whole block on return.
 // Always inlined: the buffers are then known to be LDS (ds_* instructions, not flat).
+// NT = threads of the block.
+template <int NT = kFftThreads>
 __device__ __forceinline__ float2* lds_fft(float2* A, float2* B, int ldb, int nb, const AxisPlan& pl) {
     float2* src = A;
     float2* dst = B;
@@ -144,12 +146,12 @@ __device__ __forceinline__ float2* lds_fft(float2* A, float2* B, int ldb, int nb
         const int R = pl.rad[s];
         const float2* tw = pl.tw + pl.tw_off[s];
         switch (R) {
-            case 2: fft_stage<2>(src, dst, ldb, nb, pl.N, p, tw); break;
-            case 3: fft_stage<3>(src, dst, ldb, nb, pl.N, p, tw); break;
-            case 4: fft_stage<4>(src, dst, ldb, nb, pl.N, p, tw); break;
-            case 5: fft_stage<5>(src, dst, ldb, nb, pl.N, p, tw); break;
-            case 7: fft_stage<7>(src, dst, ldb, nb, pl.N, p, tw); break;
-            default: fft_stage<8>(src, dst, ldb, nb, pl.N, p, tw); break;
+            case 2: fft_stage<2, NT>(src, dst, ldb, nb, pl.N, p, tw); break;
+            case 3: fft_stage<3, NT>(src, dst, ldb, nb, pl.N, p, tw); break;
+            case 4: fft_stage<4, NT>(src, dst, ldb, nb, pl.N, p, tw); break;
+            case 5: fft_stage<5, NT>(src, dst, ldb, nb, pl.N, p, tw); break;
+            case 7: fft_stage<7, NT>(src, dst, ldb, nb, pl.N, p, tw); break;
+            default: fft_stage<8, NT>(src, dst, ldb, nb, pl.N, p, tw); break;
         }
         __syncthreads();
         float2* t = src; src = dst; dst = t;

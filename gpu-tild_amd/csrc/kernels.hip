@@ -175,64 +175,73 @@ __global__ void __launch_bounds__(kFftThreads) k_last_fused(const RepDev* __rest
     const int q0 = wi.blk * r.nl_last;
     const int nb = min(r.nl_last, Q - q0);
     const int nlines = nf * nb;
-    const int ldb = nlines | 1;
-    float2* A0 = lds;
-    float2* B0 = lds + (size_t)N * ldb;
-    float2* C0 = lds + 2 * (size_t)N * ldb;
+    const int ldb = nlines | 1;                          // forward / coupled lines
+    const int D = r.dim, DB = r.last_dirs;               // directions per inverse batch
+    const int ldg = (nlines * DB) | 1;                   // inverse lines of one batch
+    // LDS: Wb [N x ldb] coupled fields, then two [N x ldg] buffers that first hold the
+    // forward transform's ping-pong pair and then the batched inverse transform's
+    float2* Wb = lds;
+    float2* E0 = lds + (size_t)N * ldb;
+    float2* E1 = E0 + (size_t)N * ldg;
     for (int task = threadIdx.x; task < nlines * N; task += kFftThreads) {
         const int b = task % nb, rest = task / nb, f = rest % nf, n = rest / nf;
-        A0[n * ldb + f * nb + b] = gld(r.H + (size_t)f * r.Mh + (size_t)n * Q + q0 + b);
+        E0[n * ldb + f * nb + b] = gld(r.H + (size_t)f * r.Mh + (size_t)n * Q + q0 + b);
     }
     __syncthreads();
-    float2* A = lds_fft(A0, B0, ldb, nlines, r.ax[la]);   // spectra rho_f(k)
-    float2* Wb = (A == A0) ? B0 : A0;                      // coupled fields w_f(k)
-    // coupling w_I = sum_J C_IJ rho_J (C symmetric, rows I <= J), charge w_c = G_c rho_c
+    const float2* A = lds_fft(E0, E1, ldb, nlines, r.ax[la]);   // spectra rho_f(k)
+    // coupling w_I = sum_J C_IJ rho_J (C symmetric, rows I <= J), charge w_c = G_c rho_c;
+    // this block's table rows are one contiguous run
+    const int nrows = r.ctab_rows, nlc = r.nl_last;
+    gmem<const float> cb = G(r.Ctab) + (size_t)wi.blk * N * nrows * nlc;
     for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
         const int b = task % nb, n = task / nb;
-        const size_t idx = (size_t)n * Q + q0 + b;
         const int row = n * ldb + b;
         for (int I = 0; I < T; ++I) {
             float2 acc = make_float2(0.f, 0.f);
             for (int J = 0; J < T; ++J) {
                 const int lo = I < J ? I : J, hi = I < J ? J : I;
                 const int pidx = lo * T - lo * (lo - 1) / 2 + (hi - lo);
-                const float c = G(r.Ctab)[(size_t)pidx * r.Mh + idx];
+                const float c = cb[((size_t)n * nrows + pidx) * nlc + b];
                 const float2 rh = A[row + J * nb];
                 acc.x += c * rh.x; acc.y += c * rh.y;
             }
             Wb[row + I * nb] = acc;
         }
         if (nf > T) {
-            const float c = G(r.Ctab)[(size_t)r.ctab_charge * r.Mh + idx];
+            const float c = cb[((size_t)n * nrows + r.ctab_charge) * nlc + b];
             const float2 rh = A[row + T * nb];
             Wb[row + T * nb] = make_float2(c * rh.x, c * rh.y);
         }
     }
     __syncthreads();
-    float2* B = A;                                         // work line of direction d
-    float2* C = C0;                                        // its ping-pong partner
-    for (int d = 0; d < r.dim; ++d) {
+    // the lines -i k_d w(k) (k_d = 0 on the d-Nyquist plane), DB directions per batch
+    for (int d0 = 0; d0 < D; d0 += DB) {
+        const int nd = min(DB, D - d0), nlg = nlines * nd;
         for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
             const int b = task % nb, n = task / nb;
             const int q = q0 + b;
-            int sidx, Nd;
-            if (d == 0) { sidx = q % r.Nxh; Nd = r.Nx; }
-            else if (d == la) { sidx = n; Nd = N; }
-            else { sidx = q / r.Nxh; Nd = r.Ny; }
-            float kd;
-            if (2 * sidx == Nd) kd = 0.f;
-            else kd = r.kfac[d] * (float)(2 * sidx < Nd ? sidx : sidx - Nd);
-            for (int f = 0; f < nf; ++f) {
-                float2 w = Wb[n * ldb + f * nb + b];
-                // conj(-i k w) = (k w.y, k w.x): the inverse is taken as conj(FFT(conj))
-                B[n * ldb + f * nb + b] = make_float2(kd * w.y, kd * w.x);
+            for (int dd = 0; dd < nd; ++dd) {
+                const int d = d0 + dd;
+                int sidx, Nd;
+                if (d == 0) { sidx = q % r.Nxh; Nd = r.Nx; }
+                else if (d == la) { sidx = n; Nd = N; }
+                else { sidx = q / r.Nxh; Nd = r.Ny; }
+                float kd;
+                if (2 * sidx == Nd) kd = 0.f;
+                else kd = r.kfac[d] * (float)(2 * sidx < Nd ? sidx : sidx - Nd);
+                for (int f = 0; f < nf; ++f) {
+                    const float2 w = Wb[n * ldb + f * nb + b];
+                    // conj(-i k w) = (k w.y, k w.x): the inverse is taken as conj(FFT(conj))
+                    E0[n * ldg + (f * nd + dd) * nb + b] = make_float2(kd * w.y, kd * w.x);
+                }
             }
         }
         __syncthreads();
-        const float2* res = lds_fft(B, C, ldb, nlines, r.ax[la]);
-        for (int task = threadIdx.x; task < nlines * N; task += kFftThreads) {
-            const int b = task % nb, rest = task / nb, f = rest % nf, n = rest / nf;
-            gst(r.Gk + (size_t)(f * r.dim + d) * r.Mh + (size_t)n * Q + q0 + b, conjf2(res[n * ldb + f * nb + b]));
+        const float2* res = lds_fft(E0, E1, ldg, nlg, r.ax[la]);
+        for (int task = threadIdx.x; task < nlg * N; task += kFftThreads) {
+            const int b = task % nb, rest = task / nb, fd = rest % (nf * nd), n = rest / (nf * nd);
+            const int f = fd / nd, d = d0 + fd % nd;
+            gst(r.Gk + (size_t)(f * D + d) * r.Mh + (size_t)n * Q + q0 + b, conjf2(res[n * ldg + fd * nb + b]));
         }
         __syncthreads();
     }
@@ -305,6 +314,117 @@ __global__ void __launch_bounds__(kFftThreads) k_x_c2r(const RepDev* __restrict_
             const float2 z = res[x * ldb + rr * nc + line / 2];
             G(out_override)[((size_t)line * rows_per_field + row0 + rr) * N + x] = (line & 1) ? -z.y : z.x;
         }
+    }
+}
+
+// ------------------------------------------------------------------ plane forward
+// x-axis R2C and y-axis FFT of one (field, z) density plane in a single block (3D,
+// even Ny): rows (y, y+1) share one complex x transform -- the pairing of the tiled
+// deposit with even tile heights, so H is bit-identical to deposit + separate y pass
+// -- and the unpacked half spectra are transformed along y in LDS before one write.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_plane_r2c(const RepDev* __restrict__ reps,
+                                                  const WorkItem* __restrict__ wmap) {
+    extern __shared__ float2 lds[];
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    const int Nx = r.Nx, Ny = r.Ny, Nxh = r.Nxh;
+    const int z = wi.blk % r.Nz, f = wi.blk / r.Nz;
+    const int np = Ny / 2, ldp = np | 1, ldc = Nxh | 1;
+    const size_t bsz = max((size_t)Nx * ldp, (size_t)Ny * ldc);
+    float2* A = lds;
+    float2* B = lds + bsz;
+    gmem<const float> rho = G((const float*)r.rho) + (size_t)f * r.M + (size_t)z * Ny * Nx;
+    for (int task = threadIdx.x; task < np * Nx; task += NT) {
+        const int n = task % Nx, j = task / Nx;
+        A[n * ldp + j] = make_float2(rho[(size_t)(2 * j) * Nx + n], rho[(size_t)(2 * j + 1) * Nx + n]);
+    }
+    __syncthreads();
+    float2* Z = lds_fft<NT>(A, B, ldp, np, r.ax[0]);
+    float2* C = (Z == A) ? B : A;
+    for (int task = threadIdx.x; task < np * Nxh; task += NT) {
+        const int k = task % Nxh, j = task / Nxh;
+        const float2 Zk = Z[k * ldp + j];
+        const float2 Zm = Z[((Nx - k) % Nx) * ldp + j];
+        C[(2 * j) * ldc + k] = make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y));
+        C[(2 * j + 1) * ldc + k] = make_float2(0.5f * (Zk.y + Zm.y), 0.5f * (Zm.x - Zk.x));
+    }
+    __syncthreads();
+    const float2* Y = lds_fft<NT>(C, Z, ldc, Nxh, r.ax[1]);
+    float2* H = r.H + ((size_t)f * r.Nz + z) * Ny * Nxh;
+    for (int task = threadIdx.x; task < Ny * Nxh; task += NT) {
+        const int k = task % Nxh, y = task / Nxh;
+        gst(H + (size_t)y * Nxh + k, Y[y * ldc + k]);
+    }
+}
+
+// ------------------------------------------------------------------ plane inverse
+// y-axis inverse FFT and x-axis C2R of one (field, z) plane in a single block (3D):
+// the D direction spectra of the plane (Ny x Nxh each) are loaded into LDS, inverse
+// transformed along y, then row chunks are C2R-transformed along x and written as
+// gather records (g_0, g_1, g_2, rho_f).  The spectra make one trip through HBM
+// instead of the separate y pass's read + write + re-read.  Per row the D real lines
+// share complex transforms in pairs (d0, d1), (d2, 0); the Hermitian handling is
+// that of k_x_c2r.  Needs 2 * Ny * (D*Nxh | 1) complex values of LDS.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_plane_c2r(const RepDev* __restrict__ reps,
+                                                  const WorkItem* __restrict__ wmap) {
+    extern __shared__ float2 lds[];
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    constexpr int D = 3, NC = 2;                     // real lines per row, complex lines per row
+    const int Nx = r.Nx, Ny = r.Ny, Nxh = r.Nxh;
+    const int z = wi.blk % r.Nz, f = wi.blk / r.Nz;
+    const int L1 = D * Nxh, ld1 = L1 | 1;
+    float2* A = lds;
+    float2* B = lds + (size_t)Ny * ld1;
+    for (int task = threadIdx.x; task < Ny * L1; task += NT) {
+        const int kx = task % Nxh, rest = task / Nxh, d = rest % D, y = rest / D;
+        const float2 v = gld(r.Gk + (size_t)(f * D + d) * r.Mh + ((size_t)z * Ny + y) * Nxh + kx);
+        A[y * ld1 + d * Nxh + kx] = conjf2(v);       // inverse as conj(FFT(conj))
+    }
+    __syncthreads();
+    const float2* Y = lds_fft<NT>(A, B, ld1, L1, r.ax[1]);   // conj of the y-inverse
+    float2* Fr = (Y == A) ? B : A;                            // free buffer
+    const int cr = r.plane_rows;
+    const int ldb = (cr * NC) | 1;
+    float2* X1 = Fr;
+    float2* X2 = Fr + (size_t)Nx * ldb;
+    const bool even = (Nx & 1) == 0;
+    const bool fp = r.FPtab && f < r.ntypes;
+    for (int y0 = 0; y0 < Ny; y0 += cr) {
+        const int rows = min(cr, Ny - y0);
+        const int nb = rows * NC;
+        for (int task = threadIdx.x; task < nb * Nx; task += NT) {
+            const int k = task % Nx, b = task / Nx;
+            const int rr = b / NC, j = b % NC;
+            const int la = 2 * j, lb = la + 1;
+            const bool upper = k >= Nxh;
+            const int kk = upper ? Nx - k : k;
+            const float2* yrow = Y + (size_t)(y0 + rr) * ld1;
+            float2 Av = conjf2(yrow[la * Nxh + kk]);
+            float2 Bv = lb < D ? conjf2(yrow[lb * Nxh + kk]) : make_float2(0.f, 0.f);
+            if (kk == 0 || (even && 2 * kk == Nx)) { Av.y = 0.f; Bv.y = 0.f; }
+            if (upper) { Av.y = -Av.y; Bv.y = -Bv.y; }
+            X1[k * ldb + b] = make_float2(Av.x - Bv.y, -(Av.y + Bv.x));
+        }
+        __syncthreads();
+        const float2* res = lds_fft<NT>(X1, X2, ldb, nb, r.ax[0]);
+        for (int task = threadIdx.x; task < rows * Nx; task += NT) {
+            const int x = task % Nx, rr = task / Nx;
+            float g[3];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const float2 zz = res[x * ldb + rr * NC + d / 2];
+                g[d] = (d & 1) ? -zz.y : zz.x;
+            }
+            const size_t node = ((size_t)z * Ny + y0 + rr) * Nx + x;
+            const float rho = f < r.ntypes ? G((const float*)r.rho)[(size_t)f * r.M + node] : 1.0f;
+            if (fp)   // field phases: rho f_d per unit density, as in k_x_c2r
+                for (int d = 0; d < D; ++d) g[d] += rho * G(r.FPtab)[((size_t)f * D + d) * r.M + node];
+            gst(r.G4 + (size_t)f * r.M + node, make_float4(g[0], g[1], g[2], rho));
+        }
+        __syncthreads();
     }
 }
 
@@ -667,6 +787,25 @@ hipError_t launch_x_c2r(const LaunchCtx& c, bool records, int nlines, const floa
     if (!c.n) return hipSuccess;
     if (records) hipLaunchKernelGGL(k_x_c2r<true>, dim3(c.n), dim3(kFftThreads), c.lds, c.stream, c.reps, c.wmap, nlines, in, out);
     else hipLaunchKernelGGL(k_x_c2r<false>, dim3(c.n), dim3(kFftThreads), c.lds, c.stream, c.reps, c.wmap, nlines, in, out);
+    CHECK_LAUNCH(); return hipSuccess;
+}
+hipError_t configure_kernels() {
+    hipError_t e = hipFuncSetAttribute((const void*)k_plane_c2r<kPlaneThreads>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPlaneLds);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)k_last_fused, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPlaneLds);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void*)k_plane_r2c<kPlaneThreads>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)kPlaneLds);
+}
+hipError_t launch_plane_r2c(const LaunchCtx& c) {
+    if (!c.n) return hipSuccess;
+    hipLaunchKernelGGL(k_plane_r2c<kPlaneThreads>, dim3(c.n), dim3(kPlaneThreads), c.lds, c.stream, c.reps, c.wmap);
+    CHECK_LAUNCH(); return hipSuccess;
+}
+hipError_t launch_plane_c2r(const LaunchCtx& c) {
+    if (!c.n) return hipSuccess;
+    hipLaunchKernelGGL(k_plane_c2r<kPlaneThreads>, dim3(c.n), dim3(kPlaneThreads), c.lds, c.stream, c.reps, c.wmap);
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t launch_particles(const LaunchCtx& c, bool integrate, int write_forces) {

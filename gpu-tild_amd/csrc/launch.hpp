@@ -34,6 +34,11 @@ __host__ __device__ inline MidGeom mid_geom(const RepDev& r, int axis, int nfb) 
 // one FFT buffer of N x nlines complex values, and the ping-pong pair a pass needs
 inline size_t fft_buf_bytes(int N, int nlines) { return (size_t)N * (size_t)(nlines | 1) * sizeof(float2); }
 inline size_t lds_bytes(int N, int nlines) { return 2 * fft_buf_bytes(N, nlines); }
+// fused last-axis pass: coupled fields (nf*nl lines) + two buffers of nf*DB*nl lines
+// (DB directions per inverse batch)
+inline size_t last_lds_bytes(int N, int nf, int DB, int nl) {
+    return fft_buf_bytes(N, nf * nl) + 2 * fft_buf_bytes(N, nf * DB * nl);
+}
 
 struct LaunchCtx {
     const RepDev* reps;
@@ -49,6 +54,18 @@ hipError_t launch_mid(const LaunchCtx& c, int axis, int use_gk, int nfb, int inv
 hipError_t launch_last_fused(const LaunchCtx& c, int store_spectrum);
 hipError_t launch_x_c2r(const LaunchCtx& c, bool records, int nlines, const float2* in, float* out);
 hipError_t launch_particles(const LaunchCtx& c, bool integrate, int write_forces);
+constexpr int kPlaneThreads = 512;                      // block of the plane inverse pass
+constexpr size_t kPlaneLds = 163840;                    // LDS one block may use (MI355X)
+hipError_t launch_plane_c2r(const LaunchCtx& c);
+hipError_t launch_plane_r2c(const LaunchCtx& c);
+// LDS of the forward plane pass: two buffers holding either the Ny/2 row pairs of
+// length Nx or the Ny x Nxh half spectra
+inline size_t plane_r2c_lds(const RepDev& h) {
+    const size_t a = (size_t)h.Nx * (size_t)((h.Ny / 2) | 1), b = (size_t)h.Ny * (size_t)(h.Nxh | 1);
+    return 2 * (a > b ? a : b) * sizeof(float2);
+}
+// One-time kernel attributes (dynamic LDS above 64 KiB for the plane pass).
+hipError_t configure_kernels();
 hipError_t launch_kprops(const LaunchCtx& c, int nq_max);
 hipError_t launch_bond_props(const LaunchCtx& c);
 hipError_t launch_reduce_rows(hipStream_t s, const double* in, int nrows, int n, double* out);

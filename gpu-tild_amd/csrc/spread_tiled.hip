@@ -36,27 +36,50 @@ __device__ inline int tile_bucket(const RepDev& r, const float* xp, int i) {
     return (tz * r.nty + ty) * 4 + c;
 }
 
-// Slots are claimed per wave: lanes with the same bucket (neighbours in a spatially
-// ordered replica) share one atomic, issued by the lowest of them.
+// Slot claims.  A block counts its beads in LDS over a window of kCountWin buckets
+// that starts at the block's smallest bucket -- in a spatially ordered replica that
+// is every bead of the block -- and claims global slots with one atomic per distinct
+// bucket.  Beads beyond the window claim per wave: lanes that share a bucket use one
+// atomic, issued by the lowest of them.  Slot order inside a bucket is arbitrary;
+// the deposit does not depend on it.
+constexpr int kCountWin = 1024;
+
 __global__ void __launch_bounds__(256) k_tile_count(const RepDev* __restrict__ reps,
                                                     const WorkItem* __restrict__ wmap) {
+    __shared__ int hist[kCountWin];
+    __shared__ int wmin[4];
     const WorkItem wi = wmap[blockIdx.x];
     const RepDev& r = reps[wi.rep];
     const int i = wi.blk * 256 + threadIdx.x;
     const bool valid = i < r.ns;
-    const int b = valid ? tile_bucket(r, r.X[r.step & 1], i) : -1;
+    const int b = valid ? tile_bucket(r, r.X[r.step & 1], i) : 0x7fffffff;
     const int lane = (int)__lane_id();
+    for (int k = threadIdx.x; k < kCountWin; k += 256) hist[k] = 0;
+    int m = b;
+    for (int o = 32; o > 0; o >>= 1) m = min(m, __shfl_xor(m, o));
+    if (lane == 0) wmin[threadIdx.x >> 6] = m;
+    __syncthreads();
+    const int bmin = min(min(wmin[0], wmin[1]), min(wmin[2], wmin[3]));
+    const bool inwin = valid && b - bmin < kCountWin;
+    int local = 0;
+    if (inwin) local = atomicAdd(&hist[b - bmin], 1);
+    __syncthreads();
+    for (int k = threadIdx.x; k < kCountWin; k += 256) {
+        const int c = hist[k];
+        if (c) hist[k] = atomicAdd(r.tcnt + bmin + k, c);   // first global slot of the bucket
+    }
+    __syncthreads();
+    int slot = inwin ? hist[b - bmin] + local : 0;
     const unsigned long long below = (1ull << lane) - 1ull;
-    unsigned long long todo = __ballot(valid);
-    int slot = 0;
-    while (todo) {   // wave-uniform: one round per distinct bucket in the wave
+    unsigned long long todo = __ballot(valid && !inwin);
+    while (todo) {   // wave-uniform: one round per distinct bucket beyond the window
         const int leader = __ffsll((long long)todo) - 1;
         const int bl = __shfl(b, leader);
         const unsigned long long same = __ballot(b == bl) & todo;
         int base = 0;
         if (lane == leader) base = atomicAdd(r.tcnt + bl, __popcll(same));
         base = __shfl(base, leader);
-        if (b == bl) slot = base + __popcll(same & below);
+        if (b == bl && !inwin) slot = base + __popcll(same & below);
         todo &= ~same;
     }
     if (valid) G((long long*)r.tkey)[i] = (long long)(((unsigned long long)(unsigned)slot << 32) | (unsigned)b);
@@ -234,8 +257,9 @@ __global__ void __launch_bounds__(kFftThreads) k_tile_deposit(const RepDev* __re
             v[h] = (float)((double)(long long)patch[(size_t)rr * Nx + n] * inv);
             G(r.rho)[(size_t)f * r.M + ((size_t)(z0 + lz) * r.Ny + (y0 + ly)) * Nx + n] = v[h];
         }
-        fbuf[n * ldb + j] = make_float2(v[0], v[1]);
+        if (!r.plane_fwd) fbuf[n * ldb + j] = make_float2(v[0], v[1]);
     }
+    if (r.plane_fwd) return;   // the plane pass transforms the densities
     __syncthreads();
     const float2* res = lds_fft(fbuf, fbuf + (size_t)Nx * ldb, ldb, npairs, r.ax[0]);
     for (int task = threadIdx.x; task < npairs * Nxh; task += kFftThreads) {

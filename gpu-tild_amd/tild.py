@@ -45,7 +45,8 @@ class Opts(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("use_seed", ctypes.c_int), ("seed", ctypes.c_uint32),
                 ("spread_mode", ctypes.c_int), ("tile_target", ctypes.c_int), ("tile_by", ctypes.c_int),
                 ("tile_bz", ctypes.c_int), ("reorder_every", ctypes.c_int), ("fft_mid_elems", ctypes.c_int),
-                ("fft_last_lines", ctypes.c_int), ("fft_c2r_rows", ctypes.c_int), ("reserved", ctypes.c_int * 1)]
+                ("fft_last_lines", ctypes.c_int), ("fft_c2r_rows", ctypes.c_int), ("fft_plane", ctypes.c_int),
+                ("fft_last_dirs", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
 
 
 class Props(ctypes.Structure):
@@ -224,13 +225,15 @@ class Engine:
     """One device-resident replica (tild_engine)."""
 
     def __init__(self, system, device=0, seed=None, spread_mode=0, tile_target=0, tile_by=0, tile_bz=0,
-                 reorder_every=0, fft_mid_elems=0, fft_last_lines=0, fft_c2r_rows=0):
+                 reorder_every=0, fft_mid_elems=0, fft_last_lines=0, fft_c2r_rows=0, fft_plane=0,
+                 fft_last_dirs=0):
         o = Opts()
         o.device = device
         o.spread_mode = spread_mode
         o.tile_target, o.tile_by, o.tile_bz = tile_target, tile_by, tile_bz
         o.reorder_every = reorder_every
         o.fft_mid_elems, o.fft_last_lines, o.fft_c2r_rows = fft_mid_elems, fft_last_lines, fft_c2r_rows
+        o.fft_plane, o.fft_last_dirs = fft_plane, fft_last_dirs
         if seed is not None:
             o.use_seed = 1
             o.seed = int(seed) & 0xFFFFFFFF

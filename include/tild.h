@@ -90,7 +90,12 @@ typedef struct {
     int fft_mid_elems;   /* FFT block geometry, 0 = default: complex values per block of */
     int fft_last_lines;  /* the y pass, columns per block of the fused z pass, rows per  */
     int fft_c2r_rows;    /* block of the inverse x pass.  Results do not depend on them. */
-    int reserved[1];
+    int fft_plane;       /* 3D plane passes: 0 default (forward x R2C + y FFT per plane for
+                            tiled replicas with even Ny), 1 also the fused y-inverse + x
+                            C2R per plane (pairs a row's real lines differently: the last
+                            bits move), < 0 neither */
+    int fft_last_dirs;   /* directions per inverse batch of the fused z pass (0 = 1) */
+    int reserved[3];
 } tild_opts;
 
 /* Properties of calc_properties() (calc_properties.cu:20-116): FP64 reductions. */

@@ -237,3 +237,24 @@ def test_reordered_run_matches_oracle(tmp_path):
     print(f"  10 GJF steps with reorders every 2: max |dx| = {err:.2e}")
     assert err < 1e-4
     e.close()
+
+
+@pytest.mark.parametrize("case", ["3d_mixed", "charged", "3d_fieldphase"])
+def test_plane_inverse_matches_separate_passes(case, tmp_path):
+    # the fused y-inverse + x C2R pairs a row's real lines (d0, d1), (d2, 0) instead of
+    # consecutively across fields: same transforms, FP32 rounding moves by an ulp or so
+    text, d = CASES[case](tmp_path)
+    s = tild.System(text, d)
+    out = []
+    for plane in (1, -1):
+        e = tild.Engine(s, device=0, seed=4, fft_plane=plane)
+        f = e.forces()
+        out.append((f, [e.grid_field(t, k) for t in range(e.nfields) for k in range(s.dim)], e.density(0)))
+        e.close()
+    (f0, g0, r0), (f1, g1, r1) = out
+    assert np.array_equal(r0, r1)
+    for a, b in zip(g0, g1):
+        assert np.abs(a - b).max() <= 1e-5 * max(np.abs(b).max(), 1e-30)
+    err = U.rel_norm(f0, f1)
+    print(f"[{case}] plane vs separate passes: |df|/|f| = {err:.2e}")
+    assert err < 1e-6
