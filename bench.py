@@ -250,7 +250,7 @@ def run(args):
                              f"{platform.processor() or platform.machine()}), {tc:.1f} s"}
 
     traffic = None
-    tfile = REPO / "profiles" / "traffic_c2.json"
+    tfile = REPO / "profiles" / "traffic_melt640k.json"
     if tfile.exists():
         try:
             traffic = json.loads(tfile.read_text()).get(dom)
