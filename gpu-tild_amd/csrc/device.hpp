@@ -73,7 +73,7 @@ struct RepDev {
     float* rho;                // nf*M densities (x fastest)
     float2* H;                 // nf*Mh half spectra
     float2* Gk;                // nf*dim*Mh half spectra of the gathered fields
-    float4* G4;                // nf*M (g_x, g_y, g_z, rho) records for the gather
+    float* G3;                 // nf*M*3 gather records (g_x, g_y, g_z), zero where rho <= 0
     const float* Ctab;         // coupling tables C_IJ/M (I <= J) + charge table, blocked as
                                // [column block][last-axis index][row][nl_last columns]
     int ctab_rows;             // rows of Ctab: T(T+1)/2 (+1 with charges)
@@ -123,6 +123,15 @@ __device__ __forceinline__ float4 gld(const float4* p) {
     return make_float4(v.x, v.y, v.z, v.w);
 }
 __device__ __forceinline__ void gst(float2* p, float2 a) { *(gmem<f2v>)p = f2v{a.x, a.y}; }
+// three consecutive floats (one gather record); merged into one 12-byte access
+__device__ __forceinline__ float3 gld3(const float* p) {
+    gmem<const float> q = G(p);
+    return make_float3(q[0], q[1], q[2]);
+}
+__device__ __forceinline__ void gst3(float* p, float a, float b, float c) {
+    gmem<float> q = G(p);
+    q[0] = a; q[1] = b; q[2] = c;
+}
 __device__ __forceinline__ void gst(float4* p, float4 a) { *(gmem<f4v>)p = f4v{a.x, a.y, a.z, a.w}; }
 
 // ---------------------------------------------------------------- Philox / normals

@@ -24,10 +24,14 @@ int make_axis_plan(int N, AxisPlan& pl, std::vector<float2>& tw, std::string& er
     if (N < 1 || N > kFftBudget) { err = "FFT length " + std::to_string(N) + " outside [1, " + std::to_string(kFftBudget) + "]"; return -TILD_ERR_UNSUPPORTED; }
     int n = N, e2 = 0;
     while (n % 2 == 0) { n /= 2; ++e2; }
+    // powers of two in as few stages as possible (at most 16 per stage), spread evenly:
+    // every stage ends with a barrier, and the passes are latency-bound
     std::vector<int> rad;
-    for (int k = 0; k < e2 / 3; ++k) rad.push_back(8);
-    if (e2 % 3 == 1) rad.push_back(2);
-    if (e2 % 3 == 2) rad.push_back(4);
+    const int ns2 = (e2 + 3) / 4;
+    for (int k = 0; k < ns2; ++k) {
+        const int bits = e2 / ns2 + (k < e2 % ns2 ? 1 : 0);
+        rad.push_back(1 << bits);
+    }
     for (int p : {3, 5, 7})
         while (n % p == 0) { rad.push_back(p); n /= p; }
     if (n != 1) {
@@ -144,7 +148,7 @@ size_t Batch::estimate_bytes(const System& S) {
     // X[2], XO, fout; type, charge, bond tables, integrator kind; sorted-bin keys and records
     // site ids; spare arrays, keys and sort scratch of the spatial reorder
     size_t part = ns * (4 * D * 4 + 4 + 4 + 24 + 1) + ns * (8 + 16 + 4) + ns * 4 + ns * (4 * D + 100);
-    size_t grids = nf * M * (8 + 4 + 16) + nf * Mh * 8 * (1 + D);
+    size_t grids = nf * M * (8 + 4 + 12) + nf * Mh * 8 * (1 + D);
     size_t tabs = Mh * 4 * (T * (T + 1) / 2 + 1 + nst + 1) + Mh * 8 * (D == 3 ? 6 : 3) * nst;
     if (!S.Fields.empty()) tabs += T * D * M * 4;
     return part + grids + tabs + (1 << 20);
@@ -312,7 +316,7 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
     if (int rc = dalloc(R, &h.rho, nf * M, err, true)) return rc;
     if (int rc = dalloc(R, &h.H, nf * Mh, err, true)) return rc;
     if (int rc = dalloc(R, &h.Gk, nf * D * Mh, err, true)) return rc;
-    if (int rc = dalloc(R, &h.G4, nf * M, err, true)) return rc;
+    if (int rc = dalloc(R, &h.G3, nf * M * 3, err, true)) return rc;
     R.nblk_part = (S.ns + 255) / 256;
     if (int rc = dalloc(R, &R.d_bpart, 7 * (size_t)R.nblk_part, err, true)) return rc;
     h.bpart = R.d_bpart;
@@ -485,6 +489,7 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
         add(PH_XP, nbx, lds_bytes(h.Nx, h.nl_x));
         if (h.spread_mode == 2) {
             add(PH_TPART, R.nblk_part, 0);
+            add(PH_TCOUNT, (h.ns + kCountBlock - 1) / kCountBlock, 0);
             add(PH_SCAN, 1, 0);
             add(PH_TILE, h.ntiles, tile_lds_bytes(h.Nx, h.nf, h.By, h.Bz));
         } else {
@@ -516,7 +521,7 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
     // Workgroups go to the 8 XCDs round-robin (launch block w runs on XCD w % 8).  The
     // particle-order passes hand each XCD a contiguous run of work items, i.e. one slab
     // of a spatially ordered replica, so that its L2 holds that slab's grid records.
-    for (Phase ph : {PH_PART, PH_TPART, PH_SPREADA, PH_TILE}) {
+    for (Phase ph : {PH_PART, PH_TPART, PH_TCOUNT, PH_SPREADA, PH_TILE}) {
         std::vector<WorkItem>& v = maps[ph].host;
         const size_t n = v.size(), q = n / kXcds, rm = n % kXcds;
         if (n < 2 * (size_t)kXcds) continue;
@@ -768,7 +773,7 @@ int Batch::enqueue_step(SetMaps& m, bool integrate, bool write_forces, std::stri
     auto ctx = [&](Phase p) { return LaunchCtx{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_}; };
     HIPCHK(launch_spread(ctx(PH_SPREADA)));
     HIPCHK(launch_x_r2c(ctx(PH_XA), true, 0, nullptr, nullptr));
-    HIPCHK(launch_tile_count(ctx(PH_TPART)));
+    HIPCHK(launch_tile_count(ctx(PH_TCOUNT)));
     HIPCHK(launch_tile_scan(ctx(PH_SCAN)));
     HIPCHK(launch_tile_scatter(ctx(PH_TPART)));
     HIPCHK(launch_tile_deposit(ctx(PH_TILE)));
@@ -860,7 +865,7 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
         auto rec = [&](int k) { if (e0 == hipSuccess) e0 = hipEventRecord(ev[k], stream_); };
         rec(0);
         if (e0 == hipSuccess) e0 = launch_spread(ctx(PH_SPREADA));
-        if (e0 == hipSuccess) e0 = launch_tile_count(ctx(PH_TPART));
+        if (e0 == hipSuccess) e0 = launch_tile_count(ctx(PH_TCOUNT));
         if (e0 == hipSuccess) e0 = launch_tile_scan(ctx(PH_SCAN));
         if (e0 == hipSuccess) e0 = launch_tile_scatter(ctx(PH_TPART));
         rec(1);
@@ -1022,10 +1027,10 @@ int Batch::get_grid_field(int id, int type, int dir, float* g, size_t n, std::st
         err = "get_grid_field: bad type, direction or size";
         return -TILD_ERR_ARG;
     }
-    std::vector<float4> tmp(R->M);
-    HIPCHK(hipMemcpyAsync(tmp.data(), R->h.G4 + (size_t)type * R->M, (size_t)R->M * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+    std::vector<float> tmp((size_t)R->M * 3);
+    HIPCHK(hipMemcpyAsync(tmp.data(), R->h.G3 + (size_t)type * R->M * 3, tmp.size() * sizeof(float), hipMemcpyDeviceToHost, stream_));
     HIPCHK(hipStreamSynchronize(stream_));
-    for (int i = 0; i < R->M; ++i) g[i] = dir == 0 ? tmp[i].x : (dir == 1 ? tmp[i].y : tmp[i].z);
+    for (int i = 0; i < R->M; ++i) g[i] = tmp[(size_t)i * 3 + dir];
     return 0;
 }
 

@@ -21,7 +21,7 @@ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(
 __device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }   // a * (-i)
 __device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
 
-// Forward DFT of small sizes, in place on v[0..R-1].
+// Forward DFT of small sizes (2, 3, 4, 5, 7, 8, 16), in place on v[0..R-1].
 template <int R> struct Dft;
 
 template <> struct Dft<2> {
@@ -65,6 +65,41 @@ template <> struct Dft<8> {
         v[3] = cadd(a[3], b3);   v[7] = csub(a[3], b3);
     }
 };
+// 16 = 4 x 4: four DFT-4 over the stride-4 subsequences, twiddles w16^(n2 k1), four
+// DFT-4 across them; output k1 + 4 k2.
+template <> struct Dft<16> {
+    __device__ __forceinline__ static void run(float2* v) {
+        const float c1 = 0.92387953251128675613f;   // cos(pi/8)
+        const float s1 = 0.38268343236508977173f;   // sin(pi/8)
+        const float h = 0.70710678118654752440f;
+        float2 y[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) {
+            float2 a[4] = {v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]};
+            Dft<4>::run(a);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; ++k1) y[n2][k1] = a[k1];
+        }
+        // w16^1 = (c1,-s1), w16^2 = (h,-h), w16^3 = (s1,-c1), w16^4 = -i, w16^6 = (-h,-h), w16^9 = (-c1,s1)
+        y[1][1] = cmul(y[1][1], make_float2(c1, -s1));
+        y[1][2] = make_float2(h * (y[1][2].x + y[1][2].y), h * (y[1][2].y - y[1][2].x));
+        y[1][3] = cmul(y[1][3], make_float2(s1, -c1));
+        y[2][1] = make_float2(h * (y[2][1].x + y[2][1].y), h * (y[2][1].y - y[2][1].x));
+        y[2][2] = mul_mi(y[2][2]);
+        y[2][3] = make_float2(h * (y[2][3].y - y[2][3].x), -h * (y[2][3].x + y[2][3].y));
+        y[3][1] = cmul(y[3][1], make_float2(s1, -c1));
+        y[3][2] = make_float2(h * (y[3][2].y - y[3][2].x), -h * (y[3][2].x + y[3][2].y));
+        y[3][3] = cmul(y[3][3], make_float2(-c1, s1));
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            float2 a[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+            Dft<4>::run(a);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = a[k2];
+        }
+    }
+};
+
 // Odd prime radices: real/imaginary symmetric pairs (Rader-free direct form).
 template <int R> struct DftOdd {
     __device__ __forceinline__ static void run(float2* v, const float* c, const float* s) {
@@ -151,6 +186,7 @@ __device__ __forceinline__ float2* lds_fft(float2* A, float2* B, int ldb, int nb
             case 4: fft_stage<4, NT>(src, dst, ldb, nb, pl.N, p, tw); break;
             case 5: fft_stage<5, NT>(src, dst, ldb, nb, pl.N, p, tw); break;
             case 7: fft_stage<7, NT>(src, dst, ldb, nb, pl.N, p, tw); break;
+            case 16: fft_stage<16, NT>(src, dst, ldb, nb, pl.N, p, tw); break;
             default: fft_stage<8, NT>(src, dst, ldb, nb, pl.N, p, tw); break;
         }
         __syncthreads();

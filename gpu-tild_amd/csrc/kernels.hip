@@ -300,13 +300,18 @@ __global__ void __launch_bounds__(kFftThreads) k_x_c2r(const RepDev* __restrict_
                 g[d] = (line & 1) ? -z.y : z.x;      // conj of the forward result
             }
             const size_t node = (size_t)row * N + x;
-            const float rho = f < r.ntypes ? G(r.rho)[(size_t)f * r.M + node] : 1.0f;
-            if (r.FPtab && f < r.ntypes)
-                // field phases (pair_style_fieldphases.cu:54-86): the grid force gains
-                // +-rho^2 f_d (d_prepareFieldForce then d_accumulateGridForce multiply
-                // by rho twice), i.e. rho f_d per unit density
-                for (int d = 0; d < D; ++d) g[d] += rho * G(r.FPtab)[((size_t)f * D + d) * r.M + node];
-            gst(r.G4 + (size_t)f * r.M + node, make_float4(g[0], g[1], g[2], rho));
+            if (f < r.ntypes) {
+                const float rho = G(r.rho)[(size_t)f * r.M + node];
+                if (r.FPtab)
+                    // field phases (pair_style_fieldphases.cu:54-86): the grid force gains
+                    // +-rho^2 f_d (d_prepareFieldForce then d_accumulateGridForce multiply
+                    // by rho twice), i.e. rho f_d per unit density
+                    for (int d = 0; d < D; ++d) g[d] += rho * G(r.FPtab)[((size_t)f * D + d) * r.M + node];
+                // the gather skips nodes with rho <= 0 (device_grid_utils.cu:424-452):
+                // a zero record adds nothing
+                if (!(rho > 0.f)) g[0] = g[1] = g[2] = 0.f;
+            }
+            gst3(r.G3 + ((size_t)f * r.M + node) * 3, g[0], g[1], g[2]);
         }
     } else {
         for (int task = threadIdx.x; task < rows * nlines * N; task += kFftThreads) {
@@ -419,10 +424,13 @@ __global__ void __launch_bounds__(NT) k_plane_c2r(const RepDev* __restrict__ rep
                 g[d] = (d & 1) ? -zz.y : zz.x;
             }
             const size_t node = ((size_t)z * Ny + y0 + rr) * Nx + x;
-            const float rho = f < r.ntypes ? G((const float*)r.rho)[(size_t)f * r.M + node] : 1.0f;
-            if (fp)   // field phases: rho f_d per unit density, as in k_x_c2r
-                for (int d = 0; d < D; ++d) g[d] += rho * G(r.FPtab)[((size_t)f * D + d) * r.M + node];
-            gst(r.G4 + (size_t)f * r.M + node, make_float4(g[0], g[1], g[2], rho));
+            if (f < r.ntypes) {
+                const float rho = G((const float*)r.rho)[(size_t)f * r.M + node];
+                if (fp)   // field phases: rho f_d per unit density, as in k_x_c2r
+                    for (int d = 0; d < D; ++d) g[d] += rho * G(r.FPtab)[((size_t)f * D + d) * r.M + node];
+                if (!(rho > 0.f)) g[0] = g[1] = g[2] = 0.f;   // masked node, as in k_x_c2r
+            }
+            gst3(r.G3 + ((size_t)f * r.M + node) * 3, g[0], g[1], g[2]);
         }
         __syncthreads();
     }
@@ -435,12 +443,12 @@ __global__ void __launch_bounds__(NT) k_plane_c2r(const RepDev* __restrict__ rep
 // (device_grid_utils.cu:640).  Positions are read from X[step&1] and written to
 // X[(step+1)&1], so bonds always see x_n.
 // Stencil of NP = P + 1 points per axis, fully unrolled so that every grid record of
-// the bead is requested before the first one is used.  The rho > 0 mask is applied as
-// a zero weight: adding w*0 leaves every sum unchanged, and the loads stay
-// unconditional (a branch per node would serialise them).
+// the bead is requested before the first one is used.  The rho > 0 mask is already in
+// the records (zero where rho <= 0): adding w*0 leaves every sum unchanged, and the
+// loads stay unconditional (a branch per node would serialise them).
 template <int NP>
 __device__ __forceinline__ void gather_nodes(const RepDev& r, int D, const float (&W)[3][5], const int (&n0)[3],
-                                             const float4* Gt, const float4* Gc, float qi, bool charged,
+                                             const float* Gt, const float* Gc, float qi, bool charged,
                                              float (&f)[3]) {
     int zz[NP], yy[NP], xx[NP];
     zz[0] = n0[2]; yy[0] = n0[1]; xx[0] = n0[0];
@@ -461,11 +469,10 @@ __device__ __forceinline__ void gather_nodes(const RepDev& r, int D, const float
 #pragma unroll
             for (int ix = 0; ix < NP; ++ix) {
                 const float w = D == 3 ? W[0][ix] * W[1][iy] * W[2][iz] : W[0][ix] * W[1][iy];
-                const float4 g = gld(Gt + rowb + xx[ix]);
-                const float m = g.w > 0.f ? w : 0.f;
-                f[0] += m * g.x; f[1] += m * g.y; f[2] += m * g.z;
+                const float3 g = gld3(Gt + (size_t)(rowb + xx[ix]) * 3);
+                f[0] += w * g.x; f[1] += w * g.y; f[2] += w * g.z;
                 if (charged) {
-                    const float4 e = gld(Gc + rowb + xx[ix]);
+                    const float3 e = gld3(Gc + (size_t)(rowb + xx[ix]) * 3);
                     const float wq = w * qi;
                     fc[0] += wq * e.x; fc[1] += wq * e.y; fc[2] += wq * e.z;
                 }
@@ -476,7 +483,7 @@ __device__ __forceinline__ void gather_nodes(const RepDev& r, int D, const float
 }
 
 __device__ __forceinline__ void gather_nodes_any(const RepDev& r, int D, int P, const float (&W)[3][5], const int (&n0)[3],
-                                              const float4* Gt, const float4* Gc, float qi, bool charged,
+                                              const float* Gt, const float* Gc, float qi, bool charged,
                                               float (&f)[3]) {
     float fc[3] = {0.f, 0.f, 0.f};
     const int nz = D == 3 ? P + 1 : 1;
@@ -485,11 +492,10 @@ __device__ __forceinline__ void gather_nodes_any(const RepDev& r, int D, int P, 
             const int rowb = (nzz * r.Ny + nyy) * r.Nx;
             for (int ix = 0, nxx = n0[0]; ix <= P; ++ix, nxx = wrap_next(nxx, r.Nx)) {
                 const float w = D == 3 ? W[0][ix] * W[1][iy] * W[2][iz] : W[0][ix] * W[1][iy];
-                const float4 g = gld(Gt + rowb + nxx);
-                const float m = g.w > 0.f ? w : 0.f;
-                f[0] += m * g.x; f[1] += m * g.y; f[2] += m * g.z;
+                const float3 g = gld3(Gt + (size_t)(rowb + nxx) * 3);
+                f[0] += w * g.x; f[1] += w * g.y; f[2] += w * g.z;
                 if (charged) {
-                    const float4 e = gld(Gc + rowb + nxx);
+                    const float3 e = gld3(Gc + (size_t)(rowb + nxx) * 3);
                     const float wq = w * qi;
                     fc[0] += wq * e.x; fc[1] += wq * e.y; fc[2] += wq * e.z;
                 }
@@ -521,8 +527,8 @@ __global__ void __launch_bounds__(256) k_particles(const RepDev* __restrict__ re
     const int t = G(r.tp)[i];
     const bool charged = r.q != nullptr;
     const float qi = charged ? G(r.q)[i] : 0.f;
-    const float4* Gt = r.G4 + (size_t)t * r.M;
-    const float4* Gc = r.G4 + (size_t)r.ntypes * r.M;
+    const float* Gt = r.G3 + (size_t)t * r.M * 3;
+    const float* Gc = r.G3 + (size_t)r.ntypes * r.M * 3;
     float f[3] = {0.f, 0.f, 0.f};
     // P = 1 (the default order) and P = 0 fully unrolled; higher orders loop (fully
     // unrolling 27-125 nodes would cost every order its registers)

@@ -70,6 +70,8 @@ hipError_t launch_kprops(const LaunchCtx& c, int nq_max);
 hipError_t launch_bond_props(const LaunchCtx& c);
 hipError_t launch_reduce_rows(hipStream_t s, const double* in, int nrows, int n, double* out);
 hipError_t launch_advance(hipStream_t s, RepDev* reps, const int* which, int n);
+constexpr int kCountPerThread = 1;                      // beads per thread of the bucket count
+constexpr int kCountBlock = 256 * kCountPerThread;      // beads per block of the bucket count
 hipError_t launch_tile_count(const LaunchCtx& c);
 hipError_t launch_tile_scan(const LaunchCtx& c);
 hipError_t launch_tile_scatter(const LaunchCtx& c);

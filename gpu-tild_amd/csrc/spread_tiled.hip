@@ -50,39 +50,54 @@ __global__ void __launch_bounds__(256) k_tile_count(const RepDev* __restrict__ r
     __shared__ int wmin[4];
     const WorkItem wi = wmap[blockIdx.x];
     const RepDev& r = reps[wi.rep];
-    const int i = wi.blk * 256 + threadIdx.x;
-    const bool valid = i < r.ns;
-    const int b = valid ? tile_bucket(r, r.X[r.step & 1], i) : 0x7fffffff;
+    const float* x = r.X[r.step & 1];
     const int lane = (int)__lane_id();
+    int b[kCountPerThread];
+    bool valid[kCountPerThread];
+    int m = 0x7fffffff;
+#pragma unroll
+    for (int k = 0; k < kCountPerThread; ++k) {
+        const int i = wi.blk * kCountBlock + k * 256 + threadIdx.x;
+        valid[k] = i < r.ns;
+        b[k] = valid[k] ? tile_bucket(r, x, i) : 0x7fffffff;
+        m = min(m, b[k]);
+    }
     for (int k = threadIdx.x; k < kCountWin; k += 256) hist[k] = 0;
-    int m = b;
     for (int o = 32; o > 0; o >>= 1) m = min(m, __shfl_xor(m, o));
     if (lane == 0) wmin[threadIdx.x >> 6] = m;
     __syncthreads();
     const int bmin = min(min(wmin[0], wmin[1]), min(wmin[2], wmin[3]));
-    const bool inwin = valid && b - bmin < kCountWin;
-    int local = 0;
-    if (inwin) local = atomicAdd(&hist[b - bmin], 1);
+    bool inwin[kCountPerThread];
+    int local[kCountPerThread];
+#pragma unroll
+    for (int k = 0; k < kCountPerThread; ++k) {
+        inwin[k] = valid[k] && b[k] - bmin < kCountWin;
+        local[k] = inwin[k] ? atomicAdd(&hist[b[k] - bmin], 1) : 0;
+    }
     __syncthreads();
     for (int k = threadIdx.x; k < kCountWin; k += 256) {
         const int c = hist[k];
         if (c) hist[k] = atomicAdd(r.tcnt + bmin + k, c);   // first global slot of the bucket
     }
     __syncthreads();
-    int slot = inwin ? hist[b - bmin] + local : 0;
     const unsigned long long below = (1ull << lane) - 1ull;
-    unsigned long long todo = __ballot(valid && !inwin);
-    while (todo) {   // wave-uniform: one round per distinct bucket beyond the window
-        const int leader = __ffsll((long long)todo) - 1;
-        const int bl = __shfl(b, leader);
-        const unsigned long long same = __ballot(b == bl) & todo;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(r.tcnt + bl, __popcll(same));
-        base = __shfl(base, leader);
-        if (b == bl && !inwin) slot = base + __popcll(same & below);
-        todo &= ~same;
+#pragma unroll
+    for (int k = 0; k < kCountPerThread; ++k) {
+        int slot = inwin[k] ? hist[b[k] - bmin] + local[k] : 0;
+        unsigned long long todo = __ballot(valid[k] && !inwin[k]);
+        while (todo) {   // wave-uniform: one round per distinct bucket beyond the window
+            const int leader = __ffsll((long long)todo) - 1;
+            const int bl = __shfl(b[k], leader);
+            const unsigned long long same = __ballot(b[k] == bl) & todo;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(r.tcnt + bl, __popcll(same));
+            base = __shfl(base, leader);
+            if (b[k] == bl && !inwin[k]) slot = base + __popcll(same & below);
+            todo &= ~same;
+        }
+        const int i = wi.blk * kCountBlock + k * 256 + threadIdx.x;
+        if (valid[k]) G((long long*)r.tkey)[i] = (long long)(((unsigned long long)(unsigned)slot << 32) | (unsigned)b[k]);
     }
-    if (valid) G((long long*)r.tkey)[i] = (long long)(((unsigned long long)(unsigned)slot << 32) | (unsigned)b);
 }
 
 // Exclusive scan of the bucket counts; clears the counts for the next step.

@@ -8,7 +8,7 @@ import tild
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(8, 6, 10), (40, 40, 40), (225, 21, 21), (35, 35, 35), (160, 80, 80), (125, 125), (16, 20),
-          (27, 25, 24), (7, 9, 11 - 1), (48, 45, 36)]
+          (27, 25, 24), (7, 9, 11 - 1), (48, 45, 36), (256, 64, 16), (128, 96, 32)]
 
 
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
