@@ -407,9 +407,12 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
     return id;
 }
 
-int Batch::reorder(int id, std::string& err) {
+int Batch::reorder(int id, std::string& err, bool after_step) {
     Replica& R = *reps_[id];
-    HIPCHK(launch_reorder(stream_, d_reps_, id, R.ns, R.sort_bits, R.rb));
+    if (after_step && R.h.spread_mode == 2)   // the step's tile binning is current
+        HIPCHK(launch_reorder_tiles(stream_, d_reps_, id, R.ns, R.h.ntiles, R.rb));
+    else
+        HIPCHK(launch_reorder(stream_, d_reps_, id, R.ns, R.sort_bits, R.rb));
     // mirror the device-side swaps in the host copy of the descriptor
     RepDev& h = R.h;
     std::swap(h.X[0], h.X[1]);
@@ -833,7 +836,7 @@ int Batch::step(const std::vector<int>& ids_in, int n, std::string& err, bool us
         for (int id : ids) {
             Replica& R = *reps_[id];
             if (R.reorder_every > 0 && (R.since_sort += chunk) >= R.reorder_every)
-                if (int rc = reorder(id, err)) return rc;
+                if (int rc = reorder(id, err, true)) return rc;
         }
     }
     for (int id : ids) reps_[id]->h.step = reps_[id]->step;
@@ -887,7 +890,7 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
         if (e0 == hipSuccess) e0 = launch_advance(stream_, d_reps_, m->d_ids, m->nids);
         for (int id : ids) {   // amortised spatial reorders are timed with the step counter
             Replica& R = *reps_[id];
-            if (e0 == hipSuccess && R.reorder_every > 0 && ++R.since_sort >= R.reorder_every && reorder(id, err))
+            if (e0 == hipSuccess && R.reorder_every > 0 && ++R.since_sort >= R.reorder_every && reorder(id, err, true))
                 e0 = hipErrorUnknown;
         }
         rec(8);

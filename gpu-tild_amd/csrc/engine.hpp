@@ -133,7 +133,9 @@ private:
     int build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& err);
     int build_tables(Replica& R, const System& S, std::string& err);
     int forward_spectra(int id, std::string& err);   // rho -> H (properties)
-    int reorder(int id, std::string& err);           // spatial reorder of one replica
+    // spatial reorder of one replica; after_step: the last step's tile binning matches
+    // the current slots and is reused
+    int reorder(int id, std::string& err, bool after_step = false);
     int run_steps(SetMaps& m, const std::vector<int>& ids, int n, bool use_graph, std::string& err);
     int fetch_pid(const Replica& R, std::vector<int>& pid, std::string& err);
     void free_maps(PhaseMap* maps);

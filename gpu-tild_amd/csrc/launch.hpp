@@ -106,5 +106,8 @@ hipError_t launch_nlist(hipStream_t s, const RepDev* reps, int rep, int ns, cons
 // Sort the replica's beads by grid row (stable), permute every per-bead array and
 // swap the live and spare buffers in the device descriptor.
 hipError_t launch_reorder(hipStream_t s, RepDev* reps, int rep, int ns, int bits, ReorderBufs& b);
+// Tiled replicas after a step: each tile's beads (from the step's bucket order) sorted
+// by site id, then the same permutation.
+hipError_t launch_reorder_tiles(hipStream_t s, RepDev* reps, int rep, int ns, int ntiles, ReorderBufs& b);
 
 }  // namespace tild

@@ -8,8 +8,10 @@
 // few adjacent grid rows instead of 64 scattered ones, and a contiguous run of blocks
 // covers one slab of the grid.
 //
-// The permutation is a stable radix sort of the row keys (rocPRIM), so it depends
-// only on the positions and the previous order: runs stay bit-reproducible.  Every
+// Replicas on the tiled spread reuse the bucket order of the step just taken: each
+// tile's beads sorted by site id (one LDS bitonic sort per tile).  Otherwise, and at
+// creation, the permutation is a stable radix sort of the row keys (rocPRIM).  Both
+// depend only on the positions and the previous order: runs stay bit-reproducible.  Every
 // per-step result is independent of slot order (integer deposition, per-bead gather
 // and bonds, noise keyed by the original site id `pid`), so trajectories are
 // bit-identical with and without reordering.  Bond partners are renumbered through
@@ -96,7 +98,93 @@ __global__ void k_reorder_swap(RepDev* reps, int rep) {
     swap_ptr(r.bond_t, r.bond_t_alt);
 }
 
+// ---- tiled replicas: reuse the last step's bucket order -----------------------------
+// The step's binning left every bead's bucket (tile, halo class) and slot in tkey and
+// the bucket offsets in toff, so the beads of tile t occupy [toff[4t], toff[4t+4]) of
+// the bucket order.  Inside a tile the slot order is arbitrary (atomics); sorting each
+// tile's beads by original site id makes the new order deterministic.
+
+// src[bucket position] = slot
+__global__ void __launch_bounds__(256) k_reorder_src(const RepDev* __restrict__ reps, int rep, int* __restrict__ src) {
+    const RepDev& r = reps[rep];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= r.ns) return;
+    const long long kk = G((const long long*)r.tkey)[i];
+    const int b = (int)(unsigned)(kk & 0xffffffffll), s = (int)(kk >> 32);
+    G(src)[G((const int*)r.toff)[b] + s] = i;
+}
+
+constexpr int kTileSortMax = 2048;   // beads a tile sorts in LDS; larger tiles rank in place
+
+// One block per tile: order[tile range] = the tile's slots sorted by site id.
+__global__ void __launch_bounds__(1024) k_reorder_tile_sort(const RepDev* __restrict__ reps, int rep,
+                                                           const int* __restrict__ src, int* __restrict__ order) {
+    __shared__ int key[kTileSortMax];
+    __shared__ int val[kTileSortMax];
+    const RepDev& r = reps[rep];
+    const int t = blockIdx.x;
+    const int a = G((const int*)r.toff)[4 * t], L = G((const int*)r.toff)[4 * t + 4] - a;
+    gmem<const int> pid = G((const int*)r.pid);
+    if (L > kTileSortMax) {
+        // rare dense tile: the rank of each site id inside the tile places it
+        for (int j = threadIdx.x; j < L; j += 1024) {
+            const int sj = G(src)[a + j], kj = pid[sj];
+            int rank = 0;
+            for (int k = 0; k < L; ++k) rank += pid[G(src)[a + k]] < kj;
+            G(order)[a + rank] = sj;
+        }
+        return;
+    }
+    int n2 = 64;
+    while (n2 < L) n2 <<= 1;
+    for (int j = threadIdx.x; j < n2; j += 1024) {
+        if (j < L) {
+            const int sj = G(src)[a + j];
+            key[j] = pid[sj];
+            val[j] = sj;
+        } else {
+            key[j] = 0x7fffffff;
+            val[j] = -1;
+        }
+    }
+    __syncthreads();
+    for (int k = 2; k <= n2; k <<= 1)          // bitonic sort, ascending site id
+        for (int h = k >> 1; h > 0; h >>= 1) {
+            for (int i = threadIdx.x; i < n2; i += 1024) {
+                const int ixj = i ^ h;
+                if (ixj > i) {
+                    const bool up = (i & k) == 0;
+                    const int ki = key[i], kx = key[ixj];
+                    if ((ki > kx) == up) {
+                        key[i] = kx; key[ixj] = ki;
+                        const int v = val[i]; val[i] = val[ixj]; val[ixj] = v;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int j = threadIdx.x; j < L; j += 1024) G(order)[a + j] = val[j];
+}
+
 #define CHECK_LAUNCH() do { hipError_t e__ = hipGetLastError(); if (e__ != hipSuccess) return e__; } while (0)
+
+hipError_t launch_reorder_tiles(hipStream_t s, RepDev* reps, int rep, int ns, int ntiles, ReorderBufs& b) {
+    const int nb = (ns + 255) / 256;
+    int* src = b.vals[0];
+    int* order = b.vals[1];
+    hipLaunchKernelGGL(k_reorder_src, dim3(nb), dim3(256), 0, s, reps, rep, src);
+    CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_reorder_tile_sort, dim3(ntiles), dim3(1024), 0, s, reps, rep, src, order);
+    CHECK_LAUNCH();
+    int* inv = (int*)b.keys[0];
+    hipLaunchKernelGGL(k_reorder_inv, dim3(nb), dim3(256), 0, s, ns, order, inv);
+    CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_reorder_apply, dim3(nb), dim3(256), 0, s, reps, rep, order, inv);
+    CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_reorder_swap, dim3(1), dim3(1), 0, s, reps, rep);
+    CHECK_LAUNCH();
+    return hipSuccess;
+}
 
 size_t reorder_tmp_bytes(int ns, int bits) {
     size_t bytes = 0;
