@@ -32,6 +32,7 @@ struct RepDev {
     // geometry (initialize.cu:32-48)
     int dim, Nx, Ny, Nz, Nxh, M, Mh, ns, ntypes, nf, P;
     float L[3], Lh[3], dx[3], gvol, kfac[3];   // kfac = 2 pi / L
+    float idx[3];                              // 1 / dx (FP32), for the spline weights
     // fixed-point densities: acc = round(w * 2^F), rho = acc * inv
     float fix_scale, fixq_scale;
     double fix_inv, fixq_inv;
@@ -56,6 +57,7 @@ struct RepDev {
     const float* bond_r0;
     const signed char* ikind;  // ns: 0 none, 1 EM, 2 GJF
     float delt, noise_mag;
+    int any_gjf;            // 1: some site integrates with GJF
     float gjf_b[kMaxTypes], gjf_a[kMaxTypes], gjf_c1[kMaxTypes], gjf_c2[kMaxTypes],
           gjf_noise[kMaxTypes];
     float* fout;            // ns*dim (optional)
@@ -74,6 +76,7 @@ struct RepDev {
     float2* H;                 // nf*Mh half spectra
     float2* Gk;                // nf*dim*Mh half spectra of the gathered fields
     float* G3;                 // nf*M*3 gather records (g_x, g_y, g_z), zero where rho <= 0
+    int g3_off32;              // 1: nf*M*12 bytes of records fit 32-bit byte offsets
     const float* Ctab;         // coupling tables C_IJ/M (I <= J) + charge table, blocked as
                                // [column block][last-axis index][row][nl_last columns]
     int ctab_rows;             // rows of Ctab: T(T+1)/2 (+1 with charges)
@@ -92,6 +95,7 @@ struct RepDev {
     // sorted-bin deposition: full-x slab tiles of By x Bz rows (By | Ny, Bz | Nz)
     int spread_mode;           // 1 global atomics, 2 LDS-tiled sorted bins
     int By, Bz, nty, ntz, ntiles, nbuckets;
+    float iBy, iBz;            // 1 / By, 1 / Bz (FP32) for the bucket divisions
     int* tcnt;                 // nbuckets counters (zero between steps)
     int* toff;                 // nbuckets + 1 exclusive offsets
     int2* tkey;                // ns (bucket, slot)
@@ -127,6 +131,10 @@ __device__ __forceinline__ void gst(float2* p, float2 a) { *(gmem<f2v>)p = f2v{a
 __device__ __forceinline__ float3 gld3(const float* p) {
     gmem<const float> q = G(p);
     return make_float3(q[0], q[1], q[2]);
+}
+__device__ __forceinline__ int3 gld3(const int* p) {
+    gmem<const int> q = G(p);
+    return make_int3(q[0], q[1], q[2]);
 }
 __device__ __forceinline__ void gst3(float* p, float a, float b, float c) {
     gmem<float> q = G(p);
@@ -172,8 +180,9 @@ __device__ inline void normals4(uint32_t seed, uint32_t site, uint32_t step, flo
 // ---------------------------------------------------------------- stencil
 // PME B-spline weights of orders 0-4 (device_grid_utils.cu:3-49), including the
 // order-4 term sx3 = sx2*sx2 of the reference.
-__device__ inline void spline_weights(float gdx, float H, float* W, int P) {
-    float sx = gdx / H, sx2, sx3, sx4;
+__device__ __forceinline__ void spline_weights(float gdx, float iH, float* W, int P) {
+#pragma clang fp contract(off)
+    float sx = gdx * iH, sx2, sx3, sx4;
     if (P == 0) { W[0] = 1.f; }
     else if (P == 1) { W[0] = 0.5f - sx; W[1] = 0.5f + sx; }
     else if (P == 2) {
@@ -195,22 +204,50 @@ __device__ inline void spline_weights(float gdx, float H, float* W, int P) {
     }
 }
 
-// Bin and weights of one coordinate (device_grid_utils.cu:136-146): the base node
-// uses IEEE FP32 division exactly as the reference, so bins match bit for bit.
-// n0 is the first stencil node, already wrapped into [0, N).
-__device__ inline int stencil_1d(float x, float dx, int N, int P, float* W) {
-    int g;
-    float gdx;
-    if ((P & 1) == 0) {
-        g = (int)((x + 0.5f * dx) / dx);
-        gdx = x - (float)g * dx;
-    } else {
-        g = (int)(x / dx);
-        gdx = x - ((float)g + 0.5f) * dx;
-    }
-    spline_weights(gdx, dx, W, P);
-    int n0 = (g - P / 2) % N;
-    return n0 < 0 ? n0 + N : n0;
+// First stencil node n0 in [0, N) of a bin g.  Positions stay within [0, L], so
+// g - P/2 lies in [-N, 2N) and one add or subtract wraps it; the integer remainder
+// (a long division sequence on the GPU) is only the fallback for anything else.
+__device__ __forceinline__ int wrap_node(int n, int N) {
+    if ((unsigned)n < (unsigned)N) return n;
+    if (n < 0 && n >= -N) return n + N;
+    if (n >= N && n < 2 * N) return n - N;
+    n %= N;
+    return n < 0 ? n + N : n;
+}
+
+// n / d and n % d for 0 <= n < 2^22, d >= 1, from inv = 1/d in FP32: the float
+// quotient is within one of the true quotient, and one correction step fixes it.
+__device__ __forceinline__ int div_small(int n, int d, float inv, int& rem) {
+    int q = (int)((float)n * inv);
+    int m = n - q * d;
+    if (m < 0) { --q; m += d; }
+    else if (m >= d) { ++q; m -= d; }
+    rem = m;
+    return q;
+}
+
+// Bin of one coordinate (device_grid_utils.cu:136-146).  The bin uses IEEE FP32
+// division exactly as the reference, so bins match bit for bit.
+__device__ __forceinline__ int stencil_bin(float x, float dx, int P) {
+    return (P & 1) == 0 ? (int)((x + 0.5f * dx) / dx) : (int)(x / dx);
+}
+
+// First stencil node (wrapped into [0, N)) and weights of one coordinate.  The
+// weights take the offset times 1/dx: at most an ulp from the reference's quotient,
+// and a multiply instead of a division sequence.  Contraction into FMAs is off here
+// and in the weights, so that every kernel that inlines them (the deposits of both
+// spread paths, the gather) rounds the weights the same way whatever surrounds them.
+__device__ __forceinline__ int stencil_1d(float x, float dx, float idx, int N, int P, float* W) {
+#pragma clang fp contract(off)
+    const int g = stencil_bin(x, dx, P);
+    const float gdx = (P & 1) == 0 ? x - (float)g * dx : x - ((float)g + 0.5f) * dx;
+    spline_weights(gdx, idx, W, P);
+    return wrap_node(g - P / 2, N);
+}
+
+// First stencil node only (bucket keys, bins): no weights.
+__device__ __forceinline__ int stencil_node(float x, float dx, int N, int P) {
+    return wrap_node(stencil_bin(x, dx, P) - P / 2, N);
 }
 
 __device__ inline int wrap_next(int n, int N) { return (n + 1 == N) ? 0 : n + 1; }

@@ -169,7 +169,7 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
     h.M = S.M; h.Mh = h.Nxh * h.Ny * h.Nz;
     h.ns = S.ns; h.ntypes = S.ntypes; h.nf = S.ntypes + (S.charges.enabled ? 1 : 0); h.P = S.pmeorder;
     h.nP = S.n_P_comps;
-    for (int d = 0; d < 3; ++d) { h.L[d] = S.L[d]; h.Lh[d] = S.Lh[d]; h.dx[d] = S.dx[d]; }
+    for (int d = 0; d < 3; ++d) { h.L[d] = S.L[d]; h.Lh[d] = S.Lh[d]; h.dx[d] = S.dx[d]; h.idx[d] = 1.0f / S.dx[d]; }
     h.gvol = S.gvol;
     for (int d = 0; d < D; ++d) h.kfac[d] = (float)(2.0 * kPi / (double)S.L[d]);
     R.ns = S.ns; R.dim = D; R.ntypes = S.ntypes; R.nf = h.nf; R.M = h.M; R.Mh = h.Mh; R.nP = h.nP;
@@ -288,6 +288,7 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
         if (int rc = dalloc(R, &d_k, S.ns, err)) return rc;
         HIPCHK(hipMemcpy(d_k, kind.data(), S.ns, hipMemcpyHostToDevice));
         h.ikind = d_k;
+        h.any_gjf = std::find(kind.begin(), kind.end(), (signed char)2) != kind.end() ? 1 : 0;
     }
     h.delt = S.delt;
     h.noise_mag = S.noise_mag;
@@ -317,6 +318,7 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
     if (int rc = dalloc(R, &h.H, nf * Mh, err, true)) return rc;
     if (int rc = dalloc(R, &h.Gk, nf * D * Mh, err, true)) return rc;
     if (int rc = dalloc(R, &h.G3, nf * M * 3, err, true)) return rc;
+    h.g3_off32 = nf * M * 12 <= (size_t)0xFFFFFFFFu ? 1 : 0;
     R.nblk_part = (S.ns + 255) / 256;
     if (int rc = dalloc(R, &R.d_bpart, 7 * (size_t)R.nblk_part, err, true)) return rc;
     h.bpart = R.d_bpart;
@@ -328,6 +330,7 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
         if (choose_tiles(D, S.Nx, S.pmeorder, h.nf, tile_target_, tile_by_, tile_bz_, By, Bz)) {
             h.spread_mode = 2;
             h.By = By; h.Bz = Bz;
+            h.iBy = 1.0f / (float)By; h.iBz = 1.0f / (float)Bz;
             h.nty = h.Ny / By; h.ntz = h.Nz / Bz;
             h.ntiles = h.nty * h.ntz;
             h.nbuckets = h.ntiles * 4;

@@ -35,7 +35,7 @@ __global__ void __launch_bounds__(256) k_spread(const RepDev* __restrict__ reps,
     n0[2] = 0; W[2][0] = 1.f;
     for (int d = 0; d < D; ++d) {
         int N = d == 0 ? r.Nx : (d == 1 ? r.Ny : r.Nz);
-        n0[d] = stencil_1d(x[i * D + d], r.dx[d], N, P, W[d]);
+        n0[d] = stencil_1d(x[i * D + d], r.dx[d], r.idx[d], N, P, W[d]);
     }
     const int t = r.tp[i];
     unsigned long long* acc = r.acc + (size_t)t * r.M;
@@ -442,67 +442,212 @@ __global__ void __launch_bounds__(NT) k_plane_c2r(const RepDev* __restrict__ rep
 // The gather reads g = all_f/rho directly; nodes with rho <= 0 contribute nothing
 // (device_grid_utils.cu:640).  Positions are read from X[step&1] and written to
 // X[(step+1)&1], so bonds always see x_n.
-// Stencil of NP = P + 1 points per axis, fully unrolled so that every grid record of
-// the bead is requested before the first one is used.  The rho > 0 mask is already in
-// the records (zero where rho <= 0): adding w*0 leaves every sum unchanged, and the
-// loads stay unconditional (a branch per node would serialise them).
-template <int NP>
-__device__ __forceinline__ void gather_nodes(const RepDev& r, int D, const float (&W)[3][5], const int (&n0)[3],
-                                             const float* Gt, const float* Gc, float qi, bool charged,
-                                             float (&f)[3]) {
-    int zz[NP], yy[NP], xx[NP];
-    zz[0] = n0[2]; yy[0] = n0[1]; xx[0] = n0[0];
-#pragma unroll
-    for (int k = 1; k < NP; ++k) {
-        zz[k] = wrap_next(zz[k - 1], r.Nz);
-        yy[k] = wrap_next(yy[k - 1], r.Ny);
-        xx[k] = wrap_next(xx[k - 1], r.Nx);
-    }
-    const int nzl = D == 3 ? NP : 1;
-    float fc[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-    for (int iz = 0; iz < NP; ++iz) {
-        if (iz >= nzl) break;
-#pragma unroll
-        for (int iy = 0; iy < NP; ++iy) {
-            const int rowb = (zz[iz] * r.Ny + yy[iy]) * r.Nx;
-#pragma unroll
-            for (int ix = 0; ix < NP; ++ix) {
-                const float w = D == 3 ? W[0][ix] * W[1][iy] * W[2][iz] : W[0][ix] * W[1][iy];
-                const float3 g = gld3(Gt + (size_t)(rowb + xx[ix]) * 3);
-                f[0] += w * g.x; f[1] += w * g.y; f[2] += w * g.z;
-                if (charged) {
-                    const float3 e = gld3(Gc + (size_t)(rowb + xx[ix]) * 3);
-                    const float wq = w * qi;
-                    fc[0] += wq * e.x; fc[1] += wq * e.y; fc[2] += wq * e.z;
-                }
-            }
-        }
-    }
-    for (int d = 0; d < D; ++d) f[d] += fc[d];
+//
+// The bead body is specialised on the dimension D and on the stencil width NP = P + 1
+// of the orders P = 0 and 1, so every per-axis array stays in registers and every
+// stencil loop is unrolled; NP = 0 is the any-order body (P from the descriptor).
+// The per-bead loads are issued before the first of them is used, the three bond
+// partners together after the gather (before it they would hold registers through
+// the eight gather records and cost a wave per SIMD).
+
+// Stencil sum acc += sum_nodes w * record over the records of one field, which start
+// at element fb of G3.  The rho > 0 mask is already in the records (zero where
+// rho <= 0): adding w*0 leaves every sum unchanged, and the loads stay unconditional
+// (a branch per node would serialise them).  NP > 0: unrolled, every record requested
+// before the first is used.  OFF32: every record lies within 4 GiB of G3, so a record
+// is addressed by the uniform base plus a 32-bit byte offset (one register per load
+// in flight instead of two).
+template <int NP, bool OFF32>
+__device__ __forceinline__ float3 gather_rec(const float* G3, size_t fb, int node) {
+    if constexpr (OFF32)
+        return gld3((const float*)((const char*)G3 + ((uint32_t)fb + (uint32_t)node) * 12u));
+    else
+        return gld3(G3 + (fb + (size_t)node) * 3);
 }
 
-__device__ __forceinline__ void gather_nodes_any(const RepDev& r, int D, int P, const float (&W)[3][5], const int (&n0)[3],
-                                              const float* Gt, const float* Gc, float qi, bool charged,
-                                              float (&f)[3]) {
-    float fc[3] = {0.f, 0.f, 0.f};
-    const int nz = D == 3 ? P + 1 : 1;
-    for (int iz = 0, nzz = n0[2]; iz < nz; ++iz, nzz = wrap_next(nzz, r.Nz)) {
-        for (int iy = 0, nyy = n0[1]; iy <= P; ++iy, nyy = wrap_next(nyy, r.Ny)) {
-            const int rowb = (nzz * r.Ny + nyy) * r.Nx;
-            for (int ix = 0, nxx = n0[0]; ix <= P; ++ix, nxx = wrap_next(nxx, r.Nx)) {
-                const float w = D == 3 ? W[0][ix] * W[1][iy] * W[2][iz] : W[0][ix] * W[1][iy];
-                const float3 g = gld3(Gt + (size_t)(rowb + nxx) * 3);
-                f[0] += w * g.x; f[1] += w * g.y; f[2] += w * g.z;
-                if (charged) {
-                    const float3 e = gld3(Gc + (size_t)(rowb + nxx) * 3);
-                    const float wq = w * qi;
-                    fc[0] += wq * e.x; fc[1] += wq * e.y; fc[2] += wq * e.z;
+template <int D, int NP, bool OFF32>
+__device__ __forceinline__ void gather_sum(const RepDev& r, int P, const float (&W)[3][5], const int (&n0)[3],
+                                           size_t fb, float (&acc)[3]) {
+    const float* G3 = r.G3;
+    if constexpr (NP > 0) {
+        int zz[NP], yy[NP], xx[NP];
+        zz[0] = n0[2]; yy[0] = n0[1]; xx[0] = n0[0];
+#pragma unroll
+        for (int k = 1; k < NP; ++k) {
+            zz[k] = wrap_next(zz[k - 1], r.Nz);
+            yy[k] = wrap_next(yy[k - 1], r.Ny);
+            xx[k] = wrap_next(xx[k - 1], r.Nx);
+        }
+        constexpr int nzl = D == 3 ? NP : 1;
+#pragma unroll
+        for (int iz = 0; iz < nzl; ++iz) {
+#pragma unroll
+            for (int iy = 0; iy < NP; ++iy) {
+                const int rowb = (zz[iz] * r.Ny + yy[iy]) * r.Nx;
+#pragma unroll
+                for (int ix = 0; ix < NP; ++ix) {
+                    const float w = D == 3 ? W[0][ix] * W[1][iy] * W[2][iz] : W[0][ix] * W[1][iy];
+                    const float3 g = gather_rec<NP, OFF32>(G3, fb, rowb + xx[ix]);
+                    acc[0] += w * g.x; acc[1] += w * g.y; acc[2] += w * g.z;
+                }
+            }
+        }
+    } else {
+        const int nz = D == 3 ? P + 1 : 1;
+        for (int iz = 0, nzz = n0[2]; iz < nz; ++iz, nzz = wrap_next(nzz, r.Nz)) {
+            for (int iy = 0, nyy = n0[1]; iy <= P; ++iy, nyy = wrap_next(nyy, r.Ny)) {
+                const int rowb = (nzz * r.Ny + nyy) * r.Nx;
+                for (int ix = 0, nxx = n0[0]; ix <= P; ++ix, nxx = wrap_next(nxx, r.Nx)) {
+                    const float w = D == 3 ? W[0][ix] * W[1][iy] * W[2][iz] : W[0][ix] * W[1][iy];
+                    const float3 g = gather_rec<NP, OFF32>(G3, fb, rowb + nxx);
+                    acc[0] += w * g.x; acc[1] += w * g.y; acc[2] += w * g.z;
                 }
             }
         }
     }
-    for (int d = 0; d < D; ++d) f[d] += fc[d];
+}
+
+// Gathered force of one bead: its type's records, plus q times the charge records
+// (a second sweep, so that no more than one stencil of records is in flight).
+template <int D, int NP, bool OFF32>
+__device__ __forceinline__ void gather_nodes(const RepDev& r, int P, const float (&W)[3][5], const int (&n0)[3],
+                                             int t, float qi, bool charged, float (&f)[3]) {
+    gather_sum<D, NP, OFF32>(r, P, W, n0, (size_t)t * r.M, f);
+    if (charged) {
+        __builtin_amdgcn_sched_barrier(0);
+        float fc[3] = {0.f, 0.f, 0.f};
+        gather_sum<D, NP, OFF32>(r, P, W, n0, (size_t)r.ntypes * r.M, fc);
+#pragma unroll
+        for (int d = 0; d < D; ++d) f[d] += qi * fc[d];
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void load_pos(const float* p, float (&v)[3]) {
+    if constexpr (D == 3) {
+        const float3 a = gld3(p);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z;
+    } else {
+        gmem<const float> q = G(p);
+        v[0] = q[0]; v[1] = q[1]; v[2] = 0.f;
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void store_pos(float* p, const float (&v)[3]) {
+    if constexpr (D == 3) {
+        gst3(p, v[0], v[1], v[2]);
+    } else {
+        gmem<float> q = G(p);
+        q[0] = v[0]; q[1] = v[1];
+    }
+}
+
+// Minimum image with one wrap, as device_bonds.cu: selects, not branches.
+__device__ __forceinline__ float min_image(float dr, float L, float Lh) {
+    const float lo = dr + L, hi = dr - L;
+    return dr > Lh ? hi : (dr < -Lh ? lo : dr);
+}
+
+template <bool INTEGRATE, int D, int NP, bool OFF32>
+__device__ __forceinline__ void particle_bead(const RepDev& r, int i, int write_forces) {
+    const int P = NP > 0 ? NP - 1 : r.P;
+    const float* xp = r.X[r.step & 1];
+    // per-bead loads
+    float xi[3];
+    load_pos<D>(xp + (size_t)i * D, xi);
+    const int t = G(r.tp)[i];
+    const bool charged = r.q != nullptr;
+    const float qi = charged ? G(r.q)[i] : 0.f;
+    const int3 bj = gld3(r.bond_j + (size_t)i * 3);
+    const int3 btv = gld3(r.bond_t + (size_t)i * 3);
+    // stencil
+    float W[3][5];
+    int n0[3] = {0, 0, 0};
+    W[2][0] = 1.f;
+    n0[0] = stencil_1d(xi[0], r.dx[0], r.idx[0], r.Nx, P, W[0]);
+    n0[1] = stencil_1d(xi[1], r.dx[1], r.idx[1], r.Ny, P, W[1]);
+    if constexpr (D == 3) n0[2] = stencil_1d(xi[2], r.dx[2], r.idx[2], r.Nz, P, W[2]);
+    // gather
+    float f[3] = {0.f, 0.f, 0.f};
+    gather_nodes<D, NP, OFF32>(r, P, W, n0, t, qi, charged, f);
+    // bonds, every bond evaluated from both ends (no atomics).  Slots fill from the
+    // front and the first empty one ends the list; an empty slot loads the bead itself
+    // and bond type 0 (both exist) and contributes nothing, so the loads of all three
+    // partners go out together with no branch.
+    // the site id goes out first: the noise needs only it, and can then be computed
+    // while the partners are in flight
+    uint32_t site = 0;
+    if constexpr (INTEGRATE) site = (uint32_t)G(r.pid)[i];   // original site id: slot order does not matter
+    const int js[3] = {bj.x, bj.y, bj.z}, ts[3] = {btv.x, btv.y, btv.z};
+    bool live[3];
+    live[0] = js[0] >= 0;
+    live[1] = live[0] && js[1] >= 0;
+    live[2] = live[1] && js[2] >= 0;
+    float xj[3][3], kb[3], r0[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        load_pos<D>(xp + (size_t)(live[b] ? js[b] : i) * D, xj[b]);
+        const int tb = live[b] ? ts[b] : 0;
+        kb[b] = G(r.bond_k)[tb];
+        r0[b] = G(r.bond_r0)[tb];
+    }
+    // integrator inputs go out with the partners
+    int kind = 0;
+    float xo[3] = {0.f, 0.f, 0.f}, ga = 0.f, gc1 = 0.f, gc2 = 0.f, gnm = 0.f;
+    float z[4] = {0.f, 0.f, 0.f, 0.f}, zo[4] = {0.f, 0.f, 0.f, 0.f};
+    const uint32_t stepc = (uint32_t)(r.step + 1);
+    if constexpr (INTEGRATE) {
+        kind = G(r.ikind)[i];
+        if (r.any_gjf) {
+            load_pos<D>(r.XO + (size_t)i * D, xo);
+            ga = r.gjf_a[t];
+            gc1 = r.gjf_c1[t];
+            gc2 = r.gjf_c2[t];
+            gnm = r.gjf_noise[t];
+        }
+        normals4(r.seed, site, stepc, z);
+        if (r.any_gjf && stepc > 1u) normals4(r.seed, site, stepc - 1u, zo);
+    }
+    float lf[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        float dr[3] = {0.f, 0.f, 0.f}, r2 = 0.f;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            dr[d] = min_image(xi[d] - xj[b][d], r.L[d], r.Lh[d]);
+            r2 += dr[d] * dr[d];
+        }
+        // 2 k (|dr| - r0) / |dr| from one reciprocal square root
+        const float ir = __builtin_amdgcn_rsqf(fmaxf(r2, 1.0e-4f));
+        const float mf = (live[b] && r2 > 1.0e-4f) ? 2.0f * kb[b] * (r2 * ir - r0[b]) * ir : 0.f;
+#pragma unroll
+        for (int d = 0; d < D; ++d) lf[d] -= mf * dr[d];
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) f[d] += lf[d];
+    if (write_forces && r.fout) store_pos<D>(r.fout + (size_t)i * D, f);
+    if constexpr (INTEGRATE) {
+        // kind 0 keeps x, 1 is EM (device_EM_integrator.cu:5-36), 2 is GJF
+        // (device_GJF_integrator.cu:25-58) in displacement form: with
+        // b = m/(m + dt/2D) and a = (1 - dt/2Dm)/(1 + dt/2Dm), 2b - a = 1 exactly, so
+        //   x' = 2b x - a xo + c1 f + c2 (beta + beta_old)
+        //      = x + [a (x - xo) + c1 f + c2 (beta + beta_old)].
+        // XO holds the unwrapped displacement x - xo, which removes the reference's
+        // minimum-image unwrap of xo and keeps FP32 rounding off the large terms.
+        float v[3] = {0.f, 0.f, 0.f}, dn[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const float nn = gnm * z[d], no = gnm * zo[d];
+            dn[d] = ga * xo[d] + gc1 * f[d] + gc2 * (nn + no);
+            const float em = xi[d] + r.delt * f[d] + r.noise_mag * z[d];
+            float u = kind == 2 ? xi[d] + dn[d] : em;
+            u = u > r.L[d] ? u - r.L[d] : (u < 0.0f ? u + r.L[d] : u);
+            v[d] = kind == 0 ? xi[d] : u;
+        }
+        store_pos<D>(r.X[(r.step + 1) & 1] + (size_t)i * D, v);
+        if (kind == 2) store_pos<D>(r.XO + (size_t)i * D, dn);
+    }
 }
 
 template <bool INTEGRATE>
@@ -513,90 +658,17 @@ __global__ void __launch_bounds__(256) k_particles(const RepDev* __restrict__ re
     const RepDev& r = reps[wi.rep];
     const int i = wi.blk * 256 + threadIdx.x;
     if (i >= r.ns) return;
-    const int D = r.dim, P = r.P;
-    gmem<const float> x = G((const float*)r.X[r.step & 1]);
-    float xi[3] = {0.f, 0.f, 0.f};
-    for (int d = 0; d < D; ++d) xi[d] = x[i * D + d];
-    float W[3][5];
-    int n0[3];
-    n0[2] = 0; W[2][0] = 1.f;
-    for (int d = 0; d < D; ++d) {
-        int N = d == 0 ? r.Nx : (d == 1 ? r.Ny : r.Nz);
-        n0[d] = stencil_1d(xi[d], r.dx[d], N, P, W[d]);
-    }
-    const int t = G(r.tp)[i];
-    const bool charged = r.q != nullptr;
-    const float qi = charged ? G(r.q)[i] : 0.f;
-    const float* Gt = r.G3 + (size_t)t * r.M * 3;
-    const float* Gc = r.G3 + (size_t)r.ntypes * r.M * 3;
-    float f[3] = {0.f, 0.f, 0.f};
-    // P = 1 (the default order) and P = 0 fully unrolled; higher orders loop (fully
-    // unrolling 27-125 nodes would cost every order its registers)
-    if (P == 1) gather_nodes<2>(r, D, W, n0, Gt, Gc, qi, charged, f);
-    else if (P == 0) gather_nodes<1>(r, D, W, n0, Gt, Gc, qi, charged, f);
-    else gather_nodes_any(r, D, P, W, n0, Gt, Gc, qi, charged, f);
-    // bonds: every bond evaluated from both ends, no atomics
-    float lf[3] = {0.f, 0.f, 0.f};
-    gmem<const int> bj = G(r.bond_j), bt_ = G(r.bond_t);
-    for (int b = 0; b < 3; ++b) {
-        const int j = bj[i * 3 + b];
-        if (j < 0) break;
-        const int bt = bt_[i * 3 + b];
-        float dr[3] = {0.f, 0.f, 0.f}, r2 = 0.f;
-        for (int d = 0; d < D; ++d) {
-            dr[d] = xi[d] - x[j * D + d];
-            if (dr[d] > r.Lh[d]) dr[d] -= r.L[d];
-            else if (dr[d] < -r.Lh[d]) dr[d] += r.L[d];
-            r2 += dr[d] * dr[d];
-        }
-        float mf = 0.f;
-        if (r2 > 1.0e-4f) {
-            const float rr = sqrtf(r2);
-            mf = 2.0f * G(r.bond_k)[bt] * (rr - G(r.bond_r0)[bt]) / rr;
-        }
-        for (int d = 0; d < D; ++d) lf[d] -= mf * dr[d];
-    }
-    for (int d = 0; d < D; ++d) f[d] += lf[d];
-    if (write_forces && r.fout)
-        for (int d = 0; d < D; ++d) G(r.fout)[i * D + d] = f[d];
-    if (!INTEGRATE) return;
-    gmem<float> xn = G(r.X[(r.step + 1) & 1]);
-    const int kind = G(r.ikind)[i];
-    if (kind == 0) {
-        for (int d = 0; d < D; ++d) xn[i * D + d] = xi[d];
-        return;
-    }
-    const uint32_t stepc = (uint32_t)(r.step + 1);
-    float z[4];
-    const uint32_t site = (uint32_t)G(r.pid)[i];   // original site id: slot order does not matter
-    normals4(r.seed, site, stepc, z);
-    if (kind == 1) {
-        for (int d = 0; d < D; ++d) {
-            float v = xi[d] + r.delt * f[d] + r.noise_mag * z[d];
-            if (v > r.L[d]) v -= r.L[d];
-            else if (v < 0.0f) v += r.L[d];
-            xn[i * D + d] = v;
-        }
+    // unrolled bodies: the default order P = 1 on grids whose records fit 32-bit
+    // offsets, and P = 0; any other case takes the looped body
+    if (r.P == 1 && r.g3_off32) {
+        if (r.dim == 3) particle_bead<INTEGRATE, 3, 2, true>(r, i, write_forces);
+        else particle_bead<INTEGRATE, 2, 2, true>(r, i, write_forces);
+    } else if (r.P == 0) {
+        if (r.dim == 3) particle_bead<INTEGRATE, 3, 1, false>(r, i, write_forces);
+        else particle_bead<INTEGRATE, 2, 1, false>(r, i, write_forces);
     } else {
-        float zo[4] = {0.f, 0.f, 0.f, 0.f};
-        if (stepc > 1u) normals4(r.seed, site, stepc - 1u, zo);
-        // GJF (device_GJF_integrator.cu:25-58) in displacement form: with
-        // b = m/(m + dt/2D) and a = (1 - dt/2Dm)/(1 + dt/2Dm), 2b - a = 1 exactly, so
-        //   x' = 2b x - a xo + c1 f + c2 (beta + beta_old)
-        //      = x + [a (x - xo) + c1 f + c2 (beta + beta_old)].
-        // XO holds the unwrapped displacement x - xo, which removes the reference's
-        // minimum-image unwrap of xo and keeps FP32 rounding off the large terms.
-        const float a = r.gjf_a[t], c1 = r.gjf_c1[t], c2 = r.gjf_c2[t], nm = r.gjf_noise[t];
-        gmem<float> xo = G(r.XO);
-        for (int d = 0; d < D; ++d) {
-            const float nn = nm * z[d], no = nm * zo[d];
-            const float dn = a * xo[i * D + d] + c1 * f[d] + c2 * (nn + no);
-            float v = xi[d] + dn;
-            xo[i * D + d] = dn;
-            if (v > r.L[d]) v -= r.L[d];
-            else if (v < 0.0f) v += r.L[d];
-            xn[i * D + d] = v;
-        }
+        if (r.dim == 3) particle_bead<INTEGRATE, 3, 0, false>(r, i, write_forces);
+        else particle_bead<INTEGRATE, 2, 0, false>(r, i, write_forces);
     }
 }
 
@@ -606,10 +678,9 @@ __global__ void __launch_bounds__(256) k_bins(const RepDev* __restrict__ reps, i
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= r.ns) return;
     const float* x = r.X[r.step & 1];
-    float W[5];
     for (int d = 0; d < r.dim; ++d) {
         int N = d == 0 ? r.Nx : (d == 1 ? r.Ny : r.Nz);
-        out[i * r.dim + d] = stencil_1d(x[i * r.dim + d], r.dx[d], N, r.P, W);
+        out[i * r.dim + d] = stencil_node(x[i * r.dim + d], r.dx[d], N, r.P);
     }
 }
 
@@ -726,6 +797,9 @@ __global__ void __launch_bounds__(256) k_reduce_rows(const double* __restrict__ 
     if (threadIdx.x == 0) out[q] = red[0];
 }
 
+// Step counters of the replicas that just integrated (their position parity flips).
+// A separate launch: counting the particle blocks done inside k_particles instead
+// costs each block a device-scope atomic round trip, which measured slower.
 __global__ void k_advance(RepDev* reps, const int* __restrict__ which, int n) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) reps[which[k]].step += 1;

@@ -54,6 +54,7 @@ hipError_t launch_mid(const LaunchCtx& c, int axis, int use_gk, int nfb, int inv
 hipError_t launch_last_fused(const LaunchCtx& c, int store_spectrum);
 hipError_t launch_x_c2r(const LaunchCtx& c, bool records, int nlines, const float2* in, float* out);
 hipError_t launch_particles(const LaunchCtx& c, bool integrate, int write_forces);
+hipError_t launch_advance(hipStream_t s, RepDev* reps, const int* which, int n);
 constexpr int kPlaneThreads = 512;                      // block of the plane inverse pass
 constexpr size_t kPlaneLds = 163840;                    // LDS one block may use (MI355X)
 hipError_t launch_plane_c2r(const LaunchCtx& c);
@@ -69,7 +70,6 @@ hipError_t configure_kernels();
 hipError_t launch_kprops(const LaunchCtx& c, int nq_max);
 hipError_t launch_bond_props(const LaunchCtx& c);
 hipError_t launch_reduce_rows(hipStream_t s, const double* in, int nrows, int n, double* out);
-hipError_t launch_advance(hipStream_t s, RepDev* reps, const int* which, int n);
 constexpr int kCountPerThread = 1;                      // beads per thread of the bucket count
 constexpr int kCountBlock = 256 * kCountPerThread;      // beads per block of the bucket count
 hipError_t launch_tile_count(const LaunchCtx& c);

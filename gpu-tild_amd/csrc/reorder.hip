@@ -32,9 +32,8 @@ __global__ void __launch_bounds__(256) k_sort_keys(const RepDev* __restrict__ re
     if (i >= r.ns) return;
     const int D = r.dim;
     const float* x = r.X[r.step & 1];
-    float W[5];
-    const int sy = stencil_1d(x[i * D + 1], r.dx[1], r.Ny, r.P, W);
-    const int sz = D == 3 ? stencil_1d(x[i * D + 2], r.dx[2], r.Nz, r.P, W) : 0;
+    const int sy = stencil_node(x[i * D + 1], r.dx[1], r.Ny, r.P);
+    const int sz = D == 3 ? stencil_node(x[i * D + 2], r.dx[2], r.Nz, r.P) : 0;
     uint32_t key;
     if (r.spread_mode == 2) {
         const int ty = sy / r.By, tz = sz / r.Bz;

@@ -26,12 +26,12 @@ namespace tild {
 __device__ inline int tile_bucket(const RepDev& r, const float* xp, int i) {
     const int D = r.dim, P = r.P;
     gmem<const float> x = G(xp);
-    float W[5];
-    const int sy = stencil_1d(x[i * D + 1], r.dx[1], r.Ny, P, W);
-    const int sz = D == 3 ? stencil_1d(x[i * D + 2], r.dx[2], r.Nz, P, W) : 0;
-    const int ty = sy / r.By, tz = sz / r.Bz;
-    const bool yb = (sy % r.By) + P >= r.By;
-    const bool zb = D == 3 && (sz % r.Bz) + P >= r.Bz;
+    const int sy = stencil_node(x[i * D + 1], r.dx[1], r.Ny, P);
+    const int sz = D == 3 ? stencil_node(x[i * D + 2], r.dx[2], r.Nz, P) : 0;
+    int my, mz;
+    const int ty = div_small(sy, r.By, r.iBy, my), tz = div_small(sz, r.Bz, r.iBz, mz);
+    const bool yb = my + P >= r.By;
+    const bool zb = D == 3 && mz + P >= r.Bz;
     const int c = (yb && zb) ? 2 : (yb ? 1 : (zb ? 3 : 0));
     return (tz * r.nty + ty) * 4 + c;
 }
@@ -154,9 +154,9 @@ __device__ __forceinline__ void deposit_record(const RepDev& r, const float4 rec
     const int tp = __float_as_int(rec.w);
     float W[3][5];
     int n0[3];
-    n0[0] = stencil_1d(rec.x, r.dx[0], Nx, NP - 1, W[0]);
-    n0[1] = stencil_1d(rec.y, r.dx[1], r.Ny, NP - 1, W[1]);
-    if (D == 3) n0[2] = stencil_1d(rec.z, r.dx[2], r.Nz, NP - 1, W[2]);
+    n0[0] = stencil_1d(rec.x, r.dx[0], r.idx[0], Nx, NP - 1, W[0]);
+    n0[1] = stencil_1d(rec.y, r.dx[1], r.idx[1], r.Ny, NP - 1, W[1]);
+    if (D == 3) n0[2] = stencil_1d(rec.z, r.dx[2], r.idx[2], r.Nz, NP - 1, W[2]);
     else { n0[2] = 0; W[2][0] = 1.f; }
     int xx[NP];
     xx[0] = n0[0];
@@ -193,9 +193,9 @@ __device__ __forceinline__ void deposit_record_any(const RepDev& r, const float4
     const int tp = __float_as_int(rec.w);
     float W[3][5];
     int n0[3];
-    n0[0] = stencil_1d(rec.x, r.dx[0], Nx, P, W[0]);
-    n0[1] = stencil_1d(rec.y, r.dx[1], r.Ny, P, W[1]);
-    if (D == 3) n0[2] = stencil_1d(rec.z, r.dx[2], r.Nz, P, W[2]);
+    n0[0] = stencil_1d(rec.x, r.dx[0], r.idx[0], Nx, P, W[0]);
+    n0[1] = stencil_1d(rec.y, r.dx[1], r.idx[1], r.Ny, P, W[1]);
+    if (D == 3) n0[2] = stencil_1d(rec.z, r.dx[2], r.idx[2], r.Nz, P, W[2]);
     else { n0[2] = 0; W[2][0] = 1.f; }
     const int nz = D == 3 ? P + 1 : 1;
     for (int iz = 0, zz = n0[2]; iz < nz; ++iz, zz = wrap_next(zz, r.Nz)) {
