@@ -96,7 +96,8 @@ struct RepDev {
     int spread_mode;           // 1 global atomics, 2 LDS-tiled sorted bins
     int By, Bz, nty, ntz, ntiles, nbuckets;
     float iBy, iBz;            // 1 / By, 1 / Bz (FP32) for the bucket divisions
-    int* tcnt;                 // nbuckets counters (zero between steps)
+    int* tcnt;                 // nbuckets counters (zero between steps), bucket b at b << tshift
+    int tshift;                // log2 of the counter stride: one 128-byte line per bucket
     int* toff;                 // nbuckets + 1 exclusive offsets
     int2* tkey;                // ns (bucket, slot)
     float4* trec;              // ns sorted (x, y, z, type) records
@@ -225,6 +226,16 @@ __device__ __forceinline__ int div_small(int n, int d, float inv, int& rem) {
     rem = m;
     return q;
 }
+
+// A divisor of the in-block index arithmetic: n / d and n % d for 0 <= n < 2^22 from
+// the hardware reciprocal (1 ulp, so the float quotient is still within one of the
+// true one), in a few instructions instead of the integer division sequence.
+struct FastDiv {
+    int d;
+    float inv;
+    __device__ __forceinline__ explicit FastDiv(int dd) : d(dd), inv(__builtin_amdgcn_rcpf((float)dd)) {}
+    __device__ __forceinline__ int div(int n, int& rem) const { return div_small(n, d, inv, rem); }
+};
 
 // Bin of one coordinate (device_grid_utils.cu:136-146).  The bin uses IEEE FP32
 // division exactly as the reference, so bins match bit for bit.

@@ -114,7 +114,7 @@ Batch::~Batch() {
     for (auto& kv : set_cache_) {
         free_maps(kv.second->maps);
         if (kv.second->d_ids) (void)hipFree(kv.second->d_ids);
-        if (kv.second->graph) (void)hipGraphExecDestroy(kv.second->graph);
+        drop_graphs(*kv.second);
     }
     for (auto& r : reps_)
         if (r) {
@@ -334,7 +334,10 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
             h.nty = h.Ny / By; h.ntz = h.Nz / Bz;
             h.ntiles = h.nty * h.ntz;
             h.nbuckets = h.ntiles * 4;
-            if (int rc = dalloc(R, &h.tcnt, h.nbuckets, err, true)) return rc;
+            // Every bucket counter on a line of its own: device-scope atomics on one line
+            // serialise, and a bucket line is claimed by every block of its tiles.
+            h.tshift = h.nbuckets <= (1 << 16) ? 5 : 0;
+            if (int rc = dalloc(R, &h.tcnt, (size_t)h.nbuckets << h.tshift, err, true)) return rc;
             if (int rc = dalloc(R, &h.toff, (size_t)h.nbuckets + 1, err, true)) return rc;
             if (int rc = dalloc(R, &h.tkey, S.ns, err)) return rc;
             if (int rc = dalloc(R, &h.trec, S.ns, err)) return rc;
@@ -448,7 +451,7 @@ void Batch::remove(int id) {
         if (std::find(it->first.begin(), it->first.end(), id) != it->first.end()) {
             free_maps(it->second->maps);
             if (it->second->d_ids) (void)hipFree(it->second->d_ids);
-            if (it->second->graph) (void)hipGraphExecDestroy(it->second->graph);
+            drop_graphs(*it->second);
             it = set_cache_.erase(it);
         } else ++it;
     }
@@ -457,8 +460,7 @@ void Batch::remove(int id) {
 int Batch::upload_desc(int id, std::string& err) {
     if (id >= d_cap_) {
         // captured graphs hold the descriptor array's address: drop them
-        for (auto& kv : set_cache_)
-            if (kv.second->graph) { (void)hipGraphExecDestroy(kv.second->graph); kv.second->graph = nullptr; }
+        for (auto& kv : set_cache_) drop_graphs(*kv.second);
         int cap = std::max(256, d_cap_);
         while (cap <= id) cap *= 2;
         RepDev* nd;
@@ -812,18 +814,13 @@ int Batch::step(const std::vector<int>& ids_in, int n, std::string& err, bool us
     SetMaps* m = maps_for(ids, err);
     if (!m) return -TILD_ERR_DEVICE;
     if (use_graph && !m->graph) {
-        // Capture one integrating step.  Every kernel argument is constant across steps:
-        // step counters and position parity live in the device descriptors and are
-        // advanced by the last node, so the same graph serves every step.
-        hipGraph_t g = nullptr;
-        HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_step(*m, true, false, err);
-        hipError_t ec = hipStreamEndCapture(stream_, &g);
-        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
-        HIPCHK(ec);
-        hipError_t ei = hipGraphInstantiate(&m->graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        HIPCHK(ei);
+        // Every kernel argument is constant across steps: step counters and position
+        // parity live in the device descriptors and are advanced by each step's last
+        // node, so one captured step serves every step, and kGraphSteps captured steps
+        // serve every run of that many (one graph launch instead of kGraphSteps; each
+        // launch leaves a gap of several microseconds on the device).
+        if (int rc = capture_steps(*m, 1, &m->graph, err)) return rc;
+        if (int rc = capture_steps(*m, kGraphSteps, &m->graph_run, err)) return rc;
     }
     // chunks end where a replica of the set is due for its spatial reorder; each
     // replica reorders at multiples of its own interval, whatever set it runs in
@@ -846,12 +843,34 @@ int Batch::step(const std::vector<int>& ids_in, int n, std::string& err, bool us
     return 0;
 }
 
+int Batch::capture_steps(SetMaps& m, int nsteps, hipGraphExec_t* out, std::string& err) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int s = 0; s < nsteps && !rc; ++s) rc = enqueue_step(m, true, false, err);
+    hipError_t ec = hipStreamEndCapture(stream_, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    HIPCHK(ec);
+    hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(ei);
+    return 0;
+}
+
+void Batch::drop_graphs(SetMaps& m) {
+    if (m.graph) { (void)hipGraphExecDestroy(m.graph); m.graph = nullptr; }
+    if (m.graph_run) { (void)hipGraphExecDestroy(m.graph_run); m.graph_run = nullptr; }
+}
+
 int Batch::run_steps(SetMaps& m, const std::vector<int>& ids, int n, bool use_graph, std::string& err) {
-    for (int s = 0; s < n; ++s) {
+    int s = 0;
+    if (use_graph)
+        for (; s + kGraphSteps <= n; s += kGraphSteps) HIPCHK(hipGraphLaunch(m.graph_run, stream_));
+    for (; s < n; ++s) {
         if (use_graph) HIPCHK(hipGraphLaunch(m.graph, stream_));
         else if (int rc = enqueue_step(m, true, false, err)) return rc;
-        for (int id : ids) reps_[id]->step += 1;
     }
+    for (int id : ids) reps_[id]->step += n;
     return 0;
 }
 

@@ -20,6 +20,7 @@
 namespace tild {
 
 constexpr int kXcds = 8;         // accelerator complex dies of an MI355X
+constexpr int kGraphSteps = 10;  // integrating steps per multi-step graph launch
 constexpr int kProfPhases = 8;   // spread, x_r2c, y_fwd, last_fused, y_inv, x_c2r, particles, advance
 
 // Work-map phases.  *A: replicas on the global-atomic spread; T*: replicas on the
@@ -113,6 +114,7 @@ public:
         int* d_ids = nullptr;
         int nids = 0;
         hipGraphExec_t graph = nullptr;   // one captured integrating step of this set
+        hipGraphExec_t graph_run = nullptr;   // kGraphSteps captured steps
     };
     SetMaps* maps_for(const std::vector<int>& ids, std::string& err);
 
@@ -137,6 +139,8 @@ private:
     // the current slots and is reused
     int reorder(int id, std::string& err, bool after_step = false);
     int run_steps(SetMaps& m, const std::vector<int>& ids, int n, bool use_graph, std::string& err);
+    int capture_steps(SetMaps& m, int nsteps, hipGraphExec_t* out, std::string& err);
+    static void drop_graphs(SetMaps& m);
     int fetch_pid(const Replica& R, std::vector<int>& pid, std::string& err);
     void free_maps(PhaseMap* maps);
 };

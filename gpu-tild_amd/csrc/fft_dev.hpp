@@ -149,8 +149,11 @@ template <int R, int NT>
 __device__ __forceinline__ void fft_stage(const float2* __restrict__ src, float2* __restrict__ dst, int ldb, int nb,
                                  int N, int p, const float2* __restrict__ tw) {
     const int m = N / R, nbf = m * nb;
+    const FastDiv dnb(nb), dp(p);
     for (int task = threadIdx.x; task < nbf; task += NT) {
-        const int b = task % nb, i = task / nb, k = i % p;
+        int b, k;
+        const int i = dnb.div(task, b);
+        dp.div(i, k);
         float2 v[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) v[r] = src[(i + r * m) * ldb + b];

@@ -86,20 +86,21 @@ __global__ void __launch_bounds__(kFftThreads) k_x_r2c(const RepDev* __restrict_
     float2* Hout = out_override ? out_override : r.H;
     gmem<unsigned long long> acc = G(r.acc);
     gmem<float> rho_out = G(r.rho);
+    const FastDiv dN(N);
+    const int type_rows = r.ntypes * rows_per_field;   // rows below are particle densities
     for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
-        const int b = task / N, n = task % N;
+        int n;
+        const int b = dN.div(task, n);
         const int ra = 2 * (p0 + b), rb = ra + 1;
         float a, bb = 0.f;
         if (FROM_ACC) {
-            const int fa = ra / rows_per_field;
             size_t ia = (size_t)ra * N + n;
-            a = (float)((double)(long long)acc[ia] * (fa < r.ntypes ? r.fix_inv : r.fixq_inv));
+            a = (float)((double)(long long)acc[ia] * (ra < type_rows ? r.fix_inv : r.fixq_inv));
             acc[ia] = 0ull;
             rho_out[ia] = a;
             if (rb < nrows) {
-                const int fb = rb / rows_per_field;
                 size_t ib = (size_t)rb * N + n;
-                bb = (float)((double)(long long)acc[ib] * (fb < r.ntypes ? r.fix_inv : r.fixq_inv));
+                bb = (float)((double)(long long)acc[ib] * (rb < type_rows ? r.fix_inv : r.fixq_inv));
                 acc[ib] = 0ull;
                 rho_out[ib] = bb;
             }
@@ -111,10 +112,12 @@ __global__ void __launch_bounds__(kFftThreads) k_x_r2c(const RepDev* __restrict_
     }
     __syncthreads();
     const float2* res = lds_fft(lds, lds + (size_t)N * ldb, ldb, nb, r.ax[0]);
+    const FastDiv dNxh(Nxh);
     for (int task = threadIdx.x; task < nb * Nxh; task += kFftThreads) {
-        const int b = task / Nxh, k = task % Nxh;
+        int k;
+        const int b = dNxh.div(task, k);
         const float2 Zk = res[k * ldb + b];
-        const float2 Zm = res[((N - k) % N) * ldb + b];
+        const float2 Zm = res[(k == 0 ? 0 : N - k) * ldb + b];
         const int ra = 2 * (p0 + b), rb = ra + 1;
         gst(Hout + (size_t)ra * Nxh + k, make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y)));
         if (rb < nrows) gst(Hout + (size_t)rb * Nxh + k, make_float2(0.5f * (Zk.y + Zm.y), 0.5f * (Zm.x - Zk.x)));
@@ -139,15 +142,18 @@ __global__ void __launch_bounds__(kFftThreads) k_mid(const RepDev* __restrict__ 
     const int nb = min(g.nl, g.batch - b0);
     const int ldb = nb | 1;
     float2* base = buf + (size_t)o * g.ostride + b0;
+    const FastDiv dnb(nb);
     for (int task = threadIdx.x; task < nb * g.N; task += kFftThreads) {
-        const int b = task % nb, n = task / nb;
+        int b;
+        const int n = dnb.div(task, b);
         float2 v = gld(base + (size_t)n * g.lstride + b);
         lds[n * ldb + b] = inverse ? conjf2(v) : v;
     }
     __syncthreads();
     const float2* res = lds_fft(lds, lds + (size_t)g.N * ldb, ldb, nb, r.ax[axis]);
     for (int task = threadIdx.x; task < nb * g.N; task += kFftThreads) {
-        const int b = task % nb, n = task / nb;
+        int b;
+        const int n = dnb.div(task, b);
         float2 v = res[n * ldb + b];
         gst(base + (size_t)n * g.lstride + b, inverse ? conjf2(v) : v);
     }
@@ -183,8 +189,18 @@ __global__ void __launch_bounds__(kFftThreads) k_last_fused(const RepDev* __rest
     float2* Wb = lds;
     float2* E0 = lds + (size_t)N * ldb;
     float2* E1 = E0 + (size_t)N * ldg;
+    // x and y index of each column of the block (3D: column q = y * Nxh + x)
+    int* qx = (int*)(E1 + (size_t)N * ldg);
+    int* qy = qx + nb;
+    for (int b = threadIdx.x; b < nb; b += kFftThreads) {
+        const int q = q0 + b;
+        qx[b] = la == 2 ? q % r.Nxh : q;
+        qy[b] = la == 2 ? q / r.Nxh : 0;
+    }
+    const FastDiv dnb(nb), dnf(nf);
     for (int task = threadIdx.x; task < nlines * N; task += kFftThreads) {
-        const int b = task % nb, rest = task / nb, f = rest % nf, n = rest / nf;
+        int b, f;
+        const int rest = dnb.div(task, b), n = dnf.div(rest, f);
         E0[n * ldb + f * nb + b] = gld(r.H + (size_t)f * r.Mh + (size_t)n * Q + q0 + b);
     }
     __syncthreads();
@@ -194,7 +210,8 @@ __global__ void __launch_bounds__(kFftThreads) k_last_fused(const RepDev* __rest
     const int nrows = r.ctab_rows, nlc = r.nl_last;
     gmem<const float> cb = G(r.Ctab) + (size_t)wi.blk * N * nrows * nlc;
     for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
-        const int b = task % nb, n = task / nb;
+        int b;
+        const int n = dnb.div(task, b);
         const int row = n * ldb + b;
         for (int I = 0; I < T; ++I) {
             float2 acc = make_float2(0.f, 0.f);
@@ -218,14 +235,14 @@ __global__ void __launch_bounds__(kFftThreads) k_last_fused(const RepDev* __rest
     for (int d0 = 0; d0 < D; d0 += DB) {
         const int nd = min(DB, D - d0), nlg = nlines * nd;
         for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
-            const int b = task % nb, n = task / nb;
-            const int q = q0 + b;
+            int b;
+            const int n = dnb.div(task, b);
             for (int dd = 0; dd < nd; ++dd) {
                 const int d = d0 + dd;
                 int sidx, Nd;
-                if (d == 0) { sidx = q % r.Nxh; Nd = r.Nx; }
+                if (d == 0) { sidx = qx[b]; Nd = r.Nx; }
                 else if (d == la) { sidx = n; Nd = N; }
-                else { sidx = q / r.Nxh; Nd = r.Ny; }
+                else { sidx = qy[b]; Nd = r.Ny; }
                 float kd;
                 if (2 * sidx == Nd) kd = 0.f;
                 else kd = r.kfac[d] * (float)(2 * sidx < Nd ? sidx : sidx - Nd);
@@ -238,9 +255,11 @@ __global__ void __launch_bounds__(kFftThreads) k_last_fused(const RepDev* __rest
         }
         __syncthreads();
         const float2* res = lds_fft(E0, E1, ldg, nlg, r.ax[la]);
+        const FastDiv dfd(nf * nd), dnd(nd);
         for (int task = threadIdx.x; task < nlg * N; task += kFftThreads) {
-            const int b = task % nb, rest = task / nb, fd = rest % (nf * nd), n = rest / (nf * nd);
-            const int f = fd / nd, d = d0 + fd % nd;
+            int b, fd, dr;
+            const int rest = dnb.div(task, b), n = dfd.div(rest, fd);
+            const int f = dnd.div(fd, dr), d = d0 + dr;
             gst(r.Gk + (size_t)(f * D + d) * r.Mh + (size_t)n * Q + q0 + b, conjf2(res[n * ldg + fd * nb + b]));
         }
         __syncthreads();
@@ -273,9 +292,11 @@ __global__ void __launch_bounds__(kFftThreads) k_x_c2r(const RepDev* __restrict_
     const int ldb = nb | 1;
     const float2* Gin = RECORDS ? (const float2*)r.Gk : in_override;
     const bool even = (N & 1) == 0;
+    const FastDiv dN(N), dnc(nc);
     for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
-        const int k = task % N, b = task / N;
-        const int row = row0 + b / nc, j = b % nc;
+        int k, j;
+        const int b = dN.div(task, k);
+        const int row = row0 + dnc.div(b, j);
         const int la = 2 * j, lb = la + 1;
         const bool upper = k >= Nxh;
         const int kk = upper ? N - k : k;
@@ -290,8 +311,10 @@ __global__ void __launch_bounds__(kFftThreads) k_x_c2r(const RepDev* __restrict_
     const float2* res = lds_fft(lds, lds + (size_t)N * ldb, ldb, nb, r.ax[0]);
     if (RECORDS) {
         const int D = r.dim;
+        const FastDiv dnf(r.nf);
         for (int task = threadIdx.x; task < rows * r.nf * N; task += kFftThreads) {
-            const int x = task % N, rest = task / N, f = rest % r.nf, rr = rest / r.nf;
+            int x, f;
+            const int rest = dN.div(task, x), rr = dnf.div(rest, f);
             const int row = row0 + rr;
             float g[3] = {0.f, 0.f, 0.f};
             for (int d = 0; d < D; ++d) {
@@ -314,8 +337,10 @@ __global__ void __launch_bounds__(kFftThreads) k_x_c2r(const RepDev* __restrict_
             gst3(r.G3 + ((size_t)f * r.M + node) * 3, g[0], g[1], g[2]);
         }
     } else {
+        const FastDiv dnl(nlines);
         for (int task = threadIdx.x; task < rows * nlines * N; task += kFftThreads) {
-            const int x = task % N, rest = task / N, line = rest % nlines, rr = rest / nlines;
+            int x, line;
+            const int rest = dN.div(task, x), rr = dnl.div(rest, line);
             const float2 z = res[x * ldb + rr * nc + line / 2];
             G(out_override)[((size_t)line * rows_per_field + row0 + rr) * N + x] = (line & 1) ? -z.y : z.x;
         }
@@ -340,17 +365,20 @@ __global__ void __launch_bounds__(NT) k_plane_r2c(const RepDev* __restrict__ rep
     float2* A = lds;
     float2* B = lds + bsz;
     gmem<const float> rho = G((const float*)r.rho) + (size_t)f * r.M + (size_t)z * Ny * Nx;
+    const FastDiv dNx(Nx), dNxh(Nxh);
     for (int task = threadIdx.x; task < np * Nx; task += NT) {
-        const int n = task % Nx, j = task / Nx;
+        int n;
+        const int j = dNx.div(task, n);
         A[n * ldp + j] = make_float2(rho[(size_t)(2 * j) * Nx + n], rho[(size_t)(2 * j + 1) * Nx + n]);
     }
     __syncthreads();
     float2* Z = lds_fft<NT>(A, B, ldp, np, r.ax[0]);
     float2* C = (Z == A) ? B : A;
     for (int task = threadIdx.x; task < np * Nxh; task += NT) {
-        const int k = task % Nxh, j = task / Nxh;
+        int k;
+        const int j = dNxh.div(task, k);
         const float2 Zk = Z[k * ldp + j];
-        const float2 Zm = Z[((Nx - k) % Nx) * ldp + j];
+        const float2 Zm = Z[(k == 0 ? 0 : Nx - k) * ldp + j];
         C[(2 * j) * ldc + k] = make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y));
         C[(2 * j + 1) * ldc + k] = make_float2(0.5f * (Zk.y + Zm.y), 0.5f * (Zm.x - Zk.x));
     }
@@ -358,7 +386,8 @@ __global__ void __launch_bounds__(NT) k_plane_r2c(const RepDev* __restrict__ rep
     const float2* Y = lds_fft<NT>(C, Z, ldc, Nxh, r.ax[1]);
     float2* H = r.H + ((size_t)f * r.Nz + z) * Ny * Nxh;
     for (int task = threadIdx.x; task < Ny * Nxh; task += NT) {
-        const int k = task % Nxh, y = task / Nxh;
+        int k;
+        const int y = dNxh.div(task, k);
         gst(H + (size_t)y * Nxh + k, Y[y * ldc + k]);
     }
 }
@@ -383,8 +412,10 @@ __global__ void __launch_bounds__(NT) k_plane_c2r(const RepDev* __restrict__ rep
     const int L1 = D * Nxh, ld1 = L1 | 1;
     float2* A = lds;
     float2* B = lds + (size_t)Ny * ld1;
+    const FastDiv dNx(Nx), dNxh(Nxh);
     for (int task = threadIdx.x; task < Ny * L1; task += NT) {
-        const int kx = task % Nxh, rest = task / Nxh, d = rest % D, y = rest / D;
+        int kx, d;
+        const int rest = dNxh.div(task, kx), y = FastDiv(D).div(rest, d);
         const float2 v = gld(r.Gk + (size_t)(f * D + d) * r.Mh + ((size_t)z * Ny + y) * Nxh + kx);
         A[y * ld1 + d * Nxh + kx] = conjf2(v);       // inverse as conj(FFT(conj))
     }
@@ -401,7 +432,8 @@ __global__ void __launch_bounds__(NT) k_plane_c2r(const RepDev* __restrict__ rep
         const int rows = min(cr, Ny - y0);
         const int nb = rows * NC;
         for (int task = threadIdx.x; task < nb * Nx; task += NT) {
-            const int k = task % Nx, b = task / Nx;
+            int k;
+            const int b = dNx.div(task, k);
             const int rr = b / NC, j = b % NC;
             const int la = 2 * j, lb = la + 1;
             const bool upper = k >= Nxh;
@@ -416,7 +448,8 @@ __global__ void __launch_bounds__(NT) k_plane_c2r(const RepDev* __restrict__ rep
         __syncthreads();
         const float2* res = lds_fft<NT>(X1, X2, ldb, nb, r.ax[0]);
         for (int task = threadIdx.x; task < rows * Nx; task += NT) {
-            const int x = task % Nx, rr = task / Nx;
+            int x;
+            const int rr = dNx.div(task, x);
             float g[3];
 #pragma unroll
             for (int d = 0; d < D; ++d) {

@@ -37,7 +37,8 @@ inline size_t lds_bytes(int N, int nlines) { return 2 * fft_buf_bytes(N, nlines)
 // fused last-axis pass: coupled fields (nf*nl lines) + two buffers of nf*DB*nl lines
 // (DB directions per inverse batch)
 inline size_t last_lds_bytes(int N, int nf, int DB, int nl) {
-    return fft_buf_bytes(N, nf * nl) + 2 * fft_buf_bytes(N, nf * DB * nl);
+    // coupled fields, two transform buffers, x and y index of each column
+    return fft_buf_bytes(N, nf * nl) + 2 * fft_buf_bytes(N, nf * DB * nl) + 2 * (size_t)nl * sizeof(int);
 }
 
 struct LaunchCtx {

@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(256) k_tile_count(const RepDev* __restrict__ r
     __syncthreads();
     for (int k = threadIdx.x; k < kCountWin; k += 256) {
         const int c = hist[k];
-        if (c) hist[k] = atomicAdd(r.tcnt + bmin + k, c);   // first global slot of the bucket
+        if (c) hist[k] = atomicAdd(r.tcnt + ((bmin + k) << r.tshift), c);   // first global slot of the bucket
     }
     __syncthreads();
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(256) k_tile_count(const RepDev* __restrict__ r
             const int bl = __shfl(b[k], leader);
             const unsigned long long same = __ballot(b[k] == bl) & todo;
             int base = 0;
-            if (lane == leader) base = atomicAdd(r.tcnt + bl, __popcll(same));
+            if (lane == leader) base = atomicAdd(r.tcnt + (bl << r.tshift), __popcll(same));
             base = __shfl(base, leader);
             if (b[k] == bl && !inwin[k]) slot = base + __popcll(same & below);
             todo &= ~same;
@@ -100,33 +100,68 @@ __global__ void __launch_bounds__(256) k_tile_count(const RepDev* __restrict__ r
     }
 }
 
-// Exclusive scan of the bucket counts; clears the counts for the next step.
+// Exclusive scan of the bucket counts; clears the counts for the next step.  Each
+// thread holds up to kScanPer consecutive counts in registers (their loads go out
+// together); the block scan is a wave-level shuffle scan plus a scan of the 16 wave
+// totals.  Larger bucket counts loop over the same in chunks.
+constexpr int kScanPer = 8;
+
+__device__ __forceinline__ int block_exclusive_scan_1024(int v, int* wsum, int& total) {
+    const int lane = (int)__lane_id(), w = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    if (w == 0) {
+        int t = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            const int y = __shfl_up(t, o);
+            if (lane >= o) t += y;
+        }
+        if (lane < 16) wsum[16 + lane] = t;   // inclusive wave prefix
+    }
+    __syncthreads();
+    total = wsum[31];
+    const int before = w > 0 ? wsum[16 + w - 1] : 0;
+    __syncthreads();   // wsum is reused by the next call
+    return before + x - v;
+}
+
 __global__ void __launch_bounds__(1024) k_tile_scan(const RepDev* __restrict__ reps,
                                                     const WorkItem* __restrict__ wmap) {
-    __shared__ int part[1024];
+    __shared__ int wsum[32];
     const WorkItem wi = wmap[blockIdx.x];
     const RepDev& r = reps[wi.rep];
-    const int nb = r.nbuckets;
-    const int per = (nb + 1023) / 1024;
-    const int b0 = threadIdx.x * per, b1 = min(nb, b0 + per);
-    int s = 0;
-    for (int b = b0; b < b1; ++b) s += r.tcnt[b];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {      // Hillis-Steele inclusive scan
-        int v = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
+    const int nb = r.nbuckets, sh = r.tshift;
+    gmem<int> cnt = G(r.tcnt);
+    gmem<int> off = G(r.toff);
+    int run = 0;   // buckets before this chunk
+    for (int c0 = 0; c0 < nb; c0 += 1024 * kScanPer) {
+        const int b0 = c0 + threadIdx.x * kScanPer;
+        int c[kScanPer];
+        int s = 0;
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) {
+            c[k] = b0 + k < nb ? cnt[(b0 + k) << sh] : 0;
+            s += c[k];
+        }
+        int total;
+        int pre = run + block_exclusive_scan_1024(s, wsum, total);
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k)
+            if (b0 + k < nb) {
+                off[b0 + k] = pre;
+                pre += c[k];
+                cnt[(b0 + k) << sh] = 0;
+            }
+        run += total;
     }
-    int run = threadIdx.x > 0 ? part[threadIdx.x - 1] : 0;
-    for (int b = b0; b < b1; ++b) {
-        const int c = r.tcnt[b];
-        r.toff[b] = run;
-        run += c;
-        r.tcnt[b] = 0;
-    }
-    if (threadIdx.x == 1023) r.toff[nb] = part[1023];
+    if (threadIdx.x == 0) off[nb] = run;
 }
 
 __global__ void __launch_bounds__(256) k_tile_scatter(const RepDev* __restrict__ reps,
@@ -261,13 +296,16 @@ __global__ void __launch_bounds__(kFftThreads) k_tile_deposit(const RepDev* __re
     // densities of the tile rows -> rho and the x-axis R2C (pairs of rows)
     const int npairs = (rows + 1) / 2;
     const int ldb = npairs | 1;
+    const FastDiv dNx(Nx), dNxh(Nxh), dBy(By), dBz(Bz);
     for (int task = threadIdx.x; task < npairs * Nx; task += kFftThreads) {
-        const int j = task / Nx, n = task % Nx;
+        int n;
+        const int j = dNx.div(task, n);
         float v[2] = {0.f, 0.f};
         for (int h = 0; h < 2; ++h) {
             const int rr = 2 * j + h;
             if (rr >= rows) break;
-            const int f = rr / (By * Bz), lz = (rr / By) % Bz, ly = rr % By;
+            int ly, lz;
+            const int f = dBz.div(dBy.div(rr, ly), lz);
             const double inv = f < T ? r.fix_inv : r.fixq_inv;
             v[h] = (float)((double)(long long)patch[(size_t)rr * Nx + n] * inv);
             G(r.rho)[(size_t)f * r.M + ((size_t)(z0 + lz) * r.Ny + (y0 + ly)) * Nx + n] = v[h];
@@ -278,13 +316,15 @@ __global__ void __launch_bounds__(kFftThreads) k_tile_deposit(const RepDev* __re
     __syncthreads();
     const float2* res = lds_fft(fbuf, fbuf + (size_t)Nx * ldb, ldb, npairs, r.ax[0]);
     for (int task = threadIdx.x; task < npairs * Nxh; task += kFftThreads) {
-        const int j = task / Nxh, k = task % Nxh;
+        int k;
+        const int j = dNxh.div(task, k);
         const float2 Zk = res[k * ldb + j];
-        const float2 Zm = res[((Nx - k) % Nx) * ldb + j];
+        const float2 Zm = res[(k == 0 ? 0 : Nx - k) * ldb + j];
         for (int h = 0; h < 2; ++h) {
             const int rr = 2 * j + h;
             if (rr >= rows) break;
-            const int f = rr / (By * Bz), lz = (rr / By) % Bz, ly = rr % By;
+            int ly, lz;
+            const int f = dBz.div(dBy.div(rr, ly), lz);
             const size_t row = ((size_t)f * r.Nz + (z0 + lz)) * r.Ny + (y0 + ly);
             gst(r.H + row * Nxh + k, h == 0 ? make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y))
                                             : make_float2(0.5f * (Zk.y + Zm.y), 0.5f * (Zm.x - Zk.x)));

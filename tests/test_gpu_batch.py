@@ -49,9 +49,10 @@ def test_graph_replays_match_eager_with_interleaved_work(tmp_path):
     gb = tild.Batch(device=0)
     rids = [ref.add(s, 7 + i) for i, (s, _) in enumerate(systems)]
     gids = [gb.add(s, 7 + i) for i, (s, _) in enumerate(systems)]
-    for rep in range(4):
-        ref.step(2, graph=False)
-        gb.step(2, graph=True)
+    # runs of 10 or more steps replay the 10-step graph, the rest the one-step graph
+    for rep, n in enumerate([2, 23, 2, 11]):
+        ref.step(n, graph=False)
+        gb.step(n, graph=True)
         # eager work and synchronisation between replays
         for r in gids:
             gb.properties(r)
