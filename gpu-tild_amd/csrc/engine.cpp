@@ -526,10 +526,13 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
         if (h.plane_rows <= 0) add(PH_C2R, (rows + h.nr_c2r - 1) / h.nr_c2r, lds_bytes(h.Nx, h.nr_c2r * nc));
         add(PH_KPROPS, h.nchunk > 0 ? h.nchunk : 1, 0);
     }
-    // Workgroups go to the 8 XCDs round-robin (launch block w runs on XCD w % 8).  The
-    // particle-order passes hand each XCD a contiguous run of work items, i.e. one slab
-    // of a spatially ordered replica, so that its L2 holds that slab's grid records.
-    for (Phase ph : {PH_PART, PH_TPART, PH_TCOUNT, PH_SPREADA, PH_TILE}) {
+    // Workgroups go to the 8 XCDs round-robin (launch block w runs on XCD w % 8).  Every
+    // pass hands each XCD a contiguous run of work items: for the particle-order passes
+    // one slab of a spatially ordered replica, so that its L2 holds that slab's grid
+    // records; for the FFT passes adjacent column batches, whose rows share cache lines
+    // that a single L2 then fetches once.
+    for (Phase ph : {PH_PART, PH_TPART, PH_TCOUNT, PH_SPREADA, PH_TILE, PH_XA, PH_XP, PH_MIDY_H, PH_MIDY_G,
+                     PH_MIDZ_H, PH_PY, PH_LAST, PH_C2R, PH_PLANE, PH_PLANEF}) {
         std::vector<WorkItem>& v = maps[ph].host;
         const size_t n = v.size(), q = n / kXcds, rm = n % kXcds;
         if (n < 2 * (size_t)kXcds) continue;
