@@ -2,6 +2,8 @@
 // work maps and the per-step launch sequence.
 #include "engine.hpp"
 
+#include <functional>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -22,19 +24,30 @@ int make_axis_plan(int N, AxisPlan& pl, std::vector<float2>& tw, std::string& er
     std::memset(&pl, 0, sizeof(pl));
     pl.N = N;
     if (N < 1 || N > kFftBudget) { err = "FFT length " + std::to_string(N) + " outside [1, " + std::to_string(kFftBudget) + "]"; return -TILD_ERR_UNSUPPORTED; }
-    int n = N, e2 = 0;
-    while (n % 2 == 0) { n /= 2; ++e2; }
-    // powers of two in as few stages as possible (at most 16 per stage), spread evenly:
-    // every stage ends with a barrier, and the passes are latency-bound
+    // As few stages as possible (every stage is an LDS round trip ending with a
+    // barrier, and the passes are latency-bound), then the smallest largest radix
+    // (radices spread evenly), larger radices first.
     std::vector<int> rad;
-    const int ns2 = (e2 + 3) / 4;
-    for (int k = 0; k < ns2; ++k) {
-        const int bits = e2 / ns2 + (k < e2 % ns2 ? 1 : 0);
-        rad.push_back(1 << bits);
+    {
+        static const int kRadices[] = {16, 10, 8, 7, 5, 4, 3, 2};
+        std::vector<int> cur;
+        std::function<void(int)> search = [&](int n) {
+            if (n == 1) {
+                auto mx = [](const std::vector<int>& v) { return *std::max_element(v.begin(), v.end()); };
+                if (rad.empty() || cur.size() < rad.size() || (cur.size() == rad.size() && mx(cur) < mx(rad))) rad = cur;
+                return;
+            }
+            if (!rad.empty() && cur.size() + 1 > rad.size()) return;
+            for (int R : kRadices)
+                if (n % R == 0 && (cur.empty() || R <= cur.back())) {
+                    cur.push_back(R);
+                    search(n / R);
+                    cur.pop_back();
+                }
+        };
+        if (N > 1) search(N);
     }
-    for (int p : {3, 5, 7})
-        while (n % p == 0) { rad.push_back(p); n /= p; }
-    if (n != 1) {
+    if (rad.empty() && N > 1) {
         err = "FFT length " + std::to_string(N) + " has a prime factor above 7";
         return -TILD_ERR_UNSUPPORTED;
     }

@@ -21,7 +21,7 @@ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(
 __device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }   // a * (-i)
 __device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
 
-// Forward DFT of small sizes (2, 3, 4, 5, 7, 8, 16), in place on v[0..R-1].
+// Forward DFT of small sizes (2, 3, 4, 5, 7, 8, 10, 16), in place on v[0..R-1].
 template <int R> struct Dft;
 
 template <> struct Dft<2> {
@@ -137,6 +137,27 @@ template <> struct Dft<5> {
         DftOdd<5>::run(v, c, s);
     }
 };
+// 10 = 2 x 5 by the prime-factor mapping (2 and 5 are coprime, so no twiddles
+// between the two): inputs n = (5 n1 + 2 n2) mod 10 form five DFT-2 pairs, then two
+// DFT-5 over n2 give outputs k = (5 k1 + 6 k2) mod 10.
+template <> struct Dft<10> {
+    __device__ __forceinline__ static void run(float2* v) {
+        float2 a[5], b[5];
+#pragma unroll
+        for (int n2 = 0; n2 < 5; ++n2) {
+            const float2 x0 = v[(2 * n2) % 10], x1 = v[(5 + 2 * n2) % 10];
+            a[n2] = cadd(x0, x1);   // k1 = 0
+            b[n2] = csub(x0, x1);   // k1 = 1
+        }
+        Dft<5>::run(a);
+        Dft<5>::run(b);
+#pragma unroll
+        for (int k2 = 0; k2 < 5; ++k2) {
+            v[(6 * k2) % 10] = a[k2];
+            v[(5 + 6 * k2) % 10] = b[k2];
+        }
+    }
+};
 template <> struct Dft<7> {
     __device__ __forceinline__ static void run(float2* v) {
         const float c[4] = {1.f, 0.62348980185873353053f, -0.22252093395631440429f, -0.90096886790241912624f};
@@ -189,6 +210,7 @@ __device__ __forceinline__ float2* lds_fft(float2* A, float2* B, int ldb, int nb
             case 4: fft_stage<4, NT>(src, dst, ldb, nb, pl.N, p, tw); break;
             case 5: fft_stage<5, NT>(src, dst, ldb, nb, pl.N, p, tw); break;
             case 7: fft_stage<7, NT>(src, dst, ldb, nb, pl.N, p, tw); break;
+            case 10: fft_stage<10, NT>(src, dst, ldb, nb, pl.N, p, tw); break;
             case 16: fft_stage<16, NT>(src, dst, ldb, nb, pl.N, p, tw); break;
             default: fft_stage<8, NT>(src, dst, ldb, nb, pl.N, p, tw); break;
         }
