@@ -170,23 +170,25 @@ template <int R, int NT>
 __device__ __forceinline__ void fft_stage(const float2* __restrict__ src, float2* __restrict__ dst, int ldb, int nb,
                                  int N, int p, const float2* __restrict__ tw) {
     const int m = N / R, nbf = m * nb;
-    const FastDiv dnb(nb), dp(p);
+    const FastDiv dnb(nb), dvp(p);
+    const int sst = m * ldb, dst_st = p * ldb;   // uniform strides: one add per LDS address
     for (int task = threadIdx.x; task < nbf; task += NT) {
         int b, k;
         const int i = dnb.div(task, b);
-        dp.div(i, k);
+        dvp.div(i, k);
+        const float2* sp = src + (i * ldb + b);
+        float2* op = dst + (((i - k) * R + k) * ldb + b);
         float2 v[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = src[(i + r * m) * ldb + b];
+        for (int r = 0; r < R; ++r) v[r] = sp[r * sst];
         if (p > 1) {
             const float2* w = tw + k * (R - 1);
 #pragma unroll
             for (int r = 1; r < R; ++r) v[r] = cmul(v[r], gld(w + r - 1));
         }
         Dft<R>::run(v);
-        const int o = (i - k) * R + k;
 #pragma unroll
-        for (int q = 0; q < R; ++q) dst[(o + q * p) * ldb + b] = v[q];
+        for (int q = 0; q < R; ++q) op[q * dst_st] = v[q];
     }
 }
 

@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(256) k_spread(const RepDev* __restrict__ reps,
 // FROM_ACC: rows come from the fixed-point accumulators (converted, written to rho
 // and cleared); otherwise from rho.
 template <bool FROM_ACC>
-__global__ void __launch_bounds__(kFftThreads) k_x_r2c(const RepDev* __restrict__ reps,
+__global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(8))) k_x_r2c(const RepDev* __restrict__ reps,
                                                        const WorkItem* __restrict__ wmap,
                                                        int nrows_override, const float* __restrict__ in_override,
                                                        float2* __restrict__ out_override) {
@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(kFftThreads) k_x_r2c(const RepDev* __restrict_
 // FFT along axis `axis` (1 = y, 2 = z) of nfb half-spectrum fields stored in buf
 // (x fastest, Nxh columns).  Lines are batched over the contiguous columns so that
 // every global access is a run of consecutive complex values.
-__global__ void __launch_bounds__(kFftThreads) k_mid(const RepDev* __restrict__ reps,
+__global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(8))) k_mid(const RepDev* __restrict__ reps,
                                                      const WorkItem* __restrict__ wmap,
                                                      int axis, int use_gk, int nfb_override,
                                                      int inverse, float2* __restrict__ buf_override) {
@@ -274,7 +274,7 @@ __global__ void __launch_bounds__(kFftThreads) k_last_fused(const RepDev* __rest
 // RECORDS: output float4 (g_0, g_1, g_2, rho_f) per field and node (fields are lines
 // f*dim + d of Gk); otherwise plain real rows to out_override.
 template <bool RECORDS>
-__global__ void __launch_bounds__(kFftThreads) k_x_c2r(const RepDev* __restrict__ reps,
+__global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(8))) k_x_c2r(const RepDev* __restrict__ reps,
                                                        const WorkItem* __restrict__ wmap,
                                                        int nlines_override, const float2* __restrict__ in_override,
                                                        float* __restrict__ out_override) {
