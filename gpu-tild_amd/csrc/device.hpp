@@ -43,7 +43,6 @@ struct RepDev {
     int nl_last;   // batch columns per block, fused last-axis pass
     int nr_c2r;    // rows per block, inverse x pass
     int mid_elems; // complex values per block of the middle-axis passes (0: kFftBudget)
-    int plane_rows;// rows per x chunk of the plane inverse pass (0: pass not used)
     int plane_fwd; // 1: the tiled deposit writes rho only; x R2C + y FFT run per plane
     int last_dirs; // directions per inverse batch of the fused last-axis pass (1..dim)
     // particle state

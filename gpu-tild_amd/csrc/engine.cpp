@@ -220,7 +220,7 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
     const int npairs_x = (h.nf * rows + 1) / 2;
     h.nl_x = std::max(1, std::min(kFftBudget / h.Nx, npairs_x));
     const int la = D - 1, Nla = la == 2 ? h.Nz : h.Ny, Q = la == 2 ? h.Ny * h.Nxh : h.Nxh;
-    h.last_dirs = tune_.last_dirs > 0 ? std::min(tune_.last_dirs, D) : 1;
+    h.last_dirs = tune_.last_dirs > 0 ? std::min(tune_.last_dirs, 2) : 1;
     if (Nla * h.nf > kFftBudget || last_lds_bytes(Nla, h.nf, h.last_dirs, 1) > kPlaneLds) { err = "grid too large for the fused k-space pass"; return -TILD_ERR_UNSUPPORTED; }
     // about 1,280 complex values per field set per block measured best on the
     // 80x80x160 melt (fewer, fuller blocks lose to the per-block latency chain)
@@ -232,22 +232,15 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
     // about 1,000 complex values per block measured best for the inverse x pass
     h.nr_c2r = std::max(1, std::min(1024 / (nc * h.Nx), rows));
     // optional block geometry (batching of independent lines: results do not change)
-    if (tune_.last_lines > 0) h.nl_last = std::min(h.nl_last, tune_.last_lines);
-    if (tune_.c2r_rows > 0) h.nr_c2r = std::min(h.nr_c2r, tune_.c2r_rows);
-    if (tune_.mid_elems > 0) h.mid_elems = std::min(kFftBudget, tune_.mid_elems);
-    // fused y-inverse + x C2R per (field, z) plane when the D direction spectra of a
-    // plane and their ping-pong partner fit one block's LDS
-    // (opt-in: at one 157 KiB block per CU it measured slower than the two passes on
-    // the 80x80x160 melt)
-    h.plane_rows = 0;
-    if (D == 3 && tune_.plane > 0) {
-        const size_t ld1 = (size_t)((3 * h.Nxh) | 1);
-        if (2 * (size_t)h.Ny * ld1 * sizeof(float2) <= kPlaneLds) {
-            int cr = h.Ny;
-            while (cr > 0 && 2 * (size_t)h.Nx * (size_t)((2 * cr) | 1) > (size_t)h.Ny * ld1) --cr;
-            h.plane_rows = cr;
-        }
+    if (tune_.last_lines > 0) {
+        h.nl_last = std::min(tune_.last_lines, Q);
+        while (h.nl_last > 1 && last_lds_bytes(Nla, h.nf, h.last_dirs, h.nl_last) > kPlaneLds) --h.nl_last;
     }
+    if (tune_.c2r_rows > 0) {
+        h.nr_c2r = std::min(tune_.c2r_rows, rows);
+        while (h.nr_c2r > 1 && (size_t)h.nr_c2r * nc * h.Nx > (size_t)kFftBudget) --h.nr_c2r;
+    }
+    if (tune_.mid_elems > 0) h.mid_elems = std::min(kFftBudget, tune_.mid_elems);
     h.nl_mid = mid_geom(h, 1, h.nf).nl;
 
     // particle state
@@ -524,19 +517,15 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
             else if (D == 3) add(PH_MIDY_H, g.nouter * g.nchunk, lds_bytes(g.N, g.nl));
         }
         if (D == 3) {
-            if (h.plane_rows > 0) {
-                add(PH_PLANE, h.nf * h.Nz, 2 * (size_t)h.Ny * (size_t)((3 * h.Nxh) | 1) * sizeof(float2));
-            } else {
-                MidGeom g = mid_geom(h, 1, h.nf * D);
-                add(PH_MIDY_G, g.nouter * g.nchunk, lds_bytes(g.N, g.nl));
-            }
+            MidGeom g = mid_grad_geom(h);
+            add(PH_MIDY_G, g.nouter * g.nchunk, lds_bytes(g.N, 3 * g.nl));
             MidGeom gz = mid_geom(h, 2, h.nf);
             add(PH_MIDZ_H, gz.nouter * gz.nchunk, lds_bytes(gz.N, gz.nl));
         }
         const int la = D - 1, Nla = la == 2 ? h.Nz : h.Ny, Q = la == 2 ? h.Ny * h.Nxh : h.Nxh;
         add(PH_LAST, (Q + h.nl_last - 1) / h.nl_last, last_lds_bytes(Nla, h.nf, h.last_dirs, h.nl_last));
         const int nc = (h.nf * D + 1) / 2;
-        if (h.plane_rows <= 0) add(PH_C2R, (rows + h.nr_c2r - 1) / h.nr_c2r, lds_bytes(h.Nx, h.nr_c2r * nc));
+        add(PH_C2R, (rows + h.nr_c2r - 1) / h.nr_c2r, lds_bytes(h.Nx, h.nr_c2r * nc));
         add(PH_KPROPS, h.nchunk > 0 ? h.nchunk : 1, 0);
     }
     // Workgroups go to the 8 XCDs round-robin (launch block w runs on XCD w % 8).  Every
@@ -545,7 +534,7 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
     // records; for the FFT passes adjacent column batches, whose rows share cache lines
     // that a single L2 then fetches once.
     for (Phase ph : {PH_PART, PH_TPART, PH_TCOUNT, PH_SPREADA, PH_TILE, PH_XA, PH_XP, PH_MIDY_H, PH_MIDY_G,
-                     PH_MIDZ_H, PH_PY, PH_LAST, PH_C2R, PH_PLANE, PH_PLANEF}) {
+                     PH_MIDZ_H, PH_PY, PH_LAST, PH_C2R, PH_PLANEF}) {
         std::vector<WorkItem>& v = maps[ph].host;
         const size_t n = v.size(), q = n / kXcds, rm = n % kXcds;
         if (n < 2 * (size_t)kXcds) continue;
@@ -559,7 +548,7 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
     for (int p = 0; p < PH_COUNT; ++p) {
         if (maps[p].dev) { (void)hipFree(maps[p].dev); maps[p].dev = nullptr; }
         maps[p].n = (int)maps[p].host.size();
-        if (maps[p].lds > ((p == PH_PLANE || p == PH_PLANEF || p == PH_LAST) ? kPlaneLds : (size_t)kMaxLds)) { err = "pass needs more LDS than a block may use"; return -TILD_ERR_UNSUPPORTED; }
+        if (maps[p].lds > ((p == PH_PLANEF || p == PH_LAST) ? kPlaneLds : (size_t)kMaxLds)) { err = "pass needs more LDS than a block may use"; return -TILD_ERR_UNSUPPORTED; }
         if (maps[p].n) {
             HIPCHK(hipMalloc((void**)&maps[p].dev, maps[p].n * sizeof(WorkItem)));
             HIPCHK(hipMemcpy(maps[p].dev, maps[p].host.data(), maps[p].n * sizeof(WorkItem), hipMemcpyHostToDevice));
@@ -804,9 +793,8 @@ int Batch::enqueue_step(SetMaps& m, bool integrate, bool write_forces, std::stri
     HIPCHK(launch_plane_r2c(ctx(PH_PLANEF)));
     HIPCHK(launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr));
     HIPCHK(launch_last_fused(ctx(PH_LAST), 0));
-    HIPCHK(launch_mid(ctx(PH_MIDY_G), 1, 1, 0, 1, nullptr));
+    HIPCHK(launch_mid_grad(ctx(PH_MIDY_G)));
     HIPCHK(launch_x_c2r(ctx(PH_C2R), true, 0, nullptr, nullptr));
-    HIPCHK(launch_plane_c2r(ctx(PH_PLANE)));
     HIPCHK(launch_particles(ctx(PH_PART), integrate, write_forces ? 1 : 0));
     if (integrate) HIPCHK(launch_advance(stream_, d_reps_, m.d_ids, m.nids));
     return 0;
@@ -918,10 +906,9 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
         rec(3);
         if (e0 == hipSuccess) e0 = launch_last_fused(ctx(PH_LAST), 0);
         rec(4);
-        if (e0 == hipSuccess) e0 = launch_mid(ctx(PH_MIDY_G), 1, 1, 0, 1, nullptr);
+        if (e0 == hipSuccess) e0 = launch_mid_grad(ctx(PH_MIDY_G));
         rec(5);
         if (e0 == hipSuccess) e0 = launch_x_c2r(ctx(PH_C2R), true, 0, nullptr, nullptr);
-        if (e0 == hipSuccess) e0 = launch_plane_c2r(ctx(PH_PLANE));
         rec(6);
         if (e0 == hipSuccess) e0 = launch_particles(ctx(PH_PART), true, 0);
         rec(7);

@@ -31,7 +31,7 @@ constexpr int kProfPhases = 8;   // spread, x_r2c, y_fwd, last_fused, y_inv, x_c
 // skip MIDY_G and C2R); PLANEF: x R2C + y FFT of tiled 3D replicas with even Ny
 // (their deposit writes densities only and they skip MIDY_H).
 enum Phase { PH_PART = 0, PH_SPREADA, PH_XA, PH_XP, PH_TPART, PH_SCAN, PH_TILE, PH_MIDY_H, PH_MIDY_G,
-             PH_MIDZ_H, PH_PY, PH_LAST, PH_C2R, PH_PLANE, PH_PLANEF, PH_TCOUNT, PH_KPROPS, PH_COUNT };
+             PH_MIDZ_H, PH_PY, PH_LAST, PH_C2R, PH_PLANEF, PH_TCOUNT, PH_KPROPS, PH_COUNT };
 
 struct PhaseMap {
     std::vector<WorkItem> host;

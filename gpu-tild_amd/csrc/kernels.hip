@@ -165,9 +165,11 @@ __global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_e
 //                                          both orientations, pair_style.cu:81-196)
 //   w_c(k) = G_c(k) rho_c(k)              (smeared charges, double smearing,
 //                                          device_array_utils.cu:199-241)
-// then for each direction d the line -i k_d w(k) (k_d = 0 on the d-Nyquist plane,
-// where the reference's term is anti-Hermitian and drops out of Re IFFT) is inverse
-// transformed along the last axis into Gk[f*dim + d].
+// then the gradient spectra -i k_d w(k) (k_d = 0 on the d-Nyquist plane, where the
+// reference's term is anti-Hermitian and drops out of Re IFFT) start their inverse
+// transform: along the last axis la, w and -i k_la w go to Gk[f*dim] and
+// Gk[f*dim + la]; the factors of the other directions are constant along those lines
+// and are applied by the passes of their own axes.
 __global__ void __launch_bounds__(kFftThreads) k_last_fused(const RepDev* __restrict__ reps,
                                                             const WorkItem* __restrict__ wmap,
                                                             int store_spectrum) {
@@ -182,21 +184,13 @@ __global__ void __launch_bounds__(kFftThreads) k_last_fused(const RepDev* __rest
     const int nb = min(r.nl_last, Q - q0);
     const int nlines = nf * nb;
     const int ldb = nlines | 1;                          // forward / coupled lines
-    const int D = r.dim, DB = r.last_dirs;               // directions per inverse batch
+    const int D = r.dim, DB = r.last_dirs;               // inverse outputs per batch (1 or 2)
     const int ldg = (nlines * DB) | 1;                   // inverse lines of one batch
     // LDS: Wb [N x ldb] coupled fields, then two [N x ldg] buffers that first hold the
     // forward transform's ping-pong pair and then the batched inverse transform's
     float2* Wb = lds;
     float2* E0 = lds + (size_t)N * ldb;
     float2* E1 = E0 + (size_t)N * ldg;
-    // x and y index of each column of the block (3D: column q = y * Nxh + x)
-    int* qx = (int*)(E1 + (size_t)N * ldg);
-    int* qy = qx + nb;
-    for (int b = threadIdx.x; b < nb; b += kFftThreads) {
-        const int q = q0 + b;
-        qx[b] = la == 2 ? q % r.Nxh : q;
-        qy[b] = la == 2 ? q / r.Nxh : 0;
-    }
     const FastDiv dnb(nb), dnf(nf);
     for (int task = threadIdx.x; task < nlines * N; task += kFftThreads) {
         int b, f;
@@ -231,38 +225,85 @@ __global__ void __launch_bounds__(kFftThreads) k_last_fused(const RepDev* __rest
         }
     }
     __syncthreads();
-    // the lines -i k_d w(k) (k_d = 0 on the d-Nyquist plane), DB directions per batch
-    for (int d0 = 0; d0 < D; d0 += DB) {
-        const int nd = min(DB, D - d0), nlg = nlines * nd;
+    // Two inverse lines per field: w itself, and -i k_la w (k = 0 on the Nyquist plane).
+    // The factors -i k_d of the other directions are constant along these lines, so
+    // the inverse transform of w serves all of them; each is applied in the pass of its
+    // own axis (k_mid_grad for y, k_x_c2r for x).  DB outputs per batch.
+    const int slot1 = la;                                // Gk slot of the -i k_la w line
+    for (int o0 = 0; o0 < 2; o0 += DB) {
+        const int no = min(DB, 2 - o0), nlg = nlines * no;
         for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
             int b;
             const int n = dnb.div(task, b);
-            for (int dd = 0; dd < nd; ++dd) {
-                const int d = d0 + dd;
-                int sidx, Nd;
-                if (d == 0) { sidx = qx[b]; Nd = r.Nx; }
-                else if (d == la) { sidx = n; Nd = N; }
-                else { sidx = qy[b]; Nd = r.Ny; }
-                float kd;
-                if (2 * sidx == Nd) kd = 0.f;
-                else kd = r.kfac[d] * (float)(2 * sidx < Nd ? sidx : sidx - Nd);
+            const float kl = 2 * n == N ? 0.f : r.kfac[la] * (float)(2 * n < N ? n : n - N);
+            for (int oo = 0; oo < no; ++oo) {
+                const float kd = o0 + oo == 0 ? 1.f : kl;
                 for (int f = 0; f < nf; ++f) {
                     const float2 w = Wb[n * ldb + f * nb + b];
-                    // conj(-i k w) = (k w.y, k w.x): the inverse is taken as conj(FFT(conj))
-                    E0[n * ldg + (f * nd + dd) * nb + b] = make_float2(kd * w.y, kd * w.x);
+                    // the inverse is taken as conj(FFT(conj)): conj(w) = (w.x, -w.y) and
+                    // conj(-i k w) = (k w.y, k w.x)
+                    E0[n * ldg + (f * no + oo) * nb + b] =
+                        o0 + oo == 0 ? make_float2(w.x, -w.y) : make_float2(kd * w.y, kd * w.x);
                 }
             }
         }
         __syncthreads();
         const float2* res = lds_fft(E0, E1, ldg, nlg, r.ax[la]);
-        const FastDiv dfd(nf * nd), dnd(nd);
+        const FastDiv dfo(nf * no), dno(no);
         for (int task = threadIdx.x; task < nlg * N; task += kFftThreads) {
-            int b, fd, dr;
-            const int rest = dnb.div(task, b), n = dfd.div(rest, fd);
-            const int f = dnd.div(fd, dr), d = d0 + dr;
-            gst(r.Gk + (size_t)(f * D + d) * r.Mh + (size_t)n * Q + q0 + b, conjf2(res[n * ldg + fd * nb + b]));
+            int b, fo, oo;
+            const int rest = dnb.div(task, b), n = dfo.div(rest, fo);
+            const int f = dno.div(fo, oo);
+            const int slot = o0 + oo == 0 ? 0 : slot1;
+            gst(r.Gk + (size_t)(f * D + slot) * r.Mh + (size_t)n * Q + q0 + b, conjf2(res[n * ldg + fo * nb + b]));
         }
         __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ y inverse (3D)
+// y-axis inverse of the gradient spectra.  Per field, slot 0 holds U (the last-axis
+// inverse of w) and slot 2 holds V (that of -i k_z w); the pass writes the y-inverses
+// of U (the x derivative: -i k_x follows in the x pass), of -i k_y U and of V to slots
+// 0, 1 and 2.  One block: nb columns of one (field, z) plane, all three line sets in
+// one batch; it reads its columns of slots 0 and 2 before it writes any slot.
+__global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(8)))
+k_mid_grad(const RepDev* __restrict__ reps, const WorkItem* __restrict__ wmap) {
+    extern __shared__ float2 lds[];
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    const MidGeom g = mid_grad_geom(r);
+    const int o = wi.blk / g.nchunk, c = wi.blk % g.nchunk;   // o = f * Nz + z
+    const int f = o / r.Nz, z = o - f * r.Nz;
+    const int N = r.Ny, Nxh = r.Nxh;
+    const int b0 = c * g.nl;
+    const int nb = min(g.nl, g.batch - b0);
+    const int ldb = (3 * nb) | 1;
+    const size_t plane = ((size_t)z * N) * Nxh + b0;
+    float2* U = r.Gk + (size_t)(f * 3 + 0) * r.Mh + plane;
+    float2* Y = r.Gk + (size_t)(f * 3 + 1) * r.Mh + plane;
+    float2* V = r.Gk + (size_t)(f * 3 + 2) * r.Mh + plane;
+    const FastDiv dnb(nb);
+    for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
+        int b;
+        const int n = dnb.div(task, b);
+        const float ky = 2 * n == N ? 0.f : r.kfac[1] * (float)(2 * n < N ? n : n - N);
+        const float2 u = gld(U + (size_t)n * Nxh + b), v = gld(V + (size_t)n * Nxh + b);
+        // lines of the conj(FFT(conj)) inverse: conj(u), conj(-i ky u), conj(v)
+        lds[n * ldb + b] = make_float2(u.x, -u.y);
+        lds[n * ldb + nb + b] = make_float2(ky * u.y, ky * u.x);
+        lds[n * ldb + 2 * nb + b] = make_float2(v.x, -v.y);
+    }
+    __syncthreads();
+    const float2* res = lds_fft(lds, lds + (size_t)N * ldb, ldb, 3 * nb, r.ax[1]);
+    const FastDiv dnb3(3 * nb);
+    for (int task = threadIdx.x; task < 3 * nb * N; task += kFftThreads) {
+        int sb;
+        const int n = dnb3.div(task, sb);
+        int b;
+        const int sl = dnb.div(sb, b);
+        float2* out = sl == 0 ? U : (sl == 1 ? Y : V);
+        gst(out + (size_t)n * Nxh + b, conjf2(res[n * ldb + sb]));
     }
 }
 
@@ -292,7 +333,7 @@ __global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_e
     const int ldb = nb | 1;
     const float2* Gin = RECORDS ? (const float2*)r.Gk : in_override;
     const bool even = (N & 1) == 0;
-    const FastDiv dN(N), dnc(nc);
+    const FastDiv dN(N), dnc(nc), dD(r.dim);
     for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
         int k, j;
         const int b = dN.div(task, k);
@@ -302,6 +343,16 @@ __global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_e
         const int kk = upper ? N - k : k;
         float2 A = gld(Gin + ((size_t)la * rows_per_field + row) * Nxh + kk);
         float2 Bv = lb < nlines ? gld(Gin + ((size_t)lb * rows_per_field + row) * Nxh + kk) : make_float2(0.f, 0.f);
+        if (RECORDS) {
+            // lines f*dim + 0 hold the x derivative without its factor -i k_x (constant
+            // along the y and z lines it came through): apply it to the half spectrum
+            const float kx = even && 2 * kk == N ? 0.f : r.kfac[0] * (float)kk;
+            int da, db;
+            dD.div(la, da);
+            dD.div(lb, db);
+            if (da == 0) A = make_float2(kx * A.y, -kx * A.x);
+            if (db == 0) Bv = make_float2(kx * Bv.y, -kx * Bv.x);
+        }
         if (kk == 0 || (even && 2 * kk == N)) { A.y = 0.f; Bv.y = 0.f; }
         if (upper) { A.y = -A.y; Bv.y = -Bv.y; }
         // Z = A + i B ; store conj(Z) for the conj(FFT(conj)) inverse
@@ -389,83 +440,6 @@ __global__ void __launch_bounds__(NT) k_plane_r2c(const RepDev* __restrict__ rep
         int k;
         const int y = dNxh.div(task, k);
         gst(H + (size_t)y * Nxh + k, Y[y * ldc + k]);
-    }
-}
-
-// ------------------------------------------------------------------ plane inverse
-// y-axis inverse FFT and x-axis C2R of one (field, z) plane in a single block (3D):
-// the D direction spectra of the plane (Ny x Nxh each) are loaded into LDS, inverse
-// transformed along y, then row chunks are C2R-transformed along x and written as
-// gather records (g_0, g_1, g_2, rho_f).  The spectra make one trip through HBM
-// instead of the separate y pass's read + write + re-read.  Per row the D real lines
-// share complex transforms in pairs (d0, d1), (d2, 0); the Hermitian handling is
-// that of k_x_c2r.  Needs 2 * Ny * (D*Nxh | 1) complex values of LDS.
-template <int NT>
-__global__ void __launch_bounds__(NT) k_plane_c2r(const RepDev* __restrict__ reps,
-                                                  const WorkItem* __restrict__ wmap) {
-    extern __shared__ float2 lds[];
-    const WorkItem wi = wmap[blockIdx.x];
-    const RepDev& r = reps[wi.rep];
-    constexpr int D = 3, NC = 2;                     // real lines per row, complex lines per row
-    const int Nx = r.Nx, Ny = r.Ny, Nxh = r.Nxh;
-    const int z = wi.blk % r.Nz, f = wi.blk / r.Nz;
-    const int L1 = D * Nxh, ld1 = L1 | 1;
-    float2* A = lds;
-    float2* B = lds + (size_t)Ny * ld1;
-    const FastDiv dNx(Nx), dNxh(Nxh);
-    for (int task = threadIdx.x; task < Ny * L1; task += NT) {
-        int kx, d;
-        const int rest = dNxh.div(task, kx), y = FastDiv(D).div(rest, d);
-        const float2 v = gld(r.Gk + (size_t)(f * D + d) * r.Mh + ((size_t)z * Ny + y) * Nxh + kx);
-        A[y * ld1 + d * Nxh + kx] = conjf2(v);       // inverse as conj(FFT(conj))
-    }
-    __syncthreads();
-    const float2* Y = lds_fft<NT>(A, B, ld1, L1, r.ax[1]);   // conj of the y-inverse
-    float2* Fr = (Y == A) ? B : A;                            // free buffer
-    const int cr = r.plane_rows;
-    const int ldb = (cr * NC) | 1;
-    float2* X1 = Fr;
-    float2* X2 = Fr + (size_t)Nx * ldb;
-    const bool even = (Nx & 1) == 0;
-    const bool fp = r.FPtab && f < r.ntypes;
-    for (int y0 = 0; y0 < Ny; y0 += cr) {
-        const int rows = min(cr, Ny - y0);
-        const int nb = rows * NC;
-        for (int task = threadIdx.x; task < nb * Nx; task += NT) {
-            int k;
-            const int b = dNx.div(task, k);
-            const int rr = b / NC, j = b % NC;
-            const int la = 2 * j, lb = la + 1;
-            const bool upper = k >= Nxh;
-            const int kk = upper ? Nx - k : k;
-            const float2* yrow = Y + (size_t)(y0 + rr) * ld1;
-            float2 Av = conjf2(yrow[la * Nxh + kk]);
-            float2 Bv = lb < D ? conjf2(yrow[lb * Nxh + kk]) : make_float2(0.f, 0.f);
-            if (kk == 0 || (even && 2 * kk == Nx)) { Av.y = 0.f; Bv.y = 0.f; }
-            if (upper) { Av.y = -Av.y; Bv.y = -Bv.y; }
-            X1[k * ldb + b] = make_float2(Av.x - Bv.y, -(Av.y + Bv.x));
-        }
-        __syncthreads();
-        const float2* res = lds_fft<NT>(X1, X2, ldb, nb, r.ax[0]);
-        for (int task = threadIdx.x; task < rows * Nx; task += NT) {
-            int x;
-            const int rr = dNx.div(task, x);
-            float g[3];
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const float2 zz = res[x * ldb + rr * NC + d / 2];
-                g[d] = (d & 1) ? -zz.y : zz.x;
-            }
-            const size_t node = ((size_t)z * Ny + y0 + rr) * Nx + x;
-            if (f < r.ntypes) {
-                const float rho = G((const float*)r.rho)[(size_t)f * r.M + node];
-                if (fp)   // field phases: rho f_d per unit density, as in k_x_c2r
-                    for (int d = 0; d < D; ++d) g[d] += rho * G(r.FPtab)[((size_t)f * D + d) * r.M + node];
-                if (!(rho > 0.f)) g[0] = g[1] = g[2] = 0.f;   // masked node, as in k_x_c2r
-            }
-            gst3(r.G3 + ((size_t)f * r.M + node) * 3, g[0], g[1], g[2]);
-        }
-        __syncthreads();
     }
 }
 
@@ -903,10 +877,7 @@ hipError_t launch_x_c2r(const LaunchCtx& c, bool records, int nlines, const floa
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t configure_kernels() {
-    hipError_t e = hipFuncSetAttribute((const void*)k_plane_c2r<kPlaneThreads>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPlaneLds);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute((const void*)k_last_fused, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPlaneLds);
+    hipError_t e = hipFuncSetAttribute((const void*)k_last_fused, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPlaneLds);
     if (e != hipSuccess) return e;
     return hipFuncSetAttribute((const void*)k_plane_r2c<kPlaneThreads>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)kPlaneLds);
@@ -916,9 +887,9 @@ hipError_t launch_plane_r2c(const LaunchCtx& c) {
     hipLaunchKernelGGL(k_plane_r2c<kPlaneThreads>, dim3(c.n), dim3(kPlaneThreads), c.lds, c.stream, c.reps, c.wmap);
     CHECK_LAUNCH(); return hipSuccess;
 }
-hipError_t launch_plane_c2r(const LaunchCtx& c) {
+hipError_t launch_mid_grad(const LaunchCtx& c) {
     if (!c.n) return hipSuccess;
-    hipLaunchKernelGGL(k_plane_c2r<kPlaneThreads>, dim3(c.n), dim3(kPlaneThreads), c.lds, c.stream, c.reps, c.wmap);
+    hipLaunchKernelGGL(k_mid_grad, dim3(c.n), dim3(kFftThreads), c.lds, c.stream, c.reps, c.wmap);
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t launch_particles(const LaunchCtx& c, bool integrate, int write_forces) {

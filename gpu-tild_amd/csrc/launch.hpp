@@ -31,14 +31,26 @@ __host__ __device__ inline MidGeom mid_geom(const RepDev& r, int axis, int nfb) 
     g.nl = (g.batch + g.nchunk - 1) / g.nchunk;     // balanced chunks
     return g;
 }
+// 3D y-inverse of the gradient spectra: y lines of (field, z) planes, three line sets
+// per column in one batch
+__host__ __device__ inline MidGeom mid_grad_geom(const RepDev& r) {
+    MidGeom g;
+    g.N = r.Ny; g.lstride = r.Nxh; g.batch = r.Nxh;
+    g.nouter = r.nf * r.Nz; g.ostride = r.Ny * r.Nxh;
+    int nl = (r.mid_elems > 0 ? r.mid_elems : kFftBudget) / (3 * g.N);
+    if (nl < 1) nl = 1;
+    if (nl > g.batch) nl = g.batch;
+    g.nchunk = (g.batch + nl - 1) / nl;
+    g.nl = (g.batch + g.nchunk - 1) / g.nchunk;     // balanced chunks
+    return g;
+}
 // one FFT buffer of N x nlines complex values, and the ping-pong pair a pass needs
 inline size_t fft_buf_bytes(int N, int nlines) { return (size_t)N * (size_t)(nlines | 1) * sizeof(float2); }
 inline size_t lds_bytes(int N, int nlines) { return 2 * fft_buf_bytes(N, nlines); }
 // fused last-axis pass: coupled fields (nf*nl lines) + two buffers of nf*DB*nl lines
-// (DB directions per inverse batch)
+// (DB inverse outputs per batch)
 inline size_t last_lds_bytes(int N, int nf, int DB, int nl) {
-    // coupled fields, two transform buffers, x and y index of each column
-    return fft_buf_bytes(N, nf * nl) + 2 * fft_buf_bytes(N, nf * DB * nl) + 2 * (size_t)nl * sizeof(int);
+    return fft_buf_bytes(N, nf * nl) + 2 * fft_buf_bytes(N, nf * DB * nl);
 }
 
 struct LaunchCtx {
@@ -56,9 +68,9 @@ hipError_t launch_last_fused(const LaunchCtx& c, int store_spectrum);
 hipError_t launch_x_c2r(const LaunchCtx& c, bool records, int nlines, const float2* in, float* out);
 hipError_t launch_particles(const LaunchCtx& c, bool integrate, int write_forces);
 hipError_t launch_advance(hipStream_t s, RepDev* reps, const int* which, int n);
-constexpr int kPlaneThreads = 512;                      // block of the plane inverse pass
+constexpr int kPlaneThreads = 512;                      // block of the plane forward pass
 constexpr size_t kPlaneLds = 163840;                    // LDS one block may use (MI355X)
-hipError_t launch_plane_c2r(const LaunchCtx& c);
+hipError_t launch_mid_grad(const LaunchCtx& c);
 hipError_t launch_plane_r2c(const LaunchCtx& c);
 // LDS of the forward plane pass: two buffers holding either the Ny/2 row pairs of
 // length Nx or the Ny x Nxh half spectra

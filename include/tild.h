@@ -90,11 +90,11 @@ typedef struct {
     int fft_mid_elems;   /* FFT block geometry, 0 = default: complex values per block of */
     int fft_last_lines;  /* the y pass, columns per block of the fused z pass, rows per  */
     int fft_c2r_rows;    /* block of the inverse x pass.  Results do not depend on them. */
-    int fft_plane;       /* 3D plane passes: 0 default (forward x R2C + y FFT per plane for
-                            tiled replicas with even Ny), 1 also the fused y-inverse + x
-                            C2R per plane (pairs a row's real lines differently: the last
-                            bits move), < 0 neither */
-    int fft_last_dirs;   /* directions per inverse batch of the fused z pass (0 = 1) */
+    int fft_plane;       /* 3D forward plane pass (x R2C + y FFT per plane, for tiled
+                            replicas with even Ny and even tile height): >= 0 used where
+                            it applies (default), < 0 separate passes.  Results do not
+                            depend on it. */
+    int fft_last_dirs;   /* inverse outputs per batch of the fused z pass, 1 or 2 (0 = 1) */
     int reserved[3];
 } tild_opts;
 
