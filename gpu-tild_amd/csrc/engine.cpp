@@ -781,22 +781,26 @@ int Batch::build_tables(Replica& R, const System& S, std::string& err) {
 }
 
 // ---------------------------------------------------------------- step pipeline
-int Batch::enqueue_step(SetMaps& m, bool integrate, bool write_forces, std::string& err) {
+// nsteps consecutive steps (1 when not integrating): the kernels of step k read the
+// device step counter plus k, and one launch at the end advances the counter.
+int Batch::enqueue_steps(SetMaps& m, int nsteps, bool integrate, bool write_forces, std::string& err) {
     PhaseMap* mp = m.maps;
     auto ctx = [&](Phase p) { return LaunchCtx{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_}; };
-    HIPCHK(launch_spread(ctx(PH_SPREADA)));
-    HIPCHK(launch_x_r2c(ctx(PH_XA), true, 0, nullptr, nullptr));
-    HIPCHK(launch_tile_count(ctx(PH_TCOUNT)));
-    HIPCHK(launch_tile_scan(ctx(PH_SCAN)));
-    HIPCHK(launch_tile_scatter(ctx(PH_TPART)));
-    HIPCHK(launch_tile_deposit(ctx(PH_TILE)));
-    HIPCHK(launch_plane_r2c(ctx(PH_PLANEF)));
-    HIPCHK(launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr));
-    HIPCHK(launch_last_fused(ctx(PH_LAST), 0));
-    HIPCHK(launch_mid_grad(ctx(PH_MIDY_G)));
-    HIPCHK(launch_x_c2r(ctx(PH_C2R), true, 0, nullptr, nullptr));
-    HIPCHK(launch_particles(ctx(PH_PART), integrate, write_forces ? 1 : 0));
-    if (integrate) HIPCHK(launch_advance(stream_, d_reps_, m.d_ids, m.nids));
+    for (int k = 0; k < nsteps; ++k) {
+        HIPCHK(launch_spread(ctx(PH_SPREADA), k));
+        HIPCHK(launch_x_r2c(ctx(PH_XA), true, 0, nullptr, nullptr));
+        HIPCHK(launch_tile_count(ctx(PH_TCOUNT), k));
+        HIPCHK(launch_tile_scan(ctx(PH_SCAN)));
+        HIPCHK(launch_tile_scatter(ctx(PH_TPART), k));
+        HIPCHK(launch_tile_deposit(ctx(PH_TILE)));
+        HIPCHK(launch_plane_r2c(ctx(PH_PLANEF)));
+        HIPCHK(launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr));
+        HIPCHK(launch_last_fused(ctx(PH_LAST), 0));
+        HIPCHK(launch_mid_grad(ctx(PH_MIDY_G)));
+        HIPCHK(launch_x_c2r(ctx(PH_C2R), true, 0, nullptr, nullptr));
+        HIPCHK(launch_particles(ctx(PH_PART), integrate, write_forces ? 1 : 0, k));
+    }
+    if (integrate) HIPCHK(launch_advance(stream_, d_reps_, m.d_ids, m.nids, nsteps));
     return 0;
 }
 
@@ -806,7 +810,7 @@ int Batch::forces(int id, std::string& err) {
     HIPCHK(hipSetDevice(device_));
     SetMaps* m = maps_for({id}, err);
     if (!m) return -TILD_ERR_DEVICE;
-    return enqueue_step(*m, false, true, err);
+    return enqueue_steps(*m, 1, false, true, err);
 }
 
 int Batch::step(const std::vector<int>& ids_in, int n, std::string& err, bool use_graph) {
@@ -851,7 +855,7 @@ int Batch::capture_steps(SetMaps& m, int nsteps, hipGraphExec_t* out, std::strin
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     int rc = 0;
-    for (int s = 0; s < nsteps && !rc; ++s) rc = enqueue_step(m, true, false, err);
+    rc = enqueue_steps(m, nsteps, true, false, err);
     hipError_t ec = hipStreamEndCapture(stream_, &g);
     if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
     HIPCHK(ec);
@@ -870,10 +874,10 @@ int Batch::run_steps(SetMaps& m, const std::vector<int>& ids, int n, bool use_gr
     int s = 0;
     if (use_graph)
         for (; s + kGraphSteps <= n; s += kGraphSteps) HIPCHK(hipGraphLaunch(m.graph_run, stream_));
-    for (; s < n; ++s) {
-        if (use_graph) HIPCHK(hipGraphLaunch(m.graph, stream_));
-        else if (int rc = enqueue_step(m, true, false, err)) return rc;
-    }
+    if (use_graph)
+        for (; s < n; ++s) HIPCHK(hipGraphLaunch(m.graph, stream_));
+    else if (n > s)
+        if (int rc = enqueue_steps(m, n - s, true, false, err)) return rc;
     for (int id : ids) reps_[id]->step += n;
     return 0;
 }
@@ -893,10 +897,10 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
         hipError_t e0 = hipSuccess;
         auto rec = [&](int k) { if (e0 == hipSuccess) e0 = hipEventRecord(ev[k], stream_); };
         rec(0);
-        if (e0 == hipSuccess) e0 = launch_spread(ctx(PH_SPREADA));
-        if (e0 == hipSuccess) e0 = launch_tile_count(ctx(PH_TCOUNT));
+        if (e0 == hipSuccess) e0 = launch_spread(ctx(PH_SPREADA), 0);
+        if (e0 == hipSuccess) e0 = launch_tile_count(ctx(PH_TCOUNT), 0);
         if (e0 == hipSuccess) e0 = launch_tile_scan(ctx(PH_SCAN));
-        if (e0 == hipSuccess) e0 = launch_tile_scatter(ctx(PH_TPART));
+        if (e0 == hipSuccess) e0 = launch_tile_scatter(ctx(PH_TPART), 0);
         rec(1);
         if (e0 == hipSuccess) e0 = launch_x_r2c(ctx(PH_XA), true, 0, nullptr, nullptr);
         if (e0 == hipSuccess) e0 = launch_tile_deposit(ctx(PH_TILE));
@@ -910,9 +914,9 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
         rec(5);
         if (e0 == hipSuccess) e0 = launch_x_c2r(ctx(PH_C2R), true, 0, nullptr, nullptr);
         rec(6);
-        if (e0 == hipSuccess) e0 = launch_particles(ctx(PH_PART), true, 0);
+        if (e0 == hipSuccess) e0 = launch_particles(ctx(PH_PART), true, 0, 0);
         rec(7);
-        if (e0 == hipSuccess) e0 = launch_advance(stream_, d_reps_, m->d_ids, m->nids);
+        if (e0 == hipSuccess) e0 = launch_advance(stream_, d_reps_, m->d_ids, m->nids, 1);
         for (int id : ids) {   // amortised spatial reorders are timed with the step counter
             Replica& R = *reps_[id];
             if (e0 == hipSuccess && R.reorder_every > 0 && ++R.since_sort >= R.reorder_every && reorder(id, err, true))

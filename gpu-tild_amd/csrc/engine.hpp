@@ -119,7 +119,7 @@ public:
     SetMaps* maps_for(const std::vector<int>& ids, std::string& err);
 
     // Step pipeline of a replica set with prepared maps (graph-capturable).
-    int enqueue_step(SetMaps& m, bool integrate, bool write_forces, std::string& err);
+    int enqueue_steps(SetMaps& m, int nsteps, bool integrate, bool write_forces, std::string& err);
 
 private:
     int device_;

@@ -23,13 +23,13 @@ namespace tild {
 // result is independent of thread order (unlike float atomicAdd,
 // device_grid_utils.cu:173-331).
 __global__ void __launch_bounds__(256) k_spread(const RepDev* __restrict__ reps,
-                                                const WorkItem* __restrict__ wmap) {
+                                                const WorkItem* __restrict__ wmap, int koff) {
     const WorkItem wi = wmap[blockIdx.x];
     const RepDev& r = reps[wi.rep];
     const int i = wi.blk * 256 + threadIdx.x;
     if (i >= r.ns) return;
     const int D = r.dim, P = r.P;
-    const float* x = r.X[r.step & 1];
+    const float* x = r.X[(r.step + koff) & 1];
     float W[3][5];
     int n0[3];
     n0[2] = 0; W[2][0] = 1.f;
@@ -333,30 +333,32 @@ __global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_e
     const int ldb = nb | 1;
     const float2* Gin = RECORDS ? (const float2*)r.Gk : in_override;
     const bool even = (N & 1) == 0;
-    const FastDiv dN(N), dnc(nc), dD(r.dim);
-    for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
-        int k, j;
-        const int b = dN.div(task, k);
+    const FastDiv dN(N), dNxh(Nxh), dnc(nc);
+    // lines f*dim + 0 hold the x derivative without its factor -i k_x (constant along
+    // the y and z lines it came through): it is applied here, to the half spectrum
+    unsigned long long xlines = 0ull;
+    if (RECORDS)
+        for (int L = 0; L < nlines; L += r.dim) xlines |= 1ull << L;
+    // each half-spectrum value is read once and fills column kk and its Hermitian
+    // partner N - kk
+    for (int task = threadIdx.x; task < nb * Nxh; task += kFftThreads) {
+        int kk, j;
+        const int b = dNxh.div(task, kk);
         const int row = row0 + dnc.div(b, j);
         const int la = 2 * j, lb = la + 1;
-        const bool upper = k >= Nxh;
-        const int kk = upper ? N - k : k;
         float2 A = gld(Gin + ((size_t)la * rows_per_field + row) * Nxh + kk);
         float2 Bv = lb < nlines ? gld(Gin + ((size_t)lb * rows_per_field + row) * Nxh + kk) : make_float2(0.f, 0.f);
+        const bool self = kk == 0 || (even && 2 * kk == N);   // self-conjugate column
         if (RECORDS) {
-            // lines f*dim + 0 hold the x derivative without its factor -i k_x (constant
-            // along the y and z lines it came through): apply it to the half spectrum
-            const float kx = even && 2 * kk == N ? 0.f : r.kfac[0] * (float)kk;
-            int da, db;
-            dD.div(la, da);
-            dD.div(lb, db);
-            if (da == 0) A = make_float2(kx * A.y, -kx * A.x);
-            if (db == 0) Bv = make_float2(kx * Bv.y, -kx * Bv.x);
+            const float kx = self && kk != 0 ? 0.f : r.kfac[0] * (float)kk;
+            if ((xlines >> la) & 1ull) A = make_float2(kx * A.y, -kx * A.x);
+            if ((xlines >> lb) & 1ull) Bv = make_float2(kx * Bv.y, -kx * Bv.x);
         }
-        if (kk == 0 || (even && 2 * kk == N)) { A.y = 0.f; Bv.y = 0.f; }
-        if (upper) { A.y = -A.y; Bv.y = -Bv.y; }
-        // Z = A + i B ; store conj(Z) for the conj(FFT(conj)) inverse
-        lds[k * ldb + b] = make_float2(A.x - Bv.y, -(A.y + Bv.x));
+        if (self) { A.y = 0.f; Bv.y = 0.f; }
+        // Z = A + i B ; store conj(Z) for the conj(FFT(conj)) inverse.  Column N - kk
+        // holds the Hermitian extension conj(A) + i conj(B).
+        lds[kk * ldb + b] = make_float2(A.x - Bv.y, -(A.y + Bv.x));
+        if (!self && N - kk >= Nxh) lds[(N - kk) * ldb + b] = make_float2(A.x + Bv.y, A.y - Bv.x);
     }
     __syncthreads();
     const float2* res = lds_fft(lds, lds + (size_t)N * ldb, ldb, nb, r.ax[0]);
@@ -557,9 +559,9 @@ __device__ __forceinline__ float min_image(float dr, float L, float Lh) {
 }
 
 template <bool INTEGRATE, int D, int NP, bool OFF32>
-__device__ __forceinline__ void particle_bead(const RepDev& r, int i, int write_forces) {
+__device__ __forceinline__ void particle_bead(const RepDev& r, int i, int write_forces, int step) {
     const int P = NP > 0 ? NP - 1 : r.P;
-    const float* xp = r.X[r.step & 1];
+    const float* xp = r.X[step & 1];
     // per-bead loads
     float xi[3];
     load_pos<D>(xp + (size_t)i * D, xi);
@@ -603,7 +605,7 @@ __device__ __forceinline__ void particle_bead(const RepDev& r, int i, int write_
     int kind = 0;
     float xo[3] = {0.f, 0.f, 0.f}, ga = 0.f, gc1 = 0.f, gc2 = 0.f, gnm = 0.f;
     float z[4] = {0.f, 0.f, 0.f, 0.f}, zo[4] = {0.f, 0.f, 0.f, 0.f};
-    const uint32_t stepc = (uint32_t)(r.step + 1);
+    const uint32_t stepc = (uint32_t)(step + 1);
     if constexpr (INTEGRATE) {
         kind = G(r.ikind)[i];
         if (r.any_gjf) {
@@ -652,7 +654,7 @@ __device__ __forceinline__ void particle_bead(const RepDev& r, int i, int write_
             u = u > r.L[d] ? u - r.L[d] : (u < 0.0f ? u + r.L[d] : u);
             v[d] = kind == 0 ? xi[d] : u;
         }
-        store_pos<D>(r.X[(r.step + 1) & 1] + (size_t)i * D, v);
+        store_pos<D>(r.X[(step + 1) & 1] + (size_t)i * D, v);
         if (kind == 2) store_pos<D>(r.XO + (size_t)i * D, dn);
     }
 }
@@ -660,22 +662,23 @@ __device__ __forceinline__ void particle_bead(const RepDev& r, int i, int write_
 template <bool INTEGRATE>
 __global__ void __launch_bounds__(256) k_particles(const RepDev* __restrict__ reps,
                                                    const WorkItem* __restrict__ wmap,
-                                                   int write_forces) {
+                                                   int write_forces, int koff) {
     const WorkItem wi = wmap[blockIdx.x];
     const RepDev& r = reps[wi.rep];
     const int i = wi.blk * 256 + threadIdx.x;
     if (i >= r.ns) return;
+    const int step = r.step + koff;
     // unrolled bodies: the default order P = 1 on grids whose records fit 32-bit
     // offsets, and P = 0; any other case takes the looped body
     if (r.P == 1 && r.g3_off32) {
-        if (r.dim == 3) particle_bead<INTEGRATE, 3, 2, true>(r, i, write_forces);
-        else particle_bead<INTEGRATE, 2, 2, true>(r, i, write_forces);
+        if (r.dim == 3) particle_bead<INTEGRATE, 3, 2, true>(r, i, write_forces, step);
+        else particle_bead<INTEGRATE, 2, 2, true>(r, i, write_forces, step);
     } else if (r.P == 0) {
-        if (r.dim == 3) particle_bead<INTEGRATE, 3, 1, false>(r, i, write_forces);
-        else particle_bead<INTEGRATE, 2, 1, false>(r, i, write_forces);
+        if (r.dim == 3) particle_bead<INTEGRATE, 3, 1, false>(r, i, write_forces, step);
+        else particle_bead<INTEGRATE, 2, 1, false>(r, i, write_forces, step);
     } else {
-        if (r.dim == 3) particle_bead<INTEGRATE, 3, 0, false>(r, i, write_forces);
-        else particle_bead<INTEGRATE, 2, 0, false>(r, i, write_forces);
+        if (r.dim == 3) particle_bead<INTEGRATE, 3, 0, false>(r, i, write_forces, step);
+        else particle_bead<INTEGRATE, 2, 0, false>(r, i, write_forces, step);
     }
 }
 
@@ -804,12 +807,13 @@ __global__ void __launch_bounds__(256) k_reduce_rows(const double* __restrict__ 
     if (threadIdx.x == 0) out[q] = red[0];
 }
 
-// Step counters of the replicas that just integrated (their position parity flips).
-// A separate launch: counting the particle blocks done inside k_particles instead
-// costs each block a device-scope atomic round trip, which measured slower.
-__global__ void k_advance(RepDev* reps, const int* __restrict__ which, int n) {
+// Step counters of the replicas that just integrated nsteps steps.  The kernels of a
+// run of steps read the counter plus their step's offset in the run, so one launch
+// per run advances it.  (Counting the particle blocks done inside k_particles instead
+// costs each block a device-scope atomic round trip, which measured slower.)
+__global__ void k_advance(RepDev* reps, const int* __restrict__ which, int n, int nsteps) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) reps[which[k]].step += 1;
+    if (k < n) reps[which[k]].step += nsteps;
 }
 
 // Virial source fields of one pair style (d_initVirial, device_array_utils.cu:456-494):
@@ -849,9 +853,9 @@ __global__ void k_virial_source(const RepDev* __restrict__ reps, int rep,
 // ------------------------------------------------------------------ host launchers
 #define CHECK_LAUNCH() do { hipError_t e__ = hipGetLastError(); if (e__ != hipSuccess) return e__; } while (0)
 
-hipError_t launch_spread(const LaunchCtx& c) {
+hipError_t launch_spread(const LaunchCtx& c, int koff) {
     if (!c.n) return hipSuccess;
-    hipLaunchKernelGGL(k_spread, dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap);
+    hipLaunchKernelGGL(k_spread, dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap, koff);
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t launch_x_r2c(const LaunchCtx& c, bool from_acc, int nrows_override, const float* in, float2* out) {
@@ -892,10 +896,10 @@ hipError_t launch_mid_grad(const LaunchCtx& c) {
     hipLaunchKernelGGL(k_mid_grad, dim3(c.n), dim3(kFftThreads), c.lds, c.stream, c.reps, c.wmap);
     CHECK_LAUNCH(); return hipSuccess;
 }
-hipError_t launch_particles(const LaunchCtx& c, bool integrate, int write_forces) {
+hipError_t launch_particles(const LaunchCtx& c, bool integrate, int write_forces, int koff) {
     if (!c.n) return hipSuccess;
-    if (integrate) hipLaunchKernelGGL(k_particles<true>, dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap, write_forces);
-    else hipLaunchKernelGGL(k_particles<false>, dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap, write_forces);
+    if (integrate) hipLaunchKernelGGL(k_particles<true>, dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap, write_forces, koff);
+    else hipLaunchKernelGGL(k_particles<false>, dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap, write_forces, koff);
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t launch_kprops(const LaunchCtx& c, int nq_max) {
@@ -913,9 +917,9 @@ hipError_t launch_reduce_rows(hipStream_t s, const double* in, int nrows, int n,
     hipLaunchKernelGGL(k_reduce_rows, dim3(nrows), dim3(256), 0, s, in, n, out);
     CHECK_LAUNCH(); return hipSuccess;
 }
-hipError_t launch_advance(hipStream_t s, RepDev* reps, const int* which, int n) {
+hipError_t launch_advance(hipStream_t s, RepDev* reps, const int* which, int n, int nsteps) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_advance, dim3((n + 255) / 256), dim3(256), 0, s, reps, which, n);
+    hipLaunchKernelGGL(k_advance, dim3((n + 255) / 256), dim3(256), 0, s, reps, which, n, nsteps);
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t launch_bins(hipStream_t s, const RepDev* reps, int rep, int ns, int* out) {

@@ -61,13 +61,13 @@ struct LaunchCtx {
     hipStream_t stream;
 };
 
-hipError_t launch_spread(const LaunchCtx& c);
+hipError_t launch_spread(const LaunchCtx& c, int koff);
 hipError_t launch_x_r2c(const LaunchCtx& c, bool from_acc, int nrows_override, const float* in, float2* out);
 hipError_t launch_mid(const LaunchCtx& c, int axis, int use_gk, int nfb, int inverse, float2* buf);
 hipError_t launch_last_fused(const LaunchCtx& c, int store_spectrum);
 hipError_t launch_x_c2r(const LaunchCtx& c, bool records, int nlines, const float2* in, float* out);
-hipError_t launch_particles(const LaunchCtx& c, bool integrate, int write_forces);
-hipError_t launch_advance(hipStream_t s, RepDev* reps, const int* which, int n);
+hipError_t launch_particles(const LaunchCtx& c, bool integrate, int write_forces, int koff);
+hipError_t launch_advance(hipStream_t s, RepDev* reps, const int* which, int n, int nsteps);
 constexpr int kPlaneThreads = 512;                      // block of the plane forward pass
 constexpr size_t kPlaneLds = 163840;                    // LDS one block may use (MI355X)
 hipError_t launch_mid_grad(const LaunchCtx& c);
@@ -85,9 +85,9 @@ hipError_t launch_bond_props(const LaunchCtx& c);
 hipError_t launch_reduce_rows(hipStream_t s, const double* in, int nrows, int n, double* out);
 constexpr int kCountPerThread = 1;                      // beads per thread of the bucket count
 constexpr int kCountBlock = 256 * kCountPerThread;      // beads per block of the bucket count
-hipError_t launch_tile_count(const LaunchCtx& c);
+hipError_t launch_tile_count(const LaunchCtx& c, int koff);
 hipError_t launch_tile_scan(const LaunchCtx& c);
-hipError_t launch_tile_scatter(const LaunchCtx& c);
+hipError_t launch_tile_scatter(const LaunchCtx& c, int koff);
 hipError_t launch_tile_deposit(const LaunchCtx& c);
 hipError_t launch_bins(hipStream_t s, const RepDev* reps, int rep, int ns, int* out);
 hipError_t launch_virial_source(hipStream_t s, const RepDev* reps, int rep, int M, const float* fr, float* vr);

@@ -45,12 +45,12 @@ __device__ inline int tile_bucket(const RepDev& r, const float* xp, int i) {
 constexpr int kCountWin = 1024;
 
 __global__ void __launch_bounds__(256) k_tile_count(const RepDev* __restrict__ reps,
-                                                    const WorkItem* __restrict__ wmap) {
+                                                    const WorkItem* __restrict__ wmap, int koff) {
     __shared__ int hist[kCountWin];
     __shared__ int wmin[4];
     const WorkItem wi = wmap[blockIdx.x];
     const RepDev& r = reps[wi.rep];
-    const float* x = r.X[r.step & 1];
+    const float* x = r.X[(r.step + koff) & 1];
     const int lane = (int)__lane_id();
     int b[kCountPerThread];
     bool valid[kCountPerThread];
@@ -165,13 +165,13 @@ __global__ void __launch_bounds__(1024) k_tile_scan(const RepDev* __restrict__ r
 }
 
 __global__ void __launch_bounds__(256) k_tile_scatter(const RepDev* __restrict__ reps,
-                                                      const WorkItem* __restrict__ wmap) {
+                                                      const WorkItem* __restrict__ wmap, int koff) {
     const WorkItem wi = wmap[blockIdx.x];
     const RepDev& r = reps[wi.rep];
     const int i = wi.blk * 256 + threadIdx.x;
     if (i >= r.ns) return;
     const int D = r.dim;
-    gmem<const float> x = G((const float*)r.X[r.step & 1]);
+    gmem<const float> x = G((const float*)r.X[(r.step + koff) & 1]);
     const long long kk = G((const long long*)r.tkey)[i];
     const int kb = (int)(unsigned)(kk & 0xffffffffll), ks = (int)(kk >> 32);
     const int pos = G(r.toff)[kb] + ks;
@@ -334,9 +334,9 @@ __global__ void __launch_bounds__(kFftThreads) k_tile_deposit(const RepDev* __re
 
 #define CHECK_LAUNCH() do { hipError_t e__ = hipGetLastError(); if (e__ != hipSuccess) return e__; } while (0)
 
-hipError_t launch_tile_count(const LaunchCtx& c) {
+hipError_t launch_tile_count(const LaunchCtx& c, int koff) {
     if (!c.n) return hipSuccess;
-    hipLaunchKernelGGL(k_tile_count, dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap);
+    hipLaunchKernelGGL(k_tile_count, dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap, koff);
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t launch_tile_scan(const LaunchCtx& c) {
@@ -344,9 +344,9 @@ hipError_t launch_tile_scan(const LaunchCtx& c) {
     hipLaunchKernelGGL(k_tile_scan, dim3(c.n), dim3(1024), 0, c.stream, c.reps, c.wmap);
     CHECK_LAUNCH(); return hipSuccess;
 }
-hipError_t launch_tile_scatter(const LaunchCtx& c) {
+hipError_t launch_tile_scatter(const LaunchCtx& c, int koff) {
     if (!c.n) return hipSuccess;
-    hipLaunchKernelGGL(k_tile_scatter, dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap);
+    hipLaunchKernelGGL(k_tile_scatter, dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap, koff);
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t launch_tile_deposit(const LaunchCtx& c) {
