@@ -30,25 +30,39 @@ HBM_PEAK_GBS = 8000.0
 
 
 def phase_bytes(ns, D, M, Mh, nf, T, gjf, charged, ncoef):
-    """HBM bytes of each phase for one step of one replica.  The particle kernel uses the
-    per-unit figures of the reference path's minimal traffic: per bead it reads x and
-    the type and writes x (8D + 4 B), plus the GJF state read and written (16D B) and
-    the charge (4 B); per grid node it reads each type's gather mask and D field
-    components (4T(1 + D) B) and the D electric-field components (4D B).  The FFT
-    passes count this design's minimal bytes per pass (diagnostic only)."""
-    part_in = ns * (4 * D + 4 + (4 if charged else 0))
+    """HBM bytes of each profiled kernel for one step of one replica.  The particle
+    kernel uses the per-unit figures of the reference path's minimal traffic: per bead
+    it reads x and the type and writes x (8D + 4 B), plus the GJF state read and written
+    (16D B) and the charge (4 B); per grid node it reads each type's gather mask and D
+    field components (4T(1 + D) B) and the D electric-field components (4D B).  The
+    other kernels count this design's minimal bytes (diagnostic only): bucket count x in,
+    (bucket, slot) out; scatter x, type and key in, 16-byte record out; deposit records
+    in, densities out; forward plane densities in, half spectra out; fused pass spectra
+    and coupling table in, 2 nf spectra out; y inverse 2 nf spectra in, nf D out; x C2R
+    nf D spectra and the densities in, 12-byte records out."""
+    q = 4 if charged else 0
     b = {
-        "spread": part_in + nf * M * 4,
-        "x_r2c": nf * M * 4 + nf * Mh * 8,
-        "y_fft": 2 * nf * Mh * 8 if D == 3 else 0,
-        "last_fused": nf * Mh * 8 + ncoef * Mh * 4 + nf * D * Mh * 8,
-        "y_ifft": 2 * nf * D * Mh * 8 if D == 3 else 0,
-        "x_c2r": nf * D * Mh * 8 + T * M * 4 + nf * M * 16,
-        "particles": ns * (8 * D + 4 + (16 * D if gjf else 0) + (4 if charged else 0))
+        "count": ns * (4 * D + 8),
+        "scan": 0,
+        "scatter": ns * (4 * D + 4 + 8 + 16 + q),
+        "deposit": ns * (16 + q) + nf * M * 4,
+        "plane_r2c": nf * M * 4 + nf * Mh * 8,
+        "last_fused": nf * Mh * 8 + ncoef * Mh * 4 + 2 * nf * Mh * 8,
+        "y_ifft": 2 * nf * Mh * 8 + nf * D * Mh * 8 if D == 3 else 0,
+        "x_c2r": nf * D * Mh * 8 + T * M * 4 + nf * M * 12,
+        "particles": ns * (8 * D + 4 + (16 * D if gjf else 0) + q)
                      + M * (4 * T * (1 + D) + (4 * D if charged else 0)),
         "advance": 0,
     }
     return b
+
+
+# the kernel of each profiled phase on the default (tiled, forward-plane) path, as named
+# in the rocprofv3 summaries
+PHASE_KERNEL = {"count": "k_tile_count", "scan": "k_tile_scan", "scatter": "k_tile_scatter",
+                "deposit": "k_tile_deposit", "plane_r2c": "k_plane_r2c<512>", "last_fused": "k_last_fused",
+                "y_ifft": "k_mid_grad", "x_c2r": "k_x_c2r<true>", "particles": "k_particles<true>",
+                "advance": "k_advance"}
 
 
 def step_bytes_minimal(ns, D, M, T, gjf, charged):
@@ -225,11 +239,11 @@ def run(args):
 
         prof = eng.profile(args.profile_steps)
         pb = phase_bytes(ns, D, M, Mh, T, T, True, False, T * (T + 1) // 2)
-        dom = max(prof, key=lambda k: prof[k])
+        dom = max((k for k in prof if pb[k] > 0), key=lambda k: prof[k])   # dominant kernel
         ach = pb[dom] / (prof[dom] * 1e-3) / 1e9
         step_bytes = step_bytes_minimal(ns, D, M, T, True, False)
         step_ms = sum(prof.values())
-        phases = {k: {"ms": round(v, 5), "GBps": round(pb[k] / (v * 1e-3) / 1e9, 1) if v > 0 else None}
+        phases = {k: {"ms": round(v, 5), "GBps": round(pb[k] / (v * 1e-3) / 1e9, 1) if v > 0 and pb[k] else None}
                   for k, v in prof.items()}
         eng.close()
 
@@ -253,8 +267,8 @@ def run(args):
     tfile = REPO / "profiles" / "traffic_melt640k.json"
     if tfile.exists():
         try:
-            traffic = json.loads(tfile.read_text()).get(dom)
-        except (ValueError, OSError):
+            traffic = json.loads(tfile.read_text())["per_kernel_bytes"].get(PHASE_KERNEL[dom])
+        except (ValueError, OSError, KeyError):
             traffic = None
     out = {
         "metric": "bead-steps/s per GPU + HBM roofline %; batched vs plain; p50/p99 sweep latency",
@@ -272,7 +286,7 @@ def run(args):
         "config": {"workload": "lamellar diblock melt, 12800 x N=50 = 640000 beads, box 40x40x80, "
                                "grid 80x80x160, 3 gaussian + 3 erf styles, GJF dt=0.01, plain single replica",
                    "beads": ns, "grid": Nx[:D], "property_evaluations": nprops},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": {"bound": "hbm", "kernel": PHASE_KERNEL[dom], "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes": pb[dom], "launch_ms": prof[dom]},
         "step_roofline": {"bytes": step_bytes, "ms": step_ms, "GBps": step_bytes / (step_ms * 1e-3) / 1e9,

@@ -896,33 +896,37 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
     for (int s = 0; s < n && !rc; ++s) {
         hipError_t e0 = hipSuccess;
         auto rec = [&](int k) { if (e0 == hipSuccess) e0 = hipEventRecord(ev[k], stream_); };
+        // one phase per kernel of the default path; the other path's kernel of the same
+        // role shares its phase
         rec(0);
         if (e0 == hipSuccess) e0 = launch_spread(ctx(PH_SPREADA), 0);
         if (e0 == hipSuccess) e0 = launch_tile_count(ctx(PH_TCOUNT), 0);
-        if (e0 == hipSuccess) e0 = launch_tile_scan(ctx(PH_SCAN));
-        if (e0 == hipSuccess) e0 = launch_tile_scatter(ctx(PH_TPART), 0);
         rec(1);
+        if (e0 == hipSuccess) e0 = launch_tile_scan(ctx(PH_SCAN));
+        rec(2);
+        if (e0 == hipSuccess) e0 = launch_tile_scatter(ctx(PH_TPART), 0);
+        rec(3);
         if (e0 == hipSuccess) e0 = launch_x_r2c(ctx(PH_XA), true, 0, nullptr, nullptr);
         if (e0 == hipSuccess) e0 = launch_tile_deposit(ctx(PH_TILE));
-        rec(2);
+        rec(4);
         if (e0 == hipSuccess) e0 = launch_plane_r2c(ctx(PH_PLANEF));
         if (e0 == hipSuccess) e0 = launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr);
-        rec(3);
-        if (e0 == hipSuccess) e0 = launch_last_fused(ctx(PH_LAST), 0);
-        rec(4);
-        if (e0 == hipSuccess) e0 = launch_mid_grad(ctx(PH_MIDY_G));
         rec(5);
-        if (e0 == hipSuccess) e0 = launch_x_c2r(ctx(PH_C2R), true, 0, nullptr, nullptr);
+        if (e0 == hipSuccess) e0 = launch_last_fused(ctx(PH_LAST), 0);
         rec(6);
-        if (e0 == hipSuccess) e0 = launch_particles(ctx(PH_PART), true, 0, 0);
+        if (e0 == hipSuccess) e0 = launch_mid_grad(ctx(PH_MIDY_G));
         rec(7);
+        if (e0 == hipSuccess) e0 = launch_x_c2r(ctx(PH_C2R), true, 0, nullptr, nullptr);
+        rec(8);
+        if (e0 == hipSuccess) e0 = launch_particles(ctx(PH_PART), true, 0, 0);
+        rec(9);
         if (e0 == hipSuccess) e0 = launch_advance(stream_, d_reps_, m->d_ids, m->nids, 1);
         for (int id : ids) {   // amortised spatial reorders are timed with the step counter
             Replica& R = *reps_[id];
             if (e0 == hipSuccess && R.reorder_every > 0 && ++R.since_sort >= R.reorder_every && reorder(id, err, true))
                 e0 = hipErrorUnknown;
         }
-        rec(8);
+        rec(10);
         if (e0 == hipSuccess) e0 = hipEventSynchronize(ev[kProfPhases]);
         if (e0 != hipSuccess) { err = std::string("profile: ") + hipGetErrorString(e0); rc = -TILD_ERR_DEVICE; break; }
         for (int k = 0; k < kProfPhases; ++k) {

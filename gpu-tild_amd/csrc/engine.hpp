@@ -21,7 +21,7 @@ namespace tild {
 
 constexpr int kXcds = 8;         // accelerator complex dies of an MI355X
 constexpr int kGraphSteps = 10;  // integrating steps per multi-step graph launch
-constexpr int kProfPhases = 8;   // spread, x_r2c, y_fwd, last_fused, y_inv, x_c2r, particles, advance
+constexpr int kProfPhases = 10;  // count, scan, scatter, deposit, forward plane, last_fused, y_inv, x_c2r, particles, advance
 
 // Work-map phases.  *A: replicas on the global-atomic spread; T*: replicas on the
 // LDS-tiled spread; XP: x-axis R2C of stored densities (properties).

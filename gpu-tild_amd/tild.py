@@ -298,11 +298,13 @@ class Engine:
         self._chk(lib().tild_engine_get_bins(self._h, _ip(a), a.size), "get_bins")
         return a
 
-    PHASES = ["spread", "x_r2c", "y_fft", "last_fused", "y_ifft", "x_c2r", "particles", "advance"]
+    PHASES = ["count", "scan", "scatter", "deposit", "plane_r2c", "last_fused", "y_ifft", "x_c2r", "particles",
+              "advance"]
 
     def profile(self, nsteps):
-        a = np.zeros(8, dtype=np.float64)
-        self._chk(lib().tild_engine_profile(self._h, int(nsteps), a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 8),
+        n = len(self.PHASES)
+        a = np.zeros(n, dtype=np.float64)
+        self._chk(lib().tild_engine_profile(self._h, int(nsteps), a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n),
                   "profile")
         return dict(zip(self.PHASES, a.tolist()))
 
@@ -403,9 +405,10 @@ class Batch:
     def profile(self, nsteps, replicas=None):
         ids = list(replicas) if replicas is not None else list(self.meta)
         arr = (ctypes.c_int * max(1, len(ids)))(*ids)
-        a = np.zeros(8, dtype=np.float64)
+        n = len(Engine.PHASES)
+        a = np.zeros(n, dtype=np.float64)
         self._chk(lib().tild_batch_profile(self._h, arr, len(ids), int(nsteps),
-                                           a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 8), "profile")
+                                           a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n), "profile")
         return dict(zip(Engine.PHASES, a.tolist()))
 
 
