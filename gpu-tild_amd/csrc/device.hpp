@@ -216,13 +216,15 @@ __device__ __forceinline__ int wrap_node(int n, int N) {
 }
 
 // n / d and n % d for 0 <= n < 2^22, d >= 1, from inv = 1/d in FP32: the float
-// quotient is within one of the true quotient, and one correction step fixes it.
+// quotient is within one of the true quotient, and one correction step fixes it.  The
+// product q*d stays below 2^24, so the 24-bit multiply (full rate, where the 32-bit one
+// is quarter rate) is exact; the correction is selects, not branches.
 __device__ __forceinline__ int div_small(int n, int d, float inv, int& rem) {
     int q = (int)((float)n * inv);
-    int m = n - q * d;
-    if (m < 0) { --q; m += d; }
-    else if (m >= d) { ++q; m -= d; }
-    rem = m;
+    const int m = n - (int)__umul24((unsigned)q, (unsigned)d);
+    const bool lo = m < 0, hi = m >= d;
+    q += (int)hi - (int)lo;
+    rem = lo ? m + d : (hi ? m - d : m);
     return q;
 }
 

@@ -172,12 +172,13 @@ __device__ __forceinline__ void fft_stage(const float2* __restrict__ src, float2
     const int m = N / R, nbf = m * nb;
     const FastDiv dnb(nb), dvp(p);
     const int sst = m * ldb, dst_st = p * ldb;   // uniform strides: one add per LDS address
+    // element offsets stay below 2^24 (LDS), so the 24-bit multiply is exact
     for (int task = threadIdx.x; task < nbf; task += NT) {
         int b, k;
         const int i = dnb.div(task, b);
         dvp.div(i, k);
-        const float2* sp = src + (i * ldb + b);
-        float2* op = dst + (((i - k) * R + k) * ldb + b);
+        const float2* sp = src + ((int)__umul24((unsigned)i, (unsigned)ldb) + b);
+        float2* op = dst + ((int)__umul24((unsigned)((i - k) * R + k), (unsigned)ldb) + b);
         float2 v[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) v[r] = sp[r * sst];

@@ -160,6 +160,55 @@ __global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_e
 }
 
 // ------------------------------------------------------------------ fused last axis
+// k-space coupling of one block's nodes for TT types (the same sums, in the same order,
+// as the general loop of k_last_fused): w_I = sum_J C_IJ rho_J with the symmetric table
+// rows pidx(lo, hi) = lo*TT - lo*(lo-1)/2 + (hi - lo), and the charge row.
+template <int TT>
+__device__ __forceinline__ void couple_nodes(gmem<const float> cb, int nrows, int nlc, int nb, int N, int ldb, int nf,
+                                             int ctab_charge, const FastDiv& dnb, const float2* A, float2* Wb) {
+    constexpr int NR = TT * (TT + 1) / 2;
+    const bool charged = nf > TT;
+    for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
+        int b;
+        const int n = dnb.div(task, b);
+        const int row = n * ldb + b;
+        gmem<const float> cn = cb + (size_t)n * nrows * nlc + b;
+        float c[NR];
+#pragma unroll
+        for (int k = 0; k < NR; ++k) c[k] = cn[(size_t)k * nlc];
+        const float cq = charged ? cn[(size_t)ctab_charge * nlc] : 0.f;
+        float2 rh[TT];
+#pragma unroll
+        for (int J = 0; J < TT; ++J) rh[J] = A[row + J * nb];
+#pragma unroll
+        for (int I = 0; I < TT; ++I) {
+            float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int J = 0; J < TT; ++J) {
+                const int lo = I < J ? I : J, hi = I < J ? J : I;
+                const float cc = c[lo * TT - lo * (lo - 1) / 2 + (hi - lo)];
+                acc.x += cc * rh[J].x; acc.y += cc * rh[J].y;
+            }
+            Wb[row + I * nb] = acc;
+        }
+        if (charged) {
+            const float2 rq = A[row + TT * nb];
+            Wb[row + TT * nb] = make_float2(cq * rq.x, cq * rq.y);
+        }
+    }
+}
+
+// Last-axis forward FFT of every field, the k-space coupling
+//   w_I(k) = sum_J C_IJ(k) rho_J(k)       (C includes 1/M and counts each style in
+//                                          both orientations, pair_style.cu:81-196)
+//   w_c(k) = G_c(k) rho_c(k)              (smeared charges, double smearing,
+//                                          device_array_utils.cu:199-241)
+// then the gradient spectra -i k_d w(k) (k_d = 0 on the d-Nyquist plane, where the
+// reference's term is anti-Hermitian and drops out of Re IFFT) start their inverse
+// transform: along the last axis la, w and -i k_la w go to Gk[f*dim] and
+// Gk[f*dim + la]; the factors of the other directions are constant along those lines
+// and are applied by the passes of their own axes.
+
 // Last-axis forward FFT of every field, the k-space coupling
 //   w_I(k) = sum_J C_IJ(k) rho_J(k)       (C includes 1/M and counts each style in
 //                                          both orientations, pair_style.cu:81-196)
@@ -203,7 +252,14 @@ __global__ void __launch_bounds__(kFftThreads) k_last_fused(const RepDev* __rest
     // this block's table rows are one contiguous run
     const int nrows = r.ctab_rows, nlc = r.nl_last;
     gmem<const float> cb = G(r.Ctab) + (size_t)wi.blk * N * nrows * nlc;
-    for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
+    switch (T) {   // up to four types: unrolled, every table row of a node requested at once
+        case 1: couple_nodes<1>(cb, nrows, nlc, nb, N, ldb, nf, r.ctab_charge, dnb, A, Wb); break;
+        case 2: couple_nodes<2>(cb, nrows, nlc, nb, N, ldb, nf, r.ctab_charge, dnb, A, Wb); break;
+        case 3: couple_nodes<3>(cb, nrows, nlc, nb, N, ldb, nf, r.ctab_charge, dnb, A, Wb); break;
+        case 4: couple_nodes<4>(cb, nrows, nlc, nb, N, ldb, nf, r.ctab_charge, dnb, A, Wb); break;
+        default: break;
+    }
+    for (int task = threadIdx.x; T > 4 && task < nb * N; task += kFftThreads) {
         int b;
         const int n = dnb.div(task, b);
         const int row = n * ldb + b;
