@@ -371,9 +371,10 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
         h.FPtab = d_fp;
     }
 
-    // spatial reorder: worthwhile once the gather records no longer fit one XCD's L2
+    // spatial reorder: worthwhile once the gather records no longer fit one XCD's L2;
+    // every 40 steps measured best on the 640,000-bead melt (20: -0.8 %, 80: -0.3 %)
     R.reorder_every = reorder_every_ >= 0 ? reorder_every_
-                                          : ((size_t)h.nf * M * sizeof(float4) > ((size_t)8 << 20) ? 20 : 0);
+                                          : ((size_t)h.nf * M * 3 * sizeof(float) > ((size_t)8 << 20) ? 40 : 0);
     if (R.reorder_every > 0) {
         const uint32_t maxkey = (uint32_t)(h.Ny * h.Nz - 1);
         R.sort_bits = 1;
