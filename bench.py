@@ -35,16 +35,14 @@ def phase_bytes(ns, D, M, Mh, nf, T, gjf, charged, ncoef):
     it reads x and the type and writes x (8D + 4 B), plus the GJF state read and written
     (16D B) and the charge (4 B); per grid node it reads each type's gather mask and D
     field components (4T(1 + D) B) and the D electric-field components (4D B).  The
-    other kernels count this design's minimal bytes (diagnostic only): bucket count x in,
-    (bucket, slot) out; scatter x, type and key in, 16-byte record out; deposit records
+    other kernels count this design's minimal bytes (diagnostic only): bucket claims x
+    and type in, 16-byte record out; deposit records
     in, densities out; forward plane densities in, half spectra out; fused pass spectra
     and coupling table in, 2 nf spectra out; y inverse 2 nf spectra in, nf D out; x C2R
     nf D spectra and the densities in, 12-byte records out."""
     q = 4 if charged else 0
     b = {
-        "count": ns * (4 * D + 8),
-        "scan": 0,
-        "scatter": ns * (4 * D + 4 + 8 + 16 + q),
+        "claim": ns * (4 * D + 4 + 16 + q),
         "deposit": ns * (16 + q) + nf * M * 4,
         "plane_r2c": nf * M * 4 + nf * Mh * 8,
         "last_fused": nf * Mh * 8 + ncoef * Mh * 4 + 2 * nf * Mh * 8,
@@ -59,8 +57,7 @@ def phase_bytes(ns, D, M, Mh, nf, T, gjf, charged, ncoef):
 
 # the kernel of each profiled phase on the default (tiled, forward-plane) path, as named
 # in the rocprofv3 summaries
-PHASE_KERNEL = {"count": "k_tile_count", "scan": "k_tile_scan", "scatter": "k_tile_scatter",
-                "deposit": "k_tile_deposit", "plane_r2c": "k_plane_r2c<512>", "last_fused": "k_last_fused",
+PHASE_KERNEL = {"claim": "k_tile_claim", "deposit": "k_tile_deposit", "plane_r2c": "k_plane_r2c<512>", "last_fused": "k_last_fused",
                 "y_ifft": "k_mid_grad", "x_c2r": "k_x_c2r<true>", "particles": "k_particles<true>",
                 "advance": "k_advance"}
 

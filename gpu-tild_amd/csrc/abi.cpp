@@ -146,7 +146,7 @@ tild_status tild_engine_create(const tild_system* s, const tild_opts* opts, tild
     if (opts) o = *opts;
     auto e = std::make_unique<tild_engine>();
     e->batch = std::make_unique<tild::Batch>(o.device);
-    e->batch->set_spread(o.spread_mode, o.tile_target > 0 ? o.tile_target : 512, o.tile_by, o.tile_bz);
+    e->batch->set_spread(o.spread_mode, o.tile_target > 0 ? o.tile_target : 512, o.tile_by, o.tile_bz, o.tile_slack);
     e->batch->set_reorder(o.reorder_every == 0 ? -1 : (o.reorder_every < 0 ? 0 : o.reorder_every));
     e->batch->set_fft_tuning({o.fft_mid_elems, o.fft_last_lines, o.fft_c2r_rows, o.fft_plane, o.fft_last_dirs});
     std::string err;
@@ -219,7 +219,7 @@ tild_status tild_batch_create(const tild_opts* opts, tild_batch** out) {
     if (opts) o = *opts;
     auto b = std::make_unique<tild_batch>();
     b->batch = std::make_unique<tild::Batch>(o.device);
-    b->batch->set_spread(o.spread_mode, o.tile_target > 0 ? o.tile_target : 512, o.tile_by, o.tile_bz);
+    b->batch->set_spread(o.spread_mode, o.tile_target > 0 ? o.tile_target : 512, o.tile_by, o.tile_bz, o.tile_slack);
     b->batch->set_reorder(o.reorder_every == 0 ? -1 : (o.reorder_every < 0 ? 0 : o.reorder_every));
     b->batch->set_fft_tuning({o.fft_mid_elems, o.fft_last_lines, o.fft_c2r_rows, o.fft_plane, o.fft_last_dirs});
     std::string err;

@@ -95,12 +95,22 @@ struct RepDev {
     int spread_mode;           // 1 global atomics, 2 LDS-tiled sorted bins
     int By, Bz, nty, ntz, ntiles, nbuckets;
     float iBy, iBz;            // 1 / By, 1 / Bz (FP32) for the bucket divisions
-    int* tcnt;                 // nbuckets counters (zero between steps), bucket b at b << tshift
     int tshift;                // log2 of the counter stride: one 128-byte line per bucket
+    // exact binning (spatial reorder, capacity sizing): counts, offsets, (bucket, slot)
+    int* tcnt;                 // nbuckets counters (zero between uses), bucket b at b << tshift
     int* toff;                 // nbuckets + 1 exclusive offsets
     int2* tkey;                // ns (bucket, slot)
-    float4* trec;              // ns sorted (x, y, z, type) records
-    float* trecq;              // ns sorted charges (charged decks)
+    // per-step binning into fixed bucket regions (sized at creation and each reorder)
+    int* tclaim;               // 2 x (nbuckets << tshift) slot counters, by step parity
+    int* tbase;                // nbuckets + 1: first record of each bucket's region
+    int* tcap;                 // nbuckets: region sizes
+    int tslack;                // spare slots per region (0: count/4 + 16, < 0: none)
+    float4* trec;              // records (x, y, z, type) in bucket regions
+    float* trecq;              // their charges (charged decks)
+    int* tovf;                 // overflow counts by step parity, at [0] and [32]
+    float4* tovf_rec;          // ns records that found their region full
+    int* tovf_b;               // their buckets
+    float* tovf_q;             // their charges
     // counters
     int step;                  // completed steps; X[step & 1] is current
     uint32_t seed;

@@ -160,7 +160,7 @@ size_t Batch::estimate_bytes(const System& S) {
     size_t T = S.ntypes, nst = S.styles().size();
     // X[2], XO, fout; type, charge, bond tables, integrator kind; sorted-bin keys and records
     // site ids; spare arrays, keys and sort scratch of the spatial reorder
-    size_t part = ns * (4 * D * 4 + 4 + 4 + 24 + 1) + ns * (8 + 16 + 4) + ns * 4 + ns * (4 * D + 100);
+    size_t part = ns * (4 * D * 4 + 4 + 4 + 24 + 1) + ns * (8 + 20 + 28) + ns * 4 + ns * (4 * D + 100);
     size_t grids = nf * M * (8 + 4 + 12) + nf * Mh * 8 * (1 + D);
     size_t tabs = Mh * 4 * (T * (T + 1) / 2 + 1 + nst + 1) + Mh * 8 * (D == 3 ? 6 : 3) * nst;
     if (!S.Fields.empty()) tabs += T * D * M * 4;
@@ -337,6 +337,7 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
             h.spread_mode = 2;
             h.By = By; h.Bz = Bz;
             h.iBy = 1.0f / (float)By; h.iBz = 1.0f / (float)Bz;
+            h.tslack = tile_slack_;
             h.nty = h.Ny / By; h.ntz = h.Nz / Bz;
             h.ntiles = h.nty * h.ntz;
             h.nbuckets = h.ntiles * 4;
@@ -346,8 +347,18 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
             if (int rc = dalloc(R, &h.tcnt, (size_t)h.nbuckets << h.tshift, err, true)) return rc;
             if (int rc = dalloc(R, &h.toff, (size_t)h.nbuckets + 1, err, true)) return rc;
             if (int rc = dalloc(R, &h.tkey, S.ns, err)) return rc;
-            if (int rc = dalloc(R, &h.trec, S.ns, err)) return rc;
-            if (S.charges.enabled) { if (int rc = dalloc(R, &h.trecq, S.ns, err)) return rc; }
+            if (int rc = dalloc(R, &h.tclaim, (size_t)2 * h.nbuckets << h.tshift, err, true)) return rc;
+            if (int rc = dalloc(R, &h.tbase, (size_t)h.nbuckets + 1, err, true)) return rc;
+            if (int rc = dalloc(R, &h.tcap, (size_t)h.nbuckets, err, true)) return rc;
+            // regions hold a bucket's count plus the slack (k_tile_regions), at most this many
+            const size_t nrec = (size_t)S.ns + (tile_slack_ > 0 ? (size_t)tile_slack_ * h.nbuckets
+                                                                : (tile_slack_ == 0 ? S.ns / 4 + 16 * (size_t)h.nbuckets : 0));
+            if (int rc = dalloc(R, &h.trec, nrec, err)) return rc;
+            if (S.charges.enabled) { if (int rc = dalloc(R, &h.trecq, nrec, err)) return rc; }
+            if (int rc = dalloc(R, &h.tovf, 64, err, true)) return rc;
+            if (int rc = dalloc(R, &h.tovf_rec, S.ns, err)) return rc;
+            if (int rc = dalloc(R, &h.tovf_b, S.ns, err)) return rc;
+            if (S.charges.enabled) { if (int rc = dalloc(R, &h.tovf_q, S.ns, err)) return rc; }
         } else if (spread_mode_ == 2) {
             err = "no tile shape fits this grid for the tiled spread";
             return -TILD_ERR_UNSUPPORTED;
@@ -416,16 +427,33 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
     if (RR.reorder_every > 0) {
         if (int rc = reorder(id, err)) { remove(id); return rc; }
     }
+    if (int rc = tile_regions(id, false, err)) { remove(id); return rc; }
     HIPCHK(hipStreamSynchronize(stream_));
     return id;
 }
 
+// Tiled replicas: exact binning of the current positions (tkey, toff), then, with
+// `regions`, each bucket's region in the per-step record array sized from it.
+int Batch::tile_regions(int id, bool bin_only, std::string& err) {
+    Replica& R = *reps_[id];
+    if (R.h.spread_mode != 2) return 0;
+    PhaseMap* mp = R.maps;
+    auto ctx = [&](Phase p) { return LaunchCtx{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_}; };
+    HIPCHK(launch_tile_count(ctx(PH_TCOUNT), 0));
+    HIPCHK(launch_tile_scan(ctx(PH_SCAN)));
+    if (!bin_only) HIPCHK(launch_tile_regions(stream_, d_reps_, id));
+    return 0;
+}
+
 int Batch::reorder(int id, std::string& err, bool after_step) {
     Replica& R = *reps_[id];
-    if (after_step && R.h.spread_mode == 2)   // the step's tile binning is current
+    if (after_step && R.h.spread_mode == 2) {   // bin, sort each tile by site id, permute
+        if (int rc = tile_regions(id, true, err)) return rc;
         HIPCHK(launch_reorder_tiles(stream_, d_reps_, id, R.ns, R.h.ntiles, R.rb));
-    else
+        HIPCHK(launch_tile_regions(stream_, d_reps_, id));   // bins do not move with the slots
+    } else {
         HIPCHK(launch_reorder(stream_, d_reps_, id, R.ns, R.sort_bits, R.rb));
+    }
     // mirror the device-side swaps in the host copy of the descriptor
     RepDev& h = R.h;
     std::swap(h.X[0], h.X[1]);
@@ -503,7 +531,6 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
         const int nbx = (npairs + h.nl_x - 1) / h.nl_x;
         add(PH_XP, nbx, lds_bytes(h.Nx, h.nl_x));
         if (h.spread_mode == 2) {
-            add(PH_TPART, R.nblk_part, 0);
             add(PH_TCOUNT, (h.ns + kCountBlock - 1) / kCountBlock, 0);
             add(PH_SCAN, 1, 0);
             add(PH_TILE, h.ntiles, tile_lds_bytes(h.Nx, h.nf, h.By, h.Bz));
@@ -534,7 +561,7 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
     // one slab of a spatially ordered replica, so that its L2 holds that slab's grid
     // records; for the FFT passes adjacent column batches, whose rows share cache lines
     // that a single L2 then fetches once.
-    for (Phase ph : {PH_PART, PH_TPART, PH_TCOUNT, PH_SPREADA, PH_TILE, PH_XA, PH_XP, PH_MIDY_H, PH_MIDY_G,
+    for (Phase ph : {PH_PART, PH_TCOUNT, PH_SPREADA, PH_TILE, PH_XA, PH_XP, PH_MIDY_H, PH_MIDY_G,
                      PH_MIDZ_H, PH_PY, PH_LAST, PH_C2R, PH_PLANEF}) {
         std::vector<WorkItem>& v = maps[ph].host;
         const size_t n = v.size(), q = n / kXcds, rm = n % kXcds;
@@ -790,10 +817,8 @@ int Batch::enqueue_steps(SetMaps& m, int nsteps, bool integrate, bool write_forc
     for (int k = 0; k < nsteps; ++k) {
         HIPCHK(launch_spread(ctx(PH_SPREADA), k));
         HIPCHK(launch_x_r2c(ctx(PH_XA), true, 0, nullptr, nullptr));
-        HIPCHK(launch_tile_count(ctx(PH_TCOUNT), k));
-        HIPCHK(launch_tile_scan(ctx(PH_SCAN)));
-        HIPCHK(launch_tile_scatter(ctx(PH_TPART), k));
-        HIPCHK(launch_tile_deposit(ctx(PH_TILE)));
+        HIPCHK(launch_tile_claim(ctx(PH_TCOUNT), k));
+        HIPCHK(launch_tile_deposit(ctx(PH_TILE), k));
         HIPCHK(launch_plane_r2c(ctx(PH_PLANEF)));
         HIPCHK(launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr));
         HIPCHK(launch_last_fused(ctx(PH_LAST), 0));
@@ -802,6 +827,7 @@ int Batch::enqueue_steps(SetMaps& m, int nsteps, bool integrate, bool write_forc
         HIPCHK(launch_particles(ctx(PH_PART), integrate, write_forces ? 1 : 0, k));
     }
     if (integrate) HIPCHK(launch_advance(stream_, d_reps_, m.d_ids, m.nids, nsteps));
+    else HIPCHK(launch_tile_reset(ctx(PH_SCAN), 0));   // the next step claims from the same parity
     return 0;
 }
 
@@ -901,33 +927,29 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
         // role shares its phase
         rec(0);
         if (e0 == hipSuccess) e0 = launch_spread(ctx(PH_SPREADA), 0);
-        if (e0 == hipSuccess) e0 = launch_tile_count(ctx(PH_TCOUNT), 0);
+        if (e0 == hipSuccess) e0 = launch_tile_claim(ctx(PH_TCOUNT), 0);
         rec(1);
-        if (e0 == hipSuccess) e0 = launch_tile_scan(ctx(PH_SCAN));
-        rec(2);
-        if (e0 == hipSuccess) e0 = launch_tile_scatter(ctx(PH_TPART), 0);
-        rec(3);
         if (e0 == hipSuccess) e0 = launch_x_r2c(ctx(PH_XA), true, 0, nullptr, nullptr);
-        if (e0 == hipSuccess) e0 = launch_tile_deposit(ctx(PH_TILE));
-        rec(4);
+        if (e0 == hipSuccess) e0 = launch_tile_deposit(ctx(PH_TILE), 0);
+        rec(2);
         if (e0 == hipSuccess) e0 = launch_plane_r2c(ctx(PH_PLANEF));
         if (e0 == hipSuccess) e0 = launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr);
-        rec(5);
+        rec(3);
         if (e0 == hipSuccess) e0 = launch_last_fused(ctx(PH_LAST), 0);
-        rec(6);
+        rec(4);
         if (e0 == hipSuccess) e0 = launch_mid_grad(ctx(PH_MIDY_G));
-        rec(7);
+        rec(5);
         if (e0 == hipSuccess) e0 = launch_x_c2r(ctx(PH_C2R), true, 0, nullptr, nullptr);
-        rec(8);
+        rec(6);
         if (e0 == hipSuccess) e0 = launch_particles(ctx(PH_PART), true, 0, 0);
-        rec(9);
+        rec(7);
         if (e0 == hipSuccess) e0 = launch_advance(stream_, d_reps_, m->d_ids, m->nids, 1);
         for (int id : ids) {   // amortised spatial reorders are timed with the step counter
             Replica& R = *reps_[id];
             if (e0 == hipSuccess && R.reorder_every > 0 && ++R.since_sort >= R.reorder_every && reorder(id, err, true))
                 e0 = hipErrorUnknown;
         }
-        rec(10);
+        rec(8);
         if (e0 == hipSuccess) e0 = hipEventSynchronize(ev[kProfPhases]);
         if (e0 != hipSuccess) { err = std::string("profile: ") + hipGetErrorString(e0); rc = -TILD_ERR_DEVICE; break; }
         for (int k = 0; k < kProfPhases; ++k) {
@@ -1030,6 +1052,7 @@ int Batch::set_positions(int id, const float* x, size_t n, std::string& err) {
     }
     HIPCHK(hipMemcpyAsync(R->h.X[R->step & 1], src, n * 4, hipMemcpyHostToDevice, stream_));
     HIPCHK(hipMemsetAsync(R->h.XO, 0, n * 4, stream_));   // xo = x
+    if (int rc = tile_regions(id, false, err)) return rc;   // new positions, new bucket sizes
     HIPCHK(hipStreamSynchronize(stream_));
     return 0;
 }

@@ -21,7 +21,7 @@ namespace tild {
 
 constexpr int kXcds = 8;         // accelerator complex dies of an MI355X
 constexpr int kGraphSteps = 10;  // integrating steps per multi-step graph launch
-constexpr int kProfPhases = 10;  // count, scan, scatter, deposit, forward plane, last_fused, y_inv, x_c2r, particles, advance
+constexpr int kProfPhases = 8;   // claim, deposit, forward plane, last_fused, y_inv, x_c2r, particles, advance
 
 // Work-map phases.  *A: replicas on the global-atomic spread; T*: replicas on the
 // LDS-tiled spread; XP: x-axis R2C of stored densities (properties).
@@ -30,7 +30,7 @@ constexpr int kProfPhases = 10;  // count, scan, scatter, deposit, forward plane
 // PLANE: y-inverse + x C2R of 3D replicas whose planes fit one block's LDS (they
 // skip MIDY_G and C2R); PLANEF: x R2C + y FFT of tiled 3D replicas with even Ny
 // (their deposit writes densities only and they skip MIDY_H).
-enum Phase { PH_PART = 0, PH_SPREADA, PH_XA, PH_XP, PH_TPART, PH_SCAN, PH_TILE, PH_MIDY_H, PH_MIDY_G,
+enum Phase { PH_PART = 0, PH_SPREADA, PH_XA, PH_XP, PH_SCAN, PH_TILE, PH_MIDY_H, PH_MIDY_G,
              PH_MIDZ_H, PH_PY, PH_LAST, PH_C2R, PH_PLANEF, PH_TCOUNT, PH_KPROPS, PH_COUNT };
 
 struct PhaseMap {
@@ -66,8 +66,8 @@ public:
     explicit Batch(int device) : device_(device) {}
     // spread mode for replicas added from now on: 0 auto, 1 global atomics, 2 tiled;
     // target block count and forced tile shape (0 = free) of the tiled mode.
-    void set_spread(int mode, int target_tiles = 512, int by = 0, int bz = 0) {
-        spread_mode_ = mode; tile_target_ = target_tiles; tile_by_ = by; tile_bz_ = bz;
+    void set_spread(int mode, int target_tiles = 512, int by = 0, int bz = 0, int slack = 0) {
+        spread_mode_ = mode; tile_target_ = target_tiles; tile_by_ = by; tile_bz_ = bz; tile_slack_ = slack;
     }
     // spatial reorder interval of replicas added from now on: > 0 every n steps,
     // 0 never, < 0 automatic (every 20 steps when the gather records outgrow L2)
@@ -123,7 +123,7 @@ public:
 
 private:
     int device_;
-    int spread_mode_ = 0, tile_target_ = 512, tile_by_ = 0, tile_bz_ = 0, reorder_every_ = -1;
+    int spread_mode_ = 0, tile_target_ = 512, tile_by_ = 0, tile_bz_ = 0, tile_slack_ = 0, reorder_every_ = -1;
     FftTuning tune_;
     hipStream_t stream_ = nullptr;
     std::vector<std::unique_ptr<Replica>> reps_;
@@ -140,6 +140,7 @@ private:
     int reorder(int id, std::string& err, bool after_step = false);
     int run_steps(SetMaps& m, const std::vector<int>& ids, int n, bool use_graph, std::string& err);
     int capture_steps(SetMaps& m, int nsteps, hipGraphExec_t* out, std::string& err);
+    int tile_regions(int id, bool bin_only, std::string& err);
     static void drop_graphs(SetMaps& m);
     int fetch_pid(const Replica& R, std::vector<int>& pid, std::string& err);
     void free_maps(PhaseMap* maps);

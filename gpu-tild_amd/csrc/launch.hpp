@@ -83,12 +83,13 @@ hipError_t configure_kernels();
 hipError_t launch_kprops(const LaunchCtx& c, int nq_max);
 hipError_t launch_bond_props(const LaunchCtx& c);
 hipError_t launch_reduce_rows(hipStream_t s, const double* in, int nrows, int n, double* out);
-constexpr int kCountPerThread = 1;                      // beads per thread of the bucket count
-constexpr int kCountBlock = 256 * kCountPerThread;      // beads per block of the bucket count
+constexpr int kCountBlock = 256;                        // beads per block of the bucket claims
 hipError_t launch_tile_count(const LaunchCtx& c, int koff);
 hipError_t launch_tile_scan(const LaunchCtx& c);
-hipError_t launch_tile_scatter(const LaunchCtx& c, int koff);
-hipError_t launch_tile_deposit(const LaunchCtx& c);
+hipError_t launch_tile_claim(const LaunchCtx& c, int koff);
+hipError_t launch_tile_reset(const LaunchCtx& c, int koff);
+hipError_t launch_tile_regions(hipStream_t s, const RepDev* reps, int rep);
+hipError_t launch_tile_deposit(const LaunchCtx& c, int koff);
 hipError_t launch_bins(hipStream_t s, const RepDev* reps, int rep, int ns, int* out);
 hipError_t launch_virial_source(hipStream_t s, const RepDev* reps, int rep, int M, const float* fr, float* vr);
 
@@ -119,8 +120,8 @@ hipError_t launch_nlist(hipStream_t s, const RepDev* reps, int rep, int ns, cons
 // Sort the replica's beads by grid row (stable), permute every per-bead array and
 // swap the live and spare buffers in the device descriptor.
 hipError_t launch_reorder(hipStream_t s, RepDev* reps, int rep, int ns, int bits, ReorderBufs& b);
-// Tiled replicas after a step: each tile's beads (from the step's bucket order) sorted
-// by site id, then the same permutation.
+// Tiled replicas: each tile's beads (from an exact binning, tkey and toff) sorted by
+// site id, then the same permutation.
 hipError_t launch_reorder_tiles(hipStream_t s, RepDev* reps, int rep, int ns, int ntiles, ReorderBufs& b);
 
 }  // namespace tild

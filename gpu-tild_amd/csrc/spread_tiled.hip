@@ -2,14 +2,17 @@
 // the x-axis R2C transform (replaces the global atomics of
 // device_grid_utils.cu:232-343 with per-tile LDS accumulation).
 //
-// The grid is cut into full-x slabs of By x Bz rows.  Every step:
-//   k_tile_count    bucket = (tile of the particle's first stencil row, halo class)
-//   k_tile_scan     exclusive offsets of the buckets (one block per replica)
-//   k_tile_scatter  particles copied into bucket order as (x, y, z, type) records
+// The grid is cut into full-x slabs of By x Bz rows.  A bead's bucket is (tile of its
+// first stencil row, halo class).  Every step:
+//   k_tile_claim    each bead claims a slot in its bucket and writes its (x, y, z,
+//                   type) record into the bucket's region (an overflow list takes
+//                   the beads of full regions)
 //   k_tile_deposit  one block per tile: deposits its own particles and the halo
 //                   particles of its three lower neighbours into 64-bit
 //                   fixed-point LDS accumulators, converts its rows to densities
 //                   and transforms them along x (R2C) straight from LDS.
+// The regions are sized from an exact binning (k_tile_count, k_tile_scan,
+// k_tile_regions) at creation and at every spatial reorder.
 // Integer accumulation makes the densities independent of bucket order, tile
 // shape and thread scheduling: they are bit-identical to the global-atomic path.
 //
@@ -38,66 +41,103 @@ __device__ inline int tile_bucket(const RepDev& r, const float* xp, int i) {
 
 // Slot claims.  A block counts its beads in LDS over a window of kCountWin buckets
 // that starts at the block's smallest bucket -- in a spatially ordered replica that
-// is every bead of the block -- and claims global slots with one atomic per distinct
-// bucket.  Beads beyond the window claim per wave: lanes that share a bucket use one
-// atomic, issued by the lowest of them.  Slot order inside a bucket is arbitrary;
-// the deposit does not depend on it.
+// is every bead of the block -- and claims slots from the counters `cnt` (bucket b at
+// b << tshift) with one atomic per distinct bucket.  Beads beyond the window claim per
+// wave: lanes that share a bucket use one atomic, issued by the lowest of them.  Slot
+// order inside a bucket is arbitrary; the deposit does not depend on it.  One bead per
+// thread (i = blk * 256 + threadIdx.x); every thread of the block must call it.
 constexpr int kCountWin = 1024;
 
-__global__ void __launch_bounds__(256) k_tile_count(const RepDev* __restrict__ reps,
-                                                    const WorkItem* __restrict__ wmap, int koff) {
+__device__ __forceinline__ void claim_slot(const RepDev& r, const float* x, int i, int* cnt, int& b, int& slot) {
     __shared__ int hist[kCountWin];
     __shared__ int wmin[4];
-    const WorkItem wi = wmap[blockIdx.x];
-    const RepDev& r = reps[wi.rep];
-    const float* x = r.X[(r.step + koff) & 1];
     const int lane = (int)__lane_id();
-    int b[kCountPerThread];
-    bool valid[kCountPerThread];
-    int m = 0x7fffffff;
-#pragma unroll
-    for (int k = 0; k < kCountPerThread; ++k) {
-        const int i = wi.blk * kCountBlock + k * 256 + threadIdx.x;
-        valid[k] = i < r.ns;
-        b[k] = valid[k] ? tile_bucket(r, x, i) : 0x7fffffff;
-        m = min(m, b[k]);
-    }
+    const bool valid = i < r.ns;
+    b = valid ? tile_bucket(r, x, i) : 0x7fffffff;
+    int m = b;
     for (int k = threadIdx.x; k < kCountWin; k += 256) hist[k] = 0;
     for (int o = 32; o > 0; o >>= 1) m = min(m, __shfl_xor(m, o));
     if (lane == 0) wmin[threadIdx.x >> 6] = m;
     __syncthreads();
     const int bmin = min(min(wmin[0], wmin[1]), min(wmin[2], wmin[3]));
-    bool inwin[kCountPerThread];
-    int local[kCountPerThread];
-#pragma unroll
-    for (int k = 0; k < kCountPerThread; ++k) {
-        inwin[k] = valid[k] && b[k] - bmin < kCountWin;
-        local[k] = inwin[k] ? atomicAdd(&hist[b[k] - bmin], 1) : 0;
-    }
+    const bool inwin = valid && b - bmin < kCountWin;
+    const int local = inwin ? atomicAdd(&hist[b - bmin], 1) : 0;
     __syncthreads();
     for (int k = threadIdx.x; k < kCountWin; k += 256) {
         const int c = hist[k];
-        if (c) hist[k] = atomicAdd(r.tcnt + ((bmin + k) << r.tshift), c);   // first global slot of the bucket
+        if (c) hist[k] = atomicAdd(cnt + ((bmin + k) << r.tshift), c);   // first slot of the bucket
     }
     __syncthreads();
     const unsigned long long below = (1ull << lane) - 1ull;
-#pragma unroll
-    for (int k = 0; k < kCountPerThread; ++k) {
-        int slot = inwin[k] ? hist[b[k] - bmin] + local[k] : 0;
-        unsigned long long todo = __ballot(valid[k] && !inwin[k]);
-        while (todo) {   // wave-uniform: one round per distinct bucket beyond the window
-            const int leader = __ffsll((long long)todo) - 1;
-            const int bl = __shfl(b[k], leader);
-            const unsigned long long same = __ballot(b[k] == bl) & todo;
-            int base = 0;
-            if (lane == leader) base = atomicAdd(r.tcnt + (bl << r.tshift), __popcll(same));
-            base = __shfl(base, leader);
-            if (b[k] == bl && !inwin[k]) slot = base + __popcll(same & below);
-            todo &= ~same;
-        }
-        const int i = wi.blk * kCountBlock + k * 256 + threadIdx.x;
-        if (valid[k]) G((long long*)r.tkey)[i] = (long long)(((unsigned long long)(unsigned)slot << 32) | (unsigned)b[k]);
+    slot = inwin ? hist[b - bmin] + local : 0;
+    unsigned long long todo = __ballot(valid && !inwin);
+    while (todo) {   // wave-uniform: one round per distinct bucket beyond the window
+        const int leader = __ffsll((long long)todo) - 1;
+        const int bl = __shfl(b, leader);
+        const unsigned long long same = __ballot(b == bl) & todo;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(cnt + (bl << r.tshift), __popcll(same));
+        base = __shfl(base, leader);
+        if (b == bl && !inwin) slot = base + __popcll(same & below);
+        todo &= ~same;
     }
+}
+
+// Exact binning (spatial reorder and region sizing): (bucket, slot) of every bead.
+__global__ void __launch_bounds__(256) k_tile_count(const RepDev* __restrict__ reps,
+                                                    const WorkItem* __restrict__ wmap, int koff) {
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    const float* x = r.X[(r.step + koff) & 1];
+    const int i = wi.blk * 256 + threadIdx.x;
+    int b, slot;
+    claim_slot(r, x, i, r.tcnt, b, slot);
+    if (i < r.ns) G((long long*)r.tkey)[i] = (long long)(((unsigned long long)(unsigned)slot << 32) | (unsigned)b);
+}
+
+// Slot counters of one step parity.
+__device__ __forceinline__ int* claim_counters(const RepDev& r, int parity) {
+    return r.tclaim + (((size_t)parity * r.nbuckets) << r.tshift);
+}
+
+// Per-step binning: each bead claims a slot in its bucket and writes its record
+// (x, y, z, type) straight into the bucket's region -- no scan, no second pass.  A
+// bead whose region is full goes to the overflow list, which the deposit reads too,
+// so the densities stay exact whatever the occupancy.
+__global__ void __launch_bounds__(256) k_tile_claim(const RepDev* __restrict__ reps,
+                                                    const WorkItem* __restrict__ wmap, int koff) {
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    const int step = r.step + koff;
+    const float* xp = r.X[step & 1];
+    const int i = wi.blk * 256 + threadIdx.x;
+    int b, slot;
+    claim_slot(r, xp, i, claim_counters(r, step & 1), b, slot);
+    if (i >= r.ns) return;
+    const int D = r.dim;
+    gmem<const float> x = G(xp);
+    const float4 rec = make_float4(x[i * D], x[i * D + 1], D == 3 ? x[i * D + 2] : 0.f, __int_as_float(G(r.tp)[i]));
+    if (slot < G((const int*)r.tcap)[b]) {
+        const int pos = G((const int*)r.tbase)[b] + slot;
+        gst(r.trec + pos, rec);
+        if (r.trecq) G(r.trecq)[pos] = G(r.q)[i];
+    } else {
+        const int o = atomicAdd(r.tovf + (step & 1) * 32, 1);
+        gst(r.tovf_rec + o, rec);
+        G(r.tovf_b)[o] = b;
+        if (r.trecq) G(r.tovf_q)[o] = G(r.q)[i];
+    }
+}
+
+// Clears the slot counters and overflow count of one step parity (after a force
+// evaluation that does not advance the step: the next step claims from them).
+__global__ void k_tile_reset(const RepDev* __restrict__ reps, const WorkItem* __restrict__ wmap, int koff) {
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    const int p = (r.step + koff) & 1;
+    int* cnt = claim_counters(r, p);
+    for (int b = threadIdx.x; b < r.nbuckets; b += blockDim.x) G(cnt)[b << r.tshift] = 0;
+    if (threadIdx.x == 0) G(r.tovf)[p * 32] = 0;
 }
 
 // Exclusive scan of the bucket counts; clears the counts for the next step.  Each
@@ -164,19 +204,41 @@ __global__ void __launch_bounds__(1024) k_tile_scan(const RepDev* __restrict__ r
     if (threadIdx.x == 0) off[nb] = run;
 }
 
-__global__ void __launch_bounds__(256) k_tile_scatter(const RepDev* __restrict__ reps,
-                                                      const WorkItem* __restrict__ wmap, int koff) {
-    const WorkItem wi = wmap[blockIdx.x];
-    const RepDev& r = reps[wi.rep];
-    const int i = wi.blk * 256 + threadIdx.x;
-    if (i >= r.ns) return;
-    const int D = r.dim;
-    gmem<const float> x = G((const float*)r.X[(r.step + koff) & 1]);
-    const long long kk = G((const long long*)r.tkey)[i];
-    const int kb = (int)(unsigned)(kk & 0xffffffffll), ks = (int)(kk >> 32);
-    const int pos = G(r.toff)[kb] + ks;
-    gst(r.trec + pos, make_float4(x[i * D], x[i * D + 1], D == 3 ? x[i * D + 2] : 0.f, __int_as_float(G(r.tp)[i])));
-    if (r.trecq) G(r.trecq)[pos] = G(r.q)[i];
+// Bucket regions from an exact binning (toff): each bucket gets its count plus spare
+// slots (by default a quarter plus 16), so that beads can drift between buckets for
+// the steps until the next sizing without overflowing; tbase is the exclusive scan of
+// the sizes.
+__global__ void __launch_bounds__(1024) k_tile_regions(const RepDev* __restrict__ reps, int rep) {
+    __shared__ int wsum[32];
+    const RepDev& r = reps[rep];
+    const int nb = r.nbuckets;
+    gmem<const int> off = G((const int*)r.toff);
+    gmem<int> cap = G(r.tcap);
+    gmem<int> base = G(r.tbase);
+    int run = 0;
+    for (int c0 = 0; c0 < nb; c0 += 1024 * kScanPer) {
+        const int b0 = c0 + threadIdx.x * kScanPer;
+        int c[kScanPer];
+        int s = 0;
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) {
+            const int n = b0 + k < nb ? off[b0 + k + 1] - off[b0 + k] : 0;
+            const int spare = r.tslack == 0 ? n / 4 + 16 : (r.tslack > 0 ? r.tslack : 0);
+            c[k] = b0 + k < nb ? n + spare : 0;
+            s += c[k];
+        }
+        int total;
+        int pre = run + block_exclusive_scan_1024(s, wsum, total);
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k)
+            if (b0 + k < nb) {
+                cap[b0 + k] = c[k];
+                base[b0 + k] = pre;
+                pre += c[k];
+            }
+        run += total;
+    }
+    if (threadIdx.x == 0) base[nb] = run;
 }
 
 // Deposit one record into the tile's fixed-point LDS rows [f][lz][ly][x]; stencil
@@ -253,7 +315,7 @@ __device__ __forceinline__ void deposit_record_any(const RepDev& r, const float4
 
 // Deposit one tile's rows and transform them along x.
 __global__ void __launch_bounds__(kFftThreads) k_tile_deposit(const RepDev* __restrict__ reps,
-                                                              const WorkItem* __restrict__ wmap) {
+                                                              const WorkItem* __restrict__ wmap, int koff) {
     extern __shared__ unsigned long long tile_lds[];
     const WorkItem wi = wmap[blockIdx.x];
     const RepDev& r = reps[wi.rep];
@@ -262,35 +324,48 @@ __global__ void __launch_bounds__(kFftThreads) k_tile_deposit(const RepDev* __re
     const int t = wi.blk, ty = t % r.nty, tz = t / r.nty;
     const int y0 = ty * By, z0 = tz * Bz;
     const int rows = nf * By * Bz;
+    const int par = (r.step + koff) & 1;
     unsigned long long* patch = tile_lds;                         // [f][lz][ly][x]
     float2* fbuf = (float2*)(tile_lds + (size_t)rows * Nx);
+    // the other parity's counters were last used by the previous step: clear this
+    // tile's for the next one
+    if (threadIdx.x < 4) G(claim_counters(r, par ^ 1))[(t * 4 + threadIdx.x) << r.tshift] = 0;
+    if (t == 0 && threadIdx.x == 0) G(r.tovf)[(par ^ 1) * 32] = 0;
     for (int k = threadIdx.x; k < rows * Nx; k += kFftThreads) patch[k] = 0ull;
     __syncthreads();
-    // source ranges: own tile (all classes) and the lower neighbours' halo classes
-    int ra[4], rb[4], nr = 0;
-    {
-        const int tym = (ty + r.nty - 1) % r.nty;
-        const int tzm = (tz + r.ntz - 1) % r.ntz;
-        const int own = tz * r.nty + ty;
-        gmem<const int> toff = G((const int*)r.toff);
-        ra[nr] = toff[own * 4]; rb[nr] = toff[own * 4 + 4]; ++nr;
-        const int ny_ = tz * r.nty + tym;
-        ra[nr] = toff[ny_ * 4 + 1]; rb[nr] = toff[ny_ * 4 + 3]; ++nr;
-        if (D == 3) {
-            const int nz_ = tzm * r.nty + ty;
-            ra[nr] = toff[nz_ * 4 + 2]; rb[nr] = toff[nz_ * 4 + 4]; ++nr;
-            const int nyz = tzm * r.nty + tym;
-            ra[nr] = toff[nyz * 4 + 2]; rb[nr] = toff[nyz * 4 + 3]; ++nr;
-        }
-    }
+    // source buckets: own tile (all classes) and the lower neighbours' halo classes;
+    // each bucket's records are the filled part of its region
+    const int tym = (ty + r.nty - 1) % r.nty;
+    const int tzm = (tz + r.ntz - 1) % r.ntz;
+    const int own = tz * r.nty + ty, ny_ = tz * r.nty + tym, nz_ = tzm * r.nty + ty, nyz = tzm * r.nty + tym;
+    int bk[9], nbk = 0;   // 4 own + 2 (y) + 2 (z) + 1 (yz)
+    for (int c = 0; c < 4; ++c) bk[nbk++] = own * 4 + c;
+    bk[nbk++] = ny_ * 4 + 1; bk[nbk++] = ny_ * 4 + 2;
+    if (D == 3) { bk[nbk++] = nz_ * 4 + 2; bk[nbk++] = nz_ * 4 + 3; bk[nbk++] = nyz * 4 + 2; }
+    gmem<const int> cnt = G((const int*)claim_counters(r, par));
+    gmem<const int> base = G((const int*)r.tbase);
+    gmem<const int> cap = G((const int*)r.tcap);
     const int T = r.ntypes;
-    for (int s = 0; s < nr; ++s) {
-        for (int p = ra[s] + threadIdx.x; p < rb[s]; p += kFftThreads) {
+    for (int s = 0; s < nbk; ++s) {
+        const int bb = bk[s], ra = base[bb], rb = ra + min(cnt[bb << r.tshift], cap[bb]);
+        for (int p = ra + threadIdx.x; p < rb; p += kFftThreads) {
             const float4 rec = gld(r.trec + p);
             const float qi = r.trecq ? G((const float*)r.trecq)[p] : 0.f;
             if (P == 1) deposit_record<2>(r, rec, qi, patch, y0, z0, By, Bz);
             else deposit_record_any(r, rec, qi, patch, y0, z0, By, Bz);
         }
+    }
+    // beads whose region was full: the same bucket test over the overflow list
+    const int novf = G((const int*)r.tovf)[par * 32];
+    for (int j = threadIdx.x; j < novf; j += kFftThreads) {
+        const int bb = G((const int*)r.tovf_b)[j];
+        bool mine = false;
+        for (int s = 0; s < nbk; ++s) mine |= bb == bk[s];
+        if (!mine) continue;
+        const float4 rec = gld(r.tovf_rec + j);
+        const float qi = r.trecq ? G((const float*)r.tovf_q)[j] : 0.f;
+        if (P == 1) deposit_record<2>(r, rec, qi, patch, y0, z0, By, Bz);
+        else deposit_record_any(r, rec, qi, patch, y0, z0, By, Bz);
     }
     __syncthreads();
     // densities of the tile rows -> rho and the x-axis R2C (pairs of rows)
@@ -344,14 +419,23 @@ hipError_t launch_tile_scan(const LaunchCtx& c) {
     hipLaunchKernelGGL(k_tile_scan, dim3(c.n), dim3(1024), 0, c.stream, c.reps, c.wmap);
     CHECK_LAUNCH(); return hipSuccess;
 }
-hipError_t launch_tile_scatter(const LaunchCtx& c, int koff) {
+hipError_t launch_tile_claim(const LaunchCtx& c, int koff) {
     if (!c.n) return hipSuccess;
-    hipLaunchKernelGGL(k_tile_scatter, dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap, koff);
+    hipLaunchKernelGGL(k_tile_claim, dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap, koff);
     CHECK_LAUNCH(); return hipSuccess;
 }
-hipError_t launch_tile_deposit(const LaunchCtx& c) {
+hipError_t launch_tile_reset(const LaunchCtx& c, int koff) {
     if (!c.n) return hipSuccess;
-    hipLaunchKernelGGL(k_tile_deposit, dim3(c.n), dim3(kFftThreads), c.lds, c.stream, c.reps, c.wmap);
+    hipLaunchKernelGGL(k_tile_reset, dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap, koff);
+    CHECK_LAUNCH(); return hipSuccess;
+}
+hipError_t launch_tile_regions(hipStream_t s, const RepDev* reps, int rep) {
+    hipLaunchKernelGGL(k_tile_regions, dim3(1), dim3(1024), 0, s, reps, rep);
+    CHECK_LAUNCH(); return hipSuccess;
+}
+hipError_t launch_tile_deposit(const LaunchCtx& c, int koff) {
+    if (!c.n) return hipSuccess;
+    hipLaunchKernelGGL(k_tile_deposit, dim3(c.n), dim3(kFftThreads), c.lds, c.stream, c.reps, c.wmap, koff);
     CHECK_LAUNCH(); return hipSuccess;
 }
 

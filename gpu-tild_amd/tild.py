@@ -46,7 +46,7 @@ class Opts(ctypes.Structure):
                 ("spread_mode", ctypes.c_int), ("tile_target", ctypes.c_int), ("tile_by", ctypes.c_int),
                 ("tile_bz", ctypes.c_int), ("reorder_every", ctypes.c_int), ("fft_mid_elems", ctypes.c_int),
                 ("fft_last_lines", ctypes.c_int), ("fft_c2r_rows", ctypes.c_int), ("fft_plane", ctypes.c_int),
-                ("fft_last_dirs", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
+                ("fft_last_dirs", ctypes.c_int), ("tile_slack", ctypes.c_int), ("reserved", ctypes.c_int * 2)]
 
 
 class Props(ctypes.Structure):
@@ -226,7 +226,7 @@ class Engine:
 
     def __init__(self, system, device=0, seed=None, spread_mode=0, tile_target=0, tile_by=0, tile_bz=0,
                  reorder_every=0, fft_mid_elems=0, fft_last_lines=0, fft_c2r_rows=0, fft_plane=0,
-                 fft_last_dirs=0):
+                 fft_last_dirs=0, tile_slack=0):
         o = Opts()
         o.device = device
         o.spread_mode = spread_mode
@@ -234,6 +234,7 @@ class Engine:
         o.reorder_every = reorder_every
         o.fft_mid_elems, o.fft_last_lines, o.fft_c2r_rows = fft_mid_elems, fft_last_lines, fft_c2r_rows
         o.fft_plane, o.fft_last_dirs = fft_plane, fft_last_dirs
+        o.tile_slack = tile_slack
         if seed is not None:
             o.use_seed = 1
             o.seed = int(seed) & 0xFFFFFFFF
@@ -298,8 +299,7 @@ class Engine:
         self._chk(lib().tild_engine_get_bins(self._h, _ip(a), a.size), "get_bins")
         return a
 
-    PHASES = ["count", "scan", "scatter", "deposit", "plane_r2c", "last_fused", "y_ifft", "x_c2r", "particles",
-              "advance"]
+    PHASES = ["claim", "deposit", "plane_r2c", "last_fused", "y_ifft", "x_c2r", "particles", "advance"]
 
     def profile(self, nsteps):
         n = len(self.PHASES)

@@ -95,7 +95,10 @@ typedef struct {
                             it applies (default), < 0 separate passes.  Results do not
                             depend on it. */
     int fft_last_dirs;   /* inverse outputs per batch of the fused z pass, 1 or 2 (0 = 1) */
-    int reserved[3];
+    int tile_slack;      /* tiled spread: spare slots per bucket region beyond the count at
+                            the last sizing (0 = count/4 + 16; < 0 none, so that drifting
+                            beads take the overflow path).  Results do not depend on it. */
+    int reserved[2];
 } tild_opts;
 
 /* Properties of calc_properties() (calc_properties.cu:20-116): FP64 reductions. */
@@ -148,12 +151,12 @@ tild_status tild_engine_get_grid_field(tild_engine* e, int type, int dir, float*
    computes from FP32 x / dx (device_grid_utils.cu:136-146, :153-226), wrapped. */
 tild_status tild_engine_get_bins(tild_engine* e, int* base, size_t n);
 /* nsteps of tild_engine_step with a HIP event pair around every kernel launch on the
-   engine's stream; phase_ms[k] receives the mean duration of phase k (nphase >= 10),
-   one kernel each: 0 bucket count (global-atomic spread on that path), 1 bucket scan,
-   2 bucket scatter, 3 tile deposit (accumulator conversion + x R2C on the atomic path),
-   4 forward plane pass (y FFT without it), 5 fused last-axis k-space pass, 6 inverse
-   y-axis pass, 7 x-axis C2R, 8 gather+bonds+integrate, 9 step counter and the
-   amortised spatial reorder. */
+   engine's stream; phase_ms[k] receives the mean duration of phase k (nphase >= 8),
+   one kernel each: 0 bucket claims and records (global-atomic spread on that path),
+   1 tile deposit (accumulator conversion + x R2C on the atomic path), 2 forward plane
+   pass (y FFT without it), 3 fused last-axis k-space pass, 4 inverse y-axis pass,
+   5 x-axis C2R, 6 gather+bonds+integrate, 7 step counter and the amortised spatial
+   reorder. */
 tild_status tild_engine_profile(tild_engine* e, int nsteps, double* phase_ms, int nphase);
 tild_status tild_engine_get_step(tild_engine* e, int* step);
 
