@@ -346,14 +346,27 @@ __global__ void __launch_bounds__(kFftThreads) k_tile_deposit(const RepDev* __re
     gmem<const int> base = G((const int*)r.tbase);
     gmem<const int> cap = G((const int*)r.tcap);
     const int T = r.ntypes;
-    for (int s = 0; s < nbk; ++s) {
-        const int bb = bk[s], ra = base[bb], rb = ra + min(cnt[bb << r.tshift], cap[bb]);
-        for (int p = ra + threadIdx.x; p < rb; p += kFftThreads) {
-            const float4 rec = gld(r.trec + p);
-            const float qi = r.trecq ? G((const float*)r.trecq)[p] : 0.f;
-            if (P == 1) deposit_record<2>(r, rec, qi, patch, y0, z0, By, Bz);
-            else deposit_record_any(r, rec, qi, patch, y0, z0, By, Bz);
-        }
+    // the filled parts of the source regions as one index range: record j of the block
+    // is j + shift[k] for the last region k with start[k] <= j (all threads busy, no
+    // partial sweep per region)
+    int start[9], shift[9], total = 0;
+#pragma unroll
+    for (int s = 0; s < 9; ++s) {
+        const int bb = s < nbk ? bk[s] : 0;
+        const int n = s < nbk ? min(cnt[bb << r.tshift], cap[bb]) : 0;
+        start[s] = total;
+        shift[s] = base[bb] - total;
+        total += n;
+    }
+    for (int j = threadIdx.x; j < total; j += kFftThreads) {
+        int sh = shift[0];
+#pragma unroll
+        for (int s = 1; s < 9; ++s) sh = j >= start[s] ? shift[s] : sh;
+        const int p = j + sh;
+        const float4 rec = gld(r.trec + p);
+        const float qi = r.trecq ? G((const float*)r.trecq)[p] : 0.f;
+        if (P == 1) deposit_record<2>(r, rec, qi, patch, y0, z0, By, Bz);
+        else deposit_record_any(r, rec, qi, patch, y0, z0, By, Bz);
     }
     // beads whose region was full: the same bucket test over the overflow list
     const int novf = G((const int*)r.tovf)[par * 32];
