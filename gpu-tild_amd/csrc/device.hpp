@@ -124,6 +124,11 @@ struct WorkItem { int rep, blk; };
 // with the address space spelled out it emits global_* instructions.
 template <class T> using gmem = __attribute__((address_space(1))) T*;
 template <class T> __device__ __forceinline__ gmem<T> G(T* p) { return (gmem<T>)p; }
+// Device-scope atomic add on global memory (a global_atomic instruction; through a
+// generic pointer it is a flat atomic, which also waits on LDS traffic).
+__device__ __forceinline__ int gatomic_add(int* p, int v) {
+    return __hip_atomic_fetch_add(G(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 // float2 / float4 are classes and cannot be copied out of an address-space-qualified
 // lvalue; these go through the compiler's native vector types instead.
 typedef float f2v __attribute__((ext_vector_type(2)));

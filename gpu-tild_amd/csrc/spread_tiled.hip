@@ -26,11 +26,11 @@
 
 namespace tild {
 
-__device__ inline int tile_bucket(const RepDev& r, const float* xp, int i) {
+// Bucket of a bead from its y and z coordinates (z ignored in 2D).
+__device__ __forceinline__ int tile_bucket_yz(const RepDev& r, float y, float z) {
     const int D = r.dim, P = r.P;
-    gmem<const float> x = G(xp);
-    const int sy = stencil_node(x[i * D + 1], r.dx[1], r.Ny, P);
-    const int sz = D == 3 ? stencil_node(x[i * D + 2], r.dx[2], r.Nz, P) : 0;
+    const int sy = stencil_node(y, r.dx[1], r.Ny, P);
+    const int sz = D == 3 ? stencil_node(z, r.dx[2], r.Nz, P) : 0;
     int my, mz;
     const int ty = div_small(sy, r.By, r.iBy, my), tz = div_small(sz, r.Bz, r.iBz, mz);
     const bool yb = my + P >= r.By;
@@ -39,21 +39,26 @@ __device__ inline int tile_bucket(const RepDev& r, const float* xp, int i) {
     return (tz * r.nty + ty) * 4 + c;
 }
 
+__device__ __forceinline__ int tile_bucket(const RepDev& r, const float* xp, int i) {
+    gmem<const float> x = G(xp);
+    return tile_bucket_yz(r, x[i * r.dim + 1], r.dim == 3 ? x[i * r.dim + 2] : 0.f);
+}
+
 // Slot claims.  A block counts its beads in LDS over a window of kCountWin buckets
 // that starts at the block's smallest bucket -- in a spatially ordered replica that
 // is every bead of the block -- and claims slots from the counters `cnt` (bucket b at
 // b << tshift) with one atomic per distinct bucket.  Beads beyond the window claim per
 // wave: lanes that share a bucket use one atomic, issued by the lowest of them.  Slot
 // order inside a bucket is arbitrary; the deposit does not depend on it.  One bead per
-// thread (i = blk * 256 + threadIdx.x); every thread of the block must call it.
-constexpr int kCountWin = 1024;
+// thread (i = blk * 256 + threadIdx.x, bucket b, 0x7fffffff past the end); every
+// thread of the block must call it.
+constexpr int kCountWin = 256;
 
-__device__ __forceinline__ void claim_slot(const RepDev& r, const float* x, int i, int* cnt, int& b, int& slot) {
+__device__ __forceinline__ void claim_slot(const RepDev& r, int i, int b, int* cnt, int& slot) {
     __shared__ int hist[kCountWin];
     __shared__ int wmin[4];
     const int lane = (int)__lane_id();
     const bool valid = i < r.ns;
-    b = valid ? tile_bucket(r, x, i) : 0x7fffffff;
     int m = b;
     for (int k = threadIdx.x; k < kCountWin; k += 256) hist[k] = 0;
     for (int o = 32; o > 0; o >>= 1) m = min(m, __shfl_xor(m, o));
@@ -65,7 +70,7 @@ __device__ __forceinline__ void claim_slot(const RepDev& r, const float* x, int 
     __syncthreads();
     for (int k = threadIdx.x; k < kCountWin; k += 256) {
         const int c = hist[k];
-        if (c) hist[k] = atomicAdd(cnt + ((bmin + k) << r.tshift), c);   // first slot of the bucket
+        if (c) hist[k] = gatomic_add(cnt + ((bmin + k) << r.tshift), c);   // first slot of the bucket
     }
     __syncthreads();
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -76,7 +81,7 @@ __device__ __forceinline__ void claim_slot(const RepDev& r, const float* x, int 
         const int bl = __shfl(b, leader);
         const unsigned long long same = __ballot(b == bl) & todo;
         int base = 0;
-        if (lane == leader) base = atomicAdd(cnt + (bl << r.tshift), __popcll(same));
+        if (lane == leader) base = gatomic_add(cnt + (bl << r.tshift), __popcll(same));
         base = __shfl(base, leader);
         if (b == bl && !inwin) slot = base + __popcll(same & below);
         todo &= ~same;
@@ -90,8 +95,9 @@ __global__ void __launch_bounds__(256) k_tile_count(const RepDev* __restrict__ r
     const RepDev& r = reps[wi.rep];
     const float* x = r.X[(r.step + koff) & 1];
     const int i = wi.blk * 256 + threadIdx.x;
-    int b, slot;
-    claim_slot(r, x, i, r.tcnt, b, slot);
+    const int b = i < r.ns ? tile_bucket(r, x, i) : 0x7fffffff;
+    int slot;
+    claim_slot(r, i, b, r.tcnt, slot);
     if (i < r.ns) G((long long*)r.tkey)[i] = (long long)(((unsigned long long)(unsigned)slot << 32) | (unsigned)b);
 }
 
@@ -111,21 +117,33 @@ __global__ void __launch_bounds__(256) k_tile_claim(const RepDev* __restrict__ r
     const int step = r.step + koff;
     const float* xp = r.X[step & 1];
     const int i = wi.blk * 256 + threadIdx.x;
-    int b, slot;
-    claim_slot(r, xp, i, claim_counters(r, step & 1), b, slot);
-    if (i >= r.ns) return;
+    const bool valid = i < r.ns;
+    // the record's inputs and the region bounds are requested before the slot claims,
+    // so that their latency hides behind the atomics and barriers
     const int D = r.dim;
     gmem<const float> x = G(xp);
-    const float4 rec = make_float4(x[i * D], x[i * D + 1], D == 3 ? x[i * D + 2] : 0.f, __int_as_float(G(r.tp)[i]));
-    if (slot < G((const int*)r.tcap)[b]) {
-        const int pos = G((const int*)r.tbase)[b] + slot;
+    float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
+    float qi = 0.f;
+    int b = 0x7fffffff, cap = 0, base = 0;
+    if (valid) {
+        rec = make_float4(x[i * D], x[i * D + 1], D == 3 ? x[i * D + 2] : 0.f, __int_as_float(G(r.tp)[i]));
+        if (r.trecq) qi = G(r.q)[i];
+        b = tile_bucket_yz(r, rec.y, rec.z);
+        cap = G((const int*)r.tcap)[b];
+        base = G((const int*)r.tbase)[b];
+    }
+    int slot;
+    claim_slot(r, i, b, claim_counters(r, step & 1), slot);
+    if (!valid) return;
+    if (slot < cap) {
+        const int pos = base + slot;
         gst(r.trec + pos, rec);
-        if (r.trecq) G(r.trecq)[pos] = G(r.q)[i];
+        if (r.trecq) G(r.trecq)[pos] = qi;
     } else {
-        const int o = atomicAdd(r.tovf + (step & 1) * 32, 1);
+        const int o = gatomic_add(r.tovf + (step & 1) * 32, 1);
         gst(r.tovf_rec + o, rec);
         G(r.tovf_b)[o] = b;
-        if (r.trecq) G(r.tovf_q)[o] = G(r.q)[i];
+        if (r.trecq) G(r.tovf_q)[o] = qi;
     }
 }
 
