@@ -29,26 +29,27 @@ sys.path.insert(0, str(REPO / "gpu-tild_amd"))
 HBM_PEAK_GBS = 8000.0
 
 
-def phase_bytes(ns, D, M, Mh, nf, T, gjf, charged, ncoef):
+def phase_bytes(ns, D, M, Mh, nf, T, gjf, charged, ncoef, reorder_every):
     """HBM bytes of each profiled kernel for one step of one replica.  The particle
     kernel uses the per-unit figures of the reference path's minimal traffic: per bead
     it reads x and the type and writes x (8D + 4 B), plus the GJF state read and written
     (16D B) and the charge (4 B); per grid node it reads each type's gather mask and D
-    field components (4T(1 + D) B) and the D electric-field components (4D B).  The
-    other kernels count this design's minimal bytes (diagnostic only): bucket claims x
-    and type in, 16-byte record out; deposit records
-    in, densities out; forward plane densities in, half spectra out; fused pass spectra
-    and coupling table in, 2 nf spectra out; y inverse 2 nf spectra in, nf D out; x C2R
-    nf D spectra and the densities in, 12-byte records out."""
+    field components (4T(1 + D) B) and the D electric-field components (4D B).  On top
+    of that it writes each bead's 16-byte bucket record (and charge) for the next step.
+    The other kernels count this design's minimal bytes (diagnostic only): a bucket
+    claim pass (x and type in, record out) runs once per spatial reorder; deposit
+    records in, densities out; forward plane densities in, half spectra out; fused pass
+    spectra and coupling table in, 2 nf spectra out; y inverse 2 nf spectra in, nf D
+    out; x C2R nf D spectra and the densities in, 12-byte records out."""
     q = 4 if charged else 0
     b = {
-        "claim": ns * (4 * D + 4 + 16 + q),
+        "claim": ns * (4 * D + 4 + 16 + q) / reorder_every,
         "deposit": ns * (16 + q) + nf * M * 4,
         "plane_r2c": nf * M * 4 + nf * Mh * 8,
         "last_fused": nf * Mh * 8 + ncoef * Mh * 4 + 2 * nf * Mh * 8,
         "y_ifft": 2 * nf * Mh * 8 + nf * D * Mh * 8 if D == 3 else 0,
         "x_c2r": nf * D * Mh * 8 + T * M * 4 + nf * M * 12,
-        "particles": ns * (8 * D + 4 + (16 * D if gjf else 0) + q)
+        "particles": ns * (8 * D + 4 + (16 * D if gjf else 0) + q + 16 + q)
                      + M * (4 * T * (1 + D) + (4 * D if charged else 0)),
         "advance": 0,
     }
@@ -58,7 +59,7 @@ def phase_bytes(ns, D, M, Mh, nf, T, gjf, charged, ncoef):
 # the kernel of each profiled phase on the default (tiled, forward-plane) path, as named
 # in the rocprofv3 summaries
 PHASE_KERNEL = {"claim": "k_tile_claim", "deposit": "k_tile_deposit", "plane_r2c": "k_plane_r2c<512>", "last_fused": "k_last_fused",
-                "y_ifft": "k_mid_grad", "x_c2r": "k_x_c2r<true>", "particles": "k_particles<true>",
+                "y_ifft": "k_mid_grad", "x_c2r": "k_x_c2r<true>", "particles": "k_particles<true, true>",
                 "advance": "k_advance"}
 
 
@@ -235,7 +236,9 @@ def run(args):
         value = ns * args.steps / wall
 
         prof = eng.profile(args.profile_steps)
-        pb = phase_bytes(ns, D, M, Mh, T, T, True, False, T * (T + 1) // 2)
+        # automatic reorder interval of this workload: 40 steps
+        pb = phase_bytes(ns, D, M, Mh, T, T, True, False, T * (T + 1) // 2,
+                         args.reorder_every if args.reorder_every > 0 else (40 if args.reorder_every == 0 else args.steps))
         dom = max((k for k in prof if pb[k] > 0), key=lambda k: prof[k])   # dominant kernel
         ach = pb[dom] / (prof[dom] * 1e-3) / 1e9
         step_bytes = step_bytes_minimal(ns, D, M, T, True, False)

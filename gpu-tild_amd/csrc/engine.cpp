@@ -437,6 +437,7 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
 int Batch::tile_regions(int id, bool bin_only, std::string& err) {
     Replica& R = *reps_[id];
     if (R.h.spread_mode != 2) return 0;
+    if (int rc = unclaim(id, err)) return rc;
     PhaseMap* mp = R.maps;
     auto ctx = [&](Phase p) { return LaunchCtx{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_}; };
     HIPCHK(launch_tile_count(ctx(PH_TCOUNT), 0));
@@ -445,8 +446,30 @@ int Batch::tile_regions(int id, bool bin_only, std::string& err) {
     return 0;
 }
 
+int Batch::unclaim(int id, std::string& err) {
+    Replica& R = *reps_[id];
+    if (!R.claimed) return 0;
+    PhaseMap* mp = R.maps;
+    HIPCHK(launch_tile_reset(LaunchCtx{d_reps_, mp[PH_SCAN].dev, mp[PH_SCAN].n, 0, stream_}, 0));
+    R.claimed = false;
+    return 0;
+}
+
+int Batch::ensure_claimed(SetMaps& m, const std::vector<int>& ids, std::string& err) {
+    bool all = true;
+    for (int id : ids) all = all && reps_[id]->claimed;
+    if (all) return 0;
+    // one claim pass over the set: replicas that hold claims drop them first
+    for (int id : ids)
+        if (int rc = unclaim(id, err)) return rc;
+    HIPCHK(launch_tile_claim(LaunchCtx{d_reps_, m.maps[PH_TCOUNT].dev, m.maps[PH_TCOUNT].n, 0, stream_}, 0));
+    for (int id : ids) reps_[id]->claimed = true;
+    return 0;
+}
+
 int Batch::reorder(int id, std::string& err, bool after_step) {
     Replica& R = *reps_[id];
+    if (int rc = unclaim(id, err)) return rc;   // the slots move
     if (after_step && R.h.spread_mode == 2) {   // bin, sort each tile by site id, permute
         if (int rc = tile_regions(id, true, err)) return rc;
         HIPCHK(launch_reorder_tiles(stream_, d_reps_, id, R.ns, R.h.ntiles, R.rb));
@@ -817,17 +840,17 @@ int Batch::enqueue_steps(SetMaps& m, int nsteps, bool integrate, bool write_forc
     for (int k = 0; k < nsteps; ++k) {
         HIPCHK(launch_spread(ctx(PH_SPREADA), k));
         HIPCHK(launch_x_r2c(ctx(PH_XA), true, 0, nullptr, nullptr));
-        HIPCHK(launch_tile_claim(ctx(PH_TCOUNT), k));
         HIPCHK(launch_tile_deposit(ctx(PH_TILE), k));
         HIPCHK(launch_plane_r2c(ctx(PH_PLANEF)));
         HIPCHK(launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr));
         HIPCHK(launch_last_fused(ctx(PH_LAST), 0));
         HIPCHK(launch_mid_grad(ctx(PH_MIDY_G)));
         HIPCHK(launch_x_c2r(ctx(PH_C2R), true, 0, nullptr, nullptr));
-        HIPCHK(launch_particles(ctx(PH_PART), integrate, write_forces ? 1 : 0, k));
+        // integrating steps claim the next step's records; a force evaluation leaves
+        // positions and records as they are
+        HIPCHK(launch_particles(ctx(PH_PART), integrate, write_forces ? 1 : 0, k, integrate));
     }
     if (integrate) HIPCHK(launch_advance(stream_, d_reps_, m.d_ids, m.nids, nsteps));
-    else HIPCHK(launch_tile_reset(ctx(PH_SCAN), 0));   // the next step claims from the same parity
     return 0;
 }
 
@@ -837,6 +860,7 @@ int Batch::forces(int id, std::string& err) {
     HIPCHK(hipSetDevice(device_));
     SetMaps* m = maps_for({id}, err);
     if (!m) return -TILD_ERR_DEVICE;
+    if (int rc = ensure_claimed(*m, {id}, err)) return rc;
     return enqueue_steps(*m, 1, false, true, err);
 }
 
@@ -866,6 +890,7 @@ int Batch::step(const std::vector<int>& ids_in, int n, std::string& err, bool us
             const Replica& R = *reps_[id];
             if (R.reorder_every > 0) chunk = std::min(chunk, std::max(1, R.reorder_every - R.since_sort));
         }
+        if (int rc = ensure_claimed(*m, ids, err)) return rc;
         if (int rc = run_steps(*m, ids, chunk, use_graph, err)) return rc;
         done += chunk;
         for (int id : ids) {
@@ -927,7 +952,7 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
         // role shares its phase
         rec(0);
         if (e0 == hipSuccess) e0 = launch_spread(ctx(PH_SPREADA), 0);
-        if (e0 == hipSuccess) e0 = launch_tile_claim(ctx(PH_TCOUNT), 0);
+        if (e0 == hipSuccess && ensure_claimed(*m, ids, err)) e0 = hipErrorUnknown;   // after a reorder
         rec(1);
         if (e0 == hipSuccess) e0 = launch_x_r2c(ctx(PH_XA), true, 0, nullptr, nullptr);
         if (e0 == hipSuccess) e0 = launch_tile_deposit(ctx(PH_TILE), 0);
@@ -941,7 +966,7 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
         rec(5);
         if (e0 == hipSuccess) e0 = launch_x_c2r(ctx(PH_C2R), true, 0, nullptr, nullptr);
         rec(6);
-        if (e0 == hipSuccess) e0 = launch_particles(ctx(PH_PART), true, 0, 0);
+        if (e0 == hipSuccess) e0 = launch_particles(ctx(PH_PART), true, 0, 0, true);
         rec(7);
         if (e0 == hipSuccess) e0 = launch_advance(stream_, d_reps_, m->d_ids, m->nids, 1);
         for (int id : ids) {   // amortised spatial reorders are timed with the step counter

@@ -58,6 +58,9 @@ struct Replica {
     // periodic spatial reorder of the particle arrays (0 = never)
     int reorder_every = 0, since_sort = 0, sort_bits = 0;
     bool permuted = false;            // slots no longer in site order (getters un-permute)
+    // LDS-tiled replicas: the current step's records are in the bucket regions (made
+    // by the last integrating step's particle kernel, or by a bucket-claim pass)
+    bool claimed = false;
     ReorderBufs rb;
 };
 
@@ -118,8 +121,13 @@ public:
     };
     SetMaps* maps_for(const std::vector<int>& ids, std::string& err);
 
-    // Step pipeline of a replica set with prepared maps (graph-capturable).
+    // Step pipeline of a replica set with prepared maps (graph-capturable).  The
+    // records of the first step must be claimed (ensure_claimed); integrating steps
+    // leave the next step's records claimed.
     int enqueue_steps(SetMaps& m, int nsteps, bool integrate, bool write_forces, std::string& err);
+    // Claims the current step's records of every replica of the set that has none
+    // (a replica whose claims are stale is cleared first).
+    int ensure_claimed(SetMaps& m, const std::vector<int>& ids, std::string& err);
 
 private:
     int device_;
@@ -141,6 +149,9 @@ private:
     int run_steps(SetMaps& m, const std::vector<int>& ids, int n, bool use_graph, std::string& err);
     int capture_steps(SetMaps& m, int nsteps, hipGraphExec_t* out, std::string& err);
     int tile_regions(int id, bool bin_only, std::string& err);
+    // drops the claimed records of one replica (positions or regions change): clears
+    // the slot counters of its current step parity
+    int unclaim(int id, std::string& err);
     static void drop_graphs(SetMaps& m);
     int fetch_pid(const Replica& R, std::vector<int>& pid, std::string& err);
     void free_maps(PhaseMap* maps);

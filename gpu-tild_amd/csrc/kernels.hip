@@ -8,12 +8,14 @@
 //                   inverse z-axis FFT -> Gk
 //   k_mid (inv)     inverse y-axis FFT of Gk
 //   k_x_c2r         inverse x-axis C2R -> float4 (g_x, g_y, g_z, rho) records
-//   k_particles     gather + bonds + EM/GJF update (double-buffered positions)
+//   k_particles     gather + bonds + EM/GJF update (double-buffered positions); on
+//                   LDS-tiled replicas also the next step's bucket claims
 // Log steps add k_kprops (energies/virial from the spectra), k_bond_props and
 // k_reduce (fixed-order FP64 sums).
 #include "device.hpp"
 #include "fft_dev.hpp"
 #include "launch.hpp"
+#include "tile_dev.hpp"
 
 namespace tild {
 
@@ -614,8 +616,14 @@ __device__ __forceinline__ float min_image(float dr, float L, float Lh) {
     return dr > Lh ? hi : (dr < -Lh ? lo : dr);
 }
 
+// A bead's state for the next step's bucket claim: new position, type and charge.
+struct BeadNext {
+    float4 rec;   // (x, y, z, type bits)
+    float q;
+};
+
 template <bool INTEGRATE, int D, int NP, bool OFF32>
-__device__ __forceinline__ void particle_bead(const RepDev& r, int i, int write_forces, int step) {
+__device__ __forceinline__ void particle_bead(const RepDev& r, int i, int write_forces, int step, BeadNext& nx) {
     const int P = NP > 0 ? NP - 1 : r.P;
     const float* xp = r.X[step & 1];
     // per-bead loads
@@ -712,29 +720,41 @@ __device__ __forceinline__ void particle_bead(const RepDev& r, int i, int write_
         }
         store_pos<D>(r.X[(step + 1) & 1] + (size_t)i * D, v);
         if (kind == 2) store_pos<D>(r.XO + (size_t)i * D, dn);
+        nx.rec = make_float4(v[0], v[1], v[2], __int_as_float(t));
+        nx.q = qi;
     }
 }
 
-template <bool INTEGRATE>
-__global__ void __launch_bounds__(256) k_particles(const RepDev* __restrict__ reps,
+// CLAIM (integrating steps of LDS-tiled replicas): the block then claims the next
+// step's bucket slots from the positions it has just written, which takes the bucket
+// claim pass out of every step but the first of a run.
+template <bool INTEGRATE, bool CLAIM>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_particles(const RepDev* __restrict__ reps,
                                                    const WorkItem* __restrict__ wmap,
                                                    int write_forces, int koff) {
     const WorkItem wi = wmap[blockIdx.x];
     const RepDev& r = reps[wi.rep];
     const int i = wi.blk * 256 + threadIdx.x;
-    if (i >= r.ns) return;
+    const bool valid = i < r.ns;
+    if (!CLAIM && !valid) return;
     const int step = r.step + koff;
+    BeadNext nx{make_float4(0.f, 0.f, 0.f, 0.f), 0.f};
     // unrolled bodies: the default order P = 1 on grids whose records fit 32-bit
     // offsets, and P = 0; any other case takes the looped body
-    if (r.P == 1 && r.g3_off32) {
-        if (r.dim == 3) particle_bead<INTEGRATE, 3, 2, true>(r, i, write_forces, step);
-        else particle_bead<INTEGRATE, 2, 2, true>(r, i, write_forces, step);
-    } else if (r.P == 0) {
-        if (r.dim == 3) particle_bead<INTEGRATE, 3, 1, false>(r, i, write_forces, step);
-        else particle_bead<INTEGRATE, 2, 1, false>(r, i, write_forces, step);
-    } else {
-        if (r.dim == 3) particle_bead<INTEGRATE, 3, 0, false>(r, i, write_forces, step);
-        else particle_bead<INTEGRATE, 2, 0, false>(r, i, write_forces, step);
+    if (valid) {
+        if (r.P == 1 && r.g3_off32) {
+            if (r.dim == 3) particle_bead<INTEGRATE, 3, 2, true>(r, i, write_forces, step, nx);
+            else particle_bead<INTEGRATE, 2, 2, true>(r, i, write_forces, step, nx);
+        } else if (r.P == 0) {
+            if (r.dim == 3) particle_bead<INTEGRATE, 3, 1, false>(r, i, write_forces, step, nx);
+            else particle_bead<INTEGRATE, 2, 1, false>(r, i, write_forces, step, nx);
+        } else {
+            if (r.dim == 3) particle_bead<INTEGRATE, 3, 0, false>(r, i, write_forces, step, nx);
+            else particle_bead<INTEGRATE, 2, 0, false>(r, i, write_forces, step, nx);
+        }
+    }
+    if constexpr (CLAIM) {
+        if (r.spread_mode == 2) claim_record(r, valid, nx.rec, nx.q, (step + 1) & 1);
     }
 }
 
@@ -952,10 +972,14 @@ hipError_t launch_mid_grad(const LaunchCtx& c) {
     hipLaunchKernelGGL(k_mid_grad, dim3(c.n), dim3(kFftThreads), c.lds, c.stream, c.reps, c.wmap);
     CHECK_LAUNCH(); return hipSuccess;
 }
-hipError_t launch_particles(const LaunchCtx& c, bool integrate, int write_forces, int koff) {
+hipError_t launch_particles(const LaunchCtx& c, bool integrate, int write_forces, int koff, bool claim) {
     if (!c.n) return hipSuccess;
-    if (integrate) hipLaunchKernelGGL(k_particles<true>, dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap, write_forces, koff);
-    else hipLaunchKernelGGL(k_particles<false>, dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap, write_forces, koff);
+    if (integrate && claim)
+        hipLaunchKernelGGL((k_particles<true, true>), dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap, write_forces, koff);
+    else if (integrate)
+        hipLaunchKernelGGL((k_particles<true, false>), dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap, write_forces, koff);
+    else
+        hipLaunchKernelGGL((k_particles<false, false>), dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap, write_forces, koff);
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t launch_kprops(const LaunchCtx& c, int nq_max) {

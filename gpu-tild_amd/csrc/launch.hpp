@@ -66,7 +66,8 @@ hipError_t launch_x_r2c(const LaunchCtx& c, bool from_acc, int nrows_override, c
 hipError_t launch_mid(const LaunchCtx& c, int axis, int use_gk, int nfb, int inverse, float2* buf);
 hipError_t launch_last_fused(const LaunchCtx& c, int store_spectrum);
 hipError_t launch_x_c2r(const LaunchCtx& c, bool records, int nlines, const float2* in, float* out);
-hipError_t launch_particles(const LaunchCtx& c, bool integrate, int write_forces, int koff);
+// claim: integrating steps of LDS-tiled replicas also claim the next step's records
+hipError_t launch_particles(const LaunchCtx& c, bool integrate, int write_forces, int koff, bool claim);
 hipError_t launch_advance(hipStream_t s, RepDev* reps, const int* which, int n, int nsteps);
 constexpr int kPlaneThreads = 512;                      // block of the plane forward pass
 constexpr size_t kPlaneLds = 163840;                    // LDS one block may use (MI355X)

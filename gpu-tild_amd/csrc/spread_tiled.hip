@@ -6,7 +6,8 @@
 // first stencil row, halo class).  Every step:
 //   k_tile_claim    each bead claims a slot in its bucket and writes its (x, y, z,
 //                   type) record into the bucket's region (an overflow list takes
-//                   the beads of full regions)
+//                   the beads of full regions); fused into the particle kernel of
+//                   the previous step
 //   k_tile_deposit  one block per tile: deposits its own particles and the halo
 //                   particles of its three lower neighbours into 64-bit
 //                   fixed-point LDS accumulators, converts its rows to densities
@@ -16,77 +17,16 @@
 // Integer accumulation makes the densities independent of bucket order, tile
 // shape and thread scheduling: they are bit-identical to the global-atomic path.
 //
-// Halo classes: a particle whose stencil rows also reach the next tile in y (yb)
-// and/or z (zb) is tagged 0 none, 1 y only, 2 y and z, 3 z only.  This order makes
-// every neighbour's needed subset contiguous: y-neighbour classes [1,3),
-// z-neighbour [2,4), yz-neighbour [2,3).
+// After the first step of a run the particle kernel makes the claims of the next step
+// (from the positions it has just written), so k_tile_claim runs only where no claims
+// are in the regions: at the start of a run after a spatial reorder, new positions or
+// a new replica.  Bucket and halo classes are defined in tile_dev.hpp.
 #include "device.hpp"
 #include "fft_dev.hpp"
 #include "launch.hpp"
+#include "tile_dev.hpp"
 
 namespace tild {
-
-// Bucket of a bead from its y and z coordinates (z ignored in 2D).
-__device__ __forceinline__ int tile_bucket_yz(const RepDev& r, float y, float z) {
-    const int D = r.dim, P = r.P;
-    const int sy = stencil_node(y, r.dx[1], r.Ny, P);
-    const int sz = D == 3 ? stencil_node(z, r.dx[2], r.Nz, P) : 0;
-    int my, mz;
-    const int ty = div_small(sy, r.By, r.iBy, my), tz = div_small(sz, r.Bz, r.iBz, mz);
-    const bool yb = my + P >= r.By;
-    const bool zb = D == 3 && mz + P >= r.Bz;
-    const int c = (yb && zb) ? 2 : (yb ? 1 : (zb ? 3 : 0));
-    return (tz * r.nty + ty) * 4 + c;
-}
-
-__device__ __forceinline__ int tile_bucket(const RepDev& r, const float* xp, int i) {
-    gmem<const float> x = G(xp);
-    return tile_bucket_yz(r, x[i * r.dim + 1], r.dim == 3 ? x[i * r.dim + 2] : 0.f);
-}
-
-// Slot claims.  A block counts its beads in LDS over a window of kCountWin buckets
-// that starts at the block's smallest bucket -- in a spatially ordered replica that
-// is every bead of the block -- and claims slots from the counters `cnt` (bucket b at
-// b << tshift) with one atomic per distinct bucket.  Beads beyond the window claim per
-// wave: lanes that share a bucket use one atomic, issued by the lowest of them.  Slot
-// order inside a bucket is arbitrary; the deposit does not depend on it.  One bead per
-// thread (i = blk * 256 + threadIdx.x, bucket b, 0x7fffffff past the end); every
-// thread of the block must call it.
-constexpr int kCountWin = 256;
-
-__device__ __forceinline__ void claim_slot(const RepDev& r, int i, int b, int* cnt, int& slot) {
-    __shared__ int hist[kCountWin];
-    __shared__ int wmin[4];
-    const int lane = (int)__lane_id();
-    const bool valid = i < r.ns;
-    int m = b;
-    for (int k = threadIdx.x; k < kCountWin; k += 256) hist[k] = 0;
-    for (int o = 32; o > 0; o >>= 1) m = min(m, __shfl_xor(m, o));
-    if (lane == 0) wmin[threadIdx.x >> 6] = m;
-    __syncthreads();
-    const int bmin = min(min(wmin[0], wmin[1]), min(wmin[2], wmin[3]));
-    const bool inwin = valid && b - bmin < kCountWin;
-    const int local = inwin ? atomicAdd(&hist[b - bmin], 1) : 0;
-    __syncthreads();
-    for (int k = threadIdx.x; k < kCountWin; k += 256) {
-        const int c = hist[k];
-        if (c) hist[k] = gatomic_add(cnt + ((bmin + k) << r.tshift), c);   // first slot of the bucket
-    }
-    __syncthreads();
-    const unsigned long long below = (1ull << lane) - 1ull;
-    slot = inwin ? hist[b - bmin] + local : 0;
-    unsigned long long todo = __ballot(valid && !inwin);
-    while (todo) {   // wave-uniform: one round per distinct bucket beyond the window
-        const int leader = __ffsll((long long)todo) - 1;
-        const int bl = __shfl(b, leader);
-        const unsigned long long same = __ballot(b == bl) & todo;
-        int base = 0;
-        if (lane == leader) base = gatomic_add(cnt + (bl << r.tshift), __popcll(same));
-        base = __shfl(base, leader);
-        if (b == bl && !inwin) slot = base + __popcll(same & below);
-        todo &= ~same;
-    }
-}
 
 // Exact binning (spatial reorder and region sizing): (bucket, slot) of every bead.
 __global__ void __launch_bounds__(256) k_tile_count(const RepDev* __restrict__ reps,
@@ -97,54 +37,29 @@ __global__ void __launch_bounds__(256) k_tile_count(const RepDev* __restrict__ r
     const int i = wi.blk * 256 + threadIdx.x;
     const int b = i < r.ns ? tile_bucket(r, x, i) : 0x7fffffff;
     int slot;
-    claim_slot(r, i, b, r.tcnt, slot);
+    claim_slot(r, i < r.ns, b, r.tcnt, slot);
     if (i < r.ns) G((long long*)r.tkey)[i] = (long long)(((unsigned long long)(unsigned)slot << 32) | (unsigned)b);
 }
 
-// Slot counters of one step parity.
-__device__ __forceinline__ int* claim_counters(const RepDev& r, int parity) {
-    return r.tclaim + (((size_t)parity * r.nbuckets) << r.tshift);
-}
-
-// Per-step binning: each bead claims a slot in its bucket and writes its record
-// (x, y, z, type) straight into the bucket's region -- no scan, no second pass.  A
-// bead whose region is full goes to the overflow list, which the deposit reads too,
-// so the densities stay exact whatever the occupancy.
+// Per-step binning at the start of a run: each bead claims a slot in its bucket and
+// writes its record (x, y, z, type) straight into the bucket's region -- no scan, no
+// second pass.
 __global__ void __launch_bounds__(256) k_tile_claim(const RepDev* __restrict__ reps,
                                                     const WorkItem* __restrict__ wmap, int koff) {
     const WorkItem wi = wmap[blockIdx.x];
     const RepDev& r = reps[wi.rep];
     const int step = r.step + koff;
-    const float* xp = r.X[step & 1];
     const int i = wi.blk * 256 + threadIdx.x;
     const bool valid = i < r.ns;
-    // the record's inputs and the region bounds are requested before the slot claims,
-    // so that their latency hides behind the atomics and barriers
     const int D = r.dim;
-    gmem<const float> x = G(xp);
+    gmem<const float> x = G((const float*)r.X[step & 1]);
     float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
     float qi = 0.f;
-    int b = 0x7fffffff, cap = 0, base = 0;
     if (valid) {
         rec = make_float4(x[i * D], x[i * D + 1], D == 3 ? x[i * D + 2] : 0.f, __int_as_float(G(r.tp)[i]));
         if (r.trecq) qi = G(r.q)[i];
-        b = tile_bucket_yz(r, rec.y, rec.z);
-        cap = G((const int*)r.tcap)[b];
-        base = G((const int*)r.tbase)[b];
     }
-    int slot;
-    claim_slot(r, i, b, claim_counters(r, step & 1), slot);
-    if (!valid) return;
-    if (slot < cap) {
-        const int pos = base + slot;
-        gst(r.trec + pos, rec);
-        if (r.trecq) G(r.trecq)[pos] = qi;
-    } else {
-        const int o = gatomic_add(r.tovf + (step & 1) * 32, 1);
-        gst(r.tovf_rec + o, rec);
-        G(r.tovf_b)[o] = b;
-        if (r.trecq) G(r.tovf_q)[o] = qi;
-    }
+    claim_record(r, valid, rec, qi, step & 1);
 }
 
 // Clears the slot counters and overflow count of one step parity (after a force
