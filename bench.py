@@ -52,6 +52,7 @@ def phase_bytes(ns, D, M, Mh, nf, T, gjf, charged, ncoef, reorder_every):
         "particles": ns * (8 * D + 4 + (16 * D if gjf else 0) + q + 16 + q)
                      + M * (4 * T * (1 + D) + (4 * D if charged else 0)),
         "advance": 0,
+        "reorder": 0,   # several kernels; not a roofline candidate
     }
     return b
 
@@ -60,7 +61,7 @@ def phase_bytes(ns, D, M, Mh, nf, T, gjf, charged, ncoef, reorder_every):
 # in the rocprofv3 summaries
 PHASE_KERNEL = {"claim": "k_tile_claim", "deposit": "k_tile_deposit", "plane_r2c": "k_plane_r2c<512>", "last_fused": "k_last_fused",
                 "y_ifft": "k_mid_grad", "x_c2r": "k_x_c2r<true>", "particles": "k_particles<true, true>",
-                "advance": "k_advance"}
+                "advance": "k_advance", "reorder": None}
 
 
 def step_bytes_minimal(ns, D, M, T, gjf, charged):

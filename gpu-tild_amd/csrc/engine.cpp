@@ -455,14 +455,16 @@ int Batch::unclaim(int id, std::string& err) {
     return 0;
 }
 
-int Batch::ensure_claimed(SetMaps& m, const std::vector<int>& ids, std::string& err) {
+int Batch::ensure_claimed(SetMaps& m, const std::vector<int>& ids, std::string& err,
+                          hipEvent_t t0, hipEvent_t t1, bool* launched) {
     bool all = true;
     for (int id : ids) all = all && reps_[id]->claimed;
+    if (launched) *launched = !all && m.maps[PH_TCOUNT].n > 0;
     if (all) return 0;
     // one claim pass over the set: replicas that hold claims drop them first
     for (int id : ids)
         if (int rc = unclaim(id, err)) return rc;
-    HIPCHK(launch_tile_claim(LaunchCtx{d_reps_, m.maps[PH_TCOUNT].dev, m.maps[PH_TCOUNT].n, 0, stream_}, 0));
+    HIPCHK(launch_tile_claim(LaunchCtx{d_reps_, m.maps[PH_TCOUNT].dev, m.maps[PH_TCOUNT].n, 0, stream_, t0, t1}, 0));
     for (int id : ids) reps_[id]->claimed = true;
     return 0;
 }
@@ -939,53 +941,69 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
     HIPCHK(hipSetDevice(device_));
     SetMaps* m = maps_for(ids, err);
     if (!m) return -TILD_ERR_DEVICE;
-    hipEvent_t ev[kProfPhases + 1];
-    for (int k = 0; k <= kProfPhases; ++k) HIPCHK(hipEventCreate(&ev[k]));
+    // phases 0-7 are kernels, timed by events bound to each launch's own start and stop
+    // (at most two kernels per phase: the two spread paths of a mixed set); phase 8,
+    // the spatial reorders, by an event pair around them
+    constexpr int kSlots = 2, kKernelPhases = kProfPhases - 1;
+    hipEvent_t ev[kKernelPhases][kSlots][2], rv[2];
+    for (auto& a : ev) for (auto& b : a) for (auto& e : b) HIPCHK(hipEventCreate(&e));
+    for (auto& e : rv) HIPCHK(hipEventCreate(&e));
     std::vector<double> acc(kProfPhases, 0.0);
     PhaseMap* mp = m->maps;
-    auto ctx = [&](Phase p) { return LaunchCtx{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_}; };
     int rc = 0;
     for (int s = 0; s < n && !rc; ++s) {
+        bool used[kKernelPhases][kSlots] = {};
+        auto ctx = [&](Phase p, int ph, int slot) {
+            LaunchCtx c{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_};
+            if (c.n > 0) { c.t0 = ev[ph][slot][0]; c.t1 = ev[ph][slot][1]; used[ph][slot] = true; }
+            return c;
+        };
         hipError_t e0 = hipSuccess;
-        auto rec = [&](int k) { if (e0 == hipSuccess) e0 = hipEventRecord(ev[k], stream_); };
-        // one phase per kernel of the default path; the other path's kernel of the same
-        // role shares its phase
-        rec(0);
-        if (e0 == hipSuccess) e0 = launch_spread(ctx(PH_SPREADA), 0);
-        if (e0 == hipSuccess && ensure_claimed(*m, ids, err)) e0 = hipErrorUnknown;   // after a reorder
-        rec(1);
-        if (e0 == hipSuccess) e0 = launch_x_r2c(ctx(PH_XA), true, 0, nullptr, nullptr);
-        if (e0 == hipSuccess) e0 = launch_tile_deposit(ctx(PH_TILE), 0);
-        rec(2);
-        if (e0 == hipSuccess) e0 = launch_plane_r2c(ctx(PH_PLANEF));
-        if (e0 == hipSuccess) e0 = launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr);
-        rec(3);
-        if (e0 == hipSuccess) e0 = launch_last_fused(ctx(PH_LAST), 0);
-        rec(4);
-        if (e0 == hipSuccess) e0 = launch_mid_grad(ctx(PH_MIDY_G));
-        rec(5);
-        if (e0 == hipSuccess) e0 = launch_x_c2r(ctx(PH_C2R), true, 0, nullptr, nullptr);
-        rec(6);
-        if (e0 == hipSuccess) e0 = launch_particles(ctx(PH_PART), true, 0, 0, true);
-        rec(7);
-        if (e0 == hipSuccess) e0 = launch_advance(stream_, d_reps_, m->d_ids, m->nids, 1);
-        for (int id : ids) {   // amortised spatial reorders are timed with the step counter
+        auto ok = [&](hipError_t e) { if (e0 == hipSuccess) e0 = e; };
+        ok(launch_spread(ctx(PH_SPREADA, 0, 0), 0));
+        if (e0 == hipSuccess && ensure_claimed(*m, ids, err, ev[0][1][0], ev[0][1][1], &used[0][1]))   // after a reorder
+            e0 = hipErrorUnknown;
+        ok(launch_x_r2c(ctx(PH_XA, 1, 0), true, 0, nullptr, nullptr));
+        ok(launch_tile_deposit(ctx(PH_TILE, 1, 1), 0));
+        ok(launch_plane_r2c(ctx(PH_PLANEF, 2, 0)));
+        ok(launch_mid(ctx(PH_MIDY_H, 2, 1), 1, 0, 0, 0, nullptr));
+        ok(launch_last_fused(ctx(PH_LAST, 3, 0), 0));
+        ok(launch_mid_grad(ctx(PH_MIDY_G, 4, 0)));
+        ok(launch_x_c2r(ctx(PH_C2R, 5, 0), true, 0, nullptr, nullptr));
+        ok(launch_particles(ctx(PH_PART, 6, 0), true, 0, 0, true));
+        ok(launch_advance(stream_, d_reps_, m->d_ids, m->nids, 1, ev[7][0][0], ev[7][0][1]));
+        used[7][0] = m->nids > 0;
+        bool reordered = false;
+        for (int id : ids) {   // amortised spatial reorders
             Replica& R = *reps_[id];
-            if (e0 == hipSuccess && R.reorder_every > 0 && ++R.since_sort >= R.reorder_every && reorder(id, err, true))
-                e0 = hipErrorUnknown;
+            if (e0 == hipSuccess && R.reorder_every > 0 && ++R.since_sort >= R.reorder_every) {
+                if (!reordered) ok(hipEventRecord(rv[0], stream_));
+                reordered = true;
+                if (reorder(id, err, true)) e0 = hipErrorUnknown;
+            }
         }
-        rec(8);
-        if (e0 == hipSuccess) e0 = hipEventSynchronize(ev[kProfPhases]);
+        if (reordered) ok(hipEventRecord(rv[1], stream_));
+        ok(hipStreamSynchronize(stream_));
         if (e0 != hipSuccess) { err = std::string("profile: ") + hipGetErrorString(e0); rc = -TILD_ERR_DEVICE; break; }
-        for (int k = 0; k < kProfPhases; ++k) {
+        for (int k = 0; k < kKernelPhases; ++k)
+            for (int j = 0; j < kSlots; ++j) {
+                if (!used[k][j]) continue;
+                float t = 0.f;
+                if (hipEventElapsedTime(&t, ev[k][j][0], ev[k][j][1]) != hipSuccess) {
+                    err = "profile: kernel timing unavailable"; rc = -TILD_ERR_DEVICE; break;
+                }
+                acc[k] += t;
+            }
+        if (reordered) {
             float t = 0.f;
-            (void)hipEventElapsedTime(&t, ev[k], ev[k + 1]);
-            acc[k] += t;
+            (void)hipEventElapsedTime(&t, rv[0], rv[1]);
+            acc[kKernelPhases] += t;
         }
         for (int id : ids) reps_[id]->step += 1;
     }
     for (int id : ids) reps_[id]->h.step = reps_[id]->step;
-    for (int k = 0; k <= kProfPhases; ++k) (void)hipEventDestroy(ev[k]);
+    for (auto& a : ev) for (auto& b : a) for (auto& e : b) (void)hipEventDestroy(e);
+    for (auto& e : rv) (void)hipEventDestroy(e);
     for (int k = 0; k < kProfPhases; ++k) ms[k] = acc[k] / n;
     return rc;
 }

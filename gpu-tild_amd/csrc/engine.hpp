@@ -21,7 +21,7 @@ namespace tild {
 
 constexpr int kXcds = 8;         // accelerator complex dies of an MI355X
 constexpr int kGraphSteps = 10;  // integrating steps per multi-step graph launch
-constexpr int kProfPhases = 8;   // claim, deposit, forward plane, last_fused, y_inv, x_c2r, particles, advance
+constexpr int kProfPhases = 9;   // claim, deposit, forward plane, last_fused, y_inv, x_c2r, particles, advance, reorder
 
 // Work-map phases.  *A: replicas on the global-atomic spread; T*: replicas on the
 // LDS-tiled spread; XP: x-axis R2C of stored densities (properties).
@@ -127,7 +127,10 @@ public:
     int enqueue_steps(SetMaps& m, int nsteps, bool integrate, bool write_forces, std::string& err);
     // Claims the current step's records of every replica of the set that has none
     // (a replica whose claims are stale is cleared first).
-    int ensure_claimed(SetMaps& m, const std::vector<int>& ids, std::string& err);
+    // With t0/t1 the claim launch binds them to its start and stop; *launched says
+    // whether it ran.
+    int ensure_claimed(SetMaps& m, const std::vector<int>& ids, std::string& err,
+                       hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr, bool* launched = nullptr);
 
 private:
     int device_;

@@ -931,29 +931,29 @@ __global__ void k_virial_source(const RepDev* __restrict__ reps, int rep,
 
 hipError_t launch_spread(const LaunchCtx& c, int koff) {
     if (!c.n) return hipSuccess;
-    hipLaunchKernelGGL(k_spread, dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap, koff);
+    TILD_LAUNCH(c, k_spread, dim3(c.n), dim3(256), 0, c.reps, c.wmap, koff);
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t launch_x_r2c(const LaunchCtx& c, bool from_acc, int nrows_override, const float* in, float2* out) {
     if (!c.n) return hipSuccess;
-    if (from_acc) hipLaunchKernelGGL(k_x_r2c<true>, dim3(c.n), dim3(kFftThreads), c.lds, c.stream, c.reps, c.wmap, nrows_override, in, out);
-    else hipLaunchKernelGGL(k_x_r2c<false>, dim3(c.n), dim3(kFftThreads), c.lds, c.stream, c.reps, c.wmap, nrows_override, in, out);
+    if (from_acc) TILD_LAUNCH(c, k_x_r2c<true>, dim3(c.n), dim3(kFftThreads), c.lds, c.reps, c.wmap, nrows_override, in, out);
+    else TILD_LAUNCH(c, k_x_r2c<false>, dim3(c.n), dim3(kFftThreads), c.lds, c.reps, c.wmap, nrows_override, in, out);
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t launch_mid(const LaunchCtx& c, int axis, int use_gk, int nfb, int inverse, float2* buf) {
     if (!c.n) return hipSuccess;
-    hipLaunchKernelGGL(k_mid, dim3(c.n), dim3(kFftThreads), c.lds, c.stream, c.reps, c.wmap, axis, use_gk, nfb, inverse, buf);
+    TILD_LAUNCH(c, k_mid, dim3(c.n), dim3(kFftThreads), c.lds, c.reps, c.wmap, axis, use_gk, nfb, inverse, buf);
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t launch_last_fused(const LaunchCtx& c, int store_spectrum) {
     if (!c.n) return hipSuccess;
-    hipLaunchKernelGGL(k_last_fused, dim3(c.n), dim3(kFftThreads), c.lds, c.stream, c.reps, c.wmap, store_spectrum);
+    TILD_LAUNCH(c, k_last_fused, dim3(c.n), dim3(kFftThreads), c.lds, c.reps, c.wmap, store_spectrum);
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t launch_x_c2r(const LaunchCtx& c, bool records, int nlines, const float2* in, float* out) {
     if (!c.n) return hipSuccess;
-    if (records) hipLaunchKernelGGL(k_x_c2r<true>, dim3(c.n), dim3(kFftThreads), c.lds, c.stream, c.reps, c.wmap, nlines, in, out);
-    else hipLaunchKernelGGL(k_x_c2r<false>, dim3(c.n), dim3(kFftThreads), c.lds, c.stream, c.reps, c.wmap, nlines, in, out);
+    if (records) TILD_LAUNCH(c, k_x_c2r<true>, dim3(c.n), dim3(kFftThreads), c.lds, c.reps, c.wmap, nlines, in, out);
+    else TILD_LAUNCH(c, k_x_c2r<false>, dim3(c.n), dim3(kFftThreads), c.lds, c.reps, c.wmap, nlines, in, out);
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t configure_kernels() {
@@ -964,22 +964,22 @@ hipError_t configure_kernels() {
 }
 hipError_t launch_plane_r2c(const LaunchCtx& c) {
     if (!c.n) return hipSuccess;
-    hipLaunchKernelGGL(k_plane_r2c<kPlaneThreads>, dim3(c.n), dim3(kPlaneThreads), c.lds, c.stream, c.reps, c.wmap);
+    TILD_LAUNCH(c, k_plane_r2c<kPlaneThreads>, dim3(c.n), dim3(kPlaneThreads), c.lds, c.reps, c.wmap);
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t launch_mid_grad(const LaunchCtx& c) {
     if (!c.n) return hipSuccess;
-    hipLaunchKernelGGL(k_mid_grad, dim3(c.n), dim3(kFftThreads), c.lds, c.stream, c.reps, c.wmap);
+    TILD_LAUNCH(c, k_mid_grad, dim3(c.n), dim3(kFftThreads), c.lds, c.reps, c.wmap);
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t launch_particles(const LaunchCtx& c, bool integrate, int write_forces, int koff, bool claim) {
     if (!c.n) return hipSuccess;
     if (integrate && claim)
-        hipLaunchKernelGGL((k_particles<true, true>), dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap, write_forces, koff);
+        TILD_LAUNCH(c, (k_particles<true, true>), dim3(c.n), dim3(256), 0, c.reps, c.wmap, write_forces, koff);
     else if (integrate)
-        hipLaunchKernelGGL((k_particles<true, false>), dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap, write_forces, koff);
+        TILD_LAUNCH(c, (k_particles<true, false>), dim3(c.n), dim3(256), 0, c.reps, c.wmap, write_forces, koff);
     else
-        hipLaunchKernelGGL((k_particles<false, false>), dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap, write_forces, koff);
+        TILD_LAUNCH(c, (k_particles<false, false>), dim3(c.n), dim3(256), 0, c.reps, c.wmap, write_forces, koff);
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t launch_kprops(const LaunchCtx& c, int nq_max) {
@@ -997,9 +997,10 @@ hipError_t launch_reduce_rows(hipStream_t s, const double* in, int nrows, int n,
     hipLaunchKernelGGL(k_reduce_rows, dim3(nrows), dim3(256), 0, s, in, n, out);
     CHECK_LAUNCH(); return hipSuccess;
 }
-hipError_t launch_advance(hipStream_t s, RepDev* reps, const int* which, int n, int nsteps) {
+hipError_t launch_advance(hipStream_t s, RepDev* reps, const int* which, int n, int nsteps, hipEvent_t t0, hipEvent_t t1) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_advance, dim3((n + 255) / 256), dim3(256), 0, s, reps, which, n, nsteps);
+    const LaunchCtx c{nullptr, nullptr, 0, 0, s, t0, t1};
+    TILD_LAUNCH(c, k_advance, dim3((n + 255) / 256), dim3(256), 0, reps, which, n, nsteps);
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t launch_bins(hipStream_t s, const RepDev* reps, int rep, int ns, int* out) {

@@ -3,6 +3,8 @@
 #pragma once
 #include "device.hpp"
 
+#include <hip/hip_ext.h>
+
 namespace tild {
 
 // Largest batch of complex values one FFT block transforms at once (two LDS
@@ -59,7 +61,17 @@ struct LaunchCtx {
     int n;              // blocks
     size_t lds;         // dynamic LDS bytes
     hipStream_t stream;
+    // step profile: events that take the kernel's own start and stop times
+    hipEvent_t t0 = nullptr, t1 = nullptr;
 };
+
+// Launch on c.stream; with c.t0 set, through hipExtLaunchKernel, whose events take
+// the start and stop times of the kernel itself.
+#define TILD_LAUNCH(c, kernel, grid, block, lds, ...)                                                   \
+    do {                                                                                               \
+        if ((c).t0) hipExtLaunchKernelGGL(kernel, grid, block, lds, (c).stream, (c).t0, (c).t1, 0, __VA_ARGS__); \
+        else hipLaunchKernelGGL(kernel, grid, block, lds, (c).stream, __VA_ARGS__);                   \
+    } while (0)
 
 hipError_t launch_spread(const LaunchCtx& c, int koff);
 hipError_t launch_x_r2c(const LaunchCtx& c, bool from_acc, int nrows_override, const float* in, float2* out);
@@ -68,7 +80,8 @@ hipError_t launch_last_fused(const LaunchCtx& c, int store_spectrum);
 hipError_t launch_x_c2r(const LaunchCtx& c, bool records, int nlines, const float2* in, float* out);
 // claim: integrating steps of LDS-tiled replicas also claim the next step's records
 hipError_t launch_particles(const LaunchCtx& c, bool integrate, int write_forces, int koff, bool claim);
-hipError_t launch_advance(hipStream_t s, RepDev* reps, const int* which, int n, int nsteps);
+hipError_t launch_advance(hipStream_t s, RepDev* reps, const int* which, int n, int nsteps,
+                          hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 constexpr int kPlaneThreads = 512;                      // block of the plane forward pass
 constexpr size_t kPlaneLds = 163840;                    // LDS one block may use (MI355X)
 hipError_t launch_mid_grad(const LaunchCtx& c);

@@ -367,7 +367,7 @@ hipError_t launch_tile_scan(const LaunchCtx& c) {
 }
 hipError_t launch_tile_claim(const LaunchCtx& c, int koff) {
     if (!c.n) return hipSuccess;
-    hipLaunchKernelGGL(k_tile_claim, dim3(c.n), dim3(256), 0, c.stream, c.reps, c.wmap, koff);
+    TILD_LAUNCH(c, k_tile_claim, dim3(c.n), dim3(256), 0, c.reps, c.wmap, koff);
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t launch_tile_reset(const LaunchCtx& c, int koff) {
@@ -381,7 +381,7 @@ hipError_t launch_tile_regions(hipStream_t s, const RepDev* reps, int rep) {
 }
 hipError_t launch_tile_deposit(const LaunchCtx& c, int koff) {
     if (!c.n) return hipSuccess;
-    hipLaunchKernelGGL(k_tile_deposit, dim3(c.n), dim3(kFftThreads), c.lds, c.stream, c.reps, c.wmap, koff);
+    TILD_LAUNCH(c, k_tile_deposit, dim3(c.n), dim3(kFftThreads), c.lds, c.reps, c.wmap, koff);
     CHECK_LAUNCH(); return hipSuccess;
 }
 

@@ -299,7 +299,7 @@ class Engine:
         self._chk(lib().tild_engine_get_bins(self._h, _ip(a), a.size), "get_bins")
         return a
 
-    PHASES = ["claim", "deposit", "plane_r2c", "last_fused", "y_ifft", "x_c2r", "particles", "advance"]
+    PHASES = ["claim", "deposit", "plane_r2c", "last_fused", "y_ifft", "x_c2r", "particles", "advance", "reorder"]
 
     def profile(self, nsteps):
         n = len(self.PHASES)

@@ -150,13 +150,14 @@ tild_status tild_engine_get_grid_field(tild_engine* e, int type, int dir, float*
 /* First stencil node of every coordinate, ns*Dim ints: the grid bin the reference
    computes from FP32 x / dx (device_grid_utils.cu:136-146, :153-226), wrapped. */
 tild_status tild_engine_get_bins(tild_engine* e, int* base, size_t n);
-/* nsteps of tild_engine_step with a HIP event pair around every kernel launch on the
-   engine's stream; phase_ms[k] receives the mean duration of phase k (nphase >= 8),
-   one kernel each: 0 bucket claims and records (global-atomic spread on that path),
+/* nsteps of tild_engine_step, each kernel launched with a HIP event pair bound to its
+   own start and stop (hipExtLaunchKernel) on the engine's stream; phase_ms[k] receives
+   the mean per-step duration of phase k (nphase >= 9), one kernel each: 0 bucket-claim
+   pass (only where a run starts without claims; global-atomic spread on that path),
    1 tile deposit (accumulator conversion + x R2C on the atomic path), 2 forward plane
    pass (y FFT without it), 3 fused last-axis k-space pass, 4 inverse y-axis pass,
-   5 x-axis C2R, 6 gather+bonds+integrate, 7 step counter and the amortised spatial
-   reorder. */
+   5 x-axis C2R, 6 gather+bonds+integrate+next-step claims, 7 step counter; 8 the
+   spatial reorders (an event pair around them), amortised over the steps. */
 tild_status tild_engine_profile(tild_engine* e, int nsteps, double* phase_ms, int nphase);
 tild_status tild_engine_get_step(tild_engine* e, int* step);
 

@@ -341,3 +341,28 @@ def test_reordered_runs_are_bitwise_reproducible(tmp_path):
     for k in ("Upe", "Ubond"):
         assert runs[0][1][k] == runs[1][1][k]
     assert np.array_equal(runs[0][1]["Ptens"], runs[1][1]["Ptens"])
+
+
+def test_profiled_steps_are_ordinary_steps(tmp_path):
+    # the step profile launches the same kernels with timing events bound to each launch:
+    # its steps land where ordinary steps do, bit for bit, and every kernel phase of the
+    # tiled path reports a duration
+    text, d = decks.melt3d(tmp_path, integrator="GJF", delt=0.01)
+    s = tild.System(text, d)
+    ea = tild.Engine(s, device=0, seed=9, reorder_every=3)
+    eb = tild.Engine(s, device=0, seed=9, reorder_every=3)
+    ea.step(2)
+    eb.step(2)
+    prof = ea.profile(7)
+    eb.step(7)
+    assert list(prof) == tild.Engine.PHASES and len(prof) == 9
+    for k in ("deposit", "last_fused", "x_c2r", "particles", "advance"):
+        assert prof[k] > 0.0, (k, prof)
+    assert prof["claim"] > 0.0 and prof["reorder"] > 0.0   # reorders at steps 3, 6, 9
+    assert np.array_equal(ea.positions(), eb.positions())
+    ea.forces()
+    eb.forces()
+    assert np.array_equal(ea.get_forces(), eb.get_forces())
+    print("profile: " + ", ".join(f"{k} {v * 1e3:.1f} us" for k, v in prof.items()))
+    ea.close()
+    eb.close()
