@@ -574,6 +574,47 @@ __device__ __forceinline__ void gather_sum(const RepDev& r, int P, const float (
     }
 }
 
+// The unrolled gather in two halves, so that other work can run while the records are
+// in flight: the loads (records in the order gather_sum adds them) ...
+template <int D, int NP, bool OFF32>
+__device__ __forceinline__ void gather_load(const RepDev& r, const int (&n0)[3], size_t fb,
+                                            float3 (&g)[(D == 3 ? NP : 1) * NP * NP]) {
+    int zz[NP], yy[NP], xx[NP];
+    zz[0] = n0[2]; yy[0] = n0[1]; xx[0] = n0[0];
+#pragma unroll
+    for (int k = 1; k < NP; ++k) {
+        zz[k] = wrap_next(zz[k - 1], r.Nz);
+        yy[k] = wrap_next(yy[k - 1], r.Ny);
+        xx[k] = wrap_next(xx[k - 1], r.Nx);
+    }
+    constexpr int nzl = D == 3 ? NP : 1;
+#pragma unroll
+    for (int iz = 0; iz < nzl; ++iz)
+#pragma unroll
+        for (int iy = 0; iy < NP; ++iy) {
+            const int rowb = (zz[iz] * r.Ny + yy[iy]) * r.Nx;
+#pragma unroll
+            for (int ix = 0; ix < NP; ++ix) g[(iz * NP + iy) * NP + ix] = gather_rec<NP, OFF32>(r.G3, fb, rowb + xx[ix]);
+        }
+}
+
+// ... and the weighted sum, in gather_sum's order (bit-identical to it)
+template <int D, int NP>
+__device__ __forceinline__ void gather_acc(const float (&W)[3][5], const float3 (&g)[(D == 3 ? NP : 1) * NP * NP],
+                                           float (&acc)[3]) {
+    constexpr int nzl = D == 3 ? NP : 1;
+#pragma unroll
+    for (int iz = 0; iz < nzl; ++iz)
+#pragma unroll
+        for (int iy = 0; iy < NP; ++iy)
+#pragma unroll
+            for (int ix = 0; ix < NP; ++ix) {
+                const float w = D == 3 ? W[0][ix] * W[1][iy] * W[2][iz] : W[0][ix] * W[1][iy];
+                const float3 v = g[(iz * NP + iy) * NP + ix];
+                acc[0] += w * v.x; acc[1] += w * v.y; acc[2] += w * v.z;
+            }
+}
+
 // Gathered force of one bead: its type's records, plus q times the charge records
 // (a second sweep, so that no more than one stencil of records is in flight).
 template <int D, int NP, bool OFF32>
@@ -634,16 +675,6 @@ __device__ __forceinline__ void particle_bead(const RepDev& r, int i, int write_
     const float qi = charged ? G(r.q)[i] : 0.f;
     const int3 bj = gld3(r.bond_j + (size_t)i * 3);
     const int3 btv = gld3(r.bond_t + (size_t)i * 3);
-    // stencil
-    float W[3][5];
-    int n0[3] = {0, 0, 0};
-    W[2][0] = 1.f;
-    n0[0] = stencil_1d(xi[0], r.dx[0], r.idx[0], r.Nx, P, W[0]);
-    n0[1] = stencil_1d(xi[1], r.dx[1], r.idx[1], r.Ny, P, W[1]);
-    if constexpr (D == 3) n0[2] = stencil_1d(xi[2], r.dx[2], r.idx[2], r.Nz, P, W[2]);
-    // gather
-    float f[3] = {0.f, 0.f, 0.f};
-    gather_nodes<D, NP, OFF32>(r, P, W, n0, t, qi, charged, f);
     // bonds, every bond evaluated from both ends (no atomics).  Slots fill from the
     // front and the first empty one ends the list; an empty slot loads the bead itself
     // and bond type 0 (both exist) and contributes nothing, so the loads of all three
@@ -668,8 +699,6 @@ __device__ __forceinline__ void particle_bead(const RepDev& r, int i, int write_
     // integrator inputs go out with the partners
     int kind = 0;
     float xo[3] = {0.f, 0.f, 0.f}, ga = 0.f, gc1 = 0.f, gc2 = 0.f, gnm = 0.f;
-    float z[4] = {0.f, 0.f, 0.f, 0.f}, zo[4] = {0.f, 0.f, 0.f, 0.f};
-    const uint32_t stepc = (uint32_t)(step + 1);
     if constexpr (INTEGRATE) {
         kind = G(r.ikind)[i];
         if (r.any_gjf) {
@@ -679,8 +708,40 @@ __device__ __forceinline__ void particle_bead(const RepDev& r, int i, int write_
             gc2 = r.gjf_c2[t];
             gnm = r.gjf_noise[t];
         }
-        normals4(r.seed, site, stepc, z);
-        if (r.any_gjf && stepc > 1u) normals4(r.seed, site, stepc - 1u, zo);
+    }
+    // stencil
+    float W[3][5];
+    int n0[3] = {0, 0, 0};
+    W[2][0] = 1.f;
+    n0[0] = stencil_1d(xi[0], r.dx[0], r.idx[0], r.Nx, P, W[0]);
+    n0[1] = stencil_1d(xi[1], r.dx[1], r.idx[1], r.Ny, P, W[1]);
+    if constexpr (D == 3) n0[2] = stencil_1d(xi[2], r.dx[2], r.idx[2], r.Nz, P, W[2]);
+    // gather; with the unrolled stencil the noise is computed while the type's records
+    // are in flight
+    float f[3] = {0.f, 0.f, 0.f};
+    float z[4] = {0.f, 0.f, 0.f, 0.f}, zo[4] = {0.f, 0.f, 0.f, 0.f};
+    const uint32_t stepc = (uint32_t)(step + 1);
+    if constexpr (NP > 0) {
+        float3 g[(D == 3 ? NP : 1) * NP * NP];
+        gather_load<D, NP, OFF32>(r, n0, (size_t)t * r.M, g);
+        if constexpr (INTEGRATE) {
+            normals4(r.seed, site, stepc, z);
+            if (r.any_gjf && stepc > 1u) normals4(r.seed, site, stepc - 1u, zo);
+        }
+        gather_acc<D, NP>(W, g, f);
+        if (charged) {
+            __builtin_amdgcn_sched_barrier(0);
+            float fc[3] = {0.f, 0.f, 0.f};
+            gather_sum<D, NP, OFF32>(r, P, W, n0, (size_t)r.ntypes * r.M, fc);
+#pragma unroll
+            for (int d = 0; d < D; ++d) f[d] += qi * fc[d];
+        }
+    } else {
+        gather_nodes<D, NP, OFF32>(r, P, W, n0, t, qi, charged, f);
+        if constexpr (INTEGRATE) {
+            normals4(r.seed, site, stepc, z);
+            if (r.any_gjf && stepc > 1u) normals4(r.seed, site, stepc - 1u, zo);
+        }
     }
     float lf[3] = {0.f, 0.f, 0.f};
 #pragma unroll
@@ -729,7 +790,7 @@ __device__ __forceinline__ void particle_bead(const RepDev& r, int i, int write_
 // step's bucket slots from the positions it has just written, which takes the bucket
 // claim pass out of every step but the first of a run.
 template <bool INTEGRATE, bool CLAIM>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_particles(const RepDev* __restrict__ reps,
+__global__ void __launch_bounds__(256) k_particles(const RepDev* __restrict__ reps,
                                                    const WorkItem* __restrict__ wmap,
                                                    int write_forces, int koff) {
     const WorkItem wi = wmap[blockIdx.x];

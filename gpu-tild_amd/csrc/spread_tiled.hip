@@ -246,8 +246,13 @@ __device__ __forceinline__ void deposit_record_any(const RepDev& r, const float4
     }
 }
 
-// Deposit one tile's rows and transform them along x.
-__global__ void __launch_bounds__(kFftThreads) k_tile_deposit(const RepDev* __restrict__ reps,
+// Deposit one tile's rows and transform them along x.  512 threads: a tile has about
+// 1,000 beads plus its halo share, and three blocks of 512 fit a CU (with 1,024
+// threads only one fits, and the deposit took 22.8 us instead of 17.5 on the melt;
+// with 256, 19.6 us).
+constexpr int kDepositThreads = 512;
+
+__global__ void __launch_bounds__(kDepositThreads) k_tile_deposit(const RepDev* __restrict__ reps,
                                                               const WorkItem* __restrict__ wmap, int koff) {
     extern __shared__ unsigned long long tile_lds[];
     const WorkItem wi = wmap[blockIdx.x];
@@ -264,7 +269,7 @@ __global__ void __launch_bounds__(kFftThreads) k_tile_deposit(const RepDev* __re
     // tile's for the next one
     if (threadIdx.x < 4) G(claim_counters(r, par ^ 1))[(t * 4 + threadIdx.x) << r.tshift] = 0;
     if (t == 0 && threadIdx.x == 0) G(r.tovf)[(par ^ 1) * 32] = 0;
-    for (int k = threadIdx.x; k < rows * Nx; k += kFftThreads) patch[k] = 0ull;
+    for (int k = threadIdx.x; k < rows * Nx; k += kDepositThreads) patch[k] = 0ull;
     __syncthreads();
     // source buckets: own tile (all classes) and the lower neighbours' halo classes;
     // each bucket's records are the filled part of its region
@@ -291,7 +296,7 @@ __global__ void __launch_bounds__(kFftThreads) k_tile_deposit(const RepDev* __re
         shift[s] = base[bb] - total;
         total += n;
     }
-    for (int j = threadIdx.x; j < total; j += kFftThreads) {
+    for (int j = threadIdx.x; j < total; j += kDepositThreads) {
         int sh = shift[0];
 #pragma unroll
         for (int s = 1; s < 9; ++s) sh = j >= start[s] ? shift[s] : sh;
@@ -303,7 +308,7 @@ __global__ void __launch_bounds__(kFftThreads) k_tile_deposit(const RepDev* __re
     }
     // beads whose region was full: the same bucket test over the overflow list
     const int novf = G((const int*)r.tovf)[par * 32];
-    for (int j = threadIdx.x; j < novf; j += kFftThreads) {
+    for (int j = threadIdx.x; j < novf; j += kDepositThreads) {
         const int bb = G((const int*)r.tovf_b)[j];
         bool mine = false;
         for (int s = 0; s < nbk; ++s) mine |= bb == bk[s];
@@ -318,7 +323,7 @@ __global__ void __launch_bounds__(kFftThreads) k_tile_deposit(const RepDev* __re
     const int npairs = (rows + 1) / 2;
     const int ldb = npairs | 1;
     const FastDiv dNx(Nx), dNxh(Nxh), dBy(By), dBz(Bz);
-    for (int task = threadIdx.x; task < npairs * Nx; task += kFftThreads) {
+    for (int task = threadIdx.x; task < npairs * Nx; task += kDepositThreads) {
         int n;
         const int j = dNx.div(task, n);
         float v[2] = {0.f, 0.f};
@@ -335,8 +340,8 @@ __global__ void __launch_bounds__(kFftThreads) k_tile_deposit(const RepDev* __re
     }
     if (r.plane_fwd) return;   // the plane pass transforms the densities
     __syncthreads();
-    const float2* res = lds_fft(fbuf, fbuf + (size_t)Nx * ldb, ldb, npairs, r.ax[0]);
-    for (int task = threadIdx.x; task < npairs * Nxh; task += kFftThreads) {
+    const float2* res = lds_fft<kDepositThreads>(fbuf, fbuf + (size_t)Nx * ldb, ldb, npairs, r.ax[0]);
+    for (int task = threadIdx.x; task < npairs * Nxh; task += kDepositThreads) {
         int k;
         const int j = dNxh.div(task, k);
         const float2 Zk = res[k * ldb + j];
@@ -381,7 +386,7 @@ hipError_t launch_tile_regions(hipStream_t s, const RepDev* reps, int rep) {
 }
 hipError_t launch_tile_deposit(const LaunchCtx& c, int koff) {
     if (!c.n) return hipSuccess;
-    TILD_LAUNCH(c, k_tile_deposit, dim3(c.n), dim3(kFftThreads), c.lds, c.reps, c.wmap, koff);
+    TILD_LAUNCH(c, k_tile_deposit, dim3(c.n), dim3(kDepositThreads), c.lds, c.reps, c.wmap, koff);
     CHECK_LAUNCH(); return hipSuccess;
 }
 
