@@ -165,15 +165,15 @@ __global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_e
 // k-space coupling of one block's nodes for TT types (the same sums, in the same order,
 // as the general loop of k_last_fused): w_I = sum_J C_IJ rho_J with the symmetric table
 // rows pidx(lo, hi) = lo*TT - lo*(lo-1)/2 + (hi - lo), and the charge row.
-template <int TT>
-__device__ __forceinline__ void couple_nodes(gmem<const float> cb, int nrows, int nlc, int nb, int N, int ldb, int nf,
-                                             int ctab_charge, const FastDiv& dnb, const float2* A, float2* Wb) {
+template <int TT, int NT>
+__device__ __forceinline__ void couple_nodes(gmem<const float> cb, int nrows, int nlc, int nb, int N, int ldb, int ldw,
+                                             int nf, int ctab_charge, const FastDiv& dnb, const float2* A, float2* Wb) {
     constexpr int NR = TT * (TT + 1) / 2;
     const bool charged = nf > TT;
-    for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
+    for (int task = threadIdx.x; task < nb * N; task += NT) {
         int b;
         const int n = dnb.div(task, b);
-        const int row = n * ldb + b;
+        const int row = n * ldb + b, wrow = n * ldw + b;
         gmem<const float> cn = cb + (size_t)n * nrows * nlc + b;
         float c[NR];
 #pragma unroll
@@ -191,11 +191,11 @@ __device__ __forceinline__ void couple_nodes(gmem<const float> cb, int nrows, in
                 const float cc = c[lo * TT - lo * (lo - 1) / 2 + (hi - lo)];
                 acc.x += cc * rh[J].x; acc.y += cc * rh[J].y;
             }
-            Wb[row + I * nb] = acc;
+            Wb[wrow + I * nb] = acc;
         }
         if (charged) {
             const float2 rq = A[row + TT * nb];
-            Wb[row + TT * nb] = make_float2(cq * rq.x, cq * rq.y);
+            Wb[wrow + TT * nb] = make_float2(cq * rq.x, cq * rq.y);
         }
     }
 }
@@ -210,18 +210,8 @@ __device__ __forceinline__ void couple_nodes(gmem<const float> cb, int nrows, in
 // transform: along the last axis la, w and -i k_la w go to Gk[f*dim] and
 // Gk[f*dim + la]; the factors of the other directions are constant along those lines
 // and are applied by the passes of their own axes.
-
-// Last-axis forward FFT of every field, the k-space coupling
-//   w_I(k) = sum_J C_IJ(k) rho_J(k)       (C includes 1/M and counts each style in
-//                                          both orientations, pair_style.cu:81-196)
-//   w_c(k) = G_c(k) rho_c(k)              (smeared charges, double smearing,
-//                                          device_array_utils.cu:199-241)
-// then the gradient spectra -i k_d w(k) (k_d = 0 on the d-Nyquist plane, where the
-// reference's term is anti-Hermitian and drops out of Re IFFT) start their inverse
-// transform: along the last axis la, w and -i k_la w go to Gk[f*dim] and
-// Gk[f*dim + la]; the factors of the other directions are constant along those lines
-// and are applied by the passes of their own axes.
-__global__ void __launch_bounds__(kFftThreads) k_last_fused(const RepDev* __restrict__ reps,
+template <int NT>
+__global__ void __launch_bounds__(NT) k_last_fused(const RepDev* __restrict__ reps,
                                                             const WorkItem* __restrict__ wmap,
                                                             int store_spectrum) {
     extern __shared__ float2 lds[];
@@ -234,37 +224,38 @@ __global__ void __launch_bounds__(kFftThreads) k_last_fused(const RepDev* __rest
     const int q0 = wi.blk * r.nl_last;
     const int nb = min(r.nl_last, Q - q0);
     const int nlines = nf * nb;
-    const int ldb = nlines | 1;                          // forward / coupled lines
+    const int ldb = nlines | 1;                          // forward lines
+    const int ldw = nlines;                              // coupled lines (unpadded)
     const int D = r.dim, DB = r.last_dirs;               // inverse outputs per batch (1 or 2)
     const int ldg = (nlines * DB) | 1;                   // inverse lines of one batch
-    // LDS: Wb [N x ldb] coupled fields, then two [N x ldg] buffers that first hold the
+    // LDS: Wb [N x ldw] coupled fields, then two [N x ldg] buffers that first hold the
     // forward transform's ping-pong pair and then the batched inverse transform's
     float2* Wb = lds;
-    float2* E0 = lds + (size_t)N * ldb;
+    float2* E0 = lds + (size_t)N * ldw;
     float2* E1 = E0 + (size_t)N * ldg;
     const FastDiv dnb(nb), dnf(nf);
-    for (int task = threadIdx.x; task < nlines * N; task += kFftThreads) {
+    for (int task = threadIdx.x; task < nlines * N; task += NT) {
         int b, f;
         const int rest = dnb.div(task, b), n = dnf.div(rest, f);
         E0[n * ldb + f * nb + b] = gld(r.H + (size_t)f * r.Mh + (size_t)n * Q + q0 + b);
     }
     __syncthreads();
-    const float2* A = lds_fft(E0, E1, ldb, nlines, r.ax[la]);   // spectra rho_f(k)
+    const float2* A = lds_fft<NT>(E0, E1, ldb, nlines, r.ax[la]);   // spectra rho_f(k)
     // coupling w_I = sum_J C_IJ rho_J (C symmetric, rows I <= J), charge w_c = G_c rho_c;
     // this block's table rows are one contiguous run
     const int nrows = r.ctab_rows, nlc = r.nl_last;
     gmem<const float> cb = G(r.Ctab) + (size_t)wi.blk * N * nrows * nlc;
     switch (T) {   // up to four types: unrolled, every table row of a node requested at once
-        case 1: couple_nodes<1>(cb, nrows, nlc, nb, N, ldb, nf, r.ctab_charge, dnb, A, Wb); break;
-        case 2: couple_nodes<2>(cb, nrows, nlc, nb, N, ldb, nf, r.ctab_charge, dnb, A, Wb); break;
-        case 3: couple_nodes<3>(cb, nrows, nlc, nb, N, ldb, nf, r.ctab_charge, dnb, A, Wb); break;
-        case 4: couple_nodes<4>(cb, nrows, nlc, nb, N, ldb, nf, r.ctab_charge, dnb, A, Wb); break;
+        case 1: couple_nodes<1, NT>(cb, nrows, nlc, nb, N, ldb, ldw, nf, r.ctab_charge, dnb, A, Wb); break;
+        case 2: couple_nodes<2, NT>(cb, nrows, nlc, nb, N, ldb, ldw, nf, r.ctab_charge, dnb, A, Wb); break;
+        case 3: couple_nodes<3, NT>(cb, nrows, nlc, nb, N, ldb, ldw, nf, r.ctab_charge, dnb, A, Wb); break;
+        case 4: couple_nodes<4, NT>(cb, nrows, nlc, nb, N, ldb, ldw, nf, r.ctab_charge, dnb, A, Wb); break;
         default: break;
     }
-    for (int task = threadIdx.x; T > 4 && task < nb * N; task += kFftThreads) {
+    for (int task = threadIdx.x; T > 4 && task < nb * N; task += NT) {
         int b;
         const int n = dnb.div(task, b);
-        const int row = n * ldb + b;
+        const int row = n * ldb + b, wrow = n * ldw + b;
         for (int I = 0; I < T; ++I) {
             float2 acc = make_float2(0.f, 0.f);
             for (int J = 0; J < T; ++J) {
@@ -274,12 +265,12 @@ __global__ void __launch_bounds__(kFftThreads) k_last_fused(const RepDev* __rest
                 const float2 rh = A[row + J * nb];
                 acc.x += c * rh.x; acc.y += c * rh.y;
             }
-            Wb[row + I * nb] = acc;
+            Wb[wrow + I * nb] = acc;
         }
         if (nf > T) {
             const float c = cb[((size_t)n * nrows + r.ctab_charge) * nlc + b];
             const float2 rh = A[row + T * nb];
-            Wb[row + T * nb] = make_float2(c * rh.x, c * rh.y);
+            Wb[wrow + T * nb] = make_float2(c * rh.x, c * rh.y);
         }
     }
     __syncthreads();
@@ -290,14 +281,14 @@ __global__ void __launch_bounds__(kFftThreads) k_last_fused(const RepDev* __rest
     const int slot1 = la;                                // Gk slot of the -i k_la w line
     for (int o0 = 0; o0 < 2; o0 += DB) {
         const int no = min(DB, 2 - o0), nlg = nlines * no;
-        for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
+        for (int task = threadIdx.x; task < nb * N; task += NT) {
             int b;
             const int n = dnb.div(task, b);
             const float kl = 2 * n == N ? 0.f : r.kfac[la] * (float)(2 * n < N ? n : n - N);
             for (int oo = 0; oo < no; ++oo) {
                 const float kd = o0 + oo == 0 ? 1.f : kl;
                 for (int f = 0; f < nf; ++f) {
-                    const float2 w = Wb[n * ldb + f * nb + b];
+                    const float2 w = Wb[n * ldw + f * nb + b];
                     // the inverse is taken as conj(FFT(conj)): conj(w) = (w.x, -w.y) and
                     // conj(-i k w) = (k w.y, k w.x)
                     E0[n * ldg + (f * no + oo) * nb + b] =
@@ -306,9 +297,9 @@ __global__ void __launch_bounds__(kFftThreads) k_last_fused(const RepDev* __rest
             }
         }
         __syncthreads();
-        const float2* res = lds_fft(E0, E1, ldg, nlg, r.ax[la]);
+        const float2* res = lds_fft<NT>(E0, E1, ldg, nlg, r.ax[la]);
         const FastDiv dfo(nf * no), dno(no);
-        for (int task = threadIdx.x; task < nlg * N; task += kFftThreads) {
+        for (int task = threadIdx.x; task < nlg * N; task += NT) {
             int b, fo, oo;
             const int rest = dnb.div(task, b), n = dfo.div(rest, fo);
             const int f = dno.div(fo, oo);
@@ -325,7 +316,8 @@ __global__ void __launch_bounds__(kFftThreads) k_last_fused(const RepDev* __rest
 // of U (the x derivative: -i k_x follows in the x pass), of -i k_y U and of V to slots
 // 0, 1 and 2.  One block: nb columns of one (field, z) plane, all three line sets in
 // one batch; it reads its columns of slots 0 and 2 before it writes any slot.
-__global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(8)))
+template <int NT>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8)))
 k_mid_grad(const RepDev* __restrict__ reps, const WorkItem* __restrict__ wmap) {
     extern __shared__ float2 lds[];
     const WorkItem wi = wmap[blockIdx.x];
@@ -342,7 +334,7 @@ k_mid_grad(const RepDev* __restrict__ reps, const WorkItem* __restrict__ wmap) {
     float2* Y = r.Gk + (size_t)(f * 3 + 1) * r.Mh + plane;
     float2* V = r.Gk + (size_t)(f * 3 + 2) * r.Mh + plane;
     const FastDiv dnb(nb);
-    for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
+    for (int task = threadIdx.x; task < nb * N; task += NT) {
         int b;
         const int n = dnb.div(task, b);
         const float ky = 2 * n == N ? 0.f : r.kfac[1] * (float)(2 * n < N ? n : n - N);
@@ -353,9 +345,9 @@ k_mid_grad(const RepDev* __restrict__ reps, const WorkItem* __restrict__ wmap) {
         lds[n * ldb + 2 * nb + b] = make_float2(v.x, -v.y);
     }
     __syncthreads();
-    const float2* res = lds_fft(lds, lds + (size_t)N * ldb, ldb, 3 * nb, r.ax[1]);
+    const float2* res = lds_fft<NT>(lds, lds + (size_t)N * ldb, ldb, 3 * nb, r.ax[1]);
     const FastDiv dnb3(3 * nb);
-    for (int task = threadIdx.x; task < 3 * nb * N; task += kFftThreads) {
+    for (int task = threadIdx.x; task < 3 * nb * N; task += NT) {
         int sb;
         const int n = dnb3.div(task, sb);
         int b;
@@ -372,8 +364,8 @@ k_mid_grad(const RepDev* __restrict__ reps, const WorkItem* __restrict__ wmap) {
 // dropped, which is the Hermitian projection the reference's Re(C2C inverse) takes.
 // RECORDS: output float4 (g_0, g_1, g_2, rho_f) per field and node (fields are lines
 // f*dim + d of Gk); otherwise plain real rows to out_override.
-template <bool RECORDS>
-__global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_eu(8))) k_x_c2r(const RepDev* __restrict__ reps,
+template <bool RECORDS, int NT>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) k_x_c2r(const RepDev* __restrict__ reps,
                                                        const WorkItem* __restrict__ wmap,
                                                        int nlines_override, const float2* __restrict__ in_override,
                                                        float* __restrict__ out_override) {
@@ -399,7 +391,7 @@ __global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_e
         for (int L = 0; L < nlines; L += r.dim) xlines |= 1ull << L;
     // each half-spectrum value is read once and fills column kk and its Hermitian
     // partner N - kk
-    for (int task = threadIdx.x; task < nb * Nxh; task += kFftThreads) {
+    for (int task = threadIdx.x; task < nb * Nxh; task += NT) {
         int kk, j;
         const int b = dNxh.div(task, kk);
         const int row = row0 + dnc.div(b, j);
@@ -419,11 +411,11 @@ __global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_e
         if (!self && N - kk >= Nxh) lds[(N - kk) * ldb + b] = make_float2(A.x + Bv.y, A.y - Bv.x);
     }
     __syncthreads();
-    const float2* res = lds_fft(lds, lds + (size_t)N * ldb, ldb, nb, r.ax[0]);
+    const float2* res = lds_fft<NT>(lds, lds + (size_t)N * ldb, ldb, nb, r.ax[0]);
     if (RECORDS) {
         const int D = r.dim;
         const FastDiv dnf(r.nf);
-        for (int task = threadIdx.x; task < rows * r.nf * N; task += kFftThreads) {
+        for (int task = threadIdx.x; task < rows * r.nf * N; task += NT) {
             int x, f;
             const int rest = dN.div(task, x), rr = dnf.div(rest, f);
             const int row = row0 + rr;
@@ -449,7 +441,7 @@ __global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_e
         }
     } else {
         const FastDiv dnl(nlines);
-        for (int task = threadIdx.x; task < rows * nlines * N; task += kFftThreads) {
+        for (int task = threadIdx.x; task < rows * nlines * N; task += NT) {
             int x, line;
             const int rest = dN.div(task, x), rr = dnl.div(rest, line);
             const float2 z = res[x * ldb + rr * nc + line / 2];
@@ -1008,17 +1000,17 @@ hipError_t launch_mid(const LaunchCtx& c, int axis, int use_gk, int nfb, int inv
 }
 hipError_t launch_last_fused(const LaunchCtx& c, int store_spectrum) {
     if (!c.n) return hipSuccess;
-    TILD_LAUNCH(c, k_last_fused, dim3(c.n), dim3(kFftThreads), c.lds, c.reps, c.wmap, store_spectrum);
+    TILD_LAUNCH(c, k_last_fused<kLastThreads>, dim3(c.n), dim3(kLastThreads), c.lds, c.reps, c.wmap, store_spectrum);
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t launch_x_c2r(const LaunchCtx& c, bool records, int nlines, const float2* in, float* out) {
     if (!c.n) return hipSuccess;
-    if (records) TILD_LAUNCH(c, k_x_c2r<true>, dim3(c.n), dim3(kFftThreads), c.lds, c.reps, c.wmap, nlines, in, out);
-    else TILD_LAUNCH(c, k_x_c2r<false>, dim3(c.n), dim3(kFftThreads), c.lds, c.reps, c.wmap, nlines, in, out);
+    if (records) TILD_LAUNCH(c, (k_x_c2r<true, kC2rThreads>), dim3(c.n), dim3(kC2rThreads), c.lds, c.reps, c.wmap, nlines, in, out);
+    else TILD_LAUNCH(c, (k_x_c2r<false, kFftThreads>), dim3(c.n), dim3(kFftThreads), c.lds, c.reps, c.wmap, nlines, in, out);
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t configure_kernels() {
-    hipError_t e = hipFuncSetAttribute((const void*)k_last_fused, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPlaneLds);
+    hipError_t e = hipFuncSetAttribute((const void*)k_last_fused<kLastThreads>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPlaneLds);
     if (e != hipSuccess) return e;
     return hipFuncSetAttribute((const void*)k_plane_r2c<kPlaneThreads>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)kPlaneLds);
@@ -1030,7 +1022,7 @@ hipError_t launch_plane_r2c(const LaunchCtx& c) {
 }
 hipError_t launch_mid_grad(const LaunchCtx& c) {
     if (!c.n) return hipSuccess;
-    TILD_LAUNCH(c, k_mid_grad, dim3(c.n), dim3(kFftThreads), c.lds, c.reps, c.wmap);
+    TILD_LAUNCH(c, k_mid_grad<kGradThreads>, dim3(c.n), dim3(kGradThreads), c.lds, c.reps, c.wmap);
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t launch_particles(const LaunchCtx& c, bool integrate, int write_forces, int koff, bool claim) {

@@ -49,10 +49,10 @@ __host__ __device__ inline MidGeom mid_grad_geom(const RepDev& r) {
 // one FFT buffer of N x nlines complex values, and the ping-pong pair a pass needs
 inline size_t fft_buf_bytes(int N, int nlines) { return (size_t)N * (size_t)(nlines | 1) * sizeof(float2); }
 inline size_t lds_bytes(int N, int nlines) { return 2 * fft_buf_bytes(N, nlines); }
-// fused last-axis pass: coupled fields (nf*nl lines) + two buffers of nf*DB*nl lines
-// (DB inverse outputs per batch)
+// fused last-axis pass: coupled fields (nf*nl lines, unpadded: only read and written
+// element by element) + two buffers of nf*DB*nl lines (DB inverse outputs per batch)
 inline size_t last_lds_bytes(int N, int nf, int DB, int nl) {
-    return fft_buf_bytes(N, nf * nl) + 2 * fft_buf_bytes(N, nf * DB * nl);
+    return (size_t)N * nf * nl * sizeof(float2) + 2 * fft_buf_bytes(N, nf * DB * nl);
 }
 
 struct LaunchCtx {
@@ -83,6 +83,9 @@ hipError_t launch_particles(const LaunchCtx& c, bool integrate, int write_forces
 hipError_t launch_advance(hipStream_t s, RepDev* reps, const int* which, int n, int nsteps,
                           hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 constexpr int kPlaneThreads = 512;                      // block of the plane forward pass
+constexpr int kLastThreads = 256;                       // block of the fused last-axis pass
+constexpr int kGradThreads = 256;                       // block of the y inverse pass
+constexpr int kC2rThreads = 256;                        // block of the inverse x pass (records)
 constexpr size_t kPlaneLds = 163840;                    // LDS one block may use (MI355X)
 hipError_t launch_mid_grad(const LaunchCtx& c);
 hipError_t launch_plane_r2c(const LaunchCtx& c);
