@@ -254,6 +254,44 @@ struct FastDiv {
     __device__ __forceinline__ int div(int n, int& rem) const { return div_small(n, d, inv, rem); }
 };
 
+// Mixed-radix digits of a strided loop index: i = (hi * A + mid) * B + lo for
+// i = t, t + S, t + 2S, ...  One division pair at set-up; each step adds the digits of
+// S with carries (adds and selects), where dividing i anew costs two FastDivs.
+// Requires A, B >= 1; hi may grow past any bound (the loop tests it).
+struct Digits3 {
+    int lo, mid, hi;
+    int sl, sm, sh, A, B;
+    __device__ __forceinline__ Digits3(int t, int S, const FastDiv& dB, const FastDiv& dA) : A(dA.d), B(dB.d) {
+        hi = dA.div(dB.div(t, lo), mid);
+        sh = dA.div(dB.div(S, sl), sm);
+    }
+    __device__ __forceinline__ void step() {
+        lo += sl;
+        const bool c0 = lo >= B;
+        lo = c0 ? lo - B : lo;
+        mid += sm + (int)c0;
+        const bool c1 = mid >= A;
+        mid = c1 ? mid - A : mid;
+        hi += sh + (int)c1;
+    }
+};
+
+// Two-digit form of the same: i = hi * B + lo.
+struct Digits2 {
+    int lo, hi;
+    int sl, sh, B;
+    __device__ __forceinline__ Digits2(int t, int S, const FastDiv& dB) : B(dB.d) {
+        hi = dB.div(t, lo);
+        sh = dB.div(S, sl);
+    }
+    __device__ __forceinline__ void step() {
+        lo += sl;
+        const bool c = lo >= B;
+        lo = c ? lo - B : lo;
+        hi += sh + (int)c;
+    }
+};
+
 // Bin of one coordinate (device_grid_utils.cu:136-146).  The bin uses IEEE FP32
 // division exactly as the reference, so bins match bit for bit.
 __device__ __forceinline__ int stencil_bin(float x, float dx, int P) {

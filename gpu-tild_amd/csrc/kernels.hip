@@ -170,9 +170,9 @@ __device__ __forceinline__ void couple_nodes(gmem<const float> cb, int nrows, in
                                              int nf, int ctab_charge, const FastDiv& dnb, const float2* A, float2* Wb) {
     constexpr int NR = TT * (TT + 1) / 2;
     const bool charged = nf > TT;
-    for (int task = threadIdx.x; task < nb * N; task += NT) {
-        int b;
-        const int n = dnb.div(task, b);
+    Digits2 gc(threadIdx.x, NT, dnb);
+    for (int task = threadIdx.x; task < nb * N; task += NT, gc.step()) {
+        const int b = gc.lo, n = gc.hi;
         const int row = n * ldb + b, wrow = n * ldw + b;
         gmem<const float> cn = cb + (size_t)n * nrows * nlc + b;
         float c[NR];
@@ -234,9 +234,9 @@ __global__ void __launch_bounds__(NT) k_last_fused(const RepDev* __restrict__ re
     float2* E0 = lds + (size_t)N * ldw;
     float2* E1 = E0 + (size_t)N * ldg;
     const FastDiv dnb(nb), dnf(nf);
-    for (int task = threadIdx.x; task < nlines * N; task += NT) {
-        int b, f;
-        const int rest = dnb.div(task, b), n = dnf.div(rest, f);
+    Digits3 gl(threadIdx.x, NT, dnb, dnf);
+    for (int task = threadIdx.x; task < nlines * N; task += NT, gl.step()) {
+        const int b = gl.lo, f = gl.mid, n = gl.hi;
         E0[n * ldb + f * nb + b] = gld(r.H + (size_t)f * r.Mh + (size_t)n * Q + q0 + b);
     }
     __syncthreads();
@@ -281,9 +281,9 @@ __global__ void __launch_bounds__(NT) k_last_fused(const RepDev* __restrict__ re
     const int slot1 = la;                                // Gk slot of the -i k_la w line
     for (int o0 = 0; o0 < 2; o0 += DB) {
         const int no = min(DB, 2 - o0), nlg = nlines * no;
-        for (int task = threadIdx.x; task < nb * N; task += NT) {
-            int b;
-            const int n = dnb.div(task, b);
+        Digits2 gp(threadIdx.x, NT, dnb);
+        for (int task = threadIdx.x; task < nb * N; task += NT, gp.step()) {
+            const int b = gp.lo, n = gp.hi;
             const float kl = 2 * n == N ? 0.f : r.kfac[la] * (float)(2 * n < N ? n : n - N);
             for (int oo = 0; oo < no; ++oo) {
                 const float kd = o0 + oo == 0 ? 1.f : kl;
@@ -298,11 +298,10 @@ __global__ void __launch_bounds__(NT) k_last_fused(const RepDev* __restrict__ re
         }
         __syncthreads();
         const float2* res = lds_fft<NT>(E0, E1, ldg, nlg, r.ax[la]);
-        const FastDiv dfo(nf * no), dno(no);
-        for (int task = threadIdx.x; task < nlg * N; task += NT) {
-            int b, fo, oo;
-            const int rest = dnb.div(task, b), n = dfo.div(rest, fo);
-            const int f = dno.div(fo, oo);
+        Digits3 gs(threadIdx.x, NT, dnb, FastDiv(nf * no));
+        for (int task = threadIdx.x; task < nlg * N; task += NT, gs.step()) {
+            const int b = gs.lo, fo = gs.mid, n = gs.hi;
+            const int oo = no == 2 ? (fo & 1) : 0, f = no == 2 ? (fo >> 1) : fo;   // no is 1 or 2
             const int slot = o0 + oo == 0 ? 0 : slot1;
             gst(r.Gk + (size_t)(f * D + slot) * r.Mh + (size_t)n * Q + q0 + b, conjf2(res[n * ldg + fo * nb + b]));
         }
@@ -334,9 +333,9 @@ k_mid_grad(const RepDev* __restrict__ reps, const WorkItem* __restrict__ wmap) {
     float2* Y = r.Gk + (size_t)(f * 3 + 1) * r.Mh + plane;
     float2* V = r.Gk + (size_t)(f * 3 + 2) * r.Mh + plane;
     const FastDiv dnb(nb);
-    for (int task = threadIdx.x; task < nb * N; task += NT) {
-        int b;
-        const int n = dnb.div(task, b);
+    Digits2 gl(threadIdx.x, NT, dnb);
+    for (int task = threadIdx.x; task < nb * N; task += NT, gl.step()) {
+        const int b = gl.lo, n = gl.hi;
         const float ky = 2 * n == N ? 0.f : r.kfac[1] * (float)(2 * n < N ? n : n - N);
         const float2 u = gld(U + (size_t)n * Nxh + b), v = gld(V + (size_t)n * Nxh + b);
         // lines of the conj(FFT(conj)) inverse: conj(u), conj(-i ky u), conj(v)
@@ -346,14 +345,11 @@ k_mid_grad(const RepDev* __restrict__ reps, const WorkItem* __restrict__ wmap) {
     }
     __syncthreads();
     const float2* res = lds_fft<NT>(lds, lds + (size_t)N * ldb, ldb, 3 * nb, r.ax[1]);
-    const FastDiv dnb3(3 * nb);
-    for (int task = threadIdx.x; task < 3 * nb * N; task += NT) {
-        int sb;
-        const int n = dnb3.div(task, sb);
-        int b;
-        const int sl = dnb.div(sb, b);
+    Digits3 gs(threadIdx.x, NT, dnb, FastDiv(3));
+    for (int task = threadIdx.x; task < 3 * nb * N; task += NT, gs.step()) {
+        const int b = gs.lo, sl = gs.mid, n = gs.hi;
         float2* out = sl == 0 ? U : (sl == 1 ? Y : V);
-        gst(out + (size_t)n * Nxh + b, conjf2(res[n * ldb + sb]));
+        gst(out + (size_t)n * Nxh + b, conjf2(res[n * ldb + sl * nb + b]));
     }
 }
 
@@ -391,6 +387,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) k_
         for (int L = 0; L < nlines; L += r.dim) xlines |= 1ull << L;
     // each half-spectrum value is read once and fills column kk and its Hermitian
     // partner N - kk
+    // (about two tasks per thread: a division pair each is cheaper than the set-up of
+    // digit counters here)
     for (int task = threadIdx.x; task < nb * Nxh; task += NT) {
         int kk, j;
         const int b = dNxh.div(task, kk);
@@ -469,17 +467,17 @@ __global__ void __launch_bounds__(NT) k_plane_r2c(const RepDev* __restrict__ rep
     float2* B = lds + bsz;
     gmem<const float> rho = G((const float*)r.rho) + (size_t)f * r.M + (size_t)z * Ny * Nx;
     const FastDiv dNx(Nx), dNxh(Nxh);
-    for (int task = threadIdx.x; task < np * Nx; task += NT) {
-        int n;
-        const int j = dNx.div(task, n);
+    Digits2 gl(threadIdx.x, NT, dNx);
+    for (int task = threadIdx.x; task < np * Nx; task += NT, gl.step()) {
+        const int n = gl.lo, j = gl.hi;
         A[n * ldp + j] = make_float2(rho[(size_t)(2 * j) * Nx + n], rho[(size_t)(2 * j + 1) * Nx + n]);
     }
     __syncthreads();
     float2* Z = lds_fft<NT>(A, B, ldp, np, r.ax[0]);
     float2* C = (Z == A) ? B : A;
-    for (int task = threadIdx.x; task < np * Nxh; task += NT) {
-        int k;
-        const int j = dNxh.div(task, k);
+    Digits2 gm(threadIdx.x, NT, dNxh);
+    for (int task = threadIdx.x; task < np * Nxh; task += NT, gm.step()) {
+        const int k = gm.lo, j = gm.hi;
         const float2 Zk = Z[k * ldp + j];
         const float2 Zm = Z[(k == 0 ? 0 : Nx - k) * ldp + j];
         C[(2 * j) * ldc + k] = make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y));
@@ -488,9 +486,9 @@ __global__ void __launch_bounds__(NT) k_plane_r2c(const RepDev* __restrict__ rep
     __syncthreads();
     const float2* Y = lds_fft<NT>(C, Z, ldc, Nxh, r.ax[1]);
     float2* H = r.H + ((size_t)f * r.Nz + z) * Ny * Nxh;
-    for (int task = threadIdx.x; task < Ny * Nxh; task += NT) {
-        int k;
-        const int y = dNxh.div(task, k);
+    Digits2 gs(threadIdx.x, NT, dNxh);
+    for (int task = threadIdx.x; task < Ny * Nxh; task += NT, gs.step()) {
+        const int k = gs.lo, y = gs.hi;
         gst(H + (size_t)y * Nxh + k, Y[y * ldc + k]);
     }
 }

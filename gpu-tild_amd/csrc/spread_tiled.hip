@@ -323,15 +323,20 @@ __global__ void __launch_bounds__(kDepositThreads) k_tile_deposit(const RepDev* 
     const int npairs = (rows + 1) / 2;
     const int ldb = npairs | 1;
     const FastDiv dNx(Nx), dNxh(Nxh), dBy(By), dBz(Bz);
-    for (int task = threadIdx.x; task < npairs * Nx; task += kDepositThreads) {
-        int n;
-        const int j = dNx.div(task, n);
+    // task = j * Nx + n; rows 2j and 2j + 1 are (f, lz, ly) with ly fastest
+    Digits2 gt(threadIdx.x, kDepositThreads, dNx);
+    for (int task = threadIdx.x; task < npairs * Nx; task += kDepositThreads, gt.step()) {
+        const int n = gt.lo, j = gt.hi;
         float v[2] = {0.f, 0.f};
+        int ly, lz;
+        int f = dBz.div(dBy.div(2 * j, ly), lz);
         for (int h = 0; h < 2; ++h) {
             const int rr = 2 * j + h;
             if (rr >= rows) break;
-            int ly, lz;
-            const int f = dBz.div(dBy.div(rr, ly), lz);
+            if (h) {   // the next row: ly + 1 with carries
+                ++ly;
+                if (ly == By) { ly = 0; if (++lz == Bz) { lz = 0; ++f; } }
+            }
             const double inv = f < T ? r.fix_inv : r.fixq_inv;
             v[h] = (float)((double)(long long)patch[(size_t)rr * Nx + n] * inv);
             G(r.rho)[(size_t)f * r.M + ((size_t)(z0 + lz) * r.Ny + (y0 + ly)) * Nx + n] = v[h];
