@@ -439,7 +439,7 @@ int Batch::tile_regions(int id, bool bin_only, std::string& err) {
     if (R.h.spread_mode != 2) return 0;
     if (int rc = unclaim(id, err)) return rc;
     PhaseMap* mp = R.maps;
-    auto ctx = [&](Phase p) { return LaunchCtx{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_}; };
+    auto ctx = [&](Phase p) { return LaunchCtx{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_, nullptr, nullptr, mp[p].threads}; };
     HIPCHK(launch_tile_count(ctx(PH_TCOUNT), 0));
     HIPCHK(launch_tile_scan(ctx(PH_SCAN)));
     if (!bin_only) HIPCHK(launch_tile_regions(stream_, d_reps_, id));
@@ -580,6 +580,15 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
         const int nc = (h.nf * D + 1) / 2;
         add(PH_C2R, (rows + h.nr_c2r - 1) / h.nr_c2r, lds_bytes(h.Nx, h.nr_c2r * nc));
         add(PH_KPROPS, h.nchunk > 0 ? h.nchunk : 1, 0);
+    }
+    // Deposit block size: 512 threads when the set's tiles hold ~600 beads or more on
+    // average (one or two records per thread), else 256 (small tiles leave most of a
+    // larger block idle, and the deposit of a sweep of small replicas took twice as long)
+    {
+        long long beads = 0, tiles = 0;
+        for (int id : ids)
+            if (reps_[id]->h.spread_mode == 2) { beads += reps_[id]->ns; tiles += reps_[id]->h.ntiles; }
+        maps[PH_TILE].threads = tiles > 0 && beads >= 600 * tiles ? 512 : 256;
     }
     // Workgroups go to the 8 XCDs round-robin (launch block w runs on XCD w % 8).  Every
     // pass hands each XCD a contiguous run of work items: for the particle-order passes
@@ -838,7 +847,7 @@ int Batch::build_tables(Replica& R, const System& S, std::string& err) {
 // device step counter plus k, and one launch at the end advances the counter.
 int Batch::enqueue_steps(SetMaps& m, int nsteps, bool integrate, bool write_forces, std::string& err) {
     PhaseMap* mp = m.maps;
-    auto ctx = [&](Phase p) { return LaunchCtx{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_}; };
+    auto ctx = [&](Phase p) { return LaunchCtx{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_, nullptr, nullptr, mp[p].threads}; };
     for (int k = 0; k < nsteps; ++k) {
         HIPCHK(launch_spread(ctx(PH_SPREADA), k));
         HIPCHK(launch_x_r2c(ctx(PH_XA), true, 0, nullptr, nullptr));
@@ -954,7 +963,7 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
     for (int s = 0; s < n && !rc; ++s) {
         bool used[kKernelPhases][kSlots] = {};
         auto ctx = [&](Phase p, int ph, int slot) {
-            LaunchCtx c{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_};
+            LaunchCtx c{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_, nullptr, nullptr, mp[p].threads};
             if (c.n > 0) { c.t0 = ev[ph][slot][0]; c.t1 = ev[ph][slot][1]; used[ph][slot] = true; }
             return c;
         };
@@ -1011,7 +1020,7 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
 int Batch::forward_spectra(int id, std::string& err) {
     Replica& R = *reps_[id];
     PhaseMap* mp = R.maps;
-    auto ctx = [&](Phase p) { return LaunchCtx{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_}; };
+    auto ctx = [&](Phase p) { return LaunchCtx{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_, nullptr, nullptr, mp[p].threads}; };
     HIPCHK(launch_x_r2c(ctx(PH_XP), false, 0, nullptr, nullptr));
     HIPCHK(launch_mid(ctx(PH_PY), 1, 0, 0, 0, nullptr));
     if (R.dim == 3) HIPCHK(launch_mid(ctx(PH_MIDZ_H), 2, 0, 0, 0, nullptr));
@@ -1025,7 +1034,7 @@ int Batch::properties(int id, tild_props* out, std::string& err) {
     HIPCHK(hipSetDevice(device_));
     if (int rc = forward_spectra(id, err)) return rc;
     PhaseMap* mp = R.maps;
-    auto ctx = [&](Phase p) { return LaunchCtx{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_}; };
+    auto ctx = [&](Phase p) { return LaunchCtx{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_, nullptr, nullptr, mp[p].threads}; };
     HIPCHK(launch_kprops(ctx(PH_KPROPS), R.nq));
     HIPCHK(launch_reduce_rows(stream_, R.h.partials, R.nq, R.h.nchunk, R.d_results));
     HIPCHK(launch_bond_props(ctx(PH_PART)));

@@ -38,6 +38,7 @@ struct PhaseMap {
     WorkItem* dev = nullptr;
     size_t lds = 0;
     int n = 0;
+    int threads = 0;   // block size where a pass has more than one (0: its default)
 };
 
 struct Replica {

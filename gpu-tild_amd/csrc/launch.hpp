@@ -63,6 +63,7 @@ struct LaunchCtx {
     hipStream_t stream;
     // step profile: events that take the kernel's own start and stop times
     hipEvent_t t0 = nullptr, t1 = nullptr;
+    int threads = 0;    // block size where a pass has more than one (0: its default)
 };
 
 // Launch on c.stream; with c.t0 set, through hipExtLaunchKernel, whose events take
@@ -106,6 +107,7 @@ hipError_t launch_tile_scan(const LaunchCtx& c);
 hipError_t launch_tile_claim(const LaunchCtx& c, int koff);
 hipError_t launch_tile_reset(const LaunchCtx& c, int koff);
 hipError_t launch_tile_regions(hipStream_t s, const RepDev* reps, int rep);
+// c.threads: 512 for sets whose tiles hold many beads, else 256
 hipError_t launch_tile_deposit(const LaunchCtx& c, int koff);
 hipError_t launch_bins(hipStream_t s, const RepDev* reps, int rep, int ns, int* out);
 hipError_t launch_virial_source(hipStream_t s, const RepDev* reps, int rep, int M, const float* fr, float* vr);

@@ -246,12 +246,11 @@ __device__ __forceinline__ void deposit_record_any(const RepDev& r, const float4
     }
 }
 
-// Deposit one tile's rows and transform them along x.  512 threads: a tile has about
-// 1,000 beads plus its halo share, and three blocks of 512 fit a CU (with 1,024
-// threads only one fits, and the deposit took 22.8 us instead of 17.5 on the melt;
-// with 256, 19.6 us).
-constexpr int kDepositThreads = 512;
-
+// Deposit one tile's rows and transform them along x.  NT = 512 for tiles of about
+// 1,000 beads plus their halo share (three blocks of 512 fit a CU; on the 640,000-bead
+// melt 17.5 us, against 19.6 with 256 threads and 22.8 with 1,024, where one block
+// fills a CU); NT = 256 for small tiles, where a larger block mostly idles.
+template <int kDepositThreads>
 __global__ void __launch_bounds__(kDepositThreads) k_tile_deposit(const RepDev* __restrict__ reps,
                                                               const WorkItem* __restrict__ wmap, int koff) {
     extern __shared__ unsigned long long tile_lds[];
@@ -391,7 +390,8 @@ hipError_t launch_tile_regions(hipStream_t s, const RepDev* reps, int rep) {
 }
 hipError_t launch_tile_deposit(const LaunchCtx& c, int koff) {
     if (!c.n) return hipSuccess;
-    TILD_LAUNCH(c, k_tile_deposit, dim3(c.n), dim3(kDepositThreads), c.lds, c.reps, c.wmap, koff);
+    if (c.threads >= 512) TILD_LAUNCH(c, k_tile_deposit<512>, dim3(c.n), dim3(512), c.lds, c.reps, c.wmap, koff);
+    else TILD_LAUNCH(c, k_tile_deposit<256>, dim3(c.n), dim3(256), c.lds, c.reps, c.wmap, koff);
     CHECK_LAUNCH(); return hipSuccess;
 }
 

@@ -262,6 +262,30 @@ def test_full_bucket_regions_take_the_overflow_path(case, tmp_path):
     et.close()
 
 
+@pytest.mark.parametrize("case", ["gjf", "mixed"])
+def test_large_tiles_take_the_wide_deposit(case, tmp_path):
+    # four 8 x 8-row tiles of 750 beads each run the 512-thread deposit blocks; the
+    # densities and the trajectory stay those of the global-atomic spread
+    text, d = decks.melt3d(tmp_path, n_chains=250, N=12, box=(8.0, 8.0, 8.0), grid=(16, 16, 16),
+                           integrator="GJF" if case == "gjf" else "EM", delt=0.01 if case == "gjf" else 0.005,
+                           styles=decks.MIXED if case == "mixed" else None)
+    s = tild.System(text, d)
+    assert s.ns == 3000
+    ea = tild.Engine(s, device=0, seed=2, spread_mode=1)
+    et = tild.Engine(s, device=0, seed=2, spread_mode=2, tile_target=2)
+    for n in (1, 4):
+        ea.step(n)
+        et.step(n)
+        ea.forces()
+        et.forces()
+        for t in range(ea.nfields):
+            assert np.array_equal(et.density(t), ea.density(t)), (n, t)
+        assert np.array_equal(et.positions(), ea.positions()), n
+    print(f"[{case}] ns={s.ns}, few large tiles: bit-identical to the atomic spread")
+    ea.close()
+    et.close()
+
+
 @pytest.mark.parametrize("case", ["3d", "2d", "charged"])
 def test_next_step_claims_follow_the_positions(case, tmp_path):
     # integrating steps claim the next step's bucket slots inside the particle kernel;
