@@ -14,6 +14,7 @@ constexpr int kMaxTypes = 16;      // particle types per replica
 constexpr int kMaxStages = 12;     // FFT stages per axis
 constexpr int kFftThreads = 256;   // threads of every FFT block
 constexpr int kFftElems = 8;       // complex values per thread per FFT call (LDS budget)
+constexpr int kMaxRamp = 8;        // ramped pair styles per replica
 constexpr int kMaxQuant = 128;     // property quantities per replica
 
 // Mixed-radix Stockham plan of one axis.  Stage s has radix rad[s] and
@@ -81,6 +82,13 @@ struct RepDev {
     int ctab_rows;             // rows of Ctab: T(T+1)/2 (+1 with charges)
     const float* FPtab;        // T*dim*M net static field-phase forces per type (or null)
     int ctab_charge;           // row of the charge kernel in Ctab (or -1)
+    // ramped pair styles (pair_style.cu:308-328): Ctab row ramp_row0 + j holds ramped
+    // style j's table per unit prefactor, added to pair row ramp_pair[j] with weight
+    // A_j(k) - ramp_A0[j] at 0-based step k, where A_j(0) = ramp_A0[j] and
+    // A_j(k) = ramp_c0[j] + ramp_c1[j] * (k + 1) after the first step
+    int nramp, ramp_row0;
+    int ramp_pair[kMaxRamp];
+    double ramp_A0[kMaxRamp], ramp_c0[kMaxRamp], ramp_c1[kMaxRamp];
     // property tables
     int nstyles, npairs, nq;
     int st_t1[64], st_t2[64];

@@ -679,13 +679,38 @@ int Batch::build_tables(Replica& R, const System& S, std::string& err) {
     h.nq = R.nq;
     h.nchunk = (int)std::max<size_t>(1, std::min<size_t>(512, (Mh + 4095) / 4096));
 
-    // coupling tables C_IJ (I <= J) / M, then the charge kernel
+    // coupling tables C_IJ (I <= J) / M, then the charge kernel, then one row per ramped
+    // style: its table per unit prefactor, weighted per step by the kernel
     const int ncp = T * (T + 1) / 2;
-    const int nrows = ncp + (S.charges.enabled ? 1 : 0);
+    std::vector<int> ramped;
+    for (size_t s = 0; s < styles.size(); ++s) if (styles[s]->ramp) ramped.push_back((int)s);
+    if ((int)ramped.size() > kMaxRamp) { err = "more than 8 ramped pair styles"; return -TILD_ERR_UNSUPPORTED; }
+    if (!ramped.empty() && S.max_steps <= 0) { err = "a ramp needs max_steps > 0"; return -TILD_ERR_UNSUPPORTED; }
+    const int nrows = ncp + (S.charges.enabled ? 1 : 0) + (int)ramped.size();
     h.ctab_charge = S.charges.enabled ? ncp : -1;
     std::vector<float> ctab((size_t)nrows * Mh), etab((size_t)std::max(1, R.nstyles) * Mh), eltab;
     if (S.charges.enabled) eltab.resize(Mh);
     auto pidx = [&](int I, int J) { if (I > J) std::swap(I, J); return I * T - I * (I - 1) / 2 + (J - I); };
+    // Ramp schedule of pair_style.cu:308-328.  After step s (1-based) the reference sets
+    // the table to new/cur times itself, with new = 1000 A0 + delta (s + 1), delta =
+    // 1000 (A0 - A_final) / max_steps, and cur = 1000 u[k=0]: the amplitude becomes
+    // (A0 + (A0 - A_final)(s + 1) / max_steps) / u1(0), with u1 the table per unit
+    // prefactor (u1(0) = 1 for a Gaussian, (4 pi Rp^3 / 3)^2 for erf, 4 pi Rp^3 / 3 for
+    // gaussian_erf).  The ramp therefore moves away from A_final, and the virial
+    // tables keep A0, as in the reference.  Step k (0-based) uses the table of the
+    // update after step k, none at k = 0.
+    h.nramp = (int)ramped.size();
+    h.ramp_row0 = ncp + (S.charges.enabled ? 1 : 0);
+    R.ramps.clear();
+    for (int j = 0; j < h.nramp; ++j) {
+        const PairStyle* ps = styles[ramped[j]];
+        const double A0 = ps->initial_prefactor, u10 = ps->kspace(0.0, 1.0);
+        h.ramp_pair[j] = pidx(ps->type1, ps->type2);
+        h.ramp_A0[j] = A0;
+        h.ramp_c0[j] = A0 / u10;
+        h.ramp_c1[j] = (A0 - ps->final_prefactor) / ((double)S.max_steps * u10);
+        R.ramps.push_back({ramped[j], A0, h.ramp_c0[j], h.ramp_c1[j]});
+    }
     std::vector<double> c(ncp);
     for (size_t idx = 0; idx < Mh; ++idx) {
         double kv[3];
@@ -699,6 +724,11 @@ int Batch::build_tables(Replica& R, const System& S, std::string& err) {
             etab[s * Mh + idx] = (float)(u * gvol / M);
         }
         for (int p = 0; p < ncp; ++p) ctab[(size_t)p * Mh + idx] = (float)(c[p] / M);
+        for (int j = 0; j < h.nramp; ++j) {
+            const PairStyle* ps = styles[ramped[j]];
+            const double u1 = ps->kspace(k2, 1.0) * (ps->type1 == ps->type2 ? 2.0 : 1.0);
+            ctab[(size_t)(h.ramp_row0 + j) * Mh + idx] = (float)(u1 / M);
+        }
         if (S.charges.enabled) {
             double g1 = S.charges.potential(k2);
             double a = S.charges.smearing_length;
@@ -854,7 +884,7 @@ int Batch::enqueue_steps(SetMaps& m, int nsteps, bool integrate, bool write_forc
         HIPCHK(launch_tile_deposit(ctx(PH_TILE), k));
         HIPCHK(launch_plane_r2c(ctx(PH_PLANEF)));
         HIPCHK(launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr));
-        HIPCHK(launch_last_fused(ctx(PH_LAST), 0));
+        HIPCHK(launch_last_fused(ctx(PH_LAST), 0, k));
         HIPCHK(launch_mid_grad(ctx(PH_MIDY_G)));
         HIPCHK(launch_x_c2r(ctx(PH_C2R), true, 0, nullptr, nullptr));
         // integrating steps claim the next step's records; a force evaluation leaves
@@ -976,7 +1006,7 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
         ok(launch_tile_deposit(ctx(PH_TILE, 1, 1), 0));
         ok(launch_plane_r2c(ctx(PH_PLANEF, 2, 0)));
         ok(launch_mid(ctx(PH_MIDY_H, 2, 1), 1, 0, 0, 0, nullptr));
-        ok(launch_last_fused(ctx(PH_LAST, 3, 0), 0));
+        ok(launch_last_fused(ctx(PH_LAST, 3, 0), 0, 0));
         ok(launch_mid_grad(ctx(PH_MIDY_G, 4, 0)));
         ok(launch_x_c2r(ctx(PH_C2R, 5, 0), true, 0, nullptr, nullptr));
         ok(launch_particles(ctx(PH_PART, 6, 0), true, 0, 0, true));
@@ -1047,6 +1077,13 @@ int Batch::properties(int id, tild_props* out, std::string& err) {
     out->n_styles = R.nstyles;
     double Ust = 0.0;
     out->nan_flag = 0;
+    // ramped styles: the energy of the table in force at the last step taken
+    // (0-based k = step - 1; the initial table before any step)
+    for (const RampInfo& ri : R.ramps) {
+        const int k = std::max(R.step - 1, 0);
+        const double A = k == 0 ? ri.A0 : ri.c0 + ri.c1 * (double)(k + 1);
+        res[ri.style] *= A / ri.A0;
+    }
     for (int s = 0; s < R.nstyles && s < TILD_MAX_STYLES; ++s) {
         out->U_style[s] = res[s];
         Ust += res[s];

@@ -41,6 +41,12 @@ struct PhaseMap {
     int threads = 0;   // block size where a pass has more than one (0: its default)
 };
 
+// A ramped pair style: its index and ramp schedule (RepDev::ramp_*).
+struct RampInfo {
+    int style;
+    double A0, c0, c1;
+};
+
 struct Replica {
     RepDev h{};                       // host copy of the device descriptor
     std::vector<void*> allocs;        // owned device allocations
@@ -62,6 +68,7 @@ struct Replica {
     // LDS-tiled replicas: the current step's records are in the bucket regions (made
     // by the last integrating step's particle kernel, or by a bucket-claim pass)
     bool claimed = false;
+    std::vector<RampInfo> ramps;
     ReorderBufs rb;
 };
 

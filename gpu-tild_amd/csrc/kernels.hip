@@ -162,12 +162,21 @@ __global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_e
 }
 
 // ------------------------------------------------------------------ fused last axis
+// Weight of ramped style j's row at 0-based step k (pair_style.cu:308-328, see
+// RepDev::ramp_*): A_j(k) - A_j(0).
+__device__ __forceinline__ float ramp_weight(const RepDev& r, int j, int k) {
+    const double A = k == 0 ? r.ramp_A0[j] : r.ramp_c0[j] + r.ramp_c1[j] * (double)(k + 1);
+    return (float)(A - r.ramp_A0[j]);
+}
+
 // k-space coupling of one block's nodes for TT types (the same sums, in the same order,
 // as the general loop of k_last_fused): w_I = sum_J C_IJ rho_J with the symmetric table
-// rows pidx(lo, hi) = lo*TT - lo*(lo-1)/2 + (hi - lo), and the charge row.
+// rows pidx(lo, hi) = lo*TT - lo*(lo-1)/2 + (hi - lo), and the charge row.  Ramped
+// styles add their rows, weighted, to their pairs' coefficients.
 template <int TT, int NT>
 __device__ __forceinline__ void couple_nodes(gmem<const float> cb, int nrows, int nlc, int nb, int N, int ldb, int ldw,
-                                             int nf, int ctab_charge, const FastDiv& dnb, const float2* A, float2* Wb) {
+                                             int nf, int ctab_charge, const FastDiv& dnb, const float2* A, float2* Wb,
+                                             const RepDev& r, const float* rw) {
     constexpr int NR = TT * (TT + 1) / 2;
     const bool charged = nf > TT;
     Digits2 gc(threadIdx.x, NT, dnb);
@@ -179,6 +188,11 @@ __device__ __forceinline__ void couple_nodes(gmem<const float> cb, int nrows, in
 #pragma unroll
         for (int k = 0; k < NR; ++k) c[k] = cn[(size_t)k * nlc];
         const float cq = charged ? cn[(size_t)ctab_charge * nlc] : 0.f;
+        for (int j = 0; j < r.nramp; ++j) {   // uniform trip count
+            const float v = rw[j] * cn[(size_t)(r.ramp_row0 + j) * nlc];
+#pragma unroll
+            for (int k = 0; k < NR; ++k) c[k] += r.ramp_pair[j] == k ? v : 0.f;
+        }
         float2 rh[TT];
 #pragma unroll
         for (int J = 0; J < TT; ++J) rh[J] = A[row + J * nb];
@@ -213,7 +227,7 @@ __device__ __forceinline__ void couple_nodes(gmem<const float> cb, int nrows, in
 template <int NT>
 __global__ void __launch_bounds__(NT) k_last_fused(const RepDev* __restrict__ reps,
                                                             const WorkItem* __restrict__ wmap,
-                                                            int store_spectrum) {
+                                                            int store_spectrum, int koff) {
     extern __shared__ float2 lds[];
     const WorkItem wi = wmap[blockIdx.x];
     const RepDev& r = reps[wi.rep];
@@ -233,6 +247,8 @@ __global__ void __launch_bounds__(NT) k_last_fused(const RepDev* __restrict__ re
     float2* Wb = lds;
     float2* E0 = lds + (size_t)N * ldw;
     float2* E1 = E0 + (size_t)N * ldg;
+    float* rw = (float*)(E1 + (size_t)N * ldg);          // ramp weights of this step
+    if ((int)threadIdx.x < r.nramp) rw[threadIdx.x] = ramp_weight(r, threadIdx.x, r.step + koff);
     const FastDiv dnb(nb), dnf(nf);
     Digits3 gl(threadIdx.x, NT, dnb, dnf);
     for (int task = threadIdx.x; task < nlines * N; task += NT, gl.step()) {
@@ -244,12 +260,13 @@ __global__ void __launch_bounds__(NT) k_last_fused(const RepDev* __restrict__ re
     // coupling w_I = sum_J C_IJ rho_J (C symmetric, rows I <= J), charge w_c = G_c rho_c;
     // this block's table rows are one contiguous run
     const int nrows = r.ctab_rows, nlc = r.nl_last;
+
     gmem<const float> cb = G(r.Ctab) + (size_t)wi.blk * N * nrows * nlc;
     switch (T) {   // up to four types: unrolled, every table row of a node requested at once
-        case 1: couple_nodes<1, NT>(cb, nrows, nlc, nb, N, ldb, ldw, nf, r.ctab_charge, dnb, A, Wb); break;
-        case 2: couple_nodes<2, NT>(cb, nrows, nlc, nb, N, ldb, ldw, nf, r.ctab_charge, dnb, A, Wb); break;
-        case 3: couple_nodes<3, NT>(cb, nrows, nlc, nb, N, ldb, ldw, nf, r.ctab_charge, dnb, A, Wb); break;
-        case 4: couple_nodes<4, NT>(cb, nrows, nlc, nb, N, ldb, ldw, nf, r.ctab_charge, dnb, A, Wb); break;
+        case 1: couple_nodes<1, NT>(cb, nrows, nlc, nb, N, ldb, ldw, nf, r.ctab_charge, dnb, A, Wb, r, rw); break;
+        case 2: couple_nodes<2, NT>(cb, nrows, nlc, nb, N, ldb, ldw, nf, r.ctab_charge, dnb, A, Wb, r, rw); break;
+        case 3: couple_nodes<3, NT>(cb, nrows, nlc, nb, N, ldb, ldw, nf, r.ctab_charge, dnb, A, Wb, r, rw); break;
+        case 4: couple_nodes<4, NT>(cb, nrows, nlc, nb, N, ldb, ldw, nf, r.ctab_charge, dnb, A, Wb, r, rw); break;
         default: break;
     }
     for (int task = threadIdx.x; T > 4 && task < nb * N; task += NT) {
@@ -261,7 +278,9 @@ __global__ void __launch_bounds__(NT) k_last_fused(const RepDev* __restrict__ re
             for (int J = 0; J < T; ++J) {
                 const int lo = I < J ? I : J, hi = I < J ? J : I;
                 const int pidx = lo * T - lo * (lo - 1) / 2 + (hi - lo);
-                const float c = cb[((size_t)n * nrows + pidx) * nlc + b];
+                float c = cb[((size_t)n * nrows + pidx) * nlc + b];
+                for (int j = 0; j < r.nramp; ++j)
+                    if (r.ramp_pair[j] == pidx) c += rw[j] * cb[((size_t)n * nrows + r.ramp_row0 + j) * nlc + b];
                 const float2 rh = A[row + J * nb];
                 acc.x += c * rh.x; acc.y += c * rh.y;
             }
@@ -996,9 +1015,9 @@ hipError_t launch_mid(const LaunchCtx& c, int axis, int use_gk, int nfb, int inv
     TILD_LAUNCH(c, k_mid, dim3(c.n), dim3(kFftThreads), c.lds, c.reps, c.wmap, axis, use_gk, nfb, inverse, buf);
     CHECK_LAUNCH(); return hipSuccess;
 }
-hipError_t launch_last_fused(const LaunchCtx& c, int store_spectrum) {
+hipError_t launch_last_fused(const LaunchCtx& c, int store_spectrum, int koff) {
     if (!c.n) return hipSuccess;
-    TILD_LAUNCH(c, k_last_fused<kLastThreads>, dim3(c.n), dim3(kLastThreads), c.lds, c.reps, c.wmap, store_spectrum);
+    TILD_LAUNCH(c, k_last_fused<kLastThreads>, dim3(c.n), dim3(kLastThreads), c.lds, c.reps, c.wmap, store_spectrum, koff);
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t launch_x_c2r(const LaunchCtx& c, bool records, int nlines, const float2* in, float* out) {

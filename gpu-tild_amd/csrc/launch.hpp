@@ -51,8 +51,9 @@ inline size_t fft_buf_bytes(int N, int nlines) { return (size_t)N * (size_t)(nli
 inline size_t lds_bytes(int N, int nlines) { return 2 * fft_buf_bytes(N, nlines); }
 // fused last-axis pass: coupled fields (nf*nl lines, unpadded: only read and written
 // element by element) + two buffers of nf*DB*nl lines (DB inverse outputs per batch)
+// + the ramp weights of the step
 inline size_t last_lds_bytes(int N, int nf, int DB, int nl) {
-    return (size_t)N * nf * nl * sizeof(float2) + 2 * fft_buf_bytes(N, nf * DB * nl);
+    return (size_t)N * nf * nl * sizeof(float2) + 2 * fft_buf_bytes(N, nf * DB * nl) + kMaxRamp * sizeof(float);
 }
 
 struct LaunchCtx {
@@ -77,7 +78,7 @@ struct LaunchCtx {
 hipError_t launch_spread(const LaunchCtx& c, int koff);
 hipError_t launch_x_r2c(const LaunchCtx& c, bool from_acc, int nrows_override, const float* in, float2* out);
 hipError_t launch_mid(const LaunchCtx& c, int axis, int use_gk, int nfb, int inverse, float2* buf);
-hipError_t launch_last_fused(const LaunchCtx& c, int store_spectrum);
+hipError_t launch_last_fused(const LaunchCtx& c, int store_spectrum, int koff);
 hipError_t launch_x_c2r(const LaunchCtx& c, bool records, int nlines, const float2* in, float* out);
 // claim: integrating steps of LDS-tiled replicas also claim the next step's records
 hipError_t launch_particles(const LaunchCtx& c, bool integrate, int write_forces, int koff, bool claim);

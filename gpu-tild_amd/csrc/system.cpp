@@ -466,7 +466,9 @@ int load_system(const std::string& text, const std::string& base_dir, System& S,
                         ps = g.get(); S.GaussErfs.push_back(std::move(g));
                     }
                     ps->ramp = ramp; ps->final_prefactor = Afin;
-                    if (ramp) { err = "ramped pair-style prefactors are not supported by this engine"; return kUnsupported; }
+                    // the reference's update divides by the current k = 0 value
+                    // (pair_style.cu:318-324): a zero prefactor has no ramp
+                    if (ramp && A == 0.0f) { err = "ramp from a zero prefactor"; return kUnsupported; }
                 }
             }
             else if (w == "Nx") { if (!geti(iv)) return bad("Nx needs an integer"); S.Nx[0] = iv; }

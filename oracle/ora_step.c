@@ -469,8 +469,31 @@ void ora_integrate(ora_sys* s) {
     }
 }
 
+/* PairStyle::Update, pair_style.cu:308-328, after step `step` (1-based): the table u_k
+ * and the force tables f_k are multiplied by new/cur, with
+ *   delta = 1000 (A0 - A_final) / max_steps,  new = 1000 A0 + delta step + delta,
+ *   cur = 1000 Re u_k[0]   (the current k = 0 value, not the current prefactor);
+ * vir_k is left as it is.  The reference does this in FP32; here in FP64. */
+static void update_style(ora_sys* s, ora_style* st, int step) {
+    if (!st->ramp || st->kind == ORA_FIELDPHASE) return;
+    int M = s->M, D = s->dim;
+    double delta_A = (1000.0 * (st->A - st->A_final)) / (double)s->max_steps;
+    double cur_A = 1000.0 * st->A + delta_A * (double)step;
+    double new_A = cur_A + delta_A;
+    cur_A = st->u_k[0] * 1000.0;
+    double r = new_A / cur_A;
+    for (int i = 0; i < 2 * M; ++i) st->u_k[i] *= r;
+    for (int i = 0; i < 2 * M * D; ++i) st->f_k[i] *= r;
+}
+
 void ora_run(ora_sys* s, int nsteps) {
     for (int k = 0; k < nsteps; ++k) {   /* main.cu:106-117 */
+        /* update_pairstyles() of the previous step (main.cu:199), applied when the next
+         * step starts, so that properties evaluated between steps see the table of the
+         * step just taken, as the reference's log does (calc_properties precedes
+         * update_pairstyles in the step) */
+        if (s->step > 0)
+            for (int j = 0; j < s->nst; ++j) update_style(s, &s->st[j], s->step);
         s->step += 1;
         ora_forces(s);
         ora_integrate(s);

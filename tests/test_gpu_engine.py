@@ -390,3 +390,35 @@ def test_profiled_steps_are_ordinary_steps(tmp_path):
     print("profile: " + ", ".join(f"{k} {v * 1e3:.1f} us" for k, v in prof.items()))
     ea.close()
     eb.close()
+
+
+def test_ramped_styles_match_oracle(tmp_path):
+    # Gaussian, erf and gaussian-erf prefactors ramped by the reference's update
+    # (pair_style.cu:308-328): the engine weights each ramped style's table row per
+    # step on the device, so runs of steps replayed from captured graphs (offsets
+    # 0..9 in a run) see the same schedule as single steps.  Trajectory and style
+    # energies after each run follow the oracle, which rescales its tables after every
+    # step as the reference does.
+    styles = [("gaussian", 1, 1, 1.0, 0.5), ("gaussian", 2, 2, 1.0, 0.5), ("gaussian", 1, 2, 2.0, 0.5),
+              ("erf", 1, 2, 0.1, 1.0, 0.5), ("gaussian_erf", 2, 2, 0.5, 0.6, 0.5)]
+    text, d = decks.melt3d(tmp_path, integrator="GJF", delt=0.01, styles=styles)
+    for old, new in [("gaussian 1 2 2.0 0.5", "gaussian 1 2 2.0 0.5 ramp 4.0"),
+                     ("erf 1 2 0.1 1.0 0.5", "erf 1 2 0.1 1.0 0.5 ramp 0.3"),
+                     ("gaussian_erf 2 2 0.5 0.6 0.5", "gaussian_erf 2 2 0.5 0.6 0.5 ramp 0.2"),
+                     ("max_steps 1000", "max_steps 40")]:
+        assert old in text, old
+        text = text.replace(old, new)
+    s, e, o = _pair(text, d, seed=13)
+    L = [s.info("L0"), s.info("L1"), s.info("L2")][:s.dim]
+    done = 0
+    for n in (1, 2, 12):
+        e.step(n)
+        o.run(n)
+        done += n
+        err = np.abs(U.min_image(e.positions().astype(np.float64) - o.positions(), L)).max()
+        p, po = e.properties(), o.properties()
+        print(f"  after {done} steps: max |dx| = {err:.2e}, U_styles engine {p['U_styles']} oracle {po['U_styles']}")
+        assert err < 1e-4, done
+        np.testing.assert_allclose(p["U_styles"], po["U_styles"], rtol=2e-5, atol=1e-9)
+        assert abs(p["Upe"] - po["Upe"]) <= 2e-5 * abs(po["Upe"])
+    e.close()

@@ -87,7 +87,7 @@ def test_deck_errors_after_read_data(tmp_path):
         (text.replace("integrator all EM", "integrator all VV"), 1),
         (text.replace("Nz 12\n", ""), 2),
         (text.replace("n_gaussians 3\ngaussian", "n_gaussians 3\ngaussianx"), 1),
-        (text.replace("gaussian 1 2 2.0 0.5", "gaussian 1 2 2.0 0.5 ramp 3.0"), 8),
+        (text.replace("gaussian 1 2 2.0 0.5", "gaussian 1 2 0.0 0.5 ramp 3.0"), 8),   # ramp from zero
         (text.replace("read_data input.data", "charges 1.0 1.0\nread_data input.data").replace("Dim 3", "Dim 3"), 1),
     ]
     for t, status in cases:
@@ -143,7 +143,7 @@ def test_fieldphase_decks(tmp_path):
     bad = [("fieldphase 1 2 0.5 1 0 1", 8),              # BCC: no field in the reference
            ("fieldphase 1 2 0.5 0 3 1", 2),              # lamellae along a 4th axis
            ("fieldphase 1 3 0.5 0 0 1", 1),              # type out of range
-           ("fieldphase 1 2 0.5 0 0 1 ramp 1.0", 8)]     # ramps are outside the engine
+           ("fieldphase 1 2 0.5 0 0 1 ramp 1.0", 8)]     # field-phase ramps are outside the engine
     for line, status in bad:
         with pytest.raises(tild.TildError) as ei:
             tild.System(text + "n_fieldphases 1\n" + line + "\n", d)

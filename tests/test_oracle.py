@@ -413,3 +413,43 @@ def test_fieldphase_bcc_is_rejected(tmp_path):
     text += "n_fieldphases 1\nfieldphase 1 2 0.5 1 0 1\n"
     with pytest.raises(ValueError, match="BCC"):
         O.System(text, d, seed=1)
+
+
+RAMP_STYLES = {
+    # (style tuple, its deck line, u per unit prefactor at k = 0)
+    "gaussian": (("gaussian", 1, 2, 2.0, 0.5), "gaussian 1 2 2.0 0.5", lambda Rp: 1.0),
+    "erf": (("erf", 1, 2, 0.4, 0.6, 0.5), "erf 1 2 0.4 0.6 0.5", lambda Rp: (4.0 * math.pi * Rp ** 3 / 3.0) ** 2),
+    "gaussian_erf": (("gaussian_erf", 1, 2, 0.7, 0.6, 0.5), "gaussian_erf 1 2 0.7 0.6 0.5",
+                     lambda Rp: 4.0 * math.pi * Rp ** 3 / 3.0),
+}
+
+
+@pytest.mark.parametrize("kind", list(RAMP_STYLES))
+def test_ramp_follows_the_reference_update(kind, tmp_path):
+    # PairStyle::Update (pair_style.cu:308-328) after step s multiplies the style's
+    # tables by new/cur with new = 1000 A0 + delta (s + 1), delta = 1000 (A0 - Af) /
+    # max_steps, and cur = 1000 u[k=0], the table's k = 0 value rather than its
+    # prefactor.  The amplitude after step s is therefore (A0 + (A0 - Af)(s + 1) /
+    # max_steps) / u1(0), with u1(0) the k = 0 value per unit prefactor (1 for a
+    # Gaussian, the squared sphere volume for erf, the sphere volume for gaussian_erf):
+    # the ramp moves away from Af, and erf styles jump by 1/u1(0) at the first update.
+    # The virial tables are never rescaled.  With the beads frozen (delt = 0) the style
+    # energy of the log after S steps (table of the update after step S - 1) follows
+    # that amplitude exactly, and the pressure stays at its step-0 value.
+    style, line, u10 = RAMP_STYLES[kind]
+    A0, Af, ms = float(np.float32(style[3])), 5.0, 10    # prefactors are parsed as FP32
+    text, d = decks.melt3d(tmp_path, styles=[style], delt=0.0)
+    text = text.replace(line, line + f" ramp {Af}").replace("max_steps 1000", f"max_steps {ms}")
+    assert f"ramp {Af}" in text and f"max_steps {ms}" in text
+    o = O.System(text, d, seed=1)
+    x0 = o.positions().copy()
+    p0 = o.properties()
+    U0 = p0["U_styles"][0]
+    Rp = float(np.float32(style[4])) if kind != "gaussian" else 0.0
+    for S in range(1, 7):
+        o.run(1)
+        p = o.properties()
+        amp = A0 if S == 1 else (A0 + (A0 - Af) * S / ms) / u10(Rp)
+        assert abs(p["U_styles"][0] - U0 * amp / A0) <= 1e-9 * abs(U0 * amp / A0), (kind, S)
+        np.testing.assert_allclose(p["Ptens"], p0["Ptens"], rtol=1e-12, atol=1e-15)
+    np.testing.assert_array_equal(o.positions(), x0)
