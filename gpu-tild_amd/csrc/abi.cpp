@@ -292,6 +292,13 @@ tild_status tild_engine_nlist(tild_engine* e, float rc, const int* sites, int ns
     return from_rc(rc_, err, e);
 }
 
+tild_status tild_engine_get_state(tild_engine* e, int* step, float* xo_disp, size_t n) {
+    ENGINE_COPY(get_state(e->id, step, xo_disp, n, err))
+}
+tild_status tild_engine_set_state(tild_engine* e, int step, const float* xo_disp, size_t n) {
+    ENGINE_COPY(set_state(e->id, step, xo_disp, n, err))
+}
+
 tild_status tild_engine_get_step(tild_engine* e, int* step) {
     if (!e || !step) return fail(TILD_ERR_ARG, "null argument");
     *step = e->batch->rep(e->id)->step;

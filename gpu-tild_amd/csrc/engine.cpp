@@ -1146,6 +1146,52 @@ int Batch::set_positions(int id, const float* x, size_t n, std::string& err) {
     return 0;
 }
 
+int Batch::get_state(int id, int* step, float* xo, size_t n, std::string& err) {
+    Replica* R = rep(id);
+    if (!R || !step || !xo || n != (size_t)R->ns * R->dim) { err = "get_state: size mismatch"; return -TILD_ERR_ARG; }
+    std::vector<float> tmp(n);
+    HIPCHK(hipMemcpyAsync(tmp.data(), R->h.XO, n * 4, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    std::vector<int> pid;
+    if (R->permuted) { if (int rc = fetch_pid(*R, pid, err)) return rc; }
+    unpermute(tmp.data(), xo, n, pid, R->dim);
+    *step = R->step;
+    return 0;
+}
+
+int Batch::set_state(int id, int step, const float* xo, size_t n, std::string& err) {
+    Replica* R = rep(id);
+    if (!R || !xo || step < 0 || n != (size_t)R->ns * R->dim) { err = "set_state: bad arguments"; return -TILD_ERR_ARG; }
+    // the claims in the regions belong to the current step's parity, and the other
+    // parity's counters are only cleared by the next step's deposit: clear both, here
+    // for the current parity and after the step changes for the new one
+    if (int rc = unclaim(id, err)) return rc;
+    const bool tiled = R->h.spread_mode == 2;
+    PhaseMap* mp = R->maps;
+    const LaunchCtx rctx{d_reps_, mp[PH_SCAN].dev, mp[PH_SCAN].n, 0, stream_};
+    if (tiled) HIPCHK(launch_tile_reset(rctx, 0));
+    std::vector<float> tmp;
+    const float* src = xo;
+    if (R->permuted) {   // slot p holds site pid[p]
+        std::vector<int> pid;
+        if (int rc = fetch_pid(*R, pid, err)) return rc;
+        tmp.resize(n);
+        const int D = R->dim;
+        for (int p = 0; p < R->ns; ++p)
+            for (int d = 0; d < D; ++d) tmp[(size_t)p * D + d] = xo[(size_t)pid[p] * D + d];
+        src = tmp.data();
+    }
+    HIPCHK(hipMemcpyAsync(R->h.XO, src, n * 4, hipMemcpyHostToDevice, stream_));
+    // the positions live in the buffer of the step's parity
+    if ((step & 1) != (R->step & 1))
+        HIPCHK(hipMemcpyAsync(R->h.X[step & 1], R->h.X[R->step & 1], n * 4, hipMemcpyDeviceToDevice, stream_));
+    R->step = step;
+    if (int rc = upload_desc(id, err)) return rc;
+    if (tiled) HIPCHK(launch_tile_reset(rctx, 0));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return 0;
+}
+
 int Batch::get_forces(int id, float* f, size_t n, std::string& err) {
     Replica* R = rep(id);
     if (!R || !f || n != (size_t)R->ns * R->dim) { err = "get_forces: size mismatch"; return -TILD_ERR_ARG; }

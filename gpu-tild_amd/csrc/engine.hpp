@@ -109,6 +109,9 @@ public:
 
     int get_positions(int id, float* x, size_t n, std::string& err);
     int set_positions(int id, const float* x, size_t n, std::string& err);
+    // exact restart state: step counter and GJF displacement x - xo in site order
+    int get_state(int id, int* step, float* xo, size_t n, std::string& err);
+    int set_state(int id, int step, const float* xo, size_t n, std::string& err);
     int get_forces(int id, float* f, size_t n, std::string& err);
     int get_density(int id, int type, float* rho, size_t n, std::string& err);
     int get_grid_field(int id, int type, int dir, float* g, size_t n, std::string& err);

@@ -24,6 +24,7 @@ EXPORTS = [
     "tild_engine_forces", "tild_engine_step", "tild_engine_properties",
     "tild_engine_get_positions", "tild_engine_set_positions", "tild_engine_get_forces",
     "tild_engine_get_density", "tild_engine_get_grid_field", "tild_engine_get_step",
+    "tild_engine_get_state", "tild_engine_set_state",
     "tild_engine_synchronize", "tild_engine_get_bins", "tild_engine_profile", "tild_engine_nlist", "tild_fft_r2c", "tild_fft_c2r",
     "tild_batch_create", "tild_batch_destroy", "tild_batch_last_error", "tild_batch_add", "tild_batch_remove",
     "tild_batch_step", "tild_batch_properties", "tild_batch_get_positions", "tild_batch_set_positions",
@@ -112,6 +113,8 @@ def lib():
         L.tild_engine_get_density.argtypes = [vp, ctypes.c_int, fp, sz]
         L.tild_engine_get_grid_field.argtypes = [vp, ctypes.c_int, ctypes.c_int, fp, sz]
         L.tild_engine_get_step.argtypes = [vp, ip]
+        L.tild_engine_get_state.argtypes = [vp, ip, fp, sz]
+        L.tild_engine_set_state.argtypes = [vp, ctypes.c_int, fp, sz]
         L.tild_engine_synchronize.argtypes = [vp]
         L.tild_engine_get_bins.argtypes = [vp, ip, sz]
         L.tild_engine_profile.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
@@ -328,6 +331,18 @@ class Engine:
         s = ctypes.c_int()
         self._chk(lib().tild_engine_get_step(self._h, ctypes.byref(s)), "get_step")
         return s.value
+
+    def state(self):
+        """(step, GJF displacement x - xo per site): with positions() and the seed, an
+        exact restart point."""
+        s = ctypes.c_int()
+        a = np.zeros((self.ns, self.dim), dtype=np.float32)
+        self._chk(lib().tild_engine_get_state(self._h, ctypes.byref(s), _fp(a), a.size), "get_state")
+        return s.value, a
+
+    def set_state(self, step, xo_disp):
+        a = np.ascontiguousarray(xo_disp, dtype=np.float32).reshape(self.ns, self.dim)
+        self._chk(lib().tild_engine_set_state(self._h, int(step), _fp(a), a.size), "set_state")
 
     def synchronize(self):
         self._chk(lib().tild_engine_synchronize(self._h), "synchronize")

@@ -160,6 +160,14 @@ tild_status tild_engine_get_bins(tild_engine* e, int* base, size_t n);
    spatial reorders (an event pair around them), amortised over the steps. */
 tild_status tild_engine_profile(tild_engine* e, int nsteps, double* phase_ms, int nphase);
 tild_status tild_engine_get_step(tild_engine* e, int* step);
+/* Exact restart.  The reference's resume frame holds positions only
+   (config_utils.cu:405-454, written at main.cu:205-207), so a GJF run does not resume
+   exactly there.  Here the thermal noise is keyed by (seed, site, step), so the step
+   counter and the GJF displacement x - xo of every site (ns*Dim floats, site order;
+   unused for EM sites) restore a run bit for bit together with the positions
+   (tild_engine_get/set_positions) and the same seed.  set_state after set_positions. */
+tild_status tild_engine_get_state(tild_engine* e, int* step, float* xo_disp, size_t n);
+tild_status tild_engine_set_state(tild_engine* e, int step, const float* xo_disp, size_t n);
 
 /* Cell-list neighbour lists of a site group at the current positions
    (NList::Initialize + AssignCells + BuildNList, nlist.cu:13-228; the reference builds
