@@ -188,3 +188,68 @@ def serve_job(dirpath, idx, seed=1):
         text = melt(dirpath, nch, 20, (L, L, L), (n, n, n), styles=styles, seed=seed * 1000 + idx,
                     max_steps=steps)
     return text, steps
+
+
+# ------------------------------------------------------------------ command line
+def main(argv=None):
+    """Write an initial configuration (`input.data`) and its input deck (`input`) into a
+    directory, ready for `tild-run -in input` run from there:
+
+        python tild_gen.py melt OUT --chains 400 --N 20 --box 10 10 10 --grid 20 20 20 \\
+            --fA 0.5 --chi 2.0 --integrator GJF --delt 0.01 --max-steps 5000 --seed 7
+        python tild_gen.py melt OUT --dim 2 --chains 300 --N 20 --box 20 20 --grid 40 40
+        python tild_gen.py coacervate OUT --chains-each 100 --N 25 --counter-pairs 250 \\
+            --total 8100 --box 14 14 14 --grid 28 28 28
+        python tild_gen.py lamellar640k OUT
+
+    A melt is AB diblocks (random-walk chains, fraction fA of A) with Gaussian styles
+    A-A and B-B of prefactor 1 and A-B of prefactor --chi (width 0.5)."""
+    import argparse
+    ap = argparse.ArgumentParser(prog="tild_gen", description=main.__doc__.split("\n\n")[0])
+    sub = ap.add_subparsers(dest="kind", required=True)
+    m = sub.add_parser("melt")
+    m.add_argument("out")
+    m.add_argument("--chains", type=int, required=True)
+    m.add_argument("--N", type=int, required=True)
+    m.add_argument("--box", type=float, nargs="+", required=True)
+    m.add_argument("--grid", type=int, nargs="+", required=True)
+    m.add_argument("--dim", type=int, default=3, choices=(2, 3))
+    m.add_argument("--fA", type=float, default=0.5)
+    m.add_argument("--chi", type=float, default=2.0)
+    m.add_argument("--integrator", default="EM", choices=("EM", "GJF"))
+    m.add_argument("--delt", type=float, default=0.005)
+    m.add_argument("--max-steps", type=int, default=1000)
+    m.add_argument("--pmeorder", type=int, default=1)
+    m.add_argument("--seed", type=int, default=1)
+    c = sub.add_parser("coacervate")
+    c.add_argument("out")
+    c.add_argument("--chains-each", type=int, default=100)
+    c.add_argument("--N", type=int, default=25)
+    c.add_argument("--counter-pairs", type=int, default=250)
+    c.add_argument("--total", type=int, default=8100)
+    c.add_argument("--box", type=float, nargs=3, default=(14.0, 14.0, 14.0))
+    c.add_argument("--grid", type=int, nargs=3, default=(28, 28, 28))
+    c.add_argument("--max-steps", type=int, default=2000)
+    c.add_argument("--seed", type=int, default=1)
+    w = sub.add_parser("lamellar640k")
+    w.add_argument("out")
+    w.add_argument("--seed", type=int, default=1)
+    a = ap.parse_args(argv)
+    if a.kind == "melt":
+        if len(a.box) != a.dim or len(a.grid) != a.dim:
+            ap.error("--box and --grid take one value per dimension")
+        box = tuple(a.box) + ((1.0,) if a.dim == 2 else ())
+        styles = [("gaussian", 1, 1, 1.0, 0.5), ("gaussian", 2, 2, 1.0, 0.5), ("gaussian", 1, 2, a.chi, 0.5)]
+        melt(a.out, a.chains, a.N, box, tuple(a.grid), fA=a.fA, dim=a.dim, styles=styles,
+             integrator=a.integrator, delt=a.delt, seed=a.seed, max_steps=a.max_steps, pmeorder=a.pmeorder)
+    elif a.kind == "coacervate":
+        coacervate(a.out, a.chains_each, a.N, a.counter_pairs, a.total, tuple(a.box), tuple(a.grid),
+                   seed=a.seed, max_steps=a.max_steps)
+    else:
+        lamellar_melt_640k(a.out, seed=a.seed)
+    print(f"wrote {pathlib.Path(a.out) / 'input'} and {pathlib.Path(a.out) / 'input.data'}")
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -155,3 +155,29 @@ def test_fieldphase_decks(tmp_path):
     with pytest.raises(tild.TildError) as ei:   # 2D cylinders must run along the normal
         tild.System(text2 + "n_fieldphases 1\nfieldphase 1 2 0.5 2 0 1\n", d2)
     assert ei.value.status == 2
+
+
+def test_generator_command_line_writes_runnable_decks(tmp_path):
+    # the generator's command line writes an input deck and a data file that both the
+    # engine's parser and the oracle read, with the requested sizes
+    import subprocess
+    import sys
+    gen = pathlib.Path(__file__).resolve().parents[1] / "gpu-tild_amd" / "tild_gen.py"
+    runs = [
+        (["melt", "m3", "--chains", "40", "--N", "10", "--box", "6", "6", "6", "--grid", "12", "12", "12",
+          "--integrator", "GJF", "--delt", "0.01", "--chi", "3.0"], 400, 3),
+        (["melt", "m2", "--dim", "2", "--chains", "30", "--N", "10", "--box", "10", "8", "--grid", "20", "16"], 300, 2),
+        (["coacervate", "c", "--chains-each", "12", "--N", "8", "--counter-pairs", "20", "--total", "600",
+          "--box", "7", "7", "8", "--grid", "16", "16", "18"], 600, 3),
+    ]
+    for args, ns, dim in runs:
+        out = tmp_path / args[1]
+        subprocess.run([sys.executable, str(gen), args[0], str(out)] + args[2:], check=True,
+                       capture_output=True, timeout=60)
+        text = (out / "input").read_text()
+        s = tild.System(text, out)
+        assert s.ns == ns and s.dim == dim, args
+        o = O.System(text, out, seed=1)
+        np.testing.assert_array_equal(o.positions().astype(np.float32), s.positions().astype(np.float32))
+    assert "integrator all GJF" in (tmp_path / "m3" / "input").read_text()
+    assert "gaussian 1 2 3.0 0.5" in (tmp_path / "m3" / "input").read_text()
