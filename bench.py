@@ -238,9 +238,9 @@ def run(args):
         value = ns * args.steps / wall
 
         prof = eng.profile(args.profile_steps)
-        # automatic reorder interval of this workload: 40 steps
+        # automatic reorder interval of this workload: 60 steps
         pb = phase_bytes(ns, D, M, Mh, T, T, True, False, T * (T + 1) // 2,
-                         args.reorder_every if args.reorder_every > 0 else (40 if args.reorder_every == 0 else args.steps))
+                         args.reorder_every if args.reorder_every > 0 else (60 if args.reorder_every == 0 else args.steps))
         dom = max((k for k in prof if pb[k] > 0), key=lambda k: prof[k])   # dominant kernel
         ach = pb[dom] / (prof[dom] * 1e-3) / 1e9
         step_bytes = step_bytes_minimal(ns, D, M, T, True, False)

@@ -383,9 +383,10 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
     }
 
     // spatial reorder: worthwhile once the gather records no longer fit one XCD's L2;
-    // every 40 steps measured best on the 640,000-bead melt (20: -0.8 %, 80: -0.3 %)
+    // every 60 steps measured best on the 640,000-bead melt (+0.45 % over 40; 20 and
+    // 80 lose, 80 because the gather locality decays)
     R.reorder_every = reorder_every_ >= 0 ? reorder_every_
-                                          : ((size_t)h.nf * M * 3 * sizeof(float) > ((size_t)8 << 20) ? 40 : 0);
+                                          : ((size_t)h.nf * M * 3 * sizeof(float) > ((size_t)8 << 20) ? 60 : 0);
     if (R.reorder_every > 0) {
         const uint32_t maxkey = (uint32_t)(h.Ny * h.Nz - 1);
         R.sort_bits = 1;

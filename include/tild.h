@@ -84,7 +84,7 @@ typedef struct {
     int spread_mode;     /* 0 auto, 1 global fixed-point atomics, 2 LDS-tiled sorted bins */
     int tile_target;     /* tiled spread: aim for about this many tiles (0 = 512) */
     int tile_by, tile_bz;/* tiled spread: force a By x Bz tile shape (0 = free) */
-    int reorder_every;   /* spatial reorder of the particle arrays: 0 automatic (every 40
+    int reorder_every;   /* spatial reorder of the particle arrays: 0 automatic (every 60
                             steps once the gather records outgrow L2), n > 0 every n
                             steps, < 0 never.  Results do not depend on it. */
     int fft_mid_elems;   /* FFT block geometry, 0 = default: complex values per block of */
