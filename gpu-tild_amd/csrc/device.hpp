@@ -77,6 +77,7 @@ struct RepDev {
     float2* Gk;                // nf*dim*Mh half spectra of the gathered fields
     float* G3;                 // nf*M*3 gather records (g_x, g_y, g_z), zero where rho <= 0
     int g3_off32;              // 1: nf*M*12 bytes of records fit 32-bit byte offsets
+    int grid_off32;            // 1: every grid buffer (rho, H, Gk, G3) fits 32-bit byte offsets
     const float* Ctab;         // coupling tables C_IJ/M (I <= J) + charge table, blocked as
                                // [column block][last-axis index][row][nl_last columns]
     int ctab_rows;             // rows of Ctab: T(T+1)/2 (+1 with charges)
@@ -261,6 +262,14 @@ struct FastDiv {
     __device__ __forceinline__ explicit FastDiv(int dd) : d(dd), inv(__builtin_amdgcn_rcpf((float)dd)) {}
     __device__ __forceinline__ int div(int n, int& rem) const { return div_small(n, d, inv, rem); }
 };
+
+// Element i of a buffer addressed as its base plus a 32-bit byte offset (buffers below
+// 4 GiB): the uniform base stays in scalar registers and each lane carries one 32-bit
+// offset, where a 64-bit index costs 64-bit multiply-adds per element.
+template <class T>
+__device__ __forceinline__ T* at32(T* base, uint32_t i) {
+    return (T*)((char*)base + i * (uint32_t)sizeof(T));
+}
 
 // Mixed-radix digits of a strided loop index: i = (hi * A + mid) * B + lo for
 // i = t, t + S, t + 2S, ...  One division pair at set-up; each step adds the digits of

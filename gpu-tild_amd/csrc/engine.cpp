@@ -325,6 +325,7 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
     if (int rc = dalloc(R, &h.Gk, nf * D * Mh, err, true)) return rc;
     if (int rc = dalloc(R, &h.G3, nf * M * 3, err, true)) return rc;
     h.g3_off32 = nf * M * 12 <= (size_t)0xFFFFFFFFu ? 1 : 0;
+    h.grid_off32 = h.g3_off32 && (size_t)nf * D * Mh * 8 <= (size_t)0xFFFFFFFFu ? 1 : 0;
     R.nblk_part = (S.ns + 255) / 256;
     if (int rc = dalloc(R, &R.d_bpart, 7 * (size_t)R.nblk_part, err, true)) return rc;
     h.bpart = R.d_bpart;

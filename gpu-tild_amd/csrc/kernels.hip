@@ -183,13 +183,13 @@ __device__ __forceinline__ void couple_nodes(gmem<const float> cb, int nrows, in
     for (int task = threadIdx.x; task < nb * N; task += NT, gc.step()) {
         const int b = gc.lo, n = gc.hi;
         const int row = n * ldb + b, wrow = n * ldw + b;
-        gmem<const float> cn = cb + (size_t)n * nrows * nlc + b;
+        gmem<const float> cn = cb + (n * nrows * nlc + b);   // offsets inside the block's run
         float c[NR];
 #pragma unroll
-        for (int k = 0; k < NR; ++k) c[k] = cn[(size_t)k * nlc];
-        const float cq = charged ? cn[(size_t)ctab_charge * nlc] : 0.f;
+        for (int k = 0; k < NR; ++k) c[k] = cn[k * nlc];
+        const float cq = charged ? cn[ctab_charge * nlc] : 0.f;
         for (int j = 0; j < r.nramp; ++j) {   // uniform trip count
-            const float v = rw[j] * cn[(size_t)(r.ramp_row0 + j) * nlc];
+            const float v = rw[j] * cn[(r.ramp_row0 + j) * nlc];
 #pragma unroll
             for (int k = 0; k < NR; ++k) c[k] += r.ramp_pair[j] == k ? v : 0.f;
         }
@@ -250,10 +250,13 @@ __global__ void __launch_bounds__(NT) k_last_fused(const RepDev* __restrict__ re
     float* rw = (float*)(E1 + (size_t)N * ldg);          // ramp weights of this step
     if ((int)threadIdx.x < r.nramp) rw[threadIdx.x] = ramp_weight(r, threadIdx.x, r.step + koff);
     const FastDiv dnb(nb), dnf(nf);
+    const bool o32 = r.grid_off32;   // field offsets fit 32-bit byte offsets
     Digits3 gl(threadIdx.x, NT, dnb, dnf);
     for (int task = threadIdx.x; task < nlines * N; task += NT, gl.step()) {
         const int b = gl.lo, f = gl.mid, n = gl.hi;
-        E0[n * ldb + f * nb + b] = gld(r.H + (size_t)f * r.Mh + (size_t)n * Q + q0 + b);
+        E0[n * ldb + f * nb + b] =
+            o32 ? gld(at32((const float2*)r.H, (uint32_t)f * (uint32_t)r.Mh + (uint32_t)(n * Q + q0 + b)))
+                : gld(r.H + (size_t)f * r.Mh + (size_t)n * Q + q0 + b);
     }
     __syncthreads();
     const float2* A = lds_fft<NT>(E0, E1, ldb, nlines, r.ax[la]);   // spectra rho_f(k)
@@ -278,16 +281,16 @@ __global__ void __launch_bounds__(NT) k_last_fused(const RepDev* __restrict__ re
             for (int J = 0; J < T; ++J) {
                 const int lo = I < J ? I : J, hi = I < J ? J : I;
                 const int pidx = lo * T - lo * (lo - 1) / 2 + (hi - lo);
-                float c = cb[((size_t)n * nrows + pidx) * nlc + b];
+                float c = cb[(n * nrows + pidx) * nlc + b];
                 for (int j = 0; j < r.nramp; ++j)
-                    if (r.ramp_pair[j] == pidx) c += rw[j] * cb[((size_t)n * nrows + r.ramp_row0 + j) * nlc + b];
+                    if (r.ramp_pair[j] == pidx) c += rw[j] * cb[(n * nrows + r.ramp_row0 + j) * nlc + b];
                 const float2 rh = A[row + J * nb];
                 acc.x += c * rh.x; acc.y += c * rh.y;
             }
             Wb[wrow + I * nb] = acc;
         }
         if (nf > T) {
-            const float c = cb[((size_t)n * nrows + r.ctab_charge) * nlc + b];
+            const float c = cb[(n * nrows + r.ctab_charge) * nlc + b];
             const float2 rh = A[row + T * nb];
             Wb[wrow + T * nb] = make_float2(c * rh.x, c * rh.y);
         }
@@ -322,7 +325,9 @@ __global__ void __launch_bounds__(NT) k_last_fused(const RepDev* __restrict__ re
             const int b = gs.lo, fo = gs.mid, n = gs.hi;
             const int oo = no == 2 ? (fo & 1) : 0, f = no == 2 ? (fo >> 1) : fo;   // no is 1 or 2
             const int slot = o0 + oo == 0 ? 0 : slot1;
-            gst(r.Gk + (size_t)(f * D + slot) * r.Mh + (size_t)n * Q + q0 + b, conjf2(res[n * ldg + fo * nb + b]));
+            const float2 v = conjf2(res[n * ldg + fo * nb + b]);
+            if (o32) gst(at32(r.Gk, (uint32_t)(f * D + slot) * (uint32_t)r.Mh + (uint32_t)(n * Q + q0 + b)), v);
+            else gst(r.Gk + (size_t)(f * D + slot) * r.Mh + (size_t)n * Q + q0 + b, v);
         }
         __syncthreads();
     }
@@ -356,7 +361,7 @@ k_mid_grad(const RepDev* __restrict__ reps, const WorkItem* __restrict__ wmap) {
     for (int task = threadIdx.x; task < nb * N; task += NT, gl.step()) {
         const int b = gl.lo, n = gl.hi;
         const float ky = 2 * n == N ? 0.f : r.kfac[1] * (float)(2 * n < N ? n : n - N);
-        const float2 u = gld(U + (size_t)n * Nxh + b), v = gld(V + (size_t)n * Nxh + b);
+        const float2 u = gld(U + (n * Nxh + b)), v = gld(V + (n * Nxh + b));
         // lines of the conj(FFT(conj)) inverse: conj(u), conj(-i ky u), conj(v)
         lds[n * ldb + b] = make_float2(u.x, -u.y);
         lds[n * ldb + nb + b] = make_float2(ky * u.y, ky * u.x);
@@ -368,7 +373,7 @@ k_mid_grad(const RepDev* __restrict__ reps, const WorkItem* __restrict__ wmap) {
     for (int task = threadIdx.x; task < 3 * nb * N; task += NT, gs.step()) {
         const int b = gs.lo, sl = gs.mid, n = gs.hi;
         float2* out = sl == 0 ? U : (sl == 1 ? Y : V);
-        gst(out + (size_t)n * Nxh + b, conjf2(res[n * ldb + sl * nb + b]));
+        gst(out + (n * Nxh + b), conjf2(res[n * ldb + sl * nb + b]));
     }
 }
 
@@ -379,6 +384,29 @@ k_mid_grad(const RepDev* __restrict__ reps, const WorkItem* __restrict__ wmap) {
 // dropped, which is the Hermitian projection the reference's Re(C2C inverse) takes.
 // RECORDS: output float4 (g_0, g_1, g_2, rho_f) per field and node (fields are lines
 // f*dim + d of Gk); otherwise plain real rows to out_override.
+// The gather records of one block of the inverse x pass (see k_x_c2r): D lines per
+// field unrolled, buffers addressed by 32-bit offsets; no field phases.
+template <int D, int NT>
+__device__ __forceinline__ void c2r_records(const RepDev& r, const float2* res, int ldb, int nc, int rows, int row0,
+                                            int N, const FastDiv& dN, const FastDiv& dnf) {
+    for (int task = threadIdx.x; task < rows * r.nf * N; task += NT) {
+        int x, f;
+        const int rr = dnf.div(dN.div(task, x), f);
+        float g[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int line = f * D + d;
+            const float2 z = res[x * ldb + rr * nc + line / 2];
+            g[d] = (line & 1) ? -z.y : z.x;      // conj of the forward result
+        }
+        const uint32_t fn = (uint32_t)f * (uint32_t)r.M + (uint32_t)(row0 + rr) * (uint32_t)N + (uint32_t)x;
+        // the gather skips nodes with rho <= 0 (device_grid_utils.cu:424-452): a zero
+        // record adds nothing
+        if (f < r.ntypes && !(G(at32((const float*)r.rho, fn))[0] > 0.f)) g[0] = g[1] = g[2] = 0.f;
+        gst3((float*)((char*)r.G3 + ((fn << 3) + (fn << 2))), g[0], g[1], g[2]);   // 12 fn bytes
+    }
+}
+
 template <bool RECORDS, int NT>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) k_x_c2r(const RepDev* __restrict__ reps,
                                                        const WorkItem* __restrict__ wmap,
@@ -408,13 +436,21 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) k_
     // partner N - kk
     // (about two tasks per thread: a division pair each is cheaper than the set-up of
     // digit counters here)
+    const bool o32 = RECORDS && r.grid_off32;
     for (int task = threadIdx.x; task < nb * Nxh; task += NT) {
         int kk, j;
         const int b = dNxh.div(task, kk);
         const int row = row0 + dnc.div(b, j);
         const int la = 2 * j, lb = la + 1;
-        float2 A = gld(Gin + ((size_t)la * rows_per_field + row) * Nxh + kk);
-        float2 Bv = lb < nlines ? gld(Gin + ((size_t)lb * rows_per_field + row) * Nxh + kk) : make_float2(0.f, 0.f);
+        float2 A, Bv = make_float2(0.f, 0.f);
+        if (o32) {
+            const uint32_t ia = ((uint32_t)la * (uint32_t)rows_per_field + (uint32_t)row) * (uint32_t)Nxh + (uint32_t)kk;
+            A = gld(at32(Gin, ia));
+            if (lb < nlines) Bv = gld(at32(Gin, ia + (uint32_t)rows_per_field * (uint32_t)Nxh));
+        } else {
+            A = gld(Gin + ((size_t)la * rows_per_field + row) * Nxh + kk);
+            if (lb < nlines) Bv = gld(Gin + ((size_t)lb * rows_per_field + row) * Nxh + kk);
+        }
         const bool self = kk == 0 || (even && 2 * kk == N);   // self-conjugate column
         if (RECORDS) {
             const float kx = self && kk != 0 ? 0.f : r.kfac[0] * (float)kk;
@@ -429,7 +465,12 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) k_
     }
     __syncthreads();
     const float2* res = lds_fft<NT>(lds, lds + (size_t)N * ldb, ldb, nb, r.ax[0]);
-    if (RECORDS) {
+    if (RECORDS && o32 && !r.FPtab) {
+        // records of buffers below 4 GiB: dimension unrolled, 32-bit offsets
+        const FastDiv dnf(r.nf);
+        if (r.dim == 3) c2r_records<3, NT>(r, res, ldb, nc, rows, row0, N, dN, dnf);
+        else c2r_records<2, NT>(r, res, ldb, nc, rows, row0, N, dN, dnf);
+    } else if (RECORDS) {
         const int D = r.dim;
         const FastDiv dnf(r.nf);
         for (int task = threadIdx.x; task < rows * r.nf * N; task += NT) {

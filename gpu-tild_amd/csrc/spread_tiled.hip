@@ -338,7 +338,9 @@ __global__ void __launch_bounds__(kDepositThreads) k_tile_deposit(const RepDev* 
             }
             const double inv = f < T ? r.fix_inv : r.fixq_inv;
             v[h] = (float)((double)(long long)patch[(size_t)rr * Nx + n] * inv);
-            G(r.rho)[(size_t)f * r.M + ((size_t)(z0 + lz) * r.Ny + (y0 + ly)) * Nx + n] = v[h];
+            const int node = ((z0 + lz) * r.Ny + (y0 + ly)) * Nx + n;
+            if (r.grid_off32) G(at32(r.rho, (uint32_t)f * (uint32_t)r.M + (uint32_t)node))[0] = v[h];
+            else G(r.rho)[(size_t)f * r.M + node] = v[h];
         }
         if (!r.plane_fwd) fbuf[n * ldb + j] = make_float2(v[0], v[1]);
     }
