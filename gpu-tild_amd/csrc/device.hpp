@@ -62,6 +62,7 @@ struct RepDev {
           gjf_noise[kMaxTypes];
     float* fout;            // ns*dim (optional)
     int* pid;               // ns: original site id of each slot (noise key, host order)
+    int *rsrc, *rorder, *rinv;   // ns each: scratch of the tiled spatial reorder
     // spare buffers of the periodic spatial reorder, swapped with the live arrays
     float* XO_alt;
     int* tp_alt;

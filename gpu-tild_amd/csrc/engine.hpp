@@ -31,7 +31,7 @@ constexpr int kProfPhases = 9;   // claim, deposit, forward plane, last_fused, y
 // skip MIDY_G and C2R); PLANEF: x R2C + y FFT of tiled 3D replicas with even Ny
 // (their deposit writes densities only and they skip MIDY_H).
 enum Phase { PH_PART = 0, PH_SPREADA, PH_XA, PH_XP, PH_SCAN, PH_TILE, PH_MIDY_H, PH_MIDY_G,
-             PH_MIDZ_H, PH_PY, PH_LAST, PH_C2R, PH_PLANEF, PH_TCOUNT, PH_KPROPS, PH_COUNT };
+             PH_MIDZ_H, PH_PY, PH_LAST, PH_C2R, PH_PLANEF, PH_TCOUNT, PH_KPROPS, PH_RTILE, PH_COUNT };
 
 struct PhaseMap {
     std::vector<WorkItem> host;
@@ -160,6 +160,9 @@ private:
     // spatial reorder of one replica; after_step: the last step's tile binning matches
     // the current slots and is reused
     int reorder(int id, std::string& err, bool after_step = false);
+    // the after-step reorder of several tiled replicas at once (one launch per kernel)
+    int reorder_set(const std::vector<int>& ids, std::string& err);
+    void mirror_reorder(Replica& R);   // the host copy of the descriptor follows the swaps
     int run_steps(SetMaps& m, const std::vector<int>& ids, int n, bool use_graph, std::string& err);
     int capture_steps(SetMaps& m, int nsteps, hipGraphExec_t* out, std::string& err);
     int tile_regions(int id, bool bin_only, std::string& err);

@@ -143,5 +143,10 @@ hipError_t launch_reorder(hipStream_t s, RepDev* reps, int rep, int ns, int bits
 // Tiled replicas: each tile's beads (from an exact binning, tkey and toff) sorted by
 // site id, then the same permutation.
 hipError_t launch_reorder_tiles(hipStream_t s, RepDev* reps, int rep, int ns, int ntiles, ReorderBufs& b);
+// The same for a set of tiled replicas at once: `beads` maps 256-bead blocks, `tiles`
+// one block per tile and `reps1` one block per replica (scratch in RepDev::rsrc etc.).
+hipError_t launch_reorder_tiles_set(const LaunchCtx& beads, const LaunchCtx& tiles, const LaunchCtx& reps1);
+// Bucket regions (k_tile_regions) of every replica of a set, one block each.
+hipError_t launch_tile_regions_set(const LaunchCtx& reps1);
 
 }  // namespace tild

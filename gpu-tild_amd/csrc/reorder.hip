@@ -53,12 +53,8 @@ __global__ void __launch_bounds__(256) k_reorder_inv(int ns, const int* __restri
 
 // Gather every per-bead array into the spare buffers in the new order.  Positions
 // go to the free half of the position double buffer.
-__global__ void __launch_bounds__(256) k_reorder_apply(const RepDev* __restrict__ reps, int rep,
-                                                       const int* __restrict__ order,
-                                                       const int* __restrict__ inv) {
-    const RepDev& r = reps[rep];
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= r.ns) return;
+__device__ __forceinline__ void reorder_apply(const RepDev& r, int p, const int* __restrict__ order,
+                                              const int* __restrict__ inv) {
     const int D = r.dim, i = order[p];
     const float* __restrict__ x = r.X[r.step & 1];
     float* __restrict__ xn = r.X[(r.step + 1) & 1];
@@ -77,6 +73,14 @@ __global__ void __launch_bounds__(256) k_reorder_apply(const RepDev* __restrict_
     }
 }
 
+__global__ void __launch_bounds__(256) k_reorder_apply(const RepDev* __restrict__ reps, int rep,
+                                                       const int* __restrict__ order,
+                                                       const int* __restrict__ inv) {
+    const RepDev& r = reps[rep];
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p < r.ns) reorder_apply(r, p, order, inv);
+}
+
 template <class T, class U>
 __device__ inline void swap_ptr(T*& a, U*& b) {
     T* t = a;
@@ -85,8 +89,7 @@ __device__ inline void swap_ptr(T*& a, U*& b) {
 }
 
 // Make the reordered arrays live (one thread; stream-ordered after the gather).
-__global__ void k_reorder_swap(RepDev* reps, int rep) {
-    RepDev& r = reps[rep];
+__device__ __forceinline__ void reorder_swap(RepDev& r) {
     swap_ptr(r.X[0], r.X[1]);
     swap_ptr(r.XO, r.XO_alt);
     swap_ptr(r.tp, r.tp_alt);
@@ -97,6 +100,8 @@ __global__ void k_reorder_swap(RepDev* reps, int rep) {
     swap_ptr(r.bond_t, r.bond_t_alt);
 }
 
+__global__ void k_reorder_swap(RepDev* reps, int rep) { reorder_swap(reps[rep]); }
+
 // ---- tiled replicas: reuse the last step's bucket order -----------------------------
 // The step's binning left every bead's bucket (tile, halo class) and slot in tkey and
 // the bucket offsets in toff, so the beads of tile t occupy [toff[4t], toff[4t+4]) of
@@ -104,24 +109,25 @@ __global__ void k_reorder_swap(RepDev* reps, int rep) {
 // tile's beads by original site id makes the new order deterministic.
 
 // src[bucket position] = slot
-__global__ void __launch_bounds__(256) k_reorder_src(const RepDev* __restrict__ reps, int rep, int* __restrict__ src) {
-    const RepDev& r = reps[rep];
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= r.ns) return;
+__device__ __forceinline__ void reorder_src(const RepDev& r, int i, int* __restrict__ src) {
     const long long kk = G((const long long*)r.tkey)[i];
     const int b = (int)(unsigned)(kk & 0xffffffffll), s = (int)(kk >> 32);
     G(src)[G((const int*)r.toff)[b] + s] = i;
 }
 
+__global__ void __launch_bounds__(256) k_reorder_src(const RepDev* __restrict__ reps, int rep, int* __restrict__ src) {
+    const RepDev& r = reps[rep];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < r.ns) reorder_src(r, i, src);
+}
+
 constexpr int kTileSortMax = 2048;   // beads a tile sorts in LDS; larger tiles rank in place
 
 // One block per tile: order[tile range] = the tile's slots sorted by site id.
-__global__ void __launch_bounds__(1024) k_reorder_tile_sort(const RepDev* __restrict__ reps, int rep,
-                                                           const int* __restrict__ src, int* __restrict__ order) {
+__device__ __forceinline__ void reorder_tile_sort(const RepDev& r, int t, const int* __restrict__ src,
+                                                  int* __restrict__ order) {
     __shared__ int key[kTileSortMax];
     __shared__ int val[kTileSortMax];
-    const RepDev& r = reps[rep];
-    const int t = blockIdx.x;
     const int a = G((const int*)r.toff)[4 * t], L = G((const int*)r.toff)[4 * t + 4] - a;
     gmem<const int> pid = G((const int*)r.pid);
     if (L > kTileSortMax) {
@@ -165,6 +171,40 @@ __global__ void __launch_bounds__(1024) k_reorder_tile_sort(const RepDev* __rest
     for (int j = threadIdx.x; j < L; j += 1024) G(order)[a + j] = val[j];
 }
 
+__global__ void __launch_bounds__(1024) k_reorder_tile_sort(const RepDev* __restrict__ reps, int rep,
+                                                           const int* __restrict__ src, int* __restrict__ order) {
+    reorder_tile_sort(reps[rep], blockIdx.x, src, order);
+}
+
+// ---- sets of tiled replicas: the same kernels over work maps, scratch in RepDev
+__global__ void __launch_bounds__(256) k_reorder_src_set(const RepDev* __restrict__ reps, const WorkItem* __restrict__ wmap) {
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    const int i = wi.blk * 256 + threadIdx.x;
+    if (i < r.ns) reorder_src(r, i, r.rsrc);
+}
+__global__ void __launch_bounds__(1024) k_reorder_tile_sort_set(const RepDev* __restrict__ reps,
+                                                               const WorkItem* __restrict__ wmap) {
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    reorder_tile_sort(r, wi.blk, r.rsrc, r.rorder);
+}
+__global__ void __launch_bounds__(256) k_reorder_inv_set(const RepDev* __restrict__ reps, const WorkItem* __restrict__ wmap) {
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    const int p = wi.blk * 256 + threadIdx.x;
+    if (p < r.ns) G(r.rinv)[G((const int*)r.rorder)[p]] = p;
+}
+__global__ void __launch_bounds__(256) k_reorder_apply_set(const RepDev* __restrict__ reps, const WorkItem* __restrict__ wmap) {
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    const int p = wi.blk * 256 + threadIdx.x;
+    if (p < r.ns) reorder_apply(r, p, r.rorder, r.rinv);
+}
+__global__ void __launch_bounds__(64) k_reorder_swap_set(RepDev* reps, const WorkItem* __restrict__ wmap) {
+    if (threadIdx.x == 0) reorder_swap(reps[wmap[blockIdx.x].rep]);
+}
+
 #define CHECK_LAUNCH() do { hipError_t e__ = hipGetLastError(); if (e__ != hipSuccess) return e__; } while (0)
 
 hipError_t launch_reorder_tiles(hipStream_t s, RepDev* reps, int rep, int ns, int ntiles, ReorderBufs& b) {
@@ -181,6 +221,21 @@ hipError_t launch_reorder_tiles(hipStream_t s, RepDev* reps, int rep, int ns, in
     hipLaunchKernelGGL(k_reorder_apply, dim3(nb), dim3(256), 0, s, reps, rep, order, inv);
     CHECK_LAUNCH();
     hipLaunchKernelGGL(k_reorder_swap, dim3(1), dim3(1), 0, s, reps, rep);
+    CHECK_LAUNCH();
+    return hipSuccess;
+}
+
+hipError_t launch_reorder_tiles_set(const LaunchCtx& beads, const LaunchCtx& tiles, const LaunchCtx& reps1) {
+    if (!beads.n || !tiles.n || !reps1.n) return hipSuccess;
+    hipLaunchKernelGGL(k_reorder_src_set, dim3(beads.n), dim3(256), 0, beads.stream, beads.reps, beads.wmap);
+    CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_reorder_tile_sort_set, dim3(tiles.n), dim3(1024), 0, tiles.stream, tiles.reps, tiles.wmap);
+    CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_reorder_inv_set, dim3(beads.n), dim3(256), 0, beads.stream, beads.reps, beads.wmap);
+    CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_reorder_apply_set, dim3(beads.n), dim3(256), 0, beads.stream, beads.reps, beads.wmap);
+    CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_reorder_swap_set, dim3(reps1.n), dim3(64), 0, reps1.stream, const_cast<RepDev*>(reps1.reps), reps1.wmap);
     CHECK_LAUNCH();
     return hipSuccess;
 }

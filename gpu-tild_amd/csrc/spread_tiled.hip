@@ -141,9 +141,8 @@ __global__ void __launch_bounds__(1024) k_tile_scan(const RepDev* __restrict__ r
 // slots (by default a quarter plus 16), so that beads can drift between buckets for
 // the steps until the next sizing without overflowing; tbase is the exclusive scan of
 // the sizes.
-__global__ void __launch_bounds__(1024) k_tile_regions(const RepDev* __restrict__ reps, int rep) {
+__device__ __forceinline__ void tile_regions_body(const RepDev& r) {
     __shared__ int wsum[32];
-    const RepDev& r = reps[rep];
     const int nb = r.nbuckets;
     gmem<const int> off = G((const int*)r.toff);
     gmem<int> cap = G(r.tcap);
@@ -172,6 +171,14 @@ __global__ void __launch_bounds__(1024) k_tile_regions(const RepDev* __restrict_
         run += total;
     }
     if (threadIdx.x == 0) base[nb] = run;
+}
+
+__global__ void __launch_bounds__(1024) k_tile_regions(const RepDev* __restrict__ reps, int rep) {
+    tile_regions_body(reps[rep]);
+}
+__global__ void __launch_bounds__(1024) k_tile_regions_set(const RepDev* __restrict__ reps,
+                                                          const WorkItem* __restrict__ wmap) {
+    tile_regions_body(reps[wmap[blockIdx.x].rep]);
 }
 
 // Deposit one record into the tile's fixed-point LDS rows [f][lz][ly][x]; stencil
@@ -388,6 +395,11 @@ hipError_t launch_tile_reset(const LaunchCtx& c, int koff) {
 }
 hipError_t launch_tile_regions(hipStream_t s, const RepDev* reps, int rep) {
     hipLaunchKernelGGL(k_tile_regions, dim3(1), dim3(1024), 0, s, reps, rep);
+    CHECK_LAUNCH(); return hipSuccess;
+}
+hipError_t launch_tile_regions_set(const LaunchCtx& c) {
+    if (!c.n) return hipSuccess;
+    hipLaunchKernelGGL(k_tile_regions_set, dim3(c.n), dim3(1024), 0, c.stream, c.reps, c.wmap);
     CHECK_LAUNCH(); return hipSuccess;
 }
 hipError_t launch_tile_deposit(const LaunchCtx& c, int koff) {

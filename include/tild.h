@@ -85,8 +85,9 @@ typedef struct {
     int tile_target;     /* tiled spread: aim for about this many tiles (0 = 512) */
     int tile_by, tile_bz;/* tiled spread: force a By x Bz tile shape (0 = free) */
     int reorder_every;   /* spatial reorder of the particle arrays: 0 automatic (every 60
-                            steps once the gather records outgrow L2), n > 0 every n
-                            steps, < 0 never.  Results do not depend on it. */
+                            steps once the gather records outgrow L2, or for tiled
+                            replicas of 4,096 beads or more), n > 0 every n steps,
+                            < 0 never.  Results do not depend on it. */
     int fft_mid_elems;   /* FFT block geometry, 0 = default: complex values per block of */
     int fft_last_lines;  /* the y pass, columns per block of the fused z pass, rows per  */
     int fft_c2r_rows;    /* block of the inverse x pass.  Results do not depend on them. */
