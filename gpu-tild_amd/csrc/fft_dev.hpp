@@ -193,6 +193,48 @@ __device__ __forceinline__ void fft_stage(const float2* __restrict__ src, float2
     }
 }
 
+// The same stage with the line length N and the span P of the earlier radices known
+// at compile time: k = i mod P divides by a constant, and m = N / R is a constant.
+// Same arithmetic, in the same order, as fft_stage.
+template <int R, int NT, int N, int P>
+__device__ __forceinline__ void fft_stage_c(const float2* __restrict__ src, float2* __restrict__ dst, int ldb, int nb,
+                                            const float2* __restrict__ tw) {
+    constexpr int m = N / R;
+    const int nbf = m * nb;
+    const FastDiv dnb(nb);
+    const int sst = m * ldb, dst_st = P * ldb;
+    for (int task = threadIdx.x; task < nbf; task += NT) {
+        int b;
+        const int i = dnb.div(task, b);
+        const int k = i % P;
+        const float2* sp = src + ((int)__umul24((unsigned)i, (unsigned)ldb) + b);
+        float2* op = dst + ((int)__umul24((unsigned)((i - k) * R + k), (unsigned)ldb) + b);
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = sp[r * sst];
+        if constexpr (P > 1) {
+            const float2* w = tw + k * (R - 1);
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], gld(w + r - 1));
+        }
+        Dft<R>::run(v);
+#pragma unroll
+        for (int q = 0; q < R; ++q) op[q * dst_st] = v[q];
+    }
+}
+
+// Two-stage transform with both radices known at compile time (N = R1 R2, the plans
+// of the 80- and 160-point axes): no plan is read and no radix switched on per stage.
+// Returns A, as lds_fft does for two stages.
+template <int R1, int R2, int NT>
+__device__ __forceinline__ float2* lds_fft2(float2* A, float2* B, int ldb, int nb, const AxisPlan& pl) {
+    fft_stage_c<R1, NT, R1 * R2, 1>(A, B, ldb, nb, pl.tw + pl.tw_off[0]);
+    __syncthreads();
+    fft_stage_c<R2, NT, R1 * R2, R1>(B, A, ldb, nb, pl.tw + pl.tw_off[1]);
+    __syncthreads();
+    return A;
+}
+
 // Forward FFT of nb lines held in A (element n of line b at A[n*ldb + b]), using B
 // as the ping-pong partner.  Requires N*nb <= kFftBudget and a synchronised block
 // (A fully written).  Returns the buffer holding the result (A or B); every stage
@@ -201,6 +243,10 @@ __device__ __forceinline__ void fft_stage(const float2* __restrict__ src, float2
 // NT = threads of the block.
 template <int NT = kFftThreads>
 __device__ __forceinline__ float2* lds_fft(float2* A, float2* B, int ldb, int nb, const AxisPlan& pl) {
+    if (pl.nst == 2) {   // the plans of 80 and 160 points (uniform branch)
+        if (pl.rad[0] == 10 && pl.rad[1] == 8) return lds_fft2<10, 8, NT>(A, B, ldb, nb, pl);
+        if (pl.rad[0] == 16 && pl.rad[1] == 10) return lds_fft2<16, 10, NT>(A, B, ldb, nb, pl);
+    }
     float2* src = A;
     float2* dst = B;
     int p = 1;
