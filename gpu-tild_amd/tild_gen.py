@@ -19,15 +19,17 @@ def _walks(rng, n_chains, N, box, dim, b=1.0):
     return np.mod(pos, box)
 
 
-def write_data(path, pos, types, mols, bonds, box, ntypes, dim, charges=None, nbond_types=1):
-    """Write a data file; pos (ns, dim), types/mols 1-based, bonds list of (i, j) 1-based."""
+def write_data(path, pos, types, mols, bonds, box, ntypes, dim, charges=None, nbond_types=1, masses=None):
+    """Write a data file; pos (ns, dim), types/mols 1-based, bonds list of (i, j) 1-based,
+    masses per type (default 1)."""
     ns = len(pos)
     lines = ["TILD synthetic configuration", "", f"{ns} atoms", f"{len(bonds)} bonds", "0 angles", "",
              f"{ntypes} atom types", f"{nbond_types} bond types", "0 angle types", ""]
     zhi = box[2] if dim == 3 else 1.0
     lines += [f"0.000000 {box[0]:.6f} xlo xhi", f"0.000000 {box[1]:.6f} ylo yhi",
               f"0.000000 {zhi:.6f} zlo zhi", "", "Masses", ""]
-    lines += [f"{t + 1} 1.000000" for t in range(ntypes)]
+    mass = [1.0] * ntypes if masses is None else list(masses)
+    lines += [f"{t + 1} {mass[t]:.6f}" for t in range(ntypes)]
     lines += ["", "Atoms", ""]
     out = "\n".join(lines) + "\n"
     coords = pos if dim == 3 else np.concatenate([pos, np.zeros((ns, 1))], axis=1)

@@ -2,27 +2,48 @@
  * ora_step.c -- FP64 oracle of the reference per-step routines.
  * TEST INFRASTRUCTURE ONLY (see tild_oracle.h).
  *
- * Each function names the reference routine it restates.  Parity status:
+ * Each function names the reference routine it restates.  Parity status (the tests
+ * that pin it: tests/test_oracle.py, tests/test_oracle_pins.py):
  *   derived constants      initialize.cu:32-48             pinned (Coacervate data.dat)
  *   get_k / unstack        device_array_utils.cu:9-82      pinned (data.dat; closed forms)
- *   spline weights         device_grid_utils.cu:3-49       pinned (partition of unity,
- *                                                          CIC exactness; order-4 bug kept)
- *   spread                 device_grid_utils.cu:232-343    pinned (data.dat; sum rule)
+ *   spline weights         device_grid_utils.cu:3-49       pinned (cardinal B-splines by
+ *                                                          Cox-de Boor in 3D and 2D; CIC
+ *                                                          exactness; order-4 bug kept)
+ *   spread                 device_grid_utils.cu:232-343    pinned (data.dat; sum rule 3D/2D)
  *   pair-style tables      pair_style_gaussian.cu:72-90, pair_style_erf.cu:122-157,
- *                          pair_style_gaussian_erf.cu:82-115  pinned (closed forms, k=0)
+ *                          pair_style_gaussian_erf.cu:82-115  pinned (real-space energies
+ *                                                          of the smoothed sphere / overlap)
  *   CalcForces             pair_style.cu:81-196            pinned (momentum conservation,
- *                                                          two-particle image sum, -dU/dx)
- *   CalcCharges            pair_style_charges.cu:21-111    pinned (data.dat Uelec path)
- *   gather                 device_grid_utils.cu:346-452    pinned (momentum conservation)
+ *                                                          two-particle -grad u in 3D and
+ *                                                          2D, exact for beads on nodes;
+ *                                                          -dU/dx convergence)
+ *   CalcCharges            pair_style_charges.cu:21-111    pinned (two-charge smeared
+ *                                                          Coulomb force; data.dat Uelec)
+ *   gather                 device_grid_utils.cu:346-452    pinned (the force pins above)
  *   d_bonds                device_bonds.cu:4-60            pinned (finite difference of
  *                                                          host_bonds energy; Newton III)
- *   EM / GJF               device_EM_integrator.cu:5-36,
- *                          device_GJF_integrator.cu:4-62   pinned (zero-noise closed
- *                                                          forms; noise values replace
- *                                                          cuRAND, see ora_util.c)
- *   CalcEnergy/CalcVirial  pair_style.cu:204-276           pinned (data.dat)
+ *   EM                     device_EM_integrator.cu:5-36    pinned: noise off, one step is
+ *                                                          explicit Euler x + delt f and a
+ *                                                          harmonic dimer decays as
+ *                                                          (1 - 4 k delt)^n; noise on, the
+ *                                                          dimer's bond variance is the
+ *                                                          discrete Ornstein-Uhlenbeck value
+ *                                                          and free diffusion gives 2 delt
+ *   GJF                    device_GJF_integrator.cu:4-62   pinned: noise off, a harmonic
+ *                                                          dimer follows the closed-form
+ *                                                          solution of the GJF recurrence
+ *                                                          (three mass / diffusivity pairs);
+ *                                                          noise on, <r^2> = 3/(2k) (GJF's
+ *                                                          exact harmonic sampling, D = 1)
+ *                                                          and the underdamped free MSD.
+ *                                                          Noise values replace cuRAND (see
+ *                                                          ora_util.c)
+ *   CalcEnergy/CalcVirial  pair_style.cu:204-276           pinned (data.dat; continuum pair
+ *                                                          energy and virial r_a F_b / (V D)
+ *                                                          for beads on nodes, 3D and 2D)
  *   host_bonds             bonds.cu:3-52                   pinned (data.dat Ubond, P)
  *   electrostatic energy   calc_properties.cu:106-116      pinned (data.dat Upe)
+ *   ramp Update            pair_style.cu:308-328           pinned (closed-form amplitude)
  *   NList                  nlist.cu:13-228                 pinned (brute-force O(n^2) sets)
  */
 #include "ora_internal.h"
@@ -89,9 +110,10 @@ static void style_tables(ora_sys* s, ora_style* st) {
     st->f_k = cz(2 * (size_t)M * D);
     st->f_r = cz((size_t)M * D);
     st->vir_k = cz(2 * (size_t)M * nP);
-    double kv[3];
     double A = st->A, s2 = st->sigma2, Rp = st->Rp;
+    #pragma omp parallel for schedule(static)
     for (int i = 0; i < M; ++i) {
+        double kv[3];
         double k2 = get_k(s, i, kv), u;
         if (st->kind == ORA_GAUSS) {                   /* pair_style_gaussian.cu:82-83 */
             u = A * exp(-k2 * s2 / 2.0);
@@ -113,12 +135,15 @@ static void style_tables(ora_sys* s, ora_style* st) {
     /* real-space force: Re IFFT(f_k_d), pair_style_gaussian.cu:34-38 */
     double* c = s->cpx1;
     for (int d = 0; d < D; ++d) {
+        #pragma omp parallel for schedule(static)
         for (int i = 0; i < M; ++i) { c[2 * i] = st->f_k[2 * (i * D + d)]; c[2 * i + 1] = st->f_k[2 * (i * D + d) + 1]; }
         ora_fft(c, D, s->Nx, +1);
+        #pragma omp parallel for schedule(static)
         for (int i = 0; i < M; ++i) st->f_r[i * D + d] = c[2 * i];
     }
     /* InitializeVirial, pair_style.cu:262-276 + d_initVirial device_array_utils.cu:456-494 */
     double* vr = cz((size_t)M * nP);
+    #pragma omp parallel for schedule(static)
     for (int i = 0; i < M; ++i) {
         int nn[3]; unstack(s, i, nn);
         double dr[3];
@@ -140,8 +165,10 @@ static void style_tables(ora_sys* s, ora_style* st) {
         }
     }
     for (int cc = 0; cc < nP; ++cc) {
+        #pragma omp parallel for schedule(static)
         for (int i = 0; i < M; ++i) { c[2 * i] = vr[(size_t)i * nP + cc]; c[2 * i + 1] = 0.0; }
         ora_fft(c, D, s->Nx, -1);
+        #pragma omp parallel for schedule(static)
         for (int i = 0; i < M; ++i) {
             st->vir_k[2 * ((size_t)i * nP + cc)] = c[2 * i] / M;
             st->vir_k[2 * ((size_t)i * nP + cc) + 1] = c[2 * i + 1] / M;
@@ -210,6 +237,7 @@ static void calc_forces_field(ora_sys* s, const ora_style* st) {
     int M = s->M, D = s->dim;
     double* Fd[3] = {s->fx, s->fy, s->fz};
     for (int d = 0; d < D; ++d)
+        #pragma omp parallel for schedule(static)
         for (int i = 0; i < M; ++i) {
             double fr = st->f_r[(size_t)i * D + d];
             double ra = s->rho[(size_t)st->t1 * M + i], rb = s->rho[(size_t)st->t2 * M + i];
@@ -314,15 +342,18 @@ static void calc_forces_style(ora_sys* s, ora_style* st) {
     double* Fd[3] = {s->fx, s->fy, s->fz};
     for (int pass = 0; pass < 2; ++pass) {
         int src = pass == 0 ? st->t2 : st->t1, dst = pass == 0 ? st->t1 : st->t2;
+        #pragma omp parallel for schedule(static)
         for (int i = 0; i < M; ++i) { c2[2 * i] = s->rho[(size_t)src * M + i]; c2[2 * i + 1] = 0.0; }
         ora_fft(c2, D, s->Nx, -1);
         for (int d = 0; d < D; ++d) {
+            #pragma omp parallel for schedule(static)
             for (int i = 0; i < M; ++i) {        /* d_prepareForceKSpace :298-312 */
                 double fr = st->f_k[2 * (i * D + d)], fi = st->f_k[2 * (i * D + d) + 1];
                 c1[2 * i] = (fr * c2[2 * i] - fi * c2[2 * i + 1]) / M;
                 c1[2 * i + 1] = (fr * c2[2 * i + 1] + fi * c2[2 * i]) / M;
             }
             ora_fft(c1, D, s->Nx, +1);
+            #pragma omp parallel for schedule(static)
             for (int i = 0; i < M; ++i)          /* d_accumulateGridForce :362-372 */
                 Fd[d][(size_t)dst * M + i] += s->rho[(size_t)dst * M + i] * c1[2 * i];
         }
@@ -332,13 +363,15 @@ static void calc_forces_style(ora_sys* s, ora_style* st) {
 /* Charges::CalcCharges, pair_style_charges.cu:21-111; kernels device_array_utils.cu:186-279 */
 static void calc_charges(ora_sys* s) {
     int M = s->M, D = s->dim;
-    double *c1 = s->cpx1, *c2 = s->cpx2, kv[3];
+    double *c1 = s->cpx1, *c2 = s->cpx2;
     double lb = s->bjerrum_f, a = s->smear_f;
     for (int i = 0; i < M; ++i) { c1[2 * i] = s->rho_c[i]; c1[2 * i + 1] = 0.0; }
     memcpy(c2, c1, sizeof(double) * 2 * M);
     ora_fft(c2, D, s->Nx, -1);
     for (int i = 0; i < 2 * M; ++i) c2[i] /= M;
+    #pragma omp parallel for schedule(static)
     for (int i = 0; i < M; ++i) {                /* d_prepareElectrostaticPotential */
+        double kv[3];
         double k2 = get_k(s, i, kv);
         if (k2 != 0) {
             double g = 4.0 * ORA_PI * lb / k2 * exp(-k2 / (2.0 * a * a));
@@ -347,7 +380,9 @@ static void calc_charges(ora_sys* s) {
     }
     double* Fc[3] = {s->fxc, s->fyc, s->fzc};
     for (int d = 0; d < D; ++d) {
+        #pragma omp parallel for schedule(static)
         for (int i = 0; i < M; ++i) {            /* d_prepareElectricField :225-241 */
+            double kv[3];
             double k2 = get_k(s, i, kv);
             double e = exp(-k2 / (2.0 * a * a));
             c2[2 * i] = kv[d] * c1[2 * i + 1] * e;
@@ -366,6 +401,7 @@ static void gather(ora_sys* s) {
     memset(s->f, 0, sizeof(double) * s->ns * D);
     double* Fd[3] = {s->fx, s->fy, s->fz};
     double* Fc[3] = {s->fxc, s->fyc, s->fzc};
+    #pragma omp parallel for schedule(static)
     for (int i = 0; i < s->ns; ++i) {
         if (s->do_charges)
             for (int m = 0; m < S; ++m) {
@@ -387,6 +423,7 @@ static void gather(ora_sys* s) {
 /* d_bonds, device_bonds.cu:4-60 */
 static void bonds(ora_sys* s) {
     int D = s->dim;
+    #pragma omp parallel for schedule(static)
     for (int i = 0; i < s->ns; ++i) {
         double lf[3] = {0, 0, 0};
         for (int b = 0; b < s->n_bonds[i]; ++b) {
@@ -435,6 +472,7 @@ void ora_integrate(ora_sys* s) {
     double* old = s->prev_noise;                 /* zero at setup, initialize.cu:120 */
     for (int k = 0; k < s->ninteg; ++k) {
         const ora_group* g = &s->grp[s->integ[k].group];
+        #pragma omp parallel for schedule(static)
         for (int l = 0; l < g->nsites; ++l) {
             int i = g->index[l];
             double z[4];
@@ -504,8 +542,10 @@ void ora_run(ora_sys* s, int nsteps) {
 static void style_energy_virial(ora_sys* s, ora_style* st) {
     int M = s->M, D = s->dim, nP = s->nP;
     double *c1 = s->cpx1, *c2 = s->cpx2;
+    #pragma omp parallel for schedule(static)
     for (int i = 0; i < M; ++i) { c2[2 * i] = s->rho[(size_t)st->t2 * M + i]; c2[2 * i + 1] = 0.0; }
     ora_fft(c2, D, s->Nx, -1);
+    #pragma omp parallel for schedule(static)
     for (int i = 0; i < M; ++i) {
         double ur = st->u_k[2 * i], ui = st->u_k[2 * i + 1];
         c1[2 * i] = (ur * c2[2 * i] - ui * c2[2 * i + 1]) / M;
@@ -517,6 +557,7 @@ static void style_energy_virial(ora_sys* s, ora_style* st) {
     st->energy = sum * s->gvol;
     for (int i = 0; i < 2 * M; ++i) c2[i] /= M;           /* d_divideByDimension */
     for (int c = 0; c < nP; ++c) {
+        #pragma omp parallel for schedule(static)
         for (int i = 0; i < M; ++i) {                       /* d_prepareVirial :496-507 */
             double vr = st->vir_k[2 * ((size_t)i * nP + c)], vi = st->vir_k[2 * ((size_t)i * nP + c) + 1];
             c1[2 * i] = vr * c2[2 * i] - vi * c2[2 * i + 1];
@@ -741,6 +782,14 @@ double ora_get_double(const ora_sys* s, const char* k) {
     if (!strcmp(k, "bjerrum")) return s->bjerrum_f;
     if (!strcmp(k, "smear")) return s->smear_f;
     return NAN;
+}
+
+/* Test-only settings.  "noise_mag": the thermal-noise amplitude sqrt(2 delt) of the
+   EM and GJF updates (0 switches the noise off, for closed-form checks of the drift
+   terms).  Returns 0, or -1 for an unknown key. */
+int ora_set_double(ora_sys* s, const char* k, double v) {
+    if (!strcmp(k, "noise_mag")) { s->noise_mag = v; return 0; }
+    return -1;
 }
 
 void ora_get_positions(const ora_sys* s, double* out) { memcpy(out, s->x, sizeof(double) * s->ns * s->dim); }

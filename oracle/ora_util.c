@@ -16,6 +16,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define ORA_PI 3.14159265358979323846
 
@@ -114,10 +117,11 @@ static int plan_make(fft_plan* P, int n, int sign) {
     return 0;
 }
 
-static void plan_exec(fft_plan* P, double* data) {
+/* buf0, buf1: 2n doubles of scratch each (per thread when lines run in parallel) */
+static void plan_exec(const fft_plan* P, double* data, double* buf0, double* buf1) {
     int n = P->n;
     if (n <= 1) return;
-    double *src = P->buf0, *dst = P->buf1;
+    double *src = buf0, *dst = buf1;
     memcpy(src, data, sizeof(double) * 2 * n);
     double ur[2 * 64], uo[2 * 64];
     int p = 1;
@@ -159,17 +163,13 @@ int ora_fft1d(double* data, int n, int sign) {
     fft_plan P;
     int rc = plan_make(&P, n, sign);
     for (int s = 0; s < P.nst; ++s) if (P.rad[s] > 64) rc = -3;
-    if (rc == 0) plan_exec(&P, data);
+    if (rc == 0) plan_exec(&P, data, P.buf0, P.buf1);
     plan_free(&P);
     return rc;
 }
 
 int ora_fft(double* data, int dim, const int* Nx, int sign) {
     int N[3] = {Nx[0], dim > 1 ? Nx[1] : 1, dim > 2 ? Nx[2] : 1};
-    int maxn = N[0] > N[1] ? N[0] : N[1];
-    if (N[2] > maxn) maxn = N[2];
-    double* line = (double*)malloc(sizeof(double) * 2 * maxn);
-    if (!line) return -2;
     int rc = 0;
     for (int ax = 0; ax < dim && rc == 0; ++ax) {
         fft_plan P;
@@ -178,26 +178,54 @@ int ora_fft(double* data, int dim, const int* Nx, int sign) {
         if (rc == 0) {
             size_t stride = ax == 0 ? 1 : (ax == 1 ? (size_t)N[0] : (size_t)N[0] * N[1]);
             size_t total = (size_t)N[0] * N[1] * N[2];
-            size_t nlines = total / N[ax];
-            for (size_t l = 0; l < nlines; ++l) {
-                /* line start: enumerate all indices with coordinate ax == 0 */
-                size_t base;
-                if (ax == 0) base = l * N[0];
-                else if (ax == 1) base = (l % N[0]) + (l / N[0]) * (size_t)N[0] * N[1];
-                else base = l;
-                for (int q = 0; q < N[ax]; ++q) {
-                    line[2 * q] = data[2 * (base + q * stride)];
-                    line[2 * q + 1] = data[2 * (base + q * stride) + 1];
+            long nlines = (long)(total / N[ax]);
+            int n = N[ax];
+            /* lines are independent: each thread transforms whole lines with its own
+               buffers, so the result does not depend on the thread count */
+            #pragma omp parallel
+            {
+                double* tl = (double*)malloc(sizeof(double) * 6 * (size_t)n);
+                #pragma omp for schedule(static)
+                for (long l = 0; l < nlines; ++l) {
+                    /* line start: enumerate all indices with coordinate ax == 0 */
+                    size_t base;
+                    if (ax == 0) base = (size_t)l * N[0];
+                    else if (ax == 1) base = ((size_t)l % N[0]) + ((size_t)l / N[0]) * (size_t)N[0] * N[1];
+                    else base = (size_t)l;
+                    double* ln = tl;
+                    for (int q = 0; q < n; ++q) {
+                        ln[2 * q] = data[2 * (base + q * stride)];
+                        ln[2 * q + 1] = data[2 * (base + q * stride) + 1];
+                    }
+                    plan_exec(&P, ln, tl + 2 * (size_t)n, tl + 4 * (size_t)n);
+                    for (int q = 0; q < n; ++q) {
+                        data[2 * (base + q * stride)] = ln[2 * q];
+                        data[2 * (base + q * stride) + 1] = ln[2 * q + 1];
+                    }
                 }
-                plan_exec(&P, line);
-                for (int q = 0; q < N[ax]; ++q) {
-                    data[2 * (base + q * stride)] = line[2 * q];
-                    data[2 * (base + q * stride) + 1] = line[2 * q + 1];
-                }
+                free(tl);
             }
         }
         plan_free(&P);
     }
-    free(line);
     return rc;
+}
+
+/* Worker threads of the oracle's parallel loops (FFT lines, per-node and per-site
+   loops).  Every parallel loop writes disjoint outputs and every reduction stays
+   serial, so results are bit-identical for any thread count. */
+void ora_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+int ora_get_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }

@@ -37,6 +37,9 @@ void ora_normals(uint32_t seed, uint32_t particle, uint32_t step, uint32_t strea
 int ora_fft(double* data, int dim, const int* Nx, int sign);
 /* 1-D FFT helper for tests */
 int ora_fft1d(double* data, int n, int sign);
+/* worker threads of the parallel loops; results are bit-identical for any count */
+void ora_set_threads(int n);
+int  ora_get_threads(void);
 
 /* --- system ----------------------------------------------------------- */
 /* parse an input deck (read_input.cu grammar); data files resolve against base_dir */
@@ -46,6 +49,8 @@ void ora_destroy(ora_sys* s);
 
 int    ora_get_int(const ora_sys* s, const char* key);
 double ora_get_double(const ora_sys* s, const char* key);
+/* test-only settings: "noise_mag" (0 = noise off); 0 on success, -1 unknown key */
+int    ora_set_double(ora_sys* s, const char* key, double v);
 
 void ora_get_positions(const ora_sys* s, double* out);       /* ns*D */
 void ora_set_positions(ora_sys* s, const double* in);        /* also resets GJF xo = x */

@@ -53,6 +53,10 @@ def lib():
         L.ora_fft.argtypes = [dp, ctypes.c_int, ip, ctypes.c_int]
         L.ora_fft1d.restype = ctypes.c_int
         L.ora_fft1d.argtypes = [dp, ctypes.c_int, ctypes.c_int]
+        L.ora_set_double.restype = ctypes.c_int
+        L.ora_set_double.argtypes = [vp, ctypes.c_char_p, ctypes.c_double]
+        L.ora_set_threads.argtypes = [ctypes.c_int]
+        L.ora_get_threads.restype = ctypes.c_int
         L.ora_nlist.restype = ctypes.c_int
         L.ora_nlist.argtypes = [vp, ctypes.c_double, ctypes.c_int, ip, ip, ip, ctypes.c_int, ip, ip]
         _lib = L
@@ -80,6 +84,15 @@ def normals(seed, particle, step, stream=0):
     z = np.zeros(4)
     lib().ora_normals(seed, particle, step, stream, _dp(z))
     return z
+
+
+def set_threads(n):
+    """Worker threads of the oracle's parallel loops (results do not depend on it)."""
+    lib().ora_set_threads(int(n))
+
+
+def get_threads():
+    return lib().ora_get_threads()
 
 
 def fft(a, sign=-1):
@@ -123,6 +136,11 @@ class System:
 
     def getd(self, k):
         return lib().ora_get_double(self._h, k.encode())
+
+    def set_noise(self, mag):
+        """Test-only: thermal-noise amplitude of the EM/GJF updates (0 = off)."""
+        if lib().ora_set_double(self._h, b"noise_mag", float(mag)):
+            raise ValueError("noise_mag not settable")
 
     @property
     def grid_shape(self):
