@@ -1,20 +1,28 @@
 """TILD step-engine benchmark on one MI355X.
 
-Workload (default): the large lamellar diblock melt -- 12,800 chains of N = 50
-(640,000 beads), box 40x40x80, grid 80x80x160, Gaussian + erf pair styles, GJF
-dt = 0.01, FP32 particles and fields, FP64 property reductions every 1,000 steps.
-Synthetic seeded random-walk chains (no datasets are available offline).
+Headline workload: the large lamellar diblock melt -- 12,800 chains of N = 50 (640,000
+beads), box 40x40x80, grid 80x80x160, Gaussian + erf pair styles, GJF dt = 0.01, FP32
+particles and fields, FP64 property reductions.  One replica, its steps replayed from
+captured hipGraphs (runs of 10); the spatial reorders every 60 steps are folded into
+ms_per_step whether or not the timed window holds one.  Synthetic seeded random-walk
+chains (no datasets are available offline).
 
 Prints one JSON line: whole-job bead-steps/s, the roofline of the dominant kernel
-(algorithmic bytes per launch / mean launch time from HIP events on the engine's
-stream) and the FP64 CPU oracle timed on a bounded sample of the same workload.
-Two more legs ride along: a sweep of 256 uneven replicas advanced together versus
-one at a time ("batched"), and a Poisson stream of jobs through the job server with
-p50/p99 time to result ("serving").
+(algorithmic bytes per launch / mean launch time from HIP events bound to the launches
+on the engine's stream), the FP64 oracle timed on the box's own cores (one thread and
+all the threads of the box's CPU share), and the FP32-vs-FP64 error of every quantity.
+Legs that ride along, one per benchmark configuration:
+  C1  24,000-bead diblock melt, 40^3, EM
+  C2  the headline melt, also as the "plain" path (eager launches, no graph)
+  C3  81,000-bead coacervate, 64^3, charges + Gaussian-erf, EM, and a sweep of 8
+      coacervate replicas batched vs plain
+  C4  256 uneven diblock replicas advanced together vs one at a time
+  C5  a Poisson stream of jobs through the continuous-batching server, p50/p99
 """
 import argparse
 import concurrent.futures
 import json
+import multiprocessing
 import os
 import pathlib
 import platform
@@ -27,6 +35,7 @@ REPO = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO / "gpu-tild_amd"))
 
 HBM_PEAK_GBS = 8000.0
+REORDER_EVERY = 60        # automatic spatial-reorder interval of the melts (engine default)
 
 
 def phase_bytes(ns, D, M, Mh, nf, T, gjf, charged, ncoef, reorder_every):
@@ -76,6 +85,7 @@ def step_bytes_minimal(ns, D, M, T, gjf, charged):
     return ns * bp + M * bg
 
 
+# ------------------------------------------------------------------ decks
 def _gen_sweep(args):
     root, idx, seed = args
     import tild_gen
@@ -91,23 +101,242 @@ def _gen_serve(args):
     return text, str(d), steps
 
 
-def generate(args, root):
-    """Write every deck before the GPU is touched (a pool of forked writers)."""
+def _gen_named(args):
+    root, name, seed = args
     import tild_gen
-    main_dir = pathlib.Path(root) / "main"
-    main_text = tild_gen.lamellar_melt_640k(main_dir, seed=args.seed)
-    sweep, serve = [], []
-    with concurrent.futures.ProcessPoolExecutor(max_workers=8) as ex:
-        if args.sweep_replicas > 0:
-            sweep = list(ex.map(_gen_sweep, [(root, i, args.seed) for i in range(args.sweep_replicas)]))
-        if args.serve_jobs > 0:
-            serve = list(ex.map(_gen_serve, [(root, i, args.seed) for i in range(args.serve_jobs)]))
-    return (main_text, str(main_dir)), sweep, serve
+    d = pathlib.Path(root) / name
+    fn = {"c1": tild_gen.diblock_melt_24k, "c2": tild_gen.lamellar_melt_640k, "c3": tild_gen.coacervate_81k}
+    return fn[name.split("_")[0]](d, seed=seed), str(d)
+
+
+def generate(args, root):
+    """Write every deck before the GPU is touched (a pool of fresh writer processes)."""
+    ctx = multiprocessing.get_context("spawn")
+    with concurrent.futures.ProcessPoolExecutor(max_workers=8, mp_context=ctx) as ex:
+        named = ["c1", "c2", "c3"] + [f"c3_{k}" for k in range(args.c3_replicas)]
+        fut_named = {n: ex.submit(_gen_named, (root, n, args.seed + (int(n.split("_")[1]) if "_" in n else 0)))
+                     for n in named}
+        sweep = list(ex.map(_gen_sweep, [(root, i, args.seed) for i in range(args.sweep_replicas)]))
+        serve = list(ex.map(_gen_serve, [(root, i, args.seed) for i in range(args.serve_jobs)]))
+        decks = {n: f.result() for n, f in fut_named.items()}
+    return decks, sweep, serve
+
+
+# ------------------------------------------------------------------ helpers
+def cpu_info():
+    model = platform.processor() or platform.machine()
+    try:
+        for line in pathlib.Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        share = len(os.sched_getaffinity(0))
+    except AttributeError:
+        share = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = min(share, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else share
+    return model, threads
+
+
+def _box(s):
+    return [s.info("L0"), s.info("L1"), s.info("L2")][:s.dim]
+
+
+def _rel(a, b):
+    import numpy as np
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def fp64_errors(engine_like, oracle, sysm, with_rho=True):
+    """FP32 engine against the FP64 oracle at the same positions: densities (peak
+    relative), forces (|df| / |f|), energies and pressure (relative)."""
+    import numpy as np
+    e, o = engine_like, oracle
+    f, fo = e.forces(), o.forces()
+    out = {}
+    if with_rho:
+        worst = 0.0
+        for t in range(e.nfields):
+            ro = o.density(t) if t < sysm.ntypes else o.charge_density()
+            worst = max(worst, float(np.abs(e.density(t).astype(np.float64) - ro).max() / np.abs(ro).max()))
+        out["rho"] = worst
+    out["forces"] = _rel(f, fo)
+    p, po = e.properties(), o.properties()
+    for k in ("Upe", "Ubond", "Uelec"):
+        if po[k] != 0:
+            out[k] = abs(p[k] - po[k]) / abs(po[k])
+    out["U_styles"] = float(np.max(np.abs(p["U_styles"] - po["U_styles"]) / np.abs(po["U_styles"])))
+    out["Ptens"] = float(np.abs(p["Ptens"] - po["Ptens"]).max() / np.abs(po["Ptens"]).max())
+    return out
+
+
+def traj_error(engine, oracle, sysm, n):
+    import numpy as np
+    engine.step(n)
+    oracle.run(n)
+    d = engine.positions().astype(np.float64) - oracle.positions()
+    L = np.asarray(_box(sysm))
+    d -= L * np.round(d / L)
+    return float(np.abs(d).max())
+
+
+def time_oracle(o, steps, threads):
+    """Oracle bead-steps/s over `steps` further steps of an oracle system with `threads`
+    worker threads (its results do not depend on the count)."""
+    sys.path.insert(0, str(REPO / "tests"))
+    import oracle_ctypes as O
+    O.set_threads(threads)
+    t0 = time.perf_counter()
+    o.run(steps)
+    dt = time.perf_counter() - t0
+    return o.ns * steps / dt, dt
+
+
+def leg_engine(text, d, seed, steps, warmup, graph=True):
+    """One replica on its own: `steps` steps after `warmup`, graph-replayed (runs of 10)
+    or eager launches with no graph (the plain path), inputs resident before timing."""
+    import tild
+    s = tild.System(text, d)
+    b = tild.Batch(device=0)
+    r = b.add(s, seed)
+    b.step(warmup, graph=graph)
+    b.properties(r)          # synchronises
+    t0 = time.perf_counter()
+    b.step(steps, graph=graph)
+    b.properties(r)
+    wall = time.perf_counter() - t0
+    b.close()
+    return s.ns * steps / wall, wall / steps * 1e3
+
+
+# ------------------------------------------------------------------ legs
+def leg_headline(args, text, d):
+    import numpy as np
+    import tild
+    sysm = tild.System(text, d)
+    ns, D, M = sysm.ns, sysm.dim, sysm.M
+    Nx = sysm.Nx
+    Mh = (Nx[0] // 2 + 1) * Nx[1] * Nx[2]
+    T = sysm.ntypes
+    eng = tild.Engine(sysm, device=0, seed=args.seed, reorder_every=args.reorder_every)
+    eng.step(args.warmup)
+    eng.synchronize()
+    start_step = eng.step_count()
+    log_every = 1000
+    t0 = time.perf_counter()
+    done = 0
+    nprops = 0
+    while done < args.steps:
+        n = min(log_every, args.steps - done)
+        eng.step(n)
+        done += n
+        if done % log_every == 0:
+            p = eng.properties()
+            nprops += 1
+            if not np.isfinite(p["Upe"]):
+                raise RuntimeError("non-finite energy")
+    eng.synchronize()
+    wall = time.perf_counter() - t0
+    # one property evaluation after the timed window, always
+    pend = eng.properties()
+    if not np.isfinite(pend["Upe"]):
+        raise RuntimeError("non-finite energy after the timed steps")
+    # kernel profile over two reorder intervals: phase times per launch, and the
+    # reorders (plus the bucket-claim pass that follows each) amortised per step
+    prof = eng.profile(args.profile_steps)
+    every = args.reorder_every if args.reorder_every > 0 else REORDER_EVERY
+    reorder_cost_ms = (prof["reorder"] + prof["claim"]) * args.profile_steps / max(1, args.profile_steps // every)
+    in_window = (start_step + args.steps) // every - start_step // every
+    correction_ms = (args.steps / every - in_window) * reorder_cost_ms if args.reorder_every >= 0 else 0.0
+    ms_per_step = (wall * 1e3 + correction_ms) / args.steps
+    value = ns / (ms_per_step * 1e-3)
+    pb = phase_bytes(ns, D, M, Mh, T, T, True, False, T * (T + 1) // 2, every)
+    dom = max((k for k in prof if pb[k] > 0), key=lambda k: prof[k])   # dominant kernel
+    ach = pb[dom] / (prof[dom] * 1e-3) / 1e9
+    step_bytes = step_bytes_minimal(ns, D, M, T, True, False)
+    kern_ms = sum(v for k, v in prof.items())
+    phases = {k: {"ms": round(v, 5), "GBps": round(pb[k] / (v * 1e-3) / 1e9, 1) if v > 0 and pb[k] else None}
+              for k, v in prof.items()}
+    eng.close()
+    return {
+        "value": value, "ms_per_step": ms_per_step, "wall_s": wall, "ns": ns, "Nx": Nx[:D], "props": nprops + 1,
+        "Upe_end": pend["Upe"], "reorders_in_window": in_window, "reorder_ms": reorder_cost_ms,
+        "timed_ms_per_step": wall * 1e3 / args.steps,
+        "roofline": {"bound": "hbm", "kernel": PHASE_KERNEL[dom], "achieved": ach, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes": pb[dom],
+                     "launch_ms": prof[dom]},
+        "step_roofline": {"bytes": step_bytes, "kernel_ms": kern_ms,
+                          "GBps": step_bytes / (ms_per_step * 1e-3) / 1e9,
+                          "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "phases": phases, "dominant": dom,
+    }
+
+
+def leg_config(args, name, text, d, gjf, charged, steps, err_steps, threads, cpu_steps, cpu_steps_1t):
+    """A single-replica configuration: graph-replayed and plain bead-steps/s, the step's
+    HBM roofline, FP64 errors per quantity, and the oracle on 1 and all threads."""
+    import tild
+    sys.path.insert(0, str(REPO / "tests"))
+    import oracle_ctypes as O
+    s = tild.System(text, d)
+    v_graph, ms_graph = leg_engine(text, d, args.seed, steps, 10, graph=True)
+    v_plain, ms_plain = leg_engine(text, d, args.seed, steps, 10, graph=False)
+    B = step_bytes_minimal(s.ns, s.dim, s.M, s.ntypes, gjf, charged)
+    out = {"beads": s.ns, "grid": s.Nx[:s.dim], "steps": steps,
+           "graph": {"value": v_graph, "ms_per_step": ms_graph},
+           "plain": {"value": v_plain, "ms_per_step": ms_plain},
+           "step_bytes": B, "roofline_frac": B / (ms_graph * 1e-3) / 1e12 / 8.0, "unit": "bead-steps/s"}
+    if not args.no_cpu_baseline:
+        O.set_threads(threads)
+        e = tild.Engine(s, device=0, seed=args.seed)
+        o = O.System(text, d, seed=args.seed)
+        errs = fp64_errors(e, o, s)
+        errs[f"positions_{err_steps}_steps"] = traj_error(e, o, s, err_steps)
+        e.close()
+        out["fp64_error"] = errs
+        vn, tn = time_oracle(o, cpu_steps, threads)
+        v1, t1 = time_oracle(o, cpu_steps_1t, 1)
+        O.set_threads(threads)
+        out["oracle"] = {"1_thread": {"value": v1, "steps": cpu_steps_1t, "s": t1},
+                         "all_threads": {"value": vn, "threads": threads, "steps": cpu_steps, "s": tn}}
+    return out
+
+
+def leg_c3_sweep(args, decks):
+    """Coacervate replicas (81,000 beads, 64^3 each) advanced together versus one at a
+    time with eager launches."""
+    import tild
+    systems = [tild.System(t, d) for t, d in decks]
+    b = tild.Batch(device=0)
+    ids = [b.add(s, args.seed + k) for k, s in enumerate(systems)]
+    beads = sum(s.ns for s in systems)
+    b.step(10, graph=True)
+    b.properties(ids[0])
+    t0 = time.perf_counter()
+    b.step(args.c3_sweep_steps, graph=True)
+    b.properties(ids[0])
+    tb = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for r in ids:
+        b.step(args.c3_sweep_steps, replicas=[r], graph=False)
+    b.properties(ids[0])
+    tp = time.perf_counter() - t0
+    b.close()
+    bat, pl = beads * args.c3_sweep_steps / tb, beads * args.c3_sweep_steps / tp
+    return {"replicas": len(ids), "beads_total": beads, "steps": args.c3_sweep_steps, "value": bat,
+            "plain_value": pl, "speedup_vs_plain": bat / pl, "unit": "bead-steps/s"}
 
 
 def leg_sweep(args, decks):
     """256 uneven replicas: all advanced together (one launch per phase, graph replay)
-    versus each advanced alone with eager launches."""
+    versus each advanced alone with eager launches; FP64 error of sampled replicas at
+    their final positions."""
+    import numpy as np
     import tild
     systems = [tild.System(t, d) for t, d in decks]
     b = tild.Batch(device=0)
@@ -115,60 +344,125 @@ def leg_sweep(args, decks):
     beads = sum(s.ns for s in systems)
     b.step(5, graph=True)
     b.step(5, graph=False)
+    b.properties(ids[0])
     t0 = time.perf_counter()
     b.step(args.sweep_steps, graph=True)
+    b.properties(ids[0])
     tb = time.perf_counter() - t0
     t0 = time.perf_counter()
     for r in ids:
         b.step(args.sweep_plain_steps, replicas=[r], graph=False)
+    b.properties(ids[0])
     tp = time.perf_counter() - t0
+    final = {}
+    grids = [s.Nx[0] for s in systems]
+    order = np.argsort(grids, kind="stable")
+    sample = sorted({int(order[j]) for j in np.linspace(0, len(ids) - 1, args.sweep_oracle_sample).round().astype(int)})
+    for k in sample:
+        final[k] = b.positions(ids[k])
     b.close()
+    errs = []
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, str(REPO / "tests"))
+        import oracle_ctypes as O
+        for k in sample:
+            t, d = decks[k]
+            e = tild.Engine(systems[k], device=0, seed=args.seed + k)
+            o = O.System(t, d, seed=args.seed + k)
+            e.set_positions(final[k])
+            o.set_positions(final[k].astype(np.float64))
+            er = fp64_errors(e, o, systems[k], with_rho=False)
+            er.update({"replica": k, "grid": grids[k], "beads": systems[k].ns})
+            errs.append(er)
+            e.close()
     batched = beads * args.sweep_steps / tb
     plain = beads * args.sweep_plain_steps / tp
-    grids = sorted({s.Nx[0] for s in systems})
-    return {"workload": f"{len(ids)} uneven diblock-melt replicas (grids {grids[0]}^3..{grids[-1]}^3 at dx=0.5, "
+    return {"workload": f"{len(ids)} uneven diblock-melt replicas (grids {min(grids)}^3..{max(grids)}^3 at dx=0.5, "
                         f"N in 10..40, A_AB in [1,4], rho0=3, EM)",
             "replicas": len(ids), "beads_total": beads, "steps": args.sweep_steps,
             "value": batched, "unit": "bead-steps/s", "ms_per_step": tb / args.sweep_steps * 1e3,
-            "plain_value": plain, "plain_steps": args.sweep_plain_steps, "speedup_vs_plain": batched / plain}
+            "plain_value": plain, "plain_steps": args.sweep_plain_steps, "speedup_vs_plain": batched / plain,
+            "fp64_error_final_state": errs}
 
 
 def leg_serve(args, jobs):
-    """A Poisson stream of jobs through the continuous-batching server."""
+    """A Poisson stream of jobs through the continuous-batching server; FP64 error of
+    sampled jobs at their final positions."""
     import numpy as np
     import tild
     rng = np.random.default_rng(args.seed)
     gaps = rng.exponential(1.0 / args.serve_rate, size=len(jobs))
     srv = tild.Server(device=0, chunk_steps=args.serve_chunk, use_graph=True)
     errors = []
+    ids = {}     # job index -> server job id
 
     def feeder():
         t0 = time.perf_counter()
         at = 0.0
-        for (text, d, steps), g in zip(jobs, gaps):
+        for k, ((text, d, steps), g) in enumerate(zip(jobs, gaps)):
             at += g
             delay = at - (time.perf_counter() - t0)
             if delay > 0:
                 time.sleep(delay)
             try:
-                srv.submit(text, d, args.seed, steps)
+                ids[k] = srv.submit(text, d, args.seed, steps)
             except Exception as e:   # reported in the JSON line
                 errors.append(str(e))
 
     t0 = time.perf_counter()
     th = threading.Thread(target=feeder)
     th.start()
+    last = t0
+    while th.is_alive() or srv.stats()["done"] + srv.stats()["failed"] < len(ids):
+        time.sleep(0.5)
+        if time.perf_counter() - last > 20:
+            st = srv.stats()
+            print(f"bench: serving {st['done']}/{len(jobs)} done, {st['running']} running, {st['queued']} queued",
+                  file=sys.stderr, flush=True)
+            last = time.perf_counter()
+        if time.perf_counter() - t0 > 900:
+            break
     th.join()
     srv.drain(timeout=600)
     wall = time.perf_counter() - t0
     st = srv.stats()
+    arrivals = float(np.sum(gaps))
+    # sampled jobs: FP64 error at their final positions
+    errs = []
+    if not args.no_cpu_baseline and ids:
+        sys.path.insert(0, str(REPO / "tests"))
+        import oracle_ctypes as O
+        for k in range(0, len(jobs), max(1, len(jobs) // args.serve_oracle_sample)):
+            if k not in ids:
+                continue
+            text, d, steps = jobs[k]
+            s = tild.System(text, d)
+            if s.ns > args.serve_oracle_max_beads:
+                continue
+            srv.meta[ids[k]] = (s.ns, s.dim)
+            _, x, lat = srv.result(ids[k])
+            e = tild.Engine(s, device=0, seed=args.seed)
+            o = O.System(text, d, seed=args.seed)
+            e.set_positions(x)
+            o.set_positions(x.astype(np.float64))
+            er = fp64_errors(e, o, s, with_rho=False)
+            er.update({"job": k, "dim": s.dim, "grid": s.Nx[:s.dim], "beads": s.ns, "steps": steps,
+                       "latency_s": lat})
+            errs.append(er)
+            e.close()
     srv.close()
+    idle = wall - st["busy_s"] - st["admit_s"] - st["retire_s"]
     return {"workload": f"{len(jobs)} jobs, Poisson arrivals at {args.serve_rate}/s: 2D films 64^2..256^2 and "
                         f"3D melts 32^3..96^3 (N=20, rho0=3), 1000-5000 steps each",
             "jobs": len(jobs), "done": st["done"], "failed": st["failed"] + len(errors),
-            "p50_s": st["p50_s"], "p99_s": st["p99_s"], "makespan_s": wall,
+            "p50_s": st["p50_s"], "p99_s": st["p99_s"],
+            "queue_wait_p50_s": st["wait50_s"], "queue_wait_p99_s": st["wait99_s"],
+            "service_p50_s": st["svc50_s"], "service_p99_s": st["svc99_s"],
+            "arrival_span_s": arrivals, "makespan_s": wall,
             "throughput": st["bead_steps"] / wall, "unit": "bead-steps/s",
-            "gpu_busy_frac": st["busy_s"] / wall, "peak_hbm_bytes": st["peak_bytes"]}
+            "gpu_busy_frac": st["busy_s"] / wall, "admit_frac": st["admit_s"] / wall,
+            "retire_frac": st["retire_s"] / wall, "idle_frac": idle / wall,
+            "peak_hbm_bytes": st["peak_bytes"], "fp64_error_final_state": errs}
 
 
 def traffic_probe(args):
@@ -203,97 +497,103 @@ def traffic_probe(args):
     print(json.dumps(out))
 
 
+def log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def run(args):
-    import numpy as np
-    import tild
-    import tild_gen
+    import tild  # noqa: F401  (fails loudly if the library is missing)
 
+    model, threads = cpu_info()
     with tempfile.TemporaryDirectory() as td:
-        (text, mdir), sweep_decks, serve_jobs = generate(args, td)
-        sysm = tild.System(text, mdir)
-        ns, D, M = sysm.ns, sysm.dim, sysm.M
-        Nx = sysm.Nx
-        Mh = (Nx[0] // 2 + 1) * Nx[1] * Nx[2]
-        T = sysm.ntypes
-        eng = tild.Engine(sysm, device=0, seed=args.seed, reorder_every=args.reorder_every)
-
-        eng.step(args.warmup)
-        eng.synchronize()
-        log_every = 1000
         t0 = time.perf_counter()
-        done = 0
-        nprops = 0
-        while done < args.steps:
-            n = min(log_every, args.steps - done)
-            eng.step(n)
-            done += n
-            if done % log_every == 0:
-                p = eng.properties()
-                nprops += 1
-                if not np.isfinite(p["Upe"]):
-                    raise RuntimeError("non-finite energy")
-        eng.synchronize()
-        t1 = time.perf_counter()
-        wall = t1 - t0
-        value = ns * args.steps / wall
+        decks, sweep_decks, serve_jobs = generate(args, td)
+        log(f"decks written in {time.perf_counter() - t0:.1f} s")
+        text, mdir = decks["c2"]
+        head = leg_headline(args, text, mdir)
+        log(f"C2 headline {head['value']:.3e} bead-steps/s")
+        v_plain, ms_plain = leg_engine(text, mdir, args.seed, args.plain_steps, 10, graph=False)
+        log(f"C2 plain {v_plain:.3e}")
 
-        prof = eng.profile(args.profile_steps)
-        # automatic reorder interval of this workload: 60 steps
-        pb = phase_bytes(ns, D, M, Mh, T, T, True, False, T * (T + 1) // 2,
-                         args.reorder_every if args.reorder_every > 0 else (60 if args.reorder_every == 0 else args.steps))
-        dom = max((k for k in prof if pb[k] > 0), key=lambda k: prof[k])   # dominant kernel
-        ach = pb[dom] / (prof[dom] * 1e-3) / 1e9
-        step_bytes = step_bytes_minimal(ns, D, M, T, True, False)
-        step_ms = sum(prof.values())
-        phases = {k: {"ms": round(v, 5), "GBps": round(pb[k] / (v * 1e-3) / 1e9, 1) if v > 0 and pb[k] else None}
-                  for k, v in prof.items()}
-        eng.close()
-
-        batched = leg_sweep(args, sweep_decks) if sweep_decks else None
-        serving = leg_serve(args, serve_jobs) if serve_jobs else None
-
-        cpu = None
+        cpu, c2_err = None, None
         if not args.no_cpu_baseline:
             sys.path.insert(0, str(REPO / "tests"))
             import oracle_ctypes as O
-            osys = O.System(text, mdir, seed=args.seed)
-            osys.run(1)
-            tc0 = time.perf_counter()
-            osys.run(args.cpu_steps)
-            tc = time.perf_counter() - tc0
-            cpu = {"value": ns * args.cpu_steps / tc, "unit": "bead-steps/s", "cores": 1, "kind": "port",
-                   "sample": f"{args.cpu_steps} steps of the same 640,000-bead deck (FP64 oracle, 1 thread, "
-                             f"{platform.processor() or platform.machine()}), {tc:.1f} s"}
+            import tild as T
+            O.set_threads(threads)
+            s2 = T.System(text, mdir)
+            e2 = T.Engine(s2, device=0, seed=args.seed)
+            o2 = O.System(text, mdir, seed=args.seed)
+            c2_err = fp64_errors(e2, o2, s2)
+            c2_err["positions_3_steps"] = traj_error(e2, o2, s2, 3)
+            e2.close()
+            log("C2 FP64 errors done")
+            vn, tn = time_oracle(o2, args.cpu_steps, threads)
+            v1, t1 = time_oracle(o2, args.cpu_steps_1t, 1)
+            O.set_threads(threads)
+            del o2
+            cpu = {"value": vn, "unit": "bead-steps/s", "cores": threads, "kind": "port",
+                   "sample": f"{args.cpu_steps} steps of the same 640,000-bead deck (FP64 oracle, OpenMP on "
+                             f"{threads} threads of {model}), {tn:.1f} s",
+                   "single_thread": {"value": v1, "steps": args.cpu_steps_1t, "s": t1, "cores": 1},
+                   "cpu_model": model}
+            log("C2 CPU baseline done")
+
+        configs = {}
+        if not args.skip_configs:
+            t1, d1 = decks["c1"]
+            configs["C1"] = leg_config(args, "C1", t1, d1, False, False, args.c1_steps, 20, threads, 20, 20)
+            log("C1 done")
+            t3, d3 = decks["c3"]
+            configs["C3"] = leg_config(args, "C3", t3, d3, False, True, args.c3_steps, 20, threads, 20, 3)
+            log("C3 done")
+            if args.c3_replicas > 0:
+                configs["C3"]["sweep"] = leg_c3_sweep(args, [decks[f"c3_{k}"] for k in range(args.c3_replicas)])
+                log("C3 sweep done")
+        batched = leg_sweep(args, sweep_decks) if sweep_decks else None
+        log("C4 done")
+        serving = leg_serve(args, serve_jobs) if serve_jobs else None
+        log("C5 done")
 
     traffic = None
     tfile = REPO / "profiles" / "traffic_melt640k.json"
     if tfile.exists():
         try:
-            traffic = json.loads(tfile.read_text())["per_kernel_bytes"].get(PHASE_KERNEL[dom])
+            traffic = json.loads(tfile.read_text())["per_kernel_bytes"].get(head["roofline"]["kernel"])
         except (ValueError, OSError, KeyError):
             traffic = None
+    roof = dict(head["roofline"])
+    roof["traffic"] = traffic
+    configs["C2"] = {"beads": head["ns"], "grid": head["Nx"],
+                     "graph": {"value": head["value"], "ms_per_step": head["ms_per_step"]},
+                     "plain": {"value": v_plain, "ms_per_step": ms_plain},
+                     "roofline_frac": head["step_roofline"]["frac"], "fp64_error": c2_err}
     out = {
         "metric": "bead-steps/s per GPU + HBM roofline %; batched vs plain; p50/p99 sweep latency",
-        "value": value,
+        "value": head["value"],
         "unit": "bead-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "lamellar diblock melt, 12800 x N=50 = 640000 beads, box 40x40x80, "
-                               "grid 80x80x160, 3 gaussian + 3 erf styles, GJF dt=0.01, plain single replica",
-                   "beads": ns, "grid": Nx[:D], "property_evaluations": nprops},
-        "roofline": {"bound": "hbm", "kernel": PHASE_KERNEL[dom], "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes": pb[dom], "launch_ms": prof[dom]},
-        "step_roofline": {"bytes": step_bytes, "ms": step_ms, "GBps": step_bytes / (step_ms * 1e-3) / 1e9,
-                          "frac": step_bytes / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
-        "phases": phases,
+                               "grid 80x80x160, 3 gaussian + 3 erf styles, GJF dt=0.01, single replica, "
+                               "steps replayed from hipGraphs; spatial reorders every 60 steps amortised in",
+                   "beads": head["ns"], "grid": head["Nx"], "property_evaluations": head["props"],
+                   "timed_ms_per_step": head["timed_ms_per_step"], "reorders_in_window": head["reorders_in_window"],
+                   "reorder_ms": head["reorder_ms"], "Upe_after": head["Upe_end"]},
+        "roofline": roof,
+        "step_roofline": head["step_roofline"],
+        "phases": head["phases"],
+        "plain": {"value": v_plain, "ms_per_step": ms_plain, "steps": args.plain_steps,
+                  "note": "one replica, eager launches, no graph"},
+        "oracle_error": c2_err,
+        "configs": configs,
         "batched": batched,
         "serving": serving,
         "cpu_baseline": cpu,
@@ -307,18 +607,28 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--profile-steps", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--profile-steps", type=int, default=120, help="two reorder intervals")
+    ap.add_argument("--plain-steps", type=int, default=200)
+    ap.add_argument("--cpu-steps", type=int, default=20, help="oracle steps on all threads (headline deck)")
+    ap.add_argument("--cpu-steps-1t", type=int, default=3, help="oracle steps on one thread (headline deck)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--skip-configs", action="store_true", help="skip the C1 and C3 legs")
+    ap.add_argument("--c1-steps", type=int, default=500)
+    ap.add_argument("--c3-steps", type=int, default=500)
+    ap.add_argument("--c3-replicas", type=int, default=8, help="coacervate sweep size (0 skips it)")
+    ap.add_argument("--c3-sweep-steps", type=int, default=100)
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run the kernels of the HBM-traffic counter runs (copy calibration + eager steps)")
     ap.add_argument("--probe-steps", type=int, default=5)
     ap.add_argument("--sweep-replicas", type=int, default=256, help="0 skips the batched-sweep leg")
     ap.add_argument("--sweep-steps", type=int, default=200)
     ap.add_argument("--sweep-plain-steps", type=int, default=20)
-    ap.add_argument("--serve-jobs", type=int, default=48, help="0 skips the served-stream leg")
+    ap.add_argument("--sweep-oracle-sample", type=int, default=4)
+    ap.add_argument("--serve-jobs", type=int, default=500, help="0 skips the served-stream leg")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-chunk", type=int, default=100)
+    ap.add_argument("--serve-oracle-sample", type=int, default=8)
+    ap.add_argument("--serve-oracle-max-beads", type=int, default=120000)
     ap.add_argument("--reorder-every", type=int, default=0,
                     help="spatial reorder interval (0 automatic, n > 0 every n steps, < 0 never)")
     args = ap.parse_args()

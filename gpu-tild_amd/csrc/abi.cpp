@@ -30,6 +30,10 @@ tild_status from_rc(int rc, std::string& err, tild_engine* e) {
 }
 }  // namespace
 
+namespace tild {
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace tild
+
 extern "C" {
 
 const char* tild_last_error(void) { return g_last_error.c_str(); }

@@ -20,53 +20,70 @@ namespace tild {
     } while (0)
 
 // ---------------------------------------------------------------- FFT plans
-int make_axis_plan(int N, AxisPlan& pl, std::vector<float2>& tw, std::string& err) {
+namespace {
+// Depth-first search over non-increasing radix sequences (largest radices tried
+// first): as few stages as possible (every stage is an LDS round trip ending with a
+// barrier, and the passes are latency-bound), then the smallest largest radix
+// (radices spread evenly).  Fixed arrays: N <= 2048 has at most 11 prime factors.
+struct RadixSearch {
+    static constexpr int kDepth = 16;
+    int best[kDepth], nbest = -1, cur[kDepth], ncur = 0;
+    void run(int n) {
+        if (n == 1) {
+            if (nbest < 0 || ncur < nbest || (ncur == nbest && cur[0] < best[0])) {
+                for (int i = 0; i < ncur; ++i) best[i] = cur[i];
+                nbest = ncur;
+            }
+            return;
+        }
+        if ((nbest >= 0 && ncur + 1 > nbest) || ncur == kDepth) return;
+        static const int kRadices[] = {16, 10, 8, 7, 5, 4, 3, 2};
+        for (int R : kRadices)
+            if (n % R == 0 && (ncur == 0 || R <= cur[ncur - 1])) {
+                cur[ncur++] = R;
+                run(n / R);
+                --ncur;
+            }
+    }
+};
+}  // namespace
+
+const char* plan_axis(int N, AxisPlan& pl, float2* tw, int* ntw) {
     std::memset(&pl, 0, sizeof(pl));
     pl.N = N;
-    if (N < 1 || N > kFftBudget) { err = "FFT length " + std::to_string(N) + " outside [1, " + std::to_string(kFftBudget) + "]"; return -TILD_ERR_UNSUPPORTED; }
-    // As few stages as possible (every stage is an LDS round trip ending with a
-    // barrier, and the passes are latency-bound), then the smallest largest radix
-    // (radices spread evenly), larger radices first.
-    std::vector<int> rad;
-    {
-        static const int kRadices[] = {16, 10, 8, 7, 5, 4, 3, 2};
-        std::vector<int> cur;
-        std::function<void(int)> search = [&](int n) {
-            if (n == 1) {
-                auto mx = [](const std::vector<int>& v) { return *std::max_element(v.begin(), v.end()); };
-                if (rad.empty() || cur.size() < rad.size() || (cur.size() == rad.size() && mx(cur) < mx(rad))) rad = cur;
-                return;
-            }
-            if (!rad.empty() && cur.size() + 1 > rad.size()) return;
-            for (int R : kRadices)
-                if (n % R == 0 && (cur.empty() || R <= cur.back())) {
-                    cur.push_back(R);
-                    search(n / R);
-                    cur.pop_back();
-                }
-        };
-        if (N > 1) search(N);
-    }
-    if (rad.empty() && N > 1) {
-        err = "FFT length " + std::to_string(N) + " has a prime factor above 7";
-        return -TILD_ERR_UNSUPPORTED;
-    }
-    if ((int)rad.size() > kMaxStages) { err = "too many FFT stages"; return -TILD_ERR_UNSUPPORTED; }
-    pl.nst = (int)rad.size();
-    int p = 1;
-    tw.clear();
+    if (N < 1 || N > kFftBudget) return "FFT length outside [1, 2048]";
+    RadixSearch rs;
+    if (N > 1) rs.run(N);
+    else rs.nbest = 0;
+    if (rs.nbest < 0) return "FFT length has a prime factor above 7";
+    if (rs.nbest > kMaxStages) return "too many FFT stages";
+    pl.nst = rs.nbest;
+    int p = 1, n = 0;
     for (int s = 0; s < pl.nst; ++s) {
-        int R = rad[s];
+        const int R = rs.best[s];
         pl.rad[s] = R;
-        pl.tw_off[s] = (int)tw.size();
+        pl.tw_off[s] = n;
         for (int k = 0; k < p; ++k)
             for (int r = 1; r < R; ++r) {
-                double ang = -2.0 * kPi * (double)r * (double)k / (double)(p * R);
-                tw.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
+                const double ang = -2.0 * kPi * (double)r * (double)k / (double)(p * R);
+                tw[n++] = make_float2((float)std::cos(ang), (float)std::sin(ang));
             }
         p *= R;
     }
-    if (tw.empty()) tw.push_back(make_float2(1.f, 0.f));
+    if (n == 0) tw[n++] = make_float2(1.f, 0.f);
+    *ntw = n;
+    return nullptr;
+}
+
+int make_axis_plan(int N, AxisPlan& pl, std::vector<float2>& tw, std::string& err) {
+    tw.assign((size_t)std::max(N, 1), make_float2(0.f, 0.f));
+    int ntw = 0;
+    if (const char* why = plan_axis(N, pl, tw.data(), &ntw)) {
+        err = std::string(why) + " (N = " + std::to_string(N) + ")";
+        tw.clear();
+        return -TILD_ERR_UNSUPPORTED;
+    }
+    tw.resize(ntw);
     return 0;
 }
 
@@ -1309,7 +1326,9 @@ int Batch::nlist(int id, float rc, const int* sites, int nsites, int* cell_of, i
     if (!neigh) return 0;   // size query
     if (!cell_of || !counts || cap < (size_t)std::max(a.cmax, 1)) { err = "nlist: output buffers too small"; return -TILD_ERR_ARG; }
     if (a.cmax < 1) { err = "nlist: cell capacity below one site"; return -TILD_ERR_CONFIG; }
-    a.cap = a.cmax;
+    // cells hold the reference's CELL_MAX sites; lists hold what the caller has room
+    // for (the reference's hold CELL_MAX and are overrun beyond it)
+    a.cap = (int)std::min(cap, (size_t)1 << 20);
     std::vector<unsigned char> mem(ns, sites ? 0 : 1);
     if (sites)
         for (int k = 0; k < nsites; ++k) {
@@ -1352,7 +1371,8 @@ int Batch::nlist(int id, float rc, const int* sites, int nsites, int* cell_of, i
     }
     cleanup();
     if (e != hipSuccess) { err = std::string("nlist: ") + hipGetErrorString(e); return -TILD_ERR_DEVICE; }
-    if (ovf) { err = "nlist: more sites in a cell or list than CELL_MAX (nlist.cu:49-51)"; return -TILD_ERR_CONFIG; }
+    if (ovf & 1) { err = "nlist: more sites in a cell than CELL_MAX (nlist.cu:49-51)"; return -TILD_ERR_CONFIG; }
+    if (ovf & 2) { err = "nlist: a neighbour list is longer than cap"; return -TILD_ERR_CONFIG; }
     for (int i = 0; i < ns; ++i) {
         const int c = std::min(counts[i], a.cap);
         std::copy(nb.begin() + (size_t)i * a.cap, nb.begin() + (size_t)i * a.cap + c, neigh + (size_t)i * cap);

@@ -176,6 +176,11 @@ private:
 
 // Plan of one FFT axis: radices and twiddle table (forward sign).
 int make_axis_plan(int N, AxisPlan& pl, std::vector<float2>& tw, std::string& err);
+// The same without allocating: tw has room for max(N, 1) values, *ntw receives the
+// count (N - 1, or 1 for N = 1).  Returns nullptr, or why N has no plan.
+const char* plan_axis(int N, AxisPlan& pl, float2* tw, int* ntw);
+// Message of the last failed C ABI call on this thread (tild_last_error).
+void set_last_error(const std::string& msg);
 
 // Tile shape of the sorted-bin deposition (By | Ny, Bz | Nz, halo below one tile,
 // LDS and FFT-budget limits).  Returns false when no shape fits.  `target_tiles`

@@ -93,6 +93,7 @@ void Server::loop() {
             const double ts = now();
             const int id = batch_->add(*J->sys, J->seed, err);
             std::lock_guard<std::mutex> lk(mu_);
+            admit_s_ += now() - ts;
             bytes_in_use_ -= J->bytes;
             if (id < 0) {
                 J->state = TILD_JOB_FAILED;
@@ -150,6 +151,7 @@ void Server::loop() {
             }
             if (rc) fatal_ = err;
         }
+        const double tr = now();
         for (Job* J : fin) {
             std::string e2;
             if (J->state != TILD_JOB_FAILED) {
@@ -168,7 +170,11 @@ void Server::loop() {
             J->t_done = now();
             running_.erase(std::find(running_.begin(), running_.end(), J->id));
         }
-        if (!fin.empty()) done_cv_.notify_all();
+        if (!fin.empty()) {
+            std::lock_guard<std::mutex> lk(mu_);
+            retire_s_ += now() - tr;
+            done_cv_.notify_all();
+        }
         if (rc) {
             // a device error ends the server: fail whatever is still queued
             std::lock_guard<std::mutex> lk(mu_);
@@ -247,14 +253,18 @@ static double quantile(std::vector<double> v, double q) {
 void Server::stats(tild_server_stats* s) {
     std::lock_guard<std::mutex> lk(mu_);
     *s = tild_server_stats{};
-    std::vector<double> lat;
+    std::vector<double> lat, wait, svc;
     for (auto& kv : jobs_) {
         const Job* J = kv.second.get();
         s->submitted += 1;
         if (J->state == TILD_JOB_QUEUED) s->queued += 1;
         else if (J->state == TILD_JOB_RUNNING) s->running += 1;
-        else if (J->state == TILD_JOB_DONE) { s->done += 1; lat.push_back(J->t_done - J->t_submit); }
-        else s->failed += 1;
+        else if (J->state == TILD_JOB_DONE) {
+            s->done += 1;
+            lat.push_back(J->t_done - J->t_submit);
+            wait.push_back(J->t_start - J->t_submit);
+            svc.push_back(J->t_done - J->t_start);
+        } else s->failed += 1;
     }
     s->bytes_in_use = bytes_in_use_;
     s->peak_bytes = peak_bytes_;
@@ -264,6 +274,12 @@ void Server::stats(tild_server_stats* s) {
     s->bead_steps = bead_steps_;
     s->p50_s = quantile(lat, 0.50);
     s->p99_s = quantile(lat, 0.99);
+    s->admit_s = admit_s_;
+    s->retire_s = retire_s_;
+    s->wait50_s = quantile(wait, 0.50);
+    s->wait99_s = quantile(wait, 0.99);
+    s->svc50_s = quantile(svc, 0.50);
+    s->svc99_s = quantile(svc, 0.99);
 }
 
 }  // namespace tild

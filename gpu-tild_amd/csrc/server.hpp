@@ -66,7 +66,7 @@ private:
     std::vector<int> running_;
     int next_id_ = 0;
     size_t bytes_in_use_ = 0, peak_bytes_ = 0;
-    double t0_ = 0, busy_s_ = 0;
+    double t0_ = 0, busy_s_ = 0, admit_s_ = 0, retire_s_ = 0;
     double bead_steps_ = 0;
     std::string fatal_;
 };

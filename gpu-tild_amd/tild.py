@@ -75,7 +75,9 @@ class ServerStats(ctypes.Structure):
                 ("done", ctypes.c_int), ("failed", ctypes.c_int), ("bytes_in_use", ctypes.c_size_t),
                 ("peak_bytes", ctypes.c_size_t), ("budget", ctypes.c_size_t), ("busy_s", ctypes.c_double),
                 ("elapsed_s", ctypes.c_double), ("bead_steps", ctypes.c_double), ("p50_s", ctypes.c_double),
-                ("p99_s", ctypes.c_double)]
+                ("p99_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("retire_s", ctypes.c_double),
+                ("wait50_s", ctypes.c_double), ("wait99_s", ctypes.c_double), ("svc50_s", ctypes.c_double),
+                ("svc99_s", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -311,15 +313,16 @@ class Engine:
                   "profile")
         return dict(zip(self.PHASES, a.tolist()))
 
-    def nlist(self, rc, sites=None):
-        """(cell_of, counts, neigh (ns, cell_max), cell_max, ncells) by site id."""
+    def nlist(self, rc, sites=None, cap=None):
+        """(cell_of, counts, neigh (ns, cap), cell_max, ncells) by site id; cap defaults to
+        the reference's CELL_MAX."""
         cm, nc = ctypes.c_int(), ctypes.c_int()
         sarr = None if sites is None else np.ascontiguousarray(sites, dtype=np.int32)
         sp = _ip(sarr) if sarr is not None else None
         nsites = 0 if sarr is None else len(sarr)
         self._chk(lib().tild_engine_nlist(self._h, float(rc), sp, nsites, None, None, None, 0,
                                           ctypes.byref(cm), ctypes.byref(nc)), "nlist size")
-        cap = max(cm.value, 1)
+        cap = max(cm.value, 1) if cap is None else int(cap)
         cell_of = np.zeros(self.ns, dtype=np.int32)
         counts = np.zeros(self.ns, dtype=np.int32)
         neigh = np.full((self.ns, cap), -1, dtype=np.int32)
@@ -498,3 +501,4 @@ def fft_c2r(dim, Nx, batch, d_in, d_out, stream=None):
     arr = (ctypes.c_int * 3)(*(list(Nx) + [1] * (3 - len(Nx))))
     _check(lib().tild_fft_c2r(dim, arr, batch, ctypes.c_void_p(d_in), ctypes.c_void_p(d_out),
                               ctypes.c_void_p(stream) if stream else None), "tild_fft_c2r")
+

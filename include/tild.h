@@ -176,8 +176,10 @@ tild_status tild_engine_set_state(tild_engine* e, int step, const float* xo_disp
    sites; tild_system_group_sites lists a deck group).  Outputs, by site id: cell_of[ns]
    (x-fastest cell index, -1 outside the group), counts[ns] and neigh[ns*cap], each list
    sorted ascending and including the site itself.  cap >= *cell_max (the reference's
-   CELL_MAX, nlist.cu:49-51); neigh == NULL only returns *cell_max and *ncells.  More sites
-   in a cell or a list than CELL_MAX is TILD_ERR_CONFIG (the reference overruns). */
+   CELL_MAX, nlist.cu:49-51); neigh == NULL only returns *cell_max and *ncells.  Cells hold
+   CELL_MAX sites and lists up to cap: more sites in a cell than CELL_MAX, or a list longer
+   than cap, is TILD_ERR_CONFIG.  The reference's lists hold CELL_MAX and it writes past
+   them, so a cap above CELL_MAX returns correct lists where the reference overruns. */
 tild_status tild_engine_nlist(tild_engine* e, float rc, const int* sites, int nsites,
                               int* cell_of, int* counts, int* neigh, size_t cap,
                               int* cell_max, int* ncells);
@@ -238,6 +240,10 @@ typedef struct {
     double elapsed_s;     /* since tild_server_create */
     double bead_steps;    /* sum over chunks of (beads running) x (steps) */
     double p50_s, p99_s;  /* time from submission to result of finished jobs */
+    double admit_s;       /* worker time setting up admitted jobs (tables, uploads) */
+    double retire_s;      /* worker time copying out and releasing finished jobs */
+    double wait50_s, wait99_s;  /* finished jobs: submission to admission (queueing) */
+    double svc50_s, svc99_s;    /* finished jobs: admission to result (service) */
 } tild_server_stats;
 
 tild_status tild_server_create(const tild_server_opts* opts, tild_server** out);

@@ -88,7 +88,8 @@ def test_batched_spatial_reorder_is_invisible(which, tmp_path):
     systems = [tild.System(t, d) for t, d in chosen]
     out = []
     for every in (3, -1):
-        b = tild.Batch(device=0, reorder_every=every)
+        # the tiled set is forced onto the tiled spread, so the set-wide reorder runs
+        b = tild.Batch(device=0, reorder_every=every, spread_mode=2 if which == "tiled" else 0)
         ids = [b.add(s, 40 + i) for i, s in enumerate(systems)]
         b.step(4, graph=True)
         b.step(9, graph=True)

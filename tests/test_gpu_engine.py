@@ -454,3 +454,71 @@ def test_restart_from_saved_state_is_exact(case, tmp_path):
     print(f"[{case}] restart at step 7 from saved state: 13 more steps bit-identical")
     ref.close()
     res.close()
+
+
+GROUP_CASES = {
+    # two type groups with different integrators (integrator.cu:24-39, group.cu:44-60)
+    "em_and_gjf": ("group A type 1\ngroup B type 2\nintegrator A EM\nintegrator B GJF", False),
+    "gjf_and_em_2d": ("group A type 1\ngroup B type 2\nintegrator A GJF\nintegrator B EM", False),
+    # only type 2 is integrated: type-1 sites never move
+    "one_group": ("group A type 2\nintegrator A GJF", True),
+}
+
+
+@pytest.mark.parametrize("case", list(GROUP_CASES))
+def test_group_integrators_match_oracle(case, tmp_path):
+    lines, frozen = GROUP_CASES[case]
+    if case.endswith("2d"):
+        text, d = decks.melt2d(tmp_path, delt=0.01)
+    else:
+        text, d = decks.melt3d(tmp_path, styles=decks.MIXED, delt=0.01)
+    assert "integrator all EM" in text
+    text = text.replace("integrator all EM", lines + "\ndiffusivity 2 0.5")
+    s, e, o = _pair(text, d, seed=19)
+    x0 = s.positions()
+    types = s.types()
+    L = [s.info("L0"), s.info("L1"), s.info("L2")][:s.dim]
+    for n in (1, 9, 10):
+        e.step(n)
+        o.run(n)
+    x = e.positions()
+    err = np.abs(U.min_image(x.astype(np.float64) - o.positions(), L)).max()
+    print(f"[{case}] groups {int(s.info('n_groups'))}, integrators {int(s.info('n_integrators'))}: "
+          f"20-step max |dx| vs FP64 {err:.2e}")
+    assert err < 1e-4
+    if frozen:
+        still = types == 0                    # type 1 (0-based 0) is in no integrator group
+        assert np.array_equal(x[still], x0[still])
+        assert not np.array_equal(x[~still], x0[~still])
+    p, po = e.properties(), o.properties()
+    assert abs(p["Upe"] - po["Upe"]) <= 1e-5 * abs(po["Upe"])
+    e.close()
+
+
+def _many_types(tmp, ntypes, grid):
+    import tild_gen
+    rng = np.random.default_rng(ntypes)
+    box = (2.0, 2.0, 0.5 * grid[2])
+    pos = rng.random((600, 3)) * np.asarray(box)
+    types = np.arange(600) % ntypes + 1
+    tild_gen.write_data(tmp / "input.data", pos, types, np.arange(1, 601), [], box, ntypes, 3)
+    styles = [("gaussian", t, t, 1.0, 0.5) for t in range(1, ntypes + 1)] + [("gaussian", 1, ntypes, 2.0, 0.5)]
+    return tild_gen.deck("input.data", 3, grid, styles).replace("bond 1 harmonic 1.5 0.0\n", ""), tmp
+
+
+def test_type_count_limit_of_the_fused_pass(tmp_path):
+    # the fused last-axis pass holds every field's line in one block: nf * N_last <= 2048
+    # (12 types on a 160-point axis fit, 13 do not and are refused)
+    (tmp_path / "a").mkdir()
+    (tmp_path / "b").mkdir()
+    text, d = _many_types(tmp_path / "a", 12, (4, 4, 160))
+    s, e, o = _pair(text, d)
+    ef = U.rel_norm(e.forces(), o.forces())
+    print(f"12 types, N_last 160: |df|/|f| = {ef:.2e}")
+    assert ef < 1e-5
+    e.close()
+    text, d = _many_types(tmp_path / "b", 13, (4, 4, 160))
+    s = tild.System(text, d)
+    with pytest.raises(tild.TildError) as ei:
+        tild.Engine(s, device=0)
+    assert ei.value.status == 8

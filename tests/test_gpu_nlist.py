@@ -54,6 +54,14 @@ def test_nlist_overflow_is_an_error(tmp_path):
     with pytest.raises(tild.TildError) as ei:
         e.nlist(1.5)
     assert ei.value.status == 2
+    # with room beyond CELL_MAX the lists come back whole, repeats included
+    o = O.System(text, d, seed=1)
+    cell_of, counts, neigh, cmax, nc = e.nlist(1.5, cap=8 * 1024)
+    ocell, ocounts, oneigh, _, _ = o.nlist(1.5)
+    assert counts.max() > cmax
+    assert np.array_equal(cell_of, ocell) and np.array_equal(counts, ocounts)
+    for i in range(s.ns):
+        assert np.array_equal(neigh[i, :counts[i]], oneigh[i, :counts[i]]), i
     e.close()
 
 
