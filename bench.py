@@ -333,14 +333,21 @@ def leg_c3_sweep(args, decks):
 
 
 def leg_sweep(args, decks):
-    """256 uneven replicas: all advanced together (one launch per phase, graph replay)
-    versus each advanced alone with eager launches; FP64 error of sampled replicas at
-    their final positions."""
+    """256 uneven replicas: all advanced together (one launch per phase, graph replay,
+    block geometry from the autotuner) versus each advanced alone with eager launches;
+    FP64 error of sampled replicas at their final positions."""
     import numpy as np
     import tild
     systems = [tild.System(t, d) for t, d in decks]
-    b = tild.Batch(device=0)
+    b = tild.Batch(device=0, autotune=not args.no_autotune)
+    t0 = time.perf_counter()
     ids = [b.add(s, args.seed + k) for k, s in enumerate(systems)]
+    t_add = time.perf_counter() - t0
+    geo = [b.geometry(r) for r in ids]
+    tuning = {"autotune": not args.no_autotune, "add_s": t_add,
+              "classes_measured": sum(1 for g in geo if g["tuned"] and not g["from_cache"]),
+              "from_cache": sum(1 for g in geo if g["from_cache"]),
+              "sum_tuned_step_ms": sum(g["step_ms"] for g in geo)}
     beads = sum(s.ns for s in systems)
     b.step(5, graph=True)
     b.step(5, graph=False)
@@ -382,7 +389,7 @@ def leg_sweep(args, decks):
             "replicas": len(ids), "beads_total": beads, "steps": args.sweep_steps,
             "value": batched, "unit": "bead-steps/s", "ms_per_step": tb / args.sweep_steps * 1e3,
             "plain_value": plain, "plain_steps": args.sweep_plain_steps, "speedup_vs_plain": batched / plain,
-            "fp64_error_final_state": errs}
+            "tuning": tuning, "fp64_error_final_state": errs}
 
 
 def leg_serve(args, jobs):
@@ -624,6 +631,7 @@ def main():
     ap.add_argument("--sweep-steps", type=int, default=200)
     ap.add_argument("--sweep-plain-steps", type=int, default=20)
     ap.add_argument("--sweep-oracle-sample", type=int, default=4)
+    ap.add_argument("--no-autotune", action="store_true", help="batched sweep without the block-geometry autotuner")
     ap.add_argument("--serve-jobs", type=int, default=500, help="0 skips the served-stream leg")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-chunk", type=int, default=100)

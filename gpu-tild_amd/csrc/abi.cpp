@@ -78,6 +78,7 @@ tild_status tild_system_info(const tild_system* s, const char* key, double* v) {
     else if (k == "binary_freq") *v = S.bin_freq;
     else if (k == "n_groups") *v = (double)S.Groups.size();
     else if (k == "n_integrators") *v = (double)S.Integrators.size();
+    else if (k == "n_computes") *v = (double)S.computes.size();
     else if (k == "seed") *v = S.rand_seed;
     else if (k == "L0") *v = S.L[0];
     else if (k == "L1") *v = S.L[1];
@@ -132,6 +133,15 @@ tild_status tild_system_get_bonds(const tild_system* s, int* nb, int* bt, int* t
     std::memcpy(ty, s->sys.bond_type.data(), ns * 3 * sizeof(int));
     return TILD_OK;
 }
+tild_status tild_system_compute(const tild_system* s, int i, int* type, int* freq, int* wait) {
+    if (!s || i < 0 || i >= (int)s->sys.computes.size()) return fail(TILD_ERR_ARG, "no such compute");
+    const tild::SkCompute& c = s->sys.computes[i];
+    if (type) *type = c.type;
+    if (freq) *freq = c.freq;
+    if (wait) *wait = c.wait;
+    return TILD_OK;
+}
+
 tild_status tild_system_group_sites(const tild_system* s, int g, int* sites, size_t cap, int* nsites) {
     if (!s || g < 0 || g >= (int)s->sys.Groups.size()) return fail(TILD_ERR_ARG, "no such group");
     const auto& idx = s->sys.Groups[g].index;
@@ -153,6 +163,8 @@ tild_status tild_engine_create(const tild_system* s, const tild_opts* opts, tild
     e->batch->set_spread(o.spread_mode, o.tile_target > 0 ? o.tile_target : 512, o.tile_by, o.tile_bz, o.tile_slack);
     e->batch->set_reorder(o.reorder_every == 0 ? -1 : (o.reorder_every < 0 ? 0 : o.reorder_every));
     e->batch->set_fft_tuning({o.fft_mid_elems, o.fft_last_lines, o.fft_c2r_rows, o.fft_plane, o.fft_last_dirs});
+    e->batch->set_deposit_threads(o.deposit_threads);
+    e->batch->set_autotune(o.autotune != 0);
     std::string err;
     int rc = e->batch->init(err);
     if (rc < 0) return from_rc(rc, err, nullptr);
@@ -206,6 +218,10 @@ tild_status tild_engine_get_forces(tild_engine* e, float* f, size_t n) { ENGINE_
 tild_status tild_engine_get_density(tild_engine* e, int t, float* rho, size_t n) { ENGINE_COPY(get_density(e->id, t, rho, n, err)) }
 tild_status tild_engine_get_grid_field(tild_engine* e, int t, int d, float* g, size_t n) { ENGINE_COPY(get_grid_field(e->id, t, d, g, n, err)) }
 tild_status tild_engine_synchronize(tild_engine* e) { ENGINE_COPY(sync(err)) }
+tild_status tild_engine_sk_accumulate(tild_engine* e, int slot, int t) { ENGINE_COPY(sk_accumulate(e->id, slot, t, err)) }
+tild_status tild_engine_sk_average(tild_engine* e, int slot, float* sk, size_t n, int* count) {
+    ENGINE_COPY(sk_average(e->id, slot, sk, n, count, err))
+}
 tild_status tild_engine_get_bins(tild_engine* e, int* base, size_t n) { ENGINE_COPY(get_bins(e->id, base, n, err)) }
 
 tild_status tild_engine_profile(tild_engine* e, int nsteps, double* phase_ms, int nphase) {
@@ -226,6 +242,8 @@ tild_status tild_batch_create(const tild_opts* opts, tild_batch** out) {
     b->batch->set_spread(o.spread_mode, o.tile_target > 0 ? o.tile_target : 512, o.tile_by, o.tile_bz, o.tile_slack);
     b->batch->set_reorder(o.reorder_every == 0 ? -1 : (o.reorder_every < 0 ? 0 : o.reorder_every));
     b->batch->set_fft_tuning({o.fft_mid_elems, o.fft_last_lines, o.fft_c2r_rows, o.fft_plane, o.fft_last_dirs});
+    b->batch->set_deposit_threads(o.deposit_threads);
+    b->batch->set_autotune(o.autotune != 0);
     std::string err;
     int rc = b->batch->init(err);
     if (rc < 0) return from_rc(rc, err, nullptr);
@@ -282,6 +300,31 @@ tild_status tild_batch_replica_step(tild_batch* b, int r, int* step) {
     *step = b->batch->rep(r)->step;
     return TILD_OK;
 }
+// Block geometry of a replica (the tuner's pick, or the settings it was built with).
+static tild_status geometry_of(tild::Replica* R, tild_geometry* out) {
+    if (!R || !out) return fail(TILD_ERR_ARG, "no such replica");
+    std::memset(out, 0, sizeof(*out));
+    const tild::RepDev& h = R->h;
+    out->tile_target = R->tune.tuned ? R->tune.g.tile_target : 0;
+    out->tiles = h.spread_mode == 2 ? h.ntiles : 0;
+    out->deposit_threads = R->dep_threads;
+    out->plane = h.plane_fwd;
+    out->last_lines = h.nl_last; out->last_dirs = h.last_dirs; out->mid_elems = h.mid_elems;
+    out->c2r_rows = h.nr_c2r;
+    out->tuned = R->tune.tuned ? 1 : 0;
+    out->from_cache = R->tune.from_cache ? 1 : 0;
+    out->step_ms = R->tune.step_ms;
+    return TILD_OK;
+}
+tild_status tild_batch_geometry(tild_batch* b, int r, tild_geometry* out) {
+    if (!b) return fail(TILD_ERR_ARG, "null batch");
+    return geometry_of(b->batch->rep(r), out);
+}
+tild_status tild_engine_geometry(tild_engine* e, tild_geometry* out) {
+    if (!e) return fail(TILD_ERR_ARG, "null engine");
+    return geometry_of(e->batch->rep(e->id), out);
+}
+
 tild_status tild_batch_profile(tild_batch* b, const int* replicas, int nrep, int nsteps, double* ms, int nphase) {
     if (!b || nrep < 0 || (nrep > 0 && !replicas)) return fail(TILD_ERR_ARG, "bad argument");
     std::vector<int> ids(replicas, replicas + nrep);

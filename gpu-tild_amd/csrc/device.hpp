@@ -25,8 +25,11 @@ struct AxisPlan {
     int nst;
     int rad[kMaxStages];
     int tw_off[kMaxStages];
+    int ntw;            // twiddles in the table (at most N)
     const float2* tw;
 };
+// room for the twiddles of an N-point plan (ntw <= tw_elems(N))
+__host__ __device__ inline int tw_elems(int N) { return N > 1 ? N : 1; }
 
 // Per-replica descriptor (device memory).
 struct RepDev {

@@ -71,6 +71,7 @@ const char* plan_axis(int N, AxisPlan& pl, float2* tw, int* ntw) {
         p *= R;
     }
     if (n == 0) tw[n++] = make_float2(1.f, 0.f);
+    pl.ntw = n;
     *ntw = n;
     return nullptr;
 }
@@ -146,6 +147,7 @@ Batch::~Batch() {
         if (kv.second->d_ids) (void)hipFree(kv.second->d_ids);
         drop_graphs(*kv.second);
     }
+    free_maps(reorder_maps_);
     for (auto& r : reps_)
         if (r) {
             free_maps(r->maps);
@@ -157,7 +159,7 @@ Batch::~Batch() {
 
 void Batch::free_maps(PhaseMap* maps) {
     for (int p = 0; p < PH_COUNT; ++p)
-        if (maps[p].dev) { (void)hipFree(maps[p].dev); maps[p].dev = nullptr; }
+        if (maps[p].dev) { (void)hipFree(maps[p].dev); maps[p].dev = nullptr; maps[p].cap = 0; }
 }
 
 int Batch::init(std::string& err) {
@@ -186,6 +188,16 @@ size_t Batch::estimate_bytes(const System& S) {
 
 // ---------------------------------------------------------------- replica setup
 int Batch::add(const System& S, uint32_t seed, std::string& err) {
+    if (!autotune_ || tuning_) return add_plain(S, seed, err);
+    HIPCHK(hipSetDevice(device_));
+    TuneResult t;
+    if (int rc = tune_geometry(S, seed, t, err)) return rc;
+    const int id = add_with(S, seed, t.g, err);
+    if (id >= 0) reps_[id]->tune = t;
+    return id;
+}
+
+int Batch::add_plain(const System& S, uint32_t seed, std::string& err) {
     HIPCHK(hipSetDevice(device_));
     auto Rp = std::make_unique<Replica>();
     Replica& R = *Rp;
@@ -443,6 +455,7 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
     if (id < 0) { id = (int)reps_.size(); reps_.emplace_back(); }
     reps_[id] = std::move(Rp);
     Replica& RR = *reps_[id];
+    RR.dep_threads = dep_threads_;
     if (int rc = upload_desc(id, err)) { remove(id); return rc; }
     if (int rc = build_maps({id}, RR.maps, err)) { remove(id); return rc; }
     if (int rc = build_tables(RR, S, err)) { remove(id); return rc; }
@@ -522,9 +535,8 @@ void Batch::mirror_reorder(Replica& R) {
 }
 
 int Batch::reorder_set(const std::vector<int>& ids, std::string& err) {
-    SetMaps* m = maps_for(ids, err);
-    if (!m) return -TILD_ERR_DEVICE;
-    PhaseMap* mp = m->maps;
+    if (int rc = build_maps(ids, reorder_maps_, err, true)) return rc;
+    PhaseMap* mp = reorder_maps_;
     auto ctx = [&](Phase p) { return LaunchCtx{d_reps_, mp[p].dev, mp[p].n, mp[p].lds, stream_}; };
     // the claims in the regions move with the slots: clear the current parity's
     // counters of every replica (zero already where there were none)
@@ -585,7 +597,7 @@ int Batch::upload_desc(int id, std::string& err) {
 }
 
 // ---------------------------------------------------------------- work maps
-int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& err) {
+int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& err, bool reorder_only) {
     for (int p = 0; p < PH_COUNT; ++p) { maps[p].host.clear(); maps[p].lds = 0; maps[p].n = 0; }
     for (int id : ids) {
         const Replica& R = *reps_[id];
@@ -596,6 +608,12 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
             maps[ph].lds = std::max(maps[ph].lds, lds);
         };
         add(PH_PART, R.nblk_part, 0);
+        if (reorder_only) {   // the passes of reorder_set (tiled replicas)
+            add(PH_TCOUNT, (h.ns + kCountBlock - 1) / kCountBlock, 0);
+            add(PH_SCAN, 1, 0);
+            add(PH_RTILE, h.ntiles, 0);
+            continue;
+        }
         const int npairs = (h.nf * rows + 1) / 2;
         const int nbx = (npairs + h.nl_x - 1) / h.nl_x;
         add(PH_XP, nbx, lds_bytes(h.Nx, h.nl_x));
@@ -631,9 +649,14 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
     // larger block idle, and the deposit of a sweep of small replicas took twice as long)
     {
         long long beads = 0, tiles = 0;
+        int want = -1;   // the block size the set's tiled replicas ask for, if they agree
         for (int id : ids)
-            if (reps_[id]->h.spread_mode == 2) { beads += reps_[id]->ns; tiles += reps_[id]->h.ntiles; }
-        maps[PH_TILE].threads = tiles > 0 && beads >= 600 * tiles ? 512 : 256;
+            if (reps_[id]->h.spread_mode == 2) {
+                beads += reps_[id]->ns; tiles += reps_[id]->h.ntiles;
+                const int w = reps_[id]->dep_threads;
+                want = want == -1 ? w : (want == w ? want : 0);
+            }
+        maps[PH_TILE].threads = want == 256 || want == 512 ? want : (tiles > 0 && beads >= 600 * tiles ? 512 : 256);
     }
     // Workgroups go to the 8 XCDs round-robin (launch block w runs on XCD w % 8).  Every
     // pass hands each XCD a contiguous run of work items: for the particle-order passes
@@ -653,14 +676,29 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
         v.swap(o);
     }
     for (int p = 0; p < PH_COUNT; ++p) {
-        if (maps[p].dev) { (void)hipFree(maps[p].dev); maps[p].dev = nullptr; }
         maps[p].n = (int)maps[p].host.size();
         if (maps[p].lds > ((p == PH_PLANEF || p == PH_LAST) ? kPlaneLds : (size_t)kMaxLds)) { err = "pass needs more LDS than a block may use"; return -TILD_ERR_UNSUPPORTED; }
+        if (maps[p].dev && (!reorder_only || maps[p].cap < maps[p].n)) {
+            // reused buffers are last read by the previous reorder's launches
+            HIPCHK(hipStreamSynchronize(stream_));
+            (void)hipFree(maps[p].dev);
+            maps[p].dev = nullptr;
+            maps[p].cap = 0;
+        }
         if (maps[p].n) {
-            HIPCHK(hipMalloc((void**)&maps[p].dev, maps[p].n * sizeof(WorkItem)));
-            HIPCHK(hipMemcpy(maps[p].dev, maps[p].host.data(), maps[p].n * sizeof(WorkItem), hipMemcpyHostToDevice));
+            if (!maps[p].dev) {
+                const int cap = reorder_only ? std::max(maps[p].n, 2 * maps[p].cap) : maps[p].n;
+                HIPCHK(hipMalloc((void**)&maps[p].dev, (size_t)cap * sizeof(WorkItem)));
+                maps[p].cap = cap;
+            }
+            // from pageable host memory: staged before the call returns, so the host
+            // vector may change afterwards; ordered on the stream after the previous
+            // reorder's launches
+            HIPCHK(hipMemcpyAsync(maps[p].dev, maps[p].host.data(), maps[p].n * sizeof(WorkItem),
+                                  hipMemcpyHostToDevice, stream_));
         }
     }
+    if (!reorder_only) HIPCHK(hipStreamSynchronize(stream_));
     return 0;
 }
 
@@ -1153,6 +1191,61 @@ int Batch::properties(int id, tild_props* out, std::string& err) {
     for (int c = 0; c < nP; ++c) out->Ptens[c] = P[c] + res[R.nq + 1 + c] * inv;
     out->Upe = Ust + out->Uelec + out->Ubond;
     out->step = R.step;
+    return 0;
+}
+
+// ---------------------------------------------------------------- avg_sk compute
+int Batch::sk_accumulate(int id, int slot, int type, std::string& err) {
+    Replica* Rp = rep(id);
+    if (!Rp || type < 0 || type >= Rp->ntypes || slot < 0 || slot >= kMaxSkSlots) {
+        err = "sk_accumulate: bad replica, slot or type";
+        return -TILD_ERR_ARG;
+    }
+    Replica& R = *Rp;
+    HIPCHK(hipSetDevice(device_));
+    if ((int)R.sk_buf.size() <= slot) {
+        R.sk_buf.resize(slot + 1, nullptr);
+        R.sk_type.resize(slot + 1, -1);
+        R.sk_count.resize(slot + 1, 0);
+    }
+    if (!R.sk_buf[slot]) {
+        if (int rc = dalloc(R, &R.sk_buf[slot], (size_t)R.Mh * 2, err, true)) return rc;
+        R.sk_type[slot] = type;
+    }
+    if (R.sk_type[slot] != type) { err = "sk_accumulate: slot holds another type's sum"; return -TILD_ERR_ARG; }
+    if (int rc = forward_spectra(id, err)) return rc;
+    HIPCHK(launch_sk_accumulate(stream_, d_reps_, id, type, R.Mh, R.sk_buf[slot]));
+    R.sk_count[slot] += 1;
+    return 0;
+}
+
+int Batch::sk_average(int id, int slot, float* out, size_t n, int* count, std::string& err) {
+    Replica* Rp = rep(id);
+    if (!Rp || slot < 0 || slot >= kMaxSkSlots || !out || n != 2 * (size_t)Rp->M) {
+        err = "sk_average: bad replica, slot or size";
+        return -TILD_ERR_ARG;
+    }
+    Replica& R = *Rp;
+    const int cnt = slot < (int)R.sk_count.size() ? R.sk_count[slot] : 0;
+    if (count) *count = cnt;
+    if (cnt == 0) { std::fill(out, out + n, 0.f); return 0; }
+    HIPCHK(hipSetDevice(device_));
+    std::vector<double> h((size_t)R.Mh * 2);
+    HIPCHK(hipMemcpyAsync(h.data(), R.sk_buf[slot], h.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    // full grid from the half spectrum: rho(-k) = conj rho(k) for real rho, so the
+    // square at -k is the conjugate of the square at k
+    const int Nx = R.h.Nx, Ny = R.h.Ny, Nz = R.dim == 3 ? R.h.Nz : 1, Nxh = R.h.Nxh;
+    for (int z = 0; z < Nz; ++z)
+        for (int y = 0; y < Ny; ++y)
+            for (int x = 0; x < Nx; ++x) {
+                const bool lower = x < Nxh;
+                const int hx = lower ? x : Nx - x, hy = lower ? y : (Ny - y) % Ny, hz = lower ? z : (Nz - z) % Nz;
+                const size_t hi = ((size_t)hz * Ny + hy) * Nxh + hx;
+                const size_t fi = ((size_t)z * Ny + y) * Nx + x;
+                out[2 * fi] = (float)(h[2 * hi] / cnt);
+                out[2 * fi + 1] = (float)((lower ? h[2 * hi + 1] : -h[2 * hi + 1]) / cnt);
+            }
     return 0;
 }
 

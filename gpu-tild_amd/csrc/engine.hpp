@@ -39,12 +39,30 @@ struct PhaseMap {
     size_t lds = 0;
     int n = 0;
     int threads = 0;   // block size where a pass has more than one (0: its default)
+    int cap = 0;       // work items the device buffer holds
 };
 
 // A ramped pair style: its index and ramp schedule (RepDev::ramp_*).
 struct RampInfo {
     int style;
     double A0, c0, c1;
+};
+
+constexpr int kMaxSkSlots = 64;   // avg_sk computes per replica
+
+// Block geometry of a replica: settings that change only how the work of a step is cut
+// into blocks, never a result -- the tile count of the tiled spread (tile shape), the
+// deposit block size, the forward plane pass, and the line batching of the FFT passes
+// (0 = the built-in default for each).
+struct Geometry {
+    int tile_target = 512, dep_threads = 0, plane = 0, last_lines = 0, last_dirs = 0, mid_elems = 0, c2r_rows = 0;
+};
+// What the autotuner picked for a replica: the geometry, its measured kernel time per
+// step, and whether it came from the tuning cache.
+struct TuneResult {
+    Geometry g;
+    double step_ms = 0.0;
+    bool from_cache = false, tuned = false;
 };
 
 struct Replica {
@@ -70,6 +88,12 @@ struct Replica {
     bool claimed = false;
     std::vector<RampInfo> ramps;
     ReorderBufs rb;
+    // avg_sk accumulators, one per compute slot: sum of rho(k) rho(k) / M over the half
+    // spectrum (FP64 re, im), the type it follows and the number of evaluations summed
+    std::vector<double*> sk_buf;
+    std::vector<int> sk_type, sk_count;
+    int dep_threads = 0;              // deposit block size asked for (0: by bead density)
+    TuneResult tune;                  // the autotuner's pick (tune.tuned false: not tuned)
 };
 
 class Batch {
@@ -87,6 +111,13 @@ public:
     // per middle-axis block, lines per fused last-axis block, rows per inverse-x block
     struct FftTuning { int mid_elems = 0, last_lines = 0, c2r_rows = 0, plane = 0, last_dirs = 0; };
     void set_fft_tuning(const FftTuning& t) { tune_ = t; }
+    // Deposit block size of replicas added from now on (0: by bead density, else 256 or 512).
+    void set_deposit_threads(int n) { dep_threads_ = n; }
+    // Autotuning of replicas added from now on: each replica's block geometry is taken
+    // from the tuning cache for its class (device, grid, field count, order, bead-count
+    // octave), or measured by a sweep over the candidates and stored there.  The
+    // explicit tile and FFT settings above are then ignored.
+    void set_autotune(bool on) { autotune_ = on; }
     ~Batch();
     int init(std::string& err);
 
@@ -106,6 +137,13 @@ public:
     int profile(const std::vector<int>& ids, int n, double* ms, int nph, std::string& err);
     // calc_properties() of one replica.
     int properties(int id, tild_props* out, std::string& err);
+    // Compute::doCompute for avg_sk (Compute.cu:83-117): add rho_t(k) rho_t(k) / M of
+    // the densities of the last force evaluation to compute slot `slot`'s running sum
+    // (each compute keeps its own sum, as each Compute object does)
+    int sk_accumulate(int id, int slot, int type, std::string& err);
+    // the running average over all M wavevectors (x fastest, interleaved re, im; zero
+    // before the first evaluation, as Compute::writeResults) and the evaluation count
+    int sk_average(int id, int slot, float* out, size_t n, int* count, std::string& err);
 
     int get_positions(int id, float* x, size_t n, std::string& err);
     int set_positions(int id, const float* x, size_t n, std::string& err);
@@ -144,6 +182,13 @@ public:
                        hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr, bool* launched = nullptr);
 
 private:
+    // the sweep behind set_autotune (autotune.cpp)
+    int tune_geometry(const System& S, uint32_t seed, TuneResult& out, std::string& err);
+    double measure_geometry(const System& S, uint32_t seed, const Geometry& g, std::string& err);
+    int add_with(const System& S, uint32_t seed, const Geometry& g, std::string& err);
+    int add_plain(const System& S, uint32_t seed, std::string& err);
+    bool autotune_ = false, tuning_ = false;
+    int dep_threads_ = 0;
     int device_;
     int spread_mode_ = 0, tile_target_ = 512, tile_by_ = 0, tile_bz_ = 0, tile_slack_ = 0, reorder_every_ = -1;
     FftTuning tune_;
@@ -154,7 +199,11 @@ private:
     std::map<std::vector<int>, std::unique_ptr<SetMaps>> set_cache_;
 
     int upload_desc(int id, std::string& err);
-    int build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& err);
+    // reorder_only: only the phases of a set-wide spatial reorder, into device buffers
+    // that are reused while large enough (the set of replicas due changes from one
+    // reorder to the next, so these maps are not cached per set)
+    int build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& err, bool reorder_only = false);
+    PhaseMap reorder_maps_[PH_COUNT];
     int build_tables(Replica& R, const System& S, std::string& err);
     int forward_spectra(int id, std::string& err);   // rho -> H (properties)
     // spatial reorder of one replica; after_step: the last step's tile binning matches

@@ -21,7 +21,7 @@ struct TmpDev {
 
 #define TRY(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) return TILD_ERR_DEVICE; } while (0)
 
-tild_status arg_error(tild_status s, const char* why) {
+tild_status arg_error(tild_status s, const std::string& why) {
     tild::set_last_error(why);
     return s;
 }

@@ -88,14 +88,34 @@ __global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_e
     float2* Hout = out_override ? out_override : r.H;
     gmem<unsigned long long> acc = G(r.acc);
     gmem<float> rho_out = G(r.rho);
+    float2* twl = lds + 2 * (size_t)N * ldb;   // the x axis' twiddles
+    TwStage<kFftThreads> tws;
+    tws.issue(r.ax[0]);
     const FastDiv dN(N);
     const int type_rows = r.ntypes * rows_per_field;   // rows below are particle densities
-    for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
+    if (!FROM_ACC) {
+        stage_in<8, kFftThreads, float2>(
+            nb * N,
+            [&](int task) {
+                int n;
+                const int b = dN.div(task, n);
+                const int ra = 2 * (p0 + b), rb = ra + 1;
+                const float a = rin[(size_t)ra * N + n];
+                const float bb = rb < nrows ? rin[(size_t)rb * N + n] : 0.f;
+                return make_float2(a, bb);
+            },
+            [&](int task, float2 v) {
+                int n;
+                const int b = dN.div(task, n);
+                lds[n * ldb + b] = v;
+            });
+    }
+    for (int task = threadIdx.x; FROM_ACC && task < nb * N; task += kFftThreads) {
         int n;
         const int b = dN.div(task, n);
         const int ra = 2 * (p0 + b), rb = ra + 1;
         float a, bb = 0.f;
-        if (FROM_ACC) {
+        {
             size_t ia = (size_t)ra * N + n;
             a = (float)((double)(long long)acc[ia] * (ra < type_rows ? r.fix_inv : r.fixq_inv));
             acc[ia] = 0ull;
@@ -106,14 +126,12 @@ __global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_e
                 acc[ib] = 0ull;
                 rho_out[ib] = bb;
             }
-        } else {
-            a = rin[(size_t)ra * N + n];
-            if (rb < nrows) bb = rin[(size_t)rb * N + n];
         }
         lds[n * ldb + b] = make_float2(a, bb);
     }
+    tws.commit(r.ax[0], twl);
     __syncthreads();
-    const float2* res = lds_fft(lds, lds + (size_t)N * ldb, ldb, nb, r.ax[0]);
+    const float2* res = lds_fft(lds, lds + (size_t)N * ldb, ldb, nb, r.ax[0], twl);
     const FastDiv dNxh(Nxh);
     for (int task = threadIdx.x; task < nb * Nxh; task += kFftThreads) {
         int k;
@@ -144,15 +162,25 @@ __global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_e
     const int nb = min(g.nl, g.batch - b0);
     const int ldb = nb | 1;
     float2* base = buf + (size_t)o * g.ostride + b0;
+    float2* twl = lds + 2 * (size_t)g.N * ldb;   // the axis' twiddles
+    TwStage<kFftThreads> tws;
+    tws.issue(r.ax[axis]);
     const FastDiv dnb(nb);
-    for (int task = threadIdx.x; task < nb * g.N; task += kFftThreads) {
-        int b;
-        const int n = dnb.div(task, b);
-        float2 v = gld(base + (size_t)n * g.lstride + b);
-        lds[n * ldb + b] = inverse ? conjf2(v) : v;
-    }
+    stage_in<8, kFftThreads, float2>(
+        nb * g.N,
+        [&](int task) {
+            int b;
+            const int n = dnb.div(task, b);
+            return gld(base + (size_t)n * g.lstride + b);
+        },
+        [&](int task, float2 v) {
+            int b;
+            const int n = dnb.div(task, b);
+            lds[n * ldb + b] = inverse ? conjf2(v) : v;
+        });
+    tws.commit(r.ax[axis], twl);
     __syncthreads();
-    const float2* res = lds_fft(lds, lds + (size_t)g.N * ldb, ldb, nb, r.ax[axis]);
+    const float2* res = lds_fft(lds, lds + (size_t)g.N * ldb, ldb, nb, r.ax[axis], twl);
     for (int task = threadIdx.x; task < nb * g.N; task += kFftThreads) {
         int b;
         const int n = dnb.div(task, b);
@@ -173,21 +201,43 @@ __device__ __forceinline__ float ramp_weight(const RepDev& r, int j, int k) {
 // as the general loop of k_last_fused): w_I = sum_J C_IJ rho_J with the symmetric table
 // rows pidx(lo, hi) = lo*TT - lo*(lo-1)/2 + (hi - lo), and the charge row.  Ramped
 // styles add their rows, weighted, to their pairs' coefficients.
+// Table rows of the first KC nodes (task = thread + k NT) of a thread, loaded before
+// the forward transform so that they arrive while it runs: the TT(TT+1)/2 pair rows and
+// the charge row.
+template <int TT> struct CoupleRows {
+    static constexpr int NR = TT * (TT + 1) / 2;
+    static constexpr int KC = (1280 / TT + 255) / 256;   // the default block's nodes per thread
+    float c[KC][NR + 1];
+};
+
+template <int TT, int NT>
+__device__ __forceinline__ void couple_prefetch(gmem<const float> cb, int nrows, int nlc, int nb, int N, bool charged,
+                                                int ctab_charge, const FastDiv& dnb, CoupleRows<TT>& pre) {
+    constexpr int NR = CoupleRows<TT>::NR, KC = CoupleRows<TT>::KC;
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+        int b;
+        const int n = dnb.div(min((int)threadIdx.x + k * NT, nb * N - 1), b);
+        gmem<const float> cn = cb + (n * nrows * nlc + b);
+#pragma unroll
+        for (int q = 0; q < NR; ++q) pre.c[k][q] = cn[q * nlc];
+        pre.c[k][NR] = charged ? cn[ctab_charge * nlc] : 0.f;
+    }
+}
+
 template <int TT, int NT>
 __device__ __forceinline__ void couple_nodes(gmem<const float> cb, int nrows, int nlc, int nb, int N, int ldb, int ldw,
                                              int nf, int ctab_charge, const FastDiv& dnb, const float2* A, float2* Wb,
-                                             const RepDev& r, const float* rw) {
-    constexpr int NR = TT * (TT + 1) / 2;
+                                             const RepDev& r, const float* rw, const CoupleRows<TT>& pre) {
+    constexpr int NR = CoupleRows<TT>::NR, KC = CoupleRows<TT>::KC;
     const bool charged = nf > TT;
-    Digits2 gc(threadIdx.x, NT, dnb);
-    for (int task = threadIdx.x; task < nb * N; task += NT, gc.step()) {
-        const int b = gc.lo, n = gc.hi;
+    // node `task` with its table rows in c[0..NR-1] and the charge row in c[NR]
+    auto node = [&](int task, float (&c)[NR + 1]) {
+        int b;
+        const int n = dnb.div(task, b);
         const int row = n * ldb + b, wrow = n * ldw + b;
         gmem<const float> cn = cb + (n * nrows * nlc + b);   // offsets inside the block's run
-        float c[NR];
-#pragma unroll
-        for (int k = 0; k < NR; ++k) c[k] = cn[k * nlc];
-        const float cq = charged ? cn[ctab_charge * nlc] : 0.f;
+        const float cq = c[NR];
         for (int j = 0; j < r.nramp; ++j) {   // uniform trip count
             const float v = rw[j] * cn[(r.ramp_row0 + j) * nlc];
 #pragma unroll
@@ -211,6 +261,27 @@ __device__ __forceinline__ void couple_nodes(gmem<const float> cb, int nrows, in
             const float2 rq = A[row + TT * nb];
             Wb[wrow + TT * nb] = make_float2(cq * rq.x, cq * rq.y);
         }
+    };
+    const int total = nb * N;
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+        const int task = (int)threadIdx.x + k * NT;
+        if (task < total) {
+            float c[NR + 1];
+#pragma unroll
+            for (int q = 0; q <= NR; ++q) c[q] = pre.c[k][q];
+            node(task, c);
+        }
+    }
+    for (int task = (int)threadIdx.x + KC * NT; task < total; task += NT) {   // blocks wider than the default
+        int b;
+        const int n = dnb.div(task, b);
+        gmem<const float> cn = cb + (n * nrows * nlc + b);
+        float c[NR + 1];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) c[q] = cn[q * nlc];
+        c[NR] = charged ? cn[ctab_charge * nlc] : 0.f;
+        node(task, c);
     }
 }
 
@@ -224,13 +295,10 @@ __device__ __forceinline__ void couple_nodes(gmem<const float> cb, int nrows, in
 // transform: along the last axis la, w and -i k_la w go to Gk[f*dim] and
 // Gk[f*dim + la]; the factors of the other directions are constant along those lines
 // and are applied by the passes of their own axes.
-template <int NT>
-__global__ void __launch_bounds__(NT) k_last_fused(const RepDev* __restrict__ reps,
-                                                            const WorkItem* __restrict__ wmap,
-                                                            int store_spectrum, int koff) {
-    extern __shared__ float2 lds[];
-    const WorkItem wi = wmap[blockIdx.x];
-    const RepDev& r = reps[wi.rep];
+// TT = 1..4: the type count, with the table rows of the coupling prefetched and its
+// sums unrolled; TT = 0: any type count.
+template <int TT, int NT>
+__device__ __forceinline__ void last_fused_body(const RepDev& r, const WorkItem wi, int koff, float2* lds) {
     const int la = r.dim - 1;                       // last axis
     const int N = la == 2 ? r.Nz : r.Ny;
     const int Q = la == 2 ? r.Ny * r.Nxh : r.Nxh;   // plane size = line stride
@@ -248,31 +316,39 @@ __global__ void __launch_bounds__(NT) k_last_fused(const RepDev* __restrict__ re
     float2* E0 = lds + (size_t)N * ldw;
     float2* E1 = E0 + (size_t)N * ldg;
     float* rw = (float*)(E1 + (size_t)N * ldg);          // ramp weights of this step
+    float2* twl = (float2*)(rw + kMaxRamp);              // the last axis' twiddles
+    TwStage<NT> tws;
+    tws.issue(r.ax[la]);
     if ((int)threadIdx.x < r.nramp) rw[threadIdx.x] = ramp_weight(r, threadIdx.x, r.step + koff);
     const FastDiv dnb(nb), dnf(nf);
     const bool o32 = r.grid_off32;   // field offsets fit 32-bit byte offsets
-    Digits3 gl(threadIdx.x, NT, dnb, dnf);
-    for (int task = threadIdx.x; task < nlines * N; task += NT, gl.step()) {
-        const int b = gl.lo, f = gl.mid, n = gl.hi;
-        E0[n * ldb + f * nb + b] =
-            o32 ? gld(at32((const float2*)r.H, (uint32_t)f * (uint32_t)r.Mh + (uint32_t)(n * Q + q0 + b)))
-                : gld(r.H + (size_t)f * r.Mh + (size_t)n * Q + q0 + b);
-    }
-    __syncthreads();
-    const float2* A = lds_fft<NT>(E0, E1, ldb, nlines, r.ax[la]);   // spectra rho_f(k)
-    // coupling w_I = sum_J C_IJ rho_J (C symmetric, rows I <= J), charge w_c = G_c rho_c;
-    // this block's table rows are one contiguous run
+    // this block's coupling-table rows are one contiguous run; those of the first
+    // nodes of each thread are requested before the spectra, to arrive during the
+    // forward transform
     const int nrows = r.ctab_rows, nlc = r.nl_last;
-
     gmem<const float> cb = G(r.Ctab) + (size_t)wi.blk * N * nrows * nlc;
-    switch (T) {   // up to four types: unrolled, every table row of a node requested at once
-        case 1: couple_nodes<1, NT>(cb, nrows, nlc, nb, N, ldb, ldw, nf, r.ctab_charge, dnb, A, Wb, r, rw); break;
-        case 2: couple_nodes<2, NT>(cb, nrows, nlc, nb, N, ldb, ldw, nf, r.ctab_charge, dnb, A, Wb, r, rw); break;
-        case 3: couple_nodes<3, NT>(cb, nrows, nlc, nb, N, ldb, ldw, nf, r.ctab_charge, dnb, A, Wb, r, rw); break;
-        case 4: couple_nodes<4, NT>(cb, nrows, nlc, nb, N, ldb, ldw, nf, r.ctab_charge, dnb, A, Wb, r, rw); break;
-        default: break;
-    }
-    for (int task = threadIdx.x; T > 4 && task < nb * N; task += NT) {
+    const bool charged = nf > T;
+    CoupleRows<TT == 0 ? 1 : TT> pre;
+    if constexpr (TT > 0) couple_prefetch<TT, NT>(cb, nrows, nlc, nb, N, charged, r.ctab_charge, dnb, pre);
+    stage_in<8, NT, float2>(
+        nlines * N,
+        [&](int task) {
+            int b, f;
+            const int n = dnf.div(dnb.div(task, b), f);
+            return o32 ? gld(at32((const float2*)r.H, (uint32_t)f * (uint32_t)r.Mh + (uint32_t)(n * Q + q0 + b)))
+                       : gld(r.H + (size_t)f * r.Mh + (size_t)n * Q + q0 + b);
+        },
+        [&](int task, float2 v) {
+            int b, f;
+            const int n = dnf.div(dnb.div(task, b), f);
+            E0[n * ldb + f * nb + b] = v;
+        });
+    tws.commit(r.ax[la], twl);
+    __syncthreads();
+    const float2* A = lds_fft<NT>(E0, E1, ldb, nlines, r.ax[la], twl);   // spectra rho_f(k)
+    // coupling w_I = sum_J C_IJ rho_J (C symmetric, rows I <= J), charge w_c = G_c rho_c
+    if constexpr (TT > 0) couple_nodes<TT, NT>(cb, nrows, nlc, nb, N, ldb, ldw, nf, r.ctab_charge, dnb, A, Wb, r, rw, pre);
+    for (int task = threadIdx.x; TT == 0 && task < nb * N; task += NT) {
         int b;
         const int n = dnb.div(task, b);
         const int row = n * ldb + b, wrow = n * ldw + b;
@@ -319,7 +395,7 @@ __global__ void __launch_bounds__(NT) k_last_fused(const RepDev* __restrict__ re
             }
         }
         __syncthreads();
-        const float2* res = lds_fft<NT>(E0, E1, ldg, nlg, r.ax[la]);
+        const float2* res = lds_fft<NT>(E0, E1, ldg, nlg, r.ax[la], twl);
         Digits3 gs(threadIdx.x, NT, dnb, FastDiv(nf * no));
         for (int task = threadIdx.x; task < nlg * N; task += NT, gs.step()) {
             const int b = gs.lo, fo = gs.mid, n = gs.hi;
@@ -330,6 +406,22 @@ __global__ void __launch_bounds__(NT) k_last_fused(const RepDev* __restrict__ re
             else gst(r.Gk + (size_t)(f * D + slot) * r.Mh + (size_t)n * Q + q0 + b, v);
         }
         __syncthreads();
+    }
+}
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_last_fused(const RepDev* __restrict__ reps,
+                                                   const WorkItem* __restrict__ wmap,
+                                                   int store_spectrum, int koff) {
+    extern __shared__ float2 lds[];
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    switch (r.ntypes) {   // up to four types: unrolled, table rows prefetched
+        case 1: last_fused_body<1, NT>(r, wi, koff, lds); break;
+        case 2: last_fused_body<2, NT>(r, wi, koff, lds); break;
+        case 3: last_fused_body<3, NT>(r, wi, koff, lds); break;
+        case 4: last_fused_body<4, NT>(r, wi, koff, lds); break;
+        default: last_fused_body<0, NT>(r, wi, koff, lds); break;
     }
 }
 
@@ -356,19 +448,30 @@ k_mid_grad(const RepDev* __restrict__ reps, const WorkItem* __restrict__ wmap) {
     float2* U = r.Gk + (size_t)(f * 3 + 0) * r.Mh + plane;
     float2* Y = r.Gk + (size_t)(f * 3 + 1) * r.Mh + plane;
     float2* V = r.Gk + (size_t)(f * 3 + 2) * r.Mh + plane;
+    float2* twl = lds + 2 * (size_t)N * ldb;   // the y axis' twiddles
+    TwStage<NT> tws;
+    tws.issue(r.ax[1]);
     const FastDiv dnb(nb);
-    Digits2 gl(threadIdx.x, NT, dnb);
-    for (int task = threadIdx.x; task < nb * N; task += NT, gl.step()) {
-        const int b = gl.lo, n = gl.hi;
-        const float ky = 2 * n == N ? 0.f : r.kfac[1] * (float)(2 * n < N ? n : n - N);
-        const float2 u = gld(U + (n * Nxh + b)), v = gld(V + (n * Nxh + b));
-        // lines of the conj(FFT(conj)) inverse: conj(u), conj(-i ky u), conj(v)
-        lds[n * ldb + b] = make_float2(u.x, -u.y);
-        lds[n * ldb + nb + b] = make_float2(ky * u.y, ky * u.x);
-        lds[n * ldb + 2 * nb + b] = make_float2(v.x, -v.y);
-    }
+    stage_in<4, NT, float4>(
+        nb * N,
+        [&](int task) {
+            int b;
+            const int n = dnb.div(task, b);
+            const float2 u = gld(U + (n * Nxh + b)), v = gld(V + (n * Nxh + b));
+            return make_float4(u.x, u.y, v.x, v.y);
+        },
+        [&](int task, float4 uv) {
+            int b;
+            const int n = dnb.div(task, b);
+            const float ky = 2 * n == N ? 0.f : r.kfac[1] * (float)(2 * n < N ? n : n - N);
+            // lines of the conj(FFT(conj)) inverse: conj(u), conj(-i ky u), conj(v)
+            lds[n * ldb + b] = make_float2(uv.x, -uv.y);
+            lds[n * ldb + nb + b] = make_float2(ky * uv.y, ky * uv.x);
+            lds[n * ldb + 2 * nb + b] = make_float2(uv.z, -uv.w);
+        });
+    tws.commit(r.ax[1], twl);
     __syncthreads();
-    const float2* res = lds_fft<NT>(lds, lds + (size_t)N * ldb, ldb, 3 * nb, r.ax[1]);
+    const float2* res = lds_fft<NT>(lds, lds + (size_t)N * ldb, ldb, 3 * nb, r.ax[1], twl);
     Digits3 gs(threadIdx.x, NT, dnb, FastDiv(3));
     for (int task = threadIdx.x; task < 3 * nb * N; task += NT, gs.step()) {
         const int b = gs.lo, sl = gs.mid, n = gs.hi;
@@ -437,42 +540,56 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) k_
     // (about two tasks per thread: a division pair each is cheaper than the set-up of
     // digit counters here)
     const bool o32 = RECORDS && r.grid_off32;
-    for (int task = threadIdx.x; task < nb * Nxh; task += NT) {
-        int kk, j;
-        const int b = dNxh.div(task, kk);
-        const int row = row0 + dnc.div(b, j);
-        const int la = 2 * j, lb = la + 1;
-        float2 A, Bv = make_float2(0.f, 0.f);
-        if (o32) {
-            const uint32_t ia = ((uint32_t)la * (uint32_t)rows_per_field + (uint32_t)row) * (uint32_t)Nxh + (uint32_t)kk;
-            A = gld(at32(Gin, ia));
-            if (lb < nlines) Bv = gld(at32(Gin, ia + (uint32_t)rows_per_field * (uint32_t)Nxh));
-        } else {
-            A = gld(Gin + ((size_t)la * rows_per_field + row) * Nxh + kk);
-            if (lb < nlines) Bv = gld(Gin + ((size_t)lb * rows_per_field + row) * Nxh + kk);
-        }
-        const bool self = kk == 0 || (even && 2 * kk == N);   // self-conjugate column
-        if (RECORDS) {
-            const float kx = self && kk != 0 ? 0.f : r.kfac[0] * (float)kk;
-            if ((xlines >> la) & 1ull) A = make_float2(kx * A.y, -kx * A.x);
-            if ((xlines >> lb) & 1ull) Bv = make_float2(kx * Bv.y, -kx * Bv.x);
-        }
-        if (self) { A.y = 0.f; Bv.y = 0.f; }
-        // Z = A + i B ; store conj(Z) for the conj(FFT(conj)) inverse.  Column N - kk
-        // holds the Hermitian extension conj(A) + i conj(B).
-        lds[kk * ldb + b] = make_float2(A.x - Bv.y, -(A.y + Bv.x));
-        if (!self && N - kk >= Nxh) lds[(N - kk) * ldb + b] = make_float2(A.x + Bv.y, A.y - Bv.x);
-    }
+    const bool rec32 = RECORDS && o32 && !r.FPtab;   // records with the dimension unrolled
+    const FastDiv dnf(RECORDS ? r.nf : 1);
+    float2* twl = lds + 2 * (size_t)N * ldb;   // the x axis' twiddles
+    TwStage<NT> tws;
+    tws.issue(r.ax[0]);
+    stage_in<4, NT, float4>(
+        nb * Nxh,
+        [&](int task) {
+            int kk, j;
+            const int b = dNxh.div(task, kk);
+            const int row = row0 + dnc.div(b, j);
+            const int la = 2 * j, lb = la + 1;
+            float2 A, Bv = make_float2(0.f, 0.f);
+            if (o32) {
+                const uint32_t ia = ((uint32_t)la * (uint32_t)rows_per_field + (uint32_t)row) * (uint32_t)Nxh + (uint32_t)kk;
+                A = gld(at32(Gin, ia));
+                if (lb < nlines) Bv = gld(at32(Gin, ia + (uint32_t)rows_per_field * (uint32_t)Nxh));
+            } else {
+                A = gld(Gin + ((size_t)la * rows_per_field + row) * Nxh + kk);
+                if (lb < nlines) Bv = gld(Gin + ((size_t)lb * rows_per_field + row) * Nxh + kk);
+            }
+            return make_float4(A.x, A.y, Bv.x, Bv.y);
+        },
+        [&](int task, float4 ab) {
+            int kk, j;
+            const int b = dNxh.div(task, kk);
+            dnc.div(b, j);
+            const int la = 2 * j, lb = la + 1;
+            float2 A = make_float2(ab.x, ab.y), Bv = make_float2(ab.z, ab.w);
+            const bool self = kk == 0 || (even && 2 * kk == N);   // self-conjugate column
+            if (RECORDS) {
+                const float kx = self && kk != 0 ? 0.f : r.kfac[0] * (float)kk;
+                if ((xlines >> la) & 1ull) A = make_float2(kx * A.y, -kx * A.x);
+                if ((xlines >> lb) & 1ull) Bv = make_float2(kx * Bv.y, -kx * Bv.x);
+            }
+            if (self) { A.y = 0.f; Bv.y = 0.f; }
+            // Z = A + i B ; store conj(Z) for the conj(FFT(conj)) inverse.  Column N - kk
+            // holds the Hermitian extension conj(A) + i conj(B).
+            lds[kk * ldb + b] = make_float2(A.x - Bv.y, -(A.y + Bv.x));
+            if (!self && N - kk >= Nxh) lds[(N - kk) * ldb + b] = make_float2(A.x + Bv.y, A.y - Bv.x);
+        });
+    tws.commit(r.ax[0], twl);
     __syncthreads();
-    const float2* res = lds_fft<NT>(lds, lds + (size_t)N * ldb, ldb, nb, r.ax[0]);
-    if (RECORDS && o32 && !r.FPtab) {
+    const float2* res = lds_fft<NT>(lds, lds + (size_t)N * ldb, ldb, nb, r.ax[0], twl);
+    if (rec32) {
         // records of buffers below 4 GiB: dimension unrolled, 32-bit offsets
-        const FastDiv dnf(r.nf);
         if (r.dim == 3) c2r_records<3, NT>(r, res, ldb, nc, rows, row0, N, dN, dnf);
         else c2r_records<2, NT>(r, res, ldb, nc, rows, row0, N, dN, dnf);
     } else if (RECORDS) {
         const int D = r.dim;
-        const FastDiv dnf(r.nf);
         for (int task = threadIdx.x; task < rows * r.nf * N; task += NT) {
             int x, f;
             const int rest = dN.div(task, x), rr = dnf.div(rest, f);
@@ -525,15 +642,29 @@ __global__ void __launch_bounds__(NT) k_plane_r2c(const RepDev* __restrict__ rep
     const size_t bsz = max((size_t)Nx * ldp, (size_t)Ny * ldc);
     float2* A = lds;
     float2* B = lds + bsz;
+    float2* twx = lds + 2 * bsz;               // the x and y axes' twiddles
+    float2* twy = twx + tw_elems(Nx);
+    TwStage<NT> tsx, tsy;
+    tsx.issue(r.ax[0]);
+    tsy.issue(r.ax[1]);
     gmem<const float> rho = G((const float*)r.rho) + (size_t)f * r.M + (size_t)z * Ny * Nx;
     const FastDiv dNx(Nx), dNxh(Nxh);
-    Digits2 gl(threadIdx.x, NT, dNx);
-    for (int task = threadIdx.x; task < np * Nx; task += NT, gl.step()) {
-        const int n = gl.lo, j = gl.hi;
-        A[n * ldp + j] = make_float2(rho[(size_t)(2 * j) * Nx + n], rho[(size_t)(2 * j + 1) * Nx + n]);
-    }
+    stage_in<8, NT, float2>(
+        np * Nx,
+        [&](int task) {
+            int n;
+            const int j = dNx.div(task, n);
+            return make_float2(rho[(2 * j) * Nx + n], rho[(2 * j + 1) * Nx + n]);
+        },
+        [&](int task, float2 v) {
+            int n;
+            const int j = dNx.div(task, n);
+            A[n * ldp + j] = v;
+        });
+    tsx.commit(r.ax[0], twx);
+    tsy.commit(r.ax[1], twy);
     __syncthreads();
-    float2* Z = lds_fft<NT>(A, B, ldp, np, r.ax[0]);
+    float2* Z = lds_fft<NT>(A, B, ldp, np, r.ax[0], twx);
     float2* C = (Z == A) ? B : A;
     Digits2 gm(threadIdx.x, NT, dNxh);
     for (int task = threadIdx.x; task < np * Nxh; task += NT, gm.step()) {
@@ -544,7 +675,7 @@ __global__ void __launch_bounds__(NT) k_plane_r2c(const RepDev* __restrict__ rep
         C[(2 * j + 1) * ldc + k] = make_float2(0.5f * (Zk.y + Zm.y), 0.5f * (Zm.x - Zk.x));
     }
     __syncthreads();
-    const float2* Y = lds_fft<NT>(C, Z, ldc, Nxh, r.ax[1]);
+    const float2* Y = lds_fft<NT>(C, Z, ldc, Nxh, r.ax[1], twy);
     float2* H = r.H + ((size_t)f * r.Nz + z) * Ny * Nxh;
     Digits2 gs(threadIdx.x, NT, dNxh);
     for (int task = threadIdx.x; task < Ny * Nxh; task += NT, gs.step()) {
@@ -1093,6 +1224,26 @@ hipError_t launch_particles(const LaunchCtx& c, bool integrate, int write_forces
         TILD_LAUNCH(c, (k_particles<false, false>), dim3(c.n), dim3(256), 0, c.reps, c.wmap, write_forces, koff);
     CHECK_LAUNCH(); return hipSuccess;
 }
+// avg_sk (Compute.cu:83-117): acc(k) += rho_t(k) rho_t(k) / M over the half spectrum,
+// the complex square of the unnormalised transform (d_multiplyComplex with both
+// operands rho(k), device_array_utils.cu:314-326 -- not |rho(k)|^2), summed in FP64.
+__global__ void __launch_bounds__(256) k_sk_accumulate(const RepDev* __restrict__ reps, int rep, int type,
+                                                       double* __restrict__ acc) {
+    const RepDev& r = reps[rep];
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= r.Mh) return;
+    const float2 h = gld(r.H + (size_t)type * r.Mh + k);
+    const double x = h.x, y = h.y, inv = 1.0 / (double)r.M;
+    gmem<double> a = G(acc) + 2 * (size_t)k;
+    a[0] += (x * x - y * y) * inv;
+    a[1] += 2.0 * x * y * inv;
+}
+
+hipError_t launch_sk_accumulate(hipStream_t s, const RepDev* reps, int rep, int type, int Mh, double* acc) {
+    hipLaunchKernelGGL(k_sk_accumulate, dim3((Mh + 255) / 256), dim3(256), 0, s, reps, rep, type, acc);
+    return hipGetLastError();
+}
+
 hipError_t launch_kprops(const LaunchCtx& c, int nq_max) {
     if (!c.n || nq_max <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_kprops, dim3(c.n, nq_max), dim3(256), 0, c.stream, c.reps, c.wmap);

@@ -88,6 +88,7 @@ void Server::loop() {
                 queue_.pop_front();
             }
             bytes_in_use_ = used;   // reserved under the estimate until the replica exists
+            admitting_ = (int)admit.size();   // neither queued nor running until added
         }
         for (Job* J : admit) {
             const double ts = now();
@@ -95,6 +96,7 @@ void Server::loop() {
             std::lock_guard<std::mutex> lk(mu_);
             admit_s_ += now() - ts;
             bytes_in_use_ -= J->bytes;
+            --admitting_;
             if (id < 0) {
                 J->state = TILD_JOB_FAILED;
                 J->err = err;
@@ -216,7 +218,7 @@ int Server::wait(int job, double timeout_s, std::string& err) {
 
 int Server::drain(double timeout_s, std::string& err) {
     std::unique_lock<std::mutex> lk(mu_);
-    auto idle = [&] { return queue_.empty() && running_.empty(); };
+    auto idle = [&] { return queue_.empty() && running_.empty() && admitting_ == 0; };
     if (timeout_s < 0) done_cv_.wait(lk, idle);
     else if (!done_cv_.wait_for(lk, std::chrono::duration<double>(timeout_s), idle)) {
         err = "timed out draining the server";

@@ -372,8 +372,20 @@ int load_system(const std::string& text, const std::string& base_dir, System& S,
                 S.charges.enabled = true;
             }
             else if (w == "compute") {
-                S.notes.push_back("compute ignored: " + line);
-                while (iss >> a) {}
+                // Compute::Initialize (Compute.cu:20-75): avg_sk is the only style; the
+                // type is checked against the types read so far (none before read_data)
+                std::string style;
+                iss >> style;
+                if (style != "avg_sk") return bad("Undefined compute style!");
+                SkCompute c;
+                if (!geti(iv)) return bad("Invalid type to calculate avg_sk");
+                c.type = iv - 1;
+                if (c.type < 0 || !config_read || c.type >= S.ntypes) return bad("Invalid type to calculate avg_sk");
+                while (iss >> a) {
+                    if (a == "freq") { if (!geti(iv) || iv < 1) return bad("compute avg_sk: bad freq"); c.freq = iv; }
+                    else if (a == "wait") { if (!geti(iv)) return bad("compute avg_sk: bad wait"); c.wait = iv; }
+                }
+                S.computes.push_back(c);
             }
             else if (w == "delt") { if (!getf(fv)) return bad("delt needs a number"); S.delt = fv; S.noise_mag = std::sqrt(2.f * S.delt); }
             else if (w == "diffusivity") {

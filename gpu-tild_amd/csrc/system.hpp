@@ -94,6 +94,13 @@ struct Integrator {                   // integrator.h:13-24
 // s(k) of a sphere of radius Rp, with the k -> 0 limit 4 pi Rp^3 / 3.
 double sphere_kspace(double k2, double Rp);
 
+// `compute avg_sk T [freq F] [wait W]` (Compute.cu:20-75): the running average of
+// rho_T(k) rho_T(k) / M, evaluated every `freq` steps once step > wait.
+struct SkCompute {
+    int type = 0;        // 0-based
+    int freq = 100, wait = 0;
+};
+
 struct System {
     // deck (read_input.cu:536-562 defaults)
     int Dim = 2;
@@ -124,6 +131,7 @@ struct System {
     std::vector<FieldPhase> Fields;
     std::vector<Group> Groups;
     std::vector<Integrator> Integrators;
+    std::vector<SkCompute> computes;
     std::vector<std::string> notes;   // accepted-but-ignored keywords
 
     // derived, initialize.cu:32-48

@@ -7,6 +7,8 @@
 //   data.dat + stdout log lines     main.cu:60-73, 230-293
 //   rho<t>.dat                      write_grid_data, io_utils.cu:7-31
 //   grid_densities.bin, positions.bin  init_binary_output / write_binary, io_utils.cu:85-134
+//   avg_sk_<id>.dat                 Compute::writeResults / write_kspace_data,
+//                                   Compute.cu:121-137, io_utils.cu:60-83
 // Between output steps the engine advances in one call, so no host round trip
 // happens on steps that write nothing.  Extra options: -device N, -seed S (overrides
 // rand_seed).
@@ -38,6 +40,9 @@ struct Run {
     std::vector<float> q, x, rho;
     std::string dump_name;
     std::ofstream dout;
+    struct Comp { int type, freq, wait; };
+    std::vector<Comp> computes;
+    std::vector<float> sk;   // 2*M
 };
 
 [[noreturn]] void die(const std::string& what) {
@@ -74,6 +79,30 @@ void write_traj(Run& R, int step) {
         for (int d = R.dim; d < 3; ++d) std::fprintf(f, "%f", 0.0);
         if (R.do_charges) std::fprintf(f, " %f", (double)R.q[i]);
         std::fprintf(f, "\n");
+    }
+    std::fclose(f);
+}
+
+// write_kspace_data (io_utils.cu:60-83): every wavevector but k = 0, its components
+// (get_k, array_utils.cu:17-41: FP32, 2 pi n / L folded at N / 2), then |S|, |k|,
+// Re S and Im S; 2D grids get a blank line after each x row.
+void write_kspace(const Run& R, const char* name, const float* sk) {
+    FILE* f = std::fopen(name, "w");
+    if (!f) die(std::string("cannot open ") + name);
+    const float pi2 = 6.2831853071f;
+    for (int i = 1; i < R.M; ++i) {
+        int n[3] = {i % R.Nx[0], (i / R.Nx[0]) % R.Nx[1], i / (R.Nx[0] * R.Nx[1])};
+        float kv[3], k2 = 0.f;
+        for (int d = 0; d < R.dim; ++d) {
+            kv[d] = (double)n[d] < (double)R.Nx[d] / 2.0 ? pi2 * (float)n[d] / R.L[d]
+                                                          : pi2 * (float)(n[d] - R.Nx[d]) / R.L[d];
+            k2 += kv[d] * kv[d];
+        }
+        for (int d = 0; d < R.dim; ++d) std::fprintf(f, "%f ", (double)kv[d]);
+        const float re = sk[2 * i], im = sk[2 * i + 1];
+        std::fprintf(f, "%1.5e %1.5e %1.5e %1.5e\n", (double)std::hypot(re, im), (double)std::sqrt(k2), (double)re,
+                     (double)im);
+        if (R.dim == 2 && n[0] == R.Nx[0] - 1) std::fprintf(f, "\n");
     }
     std::fclose(f);
 }
@@ -200,6 +229,12 @@ int main(int argc, char** argv) {
     if (tild_system_get_types(R.sys, R.tp.data(), R.ns) != TILD_OK) die("types");
     if (tild_system_get_molecules(R.sys, R.mol.data(), R.ns) != TILD_OK) die("molecules");
     if (R.do_charges && tild_system_get_charges(R.sys, R.q.data(), R.ns) != TILD_OK) die("charges");
+    for (int i = 0, nc = (int)info(R, "n_computes"); i < nc; ++i) {
+        Run::Comp c{};
+        if (tild_system_compute(R.sys, i, &c.type, &c.freq, &c.wait) != TILD_OK) die("compute");
+        R.computes.push_back(c);
+    }
+    R.sk.resize(2 * (size_t)R.M);
 
     tild_opts o{};
     o.device = device;
@@ -237,8 +272,16 @@ int main(int argc, char** argv) {
     while (step < R.max_steps && !die_flag) {
         int next = R.max_steps;
         for (int f : {R.log_freq, R.traj_freq, R.grid_freq, R.bin_freq}) next = std::min(next, next_multiple(step, f));
+        for (const Run::Comp& c : R.computes) next = std::min(next, next_multiple(step, c.freq));
         if (tild_engine_step(R.eng, next - step) != TILD_OK) die("step");
         step = next;
+        // computes follow the integration (main.cu:123-131), on the densities the step
+        // started from
+        for (size_t i = 0; i < R.computes.size(); ++i) {
+            const Run::Comp& c = R.computes[i];
+            if (step > c.wait && step % c.freq == 0 && tild_engine_sk_accumulate(R.eng, (int)i, c.type) != TILD_OK)
+                die("compute");
+        }
         const time_t io0 = std::time(nullptr);
         if (R.traj_freq > 0 && step % R.traj_freq == 0) { collect_x(R); write_traj(R, step); }
         if (R.grid_freq > 0 && step % R.grid_freq == 0) {
@@ -253,6 +296,13 @@ int main(int argc, char** argv) {
         if (R.log_freq > 0 && step % R.log_freq == 0) {
             if (tild_engine_properties(R.eng, &p) != TILD_OK) die("properties");
             die_flag = print_timestep(R, step, p);
+            for (size_t i = 0; i < R.computes.size(); ++i) {   // main.cu:182-184
+                if (step <= R.computes[i].wait) continue;
+                if (tild_engine_sk_average(R.eng, (int)i, R.sk.data(), R.sk.size(), nullptr) != TILD_OK) die("compute");
+                char nm[32];
+                std::snprintf(nm, sizeof(nm), "avg_sk_%d.dat", (int)i);
+                write_kspace(R, nm, R.sk.data());
+            }
         }
         t_io += (double)(std::time(nullptr) - io0);
     }

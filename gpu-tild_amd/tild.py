@@ -26,6 +26,8 @@ EXPORTS = [
     "tild_engine_get_density", "tild_engine_get_grid_field", "tild_engine_get_step",
     "tild_engine_get_state", "tild_engine_set_state",
     "tild_engine_synchronize", "tild_engine_get_bins", "tild_engine_profile", "tild_engine_nlist", "tild_fft_r2c", "tild_fft_c2r",
+    "tild_system_compute", "tild_engine_sk_accumulate", "tild_engine_sk_average",
+    "tild_engine_geometry", "tild_batch_geometry",
     "tild_batch_create", "tild_batch_destroy", "tild_batch_last_error", "tild_batch_add", "tild_batch_remove",
     "tild_batch_step", "tild_batch_properties", "tild_batch_get_positions", "tild_batch_set_positions",
     "tild_batch_get_density", "tild_batch_replica_step", "tild_batch_profile",
@@ -47,7 +49,18 @@ class Opts(ctypes.Structure):
                 ("spread_mode", ctypes.c_int), ("tile_target", ctypes.c_int), ("tile_by", ctypes.c_int),
                 ("tile_bz", ctypes.c_int), ("reorder_every", ctypes.c_int), ("fft_mid_elems", ctypes.c_int),
                 ("fft_last_lines", ctypes.c_int), ("fft_c2r_rows", ctypes.c_int), ("fft_plane", ctypes.c_int),
-                ("fft_last_dirs", ctypes.c_int), ("tile_slack", ctypes.c_int), ("reserved", ctypes.c_int * 2)]
+                ("fft_last_dirs", ctypes.c_int), ("tile_slack", ctypes.c_int), ("autotune", ctypes.c_int),
+                ("deposit_threads", ctypes.c_int)]
+
+
+class Geometry(ctypes.Structure):
+    _fields_ = [("tile_target", ctypes.c_int), ("tiles", ctypes.c_int), ("deposit_threads", ctypes.c_int),
+                ("plane", ctypes.c_int), ("last_lines", ctypes.c_int), ("last_dirs", ctypes.c_int),
+                ("mid_elems", ctypes.c_int), ("c2r_rows", ctypes.c_int), ("tuned", ctypes.c_int),
+                ("from_cache", ctypes.c_int), ("step_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Props(ctypes.Structure):
@@ -145,6 +158,11 @@ def lib():
         L.tild_server_stats_get.argtypes = [vp, ctypes.POINTER(ServerStats)]
         L.tild_fft_r2c.argtypes = [ctypes.c_int, ip, ctypes.c_int, vp, vp, vp]
         L.tild_fft_c2r.argtypes = [ctypes.c_int, ip, ctypes.c_int, vp, vp, vp]
+        L.tild_system_compute.argtypes = [vp, ctypes.c_int, ip, ip, ip]
+        L.tild_engine_sk_accumulate.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.tild_engine_sk_average.argtypes = [vp, ctypes.c_int, fp, sz, ip]
+        L.tild_engine_geometry.argtypes = [vp, vp]
+        L.tild_batch_geometry.argtypes = [vp, ctypes.c_int, vp]
         _lib = L
     return _lib
 
@@ -225,16 +243,26 @@ class System:
         _check(lib().tild_system_group_sites(self._h, g, _ip(a), a.size, ctypes.byref(n)))
         return a[:n.value]
 
+    def computes(self):
+        """The deck's avg_sk computes as (0-based type, freq, wait) tuples."""
+        out = []
+        for i in range(int(self.info("n_computes"))):
+            t, f, w = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            _check(lib().tild_system_compute(self._h, i, ctypes.byref(t), ctypes.byref(f), ctypes.byref(w)))
+            out.append((t.value, f.value, w.value))
+        return out
+
 
 class Engine:
     """One device-resident replica (tild_engine)."""
 
     def __init__(self, system, device=0, seed=None, spread_mode=0, tile_target=0, tile_by=0, tile_bz=0,
                  reorder_every=0, fft_mid_elems=0, fft_last_lines=0, fft_c2r_rows=0, fft_plane=0,
-                 fft_last_dirs=0, tile_slack=0):
+                 fft_last_dirs=0, tile_slack=0, autotune=False, deposit_threads=0):
         o = Opts()
         o.device = device
         o.spread_mode = spread_mode
+        o.autotune, o.deposit_threads = int(bool(autotune)), deposit_threads
         o.tile_target, o.tile_by, o.tile_bz = tile_target, tile_by, tile_bz
         o.reorder_every = reorder_every
         o.fft_mid_elems, o.fft_last_lines, o.fft_c2r_rows = fft_mid_elems, fft_last_lines, fft_c2r_rows
@@ -293,6 +321,25 @@ class Engine:
         a = np.zeros(self.M, dtype=np.float32)
         self._chk(lib().tild_engine_get_density(self._h, t, _fp(a), a.size), "get_density")
         return a.reshape(self.grid_shape)
+
+    def geometry(self):
+        """Block geometry in use (and the autotuner's pick, if tuned)."""
+        g = Geometry()
+        self._chk(lib().tild_engine_geometry(self._h, ctypes.byref(g)), "geometry")
+        return g.as_dict()
+
+    def sk_accumulate(self, t, slot=None):
+        """avg_sk: add rho_t(k) rho_t(k) / M of the last force evaluation's densities to
+        compute slot `slot` (default: slot t)."""
+        self._chk(lib().tild_engine_sk_accumulate(self._h, t if slot is None else slot, t), "sk_accumulate")
+
+    def sk_average(self, slot):
+        """avg_sk running average of a compute slot over all M wavevectors (complex,
+        grid shape) and the number of evaluations."""
+        a = np.zeros(2 * self.M, dtype=np.float32)
+        n = ctypes.c_int()
+        self._chk(lib().tild_engine_sk_average(self._h, slot, _fp(a), a.size, ctypes.byref(n)), "sk_average")
+        return (a[0::2] + 1j * a[1::2]).astype(np.complex64).reshape(self.grid_shape), n.value
 
     def grid_field(self, t, d):
         a = np.zeros(self.M, dtype=np.float32)
@@ -354,10 +401,11 @@ class Engine:
 class Batch:
     """Many device-resident replicas advanced together (tild_batch)."""
 
-    def __init__(self, device=0, spread_mode=0, tile_target=0, reorder_every=0):
+    def __init__(self, device=0, spread_mode=0, tile_target=0, reorder_every=0, autotune=False, deposit_threads=0):
         o = Opts()
         o.device = device
         o.spread_mode = spread_mode
+        o.autotune, o.deposit_threads = int(bool(autotune)), deposit_threads
         o.tile_target = tile_target
         o.reorder_every = reorder_every
         h = ctypes.c_void_p()
@@ -377,6 +425,12 @@ class Batch:
     def _chk(self, st, where):
         if st != 0:
             raise TildError(st, f"{where}: {lib().tild_batch_last_error(self._h).decode()}")
+
+    def geometry(self, r):
+        """Block geometry of replica r (and the autotuner's pick, if tuned)."""
+        g = Geometry()
+        self._chk(lib().tild_batch_geometry(self._h, r, ctypes.byref(g)), "geometry")
+        return g.as_dict()
 
     def add(self, system, seed):
         r = ctypes.c_int()

@@ -72,6 +72,9 @@ tild_status tild_system_get_charges(const tild_system* sys, float* q, size_t n);
 tild_status tild_system_get_bonds(const tild_system* sys, int* n_bonds, int* bonded_to,
                                   int* bond_type, size_t ns);
 /* Site list of group g (make_group_all / make_group_type, group.cu:30-60). */
+/* Compute `i` of the deck (`compute avg_sk T [freq F] [wait W]`, Compute.cu:20-75):
+   0-based type, frequency (default 100) and wait (default 0) in steps. */
+tild_status tild_system_compute(const tild_system* sys, int i, int* type, int* freq, int* wait);
 tild_status tild_system_group_sites(const tild_system* sys, int group, int* sites,
                                     size_t cap, int* nsites);
 
@@ -99,7 +102,15 @@ typedef struct {
     int tile_slack;      /* tiled spread: spare slots per bucket region beyond the count at
                             the last sizing (0 = count/4 + 16; < 0 none, so that drifting
                             beads take the overflow path).  Results do not depend on it. */
-    int reserved[2];
+    int autotune;        /* nonzero: each replica's block geometry (tile count, deposit
+                            block size, plane pass, FFT line batching) comes from the
+                            tuning cache for its class, or from a measured sweep on a
+                            scratch copy of the replica that is then cached (file
+                            $TILD_TUNE_CACHE, else ~/.cache/tild/tune.tsv); the explicit
+                            geometry settings above are ignored.  Results do not depend
+                            on it. */
+    int deposit_threads; /* tiled spread: deposit block size, 256 or 512 (0 = by the
+                            beads per tile).  Results do not depend on it. */
 } tild_opts;
 
 /* Properties of calc_properties() (calc_properties.cu:20-116): FP64 reductions. */
@@ -184,6 +195,31 @@ tild_status tild_engine_nlist(tild_engine* e, float rc, const int* sites, int ns
                               int* cell_of, int* counts, int* neigh, size_t cap,
                               int* cell_max, int* ncells);
 tild_status tild_engine_synchronize(tild_engine* e);
+/* Block geometry in use (no counterpart in the reference: its launch shapes are fixed,
+   M_Grid/M_Block and ns_Grid/ns_Block, initialize.cu:62-69).  tuned = 1 when the
+   autotuner chose it (from_cache = 1 when read from the tuning cache rather than
+   measured), with the tile target it picked and its measured kernel time per step. */
+typedef struct {
+    int tile_target;     /* tuner's tile target (0 when not tuned) */
+    int tiles;           /* tiles of the tiled spread (0: global-atomic spread) */
+    int deposit_threads; /* requested deposit block size (0: by beads per tile) */
+    int plane;           /* 1: forward plane pass */
+    int last_lines, last_dirs, mid_elems, c2r_rows;   /* FFT line batching */
+    int tuned, from_cache;
+    double step_ms;      /* kernel time per step measured by the tuner (0 if not tuned) */
+} tild_geometry;
+tild_status tild_engine_geometry(tild_engine* e, tild_geometry* out);
+/* Compute::doCompute for avg_sk (Compute.cu:83-117): adds rho_t(k) rho_t(k) / M -- the
+   complex square of the unnormalised transform, as d_multiplyComplex forms it, not
+   |rho_t(k)|^2 -- of the densities of the last force evaluation (after
+   tild_engine_step, those of the positions the last step started from, which is what
+   the reference's computes see) to the running sum of compute slot `slot` (0..63, one
+   per Compute object; a slot keeps the type of its first call), in FP64 on the device. */
+tild_status tild_engine_sk_accumulate(tild_engine* e, int slot, int type);
+/* Compute::writeResults (Compute.cu:121-137): slot's running average for all M
+   wavevectors, x fastest, interleaved (re, im) in sk[2*M]; zeros before the first
+   evaluation.  *count (may be NULL) receives the number of evaluations. */
+tild_status tild_engine_sk_average(tild_engine* e, int slot, float* sk, size_t n, int* count);
 
 /* ---------------------------------------------------------------- replica batches */
 
@@ -206,6 +242,7 @@ tild_status tild_batch_get_positions(tild_batch* b, int replica, float* x, size_
 tild_status tild_batch_set_positions(tild_batch* b, int replica, const float* x, size_t n);
 tild_status tild_batch_get_density(tild_batch* b, int replica, int type, float* rho, size_t n);
 tild_status tild_batch_replica_step(tild_batch* b, int replica, int* step);
+tild_status tild_batch_geometry(tild_batch* b, int replica, tild_geometry* out);
 /* as tild_engine_profile, over a replica set */
 tild_status tild_batch_profile(tild_batch* b, const int* replicas, int nrep, int nsteps,
                                double* phase_ms, int nphase);

@@ -63,6 +63,9 @@ struct ora_sys {
     int forces_done;
     /* properties */
     double Upe, Ubond, Uelec, Ptens[6], bondVir[6];
+    /* avg_sk computes: per type, sum of rho(k) rho(k) / M (M complex) and counts */
+    double* sk;
+    int* sk_count;
 };
 
 /* parser (ora_input.c) */

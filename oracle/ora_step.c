@@ -538,6 +538,37 @@ void ora_run(ora_sys* s, int nsteps) {
     }
 }
 
+/* Compute::doCompute for avg_sk, Compute.cu:83-117: d_prepareDensity (the type's
+ * density as a complex grid), cufftExecC2C forward, d_multiplyComplex(rho_k, rho_k)
+ * -- (a.x b.x - a.y b.y, a.x b.y + a.y b.x) / M with a = b, the quirk of squaring
+ * rho(k) rather than taking |rho(k)|^2 -- summed over the calls. */
+void ora_sk_accumulate(ora_sys* s, int t) {
+    const int M = s->M;
+    if (!s->forces_done) ora_forces(s);
+    if (!s->sk) {
+        s->sk = calloc((size_t)s->ntypes * 2 * M, sizeof(double));
+        s->sk_count = calloc((size_t)s->ntypes, sizeof(int));
+    }
+    double* c = s->cpx1;
+    for (int i = 0; i < M; ++i) { c[2 * i] = s->rho[(size_t)t * M + i]; c[2 * i + 1] = 0.0; }
+    ora_fft(c, s->dim, s->Nx, -1);
+    double* acc = s->sk + (size_t)t * 2 * M;
+    for (int i = 0; i < M; ++i) {
+        const double x = c[2 * i], y = c[2 * i + 1];
+        acc[2 * i] += (x * x - y * y) / M;
+        acc[2 * i + 1] += (x * y + y * x) / M;
+    }
+    s->sk_count[t] += 1;
+}
+
+/* Compute::writeResults, Compute.cu:121-137: the sum over the number of calls */
+int ora_sk_average(const ora_sys* s, int t, double* out) {
+    const int M = s->M;
+    const int n = s->sk ? s->sk_count[t] : 0;
+    for (int i = 0; i < 2 * M; ++i) out[i] = n > 0 ? s->sk[(size_t)t * 2 * M + i] / n : 0.0;
+    return n;
+}
+
 /* PairStyle::CalcEnergy / CalcVirial, pair_style.cu:204-260; integ_trapPBC integ_utils.cu:3-13 */
 static void style_energy_virial(ora_sys* s, ora_style* st) {
     int M = s->M, D = s->dim, nP = s->nP;
@@ -741,6 +772,7 @@ void ora_destroy(ora_sys* s) {
     free(s->n_bonds); free(s->bonded_to); free(s->bond_type); free(s->bond_kf); free(s->bond_reqf);
     free(s->rho); free(s->fx); free(s->fy); free(s->fz); free(s->rho_c); free(s->phi); free(s->efield);
     free(s->fxc); free(s->fyc); free(s->fzc); free(s->cpx1); free(s->cpx2); free(s->gridW); free(s->gridI); free(s->f); free(s->prev_noise);
+    free(s->sk); free(s->sk_count);
     free(s);
 }
 

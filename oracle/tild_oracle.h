@@ -72,6 +72,14 @@ void ora_get_grid_stencil(const ora_sys* s, int* inds, double* W);   /* ns*S eac
 /* calc_properties.cu:20 -- out: [Upe, Ubond, Uelec, Ptens[6], n_style energies...] */
 int  ora_properties(ora_sys* s, double* out, int nout);
 
+/* Compute::doCompute, avg_sk (Compute.cu:83-117): adds rho_t(k) rho_t(k) / M of the
+   densities of the last force evaluation (the complex square of the unnormalised
+   forward transform, not |rho(k)|^2) to type t's sum.  ora_sk_average writes the
+   average over all M wavevectors (interleaved re, im, x fastest; zeros before the
+   first evaluation, Compute.cu:121-137) and returns the evaluation count. */
+void ora_sk_accumulate(ora_sys* s, int type);
+int  ora_sk_average(const ora_sys* s, int type, double* out);
+
 /* integrator.cu:24 Integrate_2 for every integrator at the current step */
 void ora_integrate(ora_sys* s);
 /* main.cu:106-201 loop body (forces, integrate), nsteps times; step counter advances */

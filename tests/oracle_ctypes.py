@@ -57,6 +57,9 @@ def lib():
         L.ora_set_double.argtypes = [vp, ctypes.c_char_p, ctypes.c_double]
         L.ora_set_threads.argtypes = [ctypes.c_int]
         L.ora_get_threads.restype = ctypes.c_int
+        L.ora_sk_accumulate.argtypes = [vp, ctypes.c_int]
+        L.ora_sk_average.restype = ctypes.c_int
+        L.ora_sk_average.argtypes = [vp, ctypes.c_int, dp]
         L.ora_nlist.restype = ctypes.c_int
         L.ora_nlist.argtypes = [vp, ctypes.c_double, ctypes.c_int, ip, ip, ip, ctypes.c_int, ip, ip]
         _lib = L
@@ -226,6 +229,14 @@ class System:
         v = out[:n]
         return {"Upe": v[0], "Ubond": v[1], "Uelec": v[2], "Ptens": v[3:9].copy(),
                 "U_styles": v[9:].copy()}
+
+    def sk_accumulate(self, t):
+        lib().ora_sk_accumulate(self._h, t)
+
+    def sk_average(self, t):
+        a = np.zeros(2 * self.M)
+        n = lib().ora_sk_average(self._h, t, _dp(a))
+        return (a[0::2] + 1j * a[1::2]).reshape(self.grid_shape), n
 
     def integrate(self):
         lib().ora_integrate(self._h)

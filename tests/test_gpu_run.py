@@ -87,3 +87,40 @@ def test_tild_run_outputs(tmp_path):
     assert np.allclose(rho0[:, -1], dens[20][0], rtol=1e-7, atol=0)
     e.close()
 
+
+
+def _normalised_frame(raw):
+    """The first frame of a trajectory, lines whitespace-normalised (the reference's
+    shipped example has CRLF line ends and one space before the charge column, where
+    config_utils.cu:386-396 writes two)."""
+    lines = [b" ".join(l.split()) for l in raw.replace(b"\r\n", b"\n").split(b"\n") if l.strip()]
+    starts = [i for i, l in enumerate(lines) if l == b"ITEM: TIMESTEP"]
+    return lines[:starts[1]] if len(starts) > 1 else lines
+
+
+def test_tild_run_matches_the_reference_example_files(tmp_path):
+    """The headers of positions.bin and grid_densities.bin and the step-0 trajectory
+    frame that the reference wrote for examples/Coacervate (init_binary_output,
+    io_utils.cu:85-111; write_lammps_traj, config_utils.cu:366-402)."""
+    import gzip
+    import hashlib
+    import json
+    text, d = gpu_util.coacervate_dir(tmp_path)
+    (d / "input").write_text(re.sub(r"max_steps \d+", "max_steps 1", text))
+    r = subprocess.run([str(TILD_RUN), "-in", "input"], cwd=d, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    gold = gpu_util.GOLDEN / "coacervate"
+    # binary_freq 25000 > max_steps: both binary files hold their headers only, as the
+    # shipped ones do
+    assert (d / "grid_densities.bin").read_bytes() == (gold / "grid_densities.bin").read_bytes()
+    with gzip.open(gold / "positions.bin.gz", "rb") as f:
+        want = f.read()
+    got = (d / "positions.bin").read_bytes()
+    assert len(got) == len(want) == 427080 and got == want
+    meta = json.loads((gold / "traj_step0.json").read_text())
+    frame = _normalised_frame((d / "traj.lammpstrj").read_bytes())
+    assert [l.decode() for l in frame[:len(meta["head"])]] == meta["head"]
+    assert len(frame) == meta["lines"]
+    assert hashlib.sha256(b"\n".join(frame)).hexdigest() == meta["sha256_normalised"]
+    print("tild-run: positions.bin and grid_densities.bin headers byte-identical, step-0 frame identical "
+          "to the reference's example files", flush=True)
