@@ -5,10 +5,10 @@
 // densities are integer sums, and every FFT line is transformed by the same arithmetic
 // whatever block holds it.  So the fastest geometry can be picked by measurement.  For
 // each replica class -- device, grid, field count, stencil order, bead-count half-octave
-// and spread mode -- the tuner measures the kernel time of a step of a scratch replica
-// of the deck (HIP events bound to every launch) under the built-in default, then sweeps
-// one setting at a time over its candidates, keeping a change only when it is at least
-// 2 % faster.  Results are kept for the process and appended to a tab-separated cache
+// and spread mode -- the tuner measures the kernel time of a step of scratch copies of
+// the deck (HIP events bound to every launch; small replicas as a set of copies, the
+// way a sweep runs them) under the built-in default, then sweeps one setting at a time
+// over its candidates, keeping a change only when it is at least 2 % faster.  Results are kept for the process and appended to a tab-separated cache
 // file ($TILD_TUNE_CACHE, else $XDG_CACHE_HOME/tild/tune.tsv or ~/.cache/tild/tune.tsv),
 // which later processes read instead of sweeping again.
 #include <sys/stat.h>
@@ -111,27 +111,41 @@ int Batch::add_with(const System& S, uint32_t seed, const Geometry& g, std::stri
     return id;
 }
 
-// Kernel time of one step (ms) of a scratch replica of S with geometry g, or +inf when
-// the geometry does not fit the deck.
+// Kernel time of one step (ms per replica) of scratch copies of S with geometry g, or
+// +inf when the geometry does not fit the deck.  Small replicas are measured as a set
+// of copies (up to kTuneCopies, about kTuneBeads beads in all), as a sweep runs them:
+// alone, a small grid leaves most of the device idle and favours the geometry with the
+// most blocks, which is not the fastest once many replicas share each launch.
+constexpr int kTuneCopies = 32;
+constexpr double kTuneBeads = 4.0e5;
+
 double Batch::measure_geometry(const System& S, uint32_t seed, const Geometry& g, std::string& err) {
     constexpr double kInf = std::numeric_limits<double>::infinity();
-    tuning_ = true;
+    const int copies = std::max(1, std::min(kTuneCopies, (int)std::lround(kTuneBeads / std::max(S.ns, 1))));
+    std::vector<int> ids;
     std::string e;
-    const int id = add_with(S, seed, g, e);
-    tuning_ = false;
-    if (id < 0) {
-        if (id == -TILD_ERR_DEVICE || id == -TILD_ERR_OOM) err = e;
-        return kInf;
+    tuning_ = true;
+    for (int k = 0; k < copies; ++k) {
+        const int id = add_with(S, seed + (uint32_t)k, g, e);
+        if (id < 0) {
+            tuning_ = false;
+            for (int r : ids) remove(r);
+            if (id == -TILD_ERR_DEVICE || id == -TILD_ERR_OOM) err = e;
+            return kInf;
+        }
+        ids.push_back(id);
     }
+    tuning_ = false;
     double ms[kProfPhases] = {};
     double t = kInf;
-    if (step({id}, 3, e, false) == 0 && profile({id}, 6, ms, kProfPhases, e) == 0) {
+    if (step(ids, 3, e, false) == 0 && profile(ids, 6, ms, kProfPhases, e) == 0) {
         t = 0.0;
         for (int k = 0; k < kProfPhases - 1; ++k) t += ms[k];   // kernels; the last phase is the reorders
+        t /= copies;
     } else {
         err = e;
     }
-    remove(id);
+    for (int r : ids) remove(r);
     return t;
 }
 

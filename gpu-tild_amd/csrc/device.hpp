@@ -40,6 +40,7 @@ struct RepDev {
     // fixed-point densities: acc = round(w * 2^F), rho = acc * inv
     float fix_scale, fixq_scale;
     double fix_inv, fixq_inv;
+    int fix_s2, fix_s3;        // order-1 integer weights: 44 - F and 52 - F (fixed_weight2/3)
     // FFT plans (x, y, z) and chunk sizes of each pass
     AxisPlan ax[3];
     int nl_x;      // line pairs per block, forward x pass
@@ -54,8 +55,8 @@ struct RepDev {
     float* XO;              // ns*dim, GJF previous position
     const int* tp;          // ns
     const float* q;         // ns or null
-    const int* bond_j;      // ns*3, -1 = empty slot
-    const int* bond_t;      // ns*3
+    const int* bond;        // ns*3 slots: (partner << bond_shift) | bond type; -1 = empty
+    int bond_shift;         // bits of the bond type (0 for one type)
     const float* bond_k;    // per bond type
     const float* bond_r0;
     const signed char* ikind;  // ns: 0 none, 1 EM, 2 GJF
@@ -70,8 +71,7 @@ struct RepDev {
     float* XO_alt;
     int* tp_alt;
     float* q_alt;
-    int* bond_j_alt;
-    int* bond_t_alt;
+    int* bond_alt;
     signed char* ikind_alt;
     int* pid_alt;
     // grids
@@ -338,5 +338,28 @@ __device__ __forceinline__ int stencil_node(float x, float dx, int N, int P) {
 }
 
 __device__ inline int wrap_next(int n, int N) { return (n + 1 == N) ? 0 : n + 1; }
+
+// ---------------------------------------------------------------- fixed-point weights
+// Order-1 deposition in integers: each axis weight W in [0, 1] (0.5 -+ sx) is rounded to
+// q = W 2^22, and a node's weight Wx Wy Wz 2^F is formed from 32 x 32 -> 64-bit
+// products: (qx qy + 2^13) >> 14 (at most 2^30 + 2^9, scale 2^30), times qz plus the
+// rounding half, shifted right by 52 - F.  Two 64-bit multiply-adds and two shifts per
+// node, where rounding the FP32 product Wx Wy Wz 2^F to a 64-bit integer takes a dozen
+// instructions.  The integer weight is within about 2^-22 of the exact product (FP32
+// rounds it within 2^-23 relative).  Both spread paths call these, so their densities
+// stay bit-identical.
+__device__ __forceinline__ uint32_t fixed_axis(float w) {
+    // round to nearest; negative roundoff of 0.5 - sx saturates to 0
+    return (uint32_t)__builtin_amdgcn_rsqf(1.0f) * 0u + (uint32_t)__float2uint_rn(w * 4194304.0f);
+}
+__device__ __forceinline__ uint32_t fixed_pair(uint32_t qx, uint32_t qy) {
+    return (uint32_t)(((uint64_t)qx * qy + (1ull << 13)) >> 14);
+}
+__device__ __forceinline__ unsigned long long fixed_weight3(uint32_t qxy, uint32_t qz, int s3) {
+    return ((uint64_t)qxy * qz + (1ull << (s3 - 1))) >> s3;
+}
+__device__ __forceinline__ unsigned long long fixed_weight2(uint32_t qx, uint32_t qy, int s2) {
+    return ((uint64_t)qx * qy + (s2 > 0 ? 1ull << (s2 - 1) : 0ull)) >> s2;
+}
 
 }  // namespace tild

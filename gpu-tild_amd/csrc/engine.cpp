@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 namespace tild {
 
@@ -155,6 +156,7 @@ Batch::~Batch() {
         }
     if (d_reps_) (void)hipFree(d_reps_);
     if (stream_) (void)hipStreamDestroy(stream_);
+    if (setup_stream_) (void)hipStreamDestroy(setup_stream_);
 }
 
 void Batch::free_maps(PhaseMap* maps) {
@@ -168,6 +170,7 @@ int Batch::init(std::string& err) {
     if (device_ < 0 || device_ >= ndev) { err = "no HIP device " + std::to_string(device_); return -TILD_ERR_DEVICE; }
     HIPCHK(hipSetDevice(device_));
     HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&setup_stream_, hipStreamNonBlocking));
     HIPCHK(configure_kernels());
     return 0;
 }
@@ -199,6 +202,13 @@ int Batch::add(const System& S, uint32_t seed, std::string& err) {
 
 int Batch::add_plain(const System& S, uint32_t seed, std::string& err) {
     HIPCHK(hipSetDevice(device_));
+    // every device operation of the build goes to the set-up stream (and the build ends
+    // by synchronising it), so it overlaps whatever runs on the step stream
+    struct StreamSwap {
+        hipStream_t &a, &b;
+        StreamSwap(hipStream_t& x, hipStream_t& y) : a(x), b(y) { std::swap(a, b); }
+        ~StreamSwap() { std::swap(a, b); }
+    } swap_streams(stream_, setup_stream_);
     auto Rp = std::make_unique<Replica>();
     Replica& R = *Rp;
     RepDev& h = R.h;
@@ -224,6 +234,8 @@ int Batch::add_plain(const System& S, uint32_t seed, std::string& err) {
     for (float qq : S.q) qmax = std::max(qmax, (double)std::fabs(qq));
     int F = fixed_bits(2.0 * S.ns), Fq = fixed_bits(2.0 * S.ns * std::max(qmax, 1e-30));
     h.fix_scale = std::ldexp(1.0f, F);
+    h.fix_s2 = 44 - F;   // F <= 44
+    h.fix_s3 = 52 - F;
     h.fixq_scale = std::ldexp(1.0f, Fq);
     h.fix_inv = std::ldexp(1.0, -F) / (double)S.gvol;
     h.fixq_inv = std::ldexp(1.0, -Fq) / (double)S.gvol;
@@ -291,23 +303,25 @@ int Batch::add_plain(const System& S, uint32_t seed, std::string& err) {
         h.q = d_q;
     }
     {
-        std::vector<int> bj((size_t)S.ns * 3, -1), bt((size_t)S.ns * 3, 0);
+        // one int per bond slot: partner << s | type, with s the bits of the largest
+        // bond type (0 for one type), -1 for an empty slot; partners need 31 - s bits
+        int s = 0;
+        while ((1 << s) < std::max(S.nbond_types, 1)) ++s;
+        if (S.ns - 1 > (int)(0x7fffffffu >> s)) { err = "too many sites for the bond-type count"; return -TILD_ERR_UNSUPPORTED; }
+        h.bond_shift = s;
+        std::vector<int> bv((size_t)S.ns * 3, -1);
         for (int i = 0; i < S.ns; ++i)
-            for (int b = 0; b < S.n_bonds[i]; ++b) {
-                bj[(size_t)i * 3 + b] = S.bonded_to[(size_t)i * kMaxBonds + b];
-                bt[(size_t)i * 3 + b] = S.bond_type[(size_t)i * kMaxBonds + b];
-            }
-        int *d_bj, *d_bt;
+            for (int b = 0; b < S.n_bonds[i]; ++b)
+                bv[(size_t)i * 3 + b] = (S.bonded_to[(size_t)i * kMaxBonds + b] << s) | S.bond_type[(size_t)i * kMaxBonds + b];
+        int* d_bv;
         float *d_bk, *d_br;
-        if (int rc = dalloc(R, &d_bj, bj.size(), err)) return rc;
-        if (int rc = dalloc(R, &d_bt, bt.size(), err)) return rc;
+        if (int rc = dalloc(R, &d_bv, bv.size(), err)) return rc;
         if (int rc = dalloc(R, &d_bk, S.bond_k.size(), err)) return rc;
         if (int rc = dalloc(R, &d_br, S.bond_req.size(), err)) return rc;
-        HIPCHK(hipMemcpy(d_bj, bj.data(), bj.size() * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d_bt, bt.data(), bt.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_bv, bv.data(), bv.size() * 4, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(d_bk, S.bond_k.data(), S.bond_k.size() * 4, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(d_br, S.bond_req.data(), S.bond_req.size() * 4, hipMemcpyHostToDevice));
-        h.bond_j = d_bj; h.bond_t = d_bt; h.bond_k = d_bk; h.bond_r0 = d_br;
+        h.bond = d_bv; h.bond_k = d_bk; h.bond_r0 = d_br;
     }
     // integrators (integrator.cu:24-39): per-site kind; a site may belong to one integrator
     {
@@ -427,8 +441,7 @@ int Batch::add_plain(const System& S, uint32_t seed, std::string& err) {
         if (int rc = dalloc(R, &h.XO_alt, nsD, err)) return rc;
         if (int rc = dalloc(R, &h.tp_alt, S.ns, err)) return rc;
         if (S.charges.enabled) { if (int rc = dalloc(R, &h.q_alt, S.ns, err)) return rc; }
-        if (int rc = dalloc(R, &h.bond_j_alt, (size_t)S.ns * 3, err)) return rc;
-        if (int rc = dalloc(R, &h.bond_t_alt, (size_t)S.ns * 3, err)) return rc;
+        if (int rc = dalloc(R, &h.bond_alt, (size_t)S.ns * 3, err)) return rc;
         if (int rc = dalloc(R, &h.ikind_alt, S.ns, err)) return rc;
         if (int rc = dalloc(R, &h.pid_alt, S.ns, err)) return rc;
         for (int k = 0; k < 2; ++k) {
@@ -528,8 +541,7 @@ void Batch::mirror_reorder(Replica& R) {
     { const signed char* t = h.ikind; h.ikind = h.ikind_alt; h.ikind_alt = const_cast<signed char*>(t); }
     std::swap(h.pid, h.pid_alt);
     if (h.q) { const float* t = h.q; h.q = h.q_alt; h.q_alt = const_cast<float*>(t); }
-    { const int* t = h.bond_j; h.bond_j = h.bond_j_alt; h.bond_j_alt = const_cast<int*>(t); }
-    { const int* t = h.bond_t; h.bond_t = h.bond_t_alt; h.bond_t_alt = const_cast<int*>(t); }
+    { const int* t = h.bond; h.bond = h.bond_alt; h.bond_alt = const_cast<int*>(t); }
     R.permuted = true;
     R.since_sort = 0;
 }
@@ -736,6 +748,22 @@ static inline double k2_of(const RepDev& h, size_t idx, double* kv) {
     return k2;
 }
 
+// f(begin, end) over [0, n) in contiguous ranges on up to 16 host threads (the k-space
+// tables of a large grid take most of a replica's set-up time on one thread).
+template <class F>
+static void parallel_ranges(size_t n, F f) {
+    const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nt = std::min<size_t>({hw, 16, n / 8192 + 1});
+    if (nt <= 1) { f((size_t)0, n); return; }
+    std::vector<std::thread> th;
+    const size_t per = (n + nt - 1) / nt;
+    for (size_t t = 0; t < nt; ++t) {
+        const size_t b = t * per, e = std::min(n, b + per);
+        if (b < e) th.emplace_back([&f, b, e] { f(b, e); });
+    }
+    for (auto& t : th) t.join();
+}
+
 int Batch::build_tables(Replica& R, const System& S, std::string& err) {
     RepDev& h = R.h;
     const int T = S.ntypes, D = h.dim, nP = h.nP;
@@ -794,8 +822,9 @@ int Batch::build_tables(Replica& R, const System& S, std::string& err) {
         h.ramp_c1[j] = (A0 - ps->final_prefactor) / ((double)S.max_steps * u10);
         R.ramps.push_back({ramped[j], A0, h.ramp_c0[j], h.ramp_c1[j]});
     }
+    parallel_ranges(Mh, [&](size_t i0, size_t i1) {
     std::vector<double> c(ncp);
-    for (size_t idx = 0; idx < Mh; ++idx) {
+    for (size_t idx = i0; idx < i1; ++idx) {
         double kv[3];
         double k2 = k2_of(h, idx, kv);
         std::fill(c.begin(), c.end(), 0.0);
@@ -819,6 +848,7 @@ int Batch::build_tables(Replica& R, const System& S, std::string& err) {
             eltab[idx] = (float)(g1 / M);
         }
     }
+    });
     // The fused last-axis pass reads the coupling rows of nl_last adjacent columns for
     // every last-axis index; store them in exactly that order, block by block
     // ([column block][last-axis index][row][column]), so each block streams one
@@ -861,10 +891,11 @@ int Batch::build_tables(Replica& R, const System& S, std::string& err) {
         if (int rc = upload_desc(id, err)) return rc;
         float2 *tmpK, *tmpV;
         float *fr, *vr;
-        HIPCHK(hipMalloc((void**)&tmpK, (size_t)D * Mh * sizeof(float2)));
-        HIPCHK(hipMalloc((void**)&tmpV, (size_t)nP * Mh * sizeof(float2)));
-        HIPCHK(hipMalloc((void**)&fr, (size_t)D * h.M * sizeof(float)));
-        HIPCHK(hipMalloc((void**)&vr, (size_t)nP * h.M * sizeof(float)));
+        // stream-ordered temporaries: freeing them does not wait for other streams
+        HIPCHK(hipMallocAsync((void**)&tmpK, (size_t)D * Mh * sizeof(float2), stream_));
+        HIPCHK(hipMallocAsync((void**)&tmpV, (size_t)nP * Mh * sizeof(float2), stream_));
+        HIPCHK(hipMallocAsync((void**)&fr, (size_t)D * h.M * sizeof(float), stream_));
+        HIPCHK(hipMallocAsync((void**)&vr, (size_t)nP * h.M * sizeof(float), stream_));
         std::vector<float2> hk((size_t)D * Mh), hv((size_t)nP * Mh);
         std::vector<double> vacc((size_t)R.npairs * nP * Mh * 2, 0.0);
         const double scale = gvol / (R.V * (double)D * M * M);
@@ -873,8 +904,9 @@ int Batch::build_tables(Replica& R, const System& S, std::string& err) {
         auto iota = [&](int nblk, WorkItem** dm) -> int {
             std::vector<WorkItem> v(nblk);
             for (int b = 0; b < nblk; ++b) v[b] = WorkItem{id, b};
-            HIPCHK(hipMalloc((void**)dm, nblk * sizeof(WorkItem)));
-            HIPCHK(hipMemcpy(*dm, v.data(), nblk * sizeof(WorkItem), hipMemcpyHostToDevice));
+            HIPCHK(hipMallocAsync((void**)dm, nblk * sizeof(WorkItem), stream_));
+            HIPCHK(hipMemcpyAsync(*dm, v.data(), nblk * sizeof(WorkItem), hipMemcpyHostToDevice, stream_));
+            HIPCHK(hipStreamSynchronize(stream_));   // v is released on return
             return 0;
         };
         int rc = 0;
@@ -890,18 +922,20 @@ int Batch::build_tables(Replica& R, const System& S, std::string& err) {
         };
         for (size_t s = 0; s < styles.size() && !rc; ++s) {
             const PairStyle* ps = styles[s];
-            for (size_t idx = 0; idx < Mh; ++idx) {
-                double kv[3];
-                double k2 = k2_of(h, idx, kv);
-                double u = ps->kspace(k2, ps->initial_prefactor);
-                int n[3] = {(int)(idx % h.Nxh), (int)((idx / h.Nxh) % h.Ny), (int)(idx / h.Nxh / h.Ny)};
-                int N[3] = {h.Nx, h.Ny, h.Nz};
-                for (int d = 0; d < D; ++d) {
-                    double kd = (2 * n[d] == N[d]) ? 0.0 : kv[d];
-                    // -i k_d u (k_d = 0 on the d-Nyquist plane)
-                    hk[(size_t)d * Mh + idx] = make_float2(0.0f, (float)(-kd * u));
+            parallel_ranges(Mh, [&](size_t i0, size_t i1) {
+                for (size_t idx = i0; idx < i1; ++idx) {
+                    double kv[3];
+                    double k2 = k2_of(h, idx, kv);
+                    double u = ps->kspace(k2, ps->initial_prefactor);
+                    int n[3] = {(int)(idx % h.Nxh), (int)((idx / h.Nxh) % h.Ny), (int)(idx / h.Nxh / h.Ny)};
+                    int N[3] = {h.Nx, h.Ny, h.Nz};
+                    for (int d = 0; d < D; ++d) {
+                        double kd = (2 * n[d] == N[d]) ? 0.0 : kv[d];
+                        // -i k_d u (k_d = 0 on the d-Nyquist plane)
+                        hk[(size_t)d * Mh + idx] = make_float2(0.0f, (float)(-kd * u));
+                    }
                 }
-            }
+            });
             HIPCHK(hipMemcpyAsync(tmpK, hk.data(), hk.size() * sizeof(float2), hipMemcpyHostToDevice, stream_));
             if (D == 3) { if ((rc = run_mid(2, D, 1, tmpK))) break; }
             if ((rc = run_mid(1, D, 1, tmpK))) break;
@@ -943,9 +977,10 @@ int Batch::build_tables(Replica& R, const System& S, std::string& err) {
                     vacc[o + 1] += hv[(size_t)cc * Mh + idx].y * scale;
                 }
         }
+        for (WorkItem* m : tmp_maps) (void)hipFreeAsync(m, stream_);
+        (void)hipFreeAsync(tmpK, stream_); (void)hipFreeAsync(tmpV, stream_);
+        (void)hipFreeAsync(fr, stream_); (void)hipFreeAsync(vr, stream_);
         (void)hipStreamSynchronize(stream_);
-        for (WorkItem* m : tmp_maps) (void)hipFree(m);
-        (void)hipFree(tmpK); (void)hipFree(tmpV); (void)hipFree(fr); (void)hipFree(vr);
         if (rc) return rc;
         std::vector<float2> vt((size_t)R.npairs * nP * Mh);
         for (size_t k = 0; k < vt.size(); ++k) vt[k] = make_float2((float)vacc[2 * k], (float)vacc[2 * k + 1]);

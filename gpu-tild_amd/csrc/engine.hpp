@@ -193,6 +193,9 @@ private:
     int spread_mode_ = 0, tile_target_ = 512, tile_by_ = 0, tile_bz_ = 0, tile_slack_ = 0, reorder_every_ = -1;
     FftTuning tune_;
     hipStream_t stream_ = nullptr;
+    // add() builds a replica on its own stream, so that a caller (the job server) can
+    // build new replicas while steps of the others run on stream_
+    hipStream_t setup_stream_ = nullptr;
     std::vector<std::unique_ptr<Replica>> reps_;
     RepDev* d_reps_ = nullptr;
     int d_cap_ = 0;

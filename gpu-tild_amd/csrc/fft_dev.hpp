@@ -21,43 +21,6 @@ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(
 __device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }   // a * (-i)
 __device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
 
-// Global -> LDS staging of `total` elements by a block of NT threads, K per thread per
-// round: a round first issues all of a thread's loads, then writes them.  A loop that
-// writes each value right after loading it waits out one memory round trip per element
-// instead of one per round.  load(i) returns element i; store(i, v) writes it.
-template <int K, int NT, class T, class Load, class Store>
-__device__ __forceinline__ void stage_in(int total, Load load, Store store) {
-    for (int base = 0; base < total; base += K * NT) {
-        T v[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int i = base + k * NT + (int)threadIdx.x;
-            if (i < total) v[k] = load(i);
-        }
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int i = base + k * NT + (int)threadIdx.x;
-            if (i < total) store(i, v[k]);
-        }
-    }
-}
-
-// Twiddles of an axis plan copied into LDS (room for tw_elems(N) values), for the
-// stages to read with LDS loads instead of global ones.  issue() requests this
-// thread's first twiddle (with the caller's other loads); commit() stores it and any
-// further ones.  A barrier must separate commit() from the first stage.
-template <int NT>
-struct TwStage {
-    float2 v;
-    __device__ __forceinline__ void issue(const AxisPlan& pl) {
-        v = gld(pl.tw + min((int)threadIdx.x, pl.ntw - 1));
-    }
-    __device__ __forceinline__ void commit(const AxisPlan& pl, float2* tw_lds) const {
-        if ((int)threadIdx.x < pl.ntw) tw_lds[threadIdx.x] = v;
-        for (int i = (int)threadIdx.x + NT; i < pl.ntw; i += NT) tw_lds[i] = gld(pl.tw + i);
-    }
-};
-
 // Forward DFT of small sizes (2, 3, 4, 5, 7, 8, 10, 16), in place on v[0..R-1].
 template <int R> struct Dft;
 
@@ -222,7 +185,7 @@ __device__ __forceinline__ void fft_stage(const float2* __restrict__ src, float2
         if (p > 1) {
             const float2* w = tw + k * (R - 1);
 #pragma unroll
-            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], w[r - 1]);
+            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], gld(w + r - 1));
         }
         Dft<R>::run(v);
 #pragma unroll
@@ -252,7 +215,7 @@ __device__ __forceinline__ void fft_stage_c(const float2* __restrict__ src, floa
         if constexpr (P > 1) {
             const float2* w = tw + k * (R - 1);
 #pragma unroll
-            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], w[r - 1]);
+            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], gld(w + r - 1));
         }
         Dft<R>::run(v);
 #pragma unroll
@@ -264,11 +227,10 @@ __device__ __forceinline__ void fft_stage_c(const float2* __restrict__ src, floa
 // of the 80- and 160-point axes): no plan is read and no radix switched on per stage.
 // Returns A, as lds_fft does for two stages.
 template <int R1, int R2, int NT>
-__device__ __forceinline__ float2* lds_fft2(float2* A, float2* B, int ldb, int nb, const AxisPlan& pl,
-                                            const float2* tw) {
-    fft_stage_c<R1, NT, R1 * R2, 1>(A, B, ldb, nb, tw + pl.tw_off[0]);
+__device__ __forceinline__ float2* lds_fft2(float2* A, float2* B, int ldb, int nb, const AxisPlan& pl) {
+    fft_stage_c<R1, NT, R1 * R2, 1>(A, B, ldb, nb, pl.tw + pl.tw_off[0]);
     __syncthreads();
-    fft_stage_c<R2, NT, R1 * R2, R1>(B, A, ldb, nb, tw + pl.tw_off[1]);
+    fft_stage_c<R2, NT, R1 * R2, R1>(B, A, ldb, nb, pl.tw + pl.tw_off[1]);
     __syncthreads();
     return A;
 }
@@ -279,20 +241,18 @@ __device__ __forceinline__ float2* lds_fft2(float2* A, float2* B, int ldb, int n
 // ends with a barrier, so the result is visible to the whole block on return.
 // Always inlined: the buffers are then known to be LDS (ds_* instructions, not flat).
 // NT = threads of the block.
-// tw: the plan's twiddle table copied into LDS (TwStage).
 template <int NT = kFftThreads>
-__device__ __forceinline__ float2* lds_fft(float2* A, float2* B, int ldb, int nb, const AxisPlan& pl,
-                                           const float2* tw_lds) {
+__device__ __forceinline__ float2* lds_fft(float2* A, float2* B, int ldb, int nb, const AxisPlan& pl) {
     if (pl.nst == 2) {   // the plans of 80 and 160 points (uniform branch)
-        if (pl.rad[0] == 10 && pl.rad[1] == 8) return lds_fft2<10, 8, NT>(A, B, ldb, nb, pl, tw_lds);
-        if (pl.rad[0] == 16 && pl.rad[1] == 10) return lds_fft2<16, 10, NT>(A, B, ldb, nb, pl, tw_lds);
+        if (pl.rad[0] == 10 && pl.rad[1] == 8) return lds_fft2<10, 8, NT>(A, B, ldb, nb, pl);
+        if (pl.rad[0] == 16 && pl.rad[1] == 10) return lds_fft2<16, 10, NT>(A, B, ldb, nb, pl);
     }
     float2* src = A;
     float2* dst = B;
     int p = 1;
     for (int s = 0; s < pl.nst; ++s) {
         const int R = pl.rad[s];
-        const float2* tw = tw_lds + pl.tw_off[s];
+        const float2* tw = pl.tw + pl.tw_off[s];
         switch (R) {
             case 2: fft_stage<2, NT>(src, dst, ldb, nb, pl.N, p, tw); break;
             case 3: fft_stage<3, NT>(src, dst, ldb, nb, pl.N, p, tw); break;

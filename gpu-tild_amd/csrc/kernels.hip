@@ -50,10 +50,18 @@ __global__ void __launch_bounds__(256) k_spread(const RepDev* __restrict__ reps,
         for (int iy = 0, nyy = n0[1]; iy <= P; ++iy, nyy = wrap_next(nyy, r.Ny)) {
             const int rowb = (nzz * r.Ny + nyy) * r.Nx;
             for (int ix = 0, nxx = n0[0]; ix <= P; ++ix, nxx = wrap_next(nxx, r.Nx)) {
-                // Wx*Wy*Wz in the reference's product order (device_grid_utils.cu:329)
+                // Wx*Wy*Wz in the reference's product order (device_grid_utils.cu:329);
+                // order 1 in the integers of the tiled deposit (fixed_weight2/3)
                 float w = D == 3 ? W[0][ix] * W[1][iy] * W[2][iz] : W[0][ix] * W[1][iy];
-                long long v = __float2ll_rn(w * r.fix_scale);
-                atomicAdd(acc + rowb + nxx, (unsigned long long)v);
+                unsigned long long v;
+                if (P == 1) {
+                    const uint32_t qx = fixed_axis(W[0][ix]), qy = fixed_axis(W[1][iy]);
+                    v = D == 3 ? fixed_weight3(fixed_pair(qx, qy), fixed_axis(W[2][iz]), r.fix_s3)
+                               : fixed_weight2(qx, qy, r.fix_s2);
+                } else {
+                    v = (unsigned long long)__float2ll_rn(w * r.fix_scale);
+                }
+                atomicAdd(acc + rowb + nxx, v);
                 if (charged) {
                     long long vq = __float2ll_rn((w * qi) * r.fixq_scale);
                     atomicAdd(accq + rowb + nxx, (unsigned long long)vq);
@@ -88,34 +96,14 @@ __global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_e
     float2* Hout = out_override ? out_override : r.H;
     gmem<unsigned long long> acc = G(r.acc);
     gmem<float> rho_out = G(r.rho);
-    float2* twl = lds + 2 * (size_t)N * ldb;   // the x axis' twiddles
-    TwStage<kFftThreads> tws;
-    tws.issue(r.ax[0]);
     const FastDiv dN(N);
     const int type_rows = r.ntypes * rows_per_field;   // rows below are particle densities
-    if (!FROM_ACC) {
-        stage_in<8, kFftThreads, float2>(
-            nb * N,
-            [&](int task) {
-                int n;
-                const int b = dN.div(task, n);
-                const int ra = 2 * (p0 + b), rb = ra + 1;
-                const float a = rin[(size_t)ra * N + n];
-                const float bb = rb < nrows ? rin[(size_t)rb * N + n] : 0.f;
-                return make_float2(a, bb);
-            },
-            [&](int task, float2 v) {
-                int n;
-                const int b = dN.div(task, n);
-                lds[n * ldb + b] = v;
-            });
-    }
-    for (int task = threadIdx.x; FROM_ACC && task < nb * N; task += kFftThreads) {
+    for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
         int n;
         const int b = dN.div(task, n);
         const int ra = 2 * (p0 + b), rb = ra + 1;
         float a, bb = 0.f;
-        {
+        if (FROM_ACC) {
             size_t ia = (size_t)ra * N + n;
             a = (float)((double)(long long)acc[ia] * (ra < type_rows ? r.fix_inv : r.fixq_inv));
             acc[ia] = 0ull;
@@ -126,12 +114,14 @@ __global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_e
                 acc[ib] = 0ull;
                 rho_out[ib] = bb;
             }
+        } else {
+            a = rin[(size_t)ra * N + n];
+            if (rb < nrows) bb = rin[(size_t)rb * N + n];
         }
         lds[n * ldb + b] = make_float2(a, bb);
     }
-    tws.commit(r.ax[0], twl);
     __syncthreads();
-    const float2* res = lds_fft(lds, lds + (size_t)N * ldb, ldb, nb, r.ax[0], twl);
+    const float2* res = lds_fft(lds, lds + (size_t)N * ldb, ldb, nb, r.ax[0]);
     const FastDiv dNxh(Nxh);
     for (int task = threadIdx.x; task < nb * Nxh; task += kFftThreads) {
         int k;
@@ -162,25 +152,15 @@ __global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_e
     const int nb = min(g.nl, g.batch - b0);
     const int ldb = nb | 1;
     float2* base = buf + (size_t)o * g.ostride + b0;
-    float2* twl = lds + 2 * (size_t)g.N * ldb;   // the axis' twiddles
-    TwStage<kFftThreads> tws;
-    tws.issue(r.ax[axis]);
     const FastDiv dnb(nb);
-    stage_in<8, kFftThreads, float2>(
-        nb * g.N,
-        [&](int task) {
-            int b;
-            const int n = dnb.div(task, b);
-            return gld(base + (size_t)n * g.lstride + b);
-        },
-        [&](int task, float2 v) {
-            int b;
-            const int n = dnb.div(task, b);
-            lds[n * ldb + b] = inverse ? conjf2(v) : v;
-        });
-    tws.commit(r.ax[axis], twl);
+    for (int task = threadIdx.x; task < nb * g.N; task += kFftThreads) {
+        int b;
+        const int n = dnb.div(task, b);
+        float2 v = gld(base + (size_t)n * g.lstride + b);
+        lds[n * ldb + b] = inverse ? conjf2(v) : v;
+    }
     __syncthreads();
-    const float2* res = lds_fft(lds, lds + (size_t)g.N * ldb, ldb, nb, r.ax[axis], twl);
+    const float2* res = lds_fft(lds, lds + (size_t)g.N * ldb, ldb, nb, r.ax[axis]);
     for (int task = threadIdx.x; task < nb * g.N; task += kFftThreads) {
         int b;
         const int n = dnb.div(task, b);
@@ -201,43 +181,21 @@ __device__ __forceinline__ float ramp_weight(const RepDev& r, int j, int k) {
 // as the general loop of k_last_fused): w_I = sum_J C_IJ rho_J with the symmetric table
 // rows pidx(lo, hi) = lo*TT - lo*(lo-1)/2 + (hi - lo), and the charge row.  Ramped
 // styles add their rows, weighted, to their pairs' coefficients.
-// Table rows of the first KC nodes (task = thread + k NT) of a thread, loaded before
-// the forward transform so that they arrive while it runs: the TT(TT+1)/2 pair rows and
-// the charge row.
-template <int TT> struct CoupleRows {
-    static constexpr int NR = TT * (TT + 1) / 2;
-    static constexpr int KC = (1280 / TT + 255) / 256;   // the default block's nodes per thread
-    float c[KC][NR + 1];
-};
-
-template <int TT, int NT>
-__device__ __forceinline__ void couple_prefetch(gmem<const float> cb, int nrows, int nlc, int nb, int N, bool charged,
-                                                int ctab_charge, const FastDiv& dnb, CoupleRows<TT>& pre) {
-    constexpr int NR = CoupleRows<TT>::NR, KC = CoupleRows<TT>::KC;
-#pragma unroll
-    for (int k = 0; k < KC; ++k) {
-        int b;
-        const int n = dnb.div(min((int)threadIdx.x + k * NT, nb * N - 1), b);
-        gmem<const float> cn = cb + (n * nrows * nlc + b);
-#pragma unroll
-        for (int q = 0; q < NR; ++q) pre.c[k][q] = cn[q * nlc];
-        pre.c[k][NR] = charged ? cn[ctab_charge * nlc] : 0.f;
-    }
-}
-
 template <int TT, int NT>
 __device__ __forceinline__ void couple_nodes(gmem<const float> cb, int nrows, int nlc, int nb, int N, int ldb, int ldw,
                                              int nf, int ctab_charge, const FastDiv& dnb, const float2* A, float2* Wb,
-                                             const RepDev& r, const float* rw, const CoupleRows<TT>& pre) {
-    constexpr int NR = CoupleRows<TT>::NR, KC = CoupleRows<TT>::KC;
+                                             const RepDev& r, const float* rw) {
+    constexpr int NR = TT * (TT + 1) / 2;
     const bool charged = nf > TT;
-    // node `task` with its table rows in c[0..NR-1] and the charge row in c[NR]
-    auto node = [&](int task, float (&c)[NR + 1]) {
-        int b;
-        const int n = dnb.div(task, b);
+    Digits2 gc(threadIdx.x, NT, dnb);
+    for (int task = threadIdx.x; task < nb * N; task += NT, gc.step()) {
+        const int b = gc.lo, n = gc.hi;
         const int row = n * ldb + b, wrow = n * ldw + b;
         gmem<const float> cn = cb + (n * nrows * nlc + b);   // offsets inside the block's run
-        const float cq = c[NR];
+        float c[NR];
+#pragma unroll
+        for (int k = 0; k < NR; ++k) c[k] = cn[k * nlc];
+        const float cq = charged ? cn[ctab_charge * nlc] : 0.f;
         for (int j = 0; j < r.nramp; ++j) {   // uniform trip count
             const float v = rw[j] * cn[(r.ramp_row0 + j) * nlc];
 #pragma unroll
@@ -261,27 +219,6 @@ __device__ __forceinline__ void couple_nodes(gmem<const float> cb, int nrows, in
             const float2 rq = A[row + TT * nb];
             Wb[wrow + TT * nb] = make_float2(cq * rq.x, cq * rq.y);
         }
-    };
-    const int total = nb * N;
-#pragma unroll
-    for (int k = 0; k < KC; ++k) {
-        const int task = (int)threadIdx.x + k * NT;
-        if (task < total) {
-            float c[NR + 1];
-#pragma unroll
-            for (int q = 0; q <= NR; ++q) c[q] = pre.c[k][q];
-            node(task, c);
-        }
-    }
-    for (int task = (int)threadIdx.x + KC * NT; task < total; task += NT) {   // blocks wider than the default
-        int b;
-        const int n = dnb.div(task, b);
-        gmem<const float> cn = cb + (n * nrows * nlc + b);
-        float c[NR + 1];
-#pragma unroll
-        for (int q = 0; q < NR; ++q) c[q] = cn[q * nlc];
-        c[NR] = charged ? cn[ctab_charge * nlc] : 0.f;
-        node(task, c);
     }
 }
 
@@ -295,10 +232,13 @@ __device__ __forceinline__ void couple_nodes(gmem<const float> cb, int nrows, in
 // transform: along the last axis la, w and -i k_la w go to Gk[f*dim] and
 // Gk[f*dim + la]; the factors of the other directions are constant along those lines
 // and are applied by the passes of their own axes.
-// TT = 1..4: the type count, with the table rows of the coupling prefetched and its
-// sums unrolled; TT = 0: any type count.
-template <int TT, int NT>
-__device__ __forceinline__ void last_fused_body(const RepDev& r, const WorkItem wi, int koff, float2* lds) {
+template <int NT>
+__global__ void __launch_bounds__(NT) k_last_fused(const RepDev* __restrict__ reps,
+                                                            const WorkItem* __restrict__ wmap,
+                                                            int store_spectrum, int koff) {
+    extern __shared__ float2 lds[];
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
     const int la = r.dim - 1;                       // last axis
     const int N = la == 2 ? r.Nz : r.Ny;
     const int Q = la == 2 ? r.Ny * r.Nxh : r.Nxh;   // plane size = line stride
@@ -316,39 +256,31 @@ __device__ __forceinline__ void last_fused_body(const RepDev& r, const WorkItem 
     float2* E0 = lds + (size_t)N * ldw;
     float2* E1 = E0 + (size_t)N * ldg;
     float* rw = (float*)(E1 + (size_t)N * ldg);          // ramp weights of this step
-    float2* twl = (float2*)(rw + kMaxRamp);              // the last axis' twiddles
-    TwStage<NT> tws;
-    tws.issue(r.ax[la]);
     if ((int)threadIdx.x < r.nramp) rw[threadIdx.x] = ramp_weight(r, threadIdx.x, r.step + koff);
     const FastDiv dnb(nb), dnf(nf);
     const bool o32 = r.grid_off32;   // field offsets fit 32-bit byte offsets
-    // this block's coupling-table rows are one contiguous run; those of the first
-    // nodes of each thread are requested before the spectra, to arrive during the
-    // forward transform
-    const int nrows = r.ctab_rows, nlc = r.nl_last;
-    gmem<const float> cb = G(r.Ctab) + (size_t)wi.blk * N * nrows * nlc;
-    const bool charged = nf > T;
-    CoupleRows<TT == 0 ? 1 : TT> pre;
-    if constexpr (TT > 0) couple_prefetch<TT, NT>(cb, nrows, nlc, nb, N, charged, r.ctab_charge, dnb, pre);
-    stage_in<8, NT, float2>(
-        nlines * N,
-        [&](int task) {
-            int b, f;
-            const int n = dnf.div(dnb.div(task, b), f);
-            return o32 ? gld(at32((const float2*)r.H, (uint32_t)f * (uint32_t)r.Mh + (uint32_t)(n * Q + q0 + b)))
-                       : gld(r.H + (size_t)f * r.Mh + (size_t)n * Q + q0 + b);
-        },
-        [&](int task, float2 v) {
-            int b, f;
-            const int n = dnf.div(dnb.div(task, b), f);
-            E0[n * ldb + f * nb + b] = v;
-        });
-    tws.commit(r.ax[la], twl);
+    Digits3 gl(threadIdx.x, NT, dnb, dnf);
+    for (int task = threadIdx.x; task < nlines * N; task += NT, gl.step()) {
+        const int b = gl.lo, f = gl.mid, n = gl.hi;
+        E0[n * ldb + f * nb + b] =
+            o32 ? gld(at32((const float2*)r.H, (uint32_t)f * (uint32_t)r.Mh + (uint32_t)(n * Q + q0 + b)))
+                : gld(r.H + (size_t)f * r.Mh + (size_t)n * Q + q0 + b);
+    }
     __syncthreads();
-    const float2* A = lds_fft<NT>(E0, E1, ldb, nlines, r.ax[la], twl);   // spectra rho_f(k)
-    // coupling w_I = sum_J C_IJ rho_J (C symmetric, rows I <= J), charge w_c = G_c rho_c
-    if constexpr (TT > 0) couple_nodes<TT, NT>(cb, nrows, nlc, nb, N, ldb, ldw, nf, r.ctab_charge, dnb, A, Wb, r, rw, pre);
-    for (int task = threadIdx.x; TT == 0 && task < nb * N; task += NT) {
+    const float2* A = lds_fft<NT>(E0, E1, ldb, nlines, r.ax[la]);   // spectra rho_f(k)
+    // coupling w_I = sum_J C_IJ rho_J (C symmetric, rows I <= J), charge w_c = G_c rho_c;
+    // this block's table rows are one contiguous run
+    const int nrows = r.ctab_rows, nlc = r.nl_last;
+
+    gmem<const float> cb = G(r.Ctab) + (size_t)wi.blk * N * nrows * nlc;
+    switch (T) {   // up to four types: unrolled, every table row of a node requested at once
+        case 1: couple_nodes<1, NT>(cb, nrows, nlc, nb, N, ldb, ldw, nf, r.ctab_charge, dnb, A, Wb, r, rw); break;
+        case 2: couple_nodes<2, NT>(cb, nrows, nlc, nb, N, ldb, ldw, nf, r.ctab_charge, dnb, A, Wb, r, rw); break;
+        case 3: couple_nodes<3, NT>(cb, nrows, nlc, nb, N, ldb, ldw, nf, r.ctab_charge, dnb, A, Wb, r, rw); break;
+        case 4: couple_nodes<4, NT>(cb, nrows, nlc, nb, N, ldb, ldw, nf, r.ctab_charge, dnb, A, Wb, r, rw); break;
+        default: break;
+    }
+    for (int task = threadIdx.x; T > 4 && task < nb * N; task += NT) {
         int b;
         const int n = dnb.div(task, b);
         const int row = n * ldb + b, wrow = n * ldw + b;
@@ -395,7 +327,7 @@ __device__ __forceinline__ void last_fused_body(const RepDev& r, const WorkItem 
             }
         }
         __syncthreads();
-        const float2* res = lds_fft<NT>(E0, E1, ldg, nlg, r.ax[la], twl);
+        const float2* res = lds_fft<NT>(E0, E1, ldg, nlg, r.ax[la]);
         Digits3 gs(threadIdx.x, NT, dnb, FastDiv(nf * no));
         for (int task = threadIdx.x; task < nlg * N; task += NT, gs.step()) {
             const int b = gs.lo, fo = gs.mid, n = gs.hi;
@@ -406,22 +338,6 @@ __device__ __forceinline__ void last_fused_body(const RepDev& r, const WorkItem 
             else gst(r.Gk + (size_t)(f * D + slot) * r.Mh + (size_t)n * Q + q0 + b, v);
         }
         __syncthreads();
-    }
-}
-
-template <int NT>
-__global__ void __launch_bounds__(NT) k_last_fused(const RepDev* __restrict__ reps,
-                                                   const WorkItem* __restrict__ wmap,
-                                                   int store_spectrum, int koff) {
-    extern __shared__ float2 lds[];
-    const WorkItem wi = wmap[blockIdx.x];
-    const RepDev& r = reps[wi.rep];
-    switch (r.ntypes) {   // up to four types: unrolled, table rows prefetched
-        case 1: last_fused_body<1, NT>(r, wi, koff, lds); break;
-        case 2: last_fused_body<2, NT>(r, wi, koff, lds); break;
-        case 3: last_fused_body<3, NT>(r, wi, koff, lds); break;
-        case 4: last_fused_body<4, NT>(r, wi, koff, lds); break;
-        default: last_fused_body<0, NT>(r, wi, koff, lds); break;
     }
 }
 
@@ -448,30 +364,19 @@ k_mid_grad(const RepDev* __restrict__ reps, const WorkItem* __restrict__ wmap) {
     float2* U = r.Gk + (size_t)(f * 3 + 0) * r.Mh + plane;
     float2* Y = r.Gk + (size_t)(f * 3 + 1) * r.Mh + plane;
     float2* V = r.Gk + (size_t)(f * 3 + 2) * r.Mh + plane;
-    float2* twl = lds + 2 * (size_t)N * ldb;   // the y axis' twiddles
-    TwStage<NT> tws;
-    tws.issue(r.ax[1]);
     const FastDiv dnb(nb);
-    stage_in<4, NT, float4>(
-        nb * N,
-        [&](int task) {
-            int b;
-            const int n = dnb.div(task, b);
-            const float2 u = gld(U + (n * Nxh + b)), v = gld(V + (n * Nxh + b));
-            return make_float4(u.x, u.y, v.x, v.y);
-        },
-        [&](int task, float4 uv) {
-            int b;
-            const int n = dnb.div(task, b);
-            const float ky = 2 * n == N ? 0.f : r.kfac[1] * (float)(2 * n < N ? n : n - N);
-            // lines of the conj(FFT(conj)) inverse: conj(u), conj(-i ky u), conj(v)
-            lds[n * ldb + b] = make_float2(uv.x, -uv.y);
-            lds[n * ldb + nb + b] = make_float2(ky * uv.y, ky * uv.x);
-            lds[n * ldb + 2 * nb + b] = make_float2(uv.z, -uv.w);
-        });
-    tws.commit(r.ax[1], twl);
+    Digits2 gl(threadIdx.x, NT, dnb);
+    for (int task = threadIdx.x; task < nb * N; task += NT, gl.step()) {
+        const int b = gl.lo, n = gl.hi;
+        const float ky = 2 * n == N ? 0.f : r.kfac[1] * (float)(2 * n < N ? n : n - N);
+        const float2 u = gld(U + (n * Nxh + b)), v = gld(V + (n * Nxh + b));
+        // lines of the conj(FFT(conj)) inverse: conj(u), conj(-i ky u), conj(v)
+        lds[n * ldb + b] = make_float2(u.x, -u.y);
+        lds[n * ldb + nb + b] = make_float2(ky * u.y, ky * u.x);
+        lds[n * ldb + 2 * nb + b] = make_float2(v.x, -v.y);
+    }
     __syncthreads();
-    const float2* res = lds_fft<NT>(lds, lds + (size_t)N * ldb, ldb, 3 * nb, r.ax[1], twl);
+    const float2* res = lds_fft<NT>(lds, lds + (size_t)N * ldb, ldb, 3 * nb, r.ax[1]);
     Digits3 gs(threadIdx.x, NT, dnb, FastDiv(3));
     for (int task = threadIdx.x; task < 3 * nb * N; task += NT, gs.step()) {
         const int b = gs.lo, sl = gs.mid, n = gs.hi;
@@ -510,6 +415,64 @@ __device__ __forceinline__ void c2r_records(const RepDev& r, const float2* res, 
     }
 }
 
+// The gather-record form of the inverse x pass for NF fields in D dimensions (buffers
+// below 4 GiB, no field phases), with the line bookkeeping known at compile time: the
+// same loads, arithmetic and stores as the general body of k_x_c2r (bit-identical
+// records), with one run-time division per task instead of two and no run-time line
+// masks.
+template <int NF, int D, int NT>
+__device__ __forceinline__ void x_c2r_records(const RepDev& r, const WorkItem wi, float2* lds) {
+    constexpr int NL = NF * D, NC = (NL + 1) / 2;
+    const int N = r.Nx, Nxh = r.Nxh;
+    const int rows_per_field = r.M / r.Nx;
+    const int row0 = wi.blk * r.nr_c2r;
+    const int rows = min(r.nr_c2r, rows_per_field - row0);
+    const int nb = rows * NC;
+    const int ldb = nb | 1;
+    const float2* Gin = (const float2*)r.Gk;
+    const bool even = (N & 1) == 0;
+    const uint32_t plane = (uint32_t)rows_per_field * (uint32_t)Nxh;   // one line's half spectra
+    const float kf = r.kfac[0];
+    const FastDiv dNxh(Nxh);
+    for (int task = threadIdx.x; task < nb * Nxh; task += NT) {
+        int kk;
+        const int b = dNxh.div(task, kk);
+        const int rr = b / NC, j = b - rr * NC;
+        const int la = 2 * j, lb = la + 1;
+        const uint32_t ia = (uint32_t)la * plane + (uint32_t)(row0 + rr) * (uint32_t)Nxh + (uint32_t)kk;
+        float2 A = gld(at32(Gin, ia)), Bv = make_float2(0.f, 0.f);
+        if (lb < NL) Bv = gld(at32(Gin, ia + plane));
+        const bool self = kk == 0 || (even && 2 * kk == N);   // self-conjugate column
+        // lines f*D + 0 hold the x derivative without its factor -i k_x
+        const float kx = self && kk != 0 ? 0.f : kf * (float)kk;
+        if (la % D == 0) A = make_float2(kx * A.y, -kx * A.x);
+        if (lb < NL && lb % D == 0) Bv = make_float2(kx * Bv.y, -kx * Bv.x);
+        if (self) { A.y = 0.f; Bv.y = 0.f; }
+        lds[kk * ldb + b] = make_float2(A.x - Bv.y, -(A.y + Bv.x));
+        if (!self && N - kk >= Nxh) lds[(N - kk) * ldb + b] = make_float2(A.x + Bv.y, A.y - Bv.x);
+    }
+    __syncthreads();
+    const float2* res = lds_fft<NT>(lds, lds + (size_t)N * ldb, ldb, nb, r.ax[0]);
+    const FastDiv dN(N);
+    for (int task = threadIdx.x; task < rows * NF * N; task += NT) {
+        int x;
+        const int rest = dN.div(task, x);
+        const int rr = rest / NF, f = rest - rr * NF;
+        float g[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int line = f * D + d;
+            const float2 z = res[x * ldb + rr * NC + line / 2];
+            g[d] = (line & 1) ? -z.y : z.x;      // conj of the forward result
+        }
+        const uint32_t fn = (uint32_t)f * (uint32_t)r.M + (uint32_t)(row0 + rr) * (uint32_t)N + (uint32_t)x;
+        // the gather skips nodes with rho <= 0 (device_grid_utils.cu:424-452): a zero
+        // record adds nothing
+        if (f < r.ntypes && !(G(at32((const float*)r.rho, fn))[0] > 0.f)) g[0] = g[1] = g[2] = 0.f;
+        gst3((float*)((char*)r.G3 + ((fn << 3) + (fn << 2))), g[0], g[1], g[2]);   // 12 fn bytes
+    }
+}
+
 template <bool RECORDS, int NT>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) k_x_c2r(const RepDev* __restrict__ reps,
                                                        const WorkItem* __restrict__ wmap,
@@ -518,6 +481,11 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) k_
     extern __shared__ float2 lds[];
     const WorkItem wi = wmap[blockIdx.x];
     const RepDev& r = reps[wi.rep];
+    if (RECORDS && r.grid_off32 && !r.FPtab) {   // the common layouts, per block (uniform)
+        if (r.dim == 3 && r.nf == 2) { x_c2r_records<2, 3, NT>(r, wi, lds); return; }
+        if (r.dim == 3 && r.nf == 5) { x_c2r_records<5, 3, NT>(r, wi, lds); return; }
+        if (r.dim == 2 && r.nf == 2) { x_c2r_records<2, 2, NT>(r, wi, lds); return; }
+    }
     const int N = r.Nx, Nxh = r.Nxh;
     const int rows_per_field = r.M / r.Nx;
     const int nlines = RECORDS ? r.nf * r.dim : nlines_override;     // real lines per row
@@ -540,56 +508,42 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) k_
     // (about two tasks per thread: a division pair each is cheaper than the set-up of
     // digit counters here)
     const bool o32 = RECORDS && r.grid_off32;
-    const bool rec32 = RECORDS && o32 && !r.FPtab;   // records with the dimension unrolled
-    const FastDiv dnf(RECORDS ? r.nf : 1);
-    float2* twl = lds + 2 * (size_t)N * ldb;   // the x axis' twiddles
-    TwStage<NT> tws;
-    tws.issue(r.ax[0]);
-    stage_in<4, NT, float4>(
-        nb * Nxh,
-        [&](int task) {
-            int kk, j;
-            const int b = dNxh.div(task, kk);
-            const int row = row0 + dnc.div(b, j);
-            const int la = 2 * j, lb = la + 1;
-            float2 A, Bv = make_float2(0.f, 0.f);
-            if (o32) {
-                const uint32_t ia = ((uint32_t)la * (uint32_t)rows_per_field + (uint32_t)row) * (uint32_t)Nxh + (uint32_t)kk;
-                A = gld(at32(Gin, ia));
-                if (lb < nlines) Bv = gld(at32(Gin, ia + (uint32_t)rows_per_field * (uint32_t)Nxh));
-            } else {
-                A = gld(Gin + ((size_t)la * rows_per_field + row) * Nxh + kk);
-                if (lb < nlines) Bv = gld(Gin + ((size_t)lb * rows_per_field + row) * Nxh + kk);
-            }
-            return make_float4(A.x, A.y, Bv.x, Bv.y);
-        },
-        [&](int task, float4 ab) {
-            int kk, j;
-            const int b = dNxh.div(task, kk);
-            dnc.div(b, j);
-            const int la = 2 * j, lb = la + 1;
-            float2 A = make_float2(ab.x, ab.y), Bv = make_float2(ab.z, ab.w);
-            const bool self = kk == 0 || (even && 2 * kk == N);   // self-conjugate column
-            if (RECORDS) {
-                const float kx = self && kk != 0 ? 0.f : r.kfac[0] * (float)kk;
-                if ((xlines >> la) & 1ull) A = make_float2(kx * A.y, -kx * A.x);
-                if ((xlines >> lb) & 1ull) Bv = make_float2(kx * Bv.y, -kx * Bv.x);
-            }
-            if (self) { A.y = 0.f; Bv.y = 0.f; }
-            // Z = A + i B ; store conj(Z) for the conj(FFT(conj)) inverse.  Column N - kk
-            // holds the Hermitian extension conj(A) + i conj(B).
-            lds[kk * ldb + b] = make_float2(A.x - Bv.y, -(A.y + Bv.x));
-            if (!self && N - kk >= Nxh) lds[(N - kk) * ldb + b] = make_float2(A.x + Bv.y, A.y - Bv.x);
-        });
-    tws.commit(r.ax[0], twl);
+    for (int task = threadIdx.x; task < nb * Nxh; task += NT) {
+        int kk, j;
+        const int b = dNxh.div(task, kk);
+        const int row = row0 + dnc.div(b, j);
+        const int la = 2 * j, lb = la + 1;
+        float2 A, Bv = make_float2(0.f, 0.f);
+        if (o32) {
+            const uint32_t ia = ((uint32_t)la * (uint32_t)rows_per_field + (uint32_t)row) * (uint32_t)Nxh + (uint32_t)kk;
+            A = gld(at32(Gin, ia));
+            if (lb < nlines) Bv = gld(at32(Gin, ia + (uint32_t)rows_per_field * (uint32_t)Nxh));
+        } else {
+            A = gld(Gin + ((size_t)la * rows_per_field + row) * Nxh + kk);
+            if (lb < nlines) Bv = gld(Gin + ((size_t)lb * rows_per_field + row) * Nxh + kk);
+        }
+        const bool self = kk == 0 || (even && 2 * kk == N);   // self-conjugate column
+        if (RECORDS) {
+            const float kx = self && kk != 0 ? 0.f : r.kfac[0] * (float)kk;
+            if ((xlines >> la) & 1ull) A = make_float2(kx * A.y, -kx * A.x);
+            if ((xlines >> lb) & 1ull) Bv = make_float2(kx * Bv.y, -kx * Bv.x);
+        }
+        if (self) { A.y = 0.f; Bv.y = 0.f; }
+        // Z = A + i B ; store conj(Z) for the conj(FFT(conj)) inverse.  Column N - kk
+        // holds the Hermitian extension conj(A) + i conj(B).
+        lds[kk * ldb + b] = make_float2(A.x - Bv.y, -(A.y + Bv.x));
+        if (!self && N - kk >= Nxh) lds[(N - kk) * ldb + b] = make_float2(A.x + Bv.y, A.y - Bv.x);
+    }
     __syncthreads();
-    const float2* res = lds_fft<NT>(lds, lds + (size_t)N * ldb, ldb, nb, r.ax[0], twl);
-    if (rec32) {
+    const float2* res = lds_fft<NT>(lds, lds + (size_t)N * ldb, ldb, nb, r.ax[0]);
+    if (RECORDS && o32 && !r.FPtab) {
         // records of buffers below 4 GiB: dimension unrolled, 32-bit offsets
+        const FastDiv dnf(r.nf);
         if (r.dim == 3) c2r_records<3, NT>(r, res, ldb, nc, rows, row0, N, dN, dnf);
         else c2r_records<2, NT>(r, res, ldb, nc, rows, row0, N, dN, dnf);
     } else if (RECORDS) {
         const int D = r.dim;
+        const FastDiv dnf(r.nf);
         for (int task = threadIdx.x; task < rows * r.nf * N; task += NT) {
             int x, f;
             const int rest = dN.div(task, x), rr = dnf.div(rest, f);
@@ -642,29 +596,15 @@ __global__ void __launch_bounds__(NT) k_plane_r2c(const RepDev* __restrict__ rep
     const size_t bsz = max((size_t)Nx * ldp, (size_t)Ny * ldc);
     float2* A = lds;
     float2* B = lds + bsz;
-    float2* twx = lds + 2 * bsz;               // the x and y axes' twiddles
-    float2* twy = twx + tw_elems(Nx);
-    TwStage<NT> tsx, tsy;
-    tsx.issue(r.ax[0]);
-    tsy.issue(r.ax[1]);
     gmem<const float> rho = G((const float*)r.rho) + (size_t)f * r.M + (size_t)z * Ny * Nx;
     const FastDiv dNx(Nx), dNxh(Nxh);
-    stage_in<8, NT, float2>(
-        np * Nx,
-        [&](int task) {
-            int n;
-            const int j = dNx.div(task, n);
-            return make_float2(rho[(2 * j) * Nx + n], rho[(2 * j + 1) * Nx + n]);
-        },
-        [&](int task, float2 v) {
-            int n;
-            const int j = dNx.div(task, n);
-            A[n * ldp + j] = v;
-        });
-    tsx.commit(r.ax[0], twx);
-    tsy.commit(r.ax[1], twy);
+    Digits2 gl(threadIdx.x, NT, dNx);
+    for (int task = threadIdx.x; task < np * Nx; task += NT, gl.step()) {
+        const int n = gl.lo, j = gl.hi;
+        A[n * ldp + j] = make_float2(rho[(size_t)(2 * j) * Nx + n], rho[(size_t)(2 * j + 1) * Nx + n]);
+    }
     __syncthreads();
-    float2* Z = lds_fft<NT>(A, B, ldp, np, r.ax[0], twx);
+    float2* Z = lds_fft<NT>(A, B, ldp, np, r.ax[0]);
     float2* C = (Z == A) ? B : A;
     Digits2 gm(threadIdx.x, NT, dNxh);
     for (int task = threadIdx.x; task < np * Nxh; task += NT, gm.step()) {
@@ -675,7 +615,7 @@ __global__ void __launch_bounds__(NT) k_plane_r2c(const RepDev* __restrict__ rep
         C[(2 * j + 1) * ldc + k] = make_float2(0.5f * (Zk.y + Zm.y), 0.5f * (Zm.x - Zk.x));
     }
     __syncthreads();
-    const float2* Y = lds_fft<NT>(C, Z, ldc, Nxh, r.ax[1], twy);
+    const float2* Y = lds_fft<NT>(C, Z, ldc, Nxh, r.ax[1]);
     float2* H = r.H + ((size_t)f * r.Nz + z) * Ny * Nxh;
     Digits2 gs(threadIdx.x, NT, dNxh);
     for (int task = threadIdx.x; task < Ny * Nxh; task += NT, gs.step()) {
@@ -854,8 +794,7 @@ __device__ __forceinline__ void particle_bead(const RepDev& r, int i, int write_
     const int t = G(r.tp)[i];
     const bool charged = r.q != nullptr;
     const float qi = charged ? G(r.q)[i] : 0.f;
-    const int3 bj = gld3(r.bond_j + (size_t)i * 3);
-    const int3 btv = gld3(r.bond_t + (size_t)i * 3);
+    const int3 bv = gld3(r.bond + (size_t)i * 3);
     // bonds, every bond evaluated from both ends (no atomics).  Slots fill from the
     // front and the first empty one ends the list; an empty slot loads the bead itself
     // and bond type 0 (both exist) and contributes nothing, so the loads of all three
@@ -864,11 +803,12 @@ __device__ __forceinline__ void particle_bead(const RepDev& r, int i, int write_
     // while the partners are in flight
     uint32_t site = 0;
     if constexpr (INTEGRATE) site = (uint32_t)G(r.pid)[i];   // original site id: slot order does not matter
-    const int js[3] = {bj.x, bj.y, bj.z}, ts[3] = {btv.x, btv.y, btv.z};
+    const int bs = r.bond_shift, tmask = (1 << bs) - 1;
+    const int js[3] = {bv.x >> bs, bv.y >> bs, bv.z >> bs}, ts[3] = {bv.x & tmask, bv.y & tmask, bv.z & tmask};
     bool live[3];
-    live[0] = js[0] >= 0;
-    live[1] = live[0] && js[1] >= 0;
-    live[2] = live[1] && js[2] >= 0;
+    live[0] = bv.x >= 0;
+    live[1] = live[0] && bv.y >= 0;
+    live[2] = live[1] && bv.z >= 0;
     float xj[3][3], kb[3], r0[3];
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
@@ -1072,10 +1012,11 @@ __global__ void __launch_bounds__(256) k_bond_props(const RepDev* __restrict__ r
         const int D = r.dim;
         const float* x = r.X[r.step & 1];
         for (int b = 0; b < 3; ++b) {
-            const int j = r.bond_j[i * 3 + b];
-            if (j < 0) break;
+            const int v = r.bond[i * 3 + b];
+            if (v < 0) break;
+            const int j = v >> r.bond_shift;
             if (j < i) continue;
-            const int bt = r.bond_t[i * 3 + b];
+            const int bt = v & ((1 << r.bond_shift) - 1);
             double dr[3] = {0, 0, 0}, r2 = 0.0;
             for (int d = 0; d < D; ++d) {
                 float df = x[i * D + d] - x[j * D + d];

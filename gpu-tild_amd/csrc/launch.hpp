@@ -48,14 +48,12 @@ __host__ __device__ inline MidGeom mid_grad_geom(const RepDev& r) {
 }
 // one FFT buffer of N x nlines complex values, and the ping-pong pair a pass needs
 inline size_t fft_buf_bytes(int N, int nlines) { return (size_t)N * (size_t)(nlines | 1) * sizeof(float2); }
-// the ping-pong pair and the axis' twiddles
-inline size_t lds_bytes(int N, int nlines) { return 2 * fft_buf_bytes(N, nlines) + (size_t)tw_elems(N) * sizeof(float2); }
+inline size_t lds_bytes(int N, int nlines) { return 2 * fft_buf_bytes(N, nlines); }
 // fused last-axis pass: coupled fields (nf*nl lines, unpadded: only read and written
 // element by element) + two buffers of nf*DB*nl lines (DB inverse outputs per batch)
 // + the ramp weights of the step
 inline size_t last_lds_bytes(int N, int nf, int DB, int nl) {
-    return (size_t)N * nf * nl * sizeof(float2) + 2 * fft_buf_bytes(N, nf * DB * nl) + kMaxRamp * sizeof(float) +
-           (size_t)tw_elems(N) * sizeof(float2);
+    return (size_t)N * nf * nl * sizeof(float2) + 2 * fft_buf_bytes(N, nf * DB * nl) + kMaxRamp * sizeof(float);
 }
 
 struct LaunchCtx {
@@ -97,7 +95,7 @@ hipError_t launch_plane_r2c(const LaunchCtx& c);
 // length Nx or the Ny x Nxh half spectra
 inline size_t plane_r2c_lds(const RepDev& h) {
     const size_t a = (size_t)h.Nx * (size_t)((h.Ny / 2) | 1), b = (size_t)h.Ny * (size_t)(h.Nxh | 1);
-    return 2 * (a > b ? a : b) * sizeof(float2) + (size_t)(tw_elems(h.Nx) + tw_elems(h.Ny)) * sizeof(float2);
+    return 2 * (a > b ? a : b) * sizeof(float2);
 }
 // One-time kernel attributes (dynamic LDS above 64 KiB for the plane pass).
 hipError_t configure_kernels();

@@ -66,10 +66,10 @@ __device__ __forceinline__ void reorder_apply(const RepDev& r, int p, const int*
     r.ikind_alt[p] = r.ikind[i];
     r.pid_alt[p] = r.pid[i];
     if (r.q) r.q_alt[p] = r.q[i];
+    const int s = r.bond_shift, tmask = (1 << s) - 1;
     for (int b = 0; b < 3; ++b) {
-        const int j = r.bond_j[i * 3 + b];
-        r.bond_j_alt[p * 3 + b] = j < 0 ? -1 : inv[j];
-        r.bond_t_alt[p * 3 + b] = r.bond_t[i * 3 + b];
+        const int v = r.bond[i * 3 + b];
+        r.bond_alt[p * 3 + b] = v < 0 ? -1 : (inv[v >> s] << s) | (v & tmask);
     }
 }
 
@@ -96,8 +96,7 @@ __device__ __forceinline__ void reorder_swap(RepDev& r) {
     swap_ptr(r.ikind, r.ikind_alt);
     swap_ptr(r.pid, r.pid_alt);
     if (r.q) swap_ptr(r.q, r.q_alt);
-    swap_ptr(r.bond_j, r.bond_j_alt);
-    swap_ptr(r.bond_t, r.bond_t_alt);
+    swap_ptr(r.bond, r.bond_alt);
 }
 
 __global__ void k_reorder_swap(RepDev* reps, int rep) { reorder_swap(reps[rep]); }

@@ -20,6 +20,7 @@ Server::~Server() {
     cv_.notify_all();
     if (worker_.joinable()) worker_.join();
     batch_.reset();
+    for (hipEvent_t e : ev_) if (e) (void)hipEventDestroy(e);
 }
 
 int Server::start(std::string& err) {
@@ -27,6 +28,8 @@ int Server::start(std::string& err) {
     batch_ = std::make_unique<Batch>(opts_.device);
     batch_->set_spread(0);
     if (int rc = batch_->init(err)) return rc;
+    for (hipEvent_t& e : ev_)
+        if (hipEventCreate(&e) != hipSuccess) { err = "hipEventCreate failed"; return -TILD_ERR_DEVICE; }
     if (opts_.hbm_budget == 0) {
         size_t fr = 0, tot = 0;
         hipError_t e = hipMemGetInfo(&fr, &tot);
@@ -65,7 +68,9 @@ int Server::submit(const std::string& text, const std::string& base, uint32_t se
     return 0;
 }
 
-// Worker: admit, advance one chunk, retire; repeat.  Only this thread touches the Batch.
+// Worker: take the jobs that fit, enqueue one chunk of the running jobs, build the new
+// jobs' replicas while it runs, wait for it, retire; repeat.  Only this thread touches
+// the Batch.
 void Server::loop() {
     (void)hipSetDevice(opts_.device);
     std::string err;
@@ -90,30 +95,9 @@ void Server::loop() {
             bytes_in_use_ = used;   // reserved under the estimate until the replica exists
             admitting_ = (int)admit.size();   // neither queued nor running until added
         }
-        for (Job* J : admit) {
-            const double ts = now();
-            const int id = batch_->add(*J->sys, J->seed, err);
-            std::lock_guard<std::mutex> lk(mu_);
-            admit_s_ += now() - ts;
-            bytes_in_use_ -= J->bytes;
-            --admitting_;
-            if (id < 0) {
-                J->state = TILD_JOB_FAILED;
-                J->err = err;
-                J->t_done = now();
-                done_cv_.notify_all();
-                continue;
-            }
-            J->replica = id;
-            J->bytes = batch_->rep(id)->bytes;
-            bytes_in_use_ += J->bytes;
-            peak_bytes_ = std::max(peak_bytes_, bytes_in_use_);
-            J->t_start = ts;
-            J->state = TILD_JOB_RUNNING;
-            J->sys.reset();   // the replica holds everything the step needs
-            running_.push_back(J->id);
-        }
-        // --- one chunk of every running job, cut at the nearest finish
+        // --- one chunk of every running job, cut at the nearest finish: enqueued first,
+        // so that the admitted jobs' replicas are built (on the batch's set-up stream)
+        // while it runs
         std::vector<int> reps, ids;
         int chunk = opts_.chunk_steps;
         double beads = 0;
@@ -130,15 +114,43 @@ void Server::loop() {
                 beads += J->ns;
             }
         }
-        if (reps.empty()) continue;
         std::sort(reps.begin(), reps.end());
-        const double tc = now();
+        const bool stepping = !reps.empty() && chunk > 0;
         int rc = 0;
-        if (chunk > 0) {
+        if (stepping) {
+            (void)hipEventRecord(ev_[0], batch_->stream());
             rc = batch_->step(reps, chunk, err, opts_.use_graph != 0);
-            if (!rc) rc = batch_->sync(err);
+            (void)hipEventRecord(ev_[1], batch_->stream());
         }
-        const double dt = now() - tc;
+        for (Job* J : admit) {
+            const double ts = now();
+            std::string e1;
+            const int id = batch_->add(*J->sys, J->seed, e1);
+            std::lock_guard<std::mutex> lk(mu_);
+            admit_s_ += now() - ts;
+            bytes_in_use_ -= J->bytes;
+            --admitting_;
+            if (id < 0) {
+                J->state = TILD_JOB_FAILED;
+                J->err = e1;
+                J->t_done = now();
+                done_cv_.notify_all();
+                continue;
+            }
+            J->replica = id;
+            J->bytes = batch_->rep(id)->bytes;
+            bytes_in_use_ += J->bytes;
+            peak_bytes_ = std::max(peak_bytes_, bytes_in_use_);
+            J->t_start = ts;
+            J->state = TILD_JOB_RUNNING;
+            J->sys.reset();   // the replica holds everything the step needs
+            running_.push_back(J->id);   // joins from the next chunk on
+        }
+        if (!stepping) continue;
+        if (!rc) rc = batch_->sync(err);
+        float gpu_ms = 0.f;
+        if (!rc) (void)hipEventElapsedTime(&gpu_ms, ev_[0], ev_[1]);
+        const double dt = 1e-3 * gpu_ms;
         // --- retire the finished jobs
         std::vector<Job*> fin;
         {

@@ -68,6 +68,7 @@ private:
     int next_id_ = 0;
     size_t bytes_in_use_ = 0, peak_bytes_ = 0;
     double t0_ = 0, busy_s_ = 0, admit_s_ = 0, retire_s_ = 0;
+    hipEvent_t ev_[2] = {nullptr, nullptr};   // around each chunk's launches: its device time
     double bead_steps_ = 0;
     std::string fatal_;
 };

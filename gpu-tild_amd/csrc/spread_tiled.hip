@@ -201,6 +201,13 @@ __device__ __forceinline__ void deposit_record(const RepDev& r, const float4 rec
     for (int k = 1; k < NP; ++k) xx[k] = wrap_next(xx[k - 1], Nx);
     const int nz = D == 3 ? NP : 1;
     const bool charged = r.trecq != nullptr;
+    uint32_t qw[3][NP];   // order 1: axis weights in fixed point (fixed_weight2/3)
+    if constexpr (NP == 2) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int k = 0; k < NP; ++k) qw[d][k] = fixed_axis(W[d][k]);
+    }
     int zz = n0[2];
 #pragma unroll
     for (int iz = 0; iz < NP; ++iz, zz = wrap_next(zz, r.Nz)) {
@@ -217,7 +224,13 @@ __device__ __forceinline__ void deposit_record(const RepDev& r, const float4 rec
 #pragma unroll
             for (int ix = 0; ix < NP; ++ix) {
                 const float w = D == 3 ? W[0][ix] * W[1][iy] * W[2][iz] : W[0][ix] * W[1][iy];
-                atomicAdd(rowp + xx[ix], (unsigned long long)__float2ll_rn(w * r.fix_scale));
+                unsigned long long v;
+                if constexpr (NP == 2)
+                    v = D == 3 ? fixed_weight3(fixed_pair(qw[0][ix], qw[1][iy]), qw[2][iz], r.fix_s3)
+                               : fixed_weight2(qw[0][ix], qw[1][iy], r.fix_s2);
+                else
+                    v = (unsigned long long)__float2ll_rn(w * r.fix_scale);
+                atomicAdd(rowp + xx[ix], v);
                 if (charged) atomicAdd(rowq + xx[ix], (unsigned long long)__float2ll_rn((w * qi) * r.fixq_scale));
             }
         }
@@ -271,10 +284,6 @@ __global__ void __launch_bounds__(kDepositThreads) k_tile_deposit(const RepDev* 
     const int par = (r.step + koff) & 1;
     unsigned long long* patch = tile_lds;                         // [f][lz][ly][x]
     float2* fbuf = (float2*)(tile_lds + (size_t)rows * Nx);
-    const int ldf = ((rows + 1) / 2) | 1;
-    float2* twl = fbuf + 2 * (size_t)Nx * ldf;                    // the x axis' twiddles
-    TwStage<kDepositThreads> tws;
-    tws.issue(r.ax[0]);
     // the other parity's counters were last used by the previous step: clear this
     // tile's for the next one
     if (threadIdx.x < 4) G(claim_counters(r, par ^ 1))[(t * 4 + threadIdx.x) << r.tshift] = 0;
@@ -286,14 +295,10 @@ __global__ void __launch_bounds__(kDepositThreads) k_tile_deposit(const RepDev* 
     const int tym = (ty + r.nty - 1) % r.nty;
     const int tzm = (tz + r.ntz - 1) % r.ntz;
     const int own = tz * r.nty + ty, ny_ = tz * r.nty + tym, nz_ = tzm * r.nty + ty, nyz = tzm * r.nty + tym;
-    // source bucket s (s < nbk): 4 own + 2 (y) + 2 (z) + 1 (yz).  Every index below is
-    // a compile-time constant after unrolling, so the bucket list, region starts and
-    // shifts stay in (uniform) registers rather than in a private array.
-    const int nbk = D == 3 ? 9 : 6;
-    auto bucket = [&](int s) {
-        return s < 4 ? own * 4 + s
-                     : (s < 6 ? ny_ * 4 + (s - 3) : (s < 8 ? nz_ * 4 + (s - 4) : nyz * 4 + 2));
-    };
+    int bk[9], nbk = 0;   // 4 own + 2 (y) + 2 (z) + 1 (yz)
+    for (int c = 0; c < 4; ++c) bk[nbk++] = own * 4 + c;
+    bk[nbk++] = ny_ * 4 + 1; bk[nbk++] = ny_ * 4 + 2;
+    if (D == 3) { bk[nbk++] = nz_ * 4 + 2; bk[nbk++] = nz_ * 4 + 3; bk[nbk++] = nyz * 4 + 2; }
     gmem<const int> cnt = G((const int*)claim_counters(r, par));
     gmem<const int> base = G((const int*)r.tbase);
     gmem<const int> cap = G((const int*)r.tcap);
@@ -304,41 +309,19 @@ __global__ void __launch_bounds__(kDepositThreads) k_tile_deposit(const RepDev* 
     int start[9], shift[9], total = 0;
 #pragma unroll
     for (int s = 0; s < 9; ++s) {
-        const int bb = s < nbk ? bucket(s) : 0;
+        const int bb = s < nbk ? bk[s] : 0;
         const int n = s < nbk ? min(cnt[bb << r.tshift], cap[bb]) : 0;
-        start[s] = __builtin_amdgcn_readfirstlane(total);
-        shift[s] = __builtin_amdgcn_readfirstlane(base[bb] - total);
+        start[s] = total;
+        shift[s] = base[bb] - total;
         total += n;
     }
-    total = __builtin_amdgcn_readfirstlane(total);
-    // each record is requested one round ahead, so its load is in flight while the
-    // previous one is deposited
-    // (the shift as a sum of per-region steps: a select chain over the shifts is turned
-    // by the compiler into an indexed load from a private array)
-    int dsh[9];
-#pragma unroll
-    for (int s = 1; s < 9; ++s) dsh[s] = shift[s] - shift[s - 1];
-    auto slot = [&](int j) {
-        int sh = j + shift[0];
-#pragma unroll
-        for (int s = 1; s < 9; ++s) sh += j >= start[s] ? dsh[s] : 0;
-        return sh;
-    };
-    float4 nrec = make_float4(0.f, 0.f, 0.f, 0.f);
-    float nq = 0.f;
-    if ((int)threadIdx.x < total) {
-        const int p = slot(threadIdx.x);
-        nrec = gld(r.trec + p);
-        if (r.trecq) nq = G((const float*)r.trecq)[p];
-    }
     for (int j = threadIdx.x; j < total; j += kDepositThreads) {
-        const float4 rec = nrec;
-        const float qi = nq;
-        if (j + kDepositThreads < total) {
-            const int p = slot(j + kDepositThreads);
-            nrec = gld(r.trec + p);
-            if (r.trecq) nq = G((const float*)r.trecq)[p];
-        }
+        int sh = shift[0];
+#pragma unroll
+        for (int s = 1; s < 9; ++s) sh = j >= start[s] ? shift[s] : sh;
+        const int p = j + sh;
+        const float4 rec = gld(r.trec + p);
+        const float qi = r.trecq ? G((const float*)r.trecq)[p] : 0.f;
         if (P == 1) deposit_record<2>(r, rec, qi, patch, y0, z0, By, Bz);
         else deposit_record_any(r, rec, qi, patch, y0, z0, By, Bz);
     }
@@ -347,8 +330,7 @@ __global__ void __launch_bounds__(kDepositThreads) k_tile_deposit(const RepDev* 
     for (int j = threadIdx.x; j < novf; j += kDepositThreads) {
         const int bb = G((const int*)r.tovf_b)[j];
         bool mine = false;
-#pragma unroll
-        for (int s = 0; s < 9; ++s) mine |= s < nbk && bb == bucket(s);
+        for (int s = 0; s < nbk; ++s) mine |= bb == bk[s];
         if (!mine) continue;
         const float4 rec = gld(r.tovf_rec + j);
         const float qi = r.trecq ? G((const float*)r.tovf_q)[j] : 0.f;
@@ -383,9 +365,8 @@ __global__ void __launch_bounds__(kDepositThreads) k_tile_deposit(const RepDev* 
         if (!r.plane_fwd) fbuf[n * ldb + j] = make_float2(v[0], v[1]);
     }
     if (r.plane_fwd) return;   // the plane pass transforms the densities
-    tws.commit(r.ax[0], twl);
     __syncthreads();
-    const float2* res = lds_fft<kDepositThreads>(fbuf, fbuf + (size_t)Nx * ldb, ldb, npairs, r.ax[0], twl);
+    const float2* res = lds_fft<kDepositThreads>(fbuf, fbuf + (size_t)Nx * ldb, ldb, npairs, r.ax[0]);
     for (int task = threadIdx.x; task < npairs * Nxh; task += kDepositThreads) {
         int k;
         const int j = dNxh.div(task, k);
