@@ -1,6 +1,7 @@
 // fft_api.cpp -- tild_fft_r2c / tild_fft_c2r: the engine's Stockham passes on caller
 // device buffers (used to cross-check the transforms against rocFFT).
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -118,6 +119,85 @@ tild_status c2r_enqueue(const tild::RepDev& h, const C2rShape& sh, tild::RepDev*
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------- no-allocation inverse
+// Device scratch of tild_fft_c2r_noalloc: descriptor, twiddles, one identity work map
+// and the work copy of the spectra, at 256-byte aligned offsets.  Each device holds its
+// own copy for the life of the process; host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (the symbol's name is a host-side
+// stand-in, not a device address).  Calls that overlap on one device share it and race.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+__global__ void k_identity_map(tild::WorkItem* m, int n) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < n) m[b] = tild::WorkItem{0, b};
+}
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The current device's address of `scratch`, looked up once per device.
+tild_status scratch_address(unsigned char** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxScratchDevices)
+        return arg_error(TILD_ERR_DEVICE, "fft_c2r_noalloc: no current HIP device");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+            return arg_error(TILD_ERR_DEVICE, "fft_c2r_noalloc: hipGetSymbolAddress(scratch) failed");
+        g_scratch_ptr[dev] = (unsigned char*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return TILD_OK;
+}
+
+size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+
+}  // namespace
+
+extern "C" tild_status tild_fft_c2r_noalloc(int dim, const int* Nx, int batch, const float* d_in, float* d_out,
+                                            void* stream) {
+    using namespace tild;
+    if (batch < 1 || !d_in || !d_out) return arg_error(TILD_ERR_ARG, "fft_c2r_noalloc: bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    RepDev h;
+    float2 tw[3 * 2048];
+    int off[3] = {0, 0, 0};
+    if (tild_status st = plan_geom(dim, Nx, h, tw, off)) return st;
+    C2rShape sh;
+    if (tild_status st = c2r_shape(h, batch, sh)) return st;
+    const int ntw = off[dim - 1] + std::max(Nx[dim - 1] - 1, 1);
+    const int nmap = std::max(sh.nblk[0], std::max(sh.nblk[1], sh.nblk[2]));
+    // layout: descriptor | twiddles | identity map | work copy
+    const size_t o_tw = align256(sizeof(RepDev));
+    const size_t o_map = o_tw + align256((size_t)ntw * sizeof(float2));
+    const size_t o_work = o_map + align256((size_t)nmap * sizeof(WorkItem));
+    const size_t need = o_work + sh.work * sizeof(float2);
+    if (need > kNoAllocScratchBytes)
+        return arg_error(TILD_ERR_ARG, "fft_c2r_noalloc: transform needs " + std::to_string(need) +
+                                           " bytes of scratch, more than the " +
+                                           std::to_string(kNoAllocScratchBytes) + " available; use tild_fft_c2r");
+    unsigned char* base;
+    if (tild_status st = scratch_address(&base)) return st;
+    RepDev* d_h = (RepDev*)base;
+    float2* d_tw = (float2*)(base + o_tw);
+    WorkItem* map = (WorkItem*)(base + o_map);
+    float2* work = (float2*)(base + o_work);
+    if (tild_status st = upload_geom(h, tw, off, ntw, d_h, d_tw, s)) return st;
+    hipLaunchKernelGGL(k_identity_map, dim3((nmap + 255) / 256), dim3(256), 0, s, map, nmap);
+    TRY(hipGetLastError());
+    TRY(hipMemcpyAsync(work, d_in, sh.work * sizeof(float2), hipMemcpyDeviceToDevice, s));
+    WorkItem* maps[3] = {map, map, map};
+    if (tild_status st = c2r_enqueue(h, sh, d_h, maps, batch, work, d_out, s)) return st;
+    // the host arrays staged above live on this frame: wait before returning
+    TRY(hipStreamSynchronize(s));
+    return TILD_OK;
+}
 
 extern "C" tild_status tild_fft_r2c(int dim, const int* Nx, int batch, const float* d_in, float* d_out, void* stream) {
     using namespace tild;

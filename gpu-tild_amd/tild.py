@@ -26,7 +26,7 @@ EXPORTS = [
     "tild_engine_get_density", "tild_engine_get_grid_field", "tild_engine_get_step",
     "tild_engine_get_state", "tild_engine_set_state",
     "tild_engine_synchronize", "tild_engine_get_bins", "tild_engine_profile", "tild_engine_nlist", "tild_fft_r2c", "tild_fft_c2r",
-    "tild_system_compute", "tild_engine_sk_accumulate", "tild_engine_sk_average",
+    "tild_fft_c2r_noalloc", "tild_system_compute", "tild_engine_sk_accumulate", "tild_engine_sk_average",
     "tild_engine_geometry", "tild_batch_geometry",
     "tild_batch_create", "tild_batch_destroy", "tild_batch_last_error", "tild_batch_add", "tild_batch_remove",
     "tild_batch_step", "tild_batch_properties", "tild_batch_get_positions", "tild_batch_set_positions",
@@ -158,6 +158,7 @@ def lib():
         L.tild_server_stats_get.argtypes = [vp, ctypes.POINTER(ServerStats)]
         L.tild_fft_r2c.argtypes = [ctypes.c_int, ip, ctypes.c_int, vp, vp, vp]
         L.tild_fft_c2r.argtypes = [ctypes.c_int, ip, ctypes.c_int, vp, vp, vp]
+        L.tild_fft_c2r_noalloc.argtypes = [ctypes.c_int, ip, ctypes.c_int, vp, vp, vp]
         L.tild_system_compute.argtypes = [vp, ctypes.c_int, ip, ip, ip]
         L.tild_engine_sk_accumulate.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.tild_engine_sk_average.argtypes = [vp, ctypes.c_int, fp, sz, ip]
@@ -556,3 +557,9 @@ def fft_c2r(dim, Nx, batch, d_in, d_out, stream=None):
     _check(lib().tild_fft_c2r(dim, arr, batch, ctypes.c_void_p(d_in), ctypes.c_void_p(d_out),
                               ctypes.c_void_p(stream) if stream else None), "tild_fft_c2r")
 
+
+def fft_c2r_noalloc(dim, Nx, batch, d_in, d_out, stream=None):
+    """fft_c2r on the library's fixed device scratch: no device allocation."""
+    arr = (ctypes.c_int * 3)(*(list(Nx) + [1] * (3 - len(Nx))))
+    _check(lib().tild_fft_c2r_noalloc(dim, arr, batch, ctypes.c_void_p(d_in), ctypes.c_void_p(d_out),
+                                      ctypes.c_void_p(stream) if stream else None), "tild_fft_c2r_noalloc")

@@ -308,6 +308,15 @@ tild_status tild_fft_r2c(int dim, const int* Nx, int batch, const float* d_in,
                          float* d_out, void* stream);
 tild_status tild_fft_c2r(int dim, const int* Nx, int batch, const float* d_in,
                          float* d_out, void* stream);
+/* tild_fft_c2r without any device allocation, for callers that cannot allocate (code
+   under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+   counterpart in the reference.  Its scratch (descriptor, twiddles, work map and the
+   work copy of the batch*(Nx[0]/2+1)*Nx[1]*Nx[2] input spectra) is a fixed 32 MiB
+   device array per GPU, held until the process ends; transforms that need more return
+   TILD_ERR_ARG.  Calls that overlap on one GPU share that array and race.  Same
+   results as tild_fft_c2r, bit for bit. */
+tild_status tild_fft_c2r_noalloc(int dim, const int* Nx, int batch, const float* d_in,
+                                 float* d_out, void* stream);
 
 #ifdef __cplusplus
 }
