@@ -1031,13 +1031,15 @@ int Batch::step(const std::vector<int>& ids_in, int n, std::string& err, bool us
     HIPCHK(hipSetDevice(device_));
     SetMaps* m = maps_for(ids, err);
     if (!m) return -TILD_ERR_DEVICE;
-    if (use_graph && !m->graph) {
+    if (use_graph && n >= kGraphSteps && !m->graph_run) {
         // Every kernel argument is constant across steps: step counters and position
-        // parity live in the device descriptors and are advanced by each step's last
-        // node, so one captured step serves every step, and kGraphSteps captured steps
-        // serve every run of that many (one graph launch instead of kGraphSteps; each
-        // launch leaves a gap of several microseconds on the device).
-        if (int rc = capture_steps(*m, 1, &m->graph, err)) return rc;
+        // parity live in the device descriptors and are advanced by the run's last
+        // node, so kGraphSteps captured steps serve every run of that many.  Each graph
+        // launch leaves a gap of about 40 us on the device (32 us before the next run
+        // and 9 us before its counter update, on the 640,000-bead melt); eagerly
+        // launched steps leave none once the kernels outlast the launches, so runs
+        // shorter than kGraphSteps are launched eagerly (the melt: 10-step graphs took
+        // 103.2 us per step, 60-step graphs 102.0, single-step graphs 108.1).
         if (int rc = capture_steps(*m, kGraphSteps, &m->graph_run, err)) return rc;
     }
     // chunks end where a replica of the set is due for its spatial reorder; each
@@ -1087,17 +1089,14 @@ int Batch::capture_steps(SetMaps& m, int nsteps, hipGraphExec_t* out, std::strin
 }
 
 void Batch::drop_graphs(SetMaps& m) {
-    if (m.graph) { (void)hipGraphExecDestroy(m.graph); m.graph = nullptr; }
     if (m.graph_run) { (void)hipGraphExecDestroy(m.graph_run); m.graph_run = nullptr; }
 }
 
 int Batch::run_steps(SetMaps& m, const std::vector<int>& ids, int n, bool use_graph, std::string& err) {
     int s = 0;
-    if (use_graph)
+    if (use_graph && m.graph_run)
         for (; s + kGraphSteps <= n; s += kGraphSteps) HIPCHK(hipGraphLaunch(m.graph_run, stream_));
-    if (use_graph)
-        for (; s < n; ++s) HIPCHK(hipGraphLaunch(m.graph, stream_));
-    else if (n > s)
+    if (n > s)
         if (int rc = enqueue_steps(m, n - s, true, false, err)) return rc;
     for (int id : ids) reps_[id]->step += n;
     return 0;

@@ -20,7 +20,7 @@
 namespace tild {
 
 constexpr int kXcds = 8;         // accelerator complex dies of an MI355X
-constexpr int kGraphSteps = 10;  // integrating steps per multi-step graph launch
+constexpr int kGraphSteps = 60;  // integrating steps per graph launch: one spatial-reorder interval
 constexpr int kProfPhases = 9;   // claim, deposit, forward plane, last_fused, y_inv, x_c2r, particles, advance, reorder
 
 // Work-map phases.  *A: replicas on the global-atomic spread; T*: replicas on the
@@ -165,8 +165,7 @@ public:
         PhaseMap maps[PH_COUNT];
         int* d_ids = nullptr;
         int nids = 0;
-        hipGraphExec_t graph = nullptr;   // one captured integrating step of this set
-        hipGraphExec_t graph_run = nullptr;   // kGraphSteps captured steps
+        hipGraphExec_t graph_run = nullptr;   // kGraphSteps captured integrating steps
     };
     SetMaps* maps_for(const std::vector<int>& ids, std::string& err);
 

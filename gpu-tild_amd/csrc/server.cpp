@@ -52,6 +52,7 @@ int Server::submit(const std::string& text, const std::string& base, uint32_t se
     J->nsteps = nsteps;
     J->ns = J->sys->ns;
     J->dim = J->sys->Dim;
+    J->step_work = (double)J->sys->ns + (double)J->sys->M;
     std::lock_guard<std::mutex> lk(mu_);
     if (!fatal_.empty()) { err = fatal_; return -TILD_ERR_DEVICE; }
     if (J->bytes > opts_.hbm_budget) {
@@ -103,14 +104,24 @@ void Server::loop() {
         double beads = 0;
         {
             std::lock_guard<std::mutex> lk(mu_);
-            for (int jid : running_) {
-                Job* J = jobs_[jid].get();
-                chunk = std::min(chunk, J->nsteps - J->done_steps);
+            std::vector<Job*> run;
+            for (int jid : running_) run.push_back(jobs_[jid].get());
+            if (opts_.step_work > 0) {
+                // least remaining work first, until the chunk's work per step reaches the target
+                std::sort(run.begin(), run.end(), [](const Job* a, const Job* b) {
+                    const double wa = (a->nsteps - a->done_steps) * a->step_work;
+                    const double wb = (b->nsteps - b->done_steps) * b->step_work;
+                    return wa != wb ? wa < wb : a->id < b->id;
+                });
+                double w = 0;
+                size_t k = 0;
+                while (k < run.size() && (k == 0 || w < 1000.0 * opts_.step_work)) w += run[k++]->step_work;
+                run.resize(k);
             }
-            for (int jid : running_) {
-                Job* J = jobs_[jid].get();
+            for (Job* J : run) {
+                chunk = std::min(chunk, J->nsteps - J->done_steps);
                 reps.push_back(J->replica);
-                ids.push_back(jid);
+                ids.push_back(J->id);
                 beads += J->ns;
             }
         }

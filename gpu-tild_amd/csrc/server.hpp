@@ -36,6 +36,7 @@ struct Job {
     std::vector<float> x;
     std::string err;
     int ns = 0, dim = 3;
+    double step_work = 0;   // beads + grid nodes: the work of one step
 };
 
 class Server {

@@ -80,7 +80,8 @@ _lib = None
 
 class ServerOpts(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("chunk_steps", ctypes.c_int), ("max_replicas", ctypes.c_int),
-                ("use_graph", ctypes.c_int), ("hbm_budget", ctypes.c_size_t), ("reserved", ctypes.c_int * 8)]
+                ("use_graph", ctypes.c_int), ("hbm_budget", ctypes.c_size_t), ("step_work", ctypes.c_int),
+                ("reserved", ctypes.c_int * 7)]
 
 
 class ServerStats(ctypes.Structure):
@@ -492,10 +493,10 @@ class Server:
     thread inside the library admits, steps and retires jobs.  The ctypes calls
     release the GIL, so wait()/drain() block only the calling thread."""
 
-    def __init__(self, device=0, chunk_steps=100, max_replicas=0, use_graph=True, hbm_budget=0):
+    def __init__(self, device=0, chunk_steps=100, max_replicas=0, use_graph=True, hbm_budget=0, step_work=0):
         o = ServerOpts()
         o.device, o.chunk_steps, o.max_replicas = device, chunk_steps, max_replicas
-        o.use_graph, o.hbm_budget = 1 if use_graph else 0, int(hbm_budget)
+        o.use_graph, o.hbm_budget, o.step_work = 1 if use_graph else 0, int(hbm_budget), int(step_work)
         h = ctypes.c_void_p()
         _check(lib().tild_server_create(ctypes.byref(o), ctypes.byref(h)), "tild_server_create")
         self._h = h

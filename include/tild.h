@@ -265,7 +265,13 @@ typedef struct {
     int max_replicas;     /* running-job cap; 0 = budget only */
     int use_graph;        /* replay one captured step per running set */
     size_t hbm_budget;    /* bytes of device memory for jobs; 0 = 90% of free memory */
-    int reserved[8];
+    int step_work;        /* 0: every running job advances in every chunk.  > 0: each chunk
+                             advances the running jobs with the least remaining work
+                             (steps left x (beads + grid nodes)), taken in that order until
+                             their work per step reaches step_work thousand beads + nodes;
+                             the rest stay resident and wait.  Short jobs then finish
+                             sooner while the device stays as busy. */
+    int reserved[7];
 } tild_server_opts;
 
 enum { TILD_JOB_QUEUED = 0, TILD_JOB_RUNNING = 1, TILD_JOB_DONE = 2, TILD_JOB_FAILED = 3 };

@@ -26,15 +26,15 @@ def test_batch_equals_alone(graph, tmp_path):
     alone = []
     for i, (s, _) in enumerate(systems):
         e = tild.Engine(s, device=0, seed=100 + i)
-        e.step(7)
+        e.step(67)
         alone.append((e.positions(), e.density(0), e.properties()))
         e.close()
     b = tild.Batch(device=0)
     ids = [b.add(s, 100 + i) for i, (s, _) in enumerate(systems)]
     b.step(3, graph=graph)
-    b.step(4, graph=graph)
+    b.step(64, graph=graph)   # with graph: one replay of the 60-step graph, then 4 eager steps
     for i, r in enumerate(ids):
-        assert b.replica_step(r) == 7
+        assert b.replica_step(r) == 67
         assert np.array_equal(b.positions(r), alone[i][0]), i
         assert np.array_equal(b.density(r, 0), alone[i][1]), i
         p = b.properties(r)
@@ -49,8 +49,8 @@ def test_graph_replays_match_eager_with_interleaved_work(tmp_path):
     gb = tild.Batch(device=0)
     rids = [ref.add(s, 7 + i) for i, (s, _) in enumerate(systems)]
     gids = [gb.add(s, 7 + i) for i, (s, _) in enumerate(systems)]
-    # runs of 10 or more steps replay the 10-step graph, the rest the one-step graph
-    for rep, n in enumerate([2, 23, 2, 11]):
+    # runs of 60 or more steps replay the 60-step graph, the rest run eagerly
+    for rep, n in enumerate([2, 130, 2, 61]):
         ref.step(n, graph=False)
         gb.step(n, graph=True)
         # eager work and synchronisation between replays

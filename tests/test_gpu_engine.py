@@ -396,7 +396,7 @@ def test_ramped_styles_match_oracle(tmp_path):
     # Gaussian, erf and gaussian-erf prefactors ramped by the reference's update
     # (pair_style.cu:308-328): the engine weights each ramped style's table row per
     # step on the device, so runs of steps replayed from captured graphs (offsets
-    # 0..9 in a run) see the same schedule as single steps.  Trajectory and style
+    # 0..59 in a run) see the same schedule as single steps.  Trajectory and style
     # energies after each run follow the oracle, which rescales its tables after every
     # step as the reference does.
     styles = [("gaussian", 1, 1, 1.0, 0.5), ("gaussian", 2, 2, 1.0, 0.5), ("gaussian", 1, 2, 2.0, 0.5),
@@ -405,7 +405,7 @@ def test_ramped_styles_match_oracle(tmp_path):
     for old, new in [("gaussian 1 2 2.0 0.5", "gaussian 1 2 2.0 0.5 ramp 4.0"),
                      ("erf 1 2 0.1 1.0 0.5", "erf 1 2 0.1 1.0 0.5 ramp 0.3"),
                      ("gaussian_erf 2 2 0.5 0.6 0.5", "gaussian_erf 2 2 0.5 0.6 0.5 ramp 0.2"),
-                     ("max_steps 1000", "max_steps 40")]:
+                     ("max_steps 1000", "max_steps 200")]:
         assert old in text, old
         text = text.replace(old, new)
     s, e, o = _pair(text, d, seed=13)
@@ -422,6 +422,17 @@ def test_ramped_styles_match_oracle(tmp_path):
         np.testing.assert_allclose(p["U_styles"], po["U_styles"], rtol=2e-5, atol=1e-9)
         assert abs(p["Upe"] - po["Upe"]) <= 2e-5 * abs(po["Upe"])
     e.close()
+    # a 60-step graph replay (offsets 0..59) equals the same steps launched one by one
+    g = tild.Engine(s, device=0, seed=13)
+    b = tild.Batch(device=0)
+    r = b.add(s, 13)
+    g.step(61)                       # 60 replayed + 1 eager
+    for _ in range(61):
+        b.step(1, graph=False)
+    assert np.array_equal(g.positions(), b.positions(r))
+    np.testing.assert_array_equal(g.properties()["U_styles"], b.properties(r)["U_styles"])
+    g.close()
+    b.close()
 
 
 @pytest.mark.parametrize("case", ["gjf", "em_2d"])

@@ -33,11 +33,14 @@ def _alone(jobs):
     return out
 
 
-@pytest.mark.parametrize("graph", [True, False])
-def test_jobs_same_alone_and_in_the_mix(graph, tmp_path):
+@pytest.mark.parametrize("graph,step_work", [(True, 0), (False, 0), (True, 3)],
+                         ids=["graph", "eager", "least-work-first"])
+def test_jobs_same_alone_and_in_the_mix(graph, step_work, tmp_path):
+    # step_work 3: each chunk advances only the jobs with the least remaining work, about
+    # 3,000 beads + grid nodes per step (one or two of these jobs)
     jobs = _jobs(tmp_path)
     ref = _alone(jobs)
-    srv = tild.Server(device=0, chunk_steps=4, max_replicas=3, use_graph=graph)
+    srv = tild.Server(device=0, chunk_steps=4, max_replicas=3, use_graph=graph, step_work=step_work)
     ids = [srv.submit(t, d, seed, n, ns=r[2], dim=r[3]) for (t, d, seed, n), r in zip(jobs, ref)]
     srv.drain(timeout=120)
     for j, (x0, p0, _, _) in zip(ids, ref):
@@ -51,7 +54,8 @@ def test_jobs_same_alone_and_in_the_mix(graph, tmp_path):
     assert st["bytes_in_use"] == 0
     assert 0 < st["p50_s"] <= st["p99_s"]
     assert st["bead_steps"] == sum(r[2] * n for r, (_, _, _, n) in zip(ref, jobs))
-    print(f"server (graph={graph}): {len(jobs)} jobs bit-identical to runs alone; p50 {st['p50_s']:.3f} s, "
+    print(f"server (graph={graph}, step_work={step_work}): {len(jobs)} jobs bit-identical to runs alone; "
+          f"p50 {st['p50_s']:.3f} s, "
           f"p99 {st['p99_s']:.3f} s")
     srv.close()
 
