@@ -103,7 +103,7 @@ int Batch::add_with(const System& S, uint32_t seed, const Geometry& g, std::stri
     const FftTuning ft = tune_;
     tile_target_ = g.tile_target;
     dep_threads_ = g.dep_threads;
-    tune_ = FftTuning{g.mid_elems, g.last_lines, g.c2r_rows, g.plane, g.last_dirs};
+    tune_ = FftTuning{g.mid_elems, g.last_lines, g.c2r_rows, g.plane, g.last_dirs, 0};
     const int id = add_plain(S, seed, err);
     tile_target_ = tt;
     dep_threads_ = dt;
@@ -168,6 +168,16 @@ int Batch::tune_geometry(const System& S, uint32_t seed, TuneResult& out, std::s
         return -TILD_ERR_DEVICE;
     }
     const bool tiled_ok = spread_mode_ != 1;
+    // the register-resident fused pass (always taken where the deck's last axis and types
+    // allow it) has no batching of inverse directions
+    bool reg_ok = false;
+    {
+        const int la = S.Dim - 1, nf = S.ntypes + (S.charges.enabled ? 1 : 0);
+        AxisPlan lp;
+        std::vector<float2> tw((size_t)std::max(S.Nx[la], 1));
+        int ntw = 0;
+        reg_ok = plan_axis(S.Nx[la], lp, tw.data(), &ntw) == nullptr && last_reg_plan(lp, S.ntypes, nf);
+    }
     struct Axis { int Geometry::*field; std::vector<int> cand; bool on; };
     const std::vector<Axis> axes = {
         {&Geometry::plane, {0, -1}, S.Dim == 3 && tiled_ok},
@@ -180,6 +190,7 @@ int Batch::tune_geometry(const System& S, uint32_t seed, TuneResult& out, std::s
     };
     for (const Axis& a : axes) {
         if (!a.on) continue;
+        if (a.field == &Geometry::last_dirs && reg_ok) continue;
         for (int c : a.cand) {
             if (best.*a.field == c) continue;
             Geometry g = best;

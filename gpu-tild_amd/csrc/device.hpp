@@ -43,6 +43,7 @@ struct RepDev {
     int fix_s2, fix_s3;        // order-1 integer weights: 44 - F and 52 - F (fixed_weight2/3)
     // FFT plans (x, y, z) and chunk sizes of each pass
     AxisPlan ax[3];
+    AxisPlan axr;  // inverse plan of the last axis: a two-stage plan reversed, else ax[dim - 1]
     int nl_x;      // line pairs per block, forward x pass
     int nl_mid;    // batch columns per block, middle (y) pass
     int nl_last;   // batch columns per block, fused last-axis pass
@@ -50,6 +51,7 @@ struct RepDev {
     int mid_elems; // complex values per block of the middle-axis passes (0: kFftBudget)
     int plane_fwd; // 1: the tiled deposit writes rho only; x R2C + y FFT run per plane
     int last_dirs; // directions per inverse batch of the fused last-axis pass (1..dim)
+    int last_reg;  // 1: the fused last-axis pass runs register-resident (k_last_reg)
     // particle state
     float* X[2];            // ns*dim, double-buffered by step parity
     float* XO;              // ns*dim, GJF previous position
@@ -169,6 +171,13 @@ __device__ __forceinline__ void gst3(float* p, float a, float b, float c) {
     q[0] = a; q[1] = b; q[2] = c;
 }
 __device__ __forceinline__ void gst(float4* p, float4 a) { *(gmem<f4v>)p = f4v{a.x, a.y, a.z, a.w}; }
+
+// Weight of ramped style j's coupling row at 0-based step k (pair_style.cu:308-328, see
+// RepDev::ramp_*): A_j(k) - A_j(0).
+__device__ __forceinline__ float ramp_weight(const RepDev& r, int j, int k) {
+    const double A = k == 0 ? r.ramp_A0[j] : r.ramp_c0[j] + r.ramp_c1[j] * (double)(k + 1);
+    return (float)(A - r.ramp_A0[j]);
+}
 
 // ---------------------------------------------------------------- Philox / normals
 // Philox4x32-10 (Salmon et al., SC'11).

@@ -58,10 +58,17 @@ const char* plan_axis(int N, AxisPlan& pl, float2* tw, int* ntw) {
     else rs.nbest = 0;
     if (rs.nbest < 0) return "FFT length has a prime factor above 7";
     if (rs.nbest > kMaxStages) return "too many FFT stages";
-    pl.nst = rs.nbest;
+    fill_plan(N, rs.best, rs.nbest, pl, tw, ntw);
+    return nullptr;
+}
+
+void fill_plan(int N, const int* rad, int nst, AxisPlan& pl, float2* tw, int* ntw) {
+    std::memset(&pl, 0, sizeof(pl));
+    pl.N = N;
+    pl.nst = nst;
     int p = 1, n = 0;
     for (int s = 0; s < pl.nst; ++s) {
-        const int R = rs.best[s];
+        const int R = rad[s];
         pl.rad[s] = R;
         pl.tw_off[s] = n;
         for (int k = 0; k < p; ++k)
@@ -74,7 +81,6 @@ const char* plan_axis(int N, AxisPlan& pl, float2* tw, int* ntw) {
     if (n == 0) tw[n++] = make_float2(1.f, 0.f);
     pl.ntw = n;
     *ntw = n;
-    return nullptr;
 }
 
 int make_axis_plan(int N, AxisPlan& pl, std::vector<float2>& tw, std::string& err) {
@@ -172,6 +178,7 @@ int Batch::init(std::string& err) {
     HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&setup_stream_, hipStreamNonBlocking));
     HIPCHK(configure_kernels());
+    HIPCHK(configure_last_reg());
     return 0;
 }
 
@@ -250,10 +257,25 @@ int Batch::add_plain(const System& S, uint32_t seed, std::string& err) {
         toff[a] = (int)twall.size();
         twall.insert(twall.end(), tw.begin(), tw.end());
     }
+    // inverse plan of the last axis: a two-stage plan R1 x R2 runs reversed (R2 x R1),
+    // whose first stage reads the spectrum values k = i + R1 q that one column of the
+    // forward transform ends with (the register-resident fused pass keeps them)
+    const int lax = D - 1;
+    int toff_r = toff[lax];
+    h.axr = h.ax[lax];
+    if (h.ax[lax].nst == 2) {
+        const int rr[2] = {h.ax[lax].rad[1], h.ax[lax].rad[0]};
+        std::vector<float2> tw((size_t)h.ax[lax].N);
+        int ntw = 0;
+        fill_plan(h.ax[lax].N, rr, 2, h.axr, tw.data(), &ntw);
+        toff_r = (int)twall.size();
+        twall.insert(twall.end(), tw.begin(), tw.begin() + ntw);
+    }
     float2* d_tw;
     if (int rc = dalloc(R, &d_tw, twall.size(), err)) return rc;
     HIPCHK(hipMemcpy(d_tw, twall.data(), twall.size() * sizeof(float2), hipMemcpyHostToDevice));
     for (int a = 0; a < D; ++a) h.ax[a].tw = d_tw + toff[a];
+    h.axr.tw = d_tw + toff_r;
     if (D == 2) h.ax[2] = h.ax[1];
 
     // chunking of each pass
@@ -276,6 +298,14 @@ int Batch::add_plain(const System& S, uint32_t seed, std::string& err) {
     if (tune_.last_lines > 0) {
         h.nl_last = std::min(tune_.last_lines, Q);
         while (h.nl_last > 1 && last_lds_bytes(Nla, h.nf, h.last_dirs, h.nl_last) > kPlaneLds) --h.nl_last;
+    }
+    // the register-resident form of the fused pass where it applies (bit-identical
+    // results): as many columns per block as its threads hold, or fewer when asked
+    h.last_reg = !tune_.last_staged && last_reg_plan(h.ax[la], h.ntypes, h.nf) ? 1 : 0;
+    if (h.last_reg) {
+        h.nl_last = std::min(last_reg_max_lines(h.ax[la]), Q);
+        if (tune_.last_lines > 0) h.nl_last = std::min(h.nl_last, tune_.last_lines);
+        while (h.nl_last > 1 && last_reg_lds_bytes(Nla, h.nf, h.nl_last) > kPlaneLds) --h.nl_last;
     }
     if (tune_.c2r_rows > 0) {
         h.nr_c2r = std::min(tune_.c2r_rows, rows);
@@ -651,7 +681,8 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
             add(PH_MIDZ_H, gz.nouter * gz.nchunk, lds_bytes(gz.N, gz.nl));
         }
         const int la = D - 1, Nla = la == 2 ? h.Nz : h.Ny, Q = la == 2 ? h.Ny * h.Nxh : h.Nxh;
-        add(PH_LAST, (Q + h.nl_last - 1) / h.nl_last, last_lds_bytes(Nla, h.nf, h.last_dirs, h.nl_last));
+        if (h.last_reg) add(PH_LASTR, (Q + h.nl_last - 1) / h.nl_last, last_reg_lds_bytes(Nla, h.nf, h.nl_last));
+        else add(PH_LAST, (Q + h.nl_last - 1) / h.nl_last, last_lds_bytes(Nla, h.nf, h.last_dirs, h.nl_last));
         const int nc = (h.nf * D + 1) / 2;
         add(PH_C2R, (rows + h.nr_c2r - 1) / h.nr_c2r, lds_bytes(h.Nx, h.nr_c2r * nc));
         add(PH_KPROPS, h.nchunk > 0 ? h.nchunk : 1, 0);
@@ -676,7 +707,7 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
     // records; for the FFT passes adjacent column batches, whose rows share cache lines
     // that a single L2 then fetches once.
     for (Phase ph : {PH_PART, PH_TCOUNT, PH_SPREADA, PH_TILE, PH_XA, PH_XP, PH_MIDY_H, PH_MIDY_G,
-                     PH_MIDZ_H, PH_PY, PH_LAST, PH_C2R, PH_PLANEF}) {
+                     PH_MIDZ_H, PH_PY, PH_LAST, PH_LASTR, PH_C2R, PH_PLANEF}) {
         std::vector<WorkItem>& v = maps[ph].host;
         const size_t n = v.size(), q = n / kXcds, rm = n % kXcds;
         if (n < 2 * (size_t)kXcds) continue;
@@ -689,7 +720,7 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
     }
     for (int p = 0; p < PH_COUNT; ++p) {
         maps[p].n = (int)maps[p].host.size();
-        if (maps[p].lds > ((p == PH_PLANEF || p == PH_LAST) ? kPlaneLds : (size_t)kMaxLds)) { err = "pass needs more LDS than a block may use"; return -TILD_ERR_UNSUPPORTED; }
+        if (maps[p].lds > ((p == PH_PLANEF || p == PH_LAST || p == PH_LASTR) ? kPlaneLds : (size_t)kMaxLds)) { err = "pass needs more LDS than a block may use"; return -TILD_ERR_UNSUPPORTED; }
         if (maps[p].dev && (!reorder_only || maps[p].cap < maps[p].n)) {
             // reused buffers are last read by the previous reorder's launches
             HIPCHK(hipStreamSynchronize(stream_));
@@ -1003,6 +1034,7 @@ int Batch::enqueue_steps(SetMaps& m, int nsteps, bool integrate, bool write_forc
         HIPCHK(launch_plane_r2c(ctx(PH_PLANEF)));
         HIPCHK(launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr));
         HIPCHK(launch_last_fused(ctx(PH_LAST), 0, k));
+        HIPCHK(launch_last_reg(ctx(PH_LASTR), k));
         HIPCHK(launch_mid_grad(ctx(PH_MIDY_G)));
         HIPCHK(launch_x_c2r(ctx(PH_C2R), true, 0, nullptr, nullptr));
         // integrating steps claim the next step's records; a force evaluation leaves
@@ -1134,6 +1166,7 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
         ok(launch_plane_r2c(ctx(PH_PLANEF, 2, 0)));
         ok(launch_mid(ctx(PH_MIDY_H, 2, 1), 1, 0, 0, 0, nullptr));
         ok(launch_last_fused(ctx(PH_LAST, 3, 0), 0, 0));
+        ok(launch_last_reg(ctx(PH_LASTR, 3, 1), 0));
         ok(launch_mid_grad(ctx(PH_MIDY_G, 4, 0)));
         ok(launch_x_c2r(ctx(PH_C2R, 5, 0), true, 0, nullptr, nullptr));
         ok(launch_particles(ctx(PH_PART, 6, 0), true, 0, 0, true));

@@ -31,7 +31,7 @@ constexpr int kProfPhases = 9;   // claim, deposit, forward plane, last_fused, y
 // skip MIDY_G and C2R); PLANEF: x R2C + y FFT of tiled 3D replicas with even Ny
 // (their deposit writes densities only and they skip MIDY_H).
 enum Phase { PH_PART = 0, PH_SPREADA, PH_XA, PH_XP, PH_SCAN, PH_TILE, PH_MIDY_H, PH_MIDY_G,
-             PH_MIDZ_H, PH_PY, PH_LAST, PH_C2R, PH_PLANEF, PH_TCOUNT, PH_KPROPS, PH_RTILE, PH_COUNT };
+             PH_MIDZ_H, PH_PY, PH_LAST, PH_C2R, PH_PLANEF, PH_TCOUNT, PH_KPROPS, PH_RTILE, PH_LASTR, PH_COUNT };
 
 struct PhaseMap {
     std::vector<WorkItem> host;
@@ -109,7 +109,7 @@ public:
     void set_reorder(int every) { reorder_every_ = every; }
     // FFT block geometry of replicas added from now on (0 = default): complex values
     // per middle-axis block, lines per fused last-axis block, rows per inverse-x block
-    struct FftTuning { int mid_elems = 0, last_lines = 0, c2r_rows = 0, plane = 0, last_dirs = 0; };
+    struct FftTuning { int mid_elems = 0, last_lines = 0, c2r_rows = 0, plane = 0, last_dirs = 0, last_staged = 0; };
     void set_fft_tuning(const FftTuning& t) { tune_ = t; }
     // Deposit block size of replicas added from now on (0: by bead density, else 256 or 512).
     void set_deposit_threads(int n) { dep_threads_ = n; }
@@ -230,6 +230,8 @@ int make_axis_plan(int N, AxisPlan& pl, std::vector<float2>& tw, std::string& er
 // The same without allocating: tw has room for max(N, 1) values, *ntw receives the
 // count (N - 1, or 1 for N = 1).  Returns nullptr, or why N has no plan.
 const char* plan_axis(int N, AxisPlan& pl, float2* tw, int* ntw);
+// plan of the given radices (product N) and its twiddles (at most N entries)
+void fill_plan(int N, const int* rad, int nst, AxisPlan& pl, float2* tw, int* ntw);
 // Message of the last failed C ABI call on this thread (tild_last_error).
 void set_last_error(const std::string& msg);
 

@@ -243,9 +243,11 @@ __device__ __forceinline__ float2* lds_fft2(float2* A, float2* B, int ldb, int n
 // NT = threads of the block.
 template <int NT = kFftThreads>
 __device__ __forceinline__ float2* lds_fft(float2* A, float2* B, int ldb, int nb, const AxisPlan& pl) {
-    if (pl.nst == 2) {   // the plans of 80 and 160 points (uniform branch)
+    if (pl.nst == 2) {   // the plans of 80 and 160 points and their reverses (uniform branch)
         if (pl.rad[0] == 10 && pl.rad[1] == 8) return lds_fft2<10, 8, NT>(A, B, ldb, nb, pl);
         if (pl.rad[0] == 16 && pl.rad[1] == 10) return lds_fft2<16, 10, NT>(A, B, ldb, nb, pl);
+        if (pl.rad[0] == 8 && pl.rad[1] == 10) return lds_fft2<8, 10, NT>(A, B, ldb, nb, pl);
+        if (pl.rad[0] == 10 && pl.rad[1] == 16) return lds_fft2<10, 16, NT>(A, B, ldb, nb, pl);
     }
     float2* src = A;
     float2* dst = B;

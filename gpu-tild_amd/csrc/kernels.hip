@@ -170,12 +170,6 @@ __global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_e
 }
 
 // ------------------------------------------------------------------ fused last axis
-// Weight of ramped style j's row at 0-based step k (pair_style.cu:308-328, see
-// RepDev::ramp_*): A_j(k) - A_j(0).
-__device__ __forceinline__ float ramp_weight(const RepDev& r, int j, int k) {
-    const double A = k == 0 ? r.ramp_A0[j] : r.ramp_c0[j] + r.ramp_c1[j] * (double)(k + 1);
-    return (float)(A - r.ramp_A0[j]);
-}
 
 // k-space coupling of one block's nodes for TT types (the same sums, in the same order,
 // as the general loop of k_last_fused): w_I = sum_J C_IJ rho_J with the symmetric table
@@ -327,7 +321,7 @@ __global__ void __launch_bounds__(NT) k_last_fused(const RepDev* __restrict__ re
             }
         }
         __syncthreads();
-        const float2* res = lds_fft<NT>(E0, E1, ldg, nlg, r.ax[la]);
+        const float2* res = lds_fft<NT>(E0, E1, ldg, nlg, r.axr);   // two-stage plans reversed
         Digits3 gs(threadIdx.x, NT, dnb, FastDiv(nf * no));
         for (int task = threadIdx.x; task < nlg * N; task += NT, gs.step()) {
             const int b = gs.lo, fo = gs.mid, n = gs.hi;

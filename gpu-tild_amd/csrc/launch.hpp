@@ -90,6 +90,18 @@ constexpr int kGradThreads = 256;                       // block of the y invers
 constexpr int kC2rThreads = 256;                        // block of the inverse x pass (records)
 constexpr size_t kPlaneLds = 163840;                    // LDS one block may use (MI355X)
 hipError_t launch_mid_grad(const LaunchCtx& c);
+// Fused last-axis pass with register-resident columns (last_reg.hip): last axes with a
+// two-stage plan R1 x R2 among 16 x 10, 10 x 8, 8 x 8 and 8 x 5, at most three types
+// (plus charges).  Blocks of kLastRegThreads threads hold nl_last <= kLastRegThreads / R1
+// columns; results agree with k_last_fused to rounding.
+constexpr int kLastRegThreads = 128;
+bool last_reg_plan(const AxisPlan& pl, int ntypes, int nf);
+inline int last_reg_max_lines(const AxisPlan& pl) { return kLastRegThreads / pl.rad[0]; }
+inline size_t last_reg_lds_bytes(int N, int nf, int nl) {
+    return (size_t)N * (size_t)(3 * nf * nl + 2) * sizeof(float2) + kMaxRamp * sizeof(float);
+}
+hipError_t launch_last_reg(const LaunchCtx& c, int koff);
+hipError_t configure_last_reg();
 hipError_t launch_plane_r2c(const LaunchCtx& c);
 // LDS of the forward plane pass: two buffers holding either the Ny/2 row pairs of
 // length Nx or the Ny x Nxh half spectra

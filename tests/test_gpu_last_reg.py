@@ -1,0 +1,147 @@
+"""The register-resident fused last-axis pass (k_last_reg) against the LDS-staged one.
+
+Both passes run the same transforms (forward plan R1 x R2, inverse plan R2 x R1) and
+the same coupling sums; the compiler fuses multiplies into additions differently where
+one pass keeps in registers what the other stages through LDS, so the two agree to
+rounding (1e-6 relative) rather than bit for bit.  The register pass itself gives the
+same bits for every column count per block, and the oracle bounds it at the usual
+tolerances.  The decks cover the
+four two-stage plans the register pass takes (last axes of 160, 80, 64 and 40 points),
+one to three types, charges, ramped styles and 2D.
+"""
+import pathlib
+
+import numpy as np
+import pytest
+
+import decks
+import gpu_util as U
+import oracle_ctypes as O
+import tild
+import tild_gen
+
+pytestmark = pytest.mark.gpu
+
+THREE = [("gaussian", 1, 1, 1.0, 0.5), ("gaussian", 2, 2, 1.0, 0.5), ("gaussian", 3, 3, 0.8, 0.5),
+         ("gaussian", 1, 2, 2.0, 0.5), ("gaussian", 1, 3, 1.5, 0.5), ("erf", 2, 3, 0.1, 1.0, 0.5)]
+
+
+def three_types(tmp, grid, box, n_chains=60, N=10, seed=9):
+    """Diblock melt with a third type on every third chain's tail."""
+    d = pathlib.Path(tmp)
+    d.mkdir(parents=True, exist_ok=True)
+    rng = np.random.default_rng(seed)
+    pos, types, mols, bonds = tild_gen.diblock(rng, n_chains, N, 0.5, box, 3)
+    types = np.where((mols % 3 == 0) & (types == 2), 3, types)
+    tild_gen.write_data(d / "input.data", pos, types, mols, bonds, box, 3, 3)
+    text = tild_gen.deck("input.data", 3, grid, THREE, "GJF", 0.01, seed)
+    (d / "input").write_text(text)
+    return text, d
+
+
+def charged_three(tmp, grid, box, n_chains=40, N=10, n_ions=120, seed=10):
+    """Polycations (type 1), polyanions (type 2) and neutral counterion-like beads of type 3
+    carrying the compensating charges: three types plus the charge field."""
+    d = pathlib.Path(tmp)
+    d.mkdir(parents=True, exist_ok=True)
+    rng = np.random.default_rng(seed)
+    pos, types, mols, bonds = tild_gen.diblock(rng, n_chains, N, 0.5, box, 3)
+    types = np.where(mols % 2 == 0, 1, 2)
+    q = np.where(types == 1, 1.0, -1.0)
+    ions = rng.random((n_ions, 3)) * np.asarray(box)
+    qi = np.where(np.arange(n_ions) % 2 == 0, 1.0, -1.0)
+    pos = np.concatenate([pos, ions])
+    types = np.concatenate([types, np.full(n_ions, 3)])
+    q = np.concatenate([q, qi])
+    mols = np.concatenate([mols, np.arange(n_chains + 1, n_chains + 1 + n_ions)])
+    tild_gen.write_data(d / "input.data", pos, types, mols, bonds, box, 3, 3, charges=q)
+    styles = [("gaussian_erf", i, j, 2.0, 0.5, 0.5) for i in range(1, 4) for j in range(i, 4)]
+    text = tild_gen.deck("input.data", 3, grid, styles, "EM", 0.005, seed, charges=(2.0, 0.5))
+    (d / "input").write_text(text)
+    return text, d
+
+
+def ramped(tmp):
+    text, d = decks.melt3d(tmp, grid=(8, 8, 160), box=(4.0, 4.0, 40.0), n_chains=80, styles=decks.MIXED,
+                           integrator="GJF", delt=0.01)
+    for old, new in [("gaussian 1 2 2.0 0.5", "gaussian 1 2 2.0 0.5 ramp 4.0"),
+                     ("erf 1 2 0.1 1.0 0.5", "erf 1 2 0.1 1.0 0.5 ramp 0.3")]:
+        assert old in text, old
+        text = text.replace(old, new)
+    return text, d
+
+
+CASES = {
+    "z160": lambda t: decks.melt3d(t, grid=(8, 8, 160), box=(4.0, 4.0, 40.0), n_chains=80),
+    "z80_mixed": lambda t: decks.melt3d(t, grid=(10, 8, 80), box=(5.0, 4.0, 20.0), n_chains=60, styles=decks.MIXED,
+                                        integrator="GJF", delt=0.01),
+    "z64_three": lambda t: three_types(t, grid=(12, 10, 64), box=(6.0, 5.0, 16.0)),
+    "z40_charged": lambda t: charged_three(t, grid=(12, 12, 40), box=(6.0, 6.0, 20.0)),
+    "z160_ramp": ramped,
+    "2d_y80": lambda t: decks.melt2d(t, grid=(20, 80), box=(10.0, 20.0)),
+    "2d_y40": lambda t: decks.melt2d(t, grid=(15, 40), box=(6.0, 10.0)),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_register_pass_equals_staged_pass(case, tmp_path):
+    text, d = CASES[case](tmp_path)
+    s = tild.System(text, d)
+    reg = tild.Engine(s, device=0, seed=5)
+    staged = tild.Engine(s, device=0, seed=5, fft_last_staged=True)
+    narrow = tild.Engine(s, device=0, seed=5, fft_last_lines=3)
+    gr, gs, gn = reg.geometry(), staged.geometry(), narrow.geometry()
+    print(f"[{case}] ns={s.ns} grid={s.grid_shape} reg={gr['last_reg']} lines={gr['last_lines']} "
+          f"staged={gs['last_reg']} narrow lines={gn['last_lines']}", flush=True)
+    assert gr["last_reg"] == 1 and gs["last_reg"] == 0 and gn["last_reg"] == 1
+    assert gn["last_lines"] == 3
+    fr, fs, fn = reg.forces(), staged.forces(), narrow.forces()
+    ef = U.rel_norm(fr, fs)
+    print(f"  |f_reg - f_staged|/|f| = {ef:.2e}, bit-identical: {np.array_equal(fr, fs)}")
+    assert ef < 1e-6
+    assert np.array_equal(fr, fn)
+    for e in (reg, staged, narrow):
+        e.step(7)
+    xr, xs, xn = reg.positions(), staged.positions(), narrow.positions()
+    assert np.array_equal(xr, xn)
+    L = [s.info("L0"), s.info("L1"), s.info("L2")][:s.dim]
+    assert np.abs(U.min_image(xr.astype(np.float64) - xs, L)).max() < 1e-5
+    pr, ps = reg.properties(), staged.properties()
+    assert abs(pr["Upe"] - ps["Upe"]) <= 1e-6 * abs(ps["Upe"])
+    for e in (reg, staged, narrow):
+        e.close()
+
+
+@pytest.mark.parametrize("case", ["z160", "z64_three", "z40_charged", "2d_y80"])
+def test_register_pass_matches_oracle(case, tmp_path):
+    text, d = CASES[case](tmp_path)
+    s = tild.System(text, d)
+    e = tild.Engine(s, device=0, seed=7)
+    o = O.System(text, d, seed=7)
+    assert e.geometry()["last_reg"] == 1
+    f, fo = e.forces(), o.forces()
+    ef = U.rel_norm(f, fo)
+    p, po = e.properties(), o.properties()
+    print(f"[{case}] |df|/|f| = {ef:.2e}  Upe {p['Upe']:.9e} oracle {po['Upe']:.9e}")
+    assert ef < 1e-5
+    assert abs(p["Upe"] - po["Upe"]) <= 1e-5 * abs(po["Upe"])
+    e.close()
+
+
+def test_register_pass_in_a_mixed_batch(tmp_path):
+    # replicas of either pass share one batch: each runs its own pass, and every
+    # replica equals the same deck run alone
+    a = decks.melt3d(tmp_path / "a", grid=(8, 8, 160), box=(4.0, 4.0, 40.0), n_chains=80)
+    b = decks.melt3d(tmp_path / "b")                       # 12^3: the staged pass
+    c = charged_three(tmp_path / "c", grid=(12, 12, 40), box=(6.0, 6.0, 20.0))
+    sys_ = [tild.System(t, d) for t, d in (a, b, c)]
+    bt = tild.Batch(device=0)
+    ids = [bt.add(s, 11 + k) for k, s in enumerate(sys_)]
+    assert [bt.geometry(r)["last_reg"] for r in ids] == [1, 0, 1]
+    bt.step(9)
+    for k, s in enumerate(sys_):
+        e = tild.Engine(s, device=0, seed=11 + k)
+        e.step(9)
+        assert np.array_equal(bt.positions(ids[k]), e.positions()), k
+        e.close()
+    bt.close()
