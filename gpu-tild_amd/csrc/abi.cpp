@@ -163,7 +163,7 @@ tild_status tild_engine_create(const tild_system* s, const tild_opts* opts, tild
     e->batch->set_spread(o.spread_mode, o.tile_target > 0 ? o.tile_target : 512, o.tile_by, o.tile_bz, o.tile_slack);
     e->batch->set_reorder(o.reorder_every == 0 ? -1 : (o.reorder_every < 0 ? 0 : o.reorder_every));
     e->batch->set_fft_tuning({o.fft_mid_elems, o.fft_last_lines, o.fft_c2r_rows, o.fft_plane, o.fft_last_dirs,
-                                   o.fft_last_staged ? 1 : 0});
+                                   o.fft_staged ? 1 : 0});
     e->batch->set_deposit_threads(o.deposit_threads);
     e->batch->set_autotune(o.autotune != 0);
     std::string err;
@@ -243,7 +243,7 @@ tild_status tild_batch_create(const tild_opts* opts, tild_batch** out) {
     b->batch->set_spread(o.spread_mode, o.tile_target > 0 ? o.tile_target : 512, o.tile_by, o.tile_bz, o.tile_slack);
     b->batch->set_reorder(o.reorder_every == 0 ? -1 : (o.reorder_every < 0 ? 0 : o.reorder_every));
     b->batch->set_fft_tuning({o.fft_mid_elems, o.fft_last_lines, o.fft_c2r_rows, o.fft_plane, o.fft_last_dirs,
-                                   o.fft_last_staged ? 1 : 0});
+                                   o.fft_staged ? 1 : 0});
     b->batch->set_deposit_threads(o.deposit_threads);
     b->batch->set_autotune(o.autotune != 0);
     std::string err;
@@ -314,6 +314,7 @@ static tild_status geometry_of(tild::Replica* R, tild_geometry* out) {
     out->last_lines = h.nl_last; out->last_dirs = h.last_dirs; out->mid_elems = h.mid_elems;
     out->c2r_rows = h.nr_c2r;
     out->last_reg = h.last_reg;
+    out->grad_reg = h.grad_reg;
     out->tuned = R->tune.tuned ? 1 : 0;
     out->from_cache = R->tune.from_cache ? 1 : 0;
     out->step_ms = R->tune.step_ms;

@@ -52,6 +52,7 @@ struct RepDev {
     int plane_fwd; // 1: the tiled deposit writes rho only; x R2C + y FFT run per plane
     int last_dirs; // directions per inverse batch of the fused last-axis pass (1..dim)
     int last_reg;  // 1: the fused last-axis pass runs register-resident (k_last_reg)
+    int grad_reg;  // 1: the y inverse of the gradient spectra runs register-resident (k_grad_reg)
     // particle state
     float* X[2];            // ns*dim, double-buffered by step parity
     float* XO;              // ns*dim, GJF previous position

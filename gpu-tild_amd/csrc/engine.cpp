@@ -299,14 +299,15 @@ int Batch::add_plain(const System& S, uint32_t seed, std::string& err) {
         h.nl_last = std::min(tune_.last_lines, Q);
         while (h.nl_last > 1 && last_lds_bytes(Nla, h.nf, h.last_dirs, h.nl_last) > kPlaneLds) --h.nl_last;
     }
-    // the register-resident form of the fused pass where it applies (bit-identical
-    // results): as many columns per block as its threads hold, or fewer when asked
-    h.last_reg = !tune_.last_staged && last_reg_plan(h.ax[la], h.ntypes, h.nf) ? 1 : 0;
-    if (h.last_reg) {
-        h.nl_last = std::min(last_reg_max_lines(h.ax[la]), Q);
-        if (tune_.last_lines > 0) h.nl_last = std::min(h.nl_last, tune_.last_lines);
-        while (h.nl_last > 1 && last_reg_lds_bytes(Nla, h.nf, h.nl_last) > kPlaneLds) --h.nl_last;
-    }
+    // the register-resident form of the fused pass where it applies: as many columns
+    // per block as its threads hold (the coupling table is blocked to match)
+    // per (column, index) the pass addresses spectra and table by 32-bit offsets
+    const bool off32 = (size_t)h.nf * D * h.Mh * 8 <= (size_t)0xFFFFFFFFu &&
+                       (size_t)h.Mh * 80 <= (size_t)0xFFFFFFFFu;   // table: at most 16 rows, padded
+    h.last_reg = !tune_.staged && off32 && last_reg_plan(h.ax[la], h.ntypes, h.nf) &&
+                 last_reg_lds_bytes(h.ax[la], h.nf) <= kPlaneLds ? 1 : 0;
+    if (h.last_reg) h.nl_last = last_reg_lines(h.ax[la]);
+    h.grad_reg = !tune_.staged && D == 3 && off32 && grad_reg_plan(h.ax[1]) ? 1 : 0;
     if (tune_.c2r_rows > 0) {
         h.nr_c2r = std::min(tune_.c2r_rows, rows);
         while (h.nr_c2r > 1 && (size_t)h.nr_c2r * nc * h.Nx > (size_t)kFftBudget) --h.nr_c2r;
@@ -676,12 +677,13 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
         }
         if (D == 3) {
             MidGeom g = mid_grad_geom(h);
-            add(PH_MIDY_G, g.nouter * g.nchunk, lds_bytes(g.N, 3 * g.nl));
+            if (h.grad_reg) add(PH_MIDY_GR, h.nf * h.Nz * grad_reg_chunks(h.ax[1], h.Nxh), grad_reg_lds_bytes(h.ax[1]));
+            else add(PH_MIDY_G, g.nouter * g.nchunk, lds_bytes(g.N, 3 * g.nl));
             MidGeom gz = mid_geom(h, 2, h.nf);
             add(PH_MIDZ_H, gz.nouter * gz.nchunk, lds_bytes(gz.N, gz.nl));
         }
         const int la = D - 1, Nla = la == 2 ? h.Nz : h.Ny, Q = la == 2 ? h.Ny * h.Nxh : h.Nxh;
-        if (h.last_reg) add(PH_LASTR, (Q + h.nl_last - 1) / h.nl_last, last_reg_lds_bytes(Nla, h.nf, h.nl_last));
+        if (h.last_reg) add(PH_LASTR, (Q + h.nl_last - 1) / h.nl_last, last_reg_lds_bytes(h.ax[la], h.nf));
         else add(PH_LAST, (Q + h.nl_last - 1) / h.nl_last, last_lds_bytes(Nla, h.nf, h.last_dirs, h.nl_last));
         const int nc = (h.nf * D + 1) / 2;
         add(PH_C2R, (rows + h.nr_c2r - 1) / h.nr_c2r, lds_bytes(h.Nx, h.nr_c2r * nc));
@@ -706,7 +708,7 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
     // one slab of a spatially ordered replica, so that its L2 holds that slab's grid
     // records; for the FFT passes adjacent column batches, whose rows share cache lines
     // that a single L2 then fetches once.
-    for (Phase ph : {PH_PART, PH_TCOUNT, PH_SPREADA, PH_TILE, PH_XA, PH_XP, PH_MIDY_H, PH_MIDY_G,
+    for (Phase ph : {PH_PART, PH_TCOUNT, PH_SPREADA, PH_TILE, PH_XA, PH_XP, PH_MIDY_H, PH_MIDY_G, PH_MIDY_GR,
                      PH_MIDZ_H, PH_PY, PH_LAST, PH_LASTR, PH_C2R, PH_PLANEF}) {
         std::vector<WorkItem>& v = maps[ph].host;
         const size_t n = v.size(), q = n / kXcds, rm = n % kXcds;
@@ -1036,6 +1038,7 @@ int Batch::enqueue_steps(SetMaps& m, int nsteps, bool integrate, bool write_forc
         HIPCHK(launch_last_fused(ctx(PH_LAST), 0, k));
         HIPCHK(launch_last_reg(ctx(PH_LASTR), k));
         HIPCHK(launch_mid_grad(ctx(PH_MIDY_G)));
+        HIPCHK(launch_grad_reg(ctx(PH_MIDY_GR)));
         HIPCHK(launch_x_c2r(ctx(PH_C2R), true, 0, nullptr, nullptr));
         // integrating steps claim the next step's records; a force evaluation leaves
         // positions and records as they are
@@ -1168,6 +1171,7 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
         ok(launch_last_fused(ctx(PH_LAST, 3, 0), 0, 0));
         ok(launch_last_reg(ctx(PH_LASTR, 3, 1), 0));
         ok(launch_mid_grad(ctx(PH_MIDY_G, 4, 0)));
+        ok(launch_grad_reg(ctx(PH_MIDY_GR, 4, 1)));
         ok(launch_x_c2r(ctx(PH_C2R, 5, 0), true, 0, nullptr, nullptr));
         ok(launch_particles(ctx(PH_PART, 6, 0), true, 0, 0, true));
         ok(launch_advance(stream_, d_reps_, m->d_ids, m->nids, 1, ev[7][0][0], ev[7][0][1]));

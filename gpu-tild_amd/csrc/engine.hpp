@@ -31,7 +31,7 @@ constexpr int kProfPhases = 9;   // claim, deposit, forward plane, last_fused, y
 // skip MIDY_G and C2R); PLANEF: x R2C + y FFT of tiled 3D replicas with even Ny
 // (their deposit writes densities only and they skip MIDY_H).
 enum Phase { PH_PART = 0, PH_SPREADA, PH_XA, PH_XP, PH_SCAN, PH_TILE, PH_MIDY_H, PH_MIDY_G,
-             PH_MIDZ_H, PH_PY, PH_LAST, PH_C2R, PH_PLANEF, PH_TCOUNT, PH_KPROPS, PH_RTILE, PH_LASTR, PH_COUNT };
+             PH_MIDZ_H, PH_PY, PH_LAST, PH_C2R, PH_PLANEF, PH_TCOUNT, PH_KPROPS, PH_RTILE, PH_LASTR, PH_MIDY_GR, PH_COUNT };
 
 struct PhaseMap {
     std::vector<WorkItem> host;
@@ -109,7 +109,7 @@ public:
     void set_reorder(int every) { reorder_every_ = every; }
     // FFT block geometry of replicas added from now on (0 = default): complex values
     // per middle-axis block, lines per fused last-axis block, rows per inverse-x block
-    struct FftTuning { int mid_elems = 0, last_lines = 0, c2r_rows = 0, plane = 0, last_dirs = 0, last_staged = 0; };
+    struct FftTuning { int mid_elems = 0, last_lines = 0, c2r_rows = 0, plane = 0, last_dirs = 0, staged = 0; };
     void set_fft_tuning(const FftTuning& t) { tune_ = t; }
     // Deposit block size of replicas added from now on (0: by bead density, else 256 or 512).
     void set_deposit_threads(int n) { dep_threads_ = n; }

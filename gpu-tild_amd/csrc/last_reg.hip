@@ -29,10 +29,6 @@ namespace tild {
 
 namespace {
 
-template <class T>
-__device__ __forceinline__ T* grid_at(T* base, bool o32, size_t i) {
-    return o32 ? at32(base, (uint32_t)i) : base + i;
-}
 
 // Products that the LDS-staged pass writes to LDS before its next stage adds them:
 // kept out of FMA contraction with those additions, as they are there.
@@ -45,38 +41,50 @@ __device__ __forceinline__ float2 grad_line(float k, float2 a) {
     return make_float2(k * a.y, k * a.x);
 }
 
-// One block: nb (<= NT / R1) adjacent columns q0 .. q0 + nb - 1 of the last-axis lines,
-// TT types and CH charge fields (NF = TT + CH fields).
+// One block: NB = NT / R1 adjacent columns q0 .. q0 + NB - 1 of the last-axis lines
+// (fewer in the last block), TT types and CH charge fields (NF = TT + CH fields).
+// LDS: the forward exchange (N x ldx), the inverse exchange (N x ldy), both stages'
+// twiddles, then the ramp weights (last_reg_lds_bytes).
 template <int R1, int R2, int NT, int TT, int CH>
-__device__ __forceinline__ void last_reg_block(const RepDev& r, int blk, float2* lds, const float* rw) {
-    constexpr int N = R1 * R2, NF = TT + CH, NR = TT * (TT + 1) / 2;
+__device__ __forceinline__ void last_reg_block(const RepDev& r, int blk, float2* lds, int koff) {
+    constexpr int N = R1 * R2, NF = TT + CH, NR = TT * (TT + 1) / 2, NB = NT / R1;
+    constexpr int ldx = NF * NB + 1, ldy = 2 * NF * NB + 1;
+    constexpr int NT1 = R1 * (R2 - 1), NT2 = R2 * (R1 - 1);
     const int la = r.dim - 1, D = r.dim;
     const int Q = la == 2 ? r.Ny * r.Nxh : r.Nxh;   // plane size = line stride
-    const int nl = r.nl_last;                       // columns per block (<= NT / R1)
-    const int q0 = blk * nl;
-    const int nb = min(nl, Q - q0);
-    const int ldx = NF * nl + 1, ldy = 2 * NF * nl + 1;
+    const int q0 = blk * NB;
+    const int nb = min(NB, Q - q0);
     float2* X1 = lds;                               // N x ldx: forward exchange
-    float2* Y1 = lds + (size_t)N * ldx;             // N x ldy: inverse exchange
-    const float2* tw1 = r.ax[la].tw + r.ax[la].tw_off[1];   // w_N^{i r}, i < R1, 1 <= r < R2
-    const float2* tw2 = r.axr.tw + r.axr.tw_off[1];         // w_N^{j r}, j < R2, 1 <= r < R1
-    const bool o32 = r.grid_off32;
-    const FastDiv dnl(nl);
-    int b, i;
-    i = dnl.div((int)threadIdx.x, b);
+    float2* Y1 = lds + N * ldx;                     // N x ldy: inverse exchange
+    float2* TW1 = Y1 + N * ldy;                     // w_N^{i r}, i < R1, 1 <= r < R2 (forward stage 2)
+    float2* TW2 = TW1 + NT1;                        // w_N^{j r}, j < R2, 1 <= r < R1 (inverse stage 2)
+    float* rw = (float*)(TW2 + NT2);                // ramp weights of this step
+    const int b = (int)threadIdx.x % NB, i = (int)threadIdx.x / NB;
+    // grid buffers below 4 GiB (required on the host): 32-bit element offsets
+    const uint32_t Qu = (uint32_t)Q, Mhu = (uint32_t)r.Mh, col = (uint32_t)(q0 + b);
     const bool liveB = i < R1 && b < nb;
 
+    // set-up, visible after the first barrier: twiddles and ramp weights into LDS
+    {
+        const float2* tw1 = r.ax[la].tw + r.ax[la].tw_off[1];
+        const float2* tw2 = r.axr.tw + r.axr.tw_off[1];
+        for (int t = threadIdx.x; t < NT1; t += NT) TW1[t] = gld(tw1 + t);
+        for (int t = threadIdx.x; t < NT2; t += NT) TW2[t] = gld(tw2 + t);
+        if ((int)threadIdx.x < r.nramp) rw[threadIdx.x] = ramp_weight(r, threadIdx.x, r.step + koff);
+    }
     // coupling rows of this thread's phase-B nodes, requested before anything else
-    const int nrows = r.ctab_rows, nlc = nl;
-    gmem<const float> cb = G(r.Ctab) + (size_t)blk * N * nrows * nlc + b;
+    const int nrows = r.ctab_rows;
+    // element (n, row) of this thread's column: cb0 + (n nrows + row) NB, from the
+    // uniform table base (every offset 32-bit)
+    const uint32_t cb0 = (uint32_t)blk * (uint32_t)(N * nrows * NB) + (uint32_t)b;
+    auto ctab = [&](int n, int row) { return G(at32(r.Ctab, cb0 + (uint32_t)((n * nrows + row) * NB)))[0]; };
     float c[R2][NR], cq[R2];
     if (liveB) {
 #pragma unroll
         for (int q = 0; q < R2; ++q) {
-            gmem<const float> cn = cb + (size_t)(i + R1 * q) * nrows * nlc;
 #pragma unroll
-            for (int k = 0; k < NR; ++k) c[q][k] = cn[k * nlc];
-            cq[q] = CH ? cn[r.ctab_charge * nlc] : 0.f;
+            for (int k = 0; k < NR; ++k) c[q][k] = ctab(i + R1 * q, k);
+            cq[q] = CH ? ctab(i + R1 * q, r.ctab_charge) : 0.f;
         }
     }
 
@@ -87,12 +95,12 @@ __device__ __forceinline__ void last_reg_block(const RepDev& r, int blk, float2*
         for (int f = 0; f < NF; ++f)
 #pragma unroll
             for (int rr = 0; rr < R1; ++rr)
-                v[f][rr] = gld(grid_at((const float2*)r.H, o32, (size_t)f * r.Mh + (size_t)(i + R2 * rr) * Q + q0 + b));
+                v[f][rr] = gld(at32((const float2*)r.H, (uint32_t)f * Mhu + (uint32_t)(i + R2 * rr) * Qu + col));
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
             Dft<R1>::run(v[f]);
 #pragma unroll
-            for (int q = 0; q < R1; ++q) X1[(i * R1 + q) * ldx + f * nl + b] = v[f][q];
+            for (int q = 0; q < R1; ++q) X1[(i * R1 + q) * ldx + f * NB + b] = v[f][q];
         }
     }
     __syncthreads();
@@ -103,10 +111,10 @@ __device__ __forceinline__ void last_reg_block(const RepDev& r, int blk, float2*
 #pragma unroll
         for (int f = 0; f < NF; ++f)
 #pragma unroll
-            for (int rr = 0; rr < R2; ++rr) w[f][rr] = X1[(i + R1 * rr) * ldx + f * nl + b];
+            for (int rr = 0; rr < R2; ++rr) w[f][rr] = X1[(i + R1 * rr) * ldx + f * NB + b];
         float2 t[R2];
 #pragma unroll
-        for (int rr = 1; rr < R2; ++rr) t[rr] = gld(tw1 + i * (R2 - 1) + rr - 1);
+        for (int rr = 1; rr < R2; ++rr) t[rr] = TW1[i * (R2 - 1) + rr - 1];
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
 #pragma unroll
@@ -120,7 +128,7 @@ __device__ __forceinline__ void last_reg_block(const RepDev& r, int blk, float2*
         for (int q = 0; q < R2; ++q) {
             const int n = i + R1 * q;
             for (int j = 0; j < r.nramp; ++j) {   // uniform trip count
-                const float v = rw[j] * cb[((size_t)n * nrows + r.ramp_row0 + j) * nlc];
+                const float v = rw[j] * ctab(n, r.ramp_row0 + j);
 #pragma unroll
                 for (int k = 0; k < NR; ++k) c[q][k] += r.ramp_pair[j] == k ? v : 0.f;
             }
@@ -160,49 +168,42 @@ __device__ __forceinline__ void last_reg_block(const RepDev& r, int blk, float2*
                 }
                 Dft<R2>::run(z);
 #pragma unroll
-                for (int q = 0; q < R2; ++q) Y1[(i * R2 + q) * ldy + (o * NF + f) * nl + b] = z[q];
+                for (int q = 0; q < R2; ++q) Y1[(i * R2 + q) * ldy + (o * NF + f) * NB + b] = z[q];
             }
         }
     }
     __syncthreads();
 
     // ---- C: stage 2 of the R2 x R1 plan (radix R1, twiddles w_N^{j r}, including the
-    // unit ones of j = 0 as the staged pass multiplies them) -> rows j + R2 q
-    if (i < R2 && b < nb) {
-        const int j = i;
-        float2 t[R1];
+    // unit ones of j = 0 as the staged pass multiplies them) -> rows j + R2 q.  One task
+    // per (column, j, line), spread over all threads.
+    for (int task = threadIdx.x; task < NB * R2 * 2 * NF; task += NT) {
+        const int bc = task % NB, rest = task / NB, j = rest % R2, line = rest / R2;   // line = o NF + f
+        if (bc >= nb) continue;
+        const int o = line >= NF ? 1 : 0, f = line - o * NF;
+        float2 z[R1];
 #pragma unroll
-        for (int rr = 1; rr < R1; ++rr) t[rr] = gld(tw2 + j * (R1 - 1) + rr - 1);
+        for (int rr = 0; rr < R1; ++rr) z[rr] = Y1[(j + R2 * rr) * ldy + line * NB + bc];
 #pragma unroll
-        for (int o = 0; o < 2; ++o) {
-            const int slot = o == 0 ? 0 : la;
+        for (int rr = 1; rr < R1; ++rr) z[rr] = cmul(z[rr], TW2[j * (R1 - 1) + rr - 1]);
+        Dft<R1>::run(z);
+        const uint32_t o0 = (uint32_t)(f * D + (o == 0 ? 0 : la)) * Mhu + (uint32_t)j * Qu + (uint32_t)(q0 + bc);
+        const uint32_t step = (uint32_t)R2 * Qu;
 #pragma unroll
-            for (int f = 0; f < NF; ++f) {
-                float2 z[R1];
-#pragma unroll
-                for (int rr = 0; rr < R1; ++rr) z[rr] = Y1[(j + R2 * rr) * ldy + (o * NF + f) * nl + b];
-#pragma unroll
-                for (int rr = 1; rr < R1; ++rr) z[rr] = cmul(z[rr], t[rr]);
-                Dft<R1>::run(z);
-                const size_t base = (size_t)(f * D + slot) * r.Mh + q0 + b;
-#pragma unroll
-                for (int q = 0; q < R1; ++q)
-                    gst(grid_at(r.Gk, o32, base + (size_t)(j + R2 * q) * Q), conjf2(z[q]));
-            }
-        }
+        for (int q = 0; q < R1; ++q) gst(at32(r.Gk, o0 + (uint32_t)q * step), conjf2(z[q]));
     }
 }
 
 template <int R1, int R2, int NT>
-__device__ __forceinline__ void last_reg_types(const RepDev& r, int blk, float2* lds, const float* rw) {
+__device__ __forceinline__ void last_reg_types(const RepDev& r, int blk, float2* lds, int koff) {
     const bool ch = r.nf > r.ntypes;
     switch (r.ntypes * 2 + (ch ? 1 : 0)) {
-        case 2: last_reg_block<R1, R2, NT, 1, 0>(r, blk, lds, rw); break;
-        case 3: last_reg_block<R1, R2, NT, 1, 1>(r, blk, lds, rw); break;
-        case 4: last_reg_block<R1, R2, NT, 2, 0>(r, blk, lds, rw); break;
-        case 5: last_reg_block<R1, R2, NT, 2, 1>(r, blk, lds, rw); break;
-        case 6: last_reg_block<R1, R2, NT, 3, 0>(r, blk, lds, rw); break;
-        case 7: last_reg_block<R1, R2, NT, 3, 1>(r, blk, lds, rw); break;
+        case 2: last_reg_block<R1, R2, NT, 1, 0>(r, blk, lds, koff); break;
+        case 3: last_reg_block<R1, R2, NT, 1, 1>(r, blk, lds, koff); break;
+        case 4: last_reg_block<R1, R2, NT, 2, 0>(r, blk, lds, koff); break;
+        case 5: last_reg_block<R1, R2, NT, 2, 1>(r, blk, lds, koff); break;
+        case 6: last_reg_block<R1, R2, NT, 3, 0>(r, blk, lds, koff); break;
+        case 7: last_reg_block<R1, R2, NT, 3, 1>(r, blk, lds, koff); break;
         default: break;   // excluded by last_reg_plan on the host
     }
 }
@@ -216,20 +217,118 @@ __global__ void __launch_bounds__(NT) k_last_reg(const RepDev* __restrict__ reps
     const WorkItem wi = wmap[blockIdx.x];
     const RepDev& r = reps[wi.rep];
     const AxisPlan& pl = r.ax[r.dim - 1];
-    const int la = r.dim - 1;
-    const int N = la == 2 ? r.Nz : r.Ny;
-    // ramp weights after both exchange buffers
-    const int ldx = r.nf * r.nl_last + 1, ldy = 2 * r.nf * r.nl_last + 1;
-    float* rw = (float*)(lds + (size_t)N * (ldx + ldy));
-    if ((int)threadIdx.x < r.nramp) rw[threadIdx.x] = ramp_weight(r, threadIdx.x, r.step + koff);
-    __syncthreads();
     switch (pl.rad[0] * 32 + pl.rad[1]) {   // uniform per block
-        case 16 * 32 + 10: last_reg_types<16, 10, NT>(r, wi.blk, lds, rw); break;
-        case 10 * 32 + 8: last_reg_types<10, 8, NT>(r, wi.blk, lds, rw); break;
-        case 8 * 32 + 8: last_reg_types<8, 8, NT>(r, wi.blk, lds, rw); break;
-        case 8 * 32 + 5: last_reg_types<8, 5, NT>(r, wi.blk, lds, rw); break;
+        case 16 * 32 + 10: last_reg_types<16, 10, NT>(r, wi.blk, lds, koff); break;
+        case 10 * 32 + 8: last_reg_types<10, 8, NT>(r, wi.blk, lds, koff); break;
+        case 8 * 32 + 8: last_reg_types<8, 8, NT>(r, wi.blk, lds, koff); break;
+        case 8 * 32 + 5: last_reg_types<8, 5, NT>(r, wi.blk, lds, koff); break;
         default: break;   // excluded by last_reg_plan on the host
     }
+}
+
+// ------------------------------------------------------------------ y inverse (3D)
+// The y-axis inverse of the gradient spectra (the operation of k_mid_grad, kernels.hip)
+// for two-stage y plans R1 x R2, columns in registers between the stages: per (field,
+// z) plane and NB = NT / R1 adjacent x columns,
+//   A  thread (b, i), i < R2: rows i + R2 r of U (slot 0) and V (slot 2), the three
+//      lines conj(U), conj(-i k_y U), conj(V), DFT_R1; exchange;
+//   B  thread (b, i), i < R1: twiddles w_Ny^{i r}, DFT_R2 -> rows i + R1 q of the three
+//      outputs (slots 0, 1, 2).
+// One barrier.  A block reads its columns of slots 0 and 2 before it writes any slot,
+// and no other block touches them.
+namespace {
+template <int R1, int R2, int NT>
+__device__ __forceinline__ void grad_reg_block(const RepDev& r, int blk, float2* lds) {
+    constexpr int N = R1 * R2, NB = NT / R1, ld = 3 * NB + 1, NT1 = R1 * (R2 - 1);
+    const int Nxh = r.Nxh;
+    const int nchunk = (Nxh + NB - 1) / NB;
+    const int o = blk / nchunk, c = blk - o * nchunk;   // o = f Nz + z
+    const int f = o / r.Nz, z = o - f * r.Nz;
+    const int b0 = c * NB, nb = min(NB, Nxh - b0);
+    float2* X1 = lds;                 // N x ld
+    float2* TW1 = X1 + N * ld;        // w_Ny^{i r}, i < R1, 1 <= r < R2
+    const int b = (int)threadIdx.x % NB, i = (int)threadIdx.x / NB;
+    {
+        const float2* tw1 = r.ax[1].tw + r.ax[1].tw_off[1];
+        for (int t = threadIdx.x; t < NT1; t += NT) TW1[t] = gld(tw1 + t);
+    }
+    // element (n, column b) of slot s: 32-bit offsets from the uniform base
+    const uint32_t Mhu = (uint32_t)r.Mh, Nxu = (uint32_t)Nxh;
+    const uint32_t plane = (uint32_t)z * (uint32_t)N * Nxu + (uint32_t)(b0 + b);
+    const uint32_t sU = (uint32_t)(f * 3 + 0) * Mhu + plane, sY = (uint32_t)(f * 3 + 1) * Mhu + plane,
+                   sV = (uint32_t)(f * 3 + 2) * Mhu + plane;
+    if (i < R2 && b < nb) {
+        float2 u[R1], v[R1];
+#pragma unroll
+        for (int rr = 0; rr < R1; ++rr) {
+            const uint32_t n = (uint32_t)(i + R2 * rr) * Nxu;
+            u[rr] = gld(at32((const float2*)r.Gk, sU + n));
+            v[rr] = gld(at32((const float2*)r.Gk, sV + n));
+        }
+        float2 L[3][R1];
+#pragma unroll
+        for (int rr = 0; rr < R1; ++rr) {
+            const int n = i + R2 * rr;
+            const float ky = 2 * n == N ? 0.f : r.kfac[1] * (float)(2 * n < N ? n : n - N);
+            // lines of the conj(FFT(conj)) inverse: conj(u), conj(-i ky u), conj(v)
+            L[0][rr] = make_float2(u[rr].x, -u[rr].y);
+            L[1][rr] = grad_line(ky, u[rr]);
+            L[2][rr] = make_float2(v[rr].x, -v[rr].y);
+        }
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            Dft<R1>::run(L[l]);
+#pragma unroll
+            for (int q = 0; q < R1; ++q) X1[(i * R1 + q) * ld + l * NB + b] = L[l][q];
+        }
+    }
+    __syncthreads();
+    if (i < R1 && b < nb) {
+        float2 t[R2];
+#pragma unroll
+        for (int rr = 1; rr < R2; ++rr) t[rr] = TW1[i * (R2 - 1) + rr - 1];
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            float2 w[R2];
+#pragma unroll
+            for (int rr = 0; rr < R2; ++rr) w[rr] = X1[(i + R1 * rr) * ld + l * NB + b];
+#pragma unroll
+            for (int rr = 1; rr < R2; ++rr) w[rr] = cmul(w[rr], t[rr]);
+            Dft<R2>::run(w);
+            const uint32_t s0 = l == 0 ? sU : (l == 1 ? sY : sV);
+#pragma unroll
+            for (int q = 0; q < R2; ++q)
+                gst(at32(r.Gk, s0 + (uint32_t)(i + R1 * q) * Nxu), conjf2(w[q]));
+        }
+    }
+}
+}  // namespace
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_grad_reg(const RepDev* __restrict__ reps, const WorkItem* __restrict__ wmap) {
+    extern __shared__ float2 lds[];
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    const AxisPlan& pl = r.ax[1];
+    switch (pl.rad[0] * 32 + pl.rad[1]) {   // uniform per block
+        case 16 * 32 + 10: grad_reg_block<16, 10, NT>(r, wi.blk, lds); break;
+        case 10 * 32 + 8: grad_reg_block<10, 8, NT>(r, wi.blk, lds); break;
+        case 8 * 32 + 8: grad_reg_block<8, 8, NT>(r, wi.blk, lds); break;
+        case 8 * 32 + 5: grad_reg_block<8, 5, NT>(r, wi.blk, lds); break;
+        default: break;   // excluded by grad_reg_plan on the host
+    }
+}
+
+bool grad_reg_plan(const AxisPlan& pl) {
+    if (pl.nst != 2) return false;
+    const int k = pl.rad[0] * 32 + pl.rad[1];
+    return k == 16 * 32 + 10 || k == 10 * 32 + 8 || k == 8 * 32 + 8 || k == 8 * 32 + 5;
+}
+
+hipError_t launch_grad_reg(const LaunchCtx& c) {
+    if (!c.n) return hipSuccess;
+    TILD_LAUNCH(c, k_grad_reg<kGradRegThreads>, dim3(c.n), dim3(kGradRegThreads), c.lds, c.reps, c.wmap);
+    return hipGetLastError();
 }
 
 bool last_reg_plan(const AxisPlan& pl, int ntypes, int nf) {

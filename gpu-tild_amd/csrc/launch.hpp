@@ -92,15 +92,28 @@ constexpr size_t kPlaneLds = 163840;                    // LDS one block may use
 hipError_t launch_mid_grad(const LaunchCtx& c);
 // Fused last-axis pass with register-resident columns (last_reg.hip): last axes with a
 // two-stage plan R1 x R2 among 16 x 10, 10 x 8, 8 x 8 and 8 x 5, at most three types
-// (plus charges).  Blocks of kLastRegThreads threads hold nl_last <= kLastRegThreads / R1
+// (plus charges).  Blocks of kLastRegThreads threads hold nl_last = kLastRegThreads / R1
 // columns; results agree with k_last_fused to rounding.
 constexpr int kLastRegThreads = 128;
 bool last_reg_plan(const AxisPlan& pl, int ntypes, int nf);
-inline int last_reg_max_lines(const AxisPlan& pl) { return kLastRegThreads / pl.rad[0]; }
-inline size_t last_reg_lds_bytes(int N, int nf, int nl) {
-    return (size_t)N * (size_t)(3 * nf * nl + 2) * sizeof(float2) + kMaxRamp * sizeof(float);
+inline int last_reg_lines(const AxisPlan& pl) { return kLastRegThreads / pl.rad[0]; }
+// the two exchange buffers, both stages' twiddles (2N - R1 - R2) and the ramp weights
+inline size_t last_reg_lds_bytes(const AxisPlan& pl, int nf) {
+    const size_t N = (size_t)pl.N, nl = (size_t)last_reg_lines(pl);
+    return (N * (3 * (size_t)nf * nl + 2) + 2 * N) * sizeof(float2) + kMaxRamp * sizeof(float);
 }
 hipError_t launch_last_reg(const LaunchCtx& c, int koff);
+// The y inverse of the gradient spectra with register-resident columns (3D, two-stage
+// y plans as above): blocks of kGradRegThreads threads, grad_reg_lines(plan) columns
+// each, grad_reg_chunks per (field, z) plane.
+constexpr int kGradRegThreads = 128;
+bool grad_reg_plan(const AxisPlan& pl);
+inline int grad_reg_lines(const AxisPlan& pl) { return kGradRegThreads / pl.rad[0]; }
+inline int grad_reg_chunks(const AxisPlan& pl, int Nxh) { return (Nxh + grad_reg_lines(pl) - 1) / grad_reg_lines(pl); }
+inline size_t grad_reg_lds_bytes(const AxisPlan& pl) {
+    return ((size_t)pl.N * (3 * (size_t)grad_reg_lines(pl) + 1) + (size_t)pl.N) * sizeof(float2);
+}
+hipError_t launch_grad_reg(const LaunchCtx& c);
 hipError_t configure_last_reg();
 hipError_t launch_plane_r2c(const LaunchCtx& c);
 // LDS of the forward plane pass: two buffers holding either the Ny/2 row pairs of

@@ -50,13 +50,14 @@ class Opts(ctypes.Structure):
                 ("tile_bz", ctypes.c_int), ("reorder_every", ctypes.c_int), ("fft_mid_elems", ctypes.c_int),
                 ("fft_last_lines", ctypes.c_int), ("fft_c2r_rows", ctypes.c_int), ("fft_plane", ctypes.c_int),
                 ("fft_last_dirs", ctypes.c_int), ("tile_slack", ctypes.c_int), ("autotune", ctypes.c_int),
-                ("deposit_threads", ctypes.c_int), ("fft_last_staged", ctypes.c_int)]
+                ("deposit_threads", ctypes.c_int), ("fft_staged", ctypes.c_int)]
 
 
 class Geometry(ctypes.Structure):
     _fields_ = [("tile_target", ctypes.c_int), ("tiles", ctypes.c_int), ("deposit_threads", ctypes.c_int),
                 ("plane", ctypes.c_int), ("last_lines", ctypes.c_int), ("last_dirs", ctypes.c_int),
                 ("mid_elems", ctypes.c_int), ("c2r_rows", ctypes.c_int), ("last_reg", ctypes.c_int),
+                ("grad_reg", ctypes.c_int),
                 ("tuned", ctypes.c_int),
                 ("from_cache", ctypes.c_int), ("step_ms", ctypes.c_double)]
 
@@ -261,7 +262,7 @@ class Engine:
 
     def __init__(self, system, device=0, seed=None, spread_mode=0, tile_target=0, tile_by=0, tile_bz=0,
                  reorder_every=0, fft_mid_elems=0, fft_last_lines=0, fft_c2r_rows=0, fft_plane=0,
-                 fft_last_dirs=0, tile_slack=0, autotune=False, deposit_threads=0, fft_last_staged=False):
+                 fft_last_dirs=0, tile_slack=0, autotune=False, deposit_threads=0, fft_staged=False):
         o = Opts()
         o.device = device
         o.spread_mode = spread_mode
@@ -270,7 +271,7 @@ class Engine:
         o.reorder_every = reorder_every
         o.fft_mid_elems, o.fft_last_lines, o.fft_c2r_rows = fft_mid_elems, fft_last_lines, fft_c2r_rows
         o.fft_plane, o.fft_last_dirs = fft_plane, fft_last_dirs
-        o.fft_last_staged = int(bool(fft_last_staged))
+        o.fft_staged = int(bool(fft_staged))
         o.tile_slack = tile_slack
         if seed is not None:
             o.use_seed = 1
@@ -406,12 +407,12 @@ class Batch:
     """Many device-resident replicas advanced together (tild_batch)."""
 
     def __init__(self, device=0, spread_mode=0, tile_target=0, reorder_every=0, autotune=False, deposit_threads=0,
-                 fft_last_staged=False):
+                 fft_staged=False):
         o = Opts()
         o.device = device
         o.spread_mode = spread_mode
         o.autotune, o.deposit_threads = int(bool(autotune)), deposit_threads
-        o.fft_last_staged = int(bool(fft_last_staged))
+        o.fft_staged = int(bool(fft_staged))
         o.tile_target = tile_target
         o.reorder_every = reorder_every
         h = ctypes.c_void_p()

@@ -111,12 +111,13 @@ typedef struct {
                             on it. */
     int deposit_threads; /* tiled spread: deposit block size, 256 or 512 (0 = by the
                             beads per tile).  Results do not depend on it. */
-    int fft_last_staged; /* nonzero: the fused z pass stages every value through LDS even
-                            where its register-resident form applies (two-stage last-axis
-                            plans 16x10, 10x8, 8x8, 8x5, at most three types).  The two
-                            forms agree to rounding (about 1e-7 relative), not bit for
-                            bit: the compiler fuses multiplies into additions where one
-                            keeps in registers what the other stages through LDS. */
+    int fft_staged;      /* nonzero: the fused z pass and the y inverse stage every value
+                            through LDS even where their register-resident forms apply
+                            (two-stage plans 16x10, 10x8, 8x8, 8x5 of the axis; the z pass
+                            at most three types).  The forms agree to rounding (about 1e-7
+                            relative), not bit for bit: the compiler fuses multiplies into
+                            additions where one keeps in registers what the other stages
+                            through LDS. */
 } tild_opts;
 
 /* Properties of calc_properties() (calc_properties.cu:20-116): FP64 reductions. */
@@ -212,6 +213,7 @@ typedef struct {
     int plane;           /* 1: forward plane pass */
     int last_lines, last_dirs, mid_elems, c2r_rows;   /* FFT line batching */
     int last_reg;        /* 1: register-resident fused z pass */
+    int grad_reg;        /* 1: register-resident y inverse */
     int tuned, from_cache;
     double step_ms;      /* kernel time per step measured by the tuner (0 if not tuned) */
 } tild_geometry;
