@@ -315,6 +315,7 @@ static tild_status geometry_of(tild::Replica* R, tild_geometry* out) {
     out->c2r_rows = h.nr_c2r;
     out->last_reg = h.last_reg;
     out->grad_reg = h.grad_reg;
+    out->plane_reg = h.plane_reg;
     out->tuned = R->tune.tuned ? 1 : 0;
     out->from_cache = R->tune.from_cache ? 1 : 0;
     out->step_ms = R->tune.step_ms;

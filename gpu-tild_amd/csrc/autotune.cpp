@@ -14,6 +14,7 @@
 #include <sys/stat.h>
 
 #include <cerrno>
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -119,7 +120,7 @@ int Batch::add_with(const System& S, uint32_t seed, const Geometry& g, std::stri
 constexpr int kTuneCopies = 32;
 constexpr double kTuneBeads = 4.0e5;
 
-double Batch::measure_geometry(const System& S, uint32_t seed, const Geometry& g, std::string& err) {
+double Batch::measure_geometry(const System& S, uint32_t seed, const Geometry& g, std::string& err, int* passes) {
     constexpr double kInf = std::numeric_limits<double>::infinity();
     const int copies = std::max(1, std::min(kTuneCopies, (int)std::lround(kTuneBeads / std::max(S.ns, 1))));
     std::vector<int> ids;
@@ -136,6 +137,17 @@ double Batch::measure_geometry(const System& S, uint32_t seed, const Geometry& g
         ids.push_back(id);
     }
     tuning_ = false;
+    // which forms of the FFT passes the geometry runs: a geometry that switches between
+    // forms that agree only to rounding is not a candidate (tuning changes no result)
+    {
+        const RepDev& h = reps_[ids[0]]->h;
+        const int sig = h.plane_reg | (h.last_reg << 1) | (h.grad_reg << 2);
+        if (*passes < 0) *passes = sig;
+        else if (sig != *passes) {
+            for (int r : ids) remove(r);
+            return kInf;
+        }
+    }
     double ms[kProfPhases] = {};
     double t = kInf;
     if (step(ids, 3, e, false) == 0 && profile(ids, 6, ms, kProfPhases, e) == 0) {
@@ -162,25 +174,30 @@ int Batch::tune_geometry(const System& S, uint32_t seed, TuneResult& out, std::s
         }
     }
     Geometry best;
-    double best_t = measure_geometry(S, seed, best, err);
+    int passes = -1;
+    double best_t = measure_geometry(S, seed, best, err, &passes);
     if (!std::isfinite(best_t)) {
         if (err.empty()) err = "autotune: the default block geometry does not run this deck";
         return -TILD_ERR_DEVICE;
     }
     const bool tiled_ok = spread_mode_ != 1;
-    // the register-resident fused pass (always taken where the deck's last axis and types
-    // allow it) has no batching of inverse directions
-    bool reg_ok = false;
+    // The register-resident passes are taken wherever the deck allows them: the fused
+    // z pass has no batching of inverse directions to tune, and the forward plane pass
+    // stays on where its register form applies (the separate passes agree with it to
+    // rounding only, and tuning changes no result).
+    bool reg_ok = false, plane_reg = false;
     {
         const int la = S.Dim - 1, nf = S.ntypes + (S.charges.enabled ? 1 : 0);
-        AxisPlan lp;
-        std::vector<float2> tw((size_t)std::max(S.Nx[la], 1));
+        AxisPlan lp, px, py;
+        std::vector<float2> tw((size_t)std::max({S.Nx[0], S.Nx[1], S.Nx[la], 1}));
         int ntw = 0;
         reg_ok = plan_axis(S.Nx[la], lp, tw.data(), &ntw) == nullptr && last_reg_plan(lp, S.ntypes, nf);
+        plane_reg = S.Dim == 3 && plan_axis(S.Nx[0], px, tw.data(), &ntw) == nullptr &&
+                    plan_axis(S.Nx[1], py, tw.data(), &ntw) == nullptr && plane_reg_plan(px, py);
     }
     struct Axis { int Geometry::*field; std::vector<int> cand; bool on; };
     const std::vector<Axis> axes = {
-        {&Geometry::plane, {0, -1}, S.Dim == 3 && tiled_ok},
+        {&Geometry::plane, {0, -1}, S.Dim == 3 && tiled_ok && !plane_reg},
         {&Geometry::tile_target, {128, 256, 512, 1024}, tiled_ok},
         {&Geometry::dep_threads, {256, 512}, tiled_ok},
         {&Geometry::last_lines, {0, 2, 4, 8, 16}, true},
@@ -196,7 +213,7 @@ int Batch::tune_geometry(const System& S, uint32_t seed, TuneResult& out, std::s
             Geometry g = best;
             g.*a.field = c;
             std::string e;
-            const double t = measure_geometry(S, seed, g, e);
+            const double t = measure_geometry(S, seed, g, e, &passes);
             if (!e.empty()) { err = e; return -TILD_ERR_DEVICE; }
             if (t < 0.98 * best_t) { best = g; best_t = t; }
         }

@@ -179,6 +179,7 @@ int Batch::init(std::string& err) {
     HIPCHK(hipStreamCreateWithFlags(&setup_stream_, hipStreamNonBlocking));
     HIPCHK(configure_kernels());
     HIPCHK(configure_last_reg());
+    HIPCHK(configure_plane_reg());
     return 0;
 }
 
@@ -493,6 +494,9 @@ int Batch::add_plain(const System& S, uint32_t seed, std::string& err) {
     if (D == 3 && h.spread_mode == 2 && tune_.plane >= 0 && h.Ny % 2 == 0 && h.By % 2 == 0 &&
         plane_r2c_lds(h) <= kPlaneLds)
         h.plane_fwd = 1;
+    // its register-resident form: square planes with a two-stage plan
+    h.plane_reg = h.plane_fwd && !tune_.staged && h.grid_off32 && plane_reg_plan(h.ax[0], h.ax[1]) &&
+                  plane_reg_lds_bytes(h.ax[0]) <= kPlaneLds ? 1 : 0;
 
     int id = -1;
     for (int k = 0; k < (int)reps_.size(); ++k) if (!reps_[k]) { id = k; break; }
@@ -672,7 +676,8 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
         {
             MidGeom g = mid_geom(h, 1, h.nf);
             add(PH_PY, g.nouter * g.nchunk, lds_bytes(g.N, g.nl));
-            if (D == 3 && h.plane_fwd) add(PH_PLANEF, h.nf * h.Nz, plane_r2c_lds(h));
+            if (D == 3 && h.plane_reg) add(PH_PLANER, h.nf * h.Nz, plane_reg_lds_bytes(h.ax[0]));
+            else if (D == 3 && h.plane_fwd) add(PH_PLANEF, h.nf * h.Nz, plane_r2c_lds(h));
             else if (D == 3) add(PH_MIDY_H, g.nouter * g.nchunk, lds_bytes(g.N, g.nl));
         }
         if (D == 3) {
@@ -709,7 +714,7 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
     // records; for the FFT passes adjacent column batches, whose rows share cache lines
     // that a single L2 then fetches once.
     for (Phase ph : {PH_PART, PH_TCOUNT, PH_SPREADA, PH_TILE, PH_XA, PH_XP, PH_MIDY_H, PH_MIDY_G, PH_MIDY_GR,
-                     PH_MIDZ_H, PH_PY, PH_LAST, PH_LASTR, PH_C2R, PH_PLANEF}) {
+                     PH_MIDZ_H, PH_PY, PH_LAST, PH_LASTR, PH_C2R, PH_PLANEF, PH_PLANER}) {
         std::vector<WorkItem>& v = maps[ph].host;
         const size_t n = v.size(), q = n / kXcds, rm = n % kXcds;
         if (n < 2 * (size_t)kXcds) continue;
@@ -722,7 +727,7 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
     }
     for (int p = 0; p < PH_COUNT; ++p) {
         maps[p].n = (int)maps[p].host.size();
-        if (maps[p].lds > ((p == PH_PLANEF || p == PH_LAST || p == PH_LASTR) ? kPlaneLds : (size_t)kMaxLds)) { err = "pass needs more LDS than a block may use"; return -TILD_ERR_UNSUPPORTED; }
+        if (maps[p].lds > ((p == PH_PLANEF || p == PH_PLANER || p == PH_LAST || p == PH_LASTR) ? kPlaneLds : (size_t)kMaxLds)) { err = "pass needs more LDS than a block may use"; return -TILD_ERR_UNSUPPORTED; }
         if (maps[p].dev && (!reorder_only || maps[p].cap < maps[p].n)) {
             // reused buffers are last read by the previous reorder's launches
             HIPCHK(hipStreamSynchronize(stream_));
@@ -1034,6 +1039,7 @@ int Batch::enqueue_steps(SetMaps& m, int nsteps, bool integrate, bool write_forc
         HIPCHK(launch_x_r2c(ctx(PH_XA), true, 0, nullptr, nullptr));
         HIPCHK(launch_tile_deposit(ctx(PH_TILE), k));
         HIPCHK(launch_plane_r2c(ctx(PH_PLANEF)));
+        HIPCHK(launch_plane_reg(ctx(PH_PLANER)));
         HIPCHK(launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr));
         HIPCHK(launch_last_fused(ctx(PH_LAST), 0, k));
         HIPCHK(launch_last_reg(ctx(PH_LASTR), k));
@@ -1145,7 +1151,7 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
     // phases 0-7 are kernels, timed by events bound to each launch's own start and stop
     // (at most two kernels per phase: the two spread paths of a mixed set); phase 8,
     // the spatial reorders, by an event pair around them
-    constexpr int kSlots = 2, kKernelPhases = kProfPhases - 1;
+    constexpr int kSlots = 3, kKernelPhases = kProfPhases - 1;
     hipEvent_t ev[kKernelPhases][kSlots][2], rv[2];
     for (auto& a : ev) for (auto& b : a) for (auto& e : b) HIPCHK(hipEventCreate(&e));
     for (auto& e : rv) HIPCHK(hipEventCreate(&e));
@@ -1167,6 +1173,7 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
         ok(launch_x_r2c(ctx(PH_XA, 1, 0), true, 0, nullptr, nullptr));
         ok(launch_tile_deposit(ctx(PH_TILE, 1, 1), 0));
         ok(launch_plane_r2c(ctx(PH_PLANEF, 2, 0)));
+        ok(launch_plane_reg(ctx(PH_PLANER, 2, 2)));
         ok(launch_mid(ctx(PH_MIDY_H, 2, 1), 1, 0, 0, 0, nullptr));
         ok(launch_last_fused(ctx(PH_LAST, 3, 0), 0, 0));
         ok(launch_last_reg(ctx(PH_LASTR, 3, 1), 0));

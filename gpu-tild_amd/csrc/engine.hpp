@@ -31,7 +31,7 @@ constexpr int kProfPhases = 9;   // claim, deposit, forward plane, last_fused, y
 // skip MIDY_G and C2R); PLANEF: x R2C + y FFT of tiled 3D replicas with even Ny
 // (their deposit writes densities only and they skip MIDY_H).
 enum Phase { PH_PART = 0, PH_SPREADA, PH_XA, PH_XP, PH_SCAN, PH_TILE, PH_MIDY_H, PH_MIDY_G,
-             PH_MIDZ_H, PH_PY, PH_LAST, PH_C2R, PH_PLANEF, PH_TCOUNT, PH_KPROPS, PH_RTILE, PH_LASTR, PH_MIDY_GR, PH_COUNT };
+             PH_MIDZ_H, PH_PY, PH_LAST, PH_C2R, PH_PLANEF, PH_TCOUNT, PH_KPROPS, PH_RTILE, PH_LASTR, PH_MIDY_GR, PH_PLANER, PH_COUNT };
 
 struct PhaseMap {
     std::vector<WorkItem> host;
@@ -183,7 +183,9 @@ public:
 private:
     // the sweep behind set_autotune (autotune.cpp)
     int tune_geometry(const System& S, uint32_t seed, TuneResult& out, std::string& err);
-    double measure_geometry(const System& S, uint32_t seed, const Geometry& g, std::string& err);
+    // kernel ms per replica of a step of g; *passes: the FFT-pass forms g runs, set by the
+    // first call (< 0 on entry), and any later g running other forms measures +inf
+    double measure_geometry(const System& S, uint32_t seed, const Geometry& g, std::string& err, int* passes);
     int add_with(const System& S, uint32_t seed, const Geometry& g, std::string& err);
     int add_plain(const System& S, uint32_t seed, std::string& err);
     bool autotune_ = false, tuning_ = false;

@@ -1,6 +1,14 @@
-// last_reg.hip -- fused last-axis pass with register-resident columns (CDNA4 / gfx950).
+// fft_reg.hip -- FFT passes of the step with register-resident columns (CDNA4 / gfx950):
+// the forward plane pass (k_plane_reg), the fused last-axis pass (k_last_reg) and the y
+// inverse of the gradient spectra (k_grad_reg).  Each takes the axes whose
+// Stockham plan has two stages R1 x R2 (R1 >= R2) among 16 x 10, 10 x 8, 8 x 8 and 8 x 5;
+// a thread keeps its columns in registers between the stages, so a pass makes one to
+// three LDS exchanges where the LDS-staged forms (kernels.hip) stage every value through
+// LDS with a barrier after each phase.  (The inverse x transform into the gather
+// records stays LDS-staged: its register form reads each half-spectrum value twice,
+// for column k and its Hermitian partner N - k, and measured slower.)
 //
-// The same operation as k_last_fused (kernels.hip) -- last-axis forward FFT of every
+// k_last_reg: the same operation as k_last_fused (kernels.hip) -- last-axis forward FFT of every
 // field, the k-space coupling w_I = sum_J C_IJ rho_J (+ the charge row), and the
 // inverse transforms of w and -i k_la w -- for last axes with a two-stage plan
 // N = R1 R2 (R1 >= R2: 160 = 16 x 10, 80 = 10 x 8, 64 = 8 x 8, 40 = 8 x 5).  Each
@@ -19,8 +27,7 @@
 // The transforms and sums are those of the LDS-staged pass (whose inverse also runs the
 // reversed plan RepDev::axr); the two agree to rounding, since the compiler contracts
 // multiplies into the following additions differently where one keeps in registers
-// what the other stages through LDS.  The result does not depend on the number of
-// columns per block.
+// what the other stages through LDS.
 #include "device.hpp"
 #include "fft_dev.hpp"
 #include "launch.hpp"
@@ -194,16 +201,21 @@ __device__ __forceinline__ void last_reg_block(const RepDev& r, int blk, float2*
     }
 }
 
+template <int R1, int R2, int NT, int TT, int CH>
+__device__ __forceinline__ void last_reg_variant_block(const RepDev& r, int blk, float2* lds, int koff) {
+    if constexpr (last_reg_variant(R1, R2, TT, CH)) last_reg_block<R1, R2, NT, TT, CH>(r, blk, lds, koff);
+}
+
 template <int R1, int R2, int NT>
 __device__ __forceinline__ void last_reg_types(const RepDev& r, int blk, float2* lds, int koff) {
     const bool ch = r.nf > r.ntypes;
     switch (r.ntypes * 2 + (ch ? 1 : 0)) {
-        case 2: last_reg_block<R1, R2, NT, 1, 0>(r, blk, lds, koff); break;
-        case 3: last_reg_block<R1, R2, NT, 1, 1>(r, blk, lds, koff); break;
-        case 4: last_reg_block<R1, R2, NT, 2, 0>(r, blk, lds, koff); break;
-        case 5: last_reg_block<R1, R2, NT, 2, 1>(r, blk, lds, koff); break;
-        case 6: last_reg_block<R1, R2, NT, 3, 0>(r, blk, lds, koff); break;
-        case 7: last_reg_block<R1, R2, NT, 3, 1>(r, blk, lds, koff); break;
+        case 2: last_reg_variant_block<R1, R2, NT, 1, 0>(r, blk, lds, koff); break;
+        case 3: last_reg_variant_block<R1, R2, NT, 1, 1>(r, blk, lds, koff); break;
+        case 4: last_reg_variant_block<R1, R2, NT, 2, 0>(r, blk, lds, koff); break;
+        case 5: last_reg_variant_block<R1, R2, NT, 2, 1>(r, blk, lds, koff); break;
+        case 6: last_reg_variant_block<R1, R2, NT, 3, 0>(r, blk, lds, koff); break;
+        case 7: last_reg_variant_block<R1, R2, NT, 3, 1>(r, blk, lds, koff); break;
         default: break;   // excluded by last_reg_plan on the host
     }
 }
@@ -319,6 +331,129 @@ __global__ void __launch_bounds__(NT) k_grad_reg(const RepDev* __restrict__ reps
     }
 }
 
+// ------------------------------------------------------------------ plane forward (3D)
+// x R2C and y FFT of one (field, z) density plane (the operation of k_plane_r2c,
+// kernels.hip) for square planes Nx = Ny with a two-stage plan R1 x R2:
+//   A  task (pair j, i < R2): rows 2j and 2j+1 at x = i + R2 r as one complex line, DFT_R1;
+//   B  task (j, i < R1): twiddles, DFT_R2 -> Z_j(k), k = i + R1 q; exchange;
+//   C  task (column k < Nxh, i < R2): the half spectra of rows y = i + R2 r unpacked on
+//      the fly from Z_{y/2}(k) and Z_{y/2}(N - k), DFT_R1;
+//   D  task (k, i < R1): twiddles, DFT_R2 -> H(y = i + R1 q, k), stored.
+// Three barriers, where the staged pass has seven; loads and stores go straight to and
+// from registers.
+namespace {
+__device__ __forceinline__ float half_of(float a) {
+#pragma clang fp contract(off)
+    return 0.5f * a;
+}
+
+template <int R1, int R2, int NT>
+__device__ __forceinline__ void plane_reg_block(const RepDev& r, int blk, float2* lds) {
+    constexpr int N = R1 * R2, NP = N / 2, NXH = N / 2 + 1, ldp = NP | 1, ldc = NXH | 1;
+    constexpr int NT1 = R1 * (R2 - 1), LDA = (N * ldp > N * ldc) ? N * ldp : N * ldc;
+    const int z = blk % r.Nz, f = blk / r.Nz;
+    float2* Xa = lds;                  // x stage-1 output, later the y stage-1 output (N x ldc)
+    float2* Xb = Xa + LDA;             // Z: x transforms of the row pairs (N x ldp)
+    float2* TWx = Xb + N * ldp;        // w_N^{i r} of the x plan and of the y plan
+    float2* TWy = TWx + NT1;
+    {
+        const float2* tx = r.ax[0].tw + r.ax[0].tw_off[1];
+        const float2* ty = r.ax[1].tw + r.ax[1].tw_off[1];
+        for (int t = threadIdx.x; t < NT1; t += NT) { TWx[t] = gld(tx + t); TWy[t] = gld(ty + t); }
+    }
+    const uint32_t pbase = (uint32_t)f * (uint32_t)r.M + (uint32_t)z * (uint32_t)(N * N);
+    // A: pairs of rows as complex lines, stage 1 of the x plan
+    for (int task = threadIdx.x; task < NP * R2; task += NT) {
+        const int i = task % R2, j = task / R2;
+        float2 v[R1];
+#pragma unroll
+        for (int rr = 0; rr < R1; ++rr) {
+            const uint32_t o = pbase + (uint32_t)(2 * j * N + i + R2 * rr);
+            v[rr] = make_float2(G(at32((const float*)r.rho, o))[0], G(at32((const float*)r.rho, o + N))[0]);
+        }
+        Dft<R1>::run(v);
+#pragma unroll
+        for (int q = 0; q < R1; ++q) Xa[(i * R1 + q) * ldp + j] = v[q];
+    }
+    __syncthreads();
+    // B: stage 2 of the x plan -> Z_j(k)
+    for (int task = threadIdx.x; task < NP * R1; task += NT) {
+        const int i = task % R1, j = task / R1;
+        float2 v[R2];
+#pragma unroll
+        for (int rr = 0; rr < R2; ++rr) v[rr] = Xa[(i + R1 * rr) * ldp + j];
+#pragma unroll
+        for (int rr = 1; rr < R2; ++rr) v[rr] = cmul(v[rr], TWx[i * (R2 - 1) + rr - 1]);
+        Dft<R2>::run(v);
+#pragma unroll
+        for (int q = 0; q < R2; ++q) Xb[(i + R1 * q) * ldp + j] = v[q];
+    }
+    __syncthreads();
+    // C: unpack rows y = i + R2 r of column k (even y: the real row's spectrum
+    // (Z_k + conj Z_{N-k}) / 2, odd y: the imaginary row's (Z_k - conj Z_{N-k}) / 2i),
+    // stage 1 of the y plan
+    for (int task = threadIdx.x; task < NXH * R2; task += NT) {
+        const int k = task % NXH, i = task / NXH;
+        const int km = k == 0 ? 0 : N - k;
+        float2 v[R1];
+#pragma unroll
+        for (int rr = 0; rr < R1; ++rr) {
+            const int y = i + R2 * rr, j = y >> 1;
+            const float2 Zk = Xb[k * ldp + j], Zm = Xb[km * ldp + j];
+            v[rr] = (y & 1) == 0 ? make_float2(half_of(Zk.x + Zm.x), half_of(Zk.y - Zm.y))
+                                 : make_float2(half_of(Zk.y + Zm.y), half_of(Zm.x - Zk.x));
+        }
+        Dft<R1>::run(v);
+#pragma unroll
+        for (int q = 0; q < R1; ++q) Xa[(i * R1 + q) * ldc + k] = v[q];
+    }
+    __syncthreads();
+    // D: stage 2 of the y plan -> H(y, k)
+    const uint32_t hbase = ((uint32_t)f * (uint32_t)r.Nz + (uint32_t)z) * (uint32_t)(N * NXH);
+    for (int task = threadIdx.x; task < NXH * R1; task += NT) {
+        const int k = task % NXH, i = task / NXH;
+        float2 v[R2];
+#pragma unroll
+        for (int rr = 0; rr < R2; ++rr) v[rr] = Xa[(i + R1 * rr) * ldc + k];
+#pragma unroll
+        for (int rr = 1; rr < R2; ++rr) v[rr] = cmul(v[rr], TWy[i * (R2 - 1) + rr - 1]);
+        Dft<R2>::run(v);
+#pragma unroll
+        for (int q = 0; q < R2; ++q) gst(at32(r.H, hbase + (uint32_t)((i + R1 * q) * NXH + k)), v[q]);
+    }
+}
+}  // namespace
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_plane_reg(const RepDev* __restrict__ reps, const WorkItem* __restrict__ wmap) {
+    extern __shared__ float2 lds[];
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    const AxisPlan& pl = r.ax[0];
+    switch (pl.rad[0] * 32 + pl.rad[1]) {   // uniform per block
+        case 16 * 32 + 10: plane_reg_block<16, 10, NT>(r, wi.blk, lds); break;
+        case 10 * 32 + 8: plane_reg_block<10, 8, NT>(r, wi.blk, lds); break;
+        case 8 * 32 + 8: plane_reg_block<8, 8, NT>(r, wi.blk, lds); break;
+        case 8 * 32 + 5: plane_reg_block<8, 5, NT>(r, wi.blk, lds); break;
+        default: break;   // excluded by plane_reg_plan on the host
+    }
+}
+
+bool plane_reg_plan(const AxisPlan& px, const AxisPlan& py) {
+    return px.N == py.N && grad_reg_plan(px);
+}
+
+hipError_t launch_plane_reg(const LaunchCtx& c) {
+    if (!c.n) return hipSuccess;
+    TILD_LAUNCH(c, k_plane_reg<kPlaneRegThreads>, dim3(c.n), dim3(kPlaneRegThreads), c.lds, c.reps, c.wmap);
+    return hipGetLastError();
+}
+
+hipError_t configure_plane_reg() {
+    return hipFuncSetAttribute((const void*)k_plane_reg<kPlaneRegThreads>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)kPlaneLds);
+}
+
 bool grad_reg_plan(const AxisPlan& pl) {
     if (pl.nst != 2) return false;
     const int k = pl.rad[0] * 32 + pl.rad[1];
@@ -333,6 +468,7 @@ hipError_t launch_grad_reg(const LaunchCtx& c) {
 
 bool last_reg_plan(const AxisPlan& pl, int ntypes, int nf) {
     if (pl.nst != 2 || ntypes < 1 || ntypes > 3 || nf > ntypes + 1) return false;
+    if (!last_reg_variant(pl.rad[0], pl.rad[1], ntypes, nf - ntypes)) return false;
     const int k = pl.rad[0] * 32 + pl.rad[1];
     return k == 16 * 32 + 10 || k == 10 * 32 + 8 || k == 8 * 32 + 8 || k == 8 * 32 + 5;
 }

@@ -90,11 +90,16 @@ constexpr int kGradThreads = 256;                       // block of the y invers
 constexpr int kC2rThreads = 256;                        // block of the inverse x pass (records)
 constexpr size_t kPlaneLds = 163840;                    // LDS one block may use (MI355X)
 hipError_t launch_mid_grad(const LaunchCtx& c);
-// Fused last-axis pass with register-resident columns (last_reg.hip): last axes with a
+// Fused last-axis pass with register-resident columns (fft_reg.hip): last axes with a
 // two-stage plan R1 x R2 among 16 x 10, 10 x 8, 8 x 8 and 8 x 5, at most three types
 // (plus charges).  Blocks of kLastRegThreads threads hold nl_last = kLastRegThreads / R1
 // columns; results agree with k_last_fused to rounding.
 constexpr int kLastRegThreads = 128;
+// the variants compiled in (plan R1 x R2, types, charge field): those needing at most
+// 180 registers per thread, so that every variant runs two waves per SIMD
+__host__ __device__ constexpr bool last_reg_variant(int R1, int R2, int TT, int CH) {
+    return (R1 == 16 && R2 == 10) ? TT <= 2 : true;
+}
 bool last_reg_plan(const AxisPlan& pl, int ntypes, int nf);
 inline int last_reg_lines(const AxisPlan& pl) { return kLastRegThreads / pl.rad[0]; }
 // the two exchange buffers, both stages' twiddles (2N - R1 - R2) and the ramp weights
@@ -114,6 +119,16 @@ inline size_t grad_reg_lds_bytes(const AxisPlan& pl) {
     return ((size_t)pl.N * (3 * (size_t)grad_reg_lines(pl) + 1) + (size_t)pl.N) * sizeof(float2);
 }
 hipError_t launch_grad_reg(const LaunchCtx& c);
+// The forward plane pass with register-resident columns (square planes, two-stage
+// plans as above): one block of kPlaneRegThreads threads per (field, z) plane.
+constexpr int kPlaneRegThreads = 512;
+bool plane_reg_plan(const AxisPlan& px, const AxisPlan& py);
+inline size_t plane_reg_lds_bytes(const AxisPlan& pl) {
+    const size_t N = (size_t)pl.N, ldp = (size_t)((pl.N / 2) | 1), ldc = (size_t)((pl.N / 2 + 1) | 1);
+    return (N * (ldp > ldc ? ldp : ldc) + N * ldp + 2 * N) * sizeof(float2);
+}
+hipError_t launch_plane_reg(const LaunchCtx& c);
+hipError_t configure_plane_reg();
 hipError_t configure_last_reg();
 hipError_t launch_plane_r2c(const LaunchCtx& c);
 // LDS of the forward plane pass: two buffers holding either the Ny/2 row pairs of

@@ -111,10 +111,11 @@ typedef struct {
                             on it. */
     int deposit_threads; /* tiled spread: deposit block size, 256 or 512 (0 = by the
                             beads per tile).  Results do not depend on it. */
-    int fft_staged;      /* nonzero: the fused z pass and the y inverse stage every value
-                            through LDS even where their register-resident forms apply
-                            (two-stage plans 16x10, 10x8, 8x8, 8x5 of the axis; the z pass
-                            at most three types).  The forms agree to rounding (about 1e-7
+    int fft_staged;      /* nonzero: the forward plane pass, the fused z pass and the y
+                            inverse stage every value through LDS even where their
+                            register-resident forms apply (two-stage plans 16x10, 10x8,
+                            8x8, 8x5 of the axis; the plane pass square planes, the z
+                            pass at most three types).  The forms agree to rounding (about 1e-7
                             relative), not bit for bit: the compiler fuses multiplies into
                             additions where one keeps in registers what the other stages
                             through LDS. */
@@ -214,6 +215,7 @@ typedef struct {
     int last_lines, last_dirs, mid_elems, c2r_rows;   /* FFT line batching */
     int last_reg;        /* 1: register-resident fused z pass */
     int grad_reg;        /* 1: register-resident y inverse */
+    int plane_reg;       /* 1: register-resident forward plane pass */
     int tuned, from_cache;
     double step_ms;      /* kernel time per step measured by the tuner (0 if not tuned) */
 } tild_geometry;

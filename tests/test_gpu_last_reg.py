@@ -1,5 +1,6 @@
-"""The register-resident fused last-axis pass (k_last_reg) and y inverse (k_grad_reg)
-against the LDS-staged ones.
+"""The register-resident FFT passes -- the forward plane pass (k_plane_reg), the fused
+last-axis pass (k_last_reg) and the y inverse (k_grad_reg) -- against the LDS-staged
+ones.
 
 Both passes run the same transforms (forward plan R1 x R2, inverse plan R2 x R1) and
 the same coupling sums; the compiler fuses multiplies into additions differently where
@@ -7,8 +8,8 @@ one pass keeps in registers what the other stages through LDS, so the two agree 
 rounding (1e-6 relative) rather than bit for bit.  The oracle bounds the register pass
 at the usual tolerances, and in a batch each replica keeps the bits it has alone.  The decks cover the
 four two-stage plans the register pass takes (last axes of 160, 80, 64 and 40 points),
-one to three types, charges, ramped styles and 2D; the y inverse takes the same plans
-for y (3D only).
+one to three types, charges, ramped styles and 2D; the y inverse and the plane pass take
+the same plans for their axes (3D only; the plane pass on square x-y planes).
 """
 import pathlib
 
@@ -84,17 +85,30 @@ CASES = {
     "y80_z40": lambda t: decks.melt3d(t, grid=(10, 80, 40), box=(5.0, 20.0, 10.0), n_chains=80),
     "y64_z64_three": lambda t: three_types(t, grid=(9, 64, 64), box=(4.5, 16.0, 16.0)),
     "y40_z20_4types": lambda t: decks.charged(t, grid=(12, 40, 20)),   # four types: staged z pass
+    "x80_y40_z64": lambda t: decks.melt3d(t, grid=(80, 40, 64), box=(20.0, 10.0, 16.0), n_chains=100),
+    "x64_z40_charged": lambda t: charged_three(t, grid=(64, 12, 40), box=(16.0, 6.0, 20.0)),
+    "2d_x40_y40": lambda t: decks.melt2d(t, grid=(40, 40), box=(10.0, 10.0)),
+    "2d_x160_y16": lambda t: decks.melt2d(t, grid=(160, 16), box=(40.0, 8.0)),
+    # one register pass (with the y inverse) at a time
+    "plane_x40_y40_z20_4types": lambda t: decks.charged(t, grid=(40, 40, 20)),
+    "y40_x80_z20": lambda t: decks.melt3d(t, grid=(80, 40, 20), box=(20.0, 10.0, 5.0), n_chains=80),
+    "plane_x64_y64_z10": lambda t: decks.melt3d(t, grid=(64, 64, 10), box=(16.0, 16.0, 5.0), n_chains=100),
 }
 
-REG_LENGTHS = (160, 80, 64, 40)
+# two-stage plans the register passes take, and the variants compiled in
+PLANS = {160: (16, 10), 80: (10, 8), 64: (8, 8), 40: (8, 5)}
 
 
-def expected_flags(s):
+def expected_flags(s, g):
     nx = s.grid_shape[::-1]                     # (Nx, Ny[, Nz])
     nf = s.ntypes + (1 if s.info("do_charges") else 0)
-    last = nx[-1] in REG_LENGTHS and s.ntypes <= 3
-    grad = s.dim == 3 and nx[1] in REG_LENGTHS
-    return int(last), int(grad)
+    D = s.dim
+    last = nx[-1] in PLANS and s.ntypes <= 3 and not (PLANS[nx[-1]] == (16, 10) and s.ntypes > 2)
+    grad = D == 3 and nx[1] in PLANS
+    # the forward plane pass (where the replica runs one: tiled 3D, even Ny and tile
+    # height) on square x-y planes
+    plane = g["plane"] == 1 and nx[0] == nx[1] and nx[0] in PLANS
+    return int(last), int(grad), int(plane)
 
 
 @pytest.mark.parametrize("case", list(CASES))
@@ -105,9 +119,12 @@ def test_register_pass_equals_staged_pass(case, tmp_path):
     staged = tild.Engine(s, device=0, seed=5, fft_staged=True)
     gr, gs = reg.geometry(), staged.geometry()
     print(f"[{case}] ns={s.ns} grid={s.grid_shape} last_reg={gr['last_reg']} grad_reg={gr['grad_reg']} "
-          f"lines={gr['last_lines']}", flush=True)
-    assert (gr["last_reg"], gr["grad_reg"]) == expected_flags(s)
-    assert gs["last_reg"] == 0 and gs["grad_reg"] == 0
+          f"plane_reg={gr['plane_reg']}", flush=True)
+    flags = (gr["last_reg"], gr["grad_reg"], gr["plane_reg"])
+    assert flags == expected_flags(s, gr), (flags, gr)
+    if case.startswith("plane"):
+        assert gr["plane_reg"] == 1
+    assert gs["last_reg"] == 0 and gs["grad_reg"] == 0 and gs["plane_reg"] == 0
     fr, fs = reg.forces(), staged.forces()
     ef = U.rel_norm(fr, fs)
     print(f"  |f_reg - f_staged|/|f| = {ef:.2e}, bit-identical: {np.array_equal(fr, fs)}")
@@ -123,14 +140,16 @@ def test_register_pass_equals_staged_pass(case, tmp_path):
         e.close()
 
 
-@pytest.mark.parametrize("case", ["z160", "z64_three", "z40_charged", "2d_y80", "y80_z40", "y64_z64_three"])
+@pytest.mark.parametrize("case", ["z160", "z64_three", "z40_charged", "2d_y80", "y80_z40", "y64_z64_three",
+                                  "x80_y40_z64", "2d_x40_y40", "2d_x160_y16", "plane_x40_y40_z20_4types",
+                                  "plane_x64_y64_z10"])
 def test_register_pass_matches_oracle(case, tmp_path):
     text, d = CASES[case](tmp_path)
     s = tild.System(text, d)
     e = tild.Engine(s, device=0, seed=7)
     o = O.System(text, d, seed=7)
     g = e.geometry()
-    assert (g["last_reg"], g["grad_reg"]) == expected_flags(s)
+    assert (g["last_reg"], g["grad_reg"], g["plane_reg"]) == expected_flags(s, g)
     f, fo = e.forces(), o.forces()
     ef = U.rel_norm(f, fo)
     p, po = e.properties(), o.properties()
