@@ -2,9 +2,10 @@
 
 Headline workload: the large lamellar diblock melt -- 12,800 chains of N = 50 (640,000
 beads), box 40x40x80, grid 80x80x160, Gaussian + erf pair styles, GJF dt = 0.01, FP32
-particles and fields, FP64 property reductions.  One replica, its steps replayed from
-captured hipGraphs (runs of 10); the spatial reorders every 60 steps are folded into
-ms_per_step whether or not the timed window holds one.  Synthetic seeded random-walk
+particles and fields, FP64 property reductions.  One replica under the engine's launch
+policy (runs of 60 steps replayed from a captured hipGraph for small replicas, eager
+launches for larger ones such as this); the spatial reorders every 60 steps are folded
+into ms_per_step whether or not the timed window holds one.  Synthetic seeded random-walk
 chains (no datasets are available offline).
 
 Prints one JSON line: whole-job bead-steps/s, the roofline of the dominant kernel
@@ -13,7 +14,7 @@ on the engine's stream), the FP64 oracle timed on the box's own cores (one threa
 all the threads of the box's CPU share), and the FP32-vs-FP64 error of every quantity.
 Legs that ride along, one per benchmark configuration:
   C1  24,000-bead diblock melt, 40^3, EM
-  C2  the headline melt, also as the "plain" path (eager launches, no graph)
+  C2  the headline melt, also as the "plain" path (eager launches, never a graph)
   C3  81,000-bead coacervate, 64^3, charges + Gaussian-erf, EM, and a sweep of 8
       coacervate replicas batched vs plain
   C4  256 uneven diblock replicas advanced together vs one at a time
@@ -198,8 +199,9 @@ def time_oracle(o, steps, threads):
 
 
 def leg_engine(text, d, seed, steps, warmup, graph=True):
-    """One replica on its own: `steps` steps after `warmup`, graph-replayed (runs of 10)
-    or eager launches with no graph (the plain path), inputs resident before timing."""
+    """One replica on its own: `steps` steps after `warmup` under the engine's launch
+    policy (graph=True: captured runs of 60 steps for small replicas) or eager launches
+    with no graph (the plain path), inputs resident before timing."""
     import tild
     s = tild.System(text, d)
     b = tild.Batch(device=0)
@@ -278,8 +280,9 @@ def leg_headline(args, text, d):
 
 
 def leg_config(args, name, text, d, gjf, charged, steps, err_steps, threads, cpu_steps, cpu_steps_1t):
-    """A single-replica configuration: graph-replayed and plain bead-steps/s, the step's
-    HBM roofline, FP64 errors per quantity, and the oracle on 1 and all threads."""
+    """A single-replica configuration: bead-steps/s under the engine's launch policy and
+    with plain eager launches, the step's HBM roofline, FP64 errors per quantity, and the
+    oracle on 1 and all threads."""
     import tild
     sys.path.insert(0, str(REPO / "tests"))
     import oracle_ctypes as O
@@ -288,7 +291,7 @@ def leg_config(args, name, text, d, gjf, charged, steps, err_steps, threads, cpu
     v_plain, ms_plain = leg_engine(text, d, args.seed, steps, 10, graph=False)
     B = step_bytes_minimal(s.ns, s.dim, s.M, s.ntypes, gjf, charged)
     out = {"beads": s.ns, "grid": s.Nx[:s.dim], "steps": steps,
-           "graph": {"value": v_graph, "ms_per_step": ms_graph},
+           "engine": {"value": v_graph, "ms_per_step": ms_graph},
            "plain": {"value": v_plain, "ms_per_step": ms_plain},
            "step_bytes": B, "roofline_frac": B / (ms_graph * 1e-3) / 1e12 / 8.0, "unit": "bead-steps/s"}
     if not args.no_cpu_baseline:
@@ -333,9 +336,9 @@ def leg_c3_sweep(args, decks):
 
 
 def leg_sweep(args, decks):
-    """256 uneven replicas: all advanced together (one launch per phase, graph replay,
-    block geometry from the autotuner) versus each advanced alone with eager launches;
-    FP64 error of sampled replicas at their final positions."""
+    """256 uneven replicas: all advanced together (one launch per phase, block geometry
+    from the autotuner) versus each advanced alone with eager launches; the step's HBM
+    roofline over all replicas; FP64 error of sampled replicas at their final positions."""
     import numpy as np
     import tild
     systems = [tild.System(t, d) for t, d in decks]
@@ -384,10 +387,14 @@ def leg_sweep(args, decks):
             e.close()
     batched = beads * args.sweep_steps / tb
     plain = beads * args.sweep_plain_steps / tp
+    # minimal HBM bytes of one step of every replica (EM, uncharged)
+    step_bytes = sum(step_bytes_minimal(s.ns, s.dim, s.M, s.ntypes, False, False) for s in systems)
+    ms_step = tb / args.sweep_steps * 1e3
     return {"workload": f"{len(ids)} uneven diblock-melt replicas (grids {min(grids)}^3..{max(grids)}^3 at dx=0.5, "
                         f"N in 10..40, A_AB in [1,4], rho0=3, EM)",
             "replicas": len(ids), "beads_total": beads, "steps": args.sweep_steps,
-            "value": batched, "unit": "bead-steps/s", "ms_per_step": tb / args.sweep_steps * 1e3,
+            "value": batched, "unit": "bead-steps/s", "ms_per_step": ms_step,
+            "step_bytes": step_bytes, "roofline_frac": step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "plain_value": plain, "plain_steps": args.sweep_plain_steps, "speedup_vs_plain": batched / plain,
             "tuning": tuning, "fp64_error_final_state": errs}
 
@@ -581,7 +588,7 @@ def run(args):
     roof = dict(head["roofline"])
     roof["traffic"] = traffic
     configs["C2"] = {"beads": head["ns"], "grid": head["Nx"],
-                     "graph": {"value": head["value"], "ms_per_step": head["ms_per_step"]},
+                     "engine": {"value": head["value"], "ms_per_step": head["ms_per_step"]},
                      "plain": {"value": v_plain, "ms_per_step": ms_plain},
                      "roofline_frac": head["step_roofline"]["frac"], "fp64_error": c2_err}
     out = {
@@ -599,7 +606,8 @@ def run(args):
         "data": "synthetic",
         "config": {"workload": "lamellar diblock melt, 12800 x N=50 = 640000 beads, box 40x40x80, "
                                "grid 80x80x160, 3 gaussian + 3 erf styles, GJF dt=0.01, single replica, "
-                               "steps replayed from hipGraphs; spatial reorders every 60 steps amortised in",
+                               "engine launch policy (eager at this size); spatial reorders every 60 steps "
+                               "amortised in",
                    "beads": head["ns"], "grid": head["Nx"], "property_evaluations": head["props"],
                    "timed_ms_per_step": head["timed_ms_per_step"], "reorders_in_window": head["reorders_in_window"],
                    "reorder_ms": head["reorder_ms"], "Upe_after": head["Upe_end"]},

@@ -21,6 +21,7 @@ namespace tild {
 
 constexpr int kXcds = 8;         // accelerator complex dies of an MI355X
 constexpr int kGraphSteps = 60;  // integrating steps per graph launch: one spatial-reorder interval
+constexpr long long kGraphMaxWork = 32768;   // beads + grid nodes of a set above which steps launch eagerly
 constexpr int kProfPhases = 9;   // claim, deposit, forward plane, last_fused, y_inv, x_c2r, particles, advance, reorder
 
 // Work-map phases.  *A: replicas on the global-atomic spread; T*: replicas on the

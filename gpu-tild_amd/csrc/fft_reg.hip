@@ -16,13 +16,14 @@
 // two LDS exchanges and two barriers, where the LDS-staged form stages every value
 // through LDS and ends each of its nine phases with a barrier:
 //
-//   A  thread (b, i), i < R2: rows i + R2 r (r < R1) of column b of every field, read
-//      from HBM straight into registers; DFT_R1; exchange.
-//   B  thread (b, i), i < R1: twiddles w_N^{i r}, DFT_R2 -> rho_f(k) at k = i + R1 q;
-//      the coupling in registers; the inverse lines conj(w) and conj(-i k w) are
-//      transformed by the Stockham plan R2 x R1, whose first stage (radix R2, no
-//      twiddles) reads exactly the values k = i + R1 q this thread holds; exchange.
-//   C  thread (b, j), j < R2: twiddles w_N^{j r}, DFT_R1 -> rows j + R2 q; store.
+//   A  task (b, i < R2, field): rows i + R2 r (r < R1) of column b, read from HBM
+//      straight into registers; DFT_R1; exchange.
+//   B  thread (b, i < R1, h < 2): twiddles w_N^{i r}, DFT_R2 -> rho_f(k) at
+//      k = i + R1 q; the coupling in registers; inverse direction h (conj(w) or
+//      conj(-i k w)) is transformed by the Stockham plan R2 x R1, whose first stage
+//      (radix R2, no twiddles) reads exactly the values k = i + R1 q this thread holds;
+//      exchange.
+//   C  task (b, j < R2, line): twiddles w_N^{j r}, DFT_R1 -> rows j + R2 q; store.
 //
 // The transforms and sums are those of the LDS-staged pass (whose inverse also runs the
 // reversed plan RepDev::axr); the two agree to rounding, since the compiler contracts
@@ -48,13 +49,15 @@ __device__ __forceinline__ float2 grad_line(float k, float2 a) {
     return make_float2(k * a.y, k * a.x);
 }
 
-// One block: NB = NT / R1 adjacent columns q0 .. q0 + NB - 1 of the last-axis lines
-// (fewer in the last block), TT types and CH charge fields (NF = TT + CH fields).
+// One block: NB = NT / (2 R1) adjacent columns q0 .. q0 + NB - 1 of the last-axis lines
+// (fewer in the last block), TT types and CH charge fields (NF = TT + CH fields).  Two
+// threads share each (column, i) of the middle stage: both run the forward stage and
+// the coupling, and each transforms one of the two inverse directions.
 // LDS: the forward exchange (N x ldx), the inverse exchange (N x ldy), both stages'
 // twiddles, then the ramp weights (last_reg_lds_bytes).
 template <int R1, int R2, int NT, int TT, int CH>
 __device__ __forceinline__ void last_reg_block(const RepDev& r, int blk, float2* lds, int koff) {
-    constexpr int N = R1 * R2, NF = TT + CH, NR = TT * (TT + 1) / 2, NB = NT / R1;
+    constexpr int N = R1 * R2, NF = TT + CH, NR = TT * (TT + 1) / 2, NB = NT / (2 * R1);
     constexpr int ldx = NF * NB + 1, ldy = 2 * NF * NB + 1;
     constexpr int NT1 = R1 * (R2 - 1), NT2 = R2 * (R1 - 1);
     const int la = r.dim - 1, D = r.dim;
@@ -66,10 +69,11 @@ __device__ __forceinline__ void last_reg_block(const RepDev& r, int blk, float2*
     float2* TW1 = Y1 + N * ldy;                     // w_N^{i r}, i < R1, 1 <= r < R2 (forward stage 2)
     float2* TW2 = TW1 + NT1;                        // w_N^{j r}, j < R2, 1 <= r < R1 (inverse stage 2)
     float* rw = (float*)(TW2 + NT2);                // ramp weights of this step
-    const int b = (int)threadIdx.x % NB, i = (int)threadIdx.x / NB;
+    // phase B: thread (b, i, h), h = the inverse direction it transforms
+    const int b = (int)threadIdx.x % NB, i = ((int)threadIdx.x / NB) % R1, h = (int)threadIdx.x / (NB * R1);
     // grid buffers below 4 GiB (required on the host): 32-bit element offsets
-    const uint32_t Qu = (uint32_t)Q, Mhu = (uint32_t)r.Mh, col = (uint32_t)(q0 + b);
-    const bool liveB = i < R1 && b < nb;
+    const uint32_t Qu = (uint32_t)Q, Mhu = (uint32_t)r.Mh;
+    const bool liveB = h < 2 && b < nb;
 
     // set-up, visible after the first barrier: twiddles and ramp weights into LDS
     {
@@ -95,20 +99,18 @@ __device__ __forceinline__ void last_reg_block(const RepDev& r, int blk, float2*
         }
     }
 
-    // ---- A: rows i + R2 r of every field, DFT_R1 (Stockham stage 1: out[i R1 + q])
-    if (i < R2 && b < nb) {
-        float2 v[NF][R1];
+    // ---- A: task (b, i < R2, f): rows i + R2 r of field f, DFT_R1 (Stockham stage 1:
+    // out[i R1 + q])
+    for (int task = threadIdx.x; task < NB * R2 * NF; task += NT) {
+        const int ba = task % NB, ia = (task / NB) % R2, f = task / (NB * R2);
+        if (ba >= nb) continue;
+        float2 v[R1];
 #pragma unroll
-        for (int f = 0; f < NF; ++f)
+        for (int rr = 0; rr < R1; ++rr)
+            v[rr] = gld(at32((const float2*)r.H, (uint32_t)f * Mhu + (uint32_t)(ia + R2 * rr) * Qu + (uint32_t)(q0 + ba)));
+        Dft<R1>::run(v);
 #pragma unroll
-            for (int rr = 0; rr < R1; ++rr)
-                v[f][rr] = gld(at32((const float2*)r.H, (uint32_t)f * Mhu + (uint32_t)(i + R2 * rr) * Qu + col));
-#pragma unroll
-        for (int f = 0; f < NF; ++f) {
-            Dft<R1>::run(v[f]);
-#pragma unroll
-            for (int q = 0; q < R1; ++q) X1[(i * R1 + q) * ldx + f * NB + b] = v[f][q];
-        }
+        for (int q = 0; q < R1; ++q) X1[(ia * R1 + q) * ldx + f * NB + ba] = v[q];
     }
     __syncthreads();
 
@@ -155,28 +157,22 @@ __device__ __forceinline__ void last_reg_block(const RepDev& r, int blk, float2*
             }
             if (CH) w[TT][q] = scaled(cq[q], rh[TT]);
         }
-        // inverse lines (the inverse is conj(FFT(conj))): o = 0 conj(w) = (w.x, -w.y),
-        // o = 1 conj(-i k_la w) = (k w.y, k w.x), k_la = 0 on the Nyquist plane;
-        // first stage of the R2 x R1 plan: radix R2, out[i R2 + q]
+        // this thread's inverse lines (the inverse is conj(FFT(conj))): h = 0
+        // conj(w) = (w.x, -w.y), h = 1 conj(-i k_la w) = (k w.y, k w.x), k_la = 0 on the
+        // Nyquist plane; first stage of the R2 x R1 plan: radix R2, out[i R2 + q]
 #pragma unroll
-        for (int o = 0; o < 2; ++o) {
+        for (int f = 0; f < NF; ++f) {
+            float2 z[R2];
 #pragma unroll
-            for (int f = 0; f < NF; ++f) {
-                float2 z[R2];
-#pragma unroll
-                for (int q = 0; q < R2; ++q) {
-                    const int n = i + R1 * q;
-                    const float2 a = w[f][q];
-                    if (o == 0) z[q] = make_float2(a.x, -a.y);
-                    else {
-                        const float kl = 2 * n == N ? 0.f : r.kfac[la] * (float)(2 * n < N ? n : n - N);
-                        z[q] = grad_line(kl, a);
-                    }
-                }
-                Dft<R2>::run(z);
-#pragma unroll
-                for (int q = 0; q < R2; ++q) Y1[(i * R2 + q) * ldy + (o * NF + f) * NB + b] = z[q];
+            for (int q = 0; q < R2; ++q) {
+                const int n = i + R1 * q;
+                const float2 a = w[f][q];
+                const float kl = 2 * n == N ? 0.f : r.kfac[la] * (float)(2 * n < N ? n : n - N);
+                z[q] = h == 0 ? make_float2(a.x, -a.y) : grad_line(kl, a);
             }
+            Dft<R2>::run(z);
+#pragma unroll
+            for (int q = 0; q < R2; ++q) Y1[(i * R2 + q) * ldy + (h * NF + f) * NB + b] = z[q];
         }
     }
     __syncthreads();

@@ -92,16 +92,16 @@ constexpr size_t kPlaneLds = 163840;                    // LDS one block may use
 hipError_t launch_mid_grad(const LaunchCtx& c);
 // Fused last-axis pass with register-resident columns (fft_reg.hip): last axes with a
 // two-stage plan R1 x R2 among 16 x 10, 10 x 8, 8 x 8 and 8 x 5, at most three types
-// (plus charges).  Blocks of kLastRegThreads threads hold nl_last = kLastRegThreads / R1
+// (plus charges).  Blocks of kLastRegThreads threads hold nl_last = kLastRegThreads / (2 R1)
 // columns; results agree with k_last_fused to rounding.
-constexpr int kLastRegThreads = 128;
+constexpr int kLastRegThreads = 256;
 // the variants compiled in (plan R1 x R2, types, charge field): those needing at most
 // 180 registers per thread, so that every variant runs two waves per SIMD
 __host__ __device__ constexpr bool last_reg_variant(int R1, int R2, int TT, int CH) {
     return (R1 == 16 && R2 == 10) ? TT <= 2 : true;
 }
 bool last_reg_plan(const AxisPlan& pl, int ntypes, int nf);
-inline int last_reg_lines(const AxisPlan& pl) { return kLastRegThreads / pl.rad[0]; }
+inline int last_reg_lines(const AxisPlan& pl) { return kLastRegThreads / (2 * pl.rad[0]); }
 // the two exchange buffers, both stages' twiddles (2N - R1 - R2) and the ramp weights
 inline size_t last_reg_lds_bytes(const AxisPlan& pl, int nf) {
     const size_t N = (size_t)pl.N, nl = (size_t)last_reg_lines(pl);

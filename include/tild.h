@@ -146,8 +146,9 @@ const char* tild_engine_last_error(const tild_engine* e);
 tild_status tild_engine_forces(tild_engine* e);
 
 /* n iterations of the main-loop body (main.cu:106-117): step += 1, forces(x_n),
-   Integrate_2 for every integrator (integrator.cu:24-39) -> x_{n+1}.  For n >= 2 the
-   step is captured once in a hipGraph and replayed. */
+   Integrate_2 for every integrator (integrator.cu:24-39) -> x_{n+1}.  Small replicas
+   (beads + grid nodes up to 32,768) replay runs of 60 steps from a captured hipGraph;
+   larger ones, whose kernels outlast their launches, launch eagerly. */
 tild_status tild_engine_step(tild_engine* e, int n);
 
 /* calc_properties() after the last forces()/step: non-bonded energies and virial
@@ -237,8 +238,8 @@ tild_status tild_engine_sk_average(tild_engine* e, int slot, float* sk, size_t n
 /* Many independent replicas (uneven boxes, grids, chain counts, pair styles) resident
    on one device.  tild_batch_step advances a set of them together: every per-step
    kernel is launched once for the whole set (blocks map to (replica, tile) work
-   items), and with use_graph the step is captured once per set in a hipGraph and
-   replayed.  The reference has no counterpart (one system per process,
+   items), and with use_graph sets of up to 32,768 beads + grid nodes replay runs of
+   60 steps captured once per set in a hipGraph (larger sets launch eagerly).  The reference has no counterpart (one system per process,
    main.cu:36-227); each replica follows the same per-step semantics as tild_engine. */
 typedef struct tild_batch tild_batch;
 tild_status tild_batch_create(const tild_opts* opts, tild_batch** out);
@@ -274,7 +275,7 @@ typedef struct {
     int device;
     int chunk_steps;      /* steps between admission points; 0 = 100 */
     int max_replicas;     /* running-job cap; 0 = budget only */
-    int use_graph;        /* replay one captured step per running set */
+    int use_graph;        /* replay captured runs of steps for small running sets */
     size_t hbm_budget;    /* bytes of device memory for jobs; 0 = 90% of free memory */
     int step_work;        /* 0: every running job advances in every chunk.  > 0: each chunk
                              advances the running jobs with the least remaining work
