@@ -5,8 +5,7 @@
 // a thread keeps its columns in registers between the stages, so a pass makes one to
 // three LDS exchanges where the LDS-staged forms (kernels.hip) stage every value through
 // LDS with a barrier after each phase.  (The inverse x transform into the gather
-// records stays LDS-staged: its register form reads each half-spectrum value twice,
-// for column k and its Hermitian partner N - k, and measured slower.)
+// records stays LDS-staged: its register forms measured slower, see the design notes.)
 //
 // k_last_reg: the same operation as k_last_fused (kernels.hip) -- last-axis forward FFT of every
 // field, the k-space coupling w_I = sum_J C_IJ rho_J (+ the charge row), and the
