@@ -688,7 +688,8 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
             add(PH_MIDZ_H, gz.nouter * gz.nchunk, lds_bytes(gz.N, gz.nl));
         }
         const int la = D - 1, Nla = la == 2 ? h.Nz : h.Ny, Q = la == 2 ? h.Ny * h.Nxh : h.Nxh;
-        if (h.last_reg) add(PH_LASTR, (Q + h.nl_last - 1) / h.nl_last, last_reg_lds_bytes(h.ax[la], h.nf));
+        if (h.last_reg) add(last_reg_class(h.ntypes) ? PH_LASTR4 : PH_LASTR, (Q + h.nl_last - 1) / h.nl_last,
+                            last_reg_lds_bytes(h.ax[la], h.nf));
         else add(PH_LAST, (Q + h.nl_last - 1) / h.nl_last, last_lds_bytes(Nla, h.nf, h.last_dirs, h.nl_last));
         const int nc = (h.nf * D + 1) / 2;
         add(PH_C2R, (rows + h.nr_c2r - 1) / h.nr_c2r, lds_bytes(h.Nx, h.nr_c2r * nc));
@@ -714,7 +715,7 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
     // records; for the FFT passes adjacent column batches, whose rows share cache lines
     // that a single L2 then fetches once.
     for (Phase ph : {PH_PART, PH_TCOUNT, PH_SPREADA, PH_TILE, PH_XA, PH_XP, PH_MIDY_H, PH_MIDY_G, PH_MIDY_GR,
-                     PH_MIDZ_H, PH_PY, PH_LAST, PH_LASTR, PH_C2R, PH_PLANEF, PH_PLANER}) {
+                     PH_MIDZ_H, PH_PY, PH_LAST, PH_LASTR, PH_LASTR4, PH_C2R, PH_PLANEF, PH_PLANER}) {
         std::vector<WorkItem>& v = maps[ph].host;
         const size_t n = v.size(), q = n / kXcds, rm = n % kXcds;
         if (n < 2 * (size_t)kXcds) continue;
@@ -727,7 +728,7 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
     }
     for (int p = 0; p < PH_COUNT; ++p) {
         maps[p].n = (int)maps[p].host.size();
-        if (maps[p].lds > ((p == PH_PLANEF || p == PH_PLANER || p == PH_LAST || p == PH_LASTR) ? kPlaneLds : (size_t)kMaxLds)) { err = "pass needs more LDS than a block may use"; return -TILD_ERR_UNSUPPORTED; }
+        if (maps[p].lds > ((p == PH_PLANEF || p == PH_PLANER || p == PH_LAST || p == PH_LASTR || p == PH_LASTR4) ? kPlaneLds : (size_t)kMaxLds)) { err = "pass needs more LDS than a block may use"; return -TILD_ERR_UNSUPPORTED; }
         if (maps[p].dev && (!reorder_only || maps[p].cap < maps[p].n)) {
             // reused buffers are last read by the previous reorder's launches
             HIPCHK(hipStreamSynchronize(stream_));
@@ -1042,7 +1043,8 @@ int Batch::enqueue_steps(SetMaps& m, int nsteps, bool integrate, bool write_forc
         HIPCHK(launch_plane_reg(ctx(PH_PLANER)));
         HIPCHK(launch_mid(ctx(PH_MIDY_H), 1, 0, 0, 0, nullptr));
         HIPCHK(launch_last_fused(ctx(PH_LAST), 0, k));
-        HIPCHK(launch_last_reg(ctx(PH_LASTR), k));
+        HIPCHK(launch_last_reg(ctx(PH_LASTR), k, 0));
+        HIPCHK(launch_last_reg(ctx(PH_LASTR4), k, 1));
         HIPCHK(launch_mid_grad(ctx(PH_MIDY_G)));
         HIPCHK(launch_grad_reg(ctx(PH_MIDY_GR)));
         HIPCHK(launch_x_c2r(ctx(PH_C2R), true, 0, nullptr, nullptr));
@@ -1183,7 +1185,8 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
         ok(launch_plane_reg(ctx(PH_PLANER, 2, 2)));
         ok(launch_mid(ctx(PH_MIDY_H, 2, 1), 1, 0, 0, 0, nullptr));
         ok(launch_last_fused(ctx(PH_LAST, 3, 0), 0, 0));
-        ok(launch_last_reg(ctx(PH_LASTR, 3, 1), 0));
+        ok(launch_last_reg(ctx(PH_LASTR, 3, 1), 0, 0));
+        ok(launch_last_reg(ctx(PH_LASTR4, 3, 2), 0, 1));
         ok(launch_mid_grad(ctx(PH_MIDY_G, 4, 0)));
         ok(launch_grad_reg(ctx(PH_MIDY_GR, 4, 1)));
         ok(launch_x_c2r(ctx(PH_C2R, 5, 0), true, 0, nullptr, nullptr));

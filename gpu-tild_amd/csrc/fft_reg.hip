@@ -88,14 +88,21 @@ __device__ __forceinline__ void last_reg_block(const RepDev& r, int blk, float2*
     // uniform table base (every offset 32-bit)
     const uint32_t cb0 = (uint32_t)blk * (uint32_t)(N * nrows * NB) + (uint32_t)b;
     auto ctab = [&](int n, int row) { return G(at32(r.Ctab, cb0 + (uint32_t)((n * nrows + row) * NB)))[0]; };
-    float c[R2][NR], cq[R2];
-    if (liveB) {
+    // up to three types every row of the thread's nodes is requested at block start
+    // and held through the forward stages; with four (ten rows per node) each node's
+    // rows are requested where the coupling uses them, which keeps the pass within 256
+    // registers
+    constexpr bool kEarly = NR <= 6;
+    constexpr int QE = kEarly ? R2 : 1;
+    float c[QE][NR], cq[QE];
+    auto load_rows = [&](int q, int slot) {
 #pragma unroll
-        for (int q = 0; q < R2; ++q) {
+        for (int k = 0; k < NR; ++k) c[slot][k] = ctab(i + R1 * q, k);
+        cq[slot] = CH ? ctab(i + R1 * q, r.ctab_charge) : 0.f;
+    };
+    if (kEarly && liveB) {
 #pragma unroll
-            for (int k = 0; k < NR; ++k) c[q][k] = ctab(i + R1 * q, k);
-            cq[q] = CH ? ctab(i + R1 * q, r.ctab_charge) : 0.f;
-        }
+        for (int q = 0; q < R2; ++q) load_rows(q, q < QE ? q : 0);
     }
 
     // ---- A: task (b, i < R2, f): rows i + R2 r of field f, DFT_R1 (Stockham stage 1:
@@ -135,10 +142,12 @@ __device__ __forceinline__ void last_reg_block(const RepDev& r, int blk, float2*
 #pragma unroll
         for (int q = 0; q < R2; ++q) {
             const int n = i + R1 * q;
+            const int cs = kEarly ? q : 0;     // slot of this node's rows
+            if (!kEarly) load_rows(q, 0);
             for (int j = 0; j < r.nramp; ++j) {   // uniform trip count
                 const float v = rw[j] * ctab(n, r.ramp_row0 + j);
 #pragma unroll
-                for (int k = 0; k < NR; ++k) c[q][k] += r.ramp_pair[j] == k ? v : 0.f;
+                for (int k = 0; k < NR; ++k) c[cs][k] += r.ramp_pair[j] == k ? v : 0.f;
             }
             float2 rh[NF];
 #pragma unroll
@@ -149,12 +158,12 @@ __device__ __forceinline__ void last_reg_block(const RepDev& r, int blk, float2*
 #pragma unroll
                 for (int J = 0; J < TT; ++J) {
                     const int lo = I < J ? I : J, hi = I < J ? J : I;
-                    const float cc = c[q][lo * TT - lo * (lo - 1) / 2 + (hi - lo)];
+                    const float cc = c[cs][lo * TT - lo * (lo - 1) / 2 + (hi - lo)];
                     acc.x += cc * rh[J].x; acc.y += cc * rh[J].y;
                 }
                 w[I][q] = acc;
             }
-            if (CH) w[TT][q] = scaled(cq[q], rh[TT]);
+            if (CH) w[TT][q] = scaled(cq[cs], rh[TT]);
         }
         // this thread's inverse lines (the inverse is conj(FFT(conj))): h = 0
         // conj(w) = (w.x, -w.y), h = 1 conj(-i k_la w) = (k w.y, k w.x), k_la = 0 on the
@@ -196,28 +205,33 @@ __device__ __forceinline__ void last_reg_block(const RepDev& r, int blk, float2*
     }
 }
 
-template <int R1, int R2, int NT, int TT, int CH>
+template <int R1, int R2, int NT, int TT, int CH, int CLS>
 __device__ __forceinline__ void last_reg_variant_block(const RepDev& r, int blk, float2* lds, int koff) {
-    if constexpr (last_reg_variant(R1, R2, TT, CH)) last_reg_block<R1, R2, NT, TT, CH>(r, blk, lds, koff);
+    if constexpr (last_reg_variant(R1, R2, TT, CH) && last_reg_class(TT) == CLS)
+        last_reg_block<R1, R2, NT, TT, CH>(r, blk, lds, koff);
 }
 
-template <int R1, int R2, int NT>
+template <int R1, int R2, int NT, int CLS>
 __device__ __forceinline__ void last_reg_types(const RepDev& r, int blk, float2* lds, int koff) {
     const bool ch = r.nf > r.ntypes;
     switch (r.ntypes * 2 + (ch ? 1 : 0)) {
-        case 2: last_reg_variant_block<R1, R2, NT, 1, 0>(r, blk, lds, koff); break;
-        case 3: last_reg_variant_block<R1, R2, NT, 1, 1>(r, blk, lds, koff); break;
-        case 4: last_reg_variant_block<R1, R2, NT, 2, 0>(r, blk, lds, koff); break;
-        case 5: last_reg_variant_block<R1, R2, NT, 2, 1>(r, blk, lds, koff); break;
-        case 6: last_reg_variant_block<R1, R2, NT, 3, 0>(r, blk, lds, koff); break;
-        case 7: last_reg_variant_block<R1, R2, NT, 3, 1>(r, blk, lds, koff); break;
+        case 2: last_reg_variant_block<R1, R2, NT, 1, 0, CLS>(r, blk, lds, koff); break;
+        case 3: last_reg_variant_block<R1, R2, NT, 1, 1, CLS>(r, blk, lds, koff); break;
+        case 4: last_reg_variant_block<R1, R2, NT, 2, 0, CLS>(r, blk, lds, koff); break;
+        case 5: last_reg_variant_block<R1, R2, NT, 2, 1, CLS>(r, blk, lds, koff); break;
+        case 6: last_reg_variant_block<R1, R2, NT, 3, 0, CLS>(r, blk, lds, koff); break;
+        case 7: last_reg_variant_block<R1, R2, NT, 3, 1, CLS>(r, blk, lds, koff); break;
+        case 8: last_reg_variant_block<R1, R2, NT, 4, 0, CLS>(r, blk, lds, koff); break;
+        case 9: last_reg_variant_block<R1, R2, NT, 4, 1, CLS>(r, blk, lds, koff); break;
         default: break;   // excluded by last_reg_plan on the host
     }
 }
 
 }  // namespace
 
-template <int NT>
+// CLS: the register class of the type counts the instantiation holds (last_reg_class),
+// so that the four-type variants' registers do not lower the others' occupancy
+template <int NT, int CLS>
 __global__ void __launch_bounds__(NT) k_last_reg(const RepDev* __restrict__ reps, const WorkItem* __restrict__ wmap,
                                                  int koff) {
     extern __shared__ float2 lds[];
@@ -225,10 +239,10 @@ __global__ void __launch_bounds__(NT) k_last_reg(const RepDev* __restrict__ reps
     const RepDev& r = reps[wi.rep];
     const AxisPlan& pl = r.ax[r.dim - 1];
     switch (pl.rad[0] * 32 + pl.rad[1]) {   // uniform per block
-        case 16 * 32 + 10: last_reg_types<16, 10, NT>(r, wi.blk, lds, koff); break;
-        case 10 * 32 + 8: last_reg_types<10, 8, NT>(r, wi.blk, lds, koff); break;
-        case 8 * 32 + 8: last_reg_types<8, 8, NT>(r, wi.blk, lds, koff); break;
-        case 8 * 32 + 5: last_reg_types<8, 5, NT>(r, wi.blk, lds, koff); break;
+        case 16 * 32 + 10: last_reg_types<16, 10, NT, CLS>(r, wi.blk, lds, koff); break;
+        case 10 * 32 + 8: last_reg_types<10, 8, NT, CLS>(r, wi.blk, lds, koff); break;
+        case 8 * 32 + 8: last_reg_types<8, 8, NT, CLS>(r, wi.blk, lds, koff); break;
+        case 8 * 32 + 5: last_reg_types<8, 5, NT, CLS>(r, wi.blk, lds, koff); break;
         default: break;   // excluded by last_reg_plan on the host
     }
 }
@@ -462,20 +476,24 @@ hipError_t launch_grad_reg(const LaunchCtx& c) {
 }
 
 bool last_reg_plan(const AxisPlan& pl, int ntypes, int nf) {
-    if (pl.nst != 2 || ntypes < 1 || ntypes > 3 || nf > ntypes + 1) return false;
+    if (pl.nst != 2 || ntypes < 1 || ntypes > 4 || nf > ntypes + 1) return false;
     if (!last_reg_variant(pl.rad[0], pl.rad[1], ntypes, nf - ntypes)) return false;
     const int k = pl.rad[0] * 32 + pl.rad[1];
     return k == 16 * 32 + 10 || k == 10 * 32 + 8 || k == 8 * 32 + 8 || k == 8 * 32 + 5;
 }
 
-hipError_t launch_last_reg(const LaunchCtx& c, int koff) {
+hipError_t launch_last_reg(const LaunchCtx& c, int koff, int cls) {
     if (!c.n) return hipSuccess;
-    TILD_LAUNCH(c, k_last_reg<kLastRegThreads>, dim3(c.n), dim3(kLastRegThreads), c.lds, c.reps, c.wmap, koff);
+    if (cls == 1) TILD_LAUNCH(c, (k_last_reg<kLastRegThreads, 1>), dim3(c.n), dim3(kLastRegThreads), c.lds, c.reps, c.wmap, koff);
+    else TILD_LAUNCH(c, (k_last_reg<kLastRegThreads, 0>), dim3(c.n), dim3(kLastRegThreads), c.lds, c.reps, c.wmap, koff);
     return hipGetLastError();
 }
 
 hipError_t configure_last_reg() {
-    return hipFuncSetAttribute((const void*)k_last_reg<kLastRegThreads>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    hipError_t e = hipFuncSetAttribute((const void*)k_last_reg<kLastRegThreads, 0>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPlaneLds);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void*)k_last_reg<kLastRegThreads, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)kPlaneLds);
 }
 

@@ -91,15 +91,19 @@ constexpr int kC2rThreads = 256;                        // block of the inverse 
 constexpr size_t kPlaneLds = 163840;                    // LDS one block may use (MI355X)
 hipError_t launch_mid_grad(const LaunchCtx& c);
 // Fused last-axis pass with register-resident columns (fft_reg.hip): last axes with a
-// two-stage plan R1 x R2 among 16 x 10, 10 x 8, 8 x 8 and 8 x 5, at most three types
-// (plus charges).  Blocks of kLastRegThreads threads hold nl_last = kLastRegThreads / (2 R1)
+// two-stage plan R1 x R2 among 16 x 10, 10 x 8, 8 x 8 and 8 x 5, at most four types
+// (two on 160-point axes) plus charges.  Blocks of kLastRegThreads threads hold nl_last = kLastRegThreads / (2 R1)
 // columns; results agree with k_last_fused to rounding.
 constexpr int kLastRegThreads = 256;
-// the variants compiled in (plan R1 x R2, types, charge field): those needing at most
-// 180 registers per thread, so that every variant runs two waves per SIMD
+// the variants compiled in (plan R1 x R2, types, charge field): those that fit 256
+// registers per thread, so that every variant runs two waves per SIMD (160-point axes
+// up to two types)
 __host__ __device__ constexpr bool last_reg_variant(int R1, int R2, int TT, int CH) {
-    return (R1 == 16 && R2 == 10) ? TT <= 2 : true;
+    return (R1 == 16 && R2 == 10) ? TT <= 2 : TT <= 4;
 }
+// the kernel instantiation of a type count: up to three types (class 0), four (class 1,
+// up to 256 registers, in a kernel of its own so that the others keep theirs)
+__host__ __device__ constexpr int last_reg_class(int TT) { return TT >= 4 ? 1 : 0; }
 bool last_reg_plan(const AxisPlan& pl, int ntypes, int nf);
 inline int last_reg_lines(const AxisPlan& pl) { return kLastRegThreads / (2 * pl.rad[0]); }
 // the two exchange buffers, both stages' twiddles (2N - R1 - R2) and the ramp weights
@@ -107,7 +111,7 @@ inline size_t last_reg_lds_bytes(const AxisPlan& pl, int nf) {
     const size_t N = (size_t)pl.N, nl = (size_t)last_reg_lines(pl);
     return (N * (3 * (size_t)nf * nl + 2) + 2 * N) * sizeof(float2) + kMaxRamp * sizeof(float);
 }
-hipError_t launch_last_reg(const LaunchCtx& c, int koff);
+hipError_t launch_last_reg(const LaunchCtx& c, int koff, int cls);
 // The y inverse of the gradient spectra with register-resident columns (3D, two-stage
 // y plans as above): blocks of kGradRegThreads threads, grad_reg_lines(plan) columns
 // each, grad_reg_chunks per (field, z) plane.

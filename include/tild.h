@@ -115,7 +115,7 @@ typedef struct {
                             inverse stage every value through LDS even where their
                             register-resident forms apply (two-stage plans 16x10, 10x8,
                             8x8, 8x5 of the axis; the plane pass square planes, the z
-                            pass at most three types).  The forms agree to rounding (about 1e-7
+                            pass at most four types, two on a 160-point axis).  The forms agree to rounding (about 1e-7
                             relative), not bit for bit: the compiler fuses multiplies into
                             additions where one keeps in registers what the other stages
                             through LDS. */

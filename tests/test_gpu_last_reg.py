@@ -8,7 +8,7 @@ one pass keeps in registers what the other stages through LDS, so the two agree 
 rounding (1e-6 relative) rather than bit for bit.  The oracle bounds the register pass
 at the usual tolerances, and in a batch each replica keeps the bits it has alone.  The decks cover the
 four two-stage plans the register pass takes (last axes of 160, 80, 64 and 40 points),
-one to three types, charges, ramped styles and 2D; the y inverse and the plane pass take
+one to four types, charges, ramped styles and 2D; the y inverse and the plane pass take
 the same plans for their axes (3D only; the plane pass on square x-y planes).
 """
 import pathlib
@@ -84,7 +84,8 @@ CASES = {
     "2d_y40": lambda t: decks.melt2d(t, grid=(15, 40), box=(6.0, 10.0)),
     "y80_z40": lambda t: decks.melt3d(t, grid=(10, 80, 40), box=(5.0, 20.0, 10.0), n_chains=80),
     "y64_z64_three": lambda t: three_types(t, grid=(9, 64, 64), box=(4.5, 16.0, 16.0)),
-    "y40_z20_4types": lambda t: decks.charged(t, grid=(12, 40, 20)),   # four types: staged z pass
+    "y40_z20_4types": lambda t: decks.charged(t, grid=(12, 40, 20)),   # z = 20: staged z pass
+    "z64_4types_charged": lambda t: decks.charged(t, grid=(12, 12, 64)),
     "x80_y40_z64": lambda t: decks.melt3d(t, grid=(80, 40, 64), box=(20.0, 10.0, 16.0), n_chains=100),
     "x64_z40_charged": lambda t: charged_three(t, grid=(64, 12, 40), box=(16.0, 6.0, 20.0)),
     "2d_x40_y40": lambda t: decks.melt2d(t, grid=(40, 40), box=(10.0, 10.0)),
@@ -103,7 +104,7 @@ def expected_flags(s, g):
     nx = s.grid_shape[::-1]                     # (Nx, Ny[, Nz])
     nf = s.ntypes + (1 if s.info("do_charges") else 0)
     D = s.dim
-    last = nx[-1] in PLANS and s.ntypes <= 3 and not (PLANS[nx[-1]] == (16, 10) and s.ntypes > 2)
+    last = nx[-1] in PLANS and s.ntypes <= (2 if PLANS[nx[-1]] == (16, 10) else 4)
     grad = D == 3 and nx[1] in PLANS
     # the forward plane pass (where the replica runs one: tiled 3D, even Ny and tile
     # height) on square x-y planes
@@ -140,7 +141,8 @@ def test_register_pass_equals_staged_pass(case, tmp_path):
         e.close()
 
 
-@pytest.mark.parametrize("case", ["z160", "z64_three", "z40_charged", "2d_y80", "y80_z40", "y64_z64_three",
+@pytest.mark.parametrize("case", ["z160", "z64_three", "z40_charged", "z64_4types_charged", "2d_y80", "y80_z40",
+                                  "y64_z64_three",
                                   "x80_y40_z64", "2d_x40_y40", "2d_x160_y16", "plane_x40_y40_z20_4types",
                                   "plane_x64_y64_z10"])
 def test_register_pass_matches_oracle(case, tmp_path):
