@@ -36,6 +36,15 @@ namespace tild {
 
 namespace {
 
+bool reg_plan_pair(int k) {   // k = R1 * 32 + R2
+    switch (k) {
+#define TILD_CASE(A, B) case A * 32 + B: return true;
+        TILD_REG_PLANS(TILD_CASE)
+#undef TILD_CASE
+        default: return false;
+    }
+}
+
 
 // Products that the LDS-staged pass writes to LDS before its next stage adds them:
 // kept out of FMA contraction with those additions, as they are there.
@@ -239,10 +248,9 @@ __global__ void __launch_bounds__(NT) k_last_reg(const RepDev* __restrict__ reps
     const RepDev& r = reps[wi.rep];
     const AxisPlan& pl = r.ax[r.dim - 1];
     switch (pl.rad[0] * 32 + pl.rad[1]) {   // uniform per block
-        case 16 * 32 + 10: last_reg_types<16, 10, NT, CLS>(r, wi.blk, lds, koff); break;
-        case 10 * 32 + 8: last_reg_types<10, 8, NT, CLS>(r, wi.blk, lds, koff); break;
-        case 8 * 32 + 8: last_reg_types<8, 8, NT, CLS>(r, wi.blk, lds, koff); break;
-        case 8 * 32 + 5: last_reg_types<8, 5, NT, CLS>(r, wi.blk, lds, koff); break;
+#define TILD_CASE(A, B) case A * 32 + B: last_reg_types<A, B, NT, CLS>(r, wi.blk, lds, koff); break;
+        TILD_REG_PLANS(TILD_CASE)
+#undef TILD_CASE
         default: break;   // excluded by last_reg_plan on the host
     }
 }
@@ -332,10 +340,9 @@ __global__ void __launch_bounds__(NT) k_grad_reg(const RepDev* __restrict__ reps
     const RepDev& r = reps[wi.rep];
     const AxisPlan& pl = r.ax[1];
     switch (pl.rad[0] * 32 + pl.rad[1]) {   // uniform per block
-        case 16 * 32 + 10: grad_reg_block<16, 10, NT>(r, wi.blk, lds); break;
-        case 10 * 32 + 8: grad_reg_block<10, 8, NT>(r, wi.blk, lds); break;
-        case 8 * 32 + 8: grad_reg_block<8, 8, NT>(r, wi.blk, lds); break;
-        case 8 * 32 + 5: grad_reg_block<8, 5, NT>(r, wi.blk, lds); break;
+#define TILD_CASE(A, B) case A * 32 + B: grad_reg_block<A, B, NT>(r, wi.blk, lds); break;
+        TILD_REG_PLANS(TILD_CASE)
+#undef TILD_CASE
         default: break;   // excluded by grad_reg_plan on the host
     }
 }
@@ -440,10 +447,9 @@ __global__ void __launch_bounds__(NT) k_plane_reg(const RepDev* __restrict__ rep
     const RepDev& r = reps[wi.rep];
     const AxisPlan& pl = r.ax[0];
     switch (pl.rad[0] * 32 + pl.rad[1]) {   // uniform per block
-        case 16 * 32 + 10: plane_reg_block<16, 10, NT>(r, wi.blk, lds); break;
-        case 10 * 32 + 8: plane_reg_block<10, 8, NT>(r, wi.blk, lds); break;
-        case 8 * 32 + 8: plane_reg_block<8, 8, NT>(r, wi.blk, lds); break;
-        case 8 * 32 + 5: plane_reg_block<8, 5, NT>(r, wi.blk, lds); break;
+#define TILD_CASE(A, B) case A * 32 + B: plane_reg_block<A, B, NT>(r, wi.blk, lds); break;
+        TILD_REG_PLANS(TILD_CASE)
+#undef TILD_CASE
         default: break;   // excluded by plane_reg_plan on the host
     }
 }
@@ -466,7 +472,7 @@ hipError_t configure_plane_reg() {
 bool grad_reg_plan(const AxisPlan& pl) {
     if (pl.nst != 2) return false;
     const int k = pl.rad[0] * 32 + pl.rad[1];
-    return k == 16 * 32 + 10 || k == 10 * 32 + 8 || k == 8 * 32 + 8 || k == 8 * 32 + 5;
+    return reg_plan_pair(k);
 }
 
 hipError_t launch_grad_reg(const LaunchCtx& c) {
@@ -479,7 +485,7 @@ bool last_reg_plan(const AxisPlan& pl, int ntypes, int nf) {
     if (pl.nst != 2 || ntypes < 1 || ntypes > 4 || nf > ntypes + 1) return false;
     if (!last_reg_variant(pl.rad[0], pl.rad[1], ntypes, nf - ntypes)) return false;
     const int k = pl.rad[0] * 32 + pl.rad[1];
-    return k == 16 * 32 + 10 || k == 10 * 32 + 8 || k == 8 * 32 + 8 || k == 8 * 32 + 5;
+    return reg_plan_pair(k);
 }
 
 hipError_t launch_last_reg(const LaunchCtx& c, int koff, int cls) {

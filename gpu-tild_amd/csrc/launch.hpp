@@ -90,16 +90,19 @@ constexpr int kGradThreads = 256;                       // block of the y invers
 constexpr int kC2rThreads = 256;                        // block of the inverse x pass (records)
 constexpr size_t kPlaneLds = 163840;                    // LDS one block may use (MI355X)
 hipError_t launch_mid_grad(const LaunchCtx& c);
-// Fused last-axis pass with register-resident columns (fft_reg.hip): last axes with a
-// two-stage plan R1 x R2 among 16 x 10, 10 x 8, 8 x 8 and 8 x 5, at most four types
-// (two on 160-point axes) plus charges.  Blocks of kLastRegThreads threads hold nl_last = kLastRegThreads / (2 R1)
+// Two-stage Stockham plans R1 x R2 the register-resident passes are compiled for (the
+// benchmark workloads' axes: 160, 80, 64, 40, and the sweep's and server's 128, 50, 48,
+// 32, 30, 25 and 24 points).
+#define TILD_REG_PLANS(X) X(16, 10) X(10, 8) X(8, 8) X(8, 5) X(16, 8) X(10, 5) X(16, 3) X(8, 4) X(10, 3) X(5, 5) X(8, 3)
+// Fused last-axis pass with register-resident columns (fft_reg.hip): last axes with one
+// of those plans, at most four types (two on axes whose first radix is 16) plus charges.  Blocks of kLastRegThreads threads hold nl_last = kLastRegThreads / (2 R1)
 // columns; results agree with k_last_fused to rounding.
 constexpr int kLastRegThreads = 256;
 // the variants compiled in (plan R1 x R2, types, charge field): those that fit 256
 // registers per thread, so that every variant runs two waves per SIMD (160-point axes
 // up to two types)
 __host__ __device__ constexpr bool last_reg_variant(int R1, int R2, int TT, int CH) {
-    return (R1 == 16 && R2 == 10) ? TT <= 2 : TT <= 4;
+    return R1 == 16 ? TT <= 2 : TT <= 4;
 }
 // the kernel instantiation of a type count: up to three types (class 0), four (class 1,
 // up to 256 registers, in a kernel of its own so that the others keep theirs)

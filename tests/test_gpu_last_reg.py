@@ -2,14 +2,15 @@
 last-axis pass (k_last_reg) and the y inverse (k_grad_reg) -- against the LDS-staged
 ones.
 
-Both passes run the same transforms (forward plan R1 x R2, inverse plan R2 x R1) and
+Both forms run the same transforms (forward plan R1 x R2, inverse plan R2 x R1) and
 the same coupling sums; the compiler fuses multiplies into additions differently where
 one pass keeps in registers what the other stages through LDS, so the two agree to
-rounding (1e-6 relative) rather than bit for bit.  The oracle bounds the register pass
-at the usual tolerances, and in a batch each replica keeps the bits it has alone.  The decks cover the
-four two-stage plans the register pass takes (last axes of 160, 80, 64 and 40 points),
-one to four types, charges, ramped styles and 2D; the y inverse and the plane pass take
-the same plans for their axes (3D only; the plane pass on square x-y planes).
+rounding (1e-6 relative) rather than bit for bit.  The oracle bounds the register
+passes at the usual tolerances, and in a batch each replica keeps the bits it has
+alone.  The decks cover the two-stage plans the register passes take (axes of 160, 128,
+80, 64, 50, 48, 40, 32, 30, 25 and 24 points), one to four types, charges, ramped
+styles and 2D; the y inverse and the plane pass take the same plans for their axes (3D
+only; the plane pass on square x-y planes).
 """
 import pathlib
 
@@ -94,17 +95,21 @@ CASES = {
     "plane_x40_y40_z20_4types": lambda t: decks.charged(t, grid=(40, 40, 20)),
     "y40_x80_z20": lambda t: decks.melt3d(t, grid=(80, 40, 20), box=(20.0, 10.0, 5.0), n_chains=80),
     "plane_x64_y64_z10": lambda t: decks.melt3d(t, grid=(64, 64, 10), box=(16.0, 16.0, 5.0), n_chains=100),
+    "x48_y48_z30": lambda t: decks.melt3d(t, grid=(48, 48, 30), box=(12.0, 12.0, 7.5), n_chains=100),
+    "x24_y32_z25_three": lambda t: three_types(t, grid=(24, 32, 25), box=(6.0, 8.0, 6.25)),
+    "2d_x50_y128": lambda t: decks.melt2d(t, grid=(50, 128), box=(12.5, 32.0)),
 }
 
 # two-stage plans the register passes take, and the variants compiled in
-PLANS = {160: (16, 10), 80: (10, 8), 64: (8, 8), 40: (8, 5)}
+PLANS = {160: (16, 10), 80: (10, 8), 64: (8, 8), 40: (8, 5), 128: (16, 8), 50: (10, 5), 48: (16, 3), 32: (8, 4),
+         30: (10, 3), 25: (5, 5), 24: (8, 3)}
 
 
 def expected_flags(s, g):
     nx = s.grid_shape[::-1]                     # (Nx, Ny[, Nz])
     nf = s.ntypes + (1 if s.info("do_charges") else 0)
     D = s.dim
-    last = nx[-1] in PLANS and s.ntypes <= (2 if PLANS[nx[-1]] == (16, 10) else 4)
+    last = nx[-1] in PLANS and s.ntypes <= (2 if PLANS[nx[-1]][0] == 16 else 4)
     grad = D == 3 and nx[1] in PLANS
     # the forward plane pass (where the replica runs one: tiled 3D, even Ny and tile
     # height) on square x-y planes
@@ -144,7 +149,7 @@ def test_register_pass_equals_staged_pass(case, tmp_path):
 @pytest.mark.parametrize("case", ["z160", "z64_three", "z40_charged", "z64_4types_charged", "2d_y80", "y80_z40",
                                   "y64_z64_three",
                                   "x80_y40_z64", "2d_x40_y40", "2d_x160_y16", "plane_x40_y40_z20_4types",
-                                  "plane_x64_y64_z10"])
+                                  "plane_x64_y64_z10", "x48_y48_z30", "x24_y32_z25_three", "2d_x50_y128"])
 def test_register_pass_matches_oracle(case, tmp_path):
     text, d = CASES[case](tmp_path)
     s = tild.System(text, d)
