@@ -69,8 +69,8 @@ def phase_bytes(ns, D, M, Mh, nf, T, gjf, charged, ncoef, reorder_every):
 
 # the kernel of each profiled phase on the default (tiled, forward-plane) path, as named
 # in the rocprofv3 summaries
-PHASE_KERNEL = {"claim": "k_tile_claim", "deposit": "k_tile_deposit<512>", "plane_r2c": "k_plane_r2c<512>",
-                "last_fused": "k_last_reg<128>", "y_ifft": "k_grad_reg<128>", "x_c2r": "k_c2r_reg<128>",
+PHASE_KERNEL = {"claim": "k_tile_claim", "deposit": "k_tile_deposit<512>", "plane_r2c": "k_plane_reg<512>",
+                "last_fused": "k_last_reg<256, 0>", "y_ifft": "k_grad_reg<128>", "x_c2r": "k_x_c2r<true, 256>",
                 "particles": "k_particles<true, true>",
                 "advance": "k_advance", "reorder": None}
 
