@@ -1193,16 +1193,28 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
         ok(launch_particles(ctx(PH_PART, 6, 0), true, 0, 0, true));
         ok(launch_advance(stream_, d_reps_, m->d_ids, m->nids, 1, ev[7][0][0], ev[7][0][1]));
         used[7][0] = m->nids > 0;
-        bool reordered = false;
-        for (int id : ids) {   // amortised spatial reorders
+        // amortised spatial reorders, as step() runs them: one launch per kernel for a
+        // set of tiled replicas due together
+        std::vector<int> due;
+        bool all_tiled = true;
+        for (int id : ids) {
             Replica& R = *reps_[id];
-            if (e0 == hipSuccess && R.reorder_every > 0 && ++R.since_sort >= R.reorder_every) {
-                if (!reordered) ok(hipEventRecord(rv[0], stream_));
-                reordered = true;
-                if (reorder(id, err, true)) e0 = hipErrorUnknown;
+            if (R.reorder_every > 0 && ++R.since_sort >= R.reorder_every) {
+                due.push_back(id);
+                all_tiled = all_tiled && R.h.spread_mode == 2;
             }
         }
-        if (reordered) ok(hipEventRecord(rv[1], stream_));
+        const bool reordered = e0 == hipSuccess && !due.empty();
+        if (reordered) {
+            ok(hipEventRecord(rv[0], stream_));
+            if (due.size() > 1 && all_tiled) {
+                if (reorder_set(due, err)) e0 = hipErrorUnknown;
+            } else {
+                for (int id : due)
+                    if (e0 == hipSuccess && reorder(id, err, true)) e0 = hipErrorUnknown;
+            }
+            ok(hipEventRecord(rv[1], stream_));
+        }
         ok(hipStreamSynchronize(stream_));
         if (e0 != hipSuccess) { err = std::string("profile: ") + hipGetErrorString(e0); rc = -TILD_ERR_DEVICE; break; }
         for (int k = 0; k < kKernelPhases; ++k)
