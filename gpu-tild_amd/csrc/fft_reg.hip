@@ -116,8 +116,10 @@ __device__ __forceinline__ void last_reg_block(const RepDev& r, int blk, float2*
 
     // ---- A: task (b, i < R2, f): rows i + R2 r of field f, DFT_R1 (Stockham stage 1:
     // out[i R1 + q])
+    // lanes: column, then field, then i (the fields' LDS rows are NB values apart,
+    // which spreads a wave's writes over the banks)
     for (int task = threadIdx.x; task < NB * R2 * NF; task += NT) {
-        const int ba = task % NB, ia = (task / NB) % R2, f = task / (NB * R2);
+        const int ba = task % NB, f = (task / NB) % NF, ia = task / (NB * NF);
         if (ba >= nb) continue;
         float2 v[R1];
 #pragma unroll
@@ -197,8 +199,10 @@ __device__ __forceinline__ void last_reg_block(const RepDev& r, int blk, float2*
     // ---- C: stage 2 of the R2 x R1 plan (radix R1, twiddles w_N^{j r}, including the
     // unit ones of j = 0 as the staged pass multiplies them) -> rows j + R2 q.  One task
     // per (column, j, line), spread over all threads.
+    // lanes: column, then line, then j (lines NB values apart in LDS: a wave's reads of
+    // one row fall in distinct banks)
     for (int task = threadIdx.x; task < NB * R2 * 2 * NF; task += NT) {
-        const int bc = task % NB, rest = task / NB, j = rest % R2, line = rest / R2;   // line = o NF + f
+        const int bc = task % NB, rest = task / NB, line = rest % (2 * NF), j = rest / (2 * NF);   // line = o NF + f
         if (bc >= nb) continue;
         const int o = line >= NF ? 1 : 0, f = line - o * NF;
         float2 z[R1];
