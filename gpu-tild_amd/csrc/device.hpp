@@ -41,6 +41,7 @@ struct RepDev {
     float fix_scale, fixq_scale;
     double fix_inv, fixq_inv;
     int fix_s2, fix_s3;        // order-1 integer weights: 44 - F and 52 - F (fixed_weight2/3)
+    float fixq_s1;             // order-1 charge weights: fixq_scale 2^-22 per axis (fixed_charge)
     // FFT plans (x, y, z) and chunk sizes of each pass
     AxisPlan ax[3];
     AxisPlan axr;  // inverse plan of the last axis: a two-stage plan reversed, else ax[dim - 1]
@@ -122,7 +123,8 @@ struct RepDev {
     int* tbase;                // nbuckets + 1: first record of each bucket's region
     int* tcap;                 // nbuckets: region sizes
     int tslack;                // spare slots per region (0: count/4 + 16, < 0: none)
-    float4* trec;              // records (x, y, z, type) in bucket regions
+    int srec;                  // 1: order-1 stencil records (stencil_record), 0: (x, y, z, type)
+    float4* trec;              // records in bucket regions (16 bytes, either kind)
     float* trecq;              // their charges (charged decks)
     int* tovf;                 // overflow counts by step parity, at [0] and [32]
     float4* tovf_rec;          // ns records that found their region full
@@ -351,26 +353,38 @@ __device__ __forceinline__ int stencil_node(float x, float dx, int N, int P) {
 __device__ inline int wrap_next(int n, int N) { return (n + 1 == N) ? 0 : n + 1; }
 
 // ---------------------------------------------------------------- fixed-point weights
-// Order-1 deposition in integers: each axis weight W in [0, 1] (0.5 -+ sx) is rounded to
-// q = W 2^22, and a node's weight Wx Wy Wz 2^F is formed from 32 x 32 -> 64-bit
+// Order-1 deposition in integers: the lower weight W0 = 0.5 - sx of each axis is rounded
+// to q0 = W0 2^22 (fixed_lower) and the upper weight is its complement 2^22 - q0, so the
+// pair sums to one exactly.  A node's weight Wx Wy Wz 2^F is formed from 32 x 32 -> 64-bit
 // products: (qx qy + 2^13) >> 14 (at most 2^30 + 2^9, scale 2^30), times qz plus the
 // rounding half, shifted right by 52 - F.  Two 64-bit multiply-adds and two shifts per
 // node, where rounding the FP32 product Wx Wy Wz 2^F to a 64-bit integer takes a dozen
 // instructions.  The integer weight is within about 2^-22 of the exact product (FP32
 // rounds it within 2^-23 relative).  Both spread paths call these, so their densities
 // stay bit-identical.
-__device__ __forceinline__ uint32_t fixed_axis(float w) {
-    // round to nearest; negative roundoff of 0.5 - sx saturates to 0
-    return (uint32_t)__builtin_amdgcn_rsqf(1.0f) * 0u + (uint32_t)__float2uint_rn(w * 4194304.0f);
+constexpr uint32_t kFixedOne = 1u << 22;
+__device__ __forceinline__ uint32_t fixed_lower(float w0) {
+    // round to nearest; a negative roundoff of 0.5 - sx saturates to 0, one above 1
+    // is clamped to one
+    return min(__float2uint_rn(w0 * 4194304.0f), kFixedOne);
 }
 __device__ __forceinline__ uint32_t fixed_pair(uint32_t qx, uint32_t qy) {
     return (uint32_t)(((uint64_t)qx * qy + (1ull << 13)) >> 14);
 }
+__device__ __forceinline__ unsigned long long fixed_node(uint32_t a, uint32_t b, unsigned long long half, int sh) {
+    return ((uint64_t)a * b + half) >> sh;
+}
 __device__ __forceinline__ unsigned long long fixed_weight3(uint32_t qxy, uint32_t qz, int s3) {
-    return ((uint64_t)qxy * qz + (1ull << (s3 - 1))) >> s3;
+    return fixed_node(qxy, qz, 1ull << (s3 - 1), s3);
 }
 __device__ __forceinline__ unsigned long long fixed_weight2(uint32_t qx, uint32_t qy, int s2) {
-    return ((uint64_t)qx * qy + (s2 > 0 ? 1ull << (s2 - 1) : 0ull)) >> s2;
+    return fixed_node(qx, qy, s2 > 0 ? 1ull << (s2 - 1) : 0ull, s2);
+}
+// Charge of an order-1 node: the FP32 product of the quantised axis weights (qz = 1 in
+// 2D) times qs = charge * fixq_s1, rounded to a 64-bit integer.
+__device__ __forceinline__ unsigned long long fixed_charge(float fqx, float fqy, float fqz, float qs) {
+#pragma clang fp contract(off)
+    return (unsigned long long)__float2ll_rn(((fqx * fqy) * fqz) * qs);
 }
 
 }  // namespace tild

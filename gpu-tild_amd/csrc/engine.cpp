@@ -245,6 +245,7 @@ int Batch::add_plain(const System& S, uint32_t seed, std::string& err) {
     h.fix_s2 = 44 - F;   // F <= 44
     h.fix_s3 = 52 - F;
     h.fixq_scale = std::ldexp(1.0f, Fq);
+    h.fixq_s1 = std::ldexp(1.0f, Fq - 22 * D);   // order-1 charges: 2^-22 per quantised axis weight
     h.fix_inv = std::ldexp(1.0, -F) / (double)S.gvol;
     h.fixq_inv = std::ldexp(1.0, -Fq) / (double)S.gvol;
 
@@ -416,6 +417,8 @@ int Batch::add_plain(const System& S, uint32_t seed, std::string& err) {
             h.tslack = tile_slack_;
             h.nty = h.Ny / By; h.ntz = h.Nz / Bz;
             h.ntiles = h.nty * h.ntz;
+            // order 1: the claims write stencil records where their fields fit
+            h.srec = S.pmeorder == 1 && h.Nx <= 65536 && By <= 256 && Bz <= 256 && S.ntypes < 512 ? 1 : 0;
             h.nbuckets = h.ntiles * 4;
             // Every bucket counter on a line of its own: device-scope atomics on one line
             // serialise, and a bucket line is claimed by every block of its tiles.

@@ -45,23 +45,34 @@ __global__ void __launch_bounds__(256) k_spread(const RepDev* __restrict__ reps,
     const float qi = charged ? r.q[i] : 0.f;
     unsigned long long* accq = r.acc + (size_t)r.ntypes * r.M;
     const int nz = D == 3 ? P + 1 : 1;
+    // order 1 in the integers of the tiled deposit: lower axis weights rounded to 2^22,
+    // upper ones their complements (fixed_lower, fixed_weight2/3, fixed_charge)
+    uint32_t q[3][2];
+    float fq[3][2];
+    if (P == 1) {
+        for (int d = 0; d < 3; ++d) {
+            q[d][0] = d < D ? fixed_lower(W[d][0]) : kFixedOne;
+            q[d][1] = kFixedOne - q[d][0];
+            fq[d][0] = d < D ? (float)q[d][0] : 1.f;
+            fq[d][1] = (float)q[d][1];
+        }
+    }
+    const float qs = qi * r.fixq_s1;
     int iz0 = n0[2];
     for (int iz = 0, nzz = iz0; iz < nz; ++iz, nzz = wrap_next(nzz, r.Nz)) {
         for (int iy = 0, nyy = n0[1]; iy <= P; ++iy, nyy = wrap_next(nyy, r.Ny)) {
             const int rowb = (nzz * r.Ny + nyy) * r.Nx;
             for (int ix = 0, nxx = n0[0]; ix <= P; ++ix, nxx = wrap_next(nxx, r.Nx)) {
-                // Wx*Wy*Wz in the reference's product order (device_grid_utils.cu:329);
-                // order 1 in the integers of the tiled deposit (fixed_weight2/3)
-                float w = D == 3 ? W[0][ix] * W[1][iy] * W[2][iz] : W[0][ix] * W[1][iy];
-                unsigned long long v;
                 if (P == 1) {
-                    const uint32_t qx = fixed_axis(W[0][ix]), qy = fixed_axis(W[1][iy]);
-                    v = D == 3 ? fixed_weight3(fixed_pair(qx, qy), fixed_axis(W[2][iz]), r.fix_s3)
-                               : fixed_weight2(qx, qy, r.fix_s2);
-                } else {
-                    v = (unsigned long long)__float2ll_rn(w * r.fix_scale);
+                    const unsigned long long v = D == 3 ? fixed_weight3(fixed_pair(q[0][ix], q[1][iy]), q[2][iz], r.fix_s3)
+                                                        : fixed_weight2(q[0][ix], q[1][iy], r.fix_s2);
+                    atomicAdd(acc + rowb + nxx, v);
+                    if (charged) atomicAdd(accq + rowb + nxx, fixed_charge(fq[0][ix], fq[1][iy], fq[2][iz], qs));
+                    continue;
                 }
-                atomicAdd(acc + rowb + nxx, v);
+                // Wx*Wy*Wz in the reference's product order (device_grid_utils.cu:329)
+                const float w = D == 3 ? W[0][ix] * W[1][iy] * W[2][iz] : W[0][ix] * W[1][iy];
+                atomicAdd(acc + rowb + nxx, (unsigned long long)__float2ll_rn(w * r.fix_scale));
                 if (charged) {
                     long long vq = __float2ll_rn((w * qi) * r.fixq_scale);
                     atomicAdd(accq + rowb + nxx, (unsigned long long)vq);

@@ -201,12 +201,17 @@ __device__ __forceinline__ void deposit_record(const RepDev& r, const float4 rec
     for (int k = 1; k < NP; ++k) xx[k] = wrap_next(xx[k - 1], Nx);
     const int nz = D == 3 ? NP : 1;
     const bool charged = r.trecq != nullptr;
-    uint32_t qw[3][NP];   // order 1: axis weights in fixed point (fixed_weight2/3)
+    uint32_t qw[3][NP];   // order 1: axis weights in fixed point (fixed_weight2/3, fixed_charge)
+    float fq[3][NP];
+    const float qs = qi * r.fixq_s1;
     if constexpr (NP == 2) {
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
-#pragma unroll
-            for (int k = 0; k < NP; ++k) qw[d][k] = fixed_axis(W[d][k]);
+        for (int d = 0; d < 3; ++d) {
+            qw[d][0] = d < D ? fixed_lower(W[d][0]) : kFixedOne;
+            qw[d][1] = kFixedOne - qw[d][0];
+            fq[d][0] = d < D ? (float)qw[d][0] : 1.f;
+            fq[d][1] = (float)qw[d][1];
+        }
     }
     int zz = n0[2];
 #pragma unroll
@@ -223,15 +228,76 @@ __device__ __forceinline__ void deposit_record(const RepDev& r, const float4 rec
             unsigned long long* rowq = patch + ((size_t)(T * Bz + lz) * By + ly) * Nx;
 #pragma unroll
             for (int ix = 0; ix < NP; ++ix) {
-                const float w = D == 3 ? W[0][ix] * W[1][iy] * W[2][iz] : W[0][ix] * W[1][iy];
-                unsigned long long v;
-                if constexpr (NP == 2)
-                    v = D == 3 ? fixed_weight3(fixed_pair(qw[0][ix], qw[1][iy]), qw[2][iz], r.fix_s3)
-                               : fixed_weight2(qw[0][ix], qw[1][iy], r.fix_s2);
-                else
-                    v = (unsigned long long)__float2ll_rn(w * r.fix_scale);
-                atomicAdd(rowp + xx[ix], v);
-                if (charged) atomicAdd(rowq + xx[ix], (unsigned long long)__float2ll_rn((w * qi) * r.fixq_scale));
+                if constexpr (NP == 2) {
+                    atomicAdd(rowp + xx[ix], D == 3 ? fixed_weight3(fixed_pair(qw[0][ix], qw[1][iy]), qw[2][iz], r.fix_s3)
+                                                    : fixed_weight2(qw[0][ix], qw[1][iy], r.fix_s2));
+                    if (charged) atomicAdd(rowq + xx[ix], fixed_charge(fq[0][ix], fq[1][iy], fq[2][iz], qs));
+                } else {
+                    const float w = D == 3 ? W[0][ix] * W[1][iy] * W[2][iz] : W[0][ix] * W[1][iy];
+                    atomicAdd(rowp + xx[ix], (unsigned long long)__float2ll_rn(w * r.fix_scale));
+                    if (charged) atomicAdd(rowq + xx[ix], (unsigned long long)__float2ll_rn((w * qi) * r.fixq_scale));
+                }
+            }
+        }
+    }
+}
+
+// Block-uniform values of the stencil-record deposit, read once before the record loop
+// (the compiler reloads fields of the replica after every LDS atomic otherwise).
+struct SrecTile {
+    int Nx, By, Bz, ntypes, sh;
+    unsigned long long half;   // node weight (a b + half) >> sh: fixed_weight3 (3D), fixed_weight2 (2D)
+    float qsc;                 // fixq_s1
+    bool charged;
+    __device__ __forceinline__ SrecTile(const RepDev& r, int D)
+        : Nx(r.Nx), By(r.By), Bz(r.Bz), ntypes(r.ntypes), sh(D == 3 ? r.fix_s3 : r.fix_s2),
+          half(sh > 0 ? 1ull << (sh - 1) : 0ull), qsc(r.fixq_s1), charged(r.trecq != nullptr) {}
+};
+
+// Deposit one order-1 stencil record (stencil_record).  The record's first stencil row
+// sits at (my + dy, mz + dz) in this tile: dy, dz are 0 for the tile's own buckets and
+// -By, -Bz for the halo buckets of its lower neighbours.  The same integer weights and
+// charges as the position-record path and the global-atomic spread.
+template <int D>
+__device__ __forceinline__ void deposit_srec(const SrecTile& c, const float4 rec, float qi, unsigned long long* patch,
+                                             int dy, int dz) {
+    const uint32_t w0 = __float_as_uint(rec.x), w1 = __float_as_uint(rec.y);
+    int xx[2];
+    xx[0] = (int)(w0 & 0xffffu);
+    xx[1] = wrap_next(xx[0], c.Nx);
+    const int ly0 = (int)((w0 >> 16) & 0xffu) + dy, lz0 = (int)(w0 >> 24) + dz;
+    const int tp = (int)(w1 >> 23);
+    uint32_t qx[2], qy[2], qz[2];
+    qx[0] = w1 & 0x7fffffu; qx[1] = kFixedOne - qx[0];
+    qy[0] = __float_as_uint(rec.z); qy[1] = kFixedOne - qy[0];
+    qz[0] = D == 3 ? __float_as_uint(rec.w) : kFixedOne; qz[1] = kFixedOne - qz[0];
+    const float qs = qi * c.qsc;
+    uint32_t qxy[2][2];   // [iy][ix]
+    if constexpr (D == 3) {
+#pragma unroll
+        for (int iy = 0; iy < 2; ++iy)
+#pragma unroll
+            for (int ix = 0; ix < 2; ++ix) qxy[iy][ix] = fixed_pair(qx[ix], qy[iy]);
+    }
+#pragma unroll
+    for (int iz = 0; iz < (D == 3 ? 2 : 1); ++iz) {
+        const int lz = lz0 + iz;
+        if ((unsigned)lz >= (unsigned)c.Bz) continue;
+#pragma unroll
+        for (int iy = 0; iy < 2; ++iy) {
+            const int ly = ly0 + iy;
+            if ((unsigned)ly >= (unsigned)c.By) continue;
+            const uint32_t row = ((uint32_t)(tp * c.Bz + lz) * (uint32_t)c.By + (uint32_t)ly) * (uint32_t)c.Nx;
+#pragma unroll
+            for (int ix = 0; ix < 2; ++ix) {
+                const unsigned long long v = D == 3 ? fixed_node(qxy[iy][ix], qz[iz], c.half, c.sh)
+                                                    : fixed_node(qx[ix], qy[iy], c.half, c.sh);
+                atomicAdd(patch + row + xx[ix], v);
+                if (c.charged) {
+                    const uint32_t rowq = ((uint32_t)(c.ntypes * c.Bz + lz) * (uint32_t)c.By + (uint32_t)ly) * (uint32_t)c.Nx;
+                    atomicAdd(patch + rowq + xx[ix],
+                              fixed_charge((float)qx[ix], (float)qy[iy], D == 3 ? (float)qz[iz] : 1.f, qs));
+                }
             }
         }
     }
@@ -264,6 +330,17 @@ __device__ __forceinline__ void deposit_record_any(const RepDev& r, const float4
             }
         }
     }
+}
+
+// Record j of a block's source regions taken as one index range: j + shift[k] for the
+// last region k with start[k] <= j (start ascending, both block-uniform), summed from
+// the steps dsh[s] = shift[s] - shift[s - 1] (a chain of selects between the shifts
+// themselves is compiled into an indexed load from scratch memory).
+__device__ __forceinline__ int region_pos(int j, const int (&start)[9], const int (&dsh)[9]) {
+    int sh = dsh[0];
+#pragma unroll
+    for (int s = 1; s < 9; ++s) sh += j >= start[s] ? dsh[s] : 0;
+    return j + sh;
 }
 
 // Deposit one tile's rows and transform them along x.  NT = 512 for tiles of about
@@ -306,36 +383,76 @@ __global__ void __launch_bounds__(kDepositThreads) k_tile_deposit(const RepDev* 
     // the filled parts of the source regions as one index range: record j of the block
     // is j + shift[k] for the last region k with start[k] <= j (all threads busy, no
     // partial sweep per region)
-    int start[9], shift[9], total = 0;
+    int start[9], dsh[9], total = 0, prev = 0;
 #pragma unroll
     for (int s = 0; s < 9; ++s) {
         const int bb = s < nbk ? bk[s] : 0;
         const int n = s < nbk ? min(cnt[bb << r.tshift], cap[bb]) : 0;
-        start[s] = total;
-        shift[s] = base[bb] - total;
+        const int shift = base[bb] - total;
+        // block-uniform: kept in scalar registers
+        start[s] = __builtin_amdgcn_readfirstlane(total);
+        dsh[s] = __builtin_amdgcn_readfirstlane(shift - prev);
+        prev = shift;
         total += n;
     }
-    for (int j = threadIdx.x; j < total; j += kDepositThreads) {
-        int sh = shift[0];
+    if (r.srec) {
+        // stencil records: the region decides the row offsets (own 0-3, y neighbour
+        // 4-5, z neighbour 6-7, yz neighbour 8); four records per thread are loaded
+        // before the first is deposited
+        const SrecTile c(r, D);
+        const float* recq = r.trecq;
+        for (int j0 = threadIdx.x; j0 < total; j0 += 4 * kDepositThreads) {
+            float4 rec[4];
+            float qi[4];
 #pragma unroll
-        for (int s = 1; s < 9; ++s) sh = j >= start[s] ? shift[s] : sh;
-        const int p = j + sh;
-        const float4 rec = gld(r.trec + p);
-        const float qi = r.trecq ? G((const float*)r.trecq)[p] : 0.f;
-        if (P == 1) deposit_record<2>(r, rec, qi, patch, y0, z0, By, Bz);
-        else deposit_record_any(r, rec, qi, patch, y0, z0, By, Bz);
+            for (int k = 0; k < 4; ++k) {
+                const int j = j0 + k * kDepositThreads;
+                rec[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+                qi[k] = 0.f;
+                if (j < total) {
+                    const int p = region_pos(j, start, dsh);
+                    rec[k] = gld(r.trec + p);
+                    if (c.charged) qi[k] = G(recq)[p];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int j = j0 + k * kDepositThreads;
+                if (j >= total) break;
+                const bool hy = (j >= start[4] && j < start[6]) || j >= start[8];
+                const int dy = hy ? -By : 0, dz = j >= start[6] ? -Bz : 0;
+                if (D == 3) deposit_srec<3>(c, rec[k], qi[k], patch, dy, dz);
+                else deposit_srec<2>(c, rec[k], qi[k], patch, dy, 0);
+            }
+        }
+    } else {
+        for (int j = threadIdx.x; j < total; j += kDepositThreads) {
+            const int p = region_pos(j, start, dsh);
+            const float4 rec = gld(r.trec + p);
+            const float qi = r.trecq ? G((const float*)r.trecq)[p] : 0.f;
+            if (P == 1) deposit_record<2>(r, rec, qi, patch, y0, z0, By, Bz);
+            else deposit_record_any(r, rec, qi, patch, y0, z0, By, Bz);
+        }
     }
     // beads whose region was full: the same bucket test over the overflow list
     const int novf = G((const int*)r.tovf)[par * 32];
     for (int j = threadIdx.x; j < novf; j += kDepositThreads) {
         const int bb = G((const int*)r.tovf_b)[j];
-        bool mine = false;
-        for (int s = 0; s < nbk; ++s) mine |= bb == bk[s];
-        if (!mine) continue;
+        int src = -1;
+#pragma unroll
+        for (int s = 0; s < 9; ++s) src = s < nbk && bb == bk[s] ? s : src;
+        if (src < 0) continue;
         const float4 rec = gld(r.tovf_rec + j);
         const float qi = r.trecq ? G((const float*)r.tovf_q)[j] : 0.f;
-        if (P == 1) deposit_record<2>(r, rec, qi, patch, y0, z0, By, Bz);
-        else deposit_record_any(r, rec, qi, patch, y0, z0, By, Bz);
+        if (r.srec) {
+            const int dy = (src >= 4 && src < 6) || src == 8 ? -By : 0, dz = src >= 6 ? -Bz : 0;
+            if (D == 3) deposit_srec<3>(SrecTile(r, 3), rec, qi, patch, dy, dz);
+            else deposit_srec<2>(SrecTile(r, 2), rec, qi, patch, dy, 0);
+        } else if (P == 1) {
+            deposit_record<2>(r, rec, qi, patch, y0, z0, By, Bz);
+        } else {
+            deposit_record_any(r, rec, qi, patch, y0, z0, By, Bz);
+        }
     }
     __syncthreads();
     // densities of the tile rows -> rho and the x-axis R2C (pairs of rows)

@@ -25,6 +25,32 @@ __device__ __forceinline__ int tile_bucket_yz(const RepDev& r, float y, float z)
     return (tz * r.nty + ty) * 4 + c;
 }
 
+// Order-1 stencil record of a bead (RepDev::srec): what the deposit needs, computed once
+// by the bead's claim rather than by every tile that deposits it.
+//   x  first stencil node along x | (my, mz) << 16, << 24: the first stencil row's
+//      offsets inside the bead's tile
+//   y  lower x weight (fixed_lower) | type << 23
+//   z  lower y weight;  w  lower z weight (0 in 2D)
+// The upper weights are the complements kFixedOne - lower.  Returns the bead's bucket
+// (the same as tile_bucket_yz).  Requires Nx <= 65536, By, Bz <= 256 and fewer than 512
+// types (the engine checks).
+__device__ __forceinline__ int stencil_record(const RepDev& r, float4 rec, uint4& out) {
+    const int D = r.dim;
+    float W[3][2];
+    const int nx0 = stencil_1d(rec.x, r.dx[0], r.idx[0], r.Nx, 1, W[0]);
+    const int sy = stencil_1d(rec.y, r.dx[1], r.idx[1], r.Ny, 1, W[1]);
+    const int sz = D == 3 ? stencil_1d(rec.z, r.dx[2], r.idx[2], r.Nz, 1, W[2]) : 0;
+    int my, mz;
+    const int ty = div_small(sy, r.By, r.iBy, my), tz = div_small(sz, r.Bz, r.iBz, mz);
+    const bool yb = my + 1 >= r.By;
+    const bool zb = D == 3 && mz + 1 >= r.Bz;
+    const int c = (yb && zb) ? 2 : (yb ? 1 : (zb ? 3 : 0));
+    out = make_uint4((uint32_t)nx0 | (uint32_t)my << 16 | (uint32_t)mz << 24,
+                     fixed_lower(W[0][0]) | (uint32_t)__float_as_int(rec.w) << 23,
+                     fixed_lower(W[1][0]), D == 3 ? fixed_lower(W[2][0]) : 0u);
+    return (tz * r.nty + ty) * 4 + c;
+}
+
 __device__ __forceinline__ int tile_bucket(const RepDev& r, const float* xp, int i) {
     gmem<const float> x = G(xp);
     return tile_bucket_yz(r, x[i * r.dim + 1], r.dim == 3 ? x[i * r.dim + 2] : 0.f);
@@ -99,15 +125,22 @@ __device__ __forceinline__ int* claim_counters(const RepDev& r, int parity) {
     return r.tclaim + (((size_t)parity * r.nbuckets) << r.tshift);
 }
 
-// A bead's record (x, y, z, type) and charge into its bucket's region for the step of
-// the given parity; a bead whose region is full goes to the overflow list, which the
+// A bead's record and charge into its bucket's region for the step of the given parity:
+// its stencil record (stencil_record) where the replica deposits those, else rec =
+// (x, y, z, type).  A bead whose region is full goes to the overflow list, which the
 // deposit reads too, so the densities stay exact whatever the occupancy.  The region
 // bounds are requested before the slot claim, so that their latency hides behind the
 // atomic.  Every lane of the wave calls it (valid false for a lane without a bead).
 __device__ __forceinline__ void claim_record(const RepDev& r, bool valid, float4 rec, float qi, int parity) {
     int b = 0x7fffffff, cap = 0, base = 0;
     if (valid) {
-        b = tile_bucket_yz(r, rec.y, rec.z);
+        if (r.srec) {
+            uint4 w;
+            b = stencil_record(r, rec, w);
+            rec = make_float4(__uint_as_float(w.x), __uint_as_float(w.y), __uint_as_float(w.z), __uint_as_float(w.w));
+        } else {
+            b = tile_bucket_yz(r, rec.y, rec.z);
+        }
         cap = G((const int*)r.tcap)[b];
         base = G((const int*)r.tbase)[b];
     }
