@@ -109,6 +109,9 @@ __global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_e
     gmem<float> rho_out = G(r.rho);
     const FastDiv dN(N);
     const int type_rows = r.ntypes * rows_per_field;   // rows below are particle densities
+    // read once: a lane-dependent choice between the two fields is compiled into a
+    // vector load of the replica record
+    const double inv_t = r.fix_inv, inv_q = r.fixq_inv;
     for (int task = threadIdx.x; task < nb * N; task += kFftThreads) {
         int n;
         const int b = dN.div(task, n);
@@ -116,12 +119,12 @@ __global__ void __launch_bounds__(kFftThreads) __attribute__((amdgpu_waves_per_e
         float a, bb = 0.f;
         if (FROM_ACC) {
             size_t ia = (size_t)ra * N + n;
-            a = (float)((double)(long long)acc[ia] * (ra < type_rows ? r.fix_inv : r.fixq_inv));
+            a = (float)((double)(long long)acc[ia] * (ra < type_rows ? inv_t : inv_q));
             acc[ia] = 0ull;
             rho_out[ia] = a;
             if (rb < nrows) {
                 size_t ib = (size_t)rb * N + n;
-                bb = (float)((double)(long long)acc[ib] * (rb < type_rows ? r.fix_inv : r.fixq_inv));
+                bb = (float)((double)(long long)acc[ib] * (rb < type_rows ? inv_t : inv_q));
                 acc[ib] = 0ull;
                 rho_out[ib] = bb;
             }

@@ -459,6 +459,11 @@ __global__ void __launch_bounds__(kDepositThreads) k_tile_deposit(const RepDev* 
     const int npairs = (rows + 1) / 2;
     const int ldb = npairs | 1;
     const FastDiv dNx(Nx), dNxh(Nxh), dBy(By), dBz(Bz);
+    // block-uniform values read once: a lane-dependent choice between two fields of the
+    // replica is compiled into a vector load of the replica record, once per row
+    const double inv_t = r.fix_inv, inv_q = r.fixq_inv;
+    const int Ny = r.Ny, M = r.M, off32 = r.grid_off32, plane_fwd = r.plane_fwd;
+    float* const rho = r.rho;
     // task = j * Nx + n; rows 2j and 2j + 1 are (f, lz, ly) with ly fastest
     Digits2 gt(threadIdx.x, kDepositThreads, dNx);
     for (int task = threadIdx.x; task < npairs * Nx; task += kDepositThreads, gt.step()) {
@@ -473,15 +478,15 @@ __global__ void __launch_bounds__(kDepositThreads) k_tile_deposit(const RepDev* 
                 ++ly;
                 if (ly == By) { ly = 0; if (++lz == Bz) { lz = 0; ++f; } }
             }
-            const double inv = f < T ? r.fix_inv : r.fixq_inv;
+            const double inv = f < T ? inv_t : inv_q;
             v[h] = (float)((double)(long long)patch[(size_t)rr * Nx + n] * inv);
-            const int node = ((z0 + lz) * r.Ny + (y0 + ly)) * Nx + n;
-            if (r.grid_off32) G(at32(r.rho, (uint32_t)f * (uint32_t)r.M + (uint32_t)node))[0] = v[h];
-            else G(r.rho)[(size_t)f * r.M + node] = v[h];
+            const int node = ((z0 + lz) * Ny + (y0 + ly)) * Nx + n;
+            if (off32) G(at32(rho, (uint32_t)f * (uint32_t)M + (uint32_t)node))[0] = v[h];
+            else G(rho)[(size_t)f * M + node] = v[h];
         }
-        if (!r.plane_fwd) fbuf[n * ldb + j] = make_float2(v[0], v[1]);
+        if (!plane_fwd) fbuf[n * ldb + j] = make_float2(v[0], v[1]);
     }
-    if (r.plane_fwd) return;   // the plane pass transforms the densities
+    if (plane_fwd) return;   // the plane pass transforms the densities
     __syncthreads();
     const float2* res = lds_fft<kDepositThreads>(fbuf, fbuf + (size_t)Nx * ldb, ldb, npairs, r.ax[0]);
     for (int task = threadIdx.x; task < npairs * Nxh; task += kDepositThreads) {
