@@ -343,6 +343,24 @@ __device__ __forceinline__ int region_pos(int j, const int (&start)[9], const in
     return j + sh;
 }
 
+// Loads records j0, j0 + NT, j0 + 2 NT, j0 + 3 NT of the block's index range (zeros past
+// its end) and their charges.
+template <int NT>
+__device__ __forceinline__ void load_srec4(const RepDev& r, int j0, int total, const int (&start)[9],
+                                           const int (&dsh)[9], bool charged, float4 (&rec)[4], float (&qi)[4]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int j = j0 + k * NT;
+        rec[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        qi[k] = 0.f;
+        if (j < total) {
+            const int p = region_pos(j, start, dsh);
+            rec[k] = gld(r.trec + p);
+            if (charged) qi[k] = G((const float*)r.trecq)[p];
+        }
+    }
+}
+
 // Deposit one tile's rows and transform them along x.  NT = 512 for tiles of about
 // 1,000 beads plus their halo share (three blocks of 512 fit a CU; on the 640,000-bead
 // melt 17.5 us, against 19.6 with 256 threads and 22.8 with 1,024, where one block
@@ -361,14 +379,9 @@ __global__ void __launch_bounds__(kDepositThreads) k_tile_deposit(const RepDev* 
     const int par = (r.step + koff) & 1;
     unsigned long long* patch = tile_lds;                         // [f][lz][ly][x]
     float2* fbuf = (float2*)(tile_lds + (size_t)rows * Nx);
-    // the other parity's counters were last used by the previous step: clear this
-    // tile's for the next one
-    if (threadIdx.x < 4) G(claim_counters(r, par ^ 1))[(t * 4 + threadIdx.x) << r.tshift] = 0;
-    if (t == 0 && threadIdx.x == 0) G(r.tovf)[(par ^ 1) * 32] = 0;
-    for (int k = threadIdx.x; k < rows * Nx; k += kDepositThreads) patch[k] = 0ull;
-    __syncthreads();
     // source buckets: own tile (all classes) and the lower neighbours' halo classes;
-    // each bucket's records are the filled part of its region
+    // each bucket's records are the filled part of its region.  Their counts and
+    // bounds are requested first: the LDS rows are cleared while they are in flight.
     const int tym = (ty + r.nty - 1) % r.nty;
     const int tzm = (tz + r.ntz - 1) % r.ntz;
     const int own = tz * r.nty + ty, ny_ = tz * r.nty + tym, nz_ = tzm * r.nty + ty, nyz = tzm * r.nty + tym;
@@ -379,16 +392,38 @@ __global__ void __launch_bounds__(kDepositThreads) k_tile_deposit(const RepDev* 
     gmem<const int> cnt = G((const int*)claim_counters(r, par));
     gmem<const int> base = G((const int*)r.tbase);
     gmem<const int> cap = G((const int*)r.tcap);
+    int vcnt[9], vcap[9], vbase[9];
+#pragma unroll
+    for (int s = 0; s < 9; ++s) {
+        const int bb = s < nbk ? bk[s] : 0;
+        vcnt[s] = cnt[bb << r.tshift];
+        vcap[s] = cap[bb];
+        vbase[s] = base[bb];
+    }
+    const int novf = G((const int*)r.tovf)[par * 32];
+    // the other parity's counters were last used by the previous step: clear this
+    // tile's for the next one
+    if (threadIdx.x < 4) G(claim_counters(r, par ^ 1))[(t * 4 + threadIdx.x) << r.tshift] = 0;
+    if (t == 0 && threadIdx.x == 0) G(r.tovf)[(par ^ 1) * 32] = 0;
+    for (int k = threadIdx.x; k < rows * Nx; k += kDepositThreads) patch[k] = 0ull;
     const int T = r.ntypes;
+    const int Ny = r.Ny, M = r.M, plane_fwd = r.plane_fwd;
+    // where the plane pass transforms the densities the FFT scratch is free: it holds
+    // the first density element of each tile row (f, lz, ly)
+    int* const rowoff = (int*)fbuf;
+    if (plane_fwd)
+        for (int rr = threadIdx.x; rr < rows; rr += kDepositThreads) {
+            const int ly = rr % By, lz = (rr / By) % Bz, f = rr / (By * Bz);
+            rowoff[rr] = f * M + ((z0 + lz) * Ny + (y0 + ly)) * Nx;
+        }
     // the filled parts of the source regions as one index range: record j of the block
     // is j + shift[k] for the last region k with start[k] <= j (all threads busy, no
     // partial sweep per region)
     int start[9], dsh[9], total = 0, prev = 0;
 #pragma unroll
     for (int s = 0; s < 9; ++s) {
-        const int bb = s < nbk ? bk[s] : 0;
-        const int n = s < nbk ? min(cnt[bb << r.tshift], cap[bb]) : 0;
-        const int shift = base[bb] - total;
+        const int n = s < nbk ? min(vcnt[s], vcap[s]) : 0;
+        const int shift = vbase[s] - total;
         // block-uniform: kept in scalar registers
         start[s] = __builtin_amdgcn_readfirstlane(total);
         dsh[s] = __builtin_amdgcn_readfirstlane(shift - prev);
@@ -397,24 +432,15 @@ __global__ void __launch_bounds__(kDepositThreads) k_tile_deposit(const RepDev* 
     }
     if (r.srec) {
         // stencil records: the region decides the row offsets (own 0-3, y neighbour
-        // 4-5, z neighbour 6-7, yz neighbour 8); four records per thread are loaded
-        // before the first is deposited
+        // 4-5, z neighbour 6-7, yz neighbour 8); four records per thread are in flight
+        // at a time, the first four across the barrier that ends the clearing
         const SrecTile c(r, D);
-        const float* recq = r.trecq;
+        float4 rec[4];
+        float qi[4];
+        load_srec4<kDepositThreads>(r, threadIdx.x, total, start, dsh, c.charged, rec, qi);
+        __syncthreads();
         for (int j0 = threadIdx.x; j0 < total; j0 += 4 * kDepositThreads) {
-            float4 rec[4];
-            float qi[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int j = j0 + k * kDepositThreads;
-                rec[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-                qi[k] = 0.f;
-                if (j < total) {
-                    const int p = region_pos(j, start, dsh);
-                    rec[k] = gld(r.trec + p);
-                    if (c.charged) qi[k] = G(recq)[p];
-                }
-            }
+            if (j0 > (int)threadIdx.x) load_srec4<kDepositThreads>(r, j0, total, start, dsh, c.charged, rec, qi);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const int j = j0 + k * kDepositThreads;
@@ -426,6 +452,7 @@ __global__ void __launch_bounds__(kDepositThreads) k_tile_deposit(const RepDev* 
             }
         }
     } else {
+        __syncthreads();
         for (int j = threadIdx.x; j < total; j += kDepositThreads) {
             const int p = region_pos(j, start, dsh);
             const float4 rec = gld(r.trec + p);
@@ -435,7 +462,6 @@ __global__ void __launch_bounds__(kDepositThreads) k_tile_deposit(const RepDev* 
         }
     }
     // beads whose region was full: the same bucket test over the overflow list
-    const int novf = G((const int*)r.tovf)[par * 32];
     for (int j = threadIdx.x; j < novf; j += kDepositThreads) {
         const int bb = G((const int*)r.tovf_b)[j];
         int src = -1;
@@ -462,8 +488,24 @@ __global__ void __launch_bounds__(kDepositThreads) k_tile_deposit(const RepDev* 
     // block-uniform values read once: a lane-dependent choice between two fields of the
     // replica is compiled into a vector load of the replica record, once per row
     const double inv_t = r.fix_inv, inv_q = r.fixq_inv;
-    const int Ny = r.Ny, M = r.M, off32 = r.grid_off32, plane_fwd = r.plane_fwd;
+    const int off32 = r.grid_off32;
     float* const rho = r.rho;
+    const int trows = T * By * Bz;   // rows below are particle densities, the rest charge
+    if (plane_fwd && off32 && (Nx & 1) == 0) {
+        // densities only, two x nodes per task: (row, node pair) digits stepped, the
+        // row's first element from the table
+        const int np = Nx >> 1;
+        const FastDiv dNp(np);
+        Digits2 gq(threadIdx.x, kDepositThreads, dNp);
+        for (int task = threadIdx.x; task < rows * np; task += kDepositThreads, gq.step()) {
+            const int q = gq.lo, rr = gq.hi;
+            const double inv = rr < trows ? inv_t : inv_q;
+            const ulonglong2 a = *(const ulonglong2*)(patch + (size_t)rr * Nx + 2 * q);
+            const float2 v = make_float2((float)((double)(long long)a.x * inv), (float)((double)(long long)a.y * inv));
+            gst(at32((float2*)rho, (uint32_t)(rowoff[rr] >> 1) + (uint32_t)q), v);
+        }
+        return;
+    }
     // task = j * Nx + n; rows 2j and 2j + 1 are (f, lz, ly) with ly fastest
     Digits2 gt(threadIdx.x, kDepositThreads, dNx);
     for (int task = threadIdx.x; task < npairs * Nx; task += kDepositThreads, gt.step()) {
