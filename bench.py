@@ -50,7 +50,8 @@ def phase_bytes(ns, D, M, Mh, nf, T, gjf, charged, ncoef, reorder_every):
     claim pass (x and type in, record out) runs once per spatial reorder; deposit
     records in, densities out; forward plane densities in, half spectra out; fused pass
     spectra and coupling table in, 2 nf spectra out; y inverse 2 nf spectra in, nf D
-    out; x C2R nf D spectra and the densities in, 12-byte records out."""
+    out; x C2R nf D spectra in, 12-byte records out (its records need no density mask
+    without field phases)."""
     q = 4 if charged else 0
     b = {
         "claim": ns * (4 * D + 4 + 16 + q) / reorder_every,
@@ -58,7 +59,7 @@ def phase_bytes(ns, D, M, Mh, nf, T, gjf, charged, ncoef, reorder_every):
         "plane_r2c": nf * M * 4 + nf * Mh * 8,
         "last_fused": nf * Mh * 8 + ncoef * Mh * 4 + 2 * nf * Mh * 8,
         "y_ifft": 2 * nf * Mh * 8 + nf * D * Mh * 8 if D == 3 else 0,
-        "x_c2r": nf * D * Mh * 8 + T * M * 4 + nf * M * 12,
+        "x_c2r": nf * D * Mh * 8 + nf * M * 12,
         "particles": ns * (8 * D + 4 + (16 * D if gjf else 0) + q + 16 + q)
                      + M * (4 * T * (1 + D) + (4 * D if charged else 0)),
         "advance": 0,

@@ -400,6 +400,15 @@ k_mid_grad(const RepDev* __restrict__ reps, const WorkItem* __restrict__ wmap) {
 // dropped, which is the Hermitian projection the reference's Re(C2C inverse) takes.
 // RECORDS: output float4 (g_0, g_1, g_2, rho_f) per field and node (fields are lines
 // f*dim + d of Gk); otherwise plain real rows to out_override.
+// The gather skips nodes whose density of the bead's type is not positive
+// (device_grid_utils.cu:424-452).  Without field phases that test holds by construction
+// at every node a bead gathers from with a weight the deposit resolves (order 1: axis
+// weights of at least 2^-23; other orders: node weights of at least 2^-F): the bead's own
+// deposit makes that node's density of its type positive, the weights being never
+// negative.  A node below that resolution adds at most 2^-23 of the record to the
+// force either way.  So the records of these passes carry no mask and the pass reads no
+// densities (8 MB a step on the 640,000-bead melt); the field-phase path, which needs
+// rho itself, keeps the test.
 // The gather records of one block of the inverse x pass (see k_x_c2r): D lines per
 // field unrolled, buffers addressed by 32-bit offsets; no field phases.
 template <int D, int NT>
@@ -416,9 +425,7 @@ __device__ __forceinline__ void c2r_records(const RepDev& r, const float2* res, 
             g[d] = (line & 1) ? -z.y : z.x;      // conj of the forward result
         }
         const uint32_t fn = (uint32_t)f * (uint32_t)r.M + (uint32_t)(row0 + rr) * (uint32_t)N + (uint32_t)x;
-        // the gather skips nodes with rho <= 0 (device_grid_utils.cu:424-452): a zero
-        // record adds nothing
-        if (f < r.ntypes && !(G(at32((const float*)r.rho, fn))[0] > 0.f)) g[0] = g[1] = g[2] = 0.f;
+        // no density mask (see above)
         gst3((float*)((char*)r.G3 + ((fn << 3) + (fn << 2))), g[0], g[1], g[2]);   // 12 fn bytes
     }
 }
@@ -474,9 +481,7 @@ __device__ __forceinline__ void x_c2r_records(const RepDev& r, const WorkItem wi
             g[d] = (line & 1) ? -z.y : z.x;      // conj of the forward result
         }
         const uint32_t fn = (uint32_t)f * (uint32_t)r.M + (uint32_t)(row0 + rr) * (uint32_t)N + (uint32_t)x;
-        // the gather skips nodes with rho <= 0 (device_grid_utils.cu:424-452): a zero
-        // record adds nothing
-        if (f < r.ntypes && !(G(at32((const float*)r.rho, fn))[0] > 0.f)) g[0] = g[1] = g[2] = 0.f;
+        // no density mask (see above)
         gst3((float*)((char*)r.G3 + ((fn << 3) + (fn << 2))), g[0], g[1], g[2]);   // 12 fn bytes
     }
 }
