@@ -402,13 +402,13 @@ k_mid_grad(const RepDev* __restrict__ reps, const WorkItem* __restrict__ wmap) {
 // f*dim + d of Gk); otherwise plain real rows to out_override.
 // The gather skips nodes whose density of the bead's type is not positive
 // (device_grid_utils.cu:424-452).  Without field phases that test holds by construction
-// at every node a bead gathers from with a weight the deposit resolves (order 1: axis
-// weights of at least 2^-23; other orders: node weights of at least 2^-F): the bead's own
-// deposit makes that node's density of its type positive, the weights being never
-// negative.  A node below that resolution adds at most 2^-23 of the record to the
-// force either way.  So the records of these passes carry no mask and the pass reads no
-// densities (8 MB a step on the 640,000-bead melt); the field-phase path, which needs
-// rho itself, keeps the test.
+// at every node whose weight the deposit resolves: the bead's own deposit makes that
+// node's density of its type positive, the weights being never negative.  A node it
+// does not resolve (order 1: an axis weight below 2^-23; any order: a node weight below
+// 2^-F, F the fractional bits of the fixed-point densities, at least 23 below 10^11
+// beads) adds under 2^-23 of its record to the force.  So the records of these passes
+// carry no mask and the pass reads no densities (8 MB a step on the 640,000-bead melt);
+// the field-phase path, which needs rho itself, keeps the test.
 // The gather records of one block of the inverse x pass (see k_x_c2r): D lines per
 // field unrolled, buffers addressed by 32-bit offsets; no field phases.
 template <int D, int NT>
