@@ -39,7 +39,7 @@ HBM_PEAK_GBS = 8000.0
 REORDER_EVERY = 60        # automatic spatial-reorder interval of the melts (engine default)
 
 
-def phase_bytes(ns, D, M, Mh, nf, T, gjf, charged, ncoef, reorder_every):
+def phase_bytes(ns, D, M, Mh, nf, T, gjf, charged, ncoef, reorder_every, plane_inv=False):
     """HBM bytes of each profiled kernel for one step of one replica.  The particle
     kernel uses the per-unit figures of the reference path's minimal traffic: per bead
     it reads x and the type and writes x (8D + 4 B), plus the GJF state read and written
@@ -51,7 +51,9 @@ def phase_bytes(ns, D, M, Mh, nf, T, gjf, charged, ncoef, reorder_every):
     records in, densities out; forward plane densities in, half spectra out; fused pass
     spectra and coupling table in, 2 nf spectra out; y inverse 2 nf spectra in, nf D
     out; x C2R nf D spectra in, 12-byte records out (its records need no density mask
-    without field phases)."""
+    without field phases).  Where the y inverse and the x C2R run as one plane pass
+    (plane_inv), that pass reads the 2 nf spectra of the fused pass and writes the
+    records, and is listed under x_c2r."""
     q = 4 if charged else 0
     b = {
         "claim": ns * (4 * D + 4 + 16 + q) / reorder_every,
@@ -65,6 +67,9 @@ def phase_bytes(ns, D, M, Mh, nf, T, gjf, charged, ncoef, reorder_every):
         "advance": 0,
         "reorder": 0,   # several kernels; not a roofline candidate
     }
+    if plane_inv:
+        b["y_ifft"] = 0
+        b["x_c2r"] = 2 * nf * Mh * 8 + nf * M * 12
     return b
 
 
@@ -74,6 +79,8 @@ PHASE_KERNEL = {"claim": "k_tile_claim", "deposit": "k_tile_deposit<512>", "plan
                 "last_fused": "k_last_reg<256, 0>", "y_ifft": "k_grad_reg<128>", "x_c2r": "k_x_c2r<true, 256>",
                 "particles": "k_particles<true, true>",
                 "advance": "k_advance", "reorder": None}
+# with the inverse plane pass (square x-y planes)
+PHASE_KERNEL_PLANE_INV = dict(PHASE_KERNEL, y_ifft=None, x_c2r="k_plane_inv<512>")
 
 
 def step_bytes_minimal(ns, D, M, T, gjf, charged):
@@ -258,7 +265,9 @@ def leg_headline(args, text, d):
     correction_ms = (args.steps / every - in_window) * reorder_cost_ms if args.reorder_every >= 0 else 0.0
     ms_per_step = (wall * 1e3 + correction_ms) / args.steps
     value = ns / (ms_per_step * 1e-3)
-    pb = phase_bytes(ns, D, M, Mh, T, T, True, False, T * (T + 1) // 2, every)
+    plane_inv = bool(eng.geometry()["plane_inv"])
+    kernels = PHASE_KERNEL_PLANE_INV if plane_inv else PHASE_KERNEL
+    pb = phase_bytes(ns, D, M, Mh, T, T, True, False, T * (T + 1) // 2, every, plane_inv)
     dom = max((k for k in prof if pb[k] > 0), key=lambda k: prof[k])   # dominant kernel
     ach = pb[dom] / (prof[dom] * 1e-3) / 1e9
     step_bytes = step_bytes_minimal(ns, D, M, T, True, False)
@@ -270,13 +279,13 @@ def leg_headline(args, text, d):
         "value": value, "ms_per_step": ms_per_step, "wall_s": wall, "ns": ns, "Nx": Nx[:D], "props": nprops + 1,
         "Upe_end": pend["Upe"], "reorders_in_window": in_window, "reorder_ms": reorder_cost_ms,
         "timed_ms_per_step": wall * 1e3 / args.steps,
-        "roofline": {"bound": "hbm", "kernel": PHASE_KERNEL[dom], "achieved": ach, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": kernels[dom], "achieved": ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes": pb[dom],
                      "launch_ms": prof[dom]},
         "step_roofline": {"bytes": step_bytes, "kernel_ms": kern_ms,
                           "GBps": step_bytes / (ms_per_step * 1e-3) / 1e9,
                           "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
-        "phases": phases, "dominant": dom,
+        "phases": phases, "dominant": dom, "phase_kernels": {k: kernels[k] for k in prof},
     }
 
 

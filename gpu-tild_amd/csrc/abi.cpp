@@ -316,6 +316,7 @@ static tild_status geometry_of(tild::Replica* R, tild_geometry* out) {
     out->last_reg = h.last_reg;
     out->grad_reg = h.grad_reg;
     out->plane_reg = h.plane_reg;
+    out->plane_inv = h.plane_inv;
     out->tuned = R->tune.tuned ? 1 : 0;
     out->from_cache = R->tune.from_cache ? 1 : 0;
     out->step_ms = R->tune.step_ms;

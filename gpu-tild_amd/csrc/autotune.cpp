@@ -141,7 +141,7 @@ double Batch::measure_geometry(const System& S, uint32_t seed, const Geometry& g
     // forms that agree only to rounding is not a candidate (tuning changes no result)
     {
         const RepDev& h = reps_[ids[0]]->h;
-        const int sig = h.plane_reg | (h.last_reg << 1) | (h.grad_reg << 2);
+        const int sig = h.plane_reg | (h.last_reg << 1) | (h.grad_reg << 2) | (h.plane_inv << 3);
         if (*passes < 0) *passes = sig;
         else if (sig != *passes) {
             for (int r : ids) remove(r);
@@ -185,7 +185,7 @@ int Batch::tune_geometry(const System& S, uint32_t seed, TuneResult& out, std::s
     // z pass has no batching of inverse directions to tune, and the forward plane pass
     // stays on where its register form applies (the separate passes agree with it to
     // rounding only, and tuning changes no result).
-    bool reg_ok = false, plane_reg = false;
+    bool reg_ok = false, plane_reg = false, plane_inv = false;
     {
         const int la = S.Dim - 1, nf = S.ntypes + (S.charges.enabled ? 1 : 0);
         AxisPlan lp, px, py;
@@ -194,6 +194,9 @@ int Batch::tune_geometry(const System& S, uint32_t seed, TuneResult& out, std::s
         reg_ok = plan_axis(S.Nx[la], lp, tw.data(), &ntw) == nullptr && last_reg_plan(lp, S.ntypes, nf);
         plane_reg = S.Dim == 3 && plan_axis(S.Nx[0], px, tw.data(), &ntw) == nullptr &&
                     plan_axis(S.Nx[1], py, tw.data(), &ntw) == nullptr && plane_reg_plan(px, py);
+        // the inverse plane pass leaves no x C2R rows to batch
+        plane_inv = plane_reg && S.Fields.empty() && plane_inv_plan(px, py) && px.rad[0] == py.rad[0] &&
+                    px.rad[1] == py.rad[1];
     }
     struct Axis { int Geometry::*field; std::vector<int> cand; bool on; };
     const std::vector<Axis> axes = {
@@ -203,7 +206,7 @@ int Batch::tune_geometry(const System& S, uint32_t seed, TuneResult& out, std::s
         {&Geometry::last_lines, {0, 2, 4, 8, 16}, true},
         {&Geometry::last_dirs, {1, 2}, true},
         {&Geometry::mid_elems, {0, 1024}, true},
-        {&Geometry::c2r_rows, {0, 1, 2, 4, 8, 16}, true},
+        {&Geometry::c2r_rows, {0, 1, 2, 4, 8, 16}, !plane_inv},
     };
     for (const Axis& a : axes) {
         if (!a.on) continue;

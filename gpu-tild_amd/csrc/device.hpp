@@ -55,6 +55,7 @@ struct RepDev {
     int last_reg;  // 1: the fused last-axis pass runs register-resident (k_last_reg)
     int grad_reg;  // 1: the y inverse of the gradient spectra runs register-resident (k_grad_reg)
     int plane_reg; // 1: the forward plane pass runs register-resident (k_plane_reg)
+    int plane_inv; // 1: y inverse and x C2R run as one register-resident plane pass (k_plane_inv)
     // particle state
     float* X[2];            // ns*dim, double-buffered by step parity
     float* XO;              // ns*dim, GJF previous position

@@ -180,6 +180,7 @@ int Batch::init(std::string& err) {
     HIPCHK(configure_kernels());
     HIPCHK(configure_last_reg());
     HIPCHK(configure_plane_reg());
+    HIPCHK(configure_plane_inv());
     return 0;
 }
 
@@ -500,6 +501,10 @@ int Batch::add_plain(const System& S, uint32_t seed, std::string& err) {
     // its register-resident form: square planes with a two-stage plan
     h.plane_reg = h.plane_fwd && !tune_.staged && h.grid_off32 && plane_reg_plan(h.ax[0], h.ax[1]) &&
                   plane_reg_lds_bytes(h.ax[0]) <= kPlaneLds ? 1 : 0;
+    // the y inverse and the x C2R as one pass per (field, z) plane: square planes with an
+    // even two-stage plan, no field phases (whose records need the densities)
+    h.plane_inv = D == 3 && !tune_.staged && h.grid_off32 && !h.FPtab && plane_inv_plan(h.ax[0], h.ax[1]) &&
+                  h.ax[0].rad[0] == h.ax[1].rad[0] && h.ax[0].rad[1] == h.ax[1].rad[1] ? 1 : 0;
 
     int id = -1;
     for (int k = 0; k < (int)reps_.size(); ++k) if (!reps_[k]) { id = k; break; }
@@ -685,7 +690,8 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
         }
         if (D == 3) {
             MidGeom g = mid_grad_geom(h);
-            if (h.grad_reg) add(PH_MIDY_GR, h.nf * h.Nz * grad_reg_chunks(h.ax[1], h.Nxh), grad_reg_lds_bytes(h.ax[1]));
+            if (h.plane_inv) add(PH_PLANEI, h.nf * h.Nz, plane_inv_lds_bytes(h.ax[1]));
+            else if (h.grad_reg) add(PH_MIDY_GR, h.nf * h.Nz * grad_reg_chunks(h.ax[1], h.Nxh), grad_reg_lds_bytes(h.ax[1]));
             else add(PH_MIDY_G, g.nouter * g.nchunk, lds_bytes(g.N, 3 * g.nl));
             MidGeom gz = mid_geom(h, 2, h.nf);
             add(PH_MIDZ_H, gz.nouter * gz.nchunk, lds_bytes(gz.N, gz.nl));
@@ -695,7 +701,7 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
                             last_reg_lds_bytes(h.ax[la], h.nf));
         else add(PH_LAST, (Q + h.nl_last - 1) / h.nl_last, last_lds_bytes(Nla, h.nf, h.last_dirs, h.nl_last));
         const int nc = (h.nf * D + 1) / 2;
-        add(PH_C2R, (rows + h.nr_c2r - 1) / h.nr_c2r, lds_bytes(h.Nx, h.nr_c2r * nc));
+        if (!h.plane_inv) add(PH_C2R, (rows + h.nr_c2r - 1) / h.nr_c2r, lds_bytes(h.Nx, h.nr_c2r * nc));
         add(PH_KPROPS, h.nchunk > 0 ? h.nchunk : 1, 0);
     }
     // Deposit block size: 512 threads when the set's tiles hold ~600 beads or more on
@@ -718,7 +724,7 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
     // records; for the FFT passes adjacent column batches, whose rows share cache lines
     // that a single L2 then fetches once.
     for (Phase ph : {PH_PART, PH_TCOUNT, PH_SPREADA, PH_TILE, PH_XA, PH_XP, PH_MIDY_H, PH_MIDY_G, PH_MIDY_GR,
-                     PH_MIDZ_H, PH_PY, PH_LAST, PH_LASTR, PH_LASTR4, PH_C2R, PH_PLANEF, PH_PLANER}) {
+                     PH_MIDZ_H, PH_PY, PH_LAST, PH_LASTR, PH_LASTR4, PH_C2R, PH_PLANEF, PH_PLANER, PH_PLANEI}) {
         std::vector<WorkItem>& v = maps[ph].host;
         const size_t n = v.size(), q = n / kXcds, rm = n % kXcds;
         if (n < 2 * (size_t)kXcds) continue;
@@ -731,7 +737,7 @@ int Batch::build_maps(const std::vector<int>& ids, PhaseMap* maps, std::string& 
     }
     for (int p = 0; p < PH_COUNT; ++p) {
         maps[p].n = (int)maps[p].host.size();
-        if (maps[p].lds > ((p == PH_PLANEF || p == PH_PLANER || p == PH_LAST || p == PH_LASTR || p == PH_LASTR4) ? kPlaneLds : (size_t)kMaxLds)) { err = "pass needs more LDS than a block may use"; return -TILD_ERR_UNSUPPORTED; }
+        if (maps[p].lds > ((p == PH_PLANEF || p == PH_PLANER || p == PH_PLANEI || p == PH_LAST || p == PH_LASTR || p == PH_LASTR4) ? kPlaneLds : (size_t)kMaxLds)) { err = "pass needs more LDS than a block may use"; return -TILD_ERR_UNSUPPORTED; }
         if (maps[p].dev && (!reorder_only || maps[p].cap < maps[p].n)) {
             // reused buffers are last read by the previous reorder's launches
             HIPCHK(hipStreamSynchronize(stream_));
@@ -1051,6 +1057,7 @@ int Batch::enqueue_steps(SetMaps& m, int nsteps, bool integrate, bool write_forc
         HIPCHK(launch_mid_grad(ctx(PH_MIDY_G)));
         HIPCHK(launch_grad_reg(ctx(PH_MIDY_GR)));
         HIPCHK(launch_x_c2r(ctx(PH_C2R), true, 0, nullptr, nullptr));
+        HIPCHK(launch_plane_inv(ctx(PH_PLANEI)));
         // integrating steps claim the next step's records; a force evaluation leaves
         // positions and records as they are
         HIPCHK(launch_particles(ctx(PH_PART), integrate, write_forces ? 1 : 0, k, integrate));
@@ -1193,6 +1200,7 @@ int Batch::profile(const std::vector<int>& ids, int n, double* ms, int nph, std:
         ok(launch_mid_grad(ctx(PH_MIDY_G, 4, 0)));
         ok(launch_grad_reg(ctx(PH_MIDY_GR, 4, 1)));
         ok(launch_x_c2r(ctx(PH_C2R, 5, 0), true, 0, nullptr, nullptr));
+        ok(launch_plane_inv(ctx(PH_PLANEI, 5, 1)));
         ok(launch_particles(ctx(PH_PART, 6, 0), true, 0, 0, true));
         ok(launch_advance(stream_, d_reps_, m->d_ids, m->nids, 1, ev[7][0][0], ev[7][0][1]));
         used[7][0] = m->nids > 0;

@@ -32,7 +32,7 @@ constexpr int kProfPhases = 9;   // claim, deposit, forward plane, last_fused, y
 // skip MIDY_G and C2R); PLANEF: x R2C + y FFT of tiled 3D replicas with even Ny
 // (their deposit writes densities only and they skip MIDY_H).
 enum Phase { PH_PART = 0, PH_SPREADA, PH_XA, PH_XP, PH_SCAN, PH_TILE, PH_MIDY_H, PH_MIDY_G,
-             PH_MIDZ_H, PH_PY, PH_LAST, PH_C2R, PH_PLANEF, PH_TCOUNT, PH_KPROPS, PH_RTILE, PH_LASTR, PH_MIDY_GR, PH_PLANER, PH_LASTR4, PH_COUNT };
+             PH_MIDZ_H, PH_PY, PH_LAST, PH_C2R, PH_PLANEF, PH_TCOUNT, PH_KPROPS, PH_RTILE, PH_LASTR, PH_MIDY_GR, PH_PLANER, PH_LASTR4, PH_PLANEI, PH_COUNT };
 
 struct PhaseMap {
     std::vector<WorkItem> host;

@@ -458,6 +458,209 @@ __global__ void __launch_bounds__(NT) k_plane_reg(const RepDev* __restrict__ rep
     }
 }
 
+// ------------------------------------------------------------------ plane inverse (3D)
+// The y inverse of the gradient spectra and the x C2R into gather records of one
+// (field, z) plane in one block (the operations of k_grad_reg and k_x_c2r, kernels.hip,
+// without the half spectra of the y inverses going through HBM), for square planes
+// Nx = Ny = N with a two-stage plan R1 x R2 and even N:
+//   A  task (column k < N/2 + 1, i < R2): rows i + R2 r of U and V (slots 0 and 2),
+//      the lines conj(U), conj(-i k_y U), conj(V), DFT_R1;
+//   B  task (k, i < R1): twiddles, DFT_R2 -> the y inverses Y_l(y, k), y = i + R1 q,
+//      written back over the rows the task read;
+//   C  task (x line c, i < R2): the complex lines of the x inverse -- per row y the
+//      x derivative (-i k_x applied) plus i times the y derivative, per row pair the
+//      z derivatives of rows 2p and 2p + 1 -- extended to the full spectrum by
+//      Hermitian symmetry at k = i + R2 r, DFT_R1; held across a barrier, then written
+//      over the y inverses;
+//   D  task (c, i < R1): twiddles, DFT_R2 -> the real lines at x = i + R1 q, held
+//      across a barrier, then laid out as the plane's records in LDS and copied out.
+// Six barriers; the y and x transforms and the conjugation and Hermitian rules are
+// those of the two staged passes, which the records match to rounding (the x lines
+// pair components differently).  Records carry no density mask (see k_x_c2r).
+namespace {
+template <int R1, int R2, int NT>
+__device__ __forceinline__ void plane_inv_block(const RepDev& r, int blk, float2* lds) {
+    constexpr int N = R1 * R2, NXH = N / 2 + 1, NT1 = R1 * (R2 - 1);
+    constexpr int LA = (3 * NXH) | 1;          // row stride of the y inverses: [y][line][k]
+    constexpr int NCL = 3 * N / 2;             // x lines: N rows + N / 2 row pairs
+    constexpr int LC = NCL | 1;                // row stride of the x exchange
+    constexpr int CT = (NCL * R2 + NT - 1) / NT;   // stage C tasks per thread
+    constexpr int DT = (NCL * R1 + NT - 1) / NT;   // stage D tasks per thread
+    static_assert(N % 2 == 0 && LC <= LA && 3 * N * N <= 2 * N * LA,
+                  "plane_inv_block: even N, exchange and records within the y buffer");
+    const int z = blk % r.Nz, f = blk / r.Nz;
+    float2* X = lds;                   // N x LA: y stages, then the x exchange (N x LC)
+    float2* TW = X + N * LA;           // w_N^{i r}, i < R1, 1 <= r < R2 (same plan on x and y)
+    {
+        const float2* tw = r.ax[1].tw + r.ax[1].tw_off[1];
+        for (int t = threadIdx.x; t < NT1; t += NT) TW[t] = gld(tw + t);
+    }
+    const uint32_t Mhu = (uint32_t)r.Mh;
+    const uint32_t plane = (uint32_t)z * (uint32_t)(N * NXH);
+    const uint32_t sU = (uint32_t)(f * 3 + 0) * Mhu + plane, sV = (uint32_t)(f * 3 + 2) * Mhu + plane;
+    const float kfy = r.kfac[1], kfx = r.kfac[0];
+    // A: y stage 1 of the three lines of column k
+    for (int task = threadIdx.x; task < NXH * R2; task += NT) {
+        const int k = task % NXH, i = task / NXH;
+        float2 u[R1], v[R1];
+#pragma unroll
+        for (int rr = 0; rr < R1; ++rr) {
+            const uint32_t o = (uint32_t)((i + R2 * rr) * NXH + k);
+            u[rr] = gld(at32((const float2*)r.Gk, sU + o));
+            v[rr] = gld(at32((const float2*)r.Gk, sV + o));
+        }
+        float2 L[3][R1];
+#pragma unroll
+        for (int rr = 0; rr < R1; ++rr) {
+            const int n = i + R2 * rr;
+            const float ky = 2 * n == N ? 0.f : kfy * (float)(2 * n < N ? n : n - N);
+            // lines of the conj(FFT(conj)) inverse: conj(u), conj(-i ky u), conj(v)
+            L[0][rr] = make_float2(u[rr].x, -u[rr].y);
+            L[1][rr] = grad_line(ky, u[rr]);
+            L[2][rr] = make_float2(v[rr].x, -v[rr].y);
+        }
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            Dft<R1>::run(L[l]);
+#pragma unroll
+            for (int q = 0; q < R1; ++q) X[(i * R1 + q) * LA + l * NXH + k] = L[l][q];
+        }
+    }
+    __syncthreads();
+    // B: y stage 2; Y_l(y, k) = conj of the result, over the rows the task read
+    for (int task = threadIdx.x; task < NXH * R1; task += NT) {
+        const int k = task % NXH, i = task / NXH;
+        float2 t[R2];
+#pragma unroll
+        for (int rr = 1; rr < R2; ++rr) t[rr] = TW[i * (R2 - 1) + rr - 1];
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            float2 w[R2];
+#pragma unroll
+            for (int rr = 0; rr < R2; ++rr) w[rr] = X[(i + R1 * rr) * LA + l * NXH + k];
+#pragma unroll
+            for (int rr = 1; rr < R2; ++rr) w[rr] = cmul(w[rr], t[rr]);
+            Dft<R2>::run(w);
+#pragma unroll
+            for (int q = 0; q < R2; ++q) X[(i + R1 * q) * LA + l * NXH + k] = conjf2(w[q]);
+        }
+    }
+    __syncthreads();
+    // C: x stage 1.  Line c < N is row y = c: A = -i k_x Y_0, B = Y_1; line N + p pairs
+    // A = Y_2(2p), B = Y_2(2p + 1).  Z = A + i B with Z_{N-k} = conj A_k + i conj B_k;
+    // the self-conjugate columns (0, N/2) drop their imaginary parts; conj(Z) is
+    // transformed (the inverse as conj(FFT(conj))).
+    float2 hold[CT][R1];
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+        const int task = (int)threadIdx.x + ct * NT;
+        if (task >= NCL * R2) break;
+        const int c = task % NCL, i = task / NCL;
+        const bool rowline = c < N;
+        const int ya = rowline ? c : 2 * (c - N), yb = rowline ? c : ya + 1;
+        const int la = rowline ? 0 : 2, lb = rowline ? 1 : 2;
+#pragma unroll
+        for (int rr = 0; rr < R1; ++rr) {
+            const int kn = i + R2 * rr;                 // full-spectrum index
+            const bool upper = kn >= NXH;
+            const int kk = upper ? N - kn : kn;         // its half-spectrum column
+            float2 A = X[ya * LA + la * NXH + kk], B = X[yb * LA + lb * NXH + kk];
+            const bool self = kk == 0 || 2 * kk == N;
+            if (rowline) {   // the x derivative: -i k_x (0 on the Nyquist column)
+                const float kx = self && kk != 0 ? 0.f : kfx * (float)kk;
+                A = make_float2(kx * A.y, -kx * A.x);
+            }
+            if (self) { A.y = 0.f; B.y = 0.f; }
+            hold[ct][rr] = upper ? make_float2(A.x + B.y, A.y - B.x) : make_float2(A.x - B.y, -(A.y + B.x));
+        }
+        Dft<R1>::run(hold[ct]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+        const int task = (int)threadIdx.x + ct * NT;
+        if (task >= NCL * R2) break;
+        const int c = task % NCL, i = task / NCL;
+#pragma unroll
+        for (int q = 0; q < R1; ++q) X[(i * R1 + q) * LC + c] = hold[ct][q];
+    }
+    __syncthreads();
+    // D: x stage 2 -> the real lines A = res.x, B = -res.y at x = i + R1 q, held across
+    // a barrier and laid out as the plane's records [y][x](g_x, g_y, g_z) over the
+    // exchange; the plane's records are one contiguous run of HBM, copied out in 16-byte
+    // pieces
+    float2 out[DT][R2];
+#pragma unroll
+    for (int dt = 0; dt < DT; ++dt) {
+        const int task = (int)threadIdx.x + dt * NT;
+        if (task >= NCL * R1) break;
+        const int i = task % R1, c = task / R1;   // adjacent lanes: adjacent x
+#pragma unroll
+        for (int rr = 0; rr < R2; ++rr) out[dt][rr] = X[(i + R1 * rr) * LC + c];
+#pragma unroll
+        for (int rr = 1; rr < R2; ++rr) out[dt][rr] = cmul(out[dt][rr], TW[i * (R2 - 1) + rr - 1]);
+        Dft<R2>::run(out[dt]);
+    }
+    __syncthreads();
+    float* rec = (float*)X;
+#pragma unroll
+    for (int dt = 0; dt < DT; ++dt) {
+        const int task = (int)threadIdx.x + dt * NT;
+        if (task >= NCL * R1) break;
+        const int i = task % R1, c = task / R1;   // adjacent lanes: adjacent x
+#pragma unroll
+        for (int q = 0; q < R2; ++q) {
+            const int x = i + R1 * q;
+            if (c < N) {   // row y = c: (g_x, g_y)
+                rec[(c * N + x) * 3 + 0] = out[dt][q].x;
+                rec[(c * N + x) * 3 + 1] = -out[dt][q].y;
+            } else {       // rows 2p and 2p + 1: g_z
+                const int y0 = 2 * (c - N);
+                rec[(y0 * N + x) * 3 + 2] = out[dt][q].x;
+                rec[((y0 + 1) * N + x) * 3 + 2] = -out[dt][q].y;
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t rec0 = (uint32_t)f * (uint32_t)r.M + (uint32_t)z * (uint32_t)(N * N);
+    float4* dst = (float4*)((char*)r.G3 + ((rec0 << 3) + (rec0 << 2)));   // 12 rec0 bytes, 16-byte aligned
+    const float4* src = (const float4*)rec;
+    for (int t = threadIdx.x; t < 3 * N * N / 4; t += NT) gst(dst + t, src[t]);
+}
+}  // namespace
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_plane_inv(const RepDev* __restrict__ reps, const WorkItem* __restrict__ wmap) {
+    extern __shared__ float2 lds[];
+    const WorkItem wi = wmap[blockIdx.x];
+    const RepDev& r = reps[wi.rep];
+    const AxisPlan& pl = r.ax[1];
+    switch (pl.rad[0] * 32 + pl.rad[1]) {   // uniform per block
+#define TILD_CASE(A, B) \
+    case A * 32 + B: if constexpr ((A * B) % 2 == 0 && (A * B / 2 + 1) * (A > B ? A : B) <= NT) plane_inv_block<A, B, NT>(r, wi.blk, lds); break;
+        TILD_REG_PLANS(TILD_CASE)
+#undef TILD_CASE
+        default: break;   // excluded by plane_inv_plan on the host
+    }
+}
+
+bool plane_inv_plan(const AxisPlan& px, const AxisPlan& py) {
+    if (!(px.N == py.N && px.N % 2 == 0 && grad_reg_plan(py))) return false;
+    const int rmax = py.rad[0] > py.rad[1] ? py.rad[0] : py.rad[1];
+    return (py.N / 2 + 1) * rmax <= kPlaneInvThreads && plane_inv_lds_bytes(py) <= kPlaneLds / 2;
+}
+
+hipError_t launch_plane_inv(const LaunchCtx& c) {
+    if (!c.n) return hipSuccess;
+    TILD_LAUNCH(c, k_plane_inv<kPlaneInvThreads>, dim3(c.n), dim3(kPlaneInvThreads), c.lds, c.reps, c.wmap);
+    return hipGetLastError();
+}
+
+hipError_t configure_plane_inv() {
+    return hipFuncSetAttribute((const void*)k_plane_inv<kPlaneInvThreads>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)(kPlaneLds / 2));
+}
+
 bool plane_reg_plan(const AxisPlan& px, const AxisPlan& py) {
     return px.N == py.N && grad_reg_plan(px);
 }

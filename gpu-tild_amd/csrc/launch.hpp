@@ -136,6 +136,17 @@ inline size_t plane_reg_lds_bytes(const AxisPlan& pl) {
 }
 hipError_t launch_plane_reg(const LaunchCtx& c);
 hipError_t configure_plane_reg();
+// Register-resident inverse plane pass (y inverse + x C2R into the gather records) of
+// square planes with an even two-stage plan: one block of kPlaneInvThreads threads per
+// (field, z) plane, two blocks to a CU.
+constexpr int kPlaneInvThreads = 512;
+inline size_t plane_inv_lds_bytes(const AxisPlan& pl) {
+    const size_t N = (size_t)pl.N, la = (size_t)((3 * (pl.N / 2 + 1)) | 1);
+    return (N * la + (size_t)pl.rad[0] * (pl.rad[1] - 1)) * sizeof(float2);
+}
+bool plane_inv_plan(const AxisPlan& px, const AxisPlan& py);
+hipError_t launch_plane_inv(const LaunchCtx& c);
+hipError_t configure_plane_inv();
 hipError_t configure_last_reg();
 hipError_t launch_plane_r2c(const LaunchCtx& c);
 // LDS of the forward plane pass: two buffers holding either the Ny/2 row pairs of

@@ -57,7 +57,7 @@ class Geometry(ctypes.Structure):
     _fields_ = [("tile_target", ctypes.c_int), ("tiles", ctypes.c_int), ("deposit_threads", ctypes.c_int),
                 ("plane", ctypes.c_int), ("last_lines", ctypes.c_int), ("last_dirs", ctypes.c_int),
                 ("mid_elems", ctypes.c_int), ("c2r_rows", ctypes.c_int), ("last_reg", ctypes.c_int),
-                ("grad_reg", ctypes.c_int), ("plane_reg", ctypes.c_int),
+                ("grad_reg", ctypes.c_int), ("plane_reg", ctypes.c_int), ("plane_inv", ctypes.c_int),
                 ("tuned", ctypes.c_int),
                 ("from_cache", ctypes.c_int), ("step_ms", ctypes.c_double)]
 

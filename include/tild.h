@@ -217,6 +217,7 @@ typedef struct {
     int last_reg;        /* 1: register-resident fused z pass */
     int grad_reg;        /* 1: register-resident y inverse */
     int plane_reg;       /* 1: register-resident forward plane pass */
+    int plane_inv;       /* 1: y inverse and x C2R as one register-resident plane pass */
     int tuned, from_cache;
     double step_ms;      /* kernel time per step measured by the tuner (0 if not tuned) */
 } tild_geometry;

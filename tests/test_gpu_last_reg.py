@@ -1,6 +1,6 @@
 """The register-resident FFT passes -- the forward plane pass (k_plane_reg), the fused
-last-axis pass (k_last_reg) and the y inverse (k_grad_reg) -- against the LDS-staged
-ones.
+last-axis pass (k_last_reg), the y inverse (k_grad_reg) and the inverse plane pass (y
+inverse and x C2R in one, k_plane_inv) -- against the LDS-staged ones.
 
 Both forms run the same transforms (forward plan R1 x R2, inverse plan R2 x R1) and
 the same coupling sums; the compiler fuses multiplies into additions differently where
@@ -117,6 +117,15 @@ def expected_flags(s, g):
     return int(last), int(grad), int(plane)
 
 
+def expected_inverse_plane(s):
+    # the y inverse and x C2R as one pass: square x-y planes of an even two-stage plan
+    # whose half-plane columns times the larger radix fit the block
+    nx = s.grid_shape[::-1]
+    if s.dim != 3 or nx[0] != nx[1] or nx[0] % 2 or nx[0] not in PLANS:
+        return 0
+    return int((nx[0] // 2 + 1) * max(PLANS[nx[0]]) <= 512)
+
+
 @pytest.mark.parametrize("case", list(CASES))
 def test_register_pass_equals_staged_pass(case, tmp_path):
     text, d = CASES[case](tmp_path)
@@ -128,9 +137,11 @@ def test_register_pass_equals_staged_pass(case, tmp_path):
           f"plane_reg={gr['plane_reg']}", flush=True)
     flags = (gr["last_reg"], gr["grad_reg"], gr["plane_reg"])
     assert flags == expected_flags(s, gr), (flags, gr)
+    assert gr["plane_inv"] == expected_inverse_plane(s), gr
+    print(f"  plane_inv={gr['plane_inv']}")
     if case.startswith("plane"):
         assert gr["plane_reg"] == 1
-    assert gs["last_reg"] == 0 and gs["grad_reg"] == 0 and gs["plane_reg"] == 0
+    assert gs["last_reg"] == 0 and gs["grad_reg"] == 0 and gs["plane_reg"] == 0 and gs["plane_inv"] == 0
     fr, fs = reg.forces(), staged.forces()
     ef = U.rel_norm(fr, fs)
     print(f"  |f_reg - f_staged|/|f| = {ef:.2e}, bit-identical: {np.array_equal(fr, fs)}")
@@ -157,6 +168,7 @@ def test_register_pass_matches_oracle(case, tmp_path):
     o = O.System(text, d, seed=7)
     g = e.geometry()
     assert (g["last_reg"], g["grad_reg"], g["plane_reg"]) == expected_flags(s, g)
+    assert g["plane_inv"] == expected_inverse_plane(s)
     f, fo = e.forces(), o.forces()
     ef = U.rel_norm(f, fo)
     p, po = e.properties(), o.properties()
