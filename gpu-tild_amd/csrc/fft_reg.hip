@@ -462,7 +462,8 @@ __global__ void __launch_bounds__(NT) k_plane_reg(const RepDev* __restrict__ rep
 // The y inverse of the gradient spectra and the x C2R into gather records of one
 // (field, z) plane in one block (the operations of k_grad_reg and k_x_c2r, kernels.hip,
 // without the half spectra of the y inverses going through HBM), for square planes
-// Nx = Ny = N with a two-stage plan R1 x R2 and even N:
+// Nx = Ny = N with a two-stage plan R1 x R2 (an odd N pairs its last row's z derivative
+// with an all-zero line):
 //   A  task (column k < N/2 + 1, i < R2): rows i + R2 r of U and V (slots 0 and 2),
 //      the lines conj(U), conj(-i k_y U), conj(V), DFT_R1;
 //   B  task (k, i < R1): twiddles, DFT_R2 -> the y inverses Y_l(y, k), y = i + R1 q,
@@ -482,12 +483,11 @@ template <int R1, int R2, int NT>
 __device__ __forceinline__ void plane_inv_block(const RepDev& r, int blk, float2* lds) {
     constexpr int N = R1 * R2, NXH = N / 2 + 1, NT1 = R1 * (R2 - 1);
     constexpr int LA = (3 * NXH) | 1;          // row stride of the y inverses: [y][line][k]
-    constexpr int NCL = 3 * N / 2;             // x lines: N rows + N / 2 row pairs
+    constexpr int NCL = N + (N + 1) / 2;       // x lines: N rows + the row pairs
     constexpr int LC = NCL | 1;                // row stride of the x exchange
     constexpr int CT = (NCL * R2 + NT - 1) / NT;   // stage C tasks per thread
     constexpr int DT = (NCL * R1 + NT - 1) / NT;   // stage D tasks per thread
-    static_assert(N % 2 == 0 && LC <= LA && 3 * N * N <= 2 * N * LA,
-                  "plane_inv_block: even N, exchange and records within the y buffer");
+    static_assert(LC <= LA && 3 * N * N <= 2 * N * LA, "plane_inv_block: exchange and records within the y buffer");
     const int z = blk % r.Nz, f = blk / r.Nz;
     float2* X = lds;                   // N x LA: y stages, then the x exchange (N x LC)
     float2* TW = X + N * LA;           // w_N^{i r}, i < R1, 1 <= r < R2 (same plan on x and y)
@@ -564,7 +564,8 @@ __device__ __forceinline__ void plane_inv_block(const RepDev& r, int blk, float2
             const int kn = i + R2 * rr;                 // full-spectrum index
             const bool upper = kn >= NXH;
             const int kk = upper ? N - kn : kn;         // its half-spectrum column
-            float2 A = X[ya * LA + la * NXH + kk], B = X[yb * LA + lb * NXH + kk];
+            float2 A = X[ya * LA + la * NXH + kk];
+            float2 B = yb < N ? X[yb * LA + lb * NXH + kk] : make_float2(0.f, 0.f);
             const bool self = kk == 0 || 2 * kk == N;
             if (rowline) {   // the x derivative: -i k_x (0 on the Nyquist column)
                 const float kx = self && kk != 0 ? 0.f : kfx * (float)kk;
@@ -617,15 +618,20 @@ __device__ __forceinline__ void plane_inv_block(const RepDev& r, int blk, float2
             } else {       // rows 2p and 2p + 1: g_z
                 const int y0 = 2 * (c - N);
                 rec[(y0 * N + x) * 3 + 2] = out[dt][q].x;
-                rec[((y0 + 1) * N + x) * 3 + 2] = -out[dt][q].y;
+                if (y0 + 1 < N) rec[((y0 + 1) * N + x) * 3 + 2] = -out[dt][q].y;
             }
         }
     }
     __syncthreads();
     const uint32_t rec0 = (uint32_t)f * (uint32_t)r.M + (uint32_t)z * (uint32_t)(N * N);
-    float4* dst = (float4*)((char*)r.G3 + ((rec0 << 3) + (rec0 << 2)));   // 12 rec0 bytes, 16-byte aligned
-    const float4* src = (const float4*)rec;
-    for (int t = threadIdx.x; t < 3 * N * N / 4; t += NT) gst(dst + t, src[t]);
+    if constexpr (N % 2 == 0) {   // 12 rec0 bytes, 16-byte aligned
+        float4* dst = (float4*)((char*)r.G3 + ((rec0 << 3) + (rec0 << 2)));
+        const float4* src = (const float4*)rec;
+        for (int t = threadIdx.x; t < 3 * N * N / 4; t += NT) gst(dst + t, src[t]);
+    } else {
+        float* dst = (float*)((char*)r.G3 + ((rec0 << 3) + (rec0 << 2)));
+        for (int t = threadIdx.x; t < 3 * N * N; t += NT) G(dst)[t] = rec[t];
+    }
 }
 }  // namespace
 
@@ -637,7 +643,7 @@ __global__ void __launch_bounds__(NT) k_plane_inv(const RepDev* __restrict__ rep
     const AxisPlan& pl = r.ax[1];
     switch (pl.rad[0] * 32 + pl.rad[1]) {   // uniform per block
 #define TILD_CASE(A, B) \
-    case A * 32 + B: if constexpr ((A * B) % 2 == 0 && (A * B / 2 + 1) * (A > B ? A : B) <= NT) plane_inv_block<A, B, NT>(r, wi.blk, lds); break;
+    case A * 32 + B: if constexpr ((A * B / 2 + 1) * (A > B ? A : B) <= NT) plane_inv_block<A, B, NT>(r, wi.blk, lds); break;
         TILD_REG_PLANS(TILD_CASE)
 #undef TILD_CASE
         default: break;   // excluded by plane_inv_plan on the host
@@ -645,7 +651,7 @@ __global__ void __launch_bounds__(NT) k_plane_inv(const RepDev* __restrict__ rep
 }
 
 bool plane_inv_plan(const AxisPlan& px, const AxisPlan& py) {
-    if (!(px.N == py.N && px.N % 2 == 0 && grad_reg_plan(py))) return false;
+    if (!(px.N == py.N && grad_reg_plan(py))) return false;
     const int rmax = py.rad[0] > py.rad[1] ? py.rad[0] : py.rad[1];
     return (py.N / 2 + 1) * rmax <= kPlaneInvThreads && plane_inv_lds_bytes(py) <= kPlaneLds / 2;
 }

@@ -137,8 +137,8 @@ inline size_t plane_reg_lds_bytes(const AxisPlan& pl) {
 hipError_t launch_plane_reg(const LaunchCtx& c);
 hipError_t configure_plane_reg();
 // Register-resident inverse plane pass (y inverse + x C2R into the gather records) of
-// square planes with an even two-stage plan: one block of kPlaneInvThreads threads per
-// (field, z) plane, two blocks to a CU.
+// square planes with a two-stage plan: one block of kPlaneInvThreads threads per (field,
+// z) plane, two blocks to a CU.
 constexpr int kPlaneInvThreads = 512;
 inline size_t plane_inv_lds_bytes(const AxisPlan& pl) {
     const size_t N = (size_t)pl.N, la = (size_t)((3 * (pl.N / 2 + 1)) | 1);

@@ -98,6 +98,8 @@ CASES = {
     "x48_y48_z30": lambda t: decks.melt3d(t, grid=(48, 48, 30), box=(12.0, 12.0, 7.5), n_chains=100),
     "x24_y32_z25_three": lambda t: three_types(t, grid=(24, 32, 25), box=(6.0, 8.0, 6.25)),
     "2d_x50_y128": lambda t: decks.melt2d(t, grid=(50, 128), box=(12.5, 32.0)),
+    # odd square planes: the last row's z derivative has no partner row
+    "inverse_odd_x25_y25_z20": lambda t: decks.melt3d(t, grid=(25, 25, 20), box=(12.5, 12.5, 10.0), n_chains=100),
 }
 
 # two-stage plans the register passes take, and the variants compiled in
@@ -118,10 +120,10 @@ def expected_flags(s, g):
 
 
 def expected_inverse_plane(s):
-    # the y inverse and x C2R as one pass: square x-y planes of an even two-stage plan
-    # whose half-plane columns times the larger radix fit the block
+    # the y inverse and x C2R as one pass: square x-y planes of a two-stage plan whose
+    # half-plane columns times the larger radix fit the block
     nx = s.grid_shape[::-1]
-    if s.dim != 3 or nx[0] != nx[1] or nx[0] % 2 or nx[0] not in PLANS:
+    if s.dim != 3 or nx[0] != nx[1] or nx[0] not in PLANS:
         return 0
     return int((nx[0] // 2 + 1) * max(PLANS[nx[0]]) <= 512)
 
@@ -160,7 +162,8 @@ def test_register_pass_equals_staged_pass(case, tmp_path):
 @pytest.mark.parametrize("case", ["z160", "z64_three", "z40_charged", "z64_4types_charged", "2d_y80", "y80_z40",
                                   "y64_z64_three",
                                   "x80_y40_z64", "2d_x40_y40", "2d_x160_y16", "plane_x40_y40_z20_4types",
-                                  "plane_x64_y64_z10", "x48_y48_z30", "x24_y32_z25_three", "2d_x50_y128"])
+                                  "plane_x64_y64_z10", "x48_y48_z30", "x24_y32_z25_three", "2d_x50_y128",
+                                  "inverse_odd_x25_y25_z20"])
 def test_register_pass_matches_oracle(case, tmp_path):
     text, d = CASES[case](tmp_path)
     s = tild.System(text, d)
