@@ -10,8 +10,8 @@ chains (no datasets are available offline).
 
 Prints one JSON line: whole-job bead-steps/s, the roofline of the dominant kernel
 (algorithmic bytes per launch / mean launch time from HIP events bound to the launches
-on the engine's stream), the FP64 oracle timed on the box's own cores (one thread and
-all the threads of the box's CPU share), and the FP32-vs-FP64 error of every quantity.
+on the engine's stream), the FP64 oracle timed on the host's own cores (one thread and
+all the threads of the host's CPU share), and the FP32-vs-FP64 error of every quantity.
 Legs that ride along, one per benchmark configuration:
   C1  24,000-bead diblock melt, 40^3, EM
   C2  the headline melt, also as the "plain" path (eager launches, never a graph)
