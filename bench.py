@@ -434,21 +434,28 @@ def leg_serve(args, jobs, step_work=0, check_oracle=True):
             except Exception as e:   # reported in the JSON line
                 errors.append(str(e))
 
+    def progress():
+        st = srv.stats()
+        print(f"bench: serving {st['done']}/{len(jobs)} done, {st['running']} running, {st['queued']} queued",
+              file=sys.stderr, flush=True)
+
     t0 = time.perf_counter()
     th = threading.Thread(target=feeder)
     th.start()
-    last = t0
-    while th.is_alive() or srv.stats()["done"] + srv.stats()["failed"] < len(ids):
-        time.sleep(0.5)
-        if time.perf_counter() - last > 20:
-            st = srv.stats()
-            print(f"bench: serving {st['done']}/{len(jobs)} done, {st['running']} running, {st['queued']} queued",
-                  file=sys.stderr, flush=True)
-            last = time.perf_counter()
-        if time.perf_counter() - t0 > 900:
+    while th.is_alive():
+        th.join(timeout=20)
+        if th.is_alive():
+            progress()
+    # the makespan ends when the server has drained (a wait on its own condition, so the
+    # clock stops at the last retirement, not at the next poll)
+    while True:
+        try:
+            srv.drain(timeout=20)
             break
-    th.join()
-    srv.drain(timeout=600)
+        except tild.TildError as e:
+            if "timed out" not in str(e) or time.perf_counter() - t0 > 900:
+                raise
+            progress()
     wall = time.perf_counter() - t0
     st = srv.stats()
     arrivals = float(np.sum(gaps))

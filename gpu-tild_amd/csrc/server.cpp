@@ -70,31 +70,30 @@ int Server::submit(const std::string& text, const std::string& base, uint32_t se
 }
 
 // Worker: take the jobs that fit, enqueue one chunk of the running jobs, build the new
-// jobs' replicas while it runs, wait for it, retire; repeat.  Only this thread touches
-// the Batch.
+// jobs' replicas while it runs (those not reached before it ends wait for the next
+// chunk), wait for it, retire; repeat.  Only this thread touches the Batch.
 void Server::loop() {
     (void)hipSetDevice(opts_.device);
     std::string err;
     for (;;) {
         // --- admission (arrival order; the queue head waits until it fits)
-        std::vector<Job*> admit;
         {
             std::unique_lock<std::mutex> lk(mu_);
-            cv_.wait(lk, [&] { return stop_ || !queue_.empty() || !running_.empty(); });
+            cv_.wait(lk, [&] { return stop_ || !queue_.empty() || !running_.empty() || !pending_.empty(); });
             if (stop_) break;
             size_t used = bytes_in_use_;
-            int nrun = (int)running_.size();
+            int nrun = (int)(running_.size() + pending_.size());
             while (!queue_.empty()) {
                 Job* J = jobs_[queue_.front()].get();
                 if (used + J->bytes > opts_.hbm_budget) break;
                 if (opts_.max_replicas > 0 && nrun >= opts_.max_replicas) break;
                 used += J->bytes;
                 ++nrun;
-                admit.push_back(J);
+                pending_.push_back(J);
                 queue_.pop_front();
             }
             bytes_in_use_ = used;   // reserved under the estimate until the replica exists
-            admitting_ = (int)admit.size();   // neither queued nor running until added
+            admitting_ = (int)pending_.size();   // neither queued nor running until added
         }
         // --- one chunk of every running job, cut at the nearest finish: enqueued first,
         // so that the admitted jobs' replicas are built (on the batch's set-up stream)
@@ -133,7 +132,13 @@ void Server::loop() {
             rc = batch_->step(reps, chunk, err, opts_.use_graph != 0);
             (void)hipEventRecord(ev_[1], batch_->stream());
         }
-        for (Job* J : admit) {
+        // builds run while the chunk does; one that would start after the chunk has
+        // finished waits for the next chunk, which goes out first (at least one build a
+        // pass, one at a time while nothing runs)
+        size_t nbuilt = 0;
+        for (; nbuilt < pending_.size(); ++nbuilt) {
+            if (nbuilt > 0 && (!stepping || hipEventQuery(ev_[1]) == hipSuccess)) break;
+            Job* J = pending_[nbuilt];
             const double ts = now();
             std::string e1;
             const int id = batch_->add(*J->sys, J->seed, e1);
@@ -157,6 +162,7 @@ void Server::loop() {
             J->sys.reset();   // the replica holds everything the step needs
             running_.push_back(J->id);   // joins from the next chunk on
         }
+        pending_.erase(pending_.begin(), pending_.begin() + nbuilt);
         if (!stepping) continue;
         if (!rc) rc = batch_->sync(err);
         float gpu_ms = 0.f;
@@ -201,15 +207,17 @@ void Server::loop() {
             done_cv_.notify_all();
         }
         if (rc) {
-            // a device error ends the server: fail whatever is still queued
+            // a device error ends the server: fail whatever is still queued or unbuilt
             std::lock_guard<std::mutex> lk(mu_);
-            for (int jid : queue_) {
-                Job* J = jobs_[jid].get();
+            for (int jid : queue_) pending_.push_back(jobs_[jid].get());
+            for (Job* J : pending_) {
                 J->state = TILD_JOB_FAILED;
                 J->err = fatal_;
                 J->t_done = now();
             }
             queue_.clear();
+            pending_.clear();
+            admitting_ = 0;
             done_cv_.notify_all();
             break;
         }

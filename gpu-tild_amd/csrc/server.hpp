@@ -66,6 +66,7 @@ private:
     std::map<int, std::unique_ptr<Job>> jobs_;
     std::vector<int> running_;
     int admitting_ = 0;            // jobs taken off the queue whose replica is being built
+    std::vector<Job*> pending_;    // those jobs, in admission order (worker thread only)
     int next_id_ = 0;
     size_t bytes_in_use_ = 0, peak_bytes_ = 0;
     double t0_ = 0, busy_s_ = 0, admit_s_ = 0, retire_s_ = 0;
